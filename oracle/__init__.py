@@ -1,0 +1,217 @@
+"""CPU oracle of the edge-based flow path -- TEST INFRASTRUCTURE ONLY.
+
+numpy front end over liboracle.so (the plain-C restatement of the reference
+algorithms in this directory).  Imported by tests/, by the smoke check and by
+bench.py's CPU-baseline leg, never by the product package xyst_amd.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class Cfg(C.Structure):
+    """Physics configuration (gamma, flux kind, stab2) as read from g_cfg."""
+    _fields_ = [("gamma", C.c_double), ("flux", C.c_int), ("stab2", C.c_int),
+                ("stab2coef", C.c_double)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        I64, I32, D = C.c_int64, C.c_int32, C.c_double
+        cfgp = C.POINTER(Cfg)
+        L.orc_vol.argtypes = [I64, _f64p, _f64p, _f64p, I64, _i64p, _f64p]
+        L.orc_vol.restype = C.c_int
+        L.orc_domint.argtypes = [I64, _f64p, _f64p, _f64p, I64, _i64p, I64,
+                                 _i64p, _f64p]
+        L.orc_domint.restype = I64
+        L.orc_domsuped.argtypes = [I64, _i64p, I64, _i64p, _f64p, _i64p,
+                                   _i64p, _f64p, _i64p, _f64p, _i64p, _f64p]
+        L.orc_domsuped.restype = C.c_int
+        L.orc_symbc_setup.argtypes = [I64, _f64p, _f64p, _f64p, I64, _i64p,
+                                      _i32p, I32, _i32p, I64, _i64p, _f64p,
+                                      _u8p]
+        L.orc_symbc_setup.restype = I64
+        L.orc_grad.argtypes = [I64, C.c_int, _i64p, _i64p, _f64p, _i64p,
+                               _f64p, _i64p, _f64p, _f64p, _f64p, _f64p, I64,
+                               _i64p, _f64p, _f64p]
+        L.orc_grad.restype = None
+        L.orc_rhs.argtypes = [cfgp, I64, C.c_int, _i64p, _i64p, _f64p, _i64p,
+                              _f64p, _i64p, _f64p, _f64p, _f64p, _f64p, I64,
+                              _i64p, _u8p, _f64p, _f64p, _f64p]
+        L.orc_rhs.restype = C.c_int
+        L.orc_dt.argtypes = [cfgp, I64, C.c_int, _f64p, _f64p, D]
+        L.orc_dt.restype = D
+        L.orc_symbc.argtypes = [I64, C.c_int, _f64p, I64, _i64p, _f64p]
+        L.orc_symbc.restype = None
+        L.orc_riecg_step.argtypes = [cfgp, I64, C.c_int, _i64p, _i64p, _f64p,
+                                     _i64p, _f64p, _i64p, _f64p, _f64p,
+                                     _f64p, _f64p, I64, _i64p, _u8p, _f64p,
+                                     I64, _i64p, _f64p, D, _f64p, _f64p,
+                                     _f64p, _f64p]
+        L.orc_riecg_step.restype = C.c_int
+        L.orc_diag.argtypes = [I64, C.c_int, _f64p, _f64p, _f64p, _f64p]
+        L.orc_diag.restype = None
+        _LIB = L
+    return _LIB
+
+
+def cfg(gamma=1.4, flux="rusanov", stab2=False, stab2coef=0.2):
+    kinds = {"rusanov": 0, "hllc": 1}
+    return Cfg(gamma, kinds.get(flux, 99), int(bool(stab2)), stab2coef)
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def vol(coord, inpoel):
+    x, y, z = (_c(coord[i], np.float64) for i in range(3))
+    v = np.empty(x.size)
+    rc = lib().orc_vol(x.size, x, y, z, inpoel.shape[0],
+                       _c(inpoel, np.int64), v)
+    if rc:
+        raise ValueError("Element Jacobian non-positive")
+    return v
+
+
+def domint(coord, inpoel):
+    """Unique edges (E,2) p<q and dual-face integrals (E,3)."""
+    x, y, z = (_c(coord[i], np.float64) for i in range(3))
+    ntet = inpoel.shape[0]
+    cap = 6 * ntet
+    edge = np.empty(2 * cap, np.int64)
+    dint = np.empty(3 * cap)
+    ne = lib().orc_domint(x.size, x, y, z, ntet, _c(inpoel, np.int64), cap,
+                          edge, dint)
+    return edge[:2 * ne].reshape(-1, 2).copy(), dint[:3 * ne].reshape(-1, 3).copy()
+
+
+class Superedges:
+    """The (dsupedge, dsupint) triple of arrays consumed by riemann::grad/rhs."""
+
+    def __init__(self, sup, ints):
+        self.sup = [np.ascontiguousarray(s, dtype=np.int64).reshape(-1) for s in sup]
+        self.int = [np.ascontiguousarray(d, dtype=np.float64).reshape(-1) for d in ints]
+        self.nsup = np.array([self.sup[0].size // 4, self.sup[1].size // 3,
+                              self.sup[2].size // 2], dtype=np.int64)
+
+    @classmethod
+    def edges_only(cls, edge, dint):
+        return cls([np.zeros(0, np.int64), np.zeros(0, np.int64), edge],
+                   [np.zeros(0), np.zeros(0), dint])
+
+    def args(self):
+        # keep at least one element so ndpointer never sees a None
+        s = [a if a.size else np.zeros(1, a.dtype) for a in self.sup]
+        d = [a if a.size else np.zeros(1) for a in self.int]
+        return [self.nsup, s[0], d[0], s[1], d[1], s[2], d[2]]
+
+
+def domsuped(inpoel, edge, dint):
+    ntet = inpoel.shape[0]
+    ne = edge.shape[0]
+    nsup = np.zeros(3, np.int64)
+    sup0 = np.empty(4 * ntet, np.int64); int0 = np.empty(18 * ntet)
+    sup1 = np.empty(12 * ntet, np.int64); int1 = np.empty(36 * ntet)
+    sup2 = np.empty(2 * ne, np.int64); int2 = np.empty(3 * ne)
+    rc = lib().orc_domsuped(ntet, _c(inpoel, np.int64), ne, _c(edge, np.int64),
+                            _c(dint, np.float64), nsup, sup0, int0, sup1, int1,
+                            sup2, int2)
+    if rc:
+        raise RuntimeError("Not all edges accounted for in superedge groups")
+    n0, n1, n2 = (int(v) for v in nsup)
+    return Superedges([sup0[:4 * n0], sup1[:3 * n1], sup2[:2 * n2]],
+                      [int0[:18 * n0], int1[:9 * n1], int2[:3 * n2]])
+
+
+def symbc_setup(coord, tri, faceset, symsets):
+    x, y, z = (_c(coord[i], np.float64) for i in range(3))
+    nf = tri.shape[0]
+    cap = 3 * nf * max(1, len(symsets))
+    nodes = np.empty(cap, np.int64)
+    norms = np.empty(3 * cap)
+    besym = np.empty(3 * nf, np.uint8)
+    ss = np.array(symsets, np.int32) if len(symsets) else np.zeros(1, np.int32)
+    ns = lib().orc_symbc_setup(x.size, x, y, z, nf, _c(tri, np.int64),
+                               _c(faceset, np.int32), len(symsets), ss, cap,
+                               nodes, norms, besym)
+    if ns < 0:
+        raise RuntimeError("symmetry list overflow")
+    return nodes[:ns].copy(), norms[:3 * ns].reshape(-1, 3).copy(), besym
+
+
+def grad(se, coord, tri, U):
+    x, y, z = (_c(coord[i], np.float64) for i in range(3))
+    U = _c(U, np.float64)
+    npoin, ncomp = U.shape
+    G = np.empty((npoin, 3 * ncomp))
+    t = _c(tri, np.int64) if tri.size else np.zeros(3, np.int64)
+    lib().orc_grad(npoin, ncomp, *se.args(), x, y, z, tri.shape[0], t, U, G)
+    return G
+
+
+def rhs(c, se, coord, tri, besym, G, U):
+    x, y, z = (_c(coord[i], np.float64) for i in range(3))
+    U = _c(U, np.float64)
+    npoin, ncomp = U.shape
+    R = np.empty((npoin, ncomp))
+    t = _c(tri, np.int64) if tri.size else np.zeros(3, np.int64)
+    b = _c(besym, np.uint8) if besym.size else np.zeros(3, np.uint8)
+    rc = lib().orc_rhs(C.byref(c), npoin, ncomp, *se.args(), x, y, z,
+                       tri.shape[0], t, b, _c(G, np.float64), U, R)
+    if rc == -1:
+        raise ValueError("Flux not configured")
+    if rc:
+        raise ValueError("unsupported number of components")
+    return R
+
+
+def dt(c, U, v, cfl):
+    U = _c(U, np.float64)
+    return lib().orc_dt(C.byref(c), U.shape[0], U.shape[1], U, _c(v, np.float64), cfl)
+
+
+def riecg_step(c, se, coord, tri, besym, v, symnodes, symnorms, dt, U):
+    """One 3-stage RieCG step in place on U; returns Un (state at time n)."""
+    x, y, z = (_c(coord[i], np.float64) for i in range(3))
+    npoin, ncomp = U.shape
+    Un = np.empty_like(U)
+    G = np.empty((npoin, 3 * ncomp))
+    R = np.empty((npoin, ncomp))
+    sn = symnodes if symnodes.size else np.zeros(1, np.int64)
+    sv = _c(symnorms, np.float64).reshape(-1) if symnodes.size else np.zeros(3)
+    rc = lib().orc_riecg_step(C.byref(c), npoin, ncomp, *se.args(), x, y, z,
+                              tri.shape[0], _c(tri, np.int64),
+                              _c(besym, np.uint8), _c(v, np.float64),
+                              symnodes.size, _c(sn, np.int64), sv, dt, U, Un,
+                              G, R)
+    if rc:
+        raise ValueError("riecg step failed (%d)" % rc)
+    return Un
+
+
+def diag(U, Un, v):
+    U = _c(U, np.float64)
+    out = np.empty(2 * U.shape[1] + 1)
+    lib().orc_diag(U.shape[0], U.shape[1], U, _c(Un, np.float64),
+                   _c(v, np.float64), out)
+    return out

@@ -1,0 +1,117 @@
+/* CPU oracle for the edge-based flow path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference algorithms, used by tests/, by the
+ * smoke check and by the CPU-baseline leg of bench.py to check and to time
+ * against.  The product path (xyst_amd/) never links or calls it.
+ *
+ * Conventions follow the reference: 0-based local node ids, FP64, nodal
+ * fields stored unknown-major AoS, U(i,c) = U[i*ncomp + c] (Data.hpp:420-426).
+ * Each function names the reference lines it restates.
+ */
+#ifndef XYST_ORACLE_H
+#define XYST_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Physics configuration read from g_cfg by the reference. */
+typedef struct {
+  double gamma;       /* mat_spec_heat_ratio (EOS.hpp:30-44) */
+  int    flux;        /* 0 = rusanov, 1 = hllc (Riemann.cpp:674-684) */
+  int    stab2;       /* artificial viscosity on/off (Riemann.cpp:449-458) */
+  double stab2coef;
+} orc_cfg;
+
+/* Discretization::vol, Discretization.cpp:619-647: V/4 of every tet summed to
+ * its nodes.  Returns 0, or -1 if a tet has a non-positive Jacobian. */
+int orc_vol(int64_t npoin, const double* x, const double* y, const double* z,
+            int64_t ntet, const int64_t* inpoel, double* vol);
+
+/* RieCG::domint, RieCG.cpp:339-382.  Unique edges (p<q), sorted
+ * lexicographically, with the edge integral D accumulated in tet order.
+ * edge[2*cap], dint[3*cap].  Returns the number of edges (or -1 if cap is too
+ * small). */
+int64_t orc_domint(int64_t npoin, const double* x, const double* y,
+                   const double* z, int64_t ntet, const int64_t* inpoel,
+                   int64_t cap, int64_t* edge, double* dint);
+
+/* RieCG::domsuped, RieCG.cpp:620-736.  Greedy superedges: tets whose six edges
+ * are all free, then tet faces (first-seen order) whose three edges are free,
+ * then the remaining edges in (p,q) order.  Outputs are sized by the caller:
+ * sup0[4*ntet], int0[18*ntet], sup1[3*4*ntet], int1[9*4*ntet],
+ * sup2[2*nedge], int2[3*nedge]; nsup[3] receives the group counts. */
+int orc_domsuped(int64_t ntet, const int64_t* inpoel, int64_t nedge,
+                 const int64_t* edge, const double* dint, int64_t* nsup,
+                 int64_t* sup0, double* int0, int64_t* sup1, double* int1,
+                 int64_t* sup2, double* int2);
+
+/* RieCG::bndint + bnorm + streamable (RieCG.cpp:281-337, 483-618) restricted
+ * to symmetry BCs: inverse-distance^2-weighted point normals per side set,
+ * symmetry node list with one normal per (node, symmetry set) pair in the
+ * order of symsets[], and the besym flag of every boundary-triangle node.
+ * nfaces boundary triangles tri[3*nfaces] with side set ids faceset[nfaces].
+ * symnodes[cap], symnorms[3*cap], besym[3*nfaces].  Returns the number of
+ * (node, normal) pairs. */
+int64_t orc_symbc_setup(int64_t npoin, const double* x, const double* y,
+                        const double* z, int64_t nfaces, const int64_t* tri,
+                        const int32_t* faceset, int32_t nsym,
+                        const int32_t* symsets, int64_t cap,
+                        int64_t* symnodes, double* symnorms, uint8_t* besym);
+
+/* riemann::grad, Riemann.cpp:229-367.  G is (npoin, 3*ncomp), zeroed here. */
+void orc_grad(int64_t npoin, int ncomp, const int64_t* nsup,
+              const int64_t* sup0, const double* int0,
+              const int64_t* sup1, const double* int1,
+              const int64_t* sup2, const double* int2,
+              const double* x, const double* y, const double* z,
+              int64_t nbtri, const int64_t* triinpoel,
+              const double* U, double* G);
+
+/* riemann::rhs, Riemann.cpp:909-946 (advdom 652-766, advbnd 768-878; the
+ * source term src() is null for the problems covered here).  G is the nodal
+ * gradient already divided by the nodal volume.  R is zeroed here. */
+int orc_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp, const int64_t* nsup,
+            const int64_t* sup0, const double* int0,
+            const int64_t* sup1, const double* int1,
+            const int64_t* sup2, const double* int2,
+            const double* x, const double* y, const double* z,
+            int64_t nbtri, const int64_t* triinpoel, const uint8_t* besym,
+            const double* G, const double* U, double* R);
+
+/* RieCG::dt, RieCG.cpp:787-852 (unsteady branch): cfl * min over nodes of
+ * cbrt(vol)/max(|u|+c, 1e-8). */
+double orc_dt(const orc_cfg* cfg, int64_t npoin, int ncomp, const double* U,
+              const double* vol, double cfl);
+
+/* physics::symbc, BC.cpp:110-136 (pos = 1). */
+void orc_symbc(int64_t npoin, int ncomp, double* U, int64_t nsym,
+               const int64_t* symnodes, const double* symnorms);
+
+/* One complete RieCG time step: rhs-stage loop of RieCG::advance/grad/rhs/
+ * solve/stage (RieCG.cpp:854-1057, 1409-1421) with rkcoef {1/3,1/2,1}
+ * (RieCG.cpp:41) and symmetry BCs after each stage.  dt is given (the caller
+ * evaluates orc_dt and setdt truncation).  Un receives U at time n.
+ * Work arrays: G (npoin*3*ncomp), R (npoin*ncomp). */
+int orc_riecg_step(const orc_cfg* cfg, int64_t npoin, int ncomp,
+                   const int64_t* nsup, const int64_t* sup0, const double* int0,
+                   const int64_t* sup1, const double* int1,
+                   const int64_t* sup2, const double* int2,
+                   const double* x, const double* y, const double* z,
+                   int64_t nbtri, const int64_t* triinpoel,
+                   const uint8_t* besym, const double* vol,
+                   int64_t nsym, const int64_t* symnodes,
+                   const double* symnorms, double dt,
+                   double* U, double* Un, double* G, double* R);
+
+/* NodeDiagnostics::rhocompute, NodeDiagnostics.cpp:46-146 (no analytic
+ * solution): out[0..ncomp) = sum U^2 V, out[ncomp..2ncomp) = sum (U-Un)^2 V,
+ * out[2ncomp] = sum rhoE V. */
+void orc_diag(int64_t npoin, int ncomp, const double* U, const double* Un,
+              const double* v, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,74 @@
+"""Extract the reference's RieCG/Sod regression fixture into riecg_sod.npz.
+
+Source files (reference test data, read as data only):
+  tests/regression/inciter/RieCG/Sod/rectangle_01_1.5k.exo   input mesh
+  tests/regression/inciter/RieCG/Sod/sod.std.exo             fields after 10 steps
+  tests/regression/inciter/RieCG/Sod/diag.std                per-step diagnostics
+Run configuration (sod.q): nstep 10, term 0.2, cfl 0.5, gamma 1.4,
+symmetry BCs on side sets 2, 4, 5, 6.  The output mesh is renumbered by the
+solver, so output nodes are matched to input nodes by coordinates.
+"""
+import os
+import sys
+
+import numpy as np
+from scipy.io import netcdf_file
+
+REF = "/root/reference/tests/regression/inciter/RieCG/Sod"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "riecg_sod.npz")
+
+
+def main():
+    f = netcdf_file(os.path.join(REF, "rectangle_01_1.5k.exo"), "r", mmap=False)
+    coord = np.stack([f.variables[c].data.astype(np.float64)
+                      for c in ("coordx", "coordy", "coordz")])
+    tri_blk = f.variables["connect1"].data.astype(np.int64) - 1
+    tets = f.variables["connect2"].data.astype(np.int64) - 1
+    ntri = tri_blk.shape[0]
+    tris, sets = [], []
+    nss = f.dimensions["num_side_sets"]
+    ids = f.variables["ss_prop1"].data
+    for s in range(1, nss + 1):
+        elem = f.variables["elem_ss%d" % s].data.astype(np.int64)
+        assert (elem <= ntri).all(), "side set on tet sides not handled"
+        tris.append(tri_blk[elem - 1])
+        sets.append(np.full(elem.size, ids[s - 1], np.int32))
+    tri = np.concatenate(tris)
+    faceset = np.concatenate(sets)
+    # The solver does not use the file's triangle orientation: the
+    # partitioner rebuilds each boundary face from its tet as one of
+    # {t0,t2,t1}, {t0,t1,t3}, {t0,t3,t2}, {t1,t2,t3} (Partitioner.cpp:578-587),
+    # i.e. outward for positively oriented tets.
+    tetface = {}
+    for t in tets:
+        for f in ((t[0], t[2], t[1]), (t[0], t[1], t[3]), (t[0], t[3], t[2]),
+                  (t[1], t[2], t[3])):
+            tetface[tuple(sorted(f))] = f
+    tri = np.array([tetface[tuple(sorted(f))] for f in tri], np.int64)
+
+    g = netcdf_file(os.path.join(REF, "sod.std.exo"), "r", mmap=False)
+    Y = np.stack([g.variables[c].data.astype(np.float64)
+                  for c in ("coordx", "coordy", "coordz")])
+    # match output nodes to input nodes by coordinates
+    key = lambda P: np.lexsort(np.round(P, 12)[::-1])
+    ia, ib = key(coord), key(Y)
+    assert np.abs(coord[:, ia] - Y[:, ib]).max() < 1e-12
+    perm = np.empty(coord.shape[1], np.int64)
+    perm[ia] = ib      # input node i -> output node perm[i]
+    names = [b"".join(x).decode().strip("\x00 ")
+             for x in g.variables["name_nod_var"].data]
+    fields = np.stack([g.variables["vals_nod_var%d" % (i + 1)].data[-1][perm]
+                       for i in range(len(names))], 1)
+    t_final = float(g.variables["time_whole"].data[-1])
+
+    diag = np.loadtxt(os.path.join(REF, "diag.std"))
+    np.savez_compressed(OUT, coord=coord, inpoel=tets, triinpoel=tri,
+                        faceset=faceset, fields=fields,
+                        field_names=np.array(names), t_final=t_final,
+                        diag=diag, symsets=np.array([2, 4, 5, 6], np.int32),
+                        nstep=10, term=0.2, cfl=0.5, gamma=1.4)
+    print("wrote", OUT, fields.shape, names, t_final)
+
+
+if __name__ == "__main__":
+    sys.exit(main())
