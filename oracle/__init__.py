@@ -52,11 +52,11 @@ def lib():
         L.orc_symbc_setup.restype = I64
         L.orc_grad.argtypes = [I64, C.c_int, _i64p, _i64p, _f64p, _i64p,
                                _f64p, _i64p, _f64p, _f64p, _f64p, _f64p, I64,
-                               _i64p, _f64p, _f64p]
+                               _i64p, _f64p, _f64p, C.c_void_p]
         L.orc_grad.restype = None
         L.orc_rhs.argtypes = [cfgp, I64, C.c_int, _i64p, _i64p, _f64p, _i64p,
                               _f64p, _i64p, _f64p, _f64p, _f64p, _f64p, I64,
-                              _i64p, _u8p, _f64p, _f64p, _f64p]
+                              _i64p, _u8p, _f64p, _f64p, _f64p, C.c_void_p]
         L.orc_rhs.restype = C.c_int
         L.orc_dt.argtypes = [cfgp, I64, C.c_int, _f64p, _f64p, D]
         L.orc_dt.restype = D
@@ -159,30 +159,36 @@ def symbc_setup(coord, tri, faceset, symsets):
     return nodes[:ns].copy(), norms[:3 * ns].reshape(-1, 3).copy(), besym
 
 
-def grad(se, coord, tri, U):
+def grad(se, coord, tri, U, absacc=False):
+    """riemann::grad; with absacc also the per-node sum of |terms|."""
     x, y, z = (_c(coord[i], np.float64) for i in range(3))
     U = _c(U, np.float64)
     npoin, ncomp = U.shape
     G = np.empty((npoin, 3 * ncomp))
+    A = np.empty_like(G) if absacc else None
     t = _c(tri, np.int64) if tri.size else np.zeros(3, np.int64)
-    lib().orc_grad(npoin, ncomp, *se.args(), x, y, z, tri.shape[0], t, U, G)
-    return G
+    lib().orc_grad(npoin, ncomp, *se.args(), x, y, z, tri.shape[0], t, U, G,
+                   A.ctypes.data if absacc else None)
+    return (G, A) if absacc else G
 
 
-def rhs(c, se, coord, tri, besym, G, U):
+def rhs(c, se, coord, tri, besym, G, U, absacc=False):
+    """riemann::rhs; with absacc also the per-node sum of |terms|."""
     x, y, z = (_c(coord[i], np.float64) for i in range(3))
     U = _c(U, np.float64)
     npoin, ncomp = U.shape
     R = np.empty((npoin, ncomp))
+    A = np.empty_like(R) if absacc else None
     t = _c(tri, np.int64) if tri.size else np.zeros(3, np.int64)
     b = _c(besym, np.uint8) if besym.size else np.zeros(3, np.uint8)
     rc = lib().orc_rhs(C.byref(c), npoin, ncomp, *se.args(), x, y, z,
-                       tri.shape[0], t, b, _c(G, np.float64), U, R)
+                       tri.shape[0], t, b, _c(G, np.float64), U, R,
+                       A.ctypes.data if absacc else None)
     if rc == -1:
         raise ValueError("Flux not configured")
     if rc:
         raise ValueError("unsupported number of components")
-    return R
+    return (R, A) if absacc else R
 
 
 def dt(c, U, v, cfl):

@@ -320,10 +320,12 @@ void orc_grad(int64_t npoin, int ncomp, const int64_t* nsup,
               const int64_t* sup2, const double* int2,
               const double* x, const double* y, const double* z,
               int64_t nbtri, const int64_t* triinpoel,
-              const double* U, double* G) {
+              const double* U, double* G, double* A) {
   const int ng = 3*ncomp;
   memset(G, 0, sizeof(double) * (size_t)(npoin*ng));
+  if (A) memset(A, 0, sizeof(double) * (size_t)(npoin*ng));
 #define GG(n,k) G[(n)*ng + (k)]
+#define GA(n,k,v) do { if (A) A[(n)*ng + (k)] += fabs(v); } while (0)
   /* tet superedges, Riemann.cpp:263-289 */
   for (int64_t e = 0; e < nsup[0]; ++e) {
     const int64_t* N = sup0 + e*4;
@@ -343,6 +345,10 @@ void orc_grad(int64_t npoin, int ncomp, const int64_t* nsup,
         GG(N[1],c*3+j) = GG(N[1],c*3+j) + f[0] - f[1] - f[4];
         GG(N[2],c*3+j) = GG(N[2],c*3+j) + f[1] - f[2] - f[5];
         GG(N[3],c*3+j) = GG(N[3],c*3+j) + f[3] + f[4] + f[5];
+        GA(N[0],c*3+j, f[0]); GA(N[0],c*3+j, f[2]); GA(N[0],c*3+j, f[3]);
+        GA(N[1],c*3+j, f[0]); GA(N[1],c*3+j, f[1]); GA(N[1],c*3+j, f[4]);
+        GA(N[2],c*3+j, f[1]); GA(N[2],c*3+j, f[2]); GA(N[2],c*3+j, f[5]);
+        GA(N[3],c*3+j, f[3]); GA(N[3],c*3+j, f[4]); GA(N[3],c*3+j, f[5]);
       }
   }
   /* triangle superedges, Riemann.cpp:292-311 */
@@ -360,6 +366,9 @@ void orc_grad(int64_t npoin, int ncomp, const int64_t* nsup,
         GG(N[0],c*3+j) = GG(N[0],c*3+j) - f[0] + f[2];
         GG(N[1],c*3+j) = GG(N[1],c*3+j) + f[0] - f[1];
         GG(N[2],c*3+j) = GG(N[2],c*3+j) + f[1] - f[2];
+        GA(N[0],c*3+j, f[0]); GA(N[0],c*3+j, f[2]);
+        GA(N[1],c*3+j, f[0]); GA(N[1],c*3+j, f[1]);
+        GA(N[2],c*3+j, f[1]); GA(N[2],c*3+j, f[2]);
       }
   }
   /* edges, Riemann.cpp:314-328 */
@@ -374,6 +383,7 @@ void orc_grad(int64_t npoin, int ncomp, const int64_t* nsup,
         double f = d[j] * (u[1][c] + u[0][c]);
         GG(N[0],c*3+j) -= f;
         GG(N[1],c*3+j) += f;
+        GA(N[0],c*3+j, f); GA(N[1],c*3+j, f);
       }
   }
   /* boundary integral, Riemann.cpp:332-359 */
@@ -396,10 +406,13 @@ void orc_grad(int64_t npoin, int ncomp, const int64_t* nsup,
         GG(N[0],c*3+j) += g[j] * n[j];
         GG(N[1],c*3+j) += g[j] * n[j];
         GG(N[2],c*3+j) += g[j] * n[j];
+        GA(N[0],c*3+j, g[j]*n[j]); GA(N[1],c*3+j, g[j]*n[j]);
+        GA(N[2],c*3+j, g[j]*n[j]);
       }
     }
   }
 #undef GG
+#undef GA
 }
 
 /* ------------------------------------------------------------------- rhs -- */
@@ -608,14 +621,16 @@ int orc_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp, const int64_t* nsup,
             const int64_t* sup2, const double* int2,
             const double* x, const double* y, const double* z,
             int64_t nbtri, const int64_t* triinpoel, const uint8_t* besym,
-            const double* G, const double* U, double* R) {
+            const double* G, const double* U, double* R, double* A) {
   fluxfn flux;
   if (cfg->flux == 0) flux = rusanov;
   else if (cfg->flux == 1) flux = hllc;
   else return -1;                      /* "Flux not configured" */
   if (ncomp > MAXC || ncomp < 5) return -2;
   memset(R, 0, sizeof(double) * (size_t)(npoin*ncomp));
+  if (A) memset(A, 0, sizeof(double) * (size_t)(npoin*ncomp));
 #define RR(n,c) R[(n)*ncomp + (c)]
+#define RA(n,c,v) do { if (A) A[(n)*ncomp + (c)] += fabs(v); } while (0)
   /* advdom tet superedges, Riemann.cpp:700-725 */
   for (int64_t e = 0; e < nsup[0]; ++e) {
     const int64_t* N = sup0 + e*4;
@@ -633,6 +648,10 @@ int orc_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp, const int64_t* nsup,
       RR(N[1],c) = RR(N[1],c) + f[0][c] - f[1][c] - f[4][c];
       RR(N[2],c) = RR(N[2],c) + f[1][c] - f[2][c] - f[5][c];
       RR(N[3],c) = RR(N[3],c) + f[3][c] + f[4][c] + f[5][c];
+      RA(N[0],c,f[0][c]); RA(N[0],c,f[2][c]); RA(N[0],c,f[3][c]);
+      RA(N[1],c,f[0][c]); RA(N[1],c,f[1][c]); RA(N[1],c,f[4][c]);
+      RA(N[2],c,f[1][c]); RA(N[2],c,f[2][c]); RA(N[2],c,f[5][c]);
+      RA(N[3],c,f[3][c]); RA(N[3],c,f[4][c]); RA(N[3],c,f[5][c]);
     }
   }
   /* advdom triangle superedges, Riemann.cpp:728-746 */
@@ -648,6 +667,9 @@ int orc_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp, const int64_t* nsup,
       RR(N[0],c) = RR(N[0],c) - f[0][c] + f[2][c];
       RR(N[1],c) = RR(N[1],c) + f[0][c] - f[1][c];
       RR(N[2],c) = RR(N[2],c) + f[1][c] - f[2][c];
+      RA(N[0],c,f[0][c]); RA(N[0],c,f[2][c]);
+      RA(N[1],c,f[0][c]); RA(N[1],c,f[1][c]);
+      RA(N[2],c,f[1][c]); RA(N[2],c,f[2][c]);
     }
   }
   /* advdom edges, Riemann.cpp:749-759 */
@@ -660,6 +682,7 @@ int orc_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp, const int64_t* nsup,
     for (int c = 0; c < ncomp; ++c) {
       RR(N[0],c) -= f[c];
       RR(N[1],c) += f[c];
+      RA(N[0],c,f[c]); RA(N[1],c,f[c]);
     }
   }
   /* advbnd, Riemann.cpp:795-871 */
@@ -692,9 +715,12 @@ int orc_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp, const int64_t* nsup,
       RR(N[0],c) += fab + fca + f[c][0];
       RR(N[1],c) += fab + fbc + f[c][1];
       RR(N[2],c) += fbc + fca + f[c][2];
+      RA(N[0],c,fab + fca + f[c][0]); RA(N[1],c,fab + fbc + f[c][1]);
+      RA(N[2],c,fbc + fca + f[c][2]);
     }
   }
 #undef RR
+#undef RA
   return 0;
 }
 
@@ -735,11 +761,11 @@ int orc_riecg_step(const orc_cfg* cfg, int64_t npoin, int ncomp,
   const int ng = 3*ncomp;
   for (int stage = 0; stage < 3; ++stage) {
     orc_grad(npoin, ncomp, nsup, sup0, int0, sup1, int1, sup2, int2, x, y, z,
-             nbtri, triinpoel, U, G);
+             nbtri, triinpoel, U, G, NULL);
     for (int64_t p = 0; p < npoin; ++p)          /* RieCG.cpp:935-939 */
       for (int c = 0; c < ng; ++c) G[p*ng+c] /= vol[p];
     int rc = orc_rhs(cfg, npoin, ncomp, nsup, sup0, int0, sup1, int1, sup2,
-                     int2, x, y, z, nbtri, triinpoel, besym, G, U, R);
+                     int2, x, y, z, nbtri, triinpoel, besym, G, U, R, NULL);
     if (rc) return rc;
     if (stage == 0) memcpy(Un, U, sizeof(double)*(size_t)(npoin*ncomp));
     for (int64_t i = 0; i < npoin; ++i)          /* RieCG.cpp:1016-1021 */

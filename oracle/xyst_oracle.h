@@ -60,25 +60,28 @@ int64_t orc_symbc_setup(int64_t npoin, const double* x, const double* y,
                         const int32_t* symsets, int64_t cap,
                         int64_t* symnodes, double* symnorms, uint8_t* besym);
 
-/* riemann::grad, Riemann.cpp:229-367.  G is (npoin, 3*ncomp), zeroed here. */
+/* riemann::grad, Riemann.cpp:229-367.  G is (npoin, 3*ncomp), zeroed here.
+ * A (nullable, same shape) receives the sum of the absolute values of every
+ * term added to G -- the scale of round-off at each node. */
 void orc_grad(int64_t npoin, int ncomp, const int64_t* nsup,
               const int64_t* sup0, const double* int0,
               const int64_t* sup1, const double* int1,
               const int64_t* sup2, const double* int2,
               const double* x, const double* y, const double* z,
               int64_t nbtri, const int64_t* triinpoel,
-              const double* U, double* G);
+              const double* U, double* G, double* A);
 
 /* riemann::rhs, Riemann.cpp:909-946 (advdom 652-766, advbnd 768-878; the
  * source term src() is null for the problems covered here).  G is the nodal
- * gradient already divided by the nodal volume.  R is zeroed here. */
+ * gradient already divided by the nodal volume.  R is zeroed here.  A
+ * (nullable) receives the sum of absolute values of the terms added to R. */
 int orc_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp, const int64_t* nsup,
             const int64_t* sup0, const double* int0,
             const int64_t* sup1, const double* int1,
             const int64_t* sup2, const double* int2,
             const double* x, const double* y, const double* z,
             int64_t nbtri, const int64_t* triinpoel, const uint8_t* besym,
-            const double* G, const double* U, double* R);
+            const double* G, const double* U, double* R, double* A);
 
 /* RieCG::dt, RieCG.cpp:787-852 (unsteady branch): cfl * min over nodes of
  * cbrt(vol)/max(|u|+c, 1e-8). */

@@ -1,0 +1,74 @@
+"""The C-ABI library loads, exports every symbol include/xg.h declares, and
+its host-side preprocessing (no GPU needed) matches the oracle bit for bit."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.meshes import box_case, sod_fixture
+from xyst_amd import _xg
+from xyst_amd import riemann as X
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "xg.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(xg_\w+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_xg.LIBPATH)
+    names = _declared()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_xg.SIG), "binding table out of sync with xg.h"
+
+
+def test_context_without_gpu_fails_loudly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(_xg.XgError):
+        X.Context(0)
+
+
+@pytest.mark.parametrize("case", ["box", "sod"])
+def test_host_preprocessing_matches_oracle(case):
+    c = box_case(5, seed=8) if case == "box" else sod_fixture()[0]
+    np.testing.assert_array_equal(X.mesh_vol(c.coord, c.inpoel), c.vol)
+    e, d = X.mesh_domint(c.coord, c.inpoel)
+    np.testing.assert_array_equal(e.astype(np.int64), c.edge)
+    np.testing.assert_array_equal(d, c.dint)
+    n, nr, b = X.mesh_symbc(c.coord, c.tri, c.faceset, c.symsets)
+    np.testing.assert_array_equal(n, c.symnodes)
+    np.testing.assert_array_equal(nr, c.symnorms)
+    np.testing.assert_array_equal(b, c.besym)
+
+
+def test_rcm_is_permutation_and_reduces_bandwidth():
+    c = box_case(6, seed=2)
+    perm = X.mesh_rcm(c.npoin, c.inpoel)
+    assert np.array_equal(np.sort(perm), np.arange(c.npoin))
+    bw = lambda e: int(np.max(np.abs(e[:, 0] - e[:, 1])))
+    # a random relabelling has a large bandwidth; RCM restores a small one
+    rnd = np.random.default_rng(0).permutation(c.npoin)
+    inp = rnd[c.inpoel]
+    e0 = rnd[c.edge]
+    p2 = X.mesh_rcm(c.npoin, inp)
+    assert bw(p2[e0]) < bw(e0) / 4
+
+
+def test_error_codes():
+    with pytest.raises(_xg.XgError):
+        X.mesh_vol(np.zeros((3, 4)), np.array([[0, 1, 2, 3]]))     # J = 0
+    c = box_case(2)
+    bad = c.inpoel.copy()
+    bad[0, 0] = c.npoin + 5
+    with pytest.raises(_xg.XgError):
+        X.mesh_domint(c.coord, bad)

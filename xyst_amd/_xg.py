@@ -1,0 +1,89 @@
+"""ctypes binding of libxg.so (include/xg.h).
+
+This is the stub a Python caller adds to reach the C-ABI; it loads the
+in-tree library and fails loudly when it is missing -- there is no CPU
+fallback for any entry point.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libxg.so")
+
+XG_OK, XG_EINVAL, XG_ENOMESH, XG_EHIP, XG_EUNSUPPORTED, XG_EBLOWUP = range(6)
+_NAMES = {1: "XG_EINVAL", 2: "XG_ENOMESH", 3: "XG_EHIP", 4: "XG_EUNSUPPORTED",
+          5: "XG_EBLOWUP"}
+
+# every symbol include/xg.h declares, with (restype, argtypes)
+I64, I32, D, VP = C.c_int64, C.c_int32, C.c_double, C.c_void_p
+SIG = {
+    "xg_ctx_create": (C.c_int, [C.c_int, C.POINTER(VP)]),
+    "xg_ctx_destroy": (C.c_int, [VP]),
+    "xg_last_error": (C.c_char_p, [VP]),
+    "xg_set_stream": (C.c_int, [VP, VP]),
+    "xg_set_config": (C.c_int, [VP, VP]),
+    "xg_mesh_vol": (C.c_int, [I64, VP, VP, VP, I64, VP, VP]),
+    "xg_mesh_domint": (C.c_int, [I64, VP, VP, VP, I64, VP, I64, C.POINTER(I64), VP, VP]),
+    "xg_mesh_rcm": (C.c_int, [I64, I64, VP, VP]),
+    "xg_mesh_symbc": (C.c_int, [I64, VP, VP, VP, I64, VP, VP, I32, VP, I64,
+                                C.POINTER(I64), VP, VP, VP]),
+    "xg_mesh_upload": (C.c_int, [VP, I64, VP, VP, VP, VP, VP, VP, VP, VP, VP,
+                                 VP, I64, VP, VP]),
+    "xg_riemann_grad": (C.c_int, [VP, C.c_int, VP, VP]),
+    "xg_riemann_grad_dev": (C.c_int, [VP, C.c_int, VP, VP]),
+    "xg_riemann_rhs": (C.c_int, [VP, C.c_int, VP, VP, VP]),
+    "xg_riemann_rhs_dev": (C.c_int, [VP, C.c_int, VP, VP, VP]),
+    "xg_riecg_setup": (C.c_int, [VP, VP, I64, VP, VP]),
+    "xg_riecg_dt_dev": (C.c_int, [VP, VP, D, C.POINTER(D)]),
+    "xg_riecg_stage_dev": (C.c_int, [VP, C.c_int, D, VP, VP, VP, VP]),
+    "xg_mesh_device_bytes": (C.c_int, [VP, C.POINTER(I64)]),
+}
+
+
+class XgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (_NAMES.get(code, code), msg))
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("flux", C.c_int), ("stab2", C.c_int),
+                ("stab2coef", C.c_double)]
+
+
+_LIB = None
+
+
+def lib():
+    """Load libxg.so (built by __graft_entry__.build / make in csrc)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBPATH):
+            raise ImportError("xyst_amd: %s is not built (run __graft_entry__.build())"
+                              % LIBPATH)
+        L = C.CDLL(LIBPATH)
+        for name, (res, args) in SIG.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def ptr(a):
+    """Address of a C-contiguous numpy array or a torch tensor's storage."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+        return a.ctypes.data
+    assert a.is_contiguous(), "tensor must be contiguous"
+    return a.data_ptr()
+
+
+def check(rc, ctx=None, what=""):
+    if rc != XG_OK:
+        msg = lib().xg_last_error(ctx).decode() if ctx else what
+        raise XgError(rc, msg or what)
