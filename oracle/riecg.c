@@ -345,10 +345,17 @@ void orc_grad(int64_t npoin, int ncomp, const int64_t* nsup,
         GG(N[1],c*3+j) = GG(N[1],c*3+j) + f[0] - f[1] - f[4];
         GG(N[2],c*3+j) = GG(N[2],c*3+j) + f[1] - f[2] - f[5];
         GG(N[3],c*3+j) = GG(N[3],c*3+j) + f[3] + f[4] + f[5];
-        GA(N[0],c*3+j, f[0]); GA(N[0],c*3+j, f[2]); GA(N[0],c*3+j, f[3]);
-        GA(N[1],c*3+j, f[0]); GA(N[1],c*3+j, f[1]); GA(N[1],c*3+j, f[4]);
-        GA(N[2],c*3+j, f[1]); GA(N[2],c*3+j, f[2]); GA(N[2],c*3+j, f[5]);
-        GA(N[3],c*3+j, f[3]); GA(N[3],c*3+j, f[4]); GA(N[3],c*3+j, f[5]);
+        double s[6];
+        s[0] = d[(e*6+0)*3+j] * (fabs(u[1][c]) + fabs(u[0][c]));
+        s[1] = d[(e*6+1)*3+j] * (fabs(u[2][c]) + fabs(u[1][c]));
+        s[2] = d[(e*6+2)*3+j] * (fabs(u[0][c]) + fabs(u[2][c]));
+        s[3] = d[(e*6+3)*3+j] * (fabs(u[3][c]) + fabs(u[0][c]));
+        s[4] = d[(e*6+4)*3+j] * (fabs(u[3][c]) + fabs(u[1][c]));
+        s[5] = d[(e*6+5)*3+j] * (fabs(u[3][c]) + fabs(u[2][c]));
+        GA(N[0],c*3+j, s[0]); GA(N[0],c*3+j, s[2]); GA(N[0],c*3+j, s[3]);
+        GA(N[1],c*3+j, s[0]); GA(N[1],c*3+j, s[1]); GA(N[1],c*3+j, s[4]);
+        GA(N[2],c*3+j, s[1]); GA(N[2],c*3+j, s[2]); GA(N[2],c*3+j, s[5]);
+        GA(N[3],c*3+j, s[3]); GA(N[3],c*3+j, s[4]); GA(N[3],c*3+j, s[5]);
       }
   }
   /* triangle superedges, Riemann.cpp:292-311 */
@@ -366,9 +373,13 @@ void orc_grad(int64_t npoin, int ncomp, const int64_t* nsup,
         GG(N[0],c*3+j) = GG(N[0],c*3+j) - f[0] + f[2];
         GG(N[1],c*3+j) = GG(N[1],c*3+j) + f[0] - f[1];
         GG(N[2],c*3+j) = GG(N[2],c*3+j) + f[1] - f[2];
-        GA(N[0],c*3+j, f[0]); GA(N[0],c*3+j, f[2]);
-        GA(N[1],c*3+j, f[0]); GA(N[1],c*3+j, f[1]);
-        GA(N[2],c*3+j, f[1]); GA(N[2],c*3+j, f[2]);
+        double s[3];
+        s[0] = d[(e*3+0)*3+j] * (fabs(u[1][c]) + fabs(u[0][c]));
+        s[1] = d[(e*3+1)*3+j] * (fabs(u[2][c]) + fabs(u[1][c]));
+        s[2] = d[(e*3+2)*3+j] * (fabs(u[0][c]) + fabs(u[2][c]));
+        GA(N[0],c*3+j, s[0]); GA(N[0],c*3+j, s[2]);
+        GA(N[1],c*3+j, s[0]); GA(N[1],c*3+j, s[1]);
+        GA(N[2],c*3+j, s[1]); GA(N[2],c*3+j, s[2]);
       }
   }
   /* edges, Riemann.cpp:314-328 */
@@ -383,7 +394,8 @@ void orc_grad(int64_t npoin, int ncomp, const int64_t* nsup,
         double f = d[j] * (u[1][c] + u[0][c]);
         GG(N[0],c*3+j) -= f;
         GG(N[1],c*3+j) += f;
-        GA(N[0],c*3+j, f); GA(N[1],c*3+j, f);
+        double sf = d[j] * (fabs(u[1][c]) + fabs(u[0][c]));
+        GA(N[0],c*3+j, sf); GA(N[1],c*3+j, sf);
       }
   }
   /* boundary integral, Riemann.cpp:332-359 */
@@ -406,8 +418,8 @@ void orc_grad(int64_t npoin, int ncomp, const int64_t* nsup,
         GG(N[0],c*3+j) += g[j] * n[j];
         GG(N[1],c*3+j) += g[j] * n[j];
         GG(N[2],c*3+j) += g[j] * n[j];
-        GA(N[0],c*3+j, g[j]*n[j]); GA(N[1],c*3+j, g[j]*n[j]);
-        GA(N[2],c*3+j, g[j]*n[j]);
+        double sg = 1.5*(fabs(u[0][c]) + fabs(u[1][c]) + fabs(u[2][c]))*n[j];
+        GA(N[0],c*3+j, sg); GA(N[1],c*3+j, sg); GA(N[2],c*3+j, sg);
       }
     }
   }
@@ -485,6 +497,10 @@ static void muscl_scalar(int ncomp, int64_t p, int64_t q, const double* x,
   }
 }
 
+/* Magnitude of the products each flux component of the last flux call was
+ * built from: the scale of its round-off, used as the parity yardstick. */
+static double fsc[MAXC];
+
 /* Rusanov flux, Riemann.cpp:369-478 (symL = symR = 0 in the domain) */
 static void rusanov(const orc_cfg* cfg, int ncomp, const double* x,
                     const double* y, const double* z, const double* G,
@@ -513,6 +529,11 @@ static void rusanov(const orc_cfg* cfg, int ncomp, const double* x,
   f[2] = l[2]*vnL + r[2]*vnR + (pL + pR)*ny + fw*(r[2] - l[2]);
   f[3] = l[3]*vnL + r[3]*vnR + (pL + pR)*nz + fw*(r[3] - l[3]);
   f[4] = (l[4] + pL)*vnL + (r[4] + pR)*vnR + fw*(r[4] - l[4]);
+  double fwt = fw * (1.0 + (cfg->stab2 ? fabs(cfg->stab2coef) : 0.0));
+  for (int c = 0; c < 5; ++c)
+    fsc[c] = fabs(l[c]*vnL) + fabs(r[c]*vnR) + fwt*(fabs(l[c]) + fabs(r[c]));
+  for (int c = 1; c < 4; ++c) fsc[c] += (fabs(pL) + fabs(pR))*fabs(d[c-1]);
+  fsc[4] += fabs(pL*vnL) + fabs(pR*vnR);
   if (cfg->stab2) {
     double fws = cfg->stab2coef * fw;
     for (int c = 0; c < 5; ++c) f[c] -= fws*(l[c] - r[c]);
@@ -520,6 +541,8 @@ static void rusanov(const orc_cfg* cfg, int ncomp, const double* x,
   if (ncomp == 5) return;
   muscl_scalar(ncomp, p, q, x, y, z, G, l, r);
   double sw = fabs(vnL) > fabs(vnR) ? fabs(vnL) : fabs(vnR);
+  for (int c = 5; c < ncomp; ++c)
+    fsc[c] = fabs(l[c]*vnL) + fabs(r[c]*vnR) + sw*(fabs(l[c]) + fabs(r[c]));
   for (int c = 5; c < ncomp; ++c) f[c] = l[c]*vnL + r[c]*vnR + sw*(r[c] - l[c]);
 }
 
@@ -598,6 +621,14 @@ static void hllc(const orc_cfg* cfg, int ncomp, const double* x,
     f[3] = r[3]*qR2 + pR*nz;
     f[4] = (r[4] + pR)*qR2;
   }
+  {
+    double w = 2.0*len*(fabs(qL) + fabs(qR) + fabs(sL) + fabs(sR) + cL + cR) *
+               (1.0 + (cfg->stab2 ? fabs(cfg->stab2coef) : 0.0));
+    double pw = 2.0*len*(fabs(pL) + fabs(pR) + fabs(pS)) *
+                (1.0 + fabs(qL) + fabs(qR) + fabs(sM));
+    for (int c = 0; c < 5; ++c)
+      fsc[c] = w*(fabs(l[c]) + fabs(r[c]) + fabs(uL[c]) + fabs(uR[c])) + pw;
+  }
   if (cfg->stab2) {
     double sl = fabs(qL) + cL;
     double sr = fabs(qR) + cR;
@@ -607,8 +638,10 @@ static void hllc(const orc_cfg* cfg, int ncomp, const double* x,
   if (ncomp == 5) return;
   muscl_scalar(ncomp, p, q, x, y, z, G, l, r);
   double sw = (fabs(qL) > fabs(qR) ? fabs(qL) : fabs(qR)) * len;
-  for (int c = 5; c < ncomp; ++c)
+  for (int c = 5; c < ncomp; ++c) {
     f[c] = (l[c]*qL + r[c]*qR)*len + sw*(r[c] - l[c]);
+    fsc[c] = (fabs(l[c]*qL) + fabs(r[c]*qR))*len + sw*(fabs(r[c]) + fabs(l[c]));
+  }
 }
 
 typedef void (*fluxfn)(const orc_cfg*, int, const double*, const double*,
@@ -631,27 +664,31 @@ int orc_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp, const int64_t* nsup,
   if (A) memset(A, 0, sizeof(double) * (size_t)(npoin*ncomp));
 #define RR(n,c) R[(n)*ncomp + (c)]
 #define RA(n,c,v) do { if (A) A[(n)*ncomp + (c)] += fabs(v); } while (0)
+#define FLUX(k, dd, a, b, ua, ub, ff) \
+  do { flux(cfg, ncomp, x, y, z, G, dd, a, b, ua, ub, ff); \
+       memcpy(sc[k], fsc, sizeof(double)*(size_t)ncomp); } while (0)
+  double sc[6][MAXC];
   /* advdom tet superedges, Riemann.cpp:700-725 */
   for (int64_t e = 0; e < nsup[0]; ++e) {
     const int64_t* N = sup0 + e*4;
     double u[4][MAXC], f[6][MAXC];
     for (int a = 0; a < 4; ++a) primitive(ncomp, N[a], U, u[a]);
     const double* d = int0;
-    flux(cfg, ncomp, x, y, z, G, d+(e*6+0)*3, N[0], N[1], u[0], u[1], f[0]);
-    flux(cfg, ncomp, x, y, z, G, d+(e*6+1)*3, N[1], N[2], u[1], u[2], f[1]);
-    flux(cfg, ncomp, x, y, z, G, d+(e*6+2)*3, N[2], N[0], u[2], u[0], f[2]);
-    flux(cfg, ncomp, x, y, z, G, d+(e*6+3)*3, N[0], N[3], u[0], u[3], f[3]);
-    flux(cfg, ncomp, x, y, z, G, d+(e*6+4)*3, N[1], N[3], u[1], u[3], f[4]);
-    flux(cfg, ncomp, x, y, z, G, d+(e*6+5)*3, N[2], N[3], u[2], u[3], f[5]);
+    FLUX(0, d+(e*6+0)*3, N[0], N[1], u[0], u[1], f[0]);
+    FLUX(1, d+(e*6+1)*3, N[1], N[2], u[1], u[2], f[1]);
+    FLUX(2, d+(e*6+2)*3, N[2], N[0], u[2], u[0], f[2]);
+    FLUX(3, d+(e*6+3)*3, N[0], N[3], u[0], u[3], f[3]);
+    FLUX(4, d+(e*6+4)*3, N[1], N[3], u[1], u[3], f[4]);
+    FLUX(5, d+(e*6+5)*3, N[2], N[3], u[2], u[3], f[5]);
     for (int c = 0; c < ncomp; ++c) {
       RR(N[0],c) = RR(N[0],c) - f[0][c] + f[2][c] - f[3][c];
       RR(N[1],c) = RR(N[1],c) + f[0][c] - f[1][c] - f[4][c];
       RR(N[2],c) = RR(N[2],c) + f[1][c] - f[2][c] - f[5][c];
       RR(N[3],c) = RR(N[3],c) + f[3][c] + f[4][c] + f[5][c];
-      RA(N[0],c,f[0][c]); RA(N[0],c,f[2][c]); RA(N[0],c,f[3][c]);
-      RA(N[1],c,f[0][c]); RA(N[1],c,f[1][c]); RA(N[1],c,f[4][c]);
-      RA(N[2],c,f[1][c]); RA(N[2],c,f[2][c]); RA(N[2],c,f[5][c]);
-      RA(N[3],c,f[3][c]); RA(N[3],c,f[4][c]); RA(N[3],c,f[5][c]);
+      RA(N[0],c,sc[0][c]); RA(N[0],c,sc[2][c]); RA(N[0],c,sc[3][c]);
+      RA(N[1],c,sc[0][c]); RA(N[1],c,sc[1][c]); RA(N[1],c,sc[4][c]);
+      RA(N[2],c,sc[1][c]); RA(N[2],c,sc[2][c]); RA(N[2],c,sc[5][c]);
+      RA(N[3],c,sc[3][c]); RA(N[3],c,sc[4][c]); RA(N[3],c,sc[5][c]);
     }
   }
   /* advdom triangle superedges, Riemann.cpp:728-746 */
@@ -660,16 +697,16 @@ int orc_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp, const int64_t* nsup,
     double u[3][MAXC], f[3][MAXC];
     for (int a = 0; a < 3; ++a) primitive(ncomp, N[a], U, u[a]);
     const double* d = int1;
-    flux(cfg, ncomp, x, y, z, G, d+(e*3+0)*3, N[0], N[1], u[0], u[1], f[0]);
-    flux(cfg, ncomp, x, y, z, G, d+(e*3+1)*3, N[1], N[2], u[1], u[2], f[1]);
-    flux(cfg, ncomp, x, y, z, G, d+(e*3+2)*3, N[2], N[0], u[2], u[0], f[2]);
+    FLUX(0, d+(e*3+0)*3, N[0], N[1], u[0], u[1], f[0]);
+    FLUX(1, d+(e*3+1)*3, N[1], N[2], u[1], u[2], f[1]);
+    FLUX(2, d+(e*3+2)*3, N[2], N[0], u[2], u[0], f[2]);
     for (int c = 0; c < ncomp; ++c) {
       RR(N[0],c) = RR(N[0],c) - f[0][c] + f[2][c];
       RR(N[1],c) = RR(N[1],c) + f[0][c] - f[1][c];
       RR(N[2],c) = RR(N[2],c) + f[1][c] - f[2][c];
-      RA(N[0],c,f[0][c]); RA(N[0],c,f[2][c]);
-      RA(N[1],c,f[0][c]); RA(N[1],c,f[1][c]);
-      RA(N[2],c,f[1][c]); RA(N[2],c,f[2][c]);
+      RA(N[0],c,sc[0][c]); RA(N[0],c,sc[2][c]);
+      RA(N[1],c,sc[0][c]); RA(N[1],c,sc[1][c]);
+      RA(N[2],c,sc[1][c]); RA(N[2],c,sc[2][c]);
     }
   }
   /* advdom edges, Riemann.cpp:749-759 */
@@ -678,11 +715,11 @@ int orc_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp, const int64_t* nsup,
     double u[2][MAXC], f[MAXC];
     primitive(ncomp, N[0], U, u[0]);
     primitive(ncomp, N[1], U, u[1]);
-    flux(cfg, ncomp, x, y, z, G, int2 + e*3, N[0], N[1], u[0], u[1], f);
+    FLUX(0, int2 + e*3, N[0], N[1], u[0], u[1], f);
     for (int c = 0; c < ncomp; ++c) {
       RR(N[0],c) -= f[c];
       RR(N[1],c) += f[c];
-      RA(N[0],c,f[c]); RA(N[1],c,f[c]);
+      RA(N[0],c,sc[0][c]); RA(N[1],c,sc[0][c]);
     }
   }
   /* advbnd, Riemann.cpp:795-871 */
@@ -695,7 +732,7 @@ int orc_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp, const int64_t* nsup,
     cross(ba, ca, n);
     double nx = n[0] / 12.0, ny = n[1] / 12.0, nz = n[2] / 12.0;
     const uint8_t* sym = besym + e*3;
-    double f[MAXC][3];
+    double f[MAXC][3], fs[MAXC][3];
     for (int a = 0; a < 3; ++a) {
       const double* s = U + N[a]*ncomp;
       double r = s[0], ru = s[1], rv = s[2], rw = s[3], re = s[4];
@@ -707,6 +744,14 @@ int orc_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp, const int64_t* nsup,
       f[3][a] = rw*vn + pr*nz;
       f[4][a] = (re + pr)*vn;
       for (int c = 5; c < ncomp; ++c) f[c][a] = s[c]*vn;
+      double vs = (fabs(nx*ru) + fabs(ny*rv) + fabs(nz*rw))/fabs(r);
+      double ps = fabs(re) + 0.5*(ru*ru + rv*rv + rw*rw)/fabs(r);
+      fs[0][a] = fabs(r)*vs;
+      fs[1][a] = fabs(ru)*vs + ps*fabs(nx);
+      fs[2][a] = fabs(rv)*vs + ps*fabs(ny);
+      fs[3][a] = fabs(rw)*vs + ps*fabs(nz);
+      fs[4][a] = (fabs(re) + ps)*vs;
+      for (int c = 5; c < ncomp; ++c) fs[c][a] = fabs(s[c])*vs;
     }
     for (int c = 0; c < ncomp; ++c) {
       double fab = (f[c][0] + f[c][1])/4.0;
@@ -715,12 +760,14 @@ int orc_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp, const int64_t* nsup,
       RR(N[0],c) += fab + fca + f[c][0];
       RR(N[1],c) += fab + fbc + f[c][1];
       RR(N[2],c) += fbc + fca + f[c][2];
-      RA(N[0],c,fab + fca + f[c][0]); RA(N[1],c,fab + fbc + f[c][1]);
-      RA(N[2],c,fbc + fca + f[c][2]);
+      RA(N[0],c,1.5*fs[c][0] + 0.25*(fs[c][1] + fs[c][2]));
+      RA(N[1],c,1.5*fs[c][1] + 0.25*(fs[c][0] + fs[c][2]));
+      RA(N[2],c,1.5*fs[c][2] + 0.25*(fs[c][0] + fs[c][1]));
     }
   }
 #undef RR
 #undef RA
+#undef FLUX
   return 0;
 }
 

@@ -61,8 +61,9 @@ int64_t orc_symbc_setup(int64_t npoin, const double* x, const double* y,
                         int64_t* symnodes, double* symnorms, uint8_t* besym);
 
 /* riemann::grad, Riemann.cpp:229-367.  G is (npoin, 3*ncomp), zeroed here.
- * A (nullable, same shape) receives the sum of the absolute values of every
- * term added to G -- the scale of round-off at each node. */
+ * A (nullable, same shape) receives, for every term added to G, the
+ * magnitude of the products it is made of (|D|(|u_p|+|u_q|), ...) summed per
+ * entry: the scale of round-off at each node, used as the parity yardstick. */
 void orc_grad(int64_t npoin, int ncomp, const int64_t* nsup,
               const int64_t* sup0, const double* int0,
               const int64_t* sup1, const double* int1,
@@ -74,7 +75,8 @@ void orc_grad(int64_t npoin, int ncomp, const int64_t* nsup,
 /* riemann::rhs, Riemann.cpp:909-946 (advdom 652-766, advbnd 768-878; the
  * source term src() is null for the problems covered here).  G is the nodal
  * gradient already divided by the nodal volume.  R is zeroed here.  A
- * (nullable) receives the sum of absolute values of the terms added to R. */
+ * (nullable) receives, per entry of R, the summed magnitudes of the products
+ * each added flux is made of (e.g. fw(|l|+|r|), |p n|): the round-off scale. */
 int orc_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp, const int64_t* nsup,
             const int64_t* sup0, const double* int0,
             const int64_t* sup1, const double* int1,

@@ -1,8 +1,9 @@
 """Parity of the HIP RieCG path with the CPU oracle (needs an MI355X).
 
 Tolerance: FP64, per node and component |x_gpu - x_oracle| <= 1e-12 * A,
-where A is the oracle's sum of |terms| accumulated into that entry (the
-scale of round-off; the GPU sums the same terms in a different order).
+where A is the oracle's round-off scale of that entry: the summed magnitudes
+of the products every added term is made of (the GPU evaluates the same
+terms with fused multiply-adds and sums them in a different order).
 """
 import sys
 
