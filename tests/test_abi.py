@@ -1,6 +1,5 @@
 """The C-ABI library loads, exports every symbol include/xg.h declares, and
 its host-side preprocessing (no GPU needed) matches the oracle bit for bit."""
-import ctypes
 import os
 import re
 
@@ -22,7 +21,7 @@ def _declared():
 
 
 def test_library_exports_every_declared_symbol():
-    lib = ctypes.CDLL(_xg.LIBPATH)
+    lib = _xg.lib()      # loads PyTorch's HIP runtime first, then libxg.so
     names = _declared()
     assert len(names) >= 15
     for n in names:

@@ -60,6 +60,13 @@ def lib():
     """Load libxg.so (built by __graft_entry__.build / make in csrc)."""
     global _LIB
     if _LIB is None:
+        # PyTorch bundles its own HIP runtime (soname libamdhip64.so.7).  Load
+        # it first so that libxg.so binds to the same runtime: two HIP/HSA
+        # runtimes in one process do not share the GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIBPATH):
             raise ImportError("xyst_amd: %s is not built (run __graft_entry__.build())"
                               % LIBPATH)
