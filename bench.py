@@ -2,7 +2,8 @@
 
 Workload (the throughput configuration): 160^3 hex-split unit box, each hex
 split into 6 tets (24,576,000 tets, 4,173,281 nodes, 28,902,880 edges),
-nodes jittered +-10% of the spacing from a fixed seed, RCM-renumbered; FP64
+nodes jittered +-10% of the spacing from a fixed seed, RCM-renumbered and
+then grouped into compact cells of 2048 nodes (RCM order kept inside); FP64
 Sedov blast state (energy 1.0 deposited in the nodes within r < 0.02 of the
 centre, rho = 1, p = 1e-6 elsewhere), gamma 1.4, Rusanov flux, symmetry walls.
 
@@ -36,14 +37,15 @@ def rhs_bytes(E, N, Fb):
     return 32 * E + 224 * N + 15 * Fb
 
 
-def setup(n, seed=20261202):
+def setup(n, seed=20261202, order="rcm+block"):
     from xyst_amd import riemann as X
     from xyst_amd.meshgen import box_mesh, renumber_mesh, sedov_state
     t0 = time.time()
     m = box_mesh(n, seed=seed)
-    perm = X.mesh_rcm(m["coord"].shape[1], m["inpoel"])
-    m = renumber_mesh(m, perm)
-    log("mesh %d^3 + rcm: %.1fs" % (n, time.time() - t0))
+    m = renumber_mesh(m, X.mesh_rcm(m["coord"].shape[1], m["inpoel"]))
+    if order == "rcm+block":
+        m = renumber_mesh(m, X.mesh_block_order(m["coord"], 2048))
+    log("mesh %d^3 + %s: %.1fs" % (n, order, time.time() - t0))
     vol = X.mesh_vol(m["coord"], m["inpoel"])
     edge, dint = X.mesh_domint(m["coord"], m["inpoel"])
     sn, sv, besym = X.mesh_symbc(m["coord"], m["triinpoel"], m["faceset"],
@@ -85,6 +87,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=160)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--order", default="rcm+block", choices=["rcm", "rcm+block"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this path runs on one GPU; --gpus %d is not supported "
@@ -94,7 +97,7 @@ def main():
     import torch
     from xyst_amd import riemann as X
 
-    m, vol, edge, dint, sn, sv, besym, U = setup(args.n)
+    m, vol, edge, dint, sn, sv, besym, U = setup(args.n, order=args.order)
     N, E, Fb = m["coord"].shape[1], edge.shape[0], m["triinpoel"].shape[0]
     stream = torch.cuda.current_stream()
     ctx = X.Context(0, flux="rusanov", stream=stream)
@@ -154,7 +157,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "riecg_rhs_sedov_box%d" % args.n, "n": args.n,
                    "nodes": N, "edges": E, "tets": int(m["inpoel"].shape[0]),
-                   "boundary_faces": Fb, "flux": "rusanov", "ordering": "rcm",
+                   "boundary_faces": Fb, "flux": "rusanov", "ordering": args.order,
                    "state": "sedov, rho=1, p=1e-6, E=1 within r<0.02"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

@@ -74,6 +74,16 @@ int xg_mesh_domint(int64_t npoin, const double* x, const double* y,
 int xg_mesh_rcm(int64_t npoin, int64_t ntet, const size_t* inpoel,
                 size_t* perm);
 
+/* Block ordering for the device layout: recursive coordinate bisection of
+ * the nodes into spatially compact cells of at most cell_nodes nodes, the
+ * cells numbered consecutively and the nodes of a cell kept in their current
+ * relative order (so an RCM order within each cell survives).  Renumbering
+ * the mesh with perm[old] = new makes every rhs block a contiguous, compact
+ * node range -- the layout the rhs kernel streams best (any numbering stays
+ * correct). */
+int xg_mesh_block_order(int64_t npoin, const double* x, const double* y,
+                        const double* z, int64_t cell_nodes, size_t* perm);
+
 /* Symmetry BC data of RieCG::bndint/bnorm/streamable (RieCG.cpp:281-337,
  * 483-618): boundary triangles tri[3*ntri] (outward) with side set ids
  * faceset[ntri]; symmetry side sets symsets[nsym].  Produces one

@@ -1,0 +1,128 @@
+// CPU check of the rhs block layout (xyst_amd/csrc/xg_layout.cpp): emulates
+// the block and fix-up kernels with integer-valued fluxes and compares with a
+// direct scatter, bounds-checking every index the device kernels use.
+// Input file (raw little-endian): int64 npoin, ne, ntri; int32 p[ne], q[ne];
+// int32 tri[3*ntri]; uint8 besym[3*ntri].  Prints "ok <stats>" or exits 1.
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "../../xyst_amd/csrc/xg_layout.hpp"
+
+#define CHECK(c) do { if (!(c)) { std::fprintf(stderr, "FAIL %s:%d %s\n", __FILE__, __LINE__, #c); std::exit(1); } } while (0)
+
+static double fval(long i, int c) { return (double)((i * 37 + c * 11) % 2001 - 1000); }
+
+int main(int argc, char** argv) {
+  CHECK(argc == 2);
+  FILE* f = std::fopen(argv[1], "rb");
+  CHECK(f);
+  int64_t hdr[3];
+  CHECK(std::fread(hdr, 8, 3, f) == 3);
+  int npoin = (int)hdr[0];
+  int64_t ne = hdr[1], ntri = hdr[2];
+  std::vector<int> p(ne), q(ne), tri(3 * ntri);
+  std::vector<uint8_t> besym(3 * ntri);
+  CHECK(std::fread(p.data(), 4, ne, f) == (size_t)ne);
+  CHECK(std::fread(q.data(), 4, ne, f) == (size_t)ne);
+  CHECK(std::fread(tri.data(), 4, 3 * ntri, f) == (size_t)(3 * ntri));
+  CHECK(std::fread(besym.data(), 1, 3 * ntri, f) == (size_t)(3 * ntri));
+  std::fclose(f);
+  std::vector<double> d(3 * ne, 0.0);
+  // edge e's flux is identified by (p,q) so that layout reordering keeps it
+  auto eflux = [&](int a, int b, int c) { return fval((long)a * 7919 + b, c); };
+  auto fflux = [&](int a, int b, int cc, int k, int c) {
+    return fval((long)a * 131 + (long)b * 17 + cc + k * 5, c);
+  };
+  xg::Layout L;
+  std::string err;
+  CHECK(xg::build_layout(npoin, p, q, d, ntri, tri.data(), besym.data(),
+                         xg::kBlockNodes, xg::kBlockCap, L, err) == 0);
+  // direct scatter
+  std::vector<double> Rref((size_t)npoin * 5, 0.0), R((size_t)npoin * 5, -1e300);
+  for (int64_t e = 0; e < ne; ++e)
+    for (int c = 0; c < 5; ++c) {
+      Rref[p[e] * 5 + c] -= eflux(p[e], q[e], c);
+      Rref[q[e] * 5 + c] += eflux(p[e], q[e], c);
+    }
+  for (int64_t t = 0; t < ntri; ++t)
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 5; ++c)
+        Rref[tri[t * 3 + a] * 5 + c] += fflux(tri[t * 3], tri[t * 3 + 1], tri[t * 3 + 2], a, c);
+  // emulate k_rhs_blocks
+  std::vector<double> S((size_t)L.nslot * 5, -1e300);
+  CHECK((int64_t)L.ep.size() == ne && (int64_t)L.fa.size() == ntri);
+  std::vector<int> owned(npoin, 0);
+  long maxloc = 0, halo = 0;
+  for (size_t b = 0; b < L.blk.size(); ++b) {
+    const auto& B = L.blk[b];
+    CHECK(B.nloc <= xg::kBlockCap && B.nown <= B.nloc && B.nown > 0);
+    CHECK(B.own0 >= 0 && B.own0 + B.nown <= npoin);
+    CHECK(B.slot0 + (B.nloc - B.nown) <= L.nslot);
+    maxloc = std::max<long>(maxloc, B.nloc);
+    halo += B.nloc - B.nown;
+    for (int i = 0; i < B.nown; ++i) owned[B.own0 + i]++;
+    std::vector<double> Racc((size_t)B.nloc * 5, 0.0), F((size_t)xg::kChunkSlots * 5);
+    for (int ch = B.chunk0; ch < B.chunk1; ++ch) {
+      const auto& C = L.chunks[ch];
+      CHECK(C.count > 0 && C.count <= (C.kind ? xg::kChunkSlots / 3 : xg::kChunkSlots));
+      for (int t = 0; t < C.count; ++t) {
+        if (C.kind == 0) {
+          int e = C.first + t;
+          CHECK(e >= 0 && e < ne);
+          for (int c = 0; c < 5; ++c) F[t * 5 + c] = eflux(L.ep[e], L.eq[e], c);
+        } else {
+          int ff = C.first + t;
+          CHECK(ff >= 0 && ff < ntri);
+          for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 5; ++c)
+              F[(3 * t + a) * 5 + c] = fflux(L.fa[ff], L.fb[ff], L.fc[ff], a, c);
+        }
+      }
+      int nslots = C.kind ? 3 * C.count : C.count;
+      for (int k = 0; k < C.ntouch; ++k) {
+        CHECK(C.touch_off + k < (int)L.touched.size());
+        int node = L.touched[C.touch_off + k];
+        CHECK(node < B.nloc);
+        int e0 = k ? L.tend[C.touch_off + k - 1] : 0, e1 = L.tend[C.touch_off + k];
+        CHECK(e0 < e1 && C.ent_off + e1 <= (int)L.entries.size());
+        double acc[5] = {0, 0, 0, 0, 0};
+        for (int j = e0; j < e1; ++j) {
+          int v = L.entries[C.ent_off + j];
+          int s = v & 0x3ff;
+          CHECK(s < nslots);
+          for (int c = 0; c < 5; ++c) acc[c] += (v & 0x8000) ? -F[s * 5 + c] : F[s * 5 + c];
+        }
+        for (int c = 0; c < 5; ++c) Racc[node * 5 + c] += acc[c];
+      }
+    }
+    for (int i = 0; i < B.nown * 5; ++i) R[(size_t)B.own0 * 5 + i] = Racc[i];
+    for (int i = 0; i < (B.nloc - B.nown) * 5; ++i) S[(size_t)B.slot0 * 5 + i] = Racc[B.nown * 5 + i];
+  }
+  for (int n = 0; n < npoin; ++n) CHECK(owned[n] == 1);
+  // emulate k_rhs_fixup
+  CHECK(L.fx_off.size() == L.fx_node.size() + 1);
+  for (size_t i = 0; i < L.fx_node.size(); ++i) {
+    int n = L.fx_node[i];
+    CHECK(n >= 0 && n < npoin);
+    for (int k = L.fx_off[i]; k < L.fx_off[i + 1]; ++k) {
+      CHECK(L.fx_slot[k] >= 0 && L.fx_slot[k] < L.nslot);
+      for (int c = 0; c < 5; ++c) R[n * 5 + c] += S[(size_t)L.fx_slot[k] * 5 + c];
+    }
+  }
+  for (size_t i = 0; i < R.size(); ++i) CHECK(R[i] == Rref[i]);
+  // node incidence of the gradient
+  CHECK((int)L.inc_off.size() == npoin + 1 && L.inc_off[npoin] == 2 * ne);
+  for (int n = 0; n < npoin; ++n)
+    for (int k = L.inc_off[n]; k < L.inc_off[n + 1]; ++k) {
+      int e = L.inc_edge[k] & 0x7fffffff;
+      CHECK(e < ne);
+      bool first = L.inc_edge[k] < 0;
+      CHECK((first ? L.ep[e] : L.eq[e]) == n);
+      CHECK((first ? L.eq[e] : L.ep[e]) == L.inc_nb[k]);
+    }
+  std::printf("ok blocks=%zu chunks=%zu maxloc=%ld halo/owned=%.3f nfix=%zu\n",
+              L.blk.size(), L.chunks.size(), maxloc, (double)halo / npoin,
+              L.fx_node.size());
+  return 0;
+}

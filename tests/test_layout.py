@@ -1,0 +1,76 @@
+"""CPU emulation of the rhs block layout (tests/cpp/layout_check.cpp): the
+block and fix-up kernels' index structure reproduces a direct scatter exactly
+and every index stays within its buffer, for several node orderings."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from xyst_amd import riemann as X
+from xyst_amd.meshgen import box_mesh, renumber_mesh
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "build", "layout_check")
+
+
+def _exe():
+    src = [os.path.join(ROOT, "tests", "cpp", "layout_check.cpp"),
+           os.path.join(ROOT, "xyst_amd", "csrc", "xg_layout.cpp")]
+    if not os.path.exists(EXE) or any(os.path.getmtime(s) > os.path.getmtime(EXE) for s in src):
+        os.makedirs(os.path.dirname(EXE), exist_ok=True)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-o", EXE] + src, check=True)
+    return EXE
+
+
+def _check(tmp_path, m, p, q):
+    _, _, besym = X.mesh_symbc(m["coord"], m["triinpoel"], m["faceset"], [1, 2, 3, 4, 5, 6])
+    f = tmp_path / "layout.bin"
+    with open(f, "wb") as fh:
+        np.array([m["coord"].shape[1], p.size, m["triinpoel"].shape[0]], np.int64).tofile(fh)
+        p.astype(np.int32).tofile(fh)
+        q.astype(np.int32).tofile(fh)
+        m["triinpoel"].astype(np.int32).tofile(fh)
+        besym.astype(np.uint8).tofile(fh)
+    r = subprocess.run([_exe(), str(f)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return dict(kv.split("=") for kv in r.stdout.split()[1:])
+
+
+@pytest.mark.parametrize("order", ["lex", "rcm", "rcm+block"])
+def test_layout_matches_direct_scatter(tmp_path, order):
+    m = box_mesh(24, seed=2)
+    if order != "lex":
+        m = renumber_mesh(m, X.mesh_rcm(m["coord"].shape[1], m["inpoel"]))
+    if order == "rcm+block":
+        m = renumber_mesh(m, X.mesh_block_order(m["coord"], 2048))
+    e, _ = X.mesh_domint(m["coord"], m["inpoel"])
+    st = _check(tmp_path, m, e[:, 0].astype(np.int64), e[:, 1].astype(np.int64))
+    if order == "rcm+block":
+        assert float(st["halo/owned"]) < 0.5
+
+
+def test_layout_superedge_orientation(tmp_path):
+    """Edges in the mixed orientations of the reference's superedges."""
+    m = box_mesh(10, seed=4)
+    edge, dint = O.domint(m["coord"], m["inpoel"])
+    se = O.domsuped(m["inpoel"], edge, dint)
+    s0, s1, s2 = se.sup[0].reshape(-1, 4), se.sup[1].reshape(-1, 3), se.sup[2].reshape(-1, 2)
+    lp = [(0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)]
+    lt = [(0, 1), (1, 2), (2, 0)]
+    P = np.concatenate([s0[:, a] for a, b in lp] + [s1[:, a] for a, b in lt] + [s2[:, 0]])
+    Q = np.concatenate([s0[:, b] for a, b in lp] + [s1[:, b] for a, b in lt] + [s2[:, 1]])
+    _check(tmp_path, m, P, Q)
+
+
+def test_block_order_is_permutation_of_compact_cells():
+    m = box_mesh(16, seed=1)
+    perm = X.mesh_block_order(m["coord"], 512)
+    assert np.array_equal(np.sort(perm), np.arange(perm.size))
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.size)
+    # runs of 256 consecutive new ids lie within one or two compact cells
+    ext = [np.ptp(m["coord"][:, inv[i:i + 256]], axis=1).max()
+           for i in range(0, perm.size - 256, 256)]
+    assert np.median(ext) < 0.7      # cells of ~512 nodes are about 0.5 wide

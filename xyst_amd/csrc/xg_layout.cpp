@@ -1,0 +1,273 @@
+// Device layout of the edge-based operators (host side).
+//
+// rhs: nodes are grouped into blocks of consecutive ids; a block owns the
+// edges and boundary faces whose smallest node id lies in it, and all nodes
+// those touch that lie beyond the block are its halo.  One workgroup per
+// block accumulates the block's nodal sums in LDS: the block's work items
+// (edges, or boundary faces) are cut into chunks of at most kChunkSlots
+// flux slots; a chunk's fluxes go to LDS, then every node the chunk touches
+// sums its slots in a fixed order.  Owned nodes are written directly; halo
+// partial sums go to slots that a fix-up pass adds to their owners in block
+// order.  Every sum is therefore taken in an order fixed here: results are
+// bitwise reproducible, without atomics.
+//
+// grad: node-centric incidence lists (edge, side) so that each node gathers
+// its own edge terms.
+//
+// The layout is fastest when node ids are block-contiguous and spatially
+// compact (xg_mesh_block_order), but is correct for any numbering.
+#include "xg_layout.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+namespace xg {
+
+int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q,
+                 const std::vector<double>& d, int64_t ntri, const int* tri,
+                 const uint8_t* besym, int block_nodes, int block_cap,
+                 Layout& L, std::string& err) {
+  const int64_t ne = (int64_t)p.size();
+  L = Layout();
+  L.npoin = npoin;
+  // upper-neighbour lists (CSR by the smaller end) for halo tracking
+  std::vector<int> ucnt(npoin + 1, 0);
+  for (int64_t e = 0; e < ne; ++e) ++ucnt[std::min(p[e], q[e]) + 1];
+  std::vector<int64_t> uoff(npoin + 1, 0);
+  for (int i = 0; i < npoin; ++i) uoff[i + 1] = uoff[i] + ucnt[i + 1];
+  std::vector<int> up(uoff[npoin]);
+  {
+    std::vector<int64_t> pos(uoff.begin(), uoff.end() - 1);
+    for (int64_t e = 0; e < ne; ++e)
+      up[pos[std::min(p[e], q[e])]++] = std::max(p[e], q[e]);
+  }
+  // (boundary faces are tet faces: their nodes are edge neighbours of the
+  // face's smallest node, so the edge lists already cover them)
+  // blocks of consecutive node ids, grown while owned <= block_nodes and
+  // owned + halo <= block_cap
+  std::vector<int> bstart;
+  {
+    std::vector<int> mark(npoin, -1);   // block id that has node in its halo
+    int s = 0, b = 0;
+    while (s < npoin) {
+      int nown = 0, nhalo = 0, end = s;
+      while (end < npoin && nown < block_nodes) {
+        // adding node `end`: it leaves the halo if it was there
+        int add = 0;
+        for (int64_t k = uoff[end]; k < uoff[end + 1]; ++k)
+          if (mark[up[k]] != b && up[k] > end) ++add;
+        int was_halo = mark[end] == b ? 1 : 0;
+        if (nown > 0 && nown + 1 + nhalo - was_halo + add > block_cap) break;
+        for (int64_t k = uoff[end]; k < uoff[end + 1]; ++k)
+          if (mark[up[k]] != b && up[k] > end) { mark[up[k]] = b; ++nhalo; }
+        if (was_halo) { --nhalo; mark[end] = -2; }
+        ++nown; ++end;
+      }
+      if (nown + nhalo > block_cap) {
+        err = "layout: a single node has more neighbours than the block capacity";
+        return XG_EUNSUPPORTED;
+      }
+      bstart.push_back(s);
+      s = end; ++b;
+    }
+    bstart.push_back(npoin);
+  }
+  const int nblk = (int)bstart.size() - 1;
+  std::vector<int> blk_of(npoin);
+  for (int b = 0; b < nblk; ++b)
+    for (int i = bstart[b]; i < bstart[b + 1]; ++i) blk_of[i] = b;
+  // work items per block
+  std::vector<std::vector<int64_t>> bedges(nblk), bfaces(nblk);
+  for (int64_t e = 0; e < ne; ++e) bedges[blk_of[std::min(p[e], q[e])]].push_back(e);
+  for (int64_t f = 0; f < ntri; ++f) {
+    int m = std::min(tri[f * 3], std::min(tri[f * 3 + 1], tri[f * 3 + 2]));
+    bfaces[blk_of[m]].push_back(f);
+  }
+  L.blk.resize(nblk);
+  std::vector<std::vector<std::pair<int,int>>> fix(npoin);   // node -> (block, slot)
+  int nslot = 0;
+  std::vector<int> local(npoin, -1);
+  for (int b = 0; b < nblk; ++b) {
+    Block& B = L.blk[b];
+    B.own0 = bstart[b];
+    B.nown = bstart[b + 1] - bstart[b];
+    // halo: nodes touched by owned items outside the block, ascending
+    std::vector<int> halo;
+    for (int64_t e : bedges[b]) {
+      int m = std::max(p[e], q[e]);
+      if (m >= bstart[b + 1]) halo.push_back(m);
+    }
+    for (int64_t f : bfaces[b])
+      for (int a = 0; a < 3; ++a)
+        if (tri[f * 3 + a] >= bstart[b + 1]) halo.push_back(tri[f * 3 + a]);
+    std::sort(halo.begin(), halo.end());
+    halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+    B.nloc = B.nown + (int)halo.size();
+    if (B.nloc > block_cap) {
+      err = "layout: block exceeds LDS capacity";
+      return XG_EUNSUPPORTED;
+    }
+    B.halo_off = (int)L.halo.size();
+    B.slot0 = nslot;
+    for (size_t h = 0; h < halo.size(); ++h) {
+      L.halo.push_back(halo[h]);
+      fix[halo[h]].emplace_back(b, nslot + (int)h);
+    }
+    nslot += (int)halo.size();
+    for (int i = 0; i < B.nown; ++i) local[B.own0 + i] = i;
+    for (size_t h = 0; h < halo.size(); ++h) local[halo[h]] = B.nown + (int)h;
+    // edges ordered by (local smaller end, local larger end) for gather locality
+    auto& E = bedges[b];
+    std::sort(E.begin(), E.end(), [&](int64_t a, int64_t c) {
+      int a0 = local[std::min(p[a], q[a])], a1 = local[std::max(p[a], q[a])];
+      int c0 = local[std::min(p[c], q[c])], c1 = local[std::max(p[c], q[c])];
+      return a0 != c0 ? a0 < c0 : a1 < c1;
+    });
+    B.chunk0 = (int)L.chunks.size();
+    // chunk builder: slots -> touched nodes -> entries
+    auto emit = [&](int kind, int64_t first, int count,
+                    const std::vector<std::pair<int, uint16_t>>& ent) {
+      // ent: (local node, slot | sign<<15) in slot order; group by node
+      std::vector<std::pair<int, uint16_t>> s(ent);
+      std::stable_sort(s.begin(), s.end(),
+                       [](const auto& x, const auto& y) { return x.first < y.first; });
+      Chunk C;
+      C.kind = kind;
+      C.first = (int)first;
+      C.count = count;
+      C.touch_off = (int)L.touched.size();
+      C.ent_off = (int)L.entries.size();
+      int ntouch = 0;
+      for (size_t i = 0; i < s.size(); ++i) {
+        if (i == 0 || s[i].first != s[i - 1].first) {
+          if (i) L.tend.push_back((uint16_t)(i));
+          L.touched.push_back((uint16_t)s[i].first);
+          ++ntouch;
+        }
+        L.entries.push_back(s[i].second);
+      }
+      if (!s.empty()) L.tend.push_back((uint16_t)s.size());
+      C.ntouch = ntouch;
+      L.chunks.push_back(C);
+    };
+    for (size_t i0 = 0; i0 < E.size(); i0 += kChunkSlots) {
+      size_t i1 = std::min(E.size(), i0 + (size_t)kChunkSlots);
+      std::vector<std::pair<int, uint16_t>> ent;
+      for (size_t i = i0; i < i1; ++i) {
+        int64_t e = E[i];
+        uint16_t slot = (uint16_t)(i - i0);
+        ent.emplace_back(local[p[e]], (uint16_t)(slot | 0x8000));   // R(p) -= f
+        ent.emplace_back(local[q[e]], slot);                        // R(q) += f
+      }
+      emit(0, (int64_t)L.ep.size(), (int)(i1 - i0), ent);
+      for (size_t i = i0; i < i1; ++i) {
+        int64_t e = E[i];
+        L.ep.push_back(p[e]);
+        L.eq.push_back(q[e]);
+        L.dx.push_back(d[e * 3 + 0]);
+        L.dy.push_back(d[e * 3 + 1]);
+        L.dz.push_back(d[e * 3 + 2]);
+      }
+    }
+    auto& F = bfaces[b];
+    const int fmax = kChunkSlots / 3;
+    for (size_t i0 = 0; i0 < F.size(); i0 += fmax) {
+      size_t i1 = std::min(F.size(), i0 + (size_t)fmax);
+      std::vector<std::pair<int, uint16_t>> ent;
+      for (size_t i = i0; i < i1; ++i) {
+        int64_t f = F[i];
+        for (int a = 0; a < 3; ++a)
+          ent.emplace_back(local[tri[f * 3 + a]], (uint16_t)(3 * (i - i0) + a));
+      }
+      emit(1, (int64_t)L.fa.size(), (int)(i1 - i0), ent);
+      for (size_t i = i0; i < i1; ++i) {
+        int64_t f = F[i];
+        L.fa.push_back(tri[f * 3]);
+        L.fb.push_back(tri[f * 3 + 1]);
+        L.fc.push_back(tri[f * 3 + 2]);
+        L.fsym.push_back((uint8_t)((besym[f * 3] ? 1 : 0) | (besym[f * 3 + 1] ? 2 : 0) |
+                                   (besym[f * 3 + 2] ? 4 : 0)));
+      }
+    }
+    B.chunk1 = (int)L.chunks.size();
+    for (int i = 0; i < B.nown; ++i) local[B.own0 + i] = -1;
+    for (size_t h = 0; h < halo.size(); ++h) local[halo[h]] = -1;
+  }
+  L.nslot = nslot;
+  // fix-up lists: nodes receiving halo partials, slots in block order
+  L.fx_off.push_back(0);
+  for (int n = 0; n < npoin; ++n) {
+    if (fix[n].empty()) continue;
+    L.fx_node.push_back(n);
+    for (auto& bs : fix[n]) L.fx_slot.push_back(bs.second);
+    L.fx_off.push_back((int)L.fx_slot.size());
+  }
+  // node incidence for the node-centric gradient: for each node its edges in
+  // layout order, with the side bit (1 = node is the edge's first end)
+  {
+    const int64_t nl = (int64_t)L.ep.size();
+    std::vector<int> cnt(npoin + 1, 0);
+    for (int64_t e = 0; e < nl; ++e) { ++cnt[L.ep[e] + 1]; ++cnt[L.eq[e] + 1]; }
+    L.inc_off.assign(npoin + 1, 0);
+    for (int i = 0; i < npoin; ++i) L.inc_off[i + 1] = L.inc_off[i] + cnt[i + 1];
+    L.inc_edge.resize(L.inc_off[npoin]);
+    L.inc_nb.resize(L.inc_off[npoin]);
+    std::vector<int> pos(L.inc_off.begin(), L.inc_off.end() - 1);
+    for (int64_t e = 0; e < nl; ++e) {
+      int a = L.ep[e], c = L.eq[e];
+      L.inc_edge[pos[a]] = (int)(e | 0x80000000u); L.inc_nb[pos[a]++] = c;
+      L.inc_edge[pos[c]] = (int)e;                 L.inc_nb[pos[c]++] = a;
+    }
+  }
+  return XG_OK;
+}
+
+// Recursive coordinate bisection into cells of at most `leaf` nodes; the
+// order of nodes inside a cell is their original order.
+static void rcb(std::vector<int>& ids, size_t lo, size_t hi, const double* x,
+                const double* y, const double* z, int leaf) {
+  if (hi - lo <= (size_t)leaf) {
+    std::sort(ids.begin() + lo, ids.begin() + hi);
+    return;
+  }
+  double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+  const double* X[3] = {x, y, z};
+  for (size_t i = lo; i < hi; ++i)
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = std::min(mn[a], X[a][ids[i]]);
+      mx[a] = std::max(mx[a], X[a][ids[i]]);
+    }
+  int ax = 0;
+  for (int a = 1; a < 3; ++a)
+    if (mx[a] - mn[a] > mx[ax] - mn[ax]) ax = a;
+  // split so that the left part holds a whole number of leaves
+  size_t n = hi - lo;
+  size_t nleaf = (n + leaf - 1) / leaf;
+  size_t mid = lo + ((nleaf / 2) * n) / nleaf;
+  const double* C = X[ax];
+  std::nth_element(ids.begin() + lo, ids.begin() + mid, ids.begin() + hi,
+                   [&](int a, int b) { return C[a] < C[b] || (C[a] == C[b] && a < b); });
+  rcb(ids, lo, mid, x, y, z, leaf);
+  rcb(ids, mid, hi, x, y, z, leaf);
+}
+
+int block_order(int64_t npoin, const double* x, const double* y,
+                const double* z, int leaf, size_t* perm) {
+  std::vector<int> ids(npoin);
+  std::iota(ids.begin(), ids.end(), 0);
+  rcb(ids, 0, ids.size(), x, y, z, leaf);
+  for (int64_t i = 0; i < npoin; ++i) perm[ids[i]] = (size_t)i;
+  return XG_OK;
+}
+
+}  // namespace xg
+
+extern "C" int xg_mesh_block_order(int64_t npoin, const double* x,
+                                   const double* y, const double* z,
+                                   int64_t cell_nodes, size_t* perm) {
+  if (npoin <= 0 || npoin >= (1ll << 31) || !x || !y || !z || !perm ||
+      cell_nodes < 1)
+    return XG_EINVAL;
+  return xg::block_order(npoin, x, y, z, (int)cell_nodes, perm);
+}

@@ -1,12 +1,12 @@
 // MI355X (gfx950) device path of the RieCG edge-based Riemann/MUSCL scheme:
 // context, device mesh layout, riemann::grad/rhs kernels, RK stage, CFL
 // minimum.  All FP64.  Deterministic: no atomics, every nodal sum is taken in
-// an order fixed by the layout.
+// an order fixed by the layout (xg_layout.cpp).
 //
-// Scatter: edges are greedily coloured on the host so that no two edges of a
-// colour share a node; one launch per colour then updates both end nodes with
-// plain read-modify-writes.  Boundary-face terms are gathered per node
-// (node -> incident boundary faces, in face order) by a separate kernel.
+//   rhs:  k_rhs_blocks -- one 1024-thread workgroup per node block, nodal
+//         sums accumulated in LDS; k_rhs_fixup adds halo partials.
+//   grad: k_grad_nodes -- node-centric gather over the node's edges;
+//         k_grad_bnd adds boundary-face terms per boundary node.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,7 +15,10 @@
 #include <string>
 #include <vector>
 
-#include "xg_internal.hpp"
+#include "xg_flux.hpp"
+#include "xg_layout.hpp"
+
+using xg::Cfg;
 
 // ----------------------------------------------------------------- context --
 
@@ -38,12 +41,23 @@ struct xg_ctx {
   int64_t nedge = 0;
   int *ep = nullptr, *eq = nullptr;
   double *dx = nullptr, *dy = nullptr, *dz = nullptr;
-  std::vector<int64_t> coloff;         // host: edge offsets of each colour
+  // rhs blocks
+  int nblk = 0, nchunk = 0;
+  int4 *blk0 = nullptr, *blk1 = nullptr;     // (own0,nown,nloc,-), (slot0,chunk0,chunk1,-)
+  int4 *chk0 = nullptr;                      // (kind,first,count,touch_off)
+  int2 *chk1 = nullptr;                      // (ntouch, ent_off)
+  uint16_t *touched = nullptr, *tend = nullptr, *entries = nullptr;
+  int *fa = nullptr, *fb = nullptr, *fc = nullptr;
+  uint8_t* fsym = nullptr;
+  int nslot = 0;
+  double* slots = nullptr;
+  int nfx = 0;
+  int *fx_node = nullptr, *fx_off = nullptr, *fx_slot = nullptr;
+  // grad
+  int *inc_off = nullptr, *inc_edge = nullptr, *inc_nb = nullptr;
   int nbnode = 0;
   int *bnode = nullptr, *boff = nullptr, *bface = nullptr;
-  int *tri = nullptr;
-  unsigned char* besym = nullptr;
-  int64_t ntri = 0;
+  int* tri = nullptr;
   // RieCG stepping
   bool have_riecg = false;
   double* vol = nullptr;
@@ -60,12 +74,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kRedBlocks = 1024;
-
-struct Cfg {
-  double g;
-  int stab2;
-  double stab2coef;
-};
+constexpr int kRhsThreads = xg::kChunkSlots;
 
 int fail(xg_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -104,163 +113,22 @@ int upload(xg_ctx* c, std::vector<DevArr>& list, T** dst, const T* src,
   return XG_OK;
 }
 
+template <class T>
+int upload(xg_ctx* c, std::vector<DevArr>& list, T** dst,
+           const std::vector<T>& v) {
+  return upload(c, list, dst, v.data(), v.size());
+}
+
 inline unsigned nblocks(int64_t n) {
   return (unsigned)((n + kBlock - 1) / kBlock);
 }
 
-// ------------------------------------------------------------ device math --
-
-// riemann::primitive, Riemann.cpp:211-227
-__device__ inline void primitive(const double* __restrict__ U, int i,
-                                 double u[5]) {
-  const double* s = U + (size_t)i * 5;
-  u[0] = s[0];
-  u[1] = s[1] / u[0];
-  u[2] = s[2] / u[0];
-  u[3] = s[3] / u[0];
-  u[4] = s[4] / u[0] - 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]);
-}
-
-// MUSCL / van Leer reconstruction of the flow variables, Riemann.cpp:34-143
-__device__ inline void muscl(const double vw[3], const double* __restrict__ Gp,
-                             const double* __restrict__ Gq, double l[5],
-                             double r[5]) {
-  constexpr double eps = 1.0e-9;
-  constexpr double mc = 1.0 / 3.0;
-  double url[5], urr[5], d1[5], d3[5];
-#pragma unroll
-  for (int c = 0; c < 5; ++c) {
-    double g1 = Gp[c * 3 + 0] * vw[0] + Gp[c * 3 + 1] * vw[1] + Gp[c * 3 + 2] * vw[2];
-    double g2 = Gq[c * 3 + 0] * vw[0] + Gq[c * 3 + 1] * vw[1] + Gq[c * 3 + 2] * vw[2];
-    double delta2 = r[c] - l[c];
-    double delta1 = 2.0 * g1 - delta2;
-    double delta3 = 2.0 * g2 - delta2;
-    d1[c] = delta1;
-    d3[c] = delta3;
-    double rcL = (delta2 + eps) / (delta1 + eps);
-    double rcR = (delta2 + eps) / (delta3 + eps);
-    double rLinv = (delta1 + eps) / (delta2 + eps);
-    double rRinv = (delta3 + eps) / (delta2 + eps);
-    double phiL = (fabs(rcL) + rcL) / (fabs(rcL) + 1.0);
-    double phiR = (fabs(rcR) + rcR) / (fabs(rcR) + 1.0);
-    double phiLi = (fabs(rLinv) + rLinv) / (fabs(rLinv) + 1.0);
-    double phiRi = (fabs(rRinv) + rRinv) / (fabs(rRinv) + 1.0);
-    url[c] = l[c] + 0.25 * (delta1 * (1.0 - mc) * phiL + delta2 * (1.0 + mc) * phiLi);
-    urr[c] = r[c] - 0.25 * (delta3 * (1.0 - mc) * phiR + delta2 * (1.0 + mc) * phiRi);
-  }
-  // first order where density or internal energy would turn negative
-  bool keepL = (l[0] < d1[0] || l[4] < d1[4]);
-  bool keepR = (r[0] < -d3[0] || r[4] < -d3[4]);
-#pragma unroll
-  for (int c = 0; c < 5; ++c) {
-    if (!keepL) l[c] = url[c];
-    if (!keepR) r[c] = urr[c];
-  }
-}
-
-// Rusanov flux, Riemann.cpp:369-478 (domain edges: no symmetry masking)
-__device__ inline void rusanov(const Cfg& cfg, const double n[3], double l[5],
-                               double r[5], double f[5]) {
-  const double g = cfg.g;
-  double pL = (l[0] * l[4]) * (g - 1.0);
-  double pR = (r[0] * r[4]) * (g - 1.0);
-  double vnL = l[1] * n[0] + l[2] * n[1] + l[3] * n[2];
-  double vnR = r[1] * n[0] + r[2] * n[1] + r[3] * n[2];
-  l[4] = (l[4] + 0.5 * (l[1] * l[1] + l[2] * l[2] + l[3] * l[3])) * l[0];
-  l[1] *= l[0]; l[2] *= l[0]; l[3] *= l[0];
-  r[4] = (r[4] + 0.5 * (r[1] * r[1] + r[2] * r[2] + r[3] * r[3])) * r[0];
-  r[1] *= r[0]; r[2] *= r[0]; r[3] *= r[0];
-  double len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-  double sl = fabs(vnL) + sqrt(g * pL / l[0]) * len;
-  double sr = fabs(vnR) + sqrt(g * pR / r[0]) * len;
-  double fw = fmax(sl, sr);
-  f[0] = l[0] * vnL + r[0] * vnR + fw * (r[0] - l[0]);
-  f[1] = l[1] * vnL + r[1] * vnR + (pL + pR) * n[0] + fw * (r[1] - l[1]);
-  f[2] = l[2] * vnL + r[2] * vnR + (pL + pR) * n[1] + fw * (r[2] - l[2]);
-  f[3] = l[3] * vnL + r[3] * vnR + (pL + pR) * n[2] + fw * (r[3] - l[3]);
-  f[4] = (l[4] + pL) * vnL + (r[4] + pR) * vnR + fw * (r[4] - l[4]);
-  if (cfg.stab2) {
-    double fws = cfg.stab2coef * fw;
-#pragma unroll
-    for (int c = 0; c < 5; ++c) f[c] -= fws * (l[c] - r[c]);
-  }
-}
-
-// HLLC flux, Riemann.cpp:480-650 (domain edges: no symmetry masking)
-__device__ inline void hllc(const Cfg& cfg, const double d[3], double l[5],
-                            double r[5], double f[5]) {
-  const double g = cfg.g;
-  double nx = -d[0], ny = -d[1], nz = -d[2];
-  double len = sqrt(nx * nx + ny * ny + nz * nz);
-  nx /= len; ny /= len; nz /= len;
-  double qL = l[1] * nx + l[2] * ny + l[3] * nz;
-  double qR = r[1] * nx + r[2] * ny + r[3] * nz;
-  double pL = (l[0] * l[4]) * (g - 1.0);
-  double pR = (r[0] * r[4]) * (g - 1.0);
-  l[4] = (l[4] + 0.5 * (l[1] * l[1] + l[2] * l[2] + l[3] * l[3])) * l[0];
-  l[1] *= l[0]; l[2] *= l[0]; l[3] *= l[0];
-  r[4] = (r[4] + 0.5 * (r[1] * r[1] + r[2] * r[2] + r[3] * r[3])) * r[0];
-  r[1] *= r[0]; r[2] *= r[0]; r[3] *= r[0];
-  double cL = sqrt(g * pL / l[0]);
-  double cR = sqrt(g * pR / r[0]);
-  double sL = fmin(qL - cL, qR - cR);
-  double sR = fmax(qL + cL, qR + cR);
-  double tL = sL - qL;
-  double tR = sR - qR;
-  double sM = (r[0] * qR * tR - l[0] * qL * tL + pL - pR) / (r[0] * tR - l[0] * tL);
-  double pS = pL - l[0] * tL * (qL - sM);
-  double L2 = -2.0 * len;
-  double mx = nx * L2, my = ny * L2, mz = nz * L2;
-  if (sL > 0.0) {
-    double qL2 = qL * L2;
-    f[0] = l[0] * qL2;
-    f[1] = l[1] * qL2 + pL * mx;
-    f[2] = l[2] * qL2 + pL * my;
-    f[3] = l[3] * qL2 + pL * mz;
-    f[4] = (l[4] + pL) * qL2;
-  } else if (sL <= 0.0 && sM > 0.0) {
-    double s = sL - sM;
-    double u0 = tL * l[0] / s;
-    double u1 = (tL * l[1] + (pS - pL) * nx) / s;
-    double u2 = (tL * l[2] + (pS - pL) * ny) / s;
-    double u3 = (tL * l[3] + (pS - pL) * nz) / s;
-    double u4 = (tL * l[4] - pL * qL + pS * sM) / s;
-    double qL2 = qL * L2;
-    double sL2 = sL * L2;
-    f[0] = l[0] * qL2 + sL2 * (u0 - l[0]);
-    f[1] = l[1] * qL2 + pL * mx + sL2 * (u1 - l[1]);
-    f[2] = l[2] * qL2 + pL * my + sL2 * (u2 - l[2]);
-    f[3] = l[3] * qL2 + pL * mz + sL2 * (u3 - l[3]);
-    f[4] = (l[4] + pL) * qL2 + sL2 * (u4 - l[4]);
-  } else if (sM <= 0.0 && sR >= 0.0) {
-    double s = sR - sM;
-    double u0 = tR * r[0] / s;
-    double u1 = (tR * r[1] + (pS - pR) * nx) / s;
-    double u2 = (tR * r[2] + (pS - pR) * ny) / s;
-    double u3 = (tR * r[3] + (pS - pR) * nz) / s;
-    double u4 = (tR * r[4] - pR * qR + pS * sM) / s;
-    double qR2 = qR * L2;
-    double sR2 = sR * L2;
-    f[0] = r[0] * qR2 + sR2 * (u0 - r[0]);
-    f[1] = r[1] * qR2 + pR * mx + sR2 * (u1 - r[1]);
-    f[2] = r[2] * qR2 + pR * my + sR2 * (u2 - r[2]);
-    f[3] = r[3] * qR2 + pR * mz + sR2 * (u3 - r[3]);
-    f[4] = (r[4] + pR) * qR2 + sR2 * (u4 - r[4]);
-  } else {
-    double qR2 = qR * L2;
-    f[0] = r[0] * qR2;
-    f[1] = r[1] * qR2 + pR * mx;
-    f[2] = r[2] * qR2 + pR * my;
-    f[3] = r[3] * qR2 + pR * mz;
-    f[4] = (r[4] + pR) * qR2;
-  }
-  if (cfg.stab2) {
-    double sl = fabs(qL) + cL;
-    double sr = fabs(qR) + cR;
-    double fws = cfg.stab2coef * fmax(sl, sr) * len;
-#pragma unroll
-    for (int c = 0; c < 5; ++c) f[c] -= fws * (l[c] - r[c]);
-  }
+// Workgroup i runs on XCD i % 8; give each XCD a contiguous range of node
+// blocks so that neighbouring blocks share that XCD's L2.
+__device__ inline int xcd_block(int i, int n) {
+  int x = i & 7, j = i >> 3;
+  int q = n >> 3, r = n & 7;
+  return x * q + min(x, r) + j;
 }
 
 // ----------------------------------------------------------------- kernels --
@@ -270,30 +138,36 @@ __global__ void k_fill(int64_t n, double* __restrict__ a, double v) {
   if (i < n) a[i] = v;
 }
 
-// riemann::grad domain edges (Riemann.cpp:263-328), one colour: the edges of
-// a colour share no node, so both end updates are race free.
-__global__ __launch_bounds__(kBlock) void k_grad_edges(
-    int64_t beg, int64_t end, const int* __restrict__ ep,
-    const int* __restrict__ eq, const double* __restrict__ dx,
+// riemann::grad domain term (Riemann.cpp:263-328) gathered per node: each
+// node sums D_e*(u_p + u_q) over its edges, negated where it is the edge's
+// first end.
+__global__ __launch_bounds__(kBlock) void k_grad_nodes(
+    int npoin, const int* __restrict__ inc_off, const int* __restrict__ inc_edge,
+    const int* __restrict__ inc_nb, const double* __restrict__ dx,
     const double* __restrict__ dy, const double* __restrict__ dz,
     const double* __restrict__ U, double* __restrict__ G) {
-  int64_t e = beg + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= end) return;
-  int p = ep[e], q = eq[e];
-  double d[3] = {dx[e], dy[e], dz[e]};
-  double up[5], uq[5];
-  primitive(U, p, up);
-  primitive(U, q, uq);
-  double* Gp = G + (size_t)p * 15;
-  double* Gq = G + (size_t)q * 15;
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= npoin) return;
+  double un[5], acc[15];
+  xg::primitive(U, n, un);
 #pragma unroll
-  for (int c = 0; c < 5; ++c)
+  for (int k = 0; k < 15; ++k) acc[k] = 0.0;
+  for (int k = inc_off[n]; k < inc_off[n + 1]; ++k) {
+    int ev = inc_edge[k];
+    int e = ev & 0x7fffffff;
+    double sgn = ev < 0 ? -1.0 : 1.0;
+    double um[5];
+    xg::primitive(U, inc_nb[k], um);
+    double d[3] = {dx[e], dy[e], dz[e]};
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      double f = d[j] * (uq[c] + up[c]);
-      Gp[c * 3 + j] -= f;
-      Gq[c * 3 + j] += f;
+    for (int c = 0; c < 5; ++c) {
+      double s = um[c] + un[c];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) acc[c * 3 + j] += sgn * (d[j] * s);
     }
+  }
+#pragma unroll
+  for (int k = 0; k < 15; ++k) G[(size_t)n * 15 + k] = acc[k];
 }
 
 // riemann::grad boundary integral (Riemann.cpp:332-359) gathered per node.
@@ -315,9 +189,9 @@ __global__ __launch_bounds__(kBlock) void k_grad_bnd(
     int f = bface[k] >> 2;
     int A = tri[f * 3 + 0], B = tri[f * 3 + 1], C = tri[f * 3 + 2];
     double u[3][5];
-    primitive(U, A, u[0]);
-    primitive(U, B, u[1]);
-    primitive(U, C, u[2]);
+    xg::primitive(U, A, u[0]);
+    xg::primitive(U, B, u[1]);
+    xg::primitive(U, C, u[2]);
     double ba[3] = {x[B] - x[A], y[B] - y[A], z[B] - z[A]};
     double ca[3] = {x[C] - x[A], y[C] - y[A], z[C] - z[A]};
     double nr[3] = {(ba[1] * ca[2] - ca[1] * ba[2]) / 12.0,
@@ -337,85 +211,145 @@ __global__ __launch_bounds__(kBlock) void k_grad_bnd(
   for (int k = 0; k < 15; ++k) G[(size_t)n * 15 + k] = acc[k];
 }
 
-// advdom (Riemann.cpp:652-766) for one colour of edges.
-template <int FLUX>
-__global__ __launch_bounds__(kBlock) void k_rhs_edges(
-    int64_t beg, int64_t end, const int* __restrict__ ep,
-    const int* __restrict__ eq, const double* __restrict__ dx,
-    const double* __restrict__ dy, const double* __restrict__ dz,
-    const double* __restrict__ x, const double* __restrict__ y,
-    const double* __restrict__ z, const double* __restrict__ G,
-    const double* __restrict__ U, double* __restrict__ R, Cfg cfg) {
-  int64_t e = beg + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= end) return;
-  int p = ep[e], q = eq[e];
-  double d[3] = {dx[e], dy[e], dz[e]};
-  double l[5], r[5], f[5];
-  primitive(U, p, l);
-  primitive(U, q, r);
-  double vw[3] = {x[q] - x[p], y[q] - y[p], z[q] - z[p]};
-  muscl(vw, G + (size_t)p * 15, G + (size_t)q * 15, l, r);
-  if (FLUX == 0)
-    rusanov(cfg, d, l, r, f);
-  else
-    hllc(cfg, d, l, r, f);
-  double* Rp = R + (size_t)p * 5;
-  double* Rq = R + (size_t)q * 5;
+// advbnd (Riemann.cpp:768-878) for one boundary face: the three corner
+// contributions (fab + fca + f_a style weights) of the face.
+__device__ inline void face_flux(int A, int B, int C, int sym,
+                                 const double* __restrict__ x,
+                                 const double* __restrict__ y,
+                                 const double* __restrict__ z,
+                                 const double* __restrict__ U, double gamma,
+                                 double out[3][5]) {
+  int N[3] = {A, B, C};
+  double ba[3] = {x[B] - x[A], y[B] - y[A], z[B] - z[A]};
+  double ca[3] = {x[C] - x[A], y[C] - y[A], z[C] - z[A]};
+  double nx = (ba[1] * ca[2] - ca[1] * ba[2]) / 12.0;
+  double ny = (ba[2] * ca[0] - ca[2] * ba[0]) / 12.0;
+  double nz = (ba[0] * ca[1] - ca[0] * ba[1]) / 12.0;
+  double fl[5][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double* s = U + (size_t)N[a] * 5;
+    double r = s[0], ru = s[1], rv = s[2], rw = s[3], re = s[4];
+    double p = (re - 0.5 * (ru * ru + rv * rv + rw * rw) / r) * (gamma - 1.0);
+    double vn = ((sym >> a) & 1) ? 0.0 : (nx * ru + ny * rv + nz * rw) / r;
+    fl[0][a] = r * vn;
+    fl[1][a] = ru * vn + p * nx;
+    fl[2][a] = rv * vn + p * ny;
+    fl[3][a] = rw * vn + p * nz;
+    fl[4][a] = (re + p) * vn;
+  }
 #pragma unroll
   for (int c = 0; c < 5; ++c) {
-    Rp[c] -= f[c];
-    Rq[c] += f[c];
+    double fab = (fl[c][0] + fl[c][1]) / 4.0;
+    double fbc = (fl[c][1] + fl[c][2]) / 4.0;
+    double fca = (fl[c][2] + fl[c][0]) / 4.0;
+    out[0][c] = fab + fca + fl[c][0];
+    out[1][c] = fab + fbc + fl[c][1];
+    out[2][c] = fbc + fca + fl[c][2];
   }
 }
 
-// advbnd (Riemann.cpp:768-878) gathered per node; bface packs face*4 + the
-// node's position within the face.
-__global__ __launch_bounds__(kBlock) void k_rhs_bnd(
-    int nb, const int* __restrict__ bnode, const int* __restrict__ boff,
-    const int* __restrict__ bface, const int* __restrict__ tri,
-    const unsigned char* __restrict__ besym, const double* __restrict__ x,
-    const double* __restrict__ y, const double* __restrict__ z,
-    const double* __restrict__ U, double* __restrict__ R, double gamma) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb) return;
-  int n = bnode[i];
-  double acc[5];
+// riemann::rhs (advdom + advbnd, Riemann.cpp:652-878) for one node block.
+// Per chunk: each thread evaluates one edge flux (or one face's three corner
+// terms) into LDS; then each node the chunk touches sums its slots in slot
+// order into the block's LDS accumulator.
+template <int FLUX>
+__global__ __launch_bounds__(kRhsThreads) void k_rhs_blocks(
+    int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
+    const int4* __restrict__ chk0, const int2* __restrict__ chk1,
+    const uint16_t* __restrict__ touched, const uint16_t* __restrict__ tend,
+    const uint16_t* __restrict__ entries, const int* __restrict__ ep,
+    const int* __restrict__ eq, const double* __restrict__ dx,
+    const double* __restrict__ dy, const double* __restrict__ dz,
+    const int* __restrict__ fa, const int* __restrict__ fb,
+    const int* __restrict__ fc, const uint8_t* __restrict__ fsym,
+    const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const double* __restrict__ G,
+    const double* __restrict__ U, double* __restrict__ R,
+    double* __restrict__ S, Cfg cfg) {
+  __shared__ double Racc[xg::kBlockCap * 5];
+  __shared__ double F[xg::kChunkSlots * 5];
+  const int t = threadIdx.x;
+  const int b = xcd_block(blockIdx.x, nblk);
+  const int4 B0 = blk0[b];   // own0, nown, nloc
+  const int4 B1 = blk1[b];   // slot0, chunk0, chunk1
+  for (int i = t; i < B0.z * 5; i += kRhsThreads) Racc[i] = 0.0;
+  for (int ch = B1.y; ch < B1.z; ++ch) {
+    const int4 C0 = chk0[ch];   // kind, first, count, touch_off
+    const int2 C1 = chk1[ch];   // ntouch, ent_off
+    __syncthreads();            // previous chunk's F fully consumed
+    if (t < C0.z) {
+      if (C0.x == 0) {
+        int e = C0.y + t;
+        int p = ep[e], q = eq[e];
+        double d[3] = {dx[e], dy[e], dz[e]};
+        double l[5], r[5], f[5];
+        xg::primitive(U, p, l);
+        xg::primitive(U, q, r);
+        double vw[3] = {x[q] - x[p], y[q] - y[p], z[q] - z[p]};
+        xg::muscl(vw, G + (size_t)p * 15, G + (size_t)q * 15, l, r);
+        if (FLUX == 0)
+          xg::rusanov(cfg, d, l, r, f);
+        else
+          xg::hllc(cfg, d, l, r, f);
 #pragma unroll
-  for (int c = 0; c < 5; ++c) acc[c] = R[(size_t)n * 5 + c];
-  for (int k = boff[i]; k < boff[i + 1]; ++k) {
-    int f = bface[k] >> 2, pos = bface[k] & 3;
-    int N[3] = {tri[f * 3 + 0], tri[f * 3 + 1], tri[f * 3 + 2]};
-    double ba[3] = {x[N[1]] - x[N[0]], y[N[1]] - y[N[0]], z[N[1]] - z[N[0]]};
-    double ca[3] = {x[N[2]] - x[N[0]], y[N[2]] - y[N[0]], z[N[2]] - z[N[0]]};
-    double nx = (ba[1] * ca[2] - ca[1] * ba[2]) / 12.0;
-    double ny = (ba[2] * ca[0] - ca[2] * ba[0]) / 12.0;
-    double nz = (ba[0] * ca[1] - ca[0] * ba[1]) / 12.0;
-    double fl[5][3];
+        for (int c = 0; c < 5; ++c) F[t * 5 + c] = f[c];
+      } else {
+        int f = C0.y + t;
+        double o[3][5];
+        face_flux(fa[f], fb[f], fc[f], fsym[f], x, y, z, U, cfg.g, o);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const double* s = U + (size_t)N[a] * 5;
-      double r = s[0], ru = s[1], rv = s[2], rw = s[3], re = s[4];
-      double p = (re - 0.5 * (ru * ru + rv * rv + rw * rw) / r) * (gamma - 1.0);
-      double vn = besym[f * 3 + a] ? 0.0 : (nx * ru + ny * rv + nz * rw) / r;
-      fl[0][a] = r * vn;
-      fl[1][a] = ru * vn + p * nx;
-      fl[2][a] = rv * vn + p * ny;
-      fl[3][a] = rw * vn + p * nz;
-      fl[4][a] = (re + p) * vn;
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int c = 0; c < 5; ++c) F[(3 * t + a) * 5 + c] = o[a][c];
+      }
     }
+    __syncthreads();            // F complete
+    for (int k = t; k < C1.x; k += kRhsThreads) {
+      int node = touched[C0.w + k];
+      int e0 = k ? tend[C0.w + k - 1] : 0;
+      int e1 = tend[C0.w + k];
+      double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int j = e0; j < e1; ++j) {
+        int v = entries[C1.y + j];
+        const double* fs = F + (v & 0x3ff) * 5;
+        if (v & 0x8000) {
 #pragma unroll
-    for (int c = 0; c < 5; ++c) {
-      double fab = (fl[c][0] + fl[c][1]) / 4.0;
-      double fbc = (fl[c][1] + fl[c][2]) / 4.0;
-      double fca = (fl[c][2] + fl[c][0]) / 4.0;
-      double v = pos == 0 ? fab + fca + fl[c][0]
-               : pos == 1 ? fab + fbc + fl[c][1]
-                          : fbc + fca + fl[c][2];
-      acc[c] += v;
+          for (int c = 0; c < 5; ++c) acc[c] -= fs[c];
+        } else {
+#pragma unroll
+          for (int c = 0; c < 5; ++c) acc[c] += fs[c];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 5; ++c) Racc[node * 5 + c] += acc[c];
     }
   }
+  __syncthreads();
+  double* Ro = R + (size_t)B0.x * 5;
+  for (int i = t; i < B0.y * 5; i += kRhsThreads) Ro[i] = Racc[i];
+  double* So = S + (size_t)B1.x * 5;
+  for (int i = t; i < (B0.z - B0.y) * 5; i += kRhsThreads) So[i] = Racc[B0.y * 5 + i];
+}
+
+// Adds the halo partials of other blocks to their owner nodes, in block order.
+__global__ __launch_bounds__(kBlock) void k_rhs_fixup(
+    int nfx, const int* __restrict__ fx_node, const int* __restrict__ fx_off,
+    const int* __restrict__ fx_slot, const double* __restrict__ S,
+    double* __restrict__ R) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nfx) return;
+  double* r = R + (size_t)fx_node[i] * 5;
+  double acc[5];
 #pragma unroll
-  for (int c = 0; c < 5; ++c) R[(size_t)n * 5 + c] = acc[c];
+  for (int c = 0; c < 5; ++c) acc[c] = r[c];
+  for (int k = fx_off[i]; k < fx_off[i + 1]; ++k) {
+    const double* s = S + (size_t)fx_slot[k] * 5;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) acc[c] += s[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 5; ++c) r[c] = acc[c];
 }
 
 // G /= vol (RieCG.cpp:935-939)
@@ -516,14 +450,9 @@ int check_mesh(xg_ctx* c, int ncomp) {
 }
 
 int launch_grad(xg_ctx* c, const double* U, double* G) {
-  int64_t n = (int64_t)c->npoin * 15;
-  hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, G, 0.0);
-  for (size_t k = 0; k + 1 < c->coloff.size(); ++k) {
-    int64_t b = c->coloff[k], e = c->coloff[k + 1];
-    if (e == b) continue;
-    hipLaunchKernelGGL(k_grad_edges, dim3(nblocks(e - b)), dim3(kBlock), 0,
-                       c->stream, b, e, c->ep, c->eq, c->dx, c->dy, c->dz, U, G);
-  }
+  hipLaunchKernelGGL(k_grad_nodes, dim3(nblocks(c->npoin)), dim3(kBlock), 0,
+                     c->stream, c->npoin, c->inc_off, c->inc_edge, c->inc_nb,
+                     c->dx, c->dy, c->dz, U, G);
   if (c->nbnode)
     hipLaunchKernelGGL(k_grad_bnd, dim3(nblocks(c->nbnode)), dim3(kBlock), 0,
                        c->stream, c->nbnode, c->bnode, c->boff, c->bface,
@@ -536,24 +465,16 @@ int launch_rhs(xg_ctx* c, const double* G, const double* U, double* R) {
   if (c->cfg.flux != 0 && c->cfg.flux != 1)
     return fail(c, XG_EINVAL, "Flux not configured");
   Cfg cfg{c->cfg.gamma, c->cfg.stab2, c->cfg.stab2coef};
-  int64_t n = (int64_t)c->npoin * 5;
-  hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, R, 0.0);
-  for (size_t k = 0; k + 1 < c->coloff.size(); ++k) {
-    int64_t b = c->coloff[k], e = c->coloff[k + 1];
-    if (e == b) continue;
-    if (c->cfg.flux == 0)
-      hipLaunchKernelGGL(k_rhs_edges<0>, dim3(nblocks(e - b)), dim3(kBlock), 0,
-                         c->stream, b, e, c->ep, c->eq, c->dx, c->dy, c->dz,
-                         c->x, c->y, c->z, G, U, R, cfg);
-    else
-      hipLaunchKernelGGL(k_rhs_edges<1>, dim3(nblocks(e - b)), dim3(kBlock), 0,
-                         c->stream, b, e, c->ep, c->eq, c->dx, c->dy, c->dz,
-                         c->x, c->y, c->z, G, U, R, cfg);
-  }
-  if (c->nbnode)
-    hipLaunchKernelGGL(k_rhs_bnd, dim3(nblocks(c->nbnode)), dim3(kBlock), 0,
-                       c->stream, c->nbnode, c->bnode, c->boff, c->bface,
-                       c->tri, c->besym, c->x, c->y, c->z, U, R, c->cfg.gamma);
+  auto kern = c->cfg.flux == 0 ? k_rhs_blocks<0> : k_rhs_blocks<1>;
+  hipLaunchKernelGGL(kern, dim3(c->nblk), dim3(kRhsThreads), 0, c->stream,
+                     c->nblk, c->blk0, c->blk1, c->chk0, c->chk1, c->touched,
+                     c->tend, c->entries, c->ep, c->eq, c->dx, c->dy, c->dz,
+                     c->fa, c->fb, c->fc, c->fsym, c->x, c->y, c->z, G, U, R,
+                     c->slots, cfg);
+  if (c->nfx)
+    hipLaunchKernelGGL(k_rhs_fixup, dim3(nblocks(c->nfx)), dim3(kBlock), 0,
+                       c->stream, c->nfx, c->fx_node, c->fx_off, c->fx_slot,
+                       c->slots, R);
   XG_HIP(c, hipGetLastError());
   return XG_OK;
 }
@@ -642,7 +563,8 @@ int xg_mesh_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
   std::vector<double> d(ne * 3);
   int64_t k = 0;
   auto push = [&](size_t a, size_t b, const double* D) {
-    p[k] = (int)a; q[k] = (int)b;
+    p[k] = (int)std::min<size_t>(a, (size_t)INT32_MAX);
+    q[k] = (int)std::min<size_t>(b, (size_t)INT32_MAX);
     d[k * 3 + 0] = D[0]; d[k * 3 + 1] = D[1]; d[k * 3 + 2] = D[2];
     ++k;
   };
@@ -659,33 +581,26 @@ int xg_mesh_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
   for (int64_t f = 0; f < ntri * 3; ++f)
     if (tri[f] >= (size_t)npoin)
       return fail(c, XG_EINVAL, "xg_mesh_upload: boundary face node out of range");
-  // greedy colouring: smallest colour free at both ends
-  std::vector<uint64_t> used((size_t)npoin, 0);
-  std::vector<uint8_t> col(ne);
-  int ncol = 0;
-  for (int64_t e = 0; e < ne; ++e) {
-    uint64_t m = used[p[e]] | used[q[e]];
-    if (~m == 0) return fail(c, XG_EUNSUPPORTED, "edge colouring needs > 64 colours");
-    int cc = __builtin_ctzll(~m);
-    col[e] = (uint8_t)cc;
-    used[p[e]] |= 1ull << cc;
-    used[q[e]] |= 1ull << cc;
-    ncol = std::max(ncol, cc + 1);
+  std::vector<int> tri32(ntri * 3);
+  for (int64_t f = 0; f < ntri * 3; ++f) tri32[f] = (int)tri[f];
+  xg::Layout L;
+  int rc = xg::build_layout((int)npoin, p, q, d, ntri, tri32.data(), besym,
+                            xg::kBlockNodes, xg::kBlockCap, L, c->err);
+  if (rc) return rc;
+  std::vector<int4> b0(L.blk.size()), b1(L.blk.size());
+  for (size_t b = 0; b < L.blk.size(); ++b) {
+    const auto& B = L.blk[b];
+    b0[b] = make_int4(B.own0, B.nown, B.nloc, 0);
+    b1[b] = make_int4(B.slot0, B.chunk0, B.chunk1, 0);
   }
-  std::vector<int64_t> coloff(ncol + 1, 0);
-  for (int64_t e = 0; e < ne; ++e) ++coloff[col[e] + 1];
-  for (int cc = 0; cc < ncol; ++cc) coloff[cc + 1] += coloff[cc];
-  std::vector<int> sp(ne), sq(ne);
-  std::vector<double> sx(ne), sy(ne), sz(ne);
-  {
-    std::vector<int64_t> pos(coloff.begin(), coloff.end() - 1);
-    for (int64_t e = 0; e < ne; ++e) {
-      int64_t t = pos[col[e]]++;
-      sp[t] = p[e]; sq[t] = q[e];
-      sx[t] = d[e * 3]; sy[t] = d[e * 3 + 1]; sz[t] = d[e * 3 + 2];
-    }
+  std::vector<int4> c0(L.chunks.size());
+  std::vector<int2> c1(L.chunks.size());
+  for (size_t i = 0; i < L.chunks.size(); ++i) {
+    const auto& C = L.chunks[i];
+    c0[i] = make_int4(C.kind, C.first, C.count, C.touch_off);
+    c1[i] = make_int2(C.ntouch, C.ent_off);
   }
-  // boundary gather lists: node -> (face*4 + position), faces in order
+  // boundary gather lists for the gradient: node -> (face*4 + position)
   std::vector<int> bcnt((size_t)npoin + 1, 0);
   for (int64_t f = 0; f < ntri * 3; ++f) ++bcnt[tri[f] + 1];
   std::vector<int> bnode, boff(1, 0), bface;
@@ -703,31 +618,39 @@ int xg_mesh_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
       boff.push_back((int)bface.size());
     }
   }
-  std::vector<int> tri32(ntri * 3);
-  for (int64_t f = 0; f < ntri * 3; ++f) tri32[f] = (int)tri[f];
   // device arrays
   (void)hipStreamSynchronize(c->stream);
   dev_free_all(c->mesh_allocs);
   c->have_mesh = false;
-  int rc;
-  auto& L = c->mesh_allocs;
-  if ((rc = upload(c, L, &c->x, x, npoin)) || (rc = upload(c, L, &c->y, y, npoin)) ||
-      (rc = upload(c, L, &c->z, z, npoin)) || (rc = upload(c, L, &c->ep, sp.data(), ne)) ||
-      (rc = upload(c, L, &c->eq, sq.data(), ne)) ||
-      (rc = upload(c, L, &c->dx, sx.data(), ne)) ||
-      (rc = upload(c, L, &c->dy, sy.data(), ne)) ||
-      (rc = upload(c, L, &c->dz, sz.data(), ne)) ||
-      (rc = upload(c, L, &c->tri, tri32.data(), tri32.size())) ||
-      (rc = upload(c, L, &c->besym, (const unsigned char*)besym, (size_t)ntri * 3)) ||
-      (rc = upload(c, L, &c->bnode, bnode.data(), bnode.size())) ||
-      (rc = upload(c, L, &c->boff, boff.data(), boff.size())) ||
-      (rc = upload(c, L, &c->bface, bface.data(), bface.size())))
+  auto& M = c->mesh_allocs;
+  if ((rc = upload(c, M, &c->x, x, npoin)) || (rc = upload(c, M, &c->y, y, npoin)) ||
+      (rc = upload(c, M, &c->z, z, npoin)) || (rc = upload(c, M, &c->ep, L.ep)) ||
+      (rc = upload(c, M, &c->eq, L.eq)) || (rc = upload(c, M, &c->dx, L.dx)) ||
+      (rc = upload(c, M, &c->dy, L.dy)) || (rc = upload(c, M, &c->dz, L.dz)) ||
+      (rc = upload(c, M, &c->blk0, b0)) || (rc = upload(c, M, &c->blk1, b1)) ||
+      (rc = upload(c, M, &c->chk0, c0)) || (rc = upload(c, M, &c->chk1, c1)) ||
+      (rc = upload(c, M, &c->touched, L.touched)) ||
+      (rc = upload(c, M, &c->tend, L.tend)) ||
+      (rc = upload(c, M, &c->entries, L.entries)) ||
+      (rc = upload(c, M, &c->fa, L.fa)) || (rc = upload(c, M, &c->fb, L.fb)) ||
+      (rc = upload(c, M, &c->fc, L.fc)) || (rc = upload(c, M, &c->fsym, L.fsym)) ||
+      (rc = dev_alloc(c, M, (void**)&c->slots, (size_t)L.nslot * 5 * sizeof(double))) ||
+      (rc = upload(c, M, &c->fx_node, L.fx_node)) ||
+      (rc = upload(c, M, &c->fx_off, L.fx_off)) ||
+      (rc = upload(c, M, &c->fx_slot, L.fx_slot)) ||
+      (rc = upload(c, M, &c->inc_off, L.inc_off)) ||
+      (rc = upload(c, M, &c->inc_edge, L.inc_edge)) ||
+      (rc = upload(c, M, &c->inc_nb, L.inc_nb)) ||
+      (rc = upload(c, M, &c->tri, tri32)) || (rc = upload(c, M, &c->bnode, bnode)) ||
+      (rc = upload(c, M, &c->boff, boff)) || (rc = upload(c, M, &c->bface, bface)))
     return rc;
   c->npoin = (int)npoin;
   c->nedge = ne;
-  c->coloff = coloff;
+  c->nblk = (int)L.blk.size();
+  c->nchunk = (int)L.chunks.size();
+  c->nslot = L.nslot;
+  c->nfx = (int)L.fx_node.size();
   c->nbnode = (int)bnode.size();
-  c->ntri = ntri;
   c->have_mesh = true;
   c->have_riecg = false;
   return XG_OK;
@@ -826,9 +749,8 @@ int xg_riecg_setup(xg_ctx* c, const double* vol, int64_t nsym,
   int rc;
   auto& L = c->riecg_allocs;
   if ((rc = upload(c, L, &c->vol, vol, (size_t)c->npoin)) ||
-      (rc = upload(c, L, &c->symnode, node.data(), node.size())) ||
-      (rc = upload(c, L, &c->symoff, off.data(), off.size())) ||
-      (rc = upload(c, L, &c->symnorm, nrm.data(), nrm.size())) ||
+      (rc = upload(c, L, &c->symnode, node)) || (rc = upload(c, L, &c->symoff, off)) ||
+      (rc = upload(c, L, &c->symnorm, nrm)) ||
       (rc = dev_alloc(c, L, (void**)&c->red, sizeof(double) * (kRedBlocks + 1))))
     return rc;
   c->nsymnode = (int)node.size();
@@ -841,7 +763,7 @@ int xg_riecg_dt_dev(xg_ctx* c, const double* U, double cfl, double* dt) {
   if (!c->have_riecg) return fail(c, XG_ENOMESH, "xg_riecg_setup not called");
   if (!U || !dt) return fail(c, XG_EINVAL, "xg_riecg_dt: null argument");
   XG_HIP(c, hipSetDevice(c->device));
-  int nb = std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, nblocks(c->npoin)));
+  int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, nblocks(c->npoin)));
   hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(kBlock), 0, c->stream,
                      c->npoin, U, c->vol, c->cfg.gamma, c->red);
   hipLaunchKernelGGL(k_min_final, dim3(1), dim3(kRedBlocks), 0, c->stream, nb,

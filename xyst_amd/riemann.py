@@ -14,6 +14,7 @@ from . import _xg
 from ._xg import check, lib, ptr
 
 __all__ = ["Context", "mesh_vol", "mesh_domint", "mesh_rcm", "mesh_symbc",
+           "mesh_block_order",
            "edges_as_superedges"]
 
 FLUX = {"rusanov": 0, "hllc": 1}
@@ -58,6 +59,16 @@ def mesh_rcm(npoin, inpoel):
     inp = _idx(inpoel)
     perm = np.empty(npoin, np.uint64)
     check(lib().xg_mesh_rcm(npoin, inp.size // 4, ptr(inp), ptr(perm)), what="rcm")
+    return perm.astype(np.int64)
+
+
+def mesh_block_order(coord, cell_nodes=2048):
+    """Recursive-coordinate-bisection cell order (perm[old] = new); nodes
+    keep their relative order inside a cell."""
+    x, y, z = (_f64(coord[i]) for i in range(3))
+    perm = np.empty(x.size, np.uint64)
+    check(lib().xg_mesh_block_order(x.size, ptr(x), ptr(y), ptr(z), cell_nodes,
+                                    ptr(perm)), what="block order")
     return perm.astype(np.int64)
 
 
