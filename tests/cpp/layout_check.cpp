@@ -28,7 +28,8 @@ int main(int argc, char** argv) {
   CHECK(std::fread(tri.data(), 4, 3 * ntri, f) == (size_t)(3 * ntri));
   CHECK(std::fread(besym.data(), 1, 3 * ntri, f) == (size_t)(3 * ntri));
   std::fclose(f);
-  std::vector<double> d(3 * ne, 0.0);
+  std::vector<double> d(3 * ne);
+  for (int64_t i = 0; i < 3 * ne; ++i) d[i] = (double)((i * 13) % 17 - 8);
   // edge e's flux is identified by (p,q) so that layout reordering keeps it
   auto eflux = [&](int a, int b, int c) { return fval((long)a * 7919 + b, c); };
   auto fflux = [&](int a, int b, int cc, int k, int c) {
@@ -49,20 +50,27 @@ int main(int argc, char** argv) {
     for (int a = 0; a < 3; ++a)
       for (int c = 0; c < 5; ++c)
         Rref[tri[t * 3 + a] * 5 + c] += fflux(tri[t * 3], tri[t * 3 + 1], tri[t * 3 + 2], a, c);
-  // emulate k_rhs_blocks
+  // emulate k_rhs_blocks: local node data, per-thread node accumulators
   std::vector<double> S((size_t)L.nslot * 5, -1e300);
-  CHECK((int64_t)L.ep.size() == ne && (int64_t)L.fa.size() == ntri);
+  CHECK((int64_t)L.elpq.size() == ne && (int64_t)L.fab.size() == ntri);
   std::vector<int> owned(npoin, 0);
   long maxloc = 0, halo = 0;
   for (size_t b = 0; b < L.blk.size(); ++b) {
     const auto& B = L.blk[b];
     CHECK(B.nloc <= xg::kBlockCap && B.nown <= B.nloc && B.nown > 0);
+    CHECK(B.nloc <= xg::kChunkSlots);          // one local node per thread
     CHECK(B.own0 >= 0 && B.own0 + B.nown <= npoin);
     CHECK(B.slot0 + (B.nloc - B.nown) <= L.nslot);
+    CHECK(B.halo_off + (B.nloc - B.nown) <= (int)L.halo.size());
     maxloc = std::max<long>(maxloc, B.nloc);
     halo += B.nloc - B.nown;
     for (int i = 0; i < B.nown; ++i) owned[B.own0 + i]++;
-    std::vector<double> Racc((size_t)B.nloc * 5, 0.0), F((size_t)xg::kChunkSlots * 5);
+    std::vector<int> glob(B.nloc);
+    for (int i = 0; i < B.nloc; ++i) {
+      glob[i] = i < B.nown ? B.own0 + i : L.halo[B.halo_off + i - B.nown];
+      CHECK(glob[i] >= 0 && glob[i] < npoin);
+    }
+    std::vector<double> acc((size_t)B.nloc * 5, 0.0), F((size_t)xg::kChunkSlots * 5);
     for (int ch = B.chunk0; ch < B.chunk1; ++ch) {
       const auto& C = L.chunks[ch];
       CHECK(C.count > 0 && C.count <= (C.kind ? xg::kChunkSlots / 3 : xg::kChunkSlots));
@@ -70,34 +78,36 @@ int main(int argc, char** argv) {
         if (C.kind == 0) {
           int e = C.first + t;
           CHECK(e >= 0 && e < ne);
-          for (int c = 0; c < 5; ++c) F[t * 5 + c] = eflux(L.ep[e], L.eq[e], c);
+          int lp = L.elpq[e] & 0xffff, lq = L.elpq[e] >> 16;
+          CHECK(lp < B.nloc && lq < B.nloc);
+          CHECK(glob[lp] == L.ep[e] && glob[lq] == L.eq[e]);
+          for (int c = 0; c < 5; ++c) F[t * 5 + c] = eflux(glob[lp], glob[lq], c);
         } else {
           int ff = C.first + t;
           CHECK(ff >= 0 && ff < ntri);
+          int la = L.fab[ff] & 0xffff, lb = L.fab[ff] >> 16, lc = L.fcs[ff] & 0xffff;
+          CHECK(la < B.nloc && lb < B.nloc && lc < B.nloc);
+          CHECK((int)(L.fcs[ff] >> 16) < 8);
           for (int a = 0; a < 3; ++a)
             for (int c = 0; c < 5; ++c)
-              F[(3 * t + a) * 5 + c] = fflux(L.fa[ff], L.fb[ff], L.fc[ff], a, c);
+              F[(3 * t + a) * 5 + c] = fflux(glob[la], glob[lb], glob[lc], a, c);
         }
       }
       int nslots = C.kind ? 3 * C.count : C.count;
-      for (int k = 0; k < C.ntouch; ++k) {
-        CHECK(C.touch_off + k < (int)L.touched.size());
-        int node = L.touched[C.touch_off + k];
-        CHECK(node < B.nloc);
-        int e0 = k ? L.tend[C.touch_off + k - 1] : 0, e1 = L.tend[C.touch_off + k];
-        CHECK(e0 < e1 && C.ent_off + e1 <= (int)L.entries.size());
-        double acc[5] = {0, 0, 0, 0, 0};
+      CHECK(C.tend_off + B.nloc <= (int)L.tend.size());
+      for (int n = 0; n < B.nloc; ++n) {       // thread n owns local node n
+        int e0 = n ? L.tend[C.tend_off + n - 1] : 0, e1 = L.tend[C.tend_off + n];
+        CHECK(e0 <= e1 && C.ent_off + e1 <= (int)L.entries.size());
         for (int j = e0; j < e1; ++j) {
           int v = L.entries[C.ent_off + j];
           int s = v & 0x3ff;
           CHECK(s < nslots);
-          for (int c = 0; c < 5; ++c) acc[c] += (v & 0x8000) ? -F[s * 5 + c] : F[s * 5 + c];
+          for (int c = 0; c < 5; ++c) acc[n * 5 + c] += (v & 0x8000) ? -F[s * 5 + c] : F[s * 5 + c];
         }
-        for (int c = 0; c < 5; ++c) Racc[node * 5 + c] += acc[c];
       }
     }
-    for (int i = 0; i < B.nown * 5; ++i) R[(size_t)B.own0 * 5 + i] = Racc[i];
-    for (int i = 0; i < (B.nloc - B.nown) * 5; ++i) S[(size_t)B.slot0 * 5 + i] = Racc[B.nown * 5 + i];
+    for (int i = 0; i < B.nown * 5; ++i) R[(size_t)B.own0 * 5 + i] = acc[i];
+    for (int i = 0; i < (B.nloc - B.nown) * 5; ++i) S[(size_t)B.slot0 * 5 + i] = acc[B.nown * 5 + i];
   }
   for (int n = 0; n < npoin; ++n) CHECK(owned[n] == 1);
   // emulate k_rhs_fixup
@@ -111,16 +121,28 @@ int main(int argc, char** argv) {
     }
   }
   for (size_t i = 0; i < R.size(); ++i) CHECK(R[i] == Rref[i]);
-  // node incidence of the gradient
+  // node-centric gradient stream: every edge seen from both ends, D negated
+  // at the first end; integer-valued D and u make both sums exact
   CHECK((int)L.inc_off.size() == npoin + 1 && L.inc_off[npoin] == 2 * ne);
-  for (int n = 0; n < npoin; ++n)
-    for (int k = L.inc_off[n]; k < L.inc_off[n + 1]; ++k) {
-      int e = L.inc_edge[k] & 0x7fffffff;
-      CHECK(e < ne);
-      bool first = L.inc_edge[k] < 0;
-      CHECK((first ? L.ep[e] : L.eq[e]) == n);
-      CHECK((first ? L.eq[e] : L.ep[e]) == L.inc_nb[k]);
-    }
+  CHECK(L.inc_d.size() == (size_t)L.inc_off[npoin] * 3);
+  {
+    auto uval = [](int n) { return (double)((n * 29) % 101 - 50); };
+    std::vector<double> gref((size_t)npoin * 3, 0.0), gs((size_t)npoin * 3, 0.0);
+    for (int64_t e = 0; e < ne; ++e)
+      for (int j = 0; j < 3; ++j) {
+        double t = d[e * 3 + j] * (uval(q[e]) + uval(p[e]));
+        gref[p[e] * 3 + j] -= t;
+        gref[q[e] * 3 + j] += t;
+      }
+    for (int n = 0; n < npoin; ++n)
+      for (int k = L.inc_off[n]; k < L.inc_off[n + 1]; ++k) {
+        int m = L.inc_nb[k];
+        CHECK(m >= 0 && m < npoin && m != n);
+        for (int j = 0; j < 3; ++j)
+          gs[n * 3 + j] += L.inc_d[(size_t)k * 3 + j] * (uval(m) + uval(n));
+      }
+    for (size_t i = 0; i < gs.size(); ++i) CHECK(gs[i] == gref[i]);
+  }
   std::printf("ok blocks=%zu chunks=%zu maxloc=%ld halo/owned=%.3f nfix=%zu\n",
               L.blk.size(), L.chunks.size(), maxloc, (double)halo / npoin,
               L.fx_node.size());

@@ -11,8 +11,12 @@
 // order.  Every sum is therefore taken in an order fixed here: results are
 // bitwise reproducible, without atomics.
 //
-// grad: node-centric incidence lists (edge, side) so that each node gathers
-// its own edge terms.
+// The nodes a block touches are staged in LDS (primitive state, gradients,
+// coordinates), so a chunk's flux evaluations read LDS only; a thread owns one
+// local node and keeps its running sum in registers.
+//
+// grad: node-centric stream -- per node its edges as (neighbour, signed D),
+// contiguous, so that each node gathers its own edge terms.
 //
 // The layout is fastest when node ids are block-contiguous and spatially
 // compact (xg_mesh_block_order), but is correct for any numbering.
@@ -125,10 +129,10 @@ int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q
       return a0 != c0 ? a0 < c0 : a1 < c1;
     });
     B.chunk0 = (int)L.chunks.size();
-    // chunk builder: slots -> touched nodes -> entries
+    // chunk builder: entries (local node, slot | sign) grouped by node, with
+    // a dense table of per-node entry ends over all nloc local nodes
     auto emit = [&](int kind, int64_t first, int count,
                     const std::vector<std::pair<int, uint16_t>>& ent) {
-      // ent: (local node, slot | sign<<15) in slot order; group by node
       std::vector<std::pair<int, uint16_t>> s(ent);
       std::stable_sort(s.begin(), s.end(),
                        [](const auto& x, const auto& y) { return x.first < y.first; });
@@ -136,19 +140,13 @@ int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q
       C.kind = kind;
       C.first = (int)first;
       C.count = count;
-      C.touch_off = (int)L.touched.size();
+      C.tend_off = (int)L.tend.size();
       C.ent_off = (int)L.entries.size();
-      int ntouch = 0;
-      for (size_t i = 0; i < s.size(); ++i) {
-        if (i == 0 || s[i].first != s[i - 1].first) {
-          if (i) L.tend.push_back((uint16_t)(i));
-          L.touched.push_back((uint16_t)s[i].first);
-          ++ntouch;
-        }
-        L.entries.push_back(s[i].second);
+      size_t k = 0;
+      for (int n = 0; n < B.nloc; ++n) {
+        while (k < s.size() && s[k].first == n) L.entries.push_back(s[k++].second);
+        L.tend.push_back((uint16_t)(L.entries.size() - C.ent_off));
       }
-      if (!s.empty()) L.tend.push_back((uint16_t)s.size());
-      C.ntouch = ntouch;
       L.chunks.push_back(C);
     };
     for (size_t i0 = 0; i0 < E.size(); i0 += kChunkSlots) {
@@ -160,9 +158,10 @@ int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q
         ent.emplace_back(local[p[e]], (uint16_t)(slot | 0x8000));   // R(p) -= f
         ent.emplace_back(local[q[e]], slot);                        // R(q) += f
       }
-      emit(0, (int64_t)L.ep.size(), (int)(i1 - i0), ent);
+      emit(0, (int64_t)L.elpq.size(), (int)(i1 - i0), ent);
       for (size_t i = i0; i < i1; ++i) {
         int64_t e = E[i];
+        L.elpq.push_back((uint32_t)local[p[e]] | ((uint32_t)local[q[e]] << 16));
         L.ep.push_back(p[e]);
         L.eq.push_back(q[e]);
         L.dx.push_back(d[e * 3 + 0]);
@@ -180,14 +179,14 @@ int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q
         for (int a = 0; a < 3; ++a)
           ent.emplace_back(local[tri[f * 3 + a]], (uint16_t)(3 * (i - i0) + a));
       }
-      emit(1, (int64_t)L.fa.size(), (int)(i1 - i0), ent);
+      emit(1, (int64_t)L.fab.size(), (int)(i1 - i0), ent);
       for (size_t i = i0; i < i1; ++i) {
         int64_t f = F[i];
-        L.fa.push_back(tri[f * 3]);
-        L.fb.push_back(tri[f * 3 + 1]);
-        L.fc.push_back(tri[f * 3 + 2]);
-        L.fsym.push_back((uint8_t)((besym[f * 3] ? 1 : 0) | (besym[f * 3 + 1] ? 2 : 0) |
-                                   (besym[f * 3 + 2] ? 4 : 0)));
+        uint32_t sym = (besym[f * 3] ? 1u : 0u) | (besym[f * 3 + 1] ? 2u : 0u) |
+                       (besym[f * 3 + 2] ? 4u : 0u);
+        L.fab.push_back((uint32_t)local[tri[f * 3]] | ((uint32_t)local[tri[f * 3 + 1]] << 16));
+        L.fcs.push_back((uint32_t)local[tri[f * 3 + 2]] | (sym << 16));
+        for (int a = 0; a < 3; ++a) L.fglob.push_back(tri[f * 3 + a]);
       }
     }
     B.chunk1 = (int)L.chunks.size();
@@ -203,21 +202,27 @@ int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q
     for (auto& bs : fix[n]) L.fx_slot.push_back(bs.second);
     L.fx_off.push_back((int)L.fx_slot.size());
   }
-  // node incidence for the node-centric gradient: for each node its edges in
-  // layout order, with the side bit (1 = node is the edge's first end)
+  // node-centric gradient stream: for each node, its edges (in layout order)
+  // as (neighbour, D signed for this node: -D at the edge's first end)
   {
     const int64_t nl = (int64_t)L.ep.size();
     std::vector<int> cnt(npoin + 1, 0);
     for (int64_t e = 0; e < nl; ++e) { ++cnt[L.ep[e] + 1]; ++cnt[L.eq[e] + 1]; }
     L.inc_off.assign(npoin + 1, 0);
     for (int i = 0; i < npoin; ++i) L.inc_off[i + 1] = L.inc_off[i] + cnt[i + 1];
-    L.inc_edge.resize(L.inc_off[npoin]);
     L.inc_nb.resize(L.inc_off[npoin]);
+    L.inc_d.resize((size_t)L.inc_off[npoin] * 3);
     std::vector<int> pos(L.inc_off.begin(), L.inc_off.end() - 1);
     for (int64_t e = 0; e < nl; ++e) {
       int a = L.ep[e], c = L.eq[e];
-      L.inc_edge[pos[a]] = (int)(e | 0x80000000u); L.inc_nb[pos[a]++] = c;
-      L.inc_edge[pos[c]] = (int)e;                 L.inc_nb[pos[c]++] = a;
+      double D[3] = {L.dx[e], L.dy[e], L.dz[e]};
+      int ka = pos[a]++, kc = pos[c]++;
+      L.inc_nb[ka] = c;
+      L.inc_nb[kc] = a;
+      for (int j = 0; j < 3; ++j) {
+        L.inc_d[(size_t)ka * 3 + j] = -D[j];
+        L.inc_d[(size_t)kc * 3 + j] = D[j];
+      }
     }
   }
   return XG_OK;

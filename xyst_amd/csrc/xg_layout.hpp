@@ -10,10 +10,12 @@ namespace xg {
 
 // flux slots per chunk = threads per rhs workgroup
 constexpr int kChunkSlots = 1024;
-// nodes (owned + halo) a block may touch: its LDS accumulator holds
-// kBlockCap x 5 doubles
-constexpr int kBlockCap = 2560;
-constexpr int kBlockNodes = 2048;
+// nodes (owned + halo) a block may touch: their primitive state, gradients
+// and coordinates (23 doubles each) are staged in LDS, beside one chunk of
+// fluxes (kChunkSlots x 5 doubles)
+constexpr int kBlockCap = 640;
+// owned nodes per block (a block also stops growing at kBlockCap)
+constexpr int kBlockNodes = 448;
 
 struct Block {
   int own0 = 0, nown = 0, nloc = 0;  // owned ids [own0, own0+nown); local ids
@@ -26,27 +28,29 @@ struct Chunk {
   int kind = 0;       // 0 edges, 1 boundary faces
   int first = 0;      // first edge / face of the chunk in layout order
   int count = 0;      // number of edges / faces
-  int touch_off = 0;  // touched-node list offset
-  int ntouch = 0;
+  int tend_off = 0;   // dense per-local-node entry ends (nloc of them)
   int ent_off = 0;    // entry list offset
 };
 
 struct Layout {
   int npoin = 0;
-  // edges and faces in block/chunk order
-  std::vector<int> ep, eq;
+  // edges and faces in block/chunk order, end nodes as block-local ids
+  std::vector<uint32_t> elpq;      // local p | local q << 16
   std::vector<double> dx, dy, dz;
-  std::vector<int> fa, fb, fc;
-  std::vector<uint8_t> fsym;       // bit a: face node a is a symmetry node
+  std::vector<int> ep, eq;         // global end nodes (grad stream, checks)
+  std::vector<uint32_t> fab;       // local a | local b << 16
+  std::vector<uint32_t> fcs;       // local c | symmetry bits << 16
+  std::vector<int> fglob;          // global face nodes (checks)
   std::vector<Block> blk;
   std::vector<Chunk> chunks;
   std::vector<int> halo;
-  std::vector<uint16_t> touched;   // local node id
-  std::vector<uint16_t> tend;      // exclusive end of its entries (chunk-relative)
+  std::vector<uint16_t> tend;      // per chunk: nloc cumulative entry ends
   std::vector<uint16_t> entries;   // slot | 0x8000 for a negative sign
   int nslot = 0;
   std::vector<int> fx_node, fx_off, fx_slot;
-  std::vector<int> inc_off, inc_edge, inc_nb;
+  // node-centric gradient: per node its edges as (neighbour, signed D)
+  std::vector<int> inc_off, inc_nb;
+  std::vector<double> inc_d;       // 3 per entry, sign folded in
 };
 
 int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q,

@@ -3,8 +3,9 @@
 // minimum.  All FP64.  Deterministic: no atomics, every nodal sum is taken in
 // an order fixed by the layout (xg_layout.cpp).
 //
-//   rhs:  k_rhs_blocks -- one 1024-thread workgroup per node block, nodal
-//         sums accumulated in LDS; k_rhs_fixup adds halo partials.
+//   rhs:  k_rhs_blocks -- one 1024-thread workgroup per node block; the
+//         block's node data staged in LDS, one node sum per thread;
+//         k_rhs_fixup adds halo partials.
 //   grad: k_grad_nodes -- node-centric gather over the node's edges;
 //         k_grad_bnd adds boundary-face terms per boundary node.
 #include <hip/hip_runtime.h>
@@ -39,22 +40,23 @@ struct xg_ctx {
   int npoin = 0;
   double *x = nullptr, *y = nullptr, *z = nullptr;
   int64_t nedge = 0;
-  int *ep = nullptr, *eq = nullptr;
+  uint32_t* elpq = nullptr;
   double *dx = nullptr, *dy = nullptr, *dz = nullptr;
   // rhs blocks
   int nblk = 0, nchunk = 0;
-  int4 *blk0 = nullptr, *blk1 = nullptr;     // (own0,nown,nloc,-), (slot0,chunk0,chunk1,-)
-  int4 *chk0 = nullptr;                      // (kind,first,count,touch_off)
-  int2 *chk1 = nullptr;                      // (ntouch, ent_off)
-  uint16_t *touched = nullptr, *tend = nullptr, *entries = nullptr;
-  int *fa = nullptr, *fb = nullptr, *fc = nullptr;
-  uint8_t* fsym = nullptr;
+  int4 *blk0 = nullptr, *blk1 = nullptr;     // (own0,nown,nloc,halo_off), (slot0,chunk0,chunk1,-)
+  int *halo = nullptr;
+  int4 *chk = nullptr;                       // (kind,first,count,tend_off)
+  int *chk_ent = nullptr;                    // ent_off
+  uint16_t *tend = nullptr, *entries = nullptr;
+  uint32_t *fab = nullptr, *fcs = nullptr;
   int nslot = 0;
   double* slots = nullptr;
   int nfx = 0;
   int *fx_node = nullptr, *fx_off = nullptr, *fx_slot = nullptr;
   // grad
-  int *inc_off = nullptr, *inc_edge = nullptr, *inc_nb = nullptr;
+  int *inc_off = nullptr, *inc_nb = nullptr;
+  double* inc_d = nullptr;
   int nbnode = 0;
   int *bnode = nullptr, *boff = nullptr, *bface = nullptr;
   int* tri = nullptr;
@@ -139,31 +141,34 @@ __global__ void k_fill(int64_t n, double* __restrict__ a, double v) {
 }
 
 // riemann::grad domain term (Riemann.cpp:263-328) gathered per node: each
-// node sums D_e*(u_p + u_q) over its edges, negated where it is the edge's
-// first end.
+// node streams its own (neighbour, signed D) list and sums D*(u_p + u_q).
 __global__ __launch_bounds__(kBlock) void k_grad_nodes(
-    int npoin, const int* __restrict__ inc_off, const int* __restrict__ inc_edge,
-    const int* __restrict__ inc_nb, const double* __restrict__ dx,
-    const double* __restrict__ dy, const double* __restrict__ dz,
-    const double* __restrict__ U, double* __restrict__ G) {
+    int npoin, const int* __restrict__ inc_off, const int* __restrict__ inc_nb,
+    const double* __restrict__ inc_d, const double* __restrict__ U,
+    double* __restrict__ G) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= npoin) return;
-  double un[5], acc[15];
-  xg::primitive(U, n, un);
+  double s[5], un[5], acc[15];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) s[c] = U[(size_t)n * 5 + c];
+  xg::primitive_fast(s, un);
 #pragma unroll
   for (int k = 0; k < 15; ++k) acc[k] = 0.0;
-  for (int k = inc_off[n]; k < inc_off[n + 1]; ++k) {
-    int ev = inc_edge[k];
-    int e = ev & 0x7fffffff;
-    double sgn = ev < 0 ? -1.0 : 1.0;
-    double um[5];
-    xg::primitive(U, inc_nb[k], um);
-    double d[3] = {dx[e], dy[e], dz[e]};
+  const int k1 = inc_off[n + 1];
+  for (int k = inc_off[n]; k < k1; ++k) {
+    int m = inc_nb[k];
+    double sm[5], um[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) sm[c] = U[(size_t)m * 5 + c];
+    xg::primitive_fast(sm, um);
+    const double* d = inc_d + (size_t)k * 3;
+    double d0 = d[0], d1 = d[1], d2 = d[2];
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
-      double s = um[c] + un[c];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) acc[c * 3 + j] += sgn * (d[j] * s);
+      double t = um[c] + un[c];
+      acc[c * 3 + 0] += d0 * t;
+      acc[c * 3 + 1] += d1 * t;
+      acc[c * 3 + 2] += d2 * t;
     }
   }
 #pragma unroll
@@ -211,31 +216,36 @@ __global__ __launch_bounds__(kBlock) void k_grad_bnd(
   for (int k = 0; k < 15; ++k) G[(size_t)n * 15 + k] = acc[k];
 }
 
-// advbnd (Riemann.cpp:768-878) for one boundary face: the three corner
-// contributions (fab + fca + f_a style weights) of the face.
-__device__ inline void face_flux(int A, int B, int C, int sym,
-                                 const double* __restrict__ x,
-                                 const double* __restrict__ y,
-                                 const double* __restrict__ z,
-                                 const double* __restrict__ U, double gamma,
-                                 double out[3][5]) {
-  int N[3] = {A, B, C};
-  double ba[3] = {x[B] - x[A], y[B] - y[A], z[B] - z[A]};
-  double ca[3] = {x[C] - x[A], y[C] - y[A], z[C] - z[A]};
+// advbnd (Riemann.cpp:768-878) for one boundary face from the staged
+// primitive states: the three corner contributions (fab + fca + f_a style
+// weights).  The conserved values are rebuilt from the primitive ones.
+__device__ inline void face_flux_lds(int la, int lb, int lc, int sym,
+                                     const double* __restrict__ sP,
+                                     const double* __restrict__ sX,
+                                     double gamma, double out[3][5]) {
+  constexpr int CAP = xg::kBlockCap;
+  int N[3] = {la, lb, lc};
+  double ba[3], ca[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    ba[j] = sX[j * CAP + lb] - sX[j * CAP + la];
+    ca[j] = sX[j * CAP + lc] - sX[j * CAP + la];
+  }
   double nx = (ba[1] * ca[2] - ca[1] * ba[2]) / 12.0;
   double ny = (ba[2] * ca[0] - ca[2] * ba[0]) / 12.0;
   double nz = (ba[0] * ca[1] - ca[0] * ba[1]) / 12.0;
   double fl[5][3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const double* s = U + (size_t)N[a] * 5;
-    double r = s[0], ru = s[1], rv = s[2], rw = s[3], re = s[4];
-    double p = (re - 0.5 * (ru * ru + rv * rv + rw * rw) / r) * (gamma - 1.0);
-    double vn = ((sym >> a) & 1) ? 0.0 : (nx * ru + ny * rv + nz * rw) / r;
+    double r = sP[0 * CAP + N[a]], u = sP[1 * CAP + N[a]], v = sP[2 * CAP + N[a]];
+    double w = sP[3 * CAP + N[a]], e = sP[4 * CAP + N[a]];
+    double p = (r * e) * (gamma - 1.0);
+    double vn = ((sym >> a) & 1) ? 0.0 : (nx * u + ny * v + nz * w);
+    double re = r * (e + 0.5 * (u * u + v * v + w * w));
     fl[0][a] = r * vn;
-    fl[1][a] = ru * vn + p * nx;
-    fl[2][a] = rv * vn + p * ny;
-    fl[3][a] = rw * vn + p * nz;
+    fl[1][a] = r * u * vn + p * nx;
+    fl[2][a] = r * v * vn + p * ny;
+    fl[3][a] = r * w * vn + p * nz;
     fl[4][a] = (re + p) * vn;
   }
 #pragma unroll
@@ -250,86 +260,128 @@ __device__ inline void face_flux(int A, int B, int C, int sym,
 }
 
 // riemann::rhs (advdom + advbnd, Riemann.cpp:652-878) for one node block.
-// Per chunk: each thread evaluates one edge flux (or one face's three corner
-// terms) into LDS; then each node the chunk touches sums its slots in slot
-// order into the block's LDS accumulator.
+// The block's nodes (owned range + halo) are staged in LDS as primitive
+// state, gradient and coordinates.  Thread t owns local node t and keeps its
+// nodal sum in registers.  Per chunk: each thread evaluates one edge flux (or
+// one face's three corner terms) into LDS, then every node thread adds the
+// chunk's slots that touch it, in slot order.
 template <int FLUX>
 __global__ __launch_bounds__(kRhsThreads) void k_rhs_blocks(
     int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
-    const int4* __restrict__ chk0, const int2* __restrict__ chk1,
-    const uint16_t* __restrict__ touched, const uint16_t* __restrict__ tend,
-    const uint16_t* __restrict__ entries, const int* __restrict__ ep,
-    const int* __restrict__ eq, const double* __restrict__ dx,
-    const double* __restrict__ dy, const double* __restrict__ dz,
-    const int* __restrict__ fa, const int* __restrict__ fb,
-    const int* __restrict__ fc, const uint8_t* __restrict__ fsym,
-    const double* __restrict__ x, const double* __restrict__ y,
-    const double* __restrict__ z, const double* __restrict__ G,
-    const double* __restrict__ U, double* __restrict__ R,
-    double* __restrict__ S, Cfg cfg) {
-  __shared__ double Racc[xg::kBlockCap * 5];
+    const int* __restrict__ halo, const int4* __restrict__ chk,
+    const int* __restrict__ chk_ent, const uint16_t* __restrict__ tend,
+    const uint16_t* __restrict__ entries, const uint32_t* __restrict__ elpq,
+    const double* __restrict__ dx, const double* __restrict__ dy,
+    const double* __restrict__ dz, const uint32_t* __restrict__ fab,
+    const uint32_t* __restrict__ fcs, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ G, const double* __restrict__ U,
+    double* __restrict__ R, double* __restrict__ S, Cfg cfg) {
+  constexpr int CAP = xg::kBlockCap;
+  __shared__ double sP[5 * CAP];
+  __shared__ double sG[15 * CAP];
+  __shared__ double sX[3 * CAP];
   __shared__ double F[xg::kChunkSlots * 5];
   const int t = threadIdx.x;
   const int b = xcd_block(blockIdx.x, nblk);
-  const int4 B0 = blk0[b];   // own0, nown, nloc
+  const int4 B0 = blk0[b];   // own0, nown, nloc, halo_off
   const int4 B1 = blk1[b];   // slot0, chunk0, chunk1
-  for (int i = t; i < B0.z * 5; i += kRhsThreads) Racc[i] = 0.0;
+  const int own0 = B0.x, nown = B0.y, nloc = B0.z, nhalo = nloc - nown;
+  // stage the block's nodes: owned nodes are a contiguous id range
+  for (int i = t; i < nloc; i += kRhsThreads) {
+    int g = i < nown ? own0 + i : halo[B0.w + i - nown];
+    double s[5], u[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) s[c] = U[(size_t)g * 5 + c];
+    xg::primitive_fast(s, u);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) sP[c * CAP + i] = u[c];
+    sX[0 * CAP + i] = x[g];
+    sX[1 * CAP + i] = y[g];
+    sX[2 * CAP + i] = z[g];
+  }
+  {
+    const double* Go = G + (size_t)own0 * 15;
+    for (int k = t; k < nown * 15; k += kRhsThreads) {
+      int i = k / 15, c = k - 15 * i;
+      sG[c * CAP + i] = Go[k];
+    }
+    for (int k = t; k < nhalo * 15; k += kRhsThreads) {
+      int h = k / 15, c = k - 15 * h;
+      sG[c * CAP + nown + h] = G[(size_t)halo[B0.w + h] * 15 + c];
+    }
+  }
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int ch = B1.y; ch < B1.z; ++ch) {
-    const int4 C0 = chk0[ch];   // kind, first, count, touch_off
-    const int2 C1 = chk1[ch];   // ntouch, ent_off
-    __syncthreads();            // previous chunk's F fully consumed
-    if (t < C0.z) {
-      if (C0.x == 0) {
-        int e = C0.y + t;
-        int p = ep[e], q = eq[e];
+    const int4 C = chk[ch];   // kind, first, count, tend_off
+    __syncthreads();          // staging done / previous chunk's F consumed
+    if (t < C.z) {
+      if (C.x == 0) {
+        int e = C.y + t;
+        uint32_t pq = elpq[e];
+        int lp = pq & 0xffff, lq = pq >> 16;
         double d[3] = {dx[e], dy[e], dz[e]};
-        double l[5], r[5], f[5];
-        xg::primitive(U, p, l);
-        xg::primitive(U, q, r);
-        double vw[3] = {x[q] - x[p], y[q] - y[p], z[q] - z[p]};
-        xg::muscl(vw, G + (size_t)p * 15, G + (size_t)q * 15, l, r);
+        double l[5], r[5], f[5], g1[5], g2[5];
+        double vw[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) vw[j] = sX[j * CAP + lq] - sX[j * CAP + lp];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+          l[c] = sP[c * CAP + lp];
+          r[c] = sP[c * CAP + lq];
+          g1[c] = sG[(c * 3 + 0) * CAP + lp] * vw[0] + sG[(c * 3 + 1) * CAP + lp] * vw[1] +
+                  sG[(c * 3 + 2) * CAP + lp] * vw[2];
+          g2[c] = sG[(c * 3 + 0) * CAP + lq] * vw[0] + sG[(c * 3 + 1) * CAP + lq] * vw[1] +
+                  sG[(c * 3 + 2) * CAP + lq] * vw[2];
+        }
+        xg::muscl_fast(g1, g2, l, r);
         if (FLUX == 0)
-          xg::rusanov(cfg, d, l, r, f);
+          xg::rusanov_fast(cfg, d, l, r, f);
         else
-          xg::hllc(cfg, d, l, r, f);
+          xg::hllc_fast(cfg, d, l, r, f);
 #pragma unroll
         for (int c = 0; c < 5; ++c) F[t * 5 + c] = f[c];
       } else {
-        int f = C0.y + t;
+        int f = C.y + t;
+        uint32_t ab = fab[f], cs = fcs[f];
         double o[3][5];
-        face_flux(fa[f], fb[f], fc[f], fsym[f], x, y, z, U, cfg.g, o);
+        face_flux_lds(ab & 0xffff, ab >> 16, cs & 0xffff, cs >> 16, sP, sX, cfg.g, o);
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
           for (int c = 0; c < 5; ++c) F[(3 * t + a) * 5 + c] = o[a][c];
       }
     }
-    __syncthreads();            // F complete
-    for (int k = t; k < C1.x; k += kRhsThreads) {
-      int node = touched[C0.w + k];
-      int e0 = k ? tend[C0.w + k - 1] : 0;
-      int e1 = tend[C0.w + k];
-      double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    __syncthreads();          // F complete
+    if (t < nloc) {
+      const uint16_t* te = tend + C.w;
+      int e0 = t ? te[t - 1] : 0, e1 = te[t];
+      const uint16_t* en = entries + chk_ent[ch];
+      double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
       for (int j = e0; j < e1; ++j) {
-        int v = entries[C1.y + j];
+        int v = en[j];
         const double* fs = F + (v & 0x3ff) * 5;
         if (v & 0x8000) {
 #pragma unroll
-          for (int c = 0; c < 5; ++c) acc[c] -= fs[c];
+          for (int c = 0; c < 5; ++c) a[c] -= fs[c];
         } else {
 #pragma unroll
-          for (int c = 0; c < 5; ++c) acc[c] += fs[c];
+          for (int c = 0; c < 5; ++c) a[c] += fs[c];
         }
       }
 #pragma unroll
-      for (int c = 0; c < 5; ++c) Racc[node * 5 + c] += acc[c];
+      for (int c = 0; c < 5; ++c) acc[c] += a[c];
     }
   }
-  __syncthreads();
-  double* Ro = R + (size_t)B0.x * 5;
-  for (int i = t; i < B0.y * 5; i += kRhsThreads) Ro[i] = Racc[i];
-  double* So = S + (size_t)B1.x * 5;
-  for (int i = t; i < (B0.z - B0.y) * 5; i += kRhsThreads) So[i] = Racc[B0.y * 5 + i];
+  if (t < nown) {
+    double* o = R + (size_t)(own0 + t) * 5;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) o[c] = acc[c];
+  } else if (t < nloc) {
+    double* o = S + (size_t)(B1.x + t - nown) * 5;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) o[c] = acc[c];
+  }
 }
 
 // Adds the halo partials of other blocks to their owner nodes, in block order.
@@ -451,8 +503,7 @@ int check_mesh(xg_ctx* c, int ncomp) {
 
 int launch_grad(xg_ctx* c, const double* U, double* G) {
   hipLaunchKernelGGL(k_grad_nodes, dim3(nblocks(c->npoin)), dim3(kBlock), 0,
-                     c->stream, c->npoin, c->inc_off, c->inc_edge, c->inc_nb,
-                     c->dx, c->dy, c->dz, U, G);
+                     c->stream, c->npoin, c->inc_off, c->inc_nb, c->inc_d, U, G);
   if (c->nbnode)
     hipLaunchKernelGGL(k_grad_bnd, dim3(nblocks(c->nbnode)), dim3(kBlock), 0,
                        c->stream, c->nbnode, c->bnode, c->boff, c->bface,
@@ -467,10 +518,9 @@ int launch_rhs(xg_ctx* c, const double* G, const double* U, double* R) {
   Cfg cfg{c->cfg.gamma, c->cfg.stab2, c->cfg.stab2coef};
   auto kern = c->cfg.flux == 0 ? k_rhs_blocks<0> : k_rhs_blocks<1>;
   hipLaunchKernelGGL(kern, dim3(c->nblk), dim3(kRhsThreads), 0, c->stream,
-                     c->nblk, c->blk0, c->blk1, c->chk0, c->chk1, c->touched,
-                     c->tend, c->entries, c->ep, c->eq, c->dx, c->dy, c->dz,
-                     c->fa, c->fb, c->fc, c->fsym, c->x, c->y, c->z, G, U, R,
-                     c->slots, cfg);
+                     c->nblk, c->blk0, c->blk1, c->halo, c->chk, c->chk_ent,
+                     c->tend, c->entries, c->elpq, c->dx, c->dy, c->dz, c->fab,
+                     c->fcs, c->x, c->y, c->z, G, U, R, c->slots, cfg);
   if (c->nfx)
     hipLaunchKernelGGL(k_rhs_fixup, dim3(nblocks(c->nfx)), dim3(kBlock), 0,
                        c->stream, c->nfx, c->fx_node, c->fx_off, c->fx_slot,
@@ -590,15 +640,15 @@ int xg_mesh_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
   std::vector<int4> b0(L.blk.size()), b1(L.blk.size());
   for (size_t b = 0; b < L.blk.size(); ++b) {
     const auto& B = L.blk[b];
-    b0[b] = make_int4(B.own0, B.nown, B.nloc, 0);
+    b0[b] = make_int4(B.own0, B.nown, B.nloc, B.halo_off);
     b1[b] = make_int4(B.slot0, B.chunk0, B.chunk1, 0);
   }
   std::vector<int4> c0(L.chunks.size());
-  std::vector<int2> c1(L.chunks.size());
+  std::vector<int> c1(L.chunks.size());
   for (size_t i = 0; i < L.chunks.size(); ++i) {
     const auto& C = L.chunks[i];
-    c0[i] = make_int4(C.kind, C.first, C.count, C.touch_off);
-    c1[i] = make_int2(C.ntouch, C.ent_off);
+    c0[i] = make_int4(C.kind, C.first, C.count, C.tend_off);
+    c1[i] = C.ent_off;
   }
   // boundary gather lists for the gradient: node -> (face*4 + position)
   std::vector<int> bcnt((size_t)npoin + 1, 0);
@@ -624,23 +674,22 @@ int xg_mesh_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
   c->have_mesh = false;
   auto& M = c->mesh_allocs;
   if ((rc = upload(c, M, &c->x, x, npoin)) || (rc = upload(c, M, &c->y, y, npoin)) ||
-      (rc = upload(c, M, &c->z, z, npoin)) || (rc = upload(c, M, &c->ep, L.ep)) ||
-      (rc = upload(c, M, &c->eq, L.eq)) || (rc = upload(c, M, &c->dx, L.dx)) ||
+      (rc = upload(c, M, &c->z, z, npoin)) || (rc = upload(c, M, &c->elpq, L.elpq)) ||
+      (rc = upload(c, M, &c->dx, L.dx)) ||
       (rc = upload(c, M, &c->dy, L.dy)) || (rc = upload(c, M, &c->dz, L.dz)) ||
       (rc = upload(c, M, &c->blk0, b0)) || (rc = upload(c, M, &c->blk1, b1)) ||
-      (rc = upload(c, M, &c->chk0, c0)) || (rc = upload(c, M, &c->chk1, c1)) ||
-      (rc = upload(c, M, &c->touched, L.touched)) ||
+      (rc = upload(c, M, &c->halo, L.halo)) ||
+      (rc = upload(c, M, &c->chk, c0)) || (rc = upload(c, M, &c->chk_ent, c1)) ||
       (rc = upload(c, M, &c->tend, L.tend)) ||
       (rc = upload(c, M, &c->entries, L.entries)) ||
-      (rc = upload(c, M, &c->fa, L.fa)) || (rc = upload(c, M, &c->fb, L.fb)) ||
-      (rc = upload(c, M, &c->fc, L.fc)) || (rc = upload(c, M, &c->fsym, L.fsym)) ||
+      (rc = upload(c, M, &c->fab, L.fab)) || (rc = upload(c, M, &c->fcs, L.fcs)) ||
       (rc = dev_alloc(c, M, (void**)&c->slots, (size_t)L.nslot * 5 * sizeof(double))) ||
       (rc = upload(c, M, &c->fx_node, L.fx_node)) ||
       (rc = upload(c, M, &c->fx_off, L.fx_off)) ||
       (rc = upload(c, M, &c->fx_slot, L.fx_slot)) ||
       (rc = upload(c, M, &c->inc_off, L.inc_off)) ||
-      (rc = upload(c, M, &c->inc_edge, L.inc_edge)) ||
       (rc = upload(c, M, &c->inc_nb, L.inc_nb)) ||
+      (rc = upload(c, M, &c->inc_d, L.inc_d)) ||
       (rc = upload(c, M, &c->tri, tri32)) || (rc = upload(c, M, &c->bnode, bnode)) ||
       (rc = upload(c, M, &c->boff, boff)) || (rc = upload(c, M, &c->bface, bface)))
     return rc;
