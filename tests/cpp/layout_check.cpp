@@ -121,26 +121,49 @@ int main(int argc, char** argv) {
     }
   }
   for (size_t i = 0; i < R.size(); ++i) CHECK(R[i] == Rref[i]);
-  // node-centric gradient stream: every edge seen from both ends, D negated
-  // at the first end; integer-valued D and u make both sums exact
-  CHECK((int)L.inc_off.size() == npoin + 1 && L.inc_off[npoin] == 2 * ne);
-  CHECK(L.inc_d.size() == (size_t)L.inc_off[npoin] * 3);
+  // gradient on the same blocks: node thread n gathers sign*D*(u_n + u_other)
+  // over its chunk entries, the other end read from the edge's local pair;
+  // integer-valued D and u make both sums exact
   {
     auto uval = [](int n) { return (double)((n * 29) % 101 - 50); };
-    std::vector<double> gref((size_t)npoin * 3, 0.0), gs((size_t)npoin * 3, 0.0);
+    std::vector<double> gref((size_t)npoin * 3, 0.0), gs((size_t)npoin * 3, -1e300);
+    std::vector<double> gsl((size_t)L.nslot * 3, -1e300);
     for (int64_t e = 0; e < ne; ++e)
       for (int j = 0; j < 3; ++j) {
-        double t = d[e * 3 + j] * (uval(q[e]) + uval(p[e]));
-        gref[p[e] * 3 + j] -= t;
-        gref[q[e] * 3 + j] += t;
+        double tt = d[e * 3 + j] * (uval(q[e]) + uval(p[e]));
+        gref[p[e] * 3 + j] -= tt;
+        gref[q[e] * 3 + j] += tt;
       }
-    for (int n = 0; n < npoin; ++n)
-      for (int k = L.inc_off[n]; k < L.inc_off[n + 1]; ++k) {
-        int m = L.inc_nb[k];
-        CHECK(m >= 0 && m < npoin && m != n);
-        for (int j = 0; j < 3; ++j)
-          gs[n * 3 + j] += L.inc_d[(size_t)k * 3 + j] * (uval(m) + uval(n));
+    for (size_t b = 0; b < L.blk.size(); ++b) {
+      const auto& B = L.blk[b];
+      std::vector<int> glob(B.nloc);
+      for (int i = 0; i < B.nloc; ++i)
+        glob[i] = i < B.nown ? B.own0 + i : L.halo[B.halo_off + i - B.nown];
+      for (int n = 0; n < B.nloc; ++n) {
+        double acc[3] = {0, 0, 0};
+        for (int ch = B.chunk0; ch < B.chunk1; ++ch) {
+          const auto& C = L.chunks[ch];
+          if (C.kind) continue;
+          int e0 = n ? L.tend[C.tend_off + n - 1] : 0, e1 = L.tend[C.tend_off + n];
+          for (int j = e0; j < e1; ++j) {
+            int v = L.entries[C.ent_off + j];
+            int e = C.first + (v & 0x3ff);
+            bool first = v & 0x8000;
+            int lp = L.elpq[e] & 0xffff, lq = L.elpq[e] >> 16;
+            CHECK((first ? lp : lq) == n);
+            int o = first ? lq : lp;
+            double D[3] = {L.dx[e], L.dy[e], L.dz[e]};
+            for (int k = 0; k < 3; ++k)
+              acc[k] += (first ? -D[k] : D[k]) * (uval(glob[o]) + uval(glob[n]));
+          }
+        }
+        double* out = n < B.nown ? &gs[(size_t)(B.own0 + n) * 3] : &gsl[(size_t)(B.slot0 + n - B.nown) * 3];
+        for (int k = 0; k < 3; ++k) out[k] = acc[k];
       }
+    }
+    for (size_t i = 0; i < L.fx_node.size(); ++i)
+      for (int k = L.fx_off[i]; k < L.fx_off[i + 1]; ++k)
+        for (int j = 0; j < 3; ++j) gs[L.fx_node[i] * 3 + j] += gsl[(size_t)L.fx_slot[k] * 3 + j];
     for (size_t i = 0; i < gs.size(); ++i) CHECK(gs[i] == gref[i]);
   }
   std::printf("ok blocks=%zu chunks=%zu maxloc=%ld halo/owned=%.3f nfix=%zu\n",

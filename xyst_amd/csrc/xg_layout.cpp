@@ -15,8 +15,8 @@
 // coordinates), so a chunk's flux evaluations read LDS only; a thread owns one
 // local node and keeps its running sum in registers.
 //
-// grad: node-centric stream -- per node its edges as (neighbour, signed D),
-// contiguous, so that each node gathers its own edge terms.
+// grad uses the same blocks: each node thread gathers the D*(u_p+u_q) terms
+// of the chunk edges that touch it.
 //
 // The layout is fastest when node ids are block-contiguous and spatially
 // compact (xg_mesh_block_order), but is correct for any numbering.
@@ -201,29 +201,6 @@ int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q
     L.fx_node.push_back(n);
     for (auto& bs : fix[n]) L.fx_slot.push_back(bs.second);
     L.fx_off.push_back((int)L.fx_slot.size());
-  }
-  // node-centric gradient stream: for each node, its edges (in layout order)
-  // as (neighbour, D signed for this node: -D at the edge's first end)
-  {
-    const int64_t nl = (int64_t)L.ep.size();
-    std::vector<int> cnt(npoin + 1, 0);
-    for (int64_t e = 0; e < nl; ++e) { ++cnt[L.ep[e] + 1]; ++cnt[L.eq[e] + 1]; }
-    L.inc_off.assign(npoin + 1, 0);
-    for (int i = 0; i < npoin; ++i) L.inc_off[i + 1] = L.inc_off[i] + cnt[i + 1];
-    L.inc_nb.resize(L.inc_off[npoin]);
-    L.inc_d.resize((size_t)L.inc_off[npoin] * 3);
-    std::vector<int> pos(L.inc_off.begin(), L.inc_off.end() - 1);
-    for (int64_t e = 0; e < nl; ++e) {
-      int a = L.ep[e], c = L.eq[e];
-      double D[3] = {L.dx[e], L.dy[e], L.dz[e]};
-      int ka = pos[a]++, kc = pos[c]++;
-      L.inc_nb[ka] = c;
-      L.inc_nb[kc] = a;
-      for (int j = 0; j < 3; ++j) {
-        L.inc_d[(size_t)ka * 3 + j] = -D[j];
-        L.inc_d[(size_t)kc * 3 + j] = D[j];
-      }
-    }
   }
   return XG_OK;
 }

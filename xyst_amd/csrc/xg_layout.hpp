@@ -48,9 +48,6 @@ struct Layout {
   std::vector<uint16_t> entries;   // slot | 0x8000 for a negative sign
   int nslot = 0;
   std::vector<int> fx_node, fx_off, fx_slot;
-  // node-centric gradient: per node its edges as (neighbour, signed D)
-  std::vector<int> inc_off, inc_nb;
-  std::vector<double> inc_d;       // 3 per entry, sign folded in
 };
 
 int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q,

@@ -5,9 +5,10 @@
 //
 //   rhs:  k_rhs_blocks -- one 1024-thread workgroup per node block; the
 //         block's node data staged in LDS, one node sum per thread;
-//         k_rhs_fixup adds halo partials.
-//   grad: k_grad_nodes -- node-centric gather over the node's edges;
-//         k_grad_bnd adds boundary-face terms per boundary node.
+//         k_fixup<5> adds halo partials.
+//   grad: k_grad_blocks -- the same blocks, each node thread gathering its
+//         edges' terms from LDS-staged primitive states; k_fixup<15> adds
+//         halo partials; k_grad_bnd adds boundary-face terms.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -51,12 +52,11 @@ struct xg_ctx {
   uint16_t *tend = nullptr, *entries = nullptr;
   uint32_t *fab = nullptr, *fcs = nullptr;
   int nslot = 0;
-  double* slots = nullptr;
+  double* slots = nullptr;             // rhs halo partials (5 per slot)
+  double* gslots = nullptr;            // grad halo partials (15 per slot)
   int nfx = 0;
   int *fx_node = nullptr, *fx_off = nullptr, *fx_slot = nullptr;
-  // grad
-  int *inc_off = nullptr, *inc_nb = nullptr;
-  double* inc_d = nullptr;
+  // grad boundary faces
   int nbnode = 0;
   int *bnode = nullptr, *boff = nullptr, *bface = nullptr;
   int* tri = nullptr;
@@ -140,39 +140,88 @@ __global__ void k_fill(int64_t n, double* __restrict__ a, double v) {
   if (i < n) a[i] = v;
 }
 
-// riemann::grad domain term (Riemann.cpp:263-328) gathered per node: each
-// node streams its own (neighbour, signed D) list and sums D*(u_p + u_q).
-__global__ __launch_bounds__(kBlock) void k_grad_nodes(
-    int npoin, const int* __restrict__ inc_off, const int* __restrict__ inc_nb,
-    const double* __restrict__ inc_d, const double* __restrict__ U,
-    double* __restrict__ G) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= npoin) return;
-  double s[5], un[5], acc[15];
+// riemann::grad domain term (Riemann.cpp:263-328) on the rhs block layout.
+// The block's primitive states are staged in LDS; thread t owns local node t
+// and, chunk by chunk, adds D*(u_p + u_q) (negated at the edge's first end)
+// for the chunk's edges that touch it, in slot order.  No barrier is needed
+// after staging: each thread reads LDS and its own edges only.
+__global__ __launch_bounds__(kRhsThreads) void k_grad_blocks(
+    int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
+    const int* __restrict__ halo, const int4* __restrict__ chk,
+    const int* __restrict__ chk_ent, const uint16_t* __restrict__ tend,
+    const uint16_t* __restrict__ entries, const uint32_t* __restrict__ elpq,
+    const double* __restrict__ dx, const double* __restrict__ dy,
+    const double* __restrict__ dz, const double* __restrict__ U,
+    double* __restrict__ G, double* __restrict__ S) {
+  constexpr int CAP = xg::kBlockCap;
+  __shared__ double sP[5 * CAP];
+  const int t = threadIdx.x;
+  const int b = xcd_block(blockIdx.x, nblk);
+  const int4 B0 = blk0[b];   // own0, nown, nloc, halo_off
+  const int4 B1 = blk1[b];   // slot0, chunk0, chunk1
+  const int own0 = B0.x, nown = B0.y, nloc = B0.z;
+  double un[5];
+  if (t < nloc) {
+    int g = t < nown ? own0 + t : halo[B0.w + t - nown];
+    double s[5];
 #pragma unroll
-  for (int c = 0; c < 5; ++c) s[c] = U[(size_t)n * 5 + c];
-  xg::primitive_fast(s, un);
+    for (int c = 0; c < 5; ++c) s[c] = U[(size_t)g * 5 + c];
+    xg::primitive_fast(s, un);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) sP[c * CAP + t] = un[c];
+  }
+  __syncthreads();
+  if (t >= nloc) return;
+  double acc[15];
 #pragma unroll
   for (int k = 0; k < 15; ++k) acc[k] = 0.0;
-  const int k1 = inc_off[n + 1];
-  for (int k = inc_off[n]; k < k1; ++k) {
-    int m = inc_nb[k];
-    double sm[5], um[5];
+  for (int ch = B1.y; ch < B1.z; ++ch) {
+    const int4 C = chk[ch];   // kind, first, count, tend_off
+    if (C.x != 0) continue;   // boundary faces: k_grad_bnd
+    const uint16_t* te = tend + C.w;
+    int e0 = t ? te[t - 1] : 0, e1 = te[t];
+    const uint16_t* en = entries + chk_ent[ch];
+    for (int j = e0; j < e1; ++j) {
+      int v = en[j];
+      int e = C.y + (v & 0x3ff);
+      uint32_t pq = elpq[e];
+      bool first = (v & 0x8000) != 0;           // this node is the edge's p
+      int o = first ? (int)(pq >> 16) : (int)(pq & 0xffff);
+      double sg = first ? -1.0 : 1.0;
+      double d0 = sg * dx[e], d1 = sg * dy[e], d2 = sg * dz[e];
 #pragma unroll
-    for (int c = 0; c < 5; ++c) sm[c] = U[(size_t)m * 5 + c];
-    xg::primitive_fast(sm, um);
-    const double* d = inc_d + (size_t)k * 3;
-    double d0 = d[0], d1 = d[1], d2 = d[2];
-#pragma unroll
-    for (int c = 0; c < 5; ++c) {
-      double t = um[c] + un[c];
-      acc[c * 3 + 0] += d0 * t;
-      acc[c * 3 + 1] += d1 * t;
-      acc[c * 3 + 2] += d2 * t;
+      for (int c = 0; c < 5; ++c) {
+        double sum = sP[c * CAP + o] + un[c];
+        acc[c * 3 + 0] += d0 * sum;
+        acc[c * 3 + 1] += d1 * sum;
+        acc[c * 3 + 2] += d2 * sum;
+      }
     }
   }
+  double* o = t < nown ? G + (size_t)(own0 + t) * 15 : S + (size_t)(B1.x + t - nown) * 15;
 #pragma unroll
-  for (int k = 0; k < 15; ++k) G[(size_t)n * 15 + k] = acc[k];
+  for (int k = 0; k < 15; ++k) o[k] = acc[k];
+}
+
+// Adds halo partials (NC components per node) to their owners, block order.
+template <int NC>
+__global__ __launch_bounds__(kBlock) void k_fixup(
+    int nfx, const int* __restrict__ fx_node, const int* __restrict__ fx_off,
+    const int* __restrict__ fx_slot, const double* __restrict__ S,
+    double* __restrict__ R) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nfx) return;
+  double* r = R + (size_t)fx_node[i] * NC;
+  double acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = r[c];
+  for (int k = fx_off[i]; k < fx_off[i + 1]; ++k) {
+    const double* s = S + (size_t)fx_slot[k] * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] += s[c];
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) r[c] = acc[c];
 }
 
 // riemann::grad boundary integral (Riemann.cpp:332-359) gathered per node.
@@ -286,41 +335,52 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_blocks(
   const int b = xcd_block(blockIdx.x, nblk);
   const int4 B0 = blk0[b];   // own0, nown, nloc, halo_off
   const int4 B1 = blk1[b];   // slot0, chunk0, chunk1
-  const int own0 = B0.x, nown = B0.y, nloc = B0.z, nhalo = nloc - nown;
-  // stage the block's nodes: owned nodes are a contiguous id range
-  for (int i = t; i < nloc; i += kRhsThreads) {
-    int g = i < nown ? own0 + i : halo[B0.w + i - nown];
-    double s[5], u[5];
+  const int own0 = B0.x, nown = B0.y, nloc = B0.z;
+  // stage the block's nodes (one per thread, nloc <= threads): all 23 loads of
+  // a node are issued before the first is used
+  if (t < nloc) {
+    int g = t < nown ? own0 + t : halo[B0.w + t - nown];
+    double s[5], u[5], gg[15];
 #pragma unroll
     for (int c = 0; c < 5; ++c) s[c] = U[(size_t)g * 5 + c];
+#pragma unroll
+    for (int c = 0; c < 15; ++c) gg[c] = G[(size_t)g * 15 + c];
+    double xg_ = x[g], yg = y[g], zg = z[g];
     xg::primitive_fast(s, u);
 #pragma unroll
-    for (int c = 0; c < 5; ++c) sP[c * CAP + i] = u[c];
-    sX[0 * CAP + i] = x[g];
-    sX[1 * CAP + i] = y[g];
-    sX[2 * CAP + i] = z[g];
+    for (int c = 0; c < 5; ++c) sP[c * CAP + t] = u[c];
+#pragma unroll
+    for (int c = 0; c < 15; ++c) sG[c * CAP + t] = gg[c];
+    sX[0 * CAP + t] = xg_;
+    sX[1 * CAP + t] = yg;
+    sX[2 * CAP + t] = zg;
   }
-  {
-    const double* Go = G + (size_t)own0 * 15;
-    for (int k = t; k < nown * 15; k += kRhsThreads) {
-      int i = k / 15, c = k - 15 * i;
-      sG[c * CAP + i] = Go[k];
+  // the next chunk's edge record is loaded while the current one computes
+  uint32_t pq_n = 0;
+  double dx_n = 0.0, dy_n = 0.0, dz_n = 0.0;
+  auto fetch = [&](int ch) {
+    if (ch < B1.z) {
+      const int4 C = chk[ch];
+      if (C.x == 0 && t < C.z) {
+        int e = C.y + t;
+        pq_n = elpq[e];
+        dx_n = dx[e];
+        dy_n = dy[e];
+        dz_n = dz[e];
+      }
     }
-    for (int k = t; k < nhalo * 15; k += kRhsThreads) {
-      int h = k / 15, c = k - 15 * h;
-      sG[c * CAP + nown + h] = G[(size_t)halo[B0.w + h] * 15 + c];
-    }
-  }
+  };
+  fetch(B1.y);
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int ch = B1.y; ch < B1.z; ++ch) {
     const int4 C = chk[ch];   // kind, first, count, tend_off
+    const uint32_t pq = pq_n;
+    double d[3] = {dx_n, dy_n, dz_n};
+    fetch(ch + 1);
     __syncthreads();          // staging done / previous chunk's F consumed
     if (t < C.z) {
       if (C.x == 0) {
-        int e = C.y + t;
-        uint32_t pq = elpq[e];
         int lp = pq & 0xffff, lq = pq >> 16;
-        double d[3] = {dx[e], dy[e], dz[e]};
         double l[5], r[5], f[5], g1[5], g2[5];
         double vw[3];
 #pragma unroll
@@ -382,26 +442,6 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_blocks(
 #pragma unroll
     for (int c = 0; c < 5; ++c) o[c] = acc[c];
   }
-}
-
-// Adds the halo partials of other blocks to their owner nodes, in block order.
-__global__ __launch_bounds__(kBlock) void k_rhs_fixup(
-    int nfx, const int* __restrict__ fx_node, const int* __restrict__ fx_off,
-    const int* __restrict__ fx_slot, const double* __restrict__ S,
-    double* __restrict__ R) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nfx) return;
-  double* r = R + (size_t)fx_node[i] * 5;
-  double acc[5];
-#pragma unroll
-  for (int c = 0; c < 5; ++c) acc[c] = r[c];
-  for (int k = fx_off[i]; k < fx_off[i + 1]; ++k) {
-    const double* s = S + (size_t)fx_slot[k] * 5;
-#pragma unroll
-    for (int c = 0; c < 5; ++c) acc[c] += s[c];
-  }
-#pragma unroll
-  for (int c = 0; c < 5; ++c) r[c] = acc[c];
 }
 
 // G /= vol (RieCG.cpp:935-939)
@@ -502,8 +542,14 @@ int check_mesh(xg_ctx* c, int ncomp) {
 }
 
 int launch_grad(xg_ctx* c, const double* U, double* G) {
-  hipLaunchKernelGGL(k_grad_nodes, dim3(nblocks(c->npoin)), dim3(kBlock), 0,
-                     c->stream, c->npoin, c->inc_off, c->inc_nb, c->inc_d, U, G);
+  hipLaunchKernelGGL(k_grad_blocks, dim3(c->nblk), dim3(kRhsThreads), 0,
+                     c->stream, c->nblk, c->blk0, c->blk1, c->halo, c->chk,
+                     c->chk_ent, c->tend, c->entries, c->elpq, c->dx, c->dy,
+                     c->dz, U, G, c->gslots);
+  if (c->nfx)
+    hipLaunchKernelGGL(k_fixup<15>, dim3(nblocks(c->nfx)), dim3(kBlock), 0,
+                       c->stream, c->nfx, c->fx_node, c->fx_off, c->fx_slot,
+                       c->gslots, G);
   if (c->nbnode)
     hipLaunchKernelGGL(k_grad_bnd, dim3(nblocks(c->nbnode)), dim3(kBlock), 0,
                        c->stream, c->nbnode, c->bnode, c->boff, c->bface,
@@ -522,7 +568,7 @@ int launch_rhs(xg_ctx* c, const double* G, const double* U, double* R) {
                      c->tend, c->entries, c->elpq, c->dx, c->dy, c->dz, c->fab,
                      c->fcs, c->x, c->y, c->z, G, U, R, c->slots, cfg);
   if (c->nfx)
-    hipLaunchKernelGGL(k_rhs_fixup, dim3(nblocks(c->nfx)), dim3(kBlock), 0,
+    hipLaunchKernelGGL(k_fixup<5>, dim3(nblocks(c->nfx)), dim3(kBlock), 0,
                        c->stream, c->nfx, c->fx_node, c->fx_off, c->fx_slot,
                        c->slots, R);
   XG_HIP(c, hipGetLastError());
@@ -687,9 +733,7 @@ int xg_mesh_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
       (rc = upload(c, M, &c->fx_node, L.fx_node)) ||
       (rc = upload(c, M, &c->fx_off, L.fx_off)) ||
       (rc = upload(c, M, &c->fx_slot, L.fx_slot)) ||
-      (rc = upload(c, M, &c->inc_off, L.inc_off)) ||
-      (rc = upload(c, M, &c->inc_nb, L.inc_nb)) ||
-      (rc = upload(c, M, &c->inc_d, L.inc_d)) ||
+      (rc = dev_alloc(c, M, (void**)&c->gslots, (size_t)L.nslot * 15 * sizeof(double))) ||
       (rc = upload(c, M, &c->tri, tri32)) || (rc = upload(c, M, &c->bnode, bnode)) ||
       (rc = upload(c, M, &c->boff, boff)) || (rc = upload(c, M, &c->bface, bface)))
     return rc;
