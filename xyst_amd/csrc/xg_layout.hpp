@@ -12,10 +12,11 @@ namespace xg {
 constexpr int kChunkSlots = 1024;
 // nodes (owned + halo) a block may touch: their primitive state, gradients
 // and coordinates (23 doubles each) are staged in LDS, beside one chunk of
-// fluxes (kChunkSlots x 5 doubles)
-constexpr int kBlockCap = 640;
+// fluxes (kChunkSlots x 5 doubles) and its entry lists (2 x kChunkSlots +
+// kBlockCap 16-bit words): 158.3 KiB of the 160 KiB per CU
+constexpr int kBlockCap = 616;
 // owned nodes per block (a block also stops growing at kBlockCap)
-constexpr int kBlockNodes = 448;
+constexpr int kBlockNodes = 432;
 
 struct Block {
   int own0 = 0, nown = 0, nloc = 0;  // owned ids [own0, own0+nown); local ids

@@ -140,11 +140,33 @@ __global__ void k_fill(int64_t n, double* __restrict__ a, double v) {
   if (i < n) a[i] = v;
 }
 
+// Per-chunk bookkeeping shared by the block kernels: the chunk's entry list
+// (slot | sign bit, grouped by local node) and each local node's entry end
+// are prefetched into registers one chunk ahead with coalesced loads, then
+// placed in LDS, so the per-node accumulation loop reads LDS only.
+struct ChunkPrefetch {
+  uint16_t e0 = 0, e1 = 0, te = 0;
+  __device__ inline void load(const int4& C, int ent_off, int nloc, int t,
+                              const uint16_t* __restrict__ tend,
+                              const uint16_t* __restrict__ entries) {
+    int nent = C.z * (C.x == 0 ? 2 : 3);
+    if (t < nent) e0 = entries[ent_off + t];
+    if (t + kRhsThreads < nent) e1 = entries[ent_off + kRhsThreads + t];
+    if (t < nloc) te = tend[C.w + t];
+  }
+  __device__ inline void store(const int4& C, int nloc, int t, uint16_t* sE,
+                               uint16_t* sT) const {
+    int nent = C.z * (C.x == 0 ? 2 : 3);
+    if (t < nent) sE[t] = e0;
+    if (t + kRhsThreads < nent) sE[kRhsThreads + t] = e1;
+    if (t < nloc) sT[t] = te;
+  }
+};
+
 // riemann::grad domain term (Riemann.cpp:263-328) on the rhs block layout.
-// The block's primitive states are staged in LDS; thread t owns local node t
-// and, chunk by chunk, adds D*(u_p + u_q) (negated at the edge's first end)
-// for the chunk's edges that touch it, in slot order.  No barrier is needed
-// after staging: each thread reads LDS and its own edges only.
+// The block's primitive states are staged in LDS.  Per chunk: each thread
+// writes one edge's (u_p + u_q, D) to LDS; then the thread owning local node
+// t adds -/+ D (u_p + u_q) for the chunk's edges touching it, in slot order.
 __global__ __launch_bounds__(kRhsThreads) void k_grad_blocks(
     int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
     const int* __restrict__ halo, const int4* __restrict__ chk,
@@ -155,52 +177,88 @@ __global__ __launch_bounds__(kRhsThreads) void k_grad_blocks(
     double* __restrict__ G, double* __restrict__ S) {
   constexpr int CAP = xg::kBlockCap;
   __shared__ double sP[5 * CAP];
+  __shared__ double F[xg::kChunkSlots * 8];
+  __shared__ uint16_t sE[2 * kRhsThreads];
+  __shared__ uint16_t sT[CAP];
   const int t = threadIdx.x;
   const int b = xcd_block(blockIdx.x, nblk);
   const int4 B0 = blk0[b];   // own0, nown, nloc, halo_off
   const int4 B1 = blk1[b];   // slot0, chunk0, chunk1
   const int own0 = B0.x, nown = B0.y, nloc = B0.z;
-  double un[5];
   if (t < nloc) {
     int g = t < nown ? own0 + t : halo[B0.w + t - nown];
-    double s[5];
+    double s[5], u[5];
 #pragma unroll
     for (int c = 0; c < 5; ++c) s[c] = U[(size_t)g * 5 + c];
-    xg::primitive_fast(s, un);
+    xg::primitive_fast(s, u);
 #pragma unroll
-    for (int c = 0; c < 5; ++c) sP[c * CAP + t] = un[c];
+    for (int c = 0; c < 5; ++c) sP[c * CAP + t] = u[c];
   }
-  __syncthreads();
-  if (t >= nloc) return;
+  // prefetch of the next edge chunk (boundary-face chunks are skipped:
+  // k_grad_bnd adds those terms)
+  int ch = B1.y;
+  while (ch < B1.z && chk[ch].x != 0) ++ch;
+  ChunkPrefetch pf;
+  uint32_t pq_n = 0;
+  double dx_n = 0.0, dy_n = 0.0, dz_n = 0.0;
+  auto fetch = [&](int c) {
+    if (c < B1.z) {
+      const int4 C = chk[c];
+      pf.load(C, chk_ent[c], nloc, t, tend, entries);
+      if (t < C.z) {
+        int e = C.y + t;
+        pq_n = elpq[e];
+        dx_n = dx[e];
+        dy_n = dy[e];
+        dz_n = dz[e];
+      }
+    }
+  };
+  fetch(ch);
   double acc[15];
 #pragma unroll
   for (int k = 0; k < 15; ++k) acc[k] = 0.0;
-  for (int ch = B1.y; ch < B1.z; ++ch) {
-    const int4 C = chk[ch];   // kind, first, count, tend_off
-    if (C.x != 0) continue;   // boundary faces: k_grad_bnd
-    const uint16_t* te = tend + C.w;
-    int e0 = t ? te[t - 1] : 0, e1 = te[t];
-    const uint16_t* en = entries + chk_ent[ch];
-    for (int j = e0; j < e1; ++j) {
-      int v = en[j];
-      int e = C.y + (v & 0x3ff);
-      uint32_t pq = elpq[e];
-      bool first = (v & 0x8000) != 0;           // this node is the edge's p
-      int o = first ? (int)(pq >> 16) : (int)(pq & 0xffff);
-      double sg = first ? -1.0 : 1.0;
-      double d0 = sg * dx[e], d1 = sg * dy[e], d2 = sg * dz[e];
+  while (ch < B1.z) {
+    const int4 C = chk[ch];
+    int nx = ch + 1;
+    while (nx < B1.z && chk[nx].x != 0) ++nx;
+    const uint32_t pq = pq_n;
+    const double d0 = dx_n, d1 = dy_n, d2 = dz_n;
+    __syncthreads();          // staging done / previous chunk consumed
+    pf.store(C, nloc, t, sE, sT);
+    if (t < C.z) {
+      int lp = pq & 0xffff, lq = pq >> 16;
+      double* f = F + t * 8;
 #pragma unroll
-      for (int c = 0; c < 5; ++c) {
-        double sum = sP[c * CAP + o] + un[c];
-        acc[c * 3 + 0] += d0 * sum;
-        acc[c * 3 + 1] += d1 * sum;
-        acc[c * 3 + 2] += d2 * sum;
+      for (int c = 0; c < 5; ++c) f[c] = sP[c * CAP + lq] + sP[c * CAP + lp];
+      f[5] = d0;
+      f[6] = d1;
+      f[7] = d2;
+    }
+    fetch(nx);                // in flight during the accumulation below
+    __syncthreads();          // F, sE, sT complete
+    if (t < nloc) {
+      int e0 = t ? sT[t - 1] : 0, e1 = sT[t];
+      for (int j = e0; j < e1; ++j) {
+        int v = sE[j];
+        const double* f = F + (v & 0x3ff) * 8;
+        double sg = (v & 0x8000) ? -1.0 : 1.0;
+        double D0 = sg * f[5], D1 = sg * f[6], D2 = sg * f[7];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+          acc[c * 3 + 0] += D0 * f[c];
+          acc[c * 3 + 1] += D1 * f[c];
+          acc[c * 3 + 2] += D2 * f[c];
+        }
       }
     }
+    ch = nx;
   }
-  double* o = t < nown ? G + (size_t)(own0 + t) * 15 : S + (size_t)(B1.x + t - nown) * 15;
+  if (t < nloc) {
+    double* o = t < nown ? G + (size_t)(own0 + t) * 15 : S + (size_t)(B1.x + t - nown) * 15;
 #pragma unroll
-  for (int k = 0; k < 15; ++k) o[k] = acc[k];
+    for (int k = 0; k < 15; ++k) o[k] = acc[k];
+  }
 }
 
 // Adds halo partials (NC components per node) to their owners, block order.
@@ -331,6 +389,8 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_blocks(
   __shared__ double sG[15 * CAP];
   __shared__ double sX[3 * CAP];
   __shared__ double F[xg::kChunkSlots * 5];
+  __shared__ uint16_t sE[2 * kRhsThreads];
+  __shared__ uint16_t sT[CAP];
   const int t = threadIdx.x;
   const int b = xcd_block(blockIdx.x, nblk);
   const int4 B0 = blk0[b];   // own0, nown, nloc, halo_off
@@ -355,12 +415,15 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_blocks(
     sX[1 * CAP + t] = yg;
     sX[2 * CAP + t] = zg;
   }
-  // the next chunk's edge record is loaded while the current one computes
+  // the next chunk's edge record and entry lists are loaded while the
+  // current chunk computes
+  ChunkPrefetch pf;
   uint32_t pq_n = 0;
   double dx_n = 0.0, dy_n = 0.0, dz_n = 0.0;
-  auto fetch = [&](int ch) {
-    if (ch < B1.z) {
-      const int4 C = chk[ch];
+  auto fetch = [&](int c) {
+    if (c < B1.z) {
+      const int4 C = chk[c];
+      pf.load(C, chk_ent[c], nloc, t, tend, entries);
       if (C.x == 0 && t < C.z) {
         int e = C.y + t;
         pq_n = elpq[e];
@@ -376,8 +439,8 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_blocks(
     const int4 C = chk[ch];   // kind, first, count, tend_off
     const uint32_t pq = pq_n;
     double d[3] = {dx_n, dy_n, dz_n};
-    fetch(ch + 1);
     __syncthreads();          // staging done / previous chunk's F consumed
+    pf.store(C, nloc, t, sE, sT);
     if (t < C.z) {
       if (C.x == 0) {
         int lp = pq & 0xffff, lq = pq >> 16;
@@ -412,25 +475,17 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_blocks(
           for (int c = 0; c < 5; ++c) F[(3 * t + a) * 5 + c] = o[a][c];
       }
     }
-    __syncthreads();          // F complete
+    fetch(ch + 1);            // in flight during the accumulation below
+    __syncthreads();          // F, sE, sT complete
     if (t < nloc) {
-      const uint16_t* te = tend + C.w;
-      int e0 = t ? te[t - 1] : 0, e1 = te[t];
-      const uint16_t* en = entries + chk_ent[ch];
-      double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      int e0 = t ? sT[t - 1] : 0, e1 = sT[t];
       for (int j = e0; j < e1; ++j) {
-        int v = en[j];
+        int v = sE[j];
         const double* fs = F + (v & 0x3ff) * 5;
-        if (v & 0x8000) {
+        double sg = (v & 0x8000) ? -1.0 : 1.0;
 #pragma unroll
-          for (int c = 0; c < 5; ++c) a[c] -= fs[c];
-        } else {
-#pragma unroll
-          for (int c = 0; c < 5; ++c) a[c] += fs[c];
-        }
+        for (int c = 0; c < 5; ++c) acc[c] = fma(sg, fs[c], acc[c]);
       }
-#pragma unroll
-      for (int c = 0; c < 5; ++c) acc[c] += a[c];
     }
   }
   if (t < nown) {
