@@ -70,6 +70,25 @@ def lib():
         L.orc_riecg_step.restype = C.c_int
         L.orc_diag.argtypes = [I64, C.c_int, _f64p, _f64p, _f64p, _f64p]
         L.orc_diag.restype = None
+        L.orc_csr_build.argtypes = [I64, I64, _i64p, C.c_int, _i64p, _i64p, I64]
+        L.orc_csr_build.restype = I64
+        L.orc_prelhs.argtypes = [I64, _f64p, _f64p, _f64p, I64, _i64p, _i64p,
+                                 _i64p, _f64p]
+        L.orc_prelhs.restype = C.c_int
+        L.orc_laplace_test.argtypes = [I64, _f64p, _f64p, _f64p, I64, _i64p,
+                                       C.c_int, _i64p, _i64p, _f64p]
+        L.orc_laplace_test.restype = C.c_int
+        L.orc_csr_dirichlet.argtypes = [I64, C.c_int, _i64p, _i64p, _f64p, I64,
+                                        D, _f64p, C.c_int]
+        L.orc_csr_dirichlet.restype = None
+        L.orc_csr_mult.argtypes = [I64, _i64p, _i64p, _f64p, _f64p, _f64p]
+        L.orc_csr_mult.restype = None
+        L.orc_cg.argtypes = [I64, C.c_int, _i64p, _i64p, _f64p, _f64p, _f64p,
+                             C.c_void_p, I64, C.c_void_p, C.c_void_p,
+                             C.c_void_p, C.c_int, C.c_int, I64, D,
+                             C.POINTER(D), C.POINTER(D), C.POINTER(I64),
+                             C.POINTER(C.c_int), C.c_void_p]
+        L.orc_cg.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -221,3 +240,83 @@ def diag(U, Un, v):
     lib().orc_diag(U.shape[0], U.shape[1], U, _c(Un, np.float64),
                    _c(v, np.float64), out)
     return out
+
+
+# ---- linear solver -----------------------------------------------------------
+
+class CSR:
+    """tk::CSR layout (0-based): ia (nrow+1), ja (nnz), a (nnz)."""
+
+    def __init__(self, npoin, ncomp, ia, ja, a=None):
+        self.npoin, self.ncomp = npoin, ncomp
+        self.ia, self.ja = ia, ja
+        self.a = np.zeros(ja.size) if a is None else a
+
+    @classmethod
+    def from_tets(cls, npoin, inpoel, ncomp=1):
+        inp = _c(inpoel, np.int64)
+        ia = np.zeros(npoin * ncomp + 1, np.int64)
+        cap = (inp.shape[0] * 12 + npoin) * ncomp
+        ja = np.empty(cap, np.int64)
+        nnz = lib().orc_csr_build(npoin, inp.shape[0], inp, ncomp, ia, ja, cap)
+        if nnz < 0:
+            raise RuntimeError("CSR capacity")
+        return cls(npoin, ncomp, ia, ja[:nnz].copy())
+
+    def dense(self):
+        n = self.npoin * self.ncomp
+        M = np.zeros((n, n))
+        for i in range(n):
+            for j in range(self.ia[i], self.ia[i + 1]):
+                M[i, self.ja[j]] = self.a[j]
+        return M
+
+    def mult(self, x):
+        r = np.empty(self.npoin * self.ncomp)
+        lib().orc_csr_mult(r.size, self.ia, self.ja, self.a, _c(x, np.float64), r)
+        return r
+
+    def dirichlet(self, i, val, b, pos=0):
+        lib().orc_csr_dirichlet(self.npoin * self.ncomp, self.ncomp, self.ia,
+                                self.ja, self.a, i, val, b, pos)
+
+
+def prelhs(coord, inpoel, A):
+    x, y, z = (_c(coord[i], np.float64) for i in range(3))
+    rc = lib().orc_prelhs(x.size, x, y, z, inpoel.shape[0],
+                          _c(inpoel, np.int64), A.ia, A.ja, A.a)
+    if rc:
+        raise RuntimeError("prelhs: entry not in the sparsity pattern")
+
+
+def laplace_test(coord, inpoel, A):
+    x, y, z = (_c(coord[i], np.float64) for i in range(3))
+    rc = lib().orc_laplace_test(x.size, x, y, z, inpoel.shape[0],
+                                _c(inpoel, np.int64), A.ncomp, A.ia, A.ja, A.a)
+    if rc:
+        raise RuntimeError("laplace assembly failed (%d)" % rc)
+
+
+def cg(A, x, b, neubc=None, dirbc=None, apply=False, pc="none", maxit=1000,
+       tol=1e-10, history=False):
+    """ConjugateGradients init/setup/solve on one partition.  dirbc: dict
+    node -> list of (mask, value) per component.  x and b are updated in
+    place (as the reference updates m_x and m_b).  Returns (normb, normr,
+    it, converged[, residual history])."""
+    nd = sorted(dirbc) if dirbc else []
+    nc = A.ncomp
+    dn = np.array(nd, np.int64)
+    dm = np.array([[m for m, _ in dirbc[i]] for i in nd], np.int32).reshape(-1)
+    dv = np.array([[v for _, v in dirbc[i]] for i in nd], np.float64).reshape(-1)
+    nb, nr, it, cv = C.c_double(), C.c_double(), C.c_int64(), C.c_int()
+    hist = np.zeros(maxit) if history else None
+    neu = _c(neubc, np.float64) if neubc is not None else None
+    lib().orc_cg(A.npoin, nc, A.ia, A.ja, A.a, x, b,
+                 neu.ctypes.data if neu is not None else None, len(nd),
+                 dn.ctypes.data if nd else None, dm.ctypes.data if nd else None,
+                 dv.ctypes.data if nd else None, int(bool(apply)),
+                 {"none": 0, "jacobi": 1}[pc], maxit, tol, C.byref(nb),
+                 C.byref(nr), C.byref(it), C.byref(cv),
+                 hist.ctypes.data if history else None)
+    out = (nb.value, nr.value, it.value, bool(cv.value))
+    return out + (hist[:it.value],) if history else out

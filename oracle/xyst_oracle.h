@@ -116,6 +116,45 @@ int orc_riecg_step(const orc_cfg* cfg, int64_t npoin, int ncomp,
 void orc_diag(int64_t npoin, int ncomp, const double* U, const double* Un,
               const double* v, double* out);
 
+/* ---- linear solver (cg.c) ------------------------------------------------ */
+
+/* tk::CSR constructor (CSR.cpp:19-84) from tets: ia[npoin*ncomp+1],
+ * ja[cap]; 0-based, rows = node + edge neighbours ascending.  Returns nnz. */
+int64_t orc_csr_build(int64_t npoin, int64_t ntet, const int64_t* inpoel,
+                      int ncomp, int64_t* ia, int64_t* ja, int64_t cap);
+
+/* ChoCG::prelhs (ChoCG.cpp:146-188): a = -K (ncomp = 1). */
+int orc_prelhs(int64_t npoin, const double* x, const double* y,
+               const double* z, int64_t ntet, const int64_t* inpoel,
+               const int64_t* ia, const int64_t* ja, double* a);
+
+/* +K of TestConjugateGradients.cpp:175-199, every component. */
+int orc_laplace_test(int64_t npoin, const double* x, const double* y,
+                     const double* z, int64_t ntet, const int64_t* inpoel,
+                     int ncomp, const int64_t* ia, const int64_t* ja,
+                     double* a);
+
+/* CSR::dirichlet (CSR.cpp:106-152), one partition. */
+void orc_csr_dirichlet(int64_t nrow, int ncomp, const int64_t* ia,
+                       const int64_t* ja, double* a, int64_t i, double val,
+                       double* b, int pos);
+
+/* CSR::mult (CSR.cpp:154-172). */
+void orc_csr_mult(int64_t nrow, const int64_t* ia, const int64_t* ja,
+                  const double* a, const double* x, double* r);
+
+/* ConjugateGradients init(apply)/setup/solve (ConjugateGradients.cpp:
+ * 105-823), one partition.  dirnode ascending, dirmask/dirval ncomp per
+ * node; pc 0 = "none", 1 = "jacobi".  x is the warm start and the result; b
+ * is modified as the reference modifies it; a is restored when apply.
+ * rhist (nullable, maxit entries) receives sqrt((r,r)) per iteration. */
+int orc_cg(int64_t npoin, int ncomp, const int64_t* ia, const int64_t* ja,
+           double* a, double* x, double* b, const double* neubc,
+           int64_t ndir, const int64_t* dirnode, const int32_t* dirmask,
+           const double* dirval, int apply, int pc, int64_t maxit,
+           double tol, double* normb_out, double* normr_out, int64_t* it_out,
+           int* converged_out, double* rhist);
+
 #ifdef __cplusplus
 }
 #endif
