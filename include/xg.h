@@ -149,6 +149,69 @@ int xg_riecg_stage_dev(xg_ctx* ctx, int stage, double dt, double* U,
 /* Number of bytes the layout occupies on the device (for reporting). */
 int xg_mesh_device_bytes(const xg_ctx* ctx, int64_t* bytes);
 
+/* ---- pressure-Poisson system (ChoCG) and conjugate gradients ---------------- */
+
+/* tk::CSR(ncomp, psup) (CSR.hpp:29-31, CSR.cpp:19-84): the sparsity pattern
+ * of the node graph of the tets, ncomp unknowns per node interleaved
+ * (row i*ncomp+k), columns ascending, diagonal included; 0-based here (the
+ * reference stores 1-based ia/ja).  cap = 0 only counts: *nnz. */
+int xg_csr_pattern(int64_t npoin, int64_t ntet, const size_t* inpoel,
+                   int ncomp, int64_t cap, int64_t* nnz, size_t* ia,
+                   size_t* ja);
+
+/* ChoCG::prelhs (ChoCG.cpp:146-188): the Laplacian A(p,q) -= grad_p.grad_q/J
+ * summed over tets (J = 6 * tet volume), into the values a of the ncomp = 1
+ * pattern above. */
+int xg_poisson_matrix(int64_t npoin, const double* x, const double* y,
+                      const double* z, int64_t ntet, const size_t* inpoel,
+                      const size_t* ia, const size_t* ja, double* a);
+
+/* ConjugateGradients(A, x, b) (ConjugateGradients.hpp:53-60) on one
+ * partition: the matrix in CSR (base 0 or 1, ia has npoin*ncomp+1 entries,
+ * rows with ascending columns and a structurally symmetric pattern, as
+ * tk::CSR builds it) is copied to the device; x = b = 0. */
+int xg_cg_matrix(xg_ctx* ctx, int ncomp, int64_t npoin, const size_t* ia,
+                 const size_t* ja, const double* a, int base);
+
+/* ConjugateGradients::init (ConjugateGradients.hpp:97-104,
+ * ConjugateGradients.cpp:335-556); the caller then calls xg_cg_setup, as the
+ * reference's init continues with setup.  x, b (host, npoin*ncomp; null
+ * keeps the current ones); with apply: b += neubc (null: none), and for the
+ * ndir nodes dirnode[k] with dirmask[k*ncomp+j] set, Dirichlet value
+ * dirval[k*ncomp+j] applied to a copy of A as CSR::dirichlet does, node by
+ * node in decreasing node order.  pc: 0 "none", 1 "jacobi". */
+int xg_cg_init(xg_ctx* ctx, const double* x, const double* b,
+               const double* neubc, int64_t ndir, const size_t* dirnode,
+               const int32_t* dirmask, const double* dirval, int apply, int pc);
+
+/* ConjugateGradients::setup (ConjugateGradients.cpp:105-200): r = b - A x,
+ * preconditioner, normb = ||b||, rho = (r,z). */
+int xg_cg_setup(xg_ctx* ctx, double* normb);
+
+/* ConjugateGradients::solve (ConjugateGradients.hpp:90-94,
+ * ConjugateGradients.cpp:559-823): iterate until sqrt((r,r)) < tol*normb
+ * (normb replaced by 1 if <= 1e-14), |(p,q)| < DBL_EPSILON, or maxit;
+ * *normr = sqrt((r,r)) of the recurrence, *it iterations. */
+int xg_cg_solve(xg_ctx* ctx, int64_t maxit, double tol, double* normr,
+                int64_t* it, int* converged);
+
+/* ConjugateGradients::solution (ConjugateGradients.hpp:152) to the host, or
+ * the device address of x (valid until xg_cg_matrix/xg_cg_destroy). */
+int xg_cg_solution(xg_ctx* ctx, double* x);
+int xg_cg_device_x(xg_ctx* ctx, double** x);
+
+/* Mixed precision solve of the system set up by xg_cg_init (no counterpart
+ * in the reference): FP64 residuals r = b - A x, FP32 preconditioned CG on
+ * A e = r to ||r_e|| < inner_tol*||r|| (or inner_maxit), x += e, until the
+ * true residual ||b - A x|| < tol*normb or maxit inner iterations in total.
+ * *normr is the true FP64 residual norm. */
+int xg_cg_solve_mixed(xg_ctx* ctx, int64_t maxit, double tol,
+                      double inner_tol, int64_t inner_maxit, double* normr,
+                      int64_t* iters, int64_t* nouter, int* converged);
+
+/* Releases the solver's device memory. */
+int xg_cg_destroy(xg_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,6 +40,18 @@ SIG = {
     "xg_riecg_dt_dev": (C.c_int, [VP, VP, D, C.POINTER(D)]),
     "xg_riecg_stage_dev": (C.c_int, [VP, C.c_int, D, VP, VP, VP, VP]),
     "xg_mesh_device_bytes": (C.c_int, [VP, C.POINTER(I64)]),
+    "xg_csr_pattern": (C.c_int, [I64, I64, VP, C.c_int, I64, C.POINTER(I64), VP, VP]),
+    "xg_poisson_matrix": (C.c_int, [I64, VP, VP, VP, I64, VP, VP, VP, VP]),
+    "xg_cg_matrix": (C.c_int, [VP, C.c_int, I64, VP, VP, VP, C.c_int]),
+    "xg_cg_init": (C.c_int, [VP, VP, VP, VP, I64, VP, VP, VP, C.c_int, C.c_int]),
+    "xg_cg_setup": (C.c_int, [VP, C.POINTER(D)]),
+    "xg_cg_solve": (C.c_int, [VP, I64, D, C.POINTER(D), C.POINTER(I64),
+                              C.POINTER(C.c_int)]),
+    "xg_cg_solution": (C.c_int, [VP, VP]),
+    "xg_cg_device_x": (C.c_int, [VP, C.POINTER(VP)]),
+    "xg_cg_solve_mixed": (C.c_int, [VP, I64, D, D, I64, C.POINTER(D), C.POINTER(I64),
+                                    C.POINTER(I64), C.POINTER(C.c_int)]),
+    "xg_cg_destroy": (C.c_int, [VP]),
 }
 
 

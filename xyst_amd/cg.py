@@ -1,0 +1,168 @@
+"""Python mirror of tk::CSR / tk::ConjugateGradients / ChoCG::prelhs over the
+C-ABI (include/xg.h).
+
+`ConjugateGradients` keeps the call sequence of the reference chare
+(ConjugateGradients.hpp:53-152): construct with (A, x, b), `init(x, b, neubc,
+dirbc, apply, pc)`, `solve(maxit, tol)`, `solution()`.  The matrix, vectors
+and iteration run on the GPU; dirbc has the reference's form
+{node: [(mask, value), ...] per component}.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._xg import check, lib, ptr
+from .riemann import Context
+
+__all__ = ["csr_pattern", "poisson_matrix", "CSR", "ConjugateGradients"]
+
+PC = {"none": 0, "jacobi": 1}
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _idx(a):
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def csr_pattern(npoin, inpoel, ncomp=1):
+    """tk::CSR sparsity (CSR.cpp:19-84), 0-based: (ia, ja) as uint64."""
+    inp = _idx(inpoel)
+    nnz = C.c_int64(0)
+    L = lib()
+    check(L.xg_csr_pattern(npoin, inp.size // 4, ptr(inp), ncomp, 0, C.byref(nnz),
+                           None, None), what="csr_pattern")
+    ia = np.empty(npoin * ncomp + 1, np.uint64)
+    ja = np.empty(nnz.value, np.uint64)
+    check(L.xg_csr_pattern(npoin, inp.size // 4, ptr(inp), ncomp, nnz.value,
+                           C.byref(nnz), ptr(ia), ptr(ja)), what="csr_pattern")
+    return ia, ja
+
+
+def poisson_matrix(coord, inpoel, ia, ja):
+    """ChoCG::prelhs (ChoCG.cpp:146-188): values of A = -K on the pattern."""
+    x, y, z = (_f64(coord[i]) for i in range(3))
+    inp = _idx(inpoel)
+    ia, ja = _idx(ia), _idx(ja)
+    a = np.empty(ja.size)
+    check(lib().xg_poisson_matrix(x.size, ptr(x), ptr(y), ptr(z), inp.size // 4,
+                                  ptr(inp), ptr(ia), ptr(ja), ptr(a)),
+          what="poisson_matrix: non-positive Jacobian or bad pattern")
+    return a
+
+
+class CSR:
+    """Host container of a tk::CSR matrix (0-based ia/ja, values a)."""
+
+    def __init__(self, ncomp, ia, ja, a=None):
+        self.ncomp = ncomp
+        self.ia = _idx(ia)
+        self.ja = _idx(ja)
+        self.a = np.zeros(self.ja.size) if a is None else _f64(a).copy()
+
+    @classmethod
+    def from_tets(cls, npoin, inpoel, ncomp=1):
+        ia, ja = csr_pattern(npoin, inpoel, ncomp)
+        return cls(ncomp, ia, ja)
+
+    @classmethod
+    def poisson(cls, coord, inpoel):
+        """The ChoCG pressure matrix of a tet mesh."""
+        A = cls.from_tets(coord.shape[1], inpoel, 1)
+        A.a = poisson_matrix(coord, inpoel, A.ia, A.ja)
+        return A
+
+    @property
+    def nrow(self):
+        return self.ia.size - 1
+
+
+class ConjugateGradients:
+    """Device-resident ConjugateGradients for one partition."""
+
+    def __init__(self, A, x=None, b=None, ctx=None, device=0):
+        self.ctx = ctx if ctx is not None else Context(device)
+        self.A = A
+        self.n = A.nrow
+        self.ncomp = A.ncomp
+        h = self.ctx._h
+        self.ctx._check(lib().xg_cg_matrix(h, A.ncomp, self.n // A.ncomp,
+                                           ptr(A.ia), ptr(A.ja), ptr(A.a), 0))
+        self.normb = None
+        self.normr = None
+        self.it = 0
+        self.converged = False
+        if x is not None or b is not None:
+            self.init(x, b)
+
+    def _vec(self, v):
+        if v is None:
+            return None
+        v = _f64(v)
+        if v.size != self.n:
+            raise ValueError("vector size %d != %d rows" % (v.size, self.n))
+        return v
+
+    def init(self, x=None, b=None, neubc=None, dirbc=None, apply=False, pc="none"):
+        """ConjugateGradients::init (ConjugateGradients.cpp:335-556) + setup."""
+        if pc not in PC:
+            raise ValueError("pc must be 'none' or 'jacobi'")
+        x, b, neubc = self._vec(x), self._vec(b), self._vec(neubc)
+        dirbc = dirbc or {}
+        nodes = np.array(sorted(dirbc), np.uint64)
+        mask = np.zeros((nodes.size, self.ncomp), np.int32)
+        val = np.zeros((nodes.size, self.ncomp))
+        for k, i in enumerate(nodes):
+            bc = dirbc[int(i)]
+            if len(bc) != self.ncomp:
+                raise ValueError("Dirichlet BC at node %d: %d components, need %d"
+                                 % (i, len(bc), self.ncomp))
+            for j, (m, v) in enumerate(bc):
+                mask[k, j] = int(bool(m))
+                val[k, j] = v
+        h = self.ctx._h
+        self.ctx._check(lib().xg_cg_init(h, ptr(x), ptr(b), ptr(neubc), nodes.size,
+                                         ptr(nodes), ptr(mask), ptr(val),
+                                         int(bool(apply)), PC[pc]))
+        self.normb = self._setup_normb()
+
+    def _setup_normb(self):
+        nb = C.c_double()
+        self.ctx._check(lib().xg_cg_setup(self.ctx._h, C.byref(nb)))
+        return nb.value
+
+    def setup(self):
+        """ConjugateGradients::setup: r = b - A x, preconditioner, normb."""
+        self.normb = self._setup_normb()
+        return self.normb
+
+    def solve(self, maxit, tol):
+        """ConjugateGradients::solve; returns sqrt((r,r)) of the recurrence."""
+        nr, it, cv = C.c_double(), C.c_int64(), C.c_int()
+        self.ctx._check(lib().xg_cg_solve(self.ctx._h, maxit, tol, C.byref(nr),
+                                          C.byref(it), C.byref(cv)))
+        self.normr, self.it, self.converged = nr.value, it.value, bool(cv.value)
+        return self.normr
+
+    def solve_mixed(self, maxit, tol, inner_tol=1e-4, inner_maxit=2000):
+        """FP32 inner CG with FP64 residual refinement (no reference
+        counterpart); returns the true FP64 residual norm."""
+        nr, it, no, cv = C.c_double(), C.c_int64(), C.c_int64(), C.c_int()
+        self.ctx._check(lib().xg_cg_solve_mixed(
+            self.ctx._h, maxit, tol, inner_tol, inner_maxit, C.byref(nr), C.byref(it),
+            C.byref(no), C.byref(cv)))
+        self.normr, self.it, self.converged = nr.value, it.value, bool(cv.value)
+        self.nouter = no.value
+        return self.normr
+
+    def solution(self):
+        x = np.empty(self.n)
+        self.ctx._check(lib().xg_cg_solution(self.ctx._h, ptr(x)))
+        return x
+
+    def device_x(self):
+        p = C.c_void_p()
+        self.ctx._check(lib().xg_cg_device_x(self.ctx._h, C.byref(p)))
+        return p.value

@@ -1,0 +1,115 @@
+// The context of one GPU-resident partition (shared by the device sources).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "xg_internal.hpp"
+
+struct DevArr {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct CgSolver;   // xg_cg.hip
+
+struct xg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  xg_config cfg{1.4, 0, 0, 0.0};
+  std::string err;
+  std::vector<DevArr> mesh_allocs;     // freed on re-upload / destroy
+  std::vector<DevArr> riecg_allocs;
+  // mesh
+  bool have_mesh = false;
+  int npoin = 0;
+  double *x = nullptr, *y = nullptr, *z = nullptr;
+  int64_t nedge = 0;
+  uint32_t* elpq = nullptr;
+  double *dx = nullptr, *dy = nullptr, *dz = nullptr;
+  // rhs blocks
+  int nblk = 0, nchunk = 0;
+  int4 *blk0 = nullptr, *blk1 = nullptr;     // (own0,nown,nloc,halo_off), (slot0,chunk0,chunk1,-)
+  int *halo = nullptr;
+  int4 *chk = nullptr;                       // (kind,first,count,tend_off)
+  int *chk_ent = nullptr;                    // ent_off
+  uint16_t *tend = nullptr, *entries = nullptr;
+  uint32_t *fab = nullptr, *fcs = nullptr;
+  int nslot = 0;
+  double* slots = nullptr;             // rhs halo partials (5 per slot)
+  double* gslots = nullptr;            // grad halo partials (15 per slot)
+  int nfx = 0;
+  int *fx_node = nullptr, *fx_off = nullptr, *fx_slot = nullptr;
+  // grad boundary faces
+  int nbnode = 0;
+  int *bnode = nullptr, *boff = nullptr, *bface = nullptr;
+  int* tri = nullptr;
+  // RieCG stepping
+  bool have_riecg = false;
+  double* vol = nullptr;
+  int nsymnode = 0;
+  int *symnode = nullptr, *symoff = nullptr;
+  double* symnorm = nullptr;
+  double* red = nullptr;               // reduction partials + result
+  // staging for the host-pointer entry points
+  double *sU = nullptr, *sG = nullptr, *sR = nullptr;
+  size_t sbytes = 0;
+  // pressure-Poisson solver (xg_cg.hip)
+  CgSolver* cg = nullptr;
+};
+
+namespace xgd {
+
+constexpr int kBlock = 256;
+constexpr int kRedBlocks = 1024;
+
+inline int fail(xg_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define XG_HIP(ctx, call)                                                    \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail(ctx, XG_EHIP, std::string(#call) + ": " +                  \
+                                    hipGetErrorString(e_));                  \
+  } while (0)
+
+inline int dev_alloc(xg_ctx* c, std::vector<DevArr>& list, void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess)
+    return fail(c, XG_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  list.push_back({*p, bytes});
+  return XG_OK;
+}
+
+inline void dev_free_all(std::vector<DevArr>& list) {
+  for (auto& a : list) (void)hipFree(a.p);
+  list.clear();
+}
+
+template <class T>
+int upload(xg_ctx* c, std::vector<DevArr>& list, T** dst, const T* src,
+           size_t n) {
+  int rc = dev_alloc(c, list, (void**)dst, n * sizeof(T));
+  if (rc) return rc;
+  if (n) XG_HIP(c, hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return XG_OK;
+}
+
+template <class T>
+int upload(xg_ctx* c, std::vector<DevArr>& list, T** dst,
+           const std::vector<T>& v) {
+  return upload(c, list, dst, v.data(), v.size());
+}
+
+inline unsigned nblocks(int64_t n) {
+  return (unsigned)((n + kBlock - 1) / kBlock);
+}
+
+}  // namespace xgd

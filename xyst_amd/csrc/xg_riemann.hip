@@ -22,108 +22,12 @@
 
 using xg::Cfg;
 
-// ----------------------------------------------------------------- context --
-
-struct DevArr {
-  void* p = nullptr;
-  size_t bytes = 0;
-};
-
-struct xg_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  xg_config cfg{1.4, 0, 0, 0.0};
-  std::string err;
-  std::vector<DevArr> mesh_allocs;     // freed on re-upload / destroy
-  std::vector<DevArr> riecg_allocs;
-  // mesh
-  bool have_mesh = false;
-  int npoin = 0;
-  double *x = nullptr, *y = nullptr, *z = nullptr;
-  int64_t nedge = 0;
-  uint32_t* elpq = nullptr;
-  double *dx = nullptr, *dy = nullptr, *dz = nullptr;
-  // rhs blocks
-  int nblk = 0, nchunk = 0;
-  int4 *blk0 = nullptr, *blk1 = nullptr;     // (own0,nown,nloc,halo_off), (slot0,chunk0,chunk1,-)
-  int *halo = nullptr;
-  int4 *chk = nullptr;                       // (kind,first,count,tend_off)
-  int *chk_ent = nullptr;                    // ent_off
-  uint16_t *tend = nullptr, *entries = nullptr;
-  uint32_t *fab = nullptr, *fcs = nullptr;
-  int nslot = 0;
-  double* slots = nullptr;             // rhs halo partials (5 per slot)
-  double* gslots = nullptr;            // grad halo partials (15 per slot)
-  int nfx = 0;
-  int *fx_node = nullptr, *fx_off = nullptr, *fx_slot = nullptr;
-  // grad boundary faces
-  int nbnode = 0;
-  int *bnode = nullptr, *boff = nullptr, *bface = nullptr;
-  int* tri = nullptr;
-  // RieCG stepping
-  bool have_riecg = false;
-  double* vol = nullptr;
-  int nsymnode = 0;
-  int *symnode = nullptr, *symoff = nullptr;
-  double* symnorm = nullptr;
-  double* red = nullptr;               // reduction partials + result
-  // staging for the host-pointer entry points
-  double *sU = nullptr, *sG = nullptr, *sR = nullptr;
-  size_t sbytes = 0;
-};
+#include "xg_ctx.hpp"
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kRedBlocks = 1024;
+using namespace xgd;
 constexpr int kRhsThreads = xg::kChunkSlots;
-
-int fail(xg_ctx* c, int code, const std::string& msg) {
-  if (c) c->err = msg;
-  return code;
-}
-
-#define XG_HIP(ctx, call)                                                    \
-  do {                                                                       \
-    hipError_t e_ = (call);                                                  \
-    if (e_ != hipSuccess)                                                    \
-      return fail(ctx, XG_EHIP, std::string(#call) + ": " +                  \
-                                    hipGetErrorString(e_));                  \
-  } while (0)
-
-int dev_alloc(xg_ctx* c, std::vector<DevArr>& list, void** p, size_t bytes) {
-  *p = nullptr;
-  if (bytes == 0) bytes = 8;
-  hipError_t e = hipMalloc(p, bytes);
-  if (e != hipSuccess)
-    return fail(c, XG_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-  list.push_back({*p, bytes});
-  return XG_OK;
-}
-
-void dev_free_all(std::vector<DevArr>& list) {
-  for (auto& a : list) (void)hipFree(a.p);
-  list.clear();
-}
-
-template <class T>
-int upload(xg_ctx* c, std::vector<DevArr>& list, T** dst, const T* src,
-           size_t n) {
-  int rc = dev_alloc(c, list, (void**)dst, n * sizeof(T));
-  if (rc) return rc;
-  if (n) XG_HIP(c, hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
-  return XG_OK;
-}
-
-template <class T>
-int upload(xg_ctx* c, std::vector<DevArr>& list, T** dst,
-           const std::vector<T>& v) {
-  return upload(c, list, dst, v.data(), v.size());
-}
-
-inline unsigned nblocks(int64_t n) {
-  return (unsigned)((n + kBlock - 1) / kBlock);
-}
 
 // Workgroup i runs on XCD i % 8; give each XCD a contiguous range of node
 // blocks so that neighbouring blocks share that XCD's L2.
@@ -134,11 +38,6 @@ __device__ inline int xcd_block(int i, int n) {
 }
 
 // ----------------------------------------------------------------- kernels --
-
-__global__ void k_fill(int64_t n, double* __restrict__ a, double v) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = v;
-}
 
 // Per-chunk bookkeeping shared by the block kernels: the chunk's entry list
 // (slot | sign bit, grouped by local node) and each local node's entry end
@@ -668,6 +567,7 @@ int xg_ctx_destroy(xg_ctx* c) {
   dev_free_all(c->mesh_allocs);
   dev_free_all(c->riecg_allocs);
   if (c->sU) { (void)hipFree(c->sU); (void)hipFree(c->sG); (void)hipFree(c->sR); }
+  if (c->cg) (void)xg_cg_destroy(c);
   delete c;
   return XG_OK;
 }
