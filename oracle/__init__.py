@@ -70,6 +70,12 @@ def lib():
         L.orc_riecg_step.restype = C.c_int
         L.orc_diag.argtypes = [I64, C.c_int, _f64p, _f64p, _f64p, _f64p]
         L.orc_diag.restype = None
+        L.orc_kozak_rhs.argtypes = [cfgp, I64, C.c_int, _f64p, _f64p, _f64p, I64,
+                                    _i64p, D, C.c_void_p, _f64p, _f64p]
+        L.orc_kozak_rhs.restype = None
+        L.orc_kozak_rhs_mag.argtypes = [cfgp, I64, C.c_int, _f64p, _f64p, _f64p,
+                                        I64, _i64p, D, _f64p, _f64p, _f64p]
+        L.orc_kozak_rhs_mag.restype = None
         L.orc_csr_build.argtypes = [I64, I64, _i64p, C.c_int, _i64p, _i64p, I64]
         L.orc_csr_build.restype = I64
         L.orc_prelhs.argtypes = [I64, _f64p, _f64p, _f64p, I64, _i64p, _i64p,
@@ -240,6 +246,26 @@ def diag(U, Un, v):
     lib().orc_diag(U.shape[0], U.shape[1], U, _c(Un, np.float64),
                    _c(v, np.float64), out)
     return out
+
+
+# ---- KozCG -------------------------------------------------------------------
+
+def kozak_rhs(c, coord, inpoel, dt, U, dtp=None, absacc=False):
+    """kozak::rhs.  With absacc also returns the round-off magnitudes A."""
+    x, y, z = (_c(coord[i], np.float64) for i in range(3))
+    U = _c(U, np.float64)
+    inp = _c(inpoel, np.int64)
+    R = np.empty_like(U)
+    if absacc:
+        assert dtp is None
+        A = np.empty_like(U)
+        lib().orc_kozak_rhs_mag(C.byref(c), U.shape[0], U.shape[1], x, y, z,
+                                inp.shape[0], inp, dt, U, R, A)
+        return R, A
+    d = _c(dtp, np.float64) if dtp is not None else None
+    lib().orc_kozak_rhs(C.byref(c), U.shape[0], U.shape[1], x, y, z, inp.shape[0],
+                        inp, dt, d.ctypes.data if d is not None else None, U, R)
+    return R
 
 
 # ---- linear solver -----------------------------------------------------------

@@ -116,6 +116,22 @@ int orc_riecg_step(const orc_cfg* cfg, int64_t npoin, int ncomp,
 void orc_diag(int64_t npoin, int ncomp, const double* U, const double* Un,
               const double* v, double* out);
 
+/* ---- KozCG (kozcg.c) ------------------------------------------------------ */
+
+/* kozak::rhs, Kozak.cpp:29-180: Taylor-Galerkin rhs, R zeroed first.  dtp:
+ * per-node time step of steady mode (null: dt for every tet). */
+void orc_kozak_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp,
+                   const double* x, const double* y, const double* z,
+                   int64_t ntet, const int64_t* inpoel, double dt,
+                   const double* dtp, const double* U, double* R);
+
+/* The same evaluation with first-order round-off magnitudes: A receives, per
+ * R entry, the sum over its added terms of m(term) (kozcg.c). */
+void orc_kozak_rhs_mag(const orc_cfg* cfg, int64_t npoin, int ncomp,
+                       const double* x, const double* y, const double* z,
+                       int64_t ntet, const int64_t* inpoel, double dt,
+                       const double* U, double* R, double* A);
+
 /* ---- linear solver (cg.c) ------------------------------------------------ */
 
 /* tk::CSR constructor (CSR.cpp:19-84) from tets: ia[npoin*ncomp+1],

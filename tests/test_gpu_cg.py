@@ -205,3 +205,27 @@ def test_poisson_solve_to_1e10(n):
     _say("  mixed it=%d outer=%d true/normb=%.3e" % (cg.it, cg.nouter, truem / nb))
     assert truem < 1e-10 * nb and nr == pytest.approx(truem, rel=1e-3)
     np.testing.assert_allclose(xm, x, rtol=0, atol=1e-8 * np.abs(x).max())
+
+
+def test_cg_apply_same_dirichlet_set_new_values():
+    """Two init(apply) calls with the same Dirichlet nodes and new values and
+    rhs (the ChoCG time-step pattern): the second system equals the oracle's
+    fresh application (normb to round-off, same iteration count within one)."""
+    _say("cg apply twice")
+    coord, inpoel, A = _poisson_case(6)
+    n = coord.shape[1]
+    rng = np.random.default_rng(4)
+    nodes = np.where(coord[0] < 1e-12)[0]
+    cg = _X().ConjugateGradients(_dev_matrix(A))
+    for it in range(3):
+        b = rng.standard_normal(n) * 5.0
+        dirbc = {int(i): [(1, float(rng.standard_normal()))] for i in nodes}
+        xo, bo = np.zeros(n), b.copy()
+        nbo, _, ito, cvo = O.cg(A, xo, bo, dirbc=dirbc, apply=True, pc="jacobi",
+                                maxit=3000, tol=1e-9)
+        cg.init(np.zeros(n), b, dirbc=dirbc, apply=True, pc="jacobi")
+        cg.solve(3000, 1e-9)
+        assert cg.normb == pytest.approx(nbo, rel=1e-13), it
+        assert cvo and cg.converged and abs(cg.it - ito) <= 1
+        np.testing.assert_allclose(cg.solution(), xo, rtol=0,
+                                   atol=1e-7 * np.abs(xo).max())

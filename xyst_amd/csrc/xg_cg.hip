@@ -57,6 +57,7 @@ struct CgSolver {
   std::vector<double> a0;
   bool dev_bc = false;           // device values carry Dirichlet rows
   bool restore = false;          // restore the original values before reuse
+  std::vector<int64_t> bc_key;   // Dirichlet (node, mask...) set of the device values
   std::vector<DevArr> allocs;
   int *dia = nullptr, *dja = nullptr;
   double* da = nullptr;          // values as used by the solve
@@ -528,7 +529,6 @@ int xg_cg_init(xg_ctx* c, const double* x, const double* b, const double* neubc,
   if (b) std::memcpy(bb.data(), b, n * 8);
   else XG_HIP(c, hipMemcpy(bb.data(), s->b, n * 8, hipMemcpyDeviceToHost));
   if (neubc) for (size_t i = 0; i < n; ++i) bb[i] += neubc[i];
-  std::vector<double> a(s->a0), rr(n, 0.0);
   std::vector<std::pair<size_t, int64_t>> order(ndir);
   for (int64_t k = 0; k < ndir; ++k) {
     if (dirnode[k] * (size_t)s->ncomp >= n)
@@ -536,21 +536,57 @@ int xg_cg_init(xg_ctx* c, const double* x, const double* b, const double* neubc,
     order[k] = {dirnode[k], k};
   }
   std::sort(order.begin(), order.end());
-  for (int64_t k = ndir - 1; k >= 0; --k) {
+  // the BC-modified matrix depends only on which (node, component) pairs are
+  // Dirichlet: with the same set as the matrix on the device holds, only b is
+  // rebuilt
+  std::vector<int64_t> key;
+  key.reserve(ndir * (1 + s->ncomp));
+  for (int64_t k = 0; k < ndir; ++k) {
     int64_t i = order[k].second;
-    for (int j = 0; j < s->ncomp; ++j)
-      if (dirmask[i * s->ncomp + j])
-        xg::csr_dirichlet(s->ncomp, s->ia.data(), s->ja.data(), a.data(),
-                          (int64_t)dirnode[i], dirval[i * s->ncomp + j], rr.data(), j);
+    key.push_back((int64_t)dirnode[i]);
+    for (int j = 0; j < s->ncomp; ++j) key.push_back(dirmask[i * s->ncomp + j] ? 1 : 0);
+  }
+  const bool same = s->dev_bc && key == s->bc_key;
+  std::vector<double> rr(n, 0.0);
+  if (same) {
+    // b -= A_col * val as CSR::dirichlet accumulates it, in decreasing node
+    // order; entries of the original columns equal the progressively zeroed
+    // ones in every row that is not itself a Dirichlet row, and those rows of
+    // b are overwritten with their values below
+    for (int64_t k = ndir - 1; k >= 0; --k) {
+      int64_t i = order[k].second;
+      for (int j = 0; j < s->ncomp; ++j) {
+        if (!dirmask[i * s->ncomp + j]) continue;
+        const int64_t col = (int64_t)dirnode[i] * s->ncomp + j;
+        const double val = dirval[i * s->ncomp + j];
+        for (int64_t e = s->ia[col]; e < s->ia[col + 1]; ++e) {
+          const int64_t r = s->ja[e];
+          const int32_t* b0 = s->ja.data() + s->ia[r];
+          const int32_t* b1 = s->ja.data() + s->ia[r + 1];
+          const int32_t* f = std::lower_bound(b0, b1, (int32_t)col);
+          if (f != b1 && *f == col) rr[r] += s->a0[f - s->ja.data()] * val;
+        }
+      }
+    }
+  } else {
+    std::vector<double> a(s->a0);
+    for (int64_t k = ndir - 1; k >= 0; --k) {
+      int64_t i = order[k].second;
+      for (int j = 0; j < s->ncomp; ++j)
+        if (dirmask[i * s->ncomp + j])
+          xg::csr_dirichlet(s->ncomp, s->ia.data(), s->ja.data(), a.data(),
+                            (int64_t)dirnode[i], dirval[i * s->ncomp + j], rr.data(), j);
+    }
+    std::vector<float> a32(a.begin(), a.end());
+    XG_HIP(c, hipMemcpy(s->da, a.data(), a.size() * 8, hipMemcpyHostToDevice));
+    XG_HIP(c, hipMemcpy(s->da32, a32.data(), a32.size() * 4, hipMemcpyHostToDevice));
+    s->bc_key.swap(key);
   }
   for (size_t i = 0; i < n; ++i) bb[i] -= rr[i];
   for (int64_t k = 0; k < ndir; ++k)
     for (int j = 0; j < s->ncomp; ++j)
       if (dirmask[k * s->ncomp + j])
         bb[dirnode[k] * s->ncomp + j] = dirval[k * s->ncomp + j];
-  std::vector<float> a32(a.begin(), a.end());
-  XG_HIP(c, hipMemcpy(s->da, a.data(), a.size() * 8, hipMemcpyHostToDevice));
-  XG_HIP(c, hipMemcpy(s->da32, a32.data(), a32.size() * 4, hipMemcpyHostToDevice));
   XG_HIP(c, hipMemcpy(s->b, bb.data(), n * 8, hipMemcpyHostToDevice));
   s->dev_bc = true;
   s->restore = false;
