@@ -149,6 +149,22 @@ int xg_riecg_stage_dev(xg_ctx* ctx, int stage, double dt, double* U,
 /* Number of bytes the layout occupies on the device (for reporting). */
 int xg_mesh_device_bytes(const xg_ctx* ctx, int64_t* bytes);
 
+/* ---- kozak:: drop-in (KozCG) ------------------------------------------------- */
+
+/* The mesh arguments of kozak::rhs (Kozak.hpp:19-27): tet connectivity
+ * inpoel (4 ids per tet) and coordinates.  The device layout (node blocks,
+ * tets by block, deterministic scatter order) is built here once. */
+int xg_kozak_upload(xg_ctx* ctx, int64_t npoin, const double* x,
+                    const double* y, const double* z, int64_t ntet,
+                    const size_t* inpoel);
+
+/* kozak::rhs (Kozak.hpp:19-27, Kozak.cpp:29-180), unsteady (dtp empty) and
+ * without a source term: R(npoin, ncomp) zeroed and filled with the
+ * Taylor-Galerkin rhs of U for time step dt.  ncomp = 5. */
+int xg_kozak_rhs(xg_ctx* ctx, int ncomp, double dt, const double* U, double* R);
+int xg_kozak_rhs_dev(xg_ctx* ctx, int ncomp, double dt, const double* U,
+                     double* R);
+
 /* ---- pressure-Poisson system (ChoCG) and conjugate gradients ---------------- */
 
 /* tk::CSR(ncomp, psup) (CSR.hpp:29-31, CSR.cpp:19-84): the sparsity pattern

@@ -74,3 +74,34 @@ def test_block_order_is_permutation_of_compact_cells():
     ext = [np.ptp(m["coord"][:, inv[i:i + 256]], axis=1).max()
            for i in range(0, perm.size - 256, 256)]
     assert np.median(ext) < 0.7      # cells of ~512 nodes are about 0.5 wide
+
+
+TET_EXE = os.path.join(ROOT, "build", "tet_layout_check")
+
+
+def _tet_exe():
+    src = [os.path.join(ROOT, "tests", "cpp", "tet_layout_check.cpp"),
+           os.path.join(ROOT, "xyst_amd", "csrc", "xg_layout.cpp")]
+    hdr = os.path.join(ROOT, "xyst_amd", "csrc", "xg_layout.hpp")
+    if not os.path.exists(TET_EXE) or any(os.path.getmtime(s) > os.path.getmtime(TET_EXE)
+                                          for s in src + [hdr]):
+        os.makedirs(os.path.dirname(TET_EXE), exist_ok=True)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-o", TET_EXE] + src, check=True)
+    return TET_EXE
+
+
+@pytest.mark.parametrize("order", ["lex", "rcm+block"])
+def test_tet_layout_matches_direct_scatter(tmp_path, order):
+    """kozak::rhs layout: each tet corner reaches its node exactly once, every
+    index the kernel forms is in bounds (CPU emulation, integer fluxes)."""
+    m = box_mesh(20, seed=9)
+    if order != "lex":
+        m = renumber_mesh(m, X.mesh_rcm(m["coord"].shape[1], m["inpoel"]))
+        m = renumber_mesh(m, X.mesh_block_order(m["coord"], 2048))
+    f = tmp_path / "tets.bin"
+    with open(f, "wb") as fh:
+        np.array([m["coord"].shape[1], m["inpoel"].shape[0]], np.int64).tofile(fh)
+        m["inpoel"].astype(np.int32).tofile(fh)
+    r = subprocess.run([_tet_exe(), str(f)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.startswith("ok")

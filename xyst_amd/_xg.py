@@ -40,6 +40,9 @@ SIG = {
     "xg_riecg_dt_dev": (C.c_int, [VP, VP, D, C.POINTER(D)]),
     "xg_riecg_stage_dev": (C.c_int, [VP, C.c_int, D, VP, VP, VP, VP]),
     "xg_mesh_device_bytes": (C.c_int, [VP, C.POINTER(I64)]),
+    "xg_kozak_upload": (C.c_int, [VP, I64, VP, VP, VP, I64, VP]),
+    "xg_kozak_rhs": (C.c_int, [VP, C.c_int, D, VP, VP]),
+    "xg_kozak_rhs_dev": (C.c_int, [VP, C.c_int, D, VP, VP]),
     "xg_csr_pattern": (C.c_int, [I64, I64, VP, C.c_int, I64, C.POINTER(I64), VP, VP]),
     "xg_poisson_matrix": (C.c_int, [I64, VP, VP, VP, I64, VP, VP, VP, VP]),
     "xg_cg_matrix": (C.c_int, [VP, C.c_int, I64, VP, VP, VP, C.c_int]),

@@ -14,6 +14,7 @@ struct DevArr {
 };
 
 struct CgSolver;   // xg_cg.hip
+struct KozakMesh;  // xg_kozak.hip
 
 struct xg_ctx {
   int device = 0;
@@ -58,7 +59,11 @@ struct xg_ctx {
   size_t sbytes = 0;
   // pressure-Poisson solver (xg_cg.hip)
   CgSolver* cg = nullptr;
+  // KozCG tet layout (xg_kozak.hip)
+  KozakMesh* koz = nullptr;
 };
+
+void xg_kozak_free(xg_ctx* c);   // xg_kozak.hip
 
 namespace xgd {
 

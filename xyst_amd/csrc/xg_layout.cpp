@@ -28,6 +28,57 @@
 
 namespace xg {
 
+// Node blocks of consecutive ids for the node graph given as upper-neighbour
+// lists (uoff/up: the neighbours with larger ids of every node), grown while
+// owned <= block_nodes and owned + halo <= block_cap; bstart receives the
+// block starts followed by npoin.
+static int partition_blocks(int npoin, const std::vector<int64_t>& uoff,
+                            const std::vector<int>& up, int block_nodes,
+                            int block_cap, std::vector<int>& bstart,
+                            std::string& err) {
+  bstart.clear();
+  std::vector<int> mark(npoin, -1);   // block id that has node in its halo
+  int s = 0, b = 0;
+  while (s < npoin) {
+    int nown = 0, nhalo = 0, end = s;
+    while (end < npoin && nown < block_nodes) {
+      // adding node `end`: it leaves the halo if it was there
+      int add = 0;
+      for (int64_t k = uoff[end]; k < uoff[end + 1]; ++k)
+        if (mark[up[k]] != b && up[k] > end) ++add;
+      int was_halo = mark[end] == b ? 1 : 0;
+      if (nown > 0 && nown + 1 + nhalo - was_halo + add > block_cap) break;
+      for (int64_t k = uoff[end]; k < uoff[end + 1]; ++k)
+        if (mark[up[k]] != b && up[k] > end) { mark[up[k]] = b; ++nhalo; }
+      if (was_halo) { --nhalo; mark[end] = -2; }
+      ++nown; ++end;
+    }
+    if (nown + nhalo > block_cap) {
+      err = "layout: a single node has more neighbours than the block capacity";
+      return XG_EUNSUPPORTED;
+    }
+    bstart.push_back(s);
+    s = end; ++b;
+  }
+  bstart.push_back(npoin);
+  return XG_OK;
+}
+
+// Upper-neighbour lists of the graph with edges (p[e], q[e]).
+static void upper_lists(int npoin, const std::vector<int>& p,
+                        const std::vector<int>& q, std::vector<int64_t>& uoff,
+                        std::vector<int>& up) {
+  const int64_t ne = (int64_t)p.size();
+  std::vector<int> ucnt(npoin + 1, 0);
+  for (int64_t e = 0; e < ne; ++e) ++ucnt[std::min(p[e], q[e]) + 1];
+  uoff.assign(npoin + 1, 0);
+  for (int i = 0; i < npoin; ++i) uoff[i + 1] = uoff[i] + ucnt[i + 1];
+  up.resize(uoff[npoin]);
+  std::vector<int64_t> pos(uoff.begin(), uoff.end() - 1);
+  for (int64_t e = 0; e < ne; ++e)
+    up[pos[std::min(p[e], q[e])]++] = std::max(p[e], q[e]);
+}
+
 int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q,
                  const std::vector<double>& d, int64_t ntri, const int* tri,
                  const uint8_t* besym, int block_nodes, int block_cap,
@@ -36,47 +87,14 @@ int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q
   L = Layout();
   L.npoin = npoin;
   // upper-neighbour lists (CSR by the smaller end) for halo tracking
-  std::vector<int> ucnt(npoin + 1, 0);
-  for (int64_t e = 0; e < ne; ++e) ++ucnt[std::min(p[e], q[e]) + 1];
-  std::vector<int64_t> uoff(npoin + 1, 0);
-  for (int i = 0; i < npoin; ++i) uoff[i + 1] = uoff[i] + ucnt[i + 1];
-  std::vector<int> up(uoff[npoin]);
-  {
-    std::vector<int64_t> pos(uoff.begin(), uoff.end() - 1);
-    for (int64_t e = 0; e < ne; ++e)
-      up[pos[std::min(p[e], q[e])]++] = std::max(p[e], q[e]);
-  }
+  std::vector<int64_t> uoff;
+  std::vector<int> up;
+  upper_lists(npoin, p, q, uoff, up);
   // (boundary faces are tet faces: their nodes are edge neighbours of the
   // face's smallest node, so the edge lists already cover them)
-  // blocks of consecutive node ids, grown while owned <= block_nodes and
-  // owned + halo <= block_cap
   std::vector<int> bstart;
-  {
-    std::vector<int> mark(npoin, -1);   // block id that has node in its halo
-    int s = 0, b = 0;
-    while (s < npoin) {
-      int nown = 0, nhalo = 0, end = s;
-      while (end < npoin && nown < block_nodes) {
-        // adding node `end`: it leaves the halo if it was there
-        int add = 0;
-        for (int64_t k = uoff[end]; k < uoff[end + 1]; ++k)
-          if (mark[up[k]] != b && up[k] > end) ++add;
-        int was_halo = mark[end] == b ? 1 : 0;
-        if (nown > 0 && nown + 1 + nhalo - was_halo + add > block_cap) break;
-        for (int64_t k = uoff[end]; k < uoff[end + 1]; ++k)
-          if (mark[up[k]] != b && up[k] > end) { mark[up[k]] = b; ++nhalo; }
-        if (was_halo) { --nhalo; mark[end] = -2; }
-        ++nown; ++end;
-      }
-      if (nown + nhalo > block_cap) {
-        err = "layout: a single node has more neighbours than the block capacity";
-        return XG_EUNSUPPORTED;
-      }
-      bstart.push_back(s);
-      s = end; ++b;
-    }
-    bstart.push_back(npoin);
-  }
+  int prc = partition_blocks(npoin, uoff, up, block_nodes, block_cap, bstart, err);
+  if (prc) return prc;
   const int nblk = (int)bstart.size() - 1;
   std::vector<int> blk_of(npoin);
   for (int b = 0; b < nblk; ++b)
@@ -195,6 +213,145 @@ int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q
   }
   L.nslot = nslot;
   // fix-up lists: nodes receiving halo partials, slots in block order
+  L.fx_off.push_back(0);
+  for (int n = 0; n < npoin; ++n) {
+    if (fix[n].empty()) continue;
+    L.fx_node.push_back(n);
+    for (auto& bs : fix[n]) L.fx_slot.push_back(bs.second);
+    L.fx_off.push_back((int)L.fx_slot.size());
+  }
+  return XG_OK;
+}
+
+// kozak::rhs layout: blocks as above on the tets' node graph; a block owns the
+// tets whose smallest node lies in it.  Tets are ordered by their local node
+// ids and cut into chunks of kTetsPerChunk; slot 4*i + a holds corner a of
+// chunk tet i, and every local node sums its slots in slot order.
+int build_tet_layout(int npoin, int64_t ntet, const size_t* inpoel,
+                     int block_nodes, int block_cap, TetLayout& L,
+                     std::string& err) {
+  static const int kTetEdge[6][2] = {{0,1},{1,2},{2,0},{0,3},{1,3},{2,3}};
+  L = TetLayout();
+  L.npoin = npoin;
+  std::vector<int> p, q;
+  p.reserve(ntet * 6);
+  q.reserve(ntet * 6);
+  for (int64_t e = 0; e < ntet; ++e)
+    for (int k = 0; k < 6; ++k) {
+      size_t a = inpoel[e * 4 + kTetEdge[k][0]], b = inpoel[e * 4 + kTetEdge[k][1]];
+      if (a >= (size_t)npoin || b >= (size_t)npoin || a == b) {
+        err = "kozak layout: bad tet connectivity";
+        return XG_EINVAL;
+      }
+      p.push_back((int)a);
+      q.push_back((int)b);
+    }
+  // unique edges suffice for the partition (duplicates only repeat marks)
+  std::vector<int64_t> uoff;
+  std::vector<int> up;
+  upper_lists(npoin, p, q, uoff, up);
+  for (int i = 0; i < npoin; ++i) {
+    auto b = up.begin() + uoff[i], e = up.begin() + uoff[i + 1];
+    std::sort(b, e);
+  }
+  {  // compact duplicates
+    std::vector<int64_t> o2(npoin + 1, 0);
+    std::vector<int> u2;
+    u2.reserve(up.size() / 2);
+    for (int i = 0; i < npoin; ++i) {
+      for (int64_t k = uoff[i]; k < uoff[i + 1]; ++k)
+        if (k == uoff[i] || up[k] != up[k - 1]) u2.push_back(up[k]);
+      o2[i + 1] = (int64_t)u2.size();
+    }
+    uoff.swap(o2);
+    up.swap(u2);
+  }
+  std::vector<int>().swap(p);
+  std::vector<int>().swap(q);
+  std::vector<int> bstart;
+  int prc = partition_blocks(npoin, uoff, up, block_nodes, block_cap, bstart, err);
+  if (prc) return prc;
+  const int nblk = (int)bstart.size() - 1;
+  std::vector<int> blk_of(npoin);
+  for (int b = 0; b < nblk; ++b)
+    for (int i = bstart[b]; i < bstart[b + 1]; ++i) blk_of[i] = b;
+  std::vector<std::vector<int64_t>> btets(nblk);
+  for (int64_t e = 0; e < ntet; ++e) {
+    const size_t* N = inpoel + e * 4;
+    size_t m = std::min(std::min(N[0], N[1]), std::min(N[2], N[3]));
+    btets[blk_of[m]].push_back(e);
+  }
+  L.blk.resize(nblk);
+  std::vector<std::vector<std::pair<int,int>>> fix(npoin);
+  int nslot = 0;
+  std::vector<int> local(npoin, -1);
+  for (int b = 0; b < nblk; ++b) {
+    Block& B = L.blk[b];
+    B.own0 = bstart[b];
+    B.nown = bstart[b + 1] - bstart[b];
+    std::vector<int> halo;
+    for (int64_t e : btets[b])
+      for (int a = 0; a < 4; ++a)
+        if ((int)inpoel[e * 4 + a] >= bstart[b + 1]) halo.push_back((int)inpoel[e * 4 + a]);
+    std::sort(halo.begin(), halo.end());
+    halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+    B.nloc = B.nown + (int)halo.size();
+    if (B.nloc > block_cap) {
+      err = "kozak layout: block exceeds LDS capacity";
+      return XG_EUNSUPPORTED;
+    }
+    B.halo_off = (int)L.halo.size();
+    B.slot0 = nslot;
+    for (size_t h = 0; h < halo.size(); ++h) {
+      L.halo.push_back(halo[h]);
+      fix[halo[h]].emplace_back(b, nslot + (int)h);
+    }
+    nslot += (int)halo.size();
+    for (int i = 0; i < B.nown; ++i) local[B.own0 + i] = i;
+    for (size_t h = 0; h < halo.size(); ++h) local[halo[h]] = B.nown + (int)h;
+    auto& T = btets[b];
+    auto key = [&](int64_t e) {
+      int l[4];
+      for (int a = 0; a < 4; ++a) l[a] = local[inpoel[e * 4 + a]];
+      std::sort(l, l + 4);
+      return ((uint64_t)l[0] << 48) | ((uint64_t)l[1] << 32) | ((uint64_t)l[2] << 16) |
+             (uint64_t)l[3];
+    };
+    std::stable_sort(T.begin(), T.end(),
+                     [&](int64_t a, int64_t c) { return key(a) < key(c); });
+    B.chunk0 = (int)L.chunks.size();
+    for (size_t i0 = 0; i0 < T.size(); i0 += kTetsPerChunk) {
+      size_t i1 = std::min(T.size(), i0 + (size_t)kTetsPerChunk);
+      std::vector<std::pair<int, uint16_t>> ent;
+      for (size_t i = i0; i < i1; ++i)
+        for (int a = 0; a < 4; ++a)
+          ent.emplace_back(local[inpoel[T[i] * 4 + a]], (uint16_t)(4 * (i - i0) + a));
+      std::stable_sort(ent.begin(), ent.end(),
+                       [](const auto& x, const auto& y) { return x.first < y.first; });
+      Chunk C;
+      C.kind = 2;
+      C.first = (int)L.tab.size();
+      C.count = (int)(i1 - i0);
+      C.tend_off = (int)L.tend.size();
+      C.ent_off = (int)L.entries.size();
+      size_t k = 0;
+      for (int n = 0; n < B.nloc; ++n) {
+        while (k < ent.size() && ent[k].first == n) L.entries.push_back(ent[k++].second);
+        L.tend.push_back((uint16_t)(L.entries.size() - C.ent_off));
+      }
+      L.chunks.push_back(C);
+      for (size_t i = i0; i < i1; ++i) {
+        const size_t* N = inpoel + T[i] * 4;
+        L.tab.push_back((uint32_t)local[N[0]] | ((uint32_t)local[N[1]] << 16));
+        L.tcd.push_back((uint32_t)local[N[2]] | ((uint32_t)local[N[3]] << 16));
+        for (int a = 0; a < 4; ++a) L.tglob.push_back((int)N[a]);
+      }
+    }
+    B.chunk1 = (int)L.chunks.size();
+    for (int i = 0; i < B.nown; ++i) local[B.own0 + i] = -1;
+    for (size_t h = 0; h < halo.size(); ++h) local[halo[h]] = -1;
+  }
+  L.nslot = nslot;
   L.fx_off.push_back(0);
   for (int n = 0; n < npoin; ++n) {
     if (fix[n].empty()) continue;

@@ -51,6 +51,29 @@ struct Layout {
   std::vector<int> fx_node, fx_off, fx_slot;
 };
 
+// kozak::rhs blocks: a block stages U (5) and coordinates (3) of up to
+// kKozCap nodes in LDS beside one chunk's corner fluxes; a chunk holds
+// kChunkSlots / 4 tets, one thread per tet corner
+constexpr int kKozCap = 1024;
+constexpr int kKozNodes = 768;
+constexpr int kTetsPerChunk = kChunkSlots / 4;
+
+struct TetLayout {
+  int npoin = 0;
+  std::vector<uint32_t> tab, tcd;  // tets in block/chunk order: local a|b<<16, c|d<<16
+  std::vector<int> tglob;          // their global nodes (4 per tet, checks)
+  std::vector<Block> blk;
+  std::vector<Chunk> chunks;       // kind 2
+  std::vector<int> halo;
+  std::vector<uint16_t> tend, entries;   // entry = slot 4*i+a
+  int nslot = 0;
+  std::vector<int> fx_node, fx_off, fx_slot;
+};
+
+int build_tet_layout(int npoin, int64_t ntet, const size_t* inpoel,
+                     int block_nodes, int block_cap, TetLayout& L,
+                     std::string& err);
+
 int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q,
                  const std::vector<double>& d, int64_t ntri, const int* tri,
                  const uint8_t* besym, int block_nodes, int block_cap,

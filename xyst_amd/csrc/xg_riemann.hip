@@ -22,20 +22,13 @@
 
 using xg::Cfg;
 
+#include "xg_blocks.hpp"
 #include "xg_ctx.hpp"
 
 namespace {
 
 using namespace xgd;
 constexpr int kRhsThreads = xg::kChunkSlots;
-
-// Workgroup i runs on XCD i % 8; give each XCD a contiguous range of node
-// blocks so that neighbouring blocks share that XCD's L2.
-__device__ inline int xcd_block(int i, int n) {
-  int x = i & 7, j = i >> 3;
-  int q = n >> 3, r = n & 7;
-  return x * q + min(x, r) + j;
-}
 
 // ----------------------------------------------------------------- kernels --
 
@@ -158,27 +151,6 @@ __global__ __launch_bounds__(kRhsThreads) void k_grad_blocks(
 #pragma unroll
     for (int k = 0; k < 15; ++k) o[k] = acc[k];
   }
-}
-
-// Adds halo partials (NC components per node) to their owners, block order.
-template <int NC>
-__global__ __launch_bounds__(kBlock) void k_fixup(
-    int nfx, const int* __restrict__ fx_node, const int* __restrict__ fx_off,
-    const int* __restrict__ fx_slot, const double* __restrict__ S,
-    double* __restrict__ R) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nfx) return;
-  double* r = R + (size_t)fx_node[i] * NC;
-  double acc[NC];
-#pragma unroll
-  for (int c = 0; c < NC; ++c) acc[c] = r[c];
-  for (int k = fx_off[i]; k < fx_off[i + 1]; ++k) {
-    const double* s = S + (size_t)fx_slot[k] * NC;
-#pragma unroll
-    for (int c = 0; c < NC; ++c) acc[c] += s[c];
-  }
-#pragma unroll
-  for (int c = 0; c < NC; ++c) r[c] = acc[c];
 }
 
 // riemann::grad boundary integral (Riemann.cpp:332-359) gathered per node.
@@ -568,6 +540,7 @@ int xg_ctx_destroy(xg_ctx* c) {
   dev_free_all(c->riecg_allocs);
   if (c->sU) { (void)hipFree(c->sU); (void)hipFree(c->sG); (void)hipFree(c->sR); }
   if (c->cg) (void)xg_cg_destroy(c);
+  if (c->koz) xg_kozak_free(c);
   delete c;
   return XG_OK;
 }
