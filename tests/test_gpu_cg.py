@@ -92,10 +92,12 @@ def test_cg_apply_matches_oracle(pc):
     scale = np.abs(xo).max()
     np.testing.assert_allclose(x, xo, rtol=0, atol=1e-6 * scale)
     # true residual of the BC-modified system (bo holds its rhs)
+    # (CSR::dirichlet accumulates A_col*val; ConjugateGradients subtracts it)
     Ab = O.CSR(A.npoin, 1, A.ia, A.ja, A.a.copy())
-    bb = b + neu
+    rr = np.zeros(n)
     for i in sorted(dirbc, reverse=True):
-        Ab.dirichlet(i, dirbc[i][0][1], bb)
+        Ab.dirichlet(i, dirbc[i][0][1], rr)
+    bb = b + neu - rr
     for i in dirbc:
         bb[i] = dirbc[i][0][1]
     np.testing.assert_allclose(bb, bo, rtol=0, atol=1e-12 * np.abs(bo).max())
