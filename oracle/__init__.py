@@ -70,6 +70,28 @@ def lib():
         L.orc_riecg_step.restype = C.c_int
         L.orc_diag.argtypes = [I64, C.c_int, _f64p, _f64p, _f64p, _f64p]
         L.orc_diag.restype = None
+        L.orc_chocg_domint.argtypes = L.orc_domint.argtypes
+        L.orc_chocg_domint.restype = I64
+        L.orc_domsuped5.argtypes = L.orc_domsuped.argtypes
+        L.orc_domsuped5.restype = C.c_int
+        se = [_i64p, _i64p, _f64p, _i64p, _f64p, _i64p, _f64p]
+        L.orc_chorin_div.argtypes = se + [_f64p, _f64p, _f64p, I64, _i64p, D, _f64p,
+                                          _f64p, C.c_int, _f64p, C.c_int, _f64p,
+                                          C.c_int, C.c_void_p]
+        L.orc_chorin_div.restype = None
+        L.orc_chorin_vgrad.argtypes = se + [_f64p, _f64p, _f64p, I64, _i64p, _f64p,
+                                            C.c_int, _f64p, C.c_void_p]
+        L.orc_chorin_vgrad.restype = None
+        L.orc_chorin_grad.argtypes = se + [_f64p, _f64p, _f64p, I64, _i64p, _f64p,
+                                           _f64p, C.c_int, C.c_void_p]
+        L.orc_chorin_grad.restype = None
+        L.orc_chorin_flux.argtypes = se + [_f64p, _f64p, _f64p, I64, _i64p, I64,
+                                           _f64p, C.c_int, _f64p, C.c_int, D, _f64p,
+                                           C.c_void_p]
+        L.orc_chorin_flux.restype = None
+        L.orc_chorin_rhs.argtypes = [C.POINTER(ChoCfg), I64, C.c_int] + se + [
+            _f64p, _f64p, _f64p, I64, _i64p, _f64p, _f64p, _f64p, _f64p, C.c_void_p]
+        L.orc_chorin_rhs.restype = C.c_int
         L.orc_kozak_rhs.argtypes = [cfgp, I64, C.c_int, _f64p, _f64p, _f64p, I64,
                                     _i64p, D, C.c_void_p, _f64p, _f64p]
         L.orc_kozak_rhs.restype = None
@@ -130,6 +152,18 @@ def domint(coord, inpoel):
     return edge[:2 * ne].reshape(-1, 2).copy(), dint[:3 * ne].reshape(-1, 3).copy()
 
 
+def chocg_domint(coord, inpoel):
+    """ChoCG::domint: unique edges (E,2) p<q and 5 integrals per edge."""
+    x, y, z = (_c(coord[i], np.float64) for i in range(3))
+    ntet = inpoel.shape[0]
+    cap = 6 * ntet
+    edge = np.empty(2 * cap, np.int64)
+    dint = np.empty(5 * cap)
+    ne = lib().orc_chocg_domint(x.size, x, y, z, ntet, _c(inpoel, np.int64), cap,
+                                edge, dint)
+    return edge[:2 * ne].reshape(-1, 2).copy(), dint[:5 * ne].reshape(-1, 5).copy()
+
+
 class Superedges:
     """The (dsupedge, dsupint) triple of arrays consumed by riemann::grad/rhs."""
 
@@ -152,20 +186,101 @@ class Superedges:
 
 
 def domsuped(inpoel, edge, dint):
+    """Superedges of RieCG (stride 3) or ChoCG (stride 5) edge integrals."""
+    st = dint.shape[1] if dint.ndim == 2 else 3
     ntet = inpoel.shape[0]
     ne = edge.shape[0]
     nsup = np.zeros(3, np.int64)
-    sup0 = np.empty(4 * ntet, np.int64); int0 = np.empty(18 * ntet)
-    sup1 = np.empty(12 * ntet, np.int64); int1 = np.empty(36 * ntet)
-    sup2 = np.empty(2 * ne, np.int64); int2 = np.empty(3 * ne)
-    rc = lib().orc_domsuped(ntet, _c(inpoel, np.int64), ne, _c(edge, np.int64),
-                            _c(dint, np.float64), nsup, sup0, int0, sup1, int1,
-                            sup2, int2)
+    sup0 = np.empty(4 * ntet, np.int64); int0 = np.empty(6 * st * ntet)
+    sup1 = np.empty(12 * ntet, np.int64); int1 = np.empty(12 * st * ntet)
+    sup2 = np.empty(2 * ne, np.int64); int2 = np.empty(st * ne)
+    fn = lib().orc_domsuped if st == 3 else lib().orc_domsuped5
+    rc = fn(ntet, _c(inpoel, np.int64), ne, _c(edge, np.int64),
+            _c(dint, np.float64), nsup, sup0, int0, sup1, int1, sup2, int2)
     if rc:
         raise RuntimeError("Not all edges accounted for in superedge groups")
     n0, n1, n2 = (int(v) for v in nsup)
     return Superedges([sup0[:4 * n0], sup1[:3 * n1], sup2[:2 * n2]],
-                      [int0[:18 * n0], int1[:9 * n1], int2[:3 * n2]])
+                      [int0[:6 * st * n0], int1[:3 * st * n1], int2[:st * n2]])
+
+
+# ---- ChoCG operators ----------------------------------------------------------
+
+class ChoCfg(C.Structure):
+    _fields_ = [("flux", C.c_int), ("stab", C.c_int), ("stab2", C.c_int),
+                ("stab2coef", C.c_double), ("mu", C.c_double), ("diff", C.c_double)]
+
+
+def chocfg(flux="damp2", stab=True, stab2=False, stab2coef=0.2, mu=0.0, diff=0.0):
+    return ChoCfg({"damp2": 0, "damp4": 1}.get(flux, 99), int(bool(stab)),
+                  int(bool(stab2)), stab2coef, mu, diff)
+
+
+def _xyz(coord):
+    return [_c(coord[i], np.float64) for i in range(3)]
+
+
+def _tri(tri):
+    return _c(tri, np.int64) if tri.size else np.zeros(3, np.int64)
+
+
+def chorin_div(se, coord, tri, dt, P, G, U, D=None, stab=False, absacc=False):
+    """chorin::div into D (zeros if not given); U (npoin, >=3)."""
+    U = _c(U, np.float64)
+    G = _c(G, np.float64)
+    n = U.shape[0]
+    D = np.zeros(n) if D is None else D
+    A = np.zeros(n) if absacc else None
+    lib().orc_chorin_div(*se.args(), *_xyz(coord), tri.shape[0], _tri(tri), dt,
+                         _c(P, np.float64), G, G.shape[1], U, U.shape[1], D,
+                         int(bool(stab)), A.ctypes.data if absacc else None)
+    return (D, A) if absacc else D
+
+
+def chorin_vgrad(se, coord, tri, U, absacc=False):
+    U = _c(U, np.float64)
+    n, nc = U.shape
+    G = np.zeros((n, 3 * nc))
+    A = np.zeros_like(G) if absacc else None
+    lib().orc_chorin_vgrad(*se.args(), *_xyz(coord), tri.shape[0], _tri(tri), U, nc,
+                           G, A.ctypes.data if absacc else None)
+    return (G, A) if absacc else G
+
+
+def chorin_grad(se, coord, tri, U, absacc=False):
+    U = _c(U, np.float64)
+    G = np.zeros((U.size, 3))
+    A = np.zeros_like(G) if absacc else None
+    lib().orc_chorin_grad(*se.args(), *_xyz(coord), tri.shape[0], _tri(tri), U, G, 3,
+                          A.ctypes.data if absacc else None)
+    return (G, A) if absacc else G
+
+
+def chorin_flux(se, coord, tri, U, G, mu, absacc=False):
+    U = _c(U, np.float64)
+    G = _c(G, np.float64)
+    n = U.shape[0]
+    F = np.zeros((n, 3))
+    A = np.zeros_like(F) if absacc else None
+    lib().orc_chorin_flux(*se.args(), *_xyz(coord), tri.shape[0], _tri(tri), n, U,
+                          U.shape[1], G, G.shape[1], mu, F,
+                          A.ctypes.data if absacc else None)
+    return (F, A) if absacc else F
+
+
+def chorin_rhs(c, se, coord, tri, P, U, G, absacc=False):
+    U = _c(U, np.float64)
+    n, nc = U.shape
+    R = np.empty((n, nc))
+    A = np.empty_like(R) if absacc else None
+    rc = lib().orc_chorin_rhs(C.byref(c), n, nc, *se.args(), *_xyz(coord),
+                              tri.shape[0], _tri(tri), _c(P, np.float64), U,
+                              _c(G, np.float64), R, A.ctypes.data if absacc else None)
+    if rc == -1:
+        raise ValueError("Flux not correctly configured")
+    if rc:
+        raise ValueError("unsupported number of components")
+    return (R, A) if absacc else R
 
 
 def symbc_setup(coord, tri, faceset, symsets):

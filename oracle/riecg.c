@@ -65,13 +65,15 @@ static int edrec_cmp(const void* a, const void* b) {
   return (u->rec > v->rec) - (u->rec < v->rec);
 }
 
-int64_t orc_domint(int64_t npoin, const double* x, const double* y,
-                   const double* z, int64_t ntet, const int64_t* inpoel,
-                   int64_t cap, int64_t* edge, double* dint) {
-  (void)npoin;
+/* stride 3: RieCG::domint (RieCG.cpp:339-382); stride 5: ChoCG::domint
+ * (ChoCG.cpp:398-445), which also sums J/120 and dot(grad_p,grad_q)/J/6 */
+static int64_t domint_stride(int stride, const double* x, const double* y,
+                             const double* z, int64_t ntet,
+                             const int64_t* inpoel, int64_t cap, int64_t* edge,
+                             double* dint) {
   int64_t nrec = ntet * 6;
   edrec* r = (edrec*)malloc(sizeof(edrec) * (size_t)nrec);
-  double* c = (double*)malloc(sizeof(double) * 3 * (size_t)nrec);
+  double* c = (double*)malloc(sizeof(double) * (size_t)stride * (size_t)nrec);
   for (int64_t e = 0; e < ntet; ++e) {
     const int64_t* N = inpoel + e*4;
     double ba[3] = {x[N[1]]-x[N[0]], y[N[1]]-y[N[0]], z[N[1]]-z[N[0]]};
@@ -83,6 +85,7 @@ int64_t orc_domint(int64_t npoin, const double* x, const double* y,
     cross(ba, ca, grad[3]);
     for (int i = 0; i < 3; ++i)
       grad[0][i] = -grad[1][i]-grad[2][i]-grad[3][i];
+    const double J = ba[0]*grad[1][0] + ba[1]*grad[1][1] + ba[2]*grad[1][2];
     for (int k = 0; k < 6; ++k) {
       int p = lpoed[k][0], q = lpoed[k][1];
       int64_t a = N[p], b = N[q];
@@ -91,7 +94,12 @@ int64_t orc_domint(int64_t npoin, const double* x, const double* y,
       int64_t i = e*6 + k;
       r[i].p = a; r[i].q = b; r[i].rec = i;
       for (int j = 0; j < 3; ++j)
-        c[i*3+j] = sig * (grad[p][j] - grad[q][j]) / 48.0;
+        c[i*stride+j] = sig * (grad[p][j] - grad[q][j]) / 48.0;
+      if (stride == 5) {
+        c[i*5+3] = J / 120.0;
+        c[i*5+4] = (grad[p][0]*grad[q][0] + grad[p][1]*grad[q][1] +
+                    grad[p][2]*grad[q][2]) / J / 6.0;
+      }
     }
   }
   qsort(r, (size_t)nrec, sizeof(edrec), edrec_cmp);
@@ -100,14 +108,28 @@ int64_t orc_domint(int64_t npoin, const double* x, const double* y,
     if (i == 0 || r[i].p != r[i-1].p || r[i].q != r[i-1].q) {
       if (ne == cap) { free(r); free(c); return -1; }
       edge[ne*2+0] = r[i].p; edge[ne*2+1] = r[i].q;
-      dint[ne*3+0] = dint[ne*3+1] = dint[ne*3+2] = 0.0;
+      for (int j = 0; j < stride; ++j) dint[ne*stride+j] = 0.0;
       ++ne;
     }
     /* accumulation in tet order, as into the hash map at RieCG.cpp:377-379 */
-    for (int j = 0; j < 3; ++j) dint[(ne-1)*3+j] += c[r[i].rec*3+j];
+    for (int j = 0; j < stride; ++j) dint[(ne-1)*stride+j] += c[r[i].rec*stride+j];
   }
   free(r); free(c);
   return ne;
+}
+
+int64_t orc_domint(int64_t npoin, const double* x, const double* y,
+                   const double* z, int64_t ntet, const int64_t* inpoel,
+                   int64_t cap, int64_t* edge, double* dint) {
+  (void)npoin;
+  return domint_stride(3, x, y, z, ntet, inpoel, cap, edge, dint);
+}
+
+int64_t orc_chocg_domint(int64_t npoin, const double* x, const double* y,
+                         const double* z, int64_t ntet, const int64_t* inpoel,
+                         int64_t cap, int64_t* edge, double* dint) {
+  (void)npoin;
+  return domint_stride(5, x, y, z, ntet, inpoel, cap, edge, dint);
 }
 
 /* ------------------------------------------------------------- domsuped -- */
@@ -146,10 +168,14 @@ static void sort3(const int64_t* n, int64_t* k) {
   k[0] = a; k[1] = b; k[2] = c;
 }
 
-int orc_domsuped(int64_t ntet, const int64_t* inpoel, int64_t nedge,
-                 const int64_t* edge, const double* dint, int64_t* nsup,
-                 int64_t* sup0, double* int0, int64_t* sup1, double* int1,
-                 int64_t* sup2, double* int2) {
+/* Superedge grouping for edge integrals of `stride` values per edge, the
+ * first three of which (the normal components) change sign with the edge
+ * orientation; RieCG uses stride 3, ChoCG stride 5 (ChoCG.cpp:687-813). */
+static int domsuped_stride(int stride, int64_t ntet, const int64_t* inpoel,
+                           int64_t nedge, const int64_t* edge,
+                           const double* dint, int64_t* nsup, int64_t* sup0,
+                           double* int0, int64_t* sup1, double* int1,
+                           int64_t* sup2, double* int2) {
   char* used = (char*)calloc((size_t)nedge, 1);
   int64_t n0 = 0, n1 = 0, n2 = 0;
   /* tet superedges, RieCG.cpp:657-680 */
@@ -167,8 +193,8 @@ int orc_domsuped(int64_t ntet, const int64_t* inpoel, int64_t nedge,
     if (f == 6) {
       for (int i = 0; i < 4; ++i) sup0[n0*4+i] = N[i];
       for (int k = 0; k < 6; ++k) {
-        for (int j = 0; j < 3; ++j)
-          int0[(n0*6+k)*3+j] = sig[k] * dint[id[k]*3+j];
+        for (int j = 0; j < stride; ++j)
+          int0[(n0*6+k)*stride+j] = (j < 3 ? sig[k] : 1.0) * dint[id[k]*stride+j];
         used[id[k]] = 1;
       }
       ++n0;
@@ -206,8 +232,8 @@ int orc_domsuped(int64_t ntet, const int64_t* inpoel, int64_t nedge,
     if (f == 3) {
       for (int j = 0; j < 3; ++j) sup1[n1*3+j] = N[j];
       for (int k = 0; k < 3; ++k) {
-        for (int j = 0; j < 3; ++j)
-          int1[(n1*3+k)*3+j] = sig[k] * dint[id[k]*3+j];
+        for (int j = 0; j < stride; ++j)
+          int1[(n1*3+k)*stride+j] = (j < 3 ? sig[k] : 1.0) * dint[id[k]*stride+j];
         used[id[k]] = 1;
       }
       ++n1;
@@ -219,12 +245,28 @@ int orc_domsuped(int64_t ntet, const int64_t* inpoel, int64_t nedge,
     if (used[m]) continue;
     sup2[n2*2+0] = edge[m*2+0];
     sup2[n2*2+1] = edge[m*2+1];
-    for (int j = 0; j < 3; ++j) int2[n2*3+j] = dint[m*3+j];
+    for (int j = 0; j < stride; ++j) int2[n2*stride+j] = dint[m*stride+j];
     ++n2;
   }
   free(used);
   nsup[0] = n0; nsup[1] = n1; nsup[2] = n2;
   return (n0*6 + n1*3 + n2 == nedge) ? 0 : -1;   /* RieCG.cpp:733-735 */
+}
+
+int orc_domsuped(int64_t ntet, const int64_t* inpoel, int64_t nedge,
+                 const int64_t* edge, const double* dint, int64_t* nsup,
+                 int64_t* sup0, double* int0, int64_t* sup1, double* int1,
+                 int64_t* sup2, double* int2) {
+  return domsuped_stride(3, ntet, inpoel, nedge, edge, dint, nsup, sup0, int0,
+                         sup1, int1, sup2, int2);
+}
+
+int orc_domsuped5(int64_t ntet, const int64_t* inpoel, int64_t nedge,
+                  const int64_t* edge, const double* dint, int64_t* nsup,
+                  int64_t* sup0, double* int0, int64_t* sup1, double* int1,
+                  int64_t* sup2, double* int2) {
+  return domsuped_stride(5, ntet, inpoel, nedge, edge, dint, nsup, sup0, int0,
+                         sup1, int1, sup2, int2);
 }
 
 /* ------------------------------------------------------------ symmetry BC -- */

@@ -132,6 +132,72 @@ void orc_kozak_rhs_mag(const orc_cfg* cfg, int64_t npoin, int ncomp,
                        int64_t ntet, const int64_t* inpoel, double dt,
                        const double* U, double* R, double* A);
 
+/* ---- ChoCG (chocg.c, riecg.c) -------------------------------------------- */
+
+/* ChoCG configuration read from g_cfg (InciterConfig.cpp:1724-1757) */
+typedef struct {
+  int    flux;        /* 0 = damp2, 1 = damp4 (Chorin.cpp:862-866) */
+  int    stab;        /* |vn_L + vn_R|/2 stabilization */
+  int    stab2;
+  double stab2coef;
+  double mu;          /* mat_dyn_viscosity */
+  double diff;        /* mat_dyn_diffusivity */
+} orc_chocfg;
+
+/* ChoCG::domint, ChoCG.cpp:398-445: as orc_domint with 5 values per edge
+ * (signed normal components, J/120, dot(grad_p,grad_q)/J/6). */
+int64_t orc_chocg_domint(int64_t npoin, const double* x, const double* y,
+                         const double* z, int64_t ntet, const int64_t* inpoel,
+                         int64_t cap, int64_t* edge, double* dint);
+
+/* ChoCG::domsuped, ChoCG.cpp:687-813: orc_domsuped with stride-5 integrals
+ * (only the first three components change sign with the orientation). */
+int orc_domsuped5(int64_t ntet, const int64_t* inpoel, int64_t nedge,
+                  const int64_t* edge, const double* dint, int64_t* nsup,
+                  int64_t* sup0, double* int0, int64_t* sup1, double* int1,
+                  int64_t* sup2, double* int2);
+
+/* chorin::div, Chorin.cpp:34-209 (accumulates into D; P, G pressure and its
+ * gradient (ldg columns, 0..2 used) enter only with stab) */
+void orc_chorin_div(const int64_t* nsup, const int64_t* sup0, const double* int0,
+                    const int64_t* sup1, const double* int1, const int64_t* sup2,
+                    const double* int2, const double* x, const double* y,
+                    const double* z, int64_t ntri, const int64_t* tri, double dt,
+                    const double* P, const double* G, int ldg, const double* U,
+                    int ldu, double* D, int stab, double* A);
+
+/* chorin::vgrad, Chorin.cpp:211-334 (accumulates G(npoin, 3*ncomp)) */
+void orc_chorin_vgrad(const int64_t* nsup, const int64_t* sup0, const double* int0,
+                      const int64_t* sup1, const double* int1, const int64_t* sup2,
+                      const double* int2, const double* x, const double* y,
+                      const double* z, int64_t ntri, const int64_t* tri,
+                      const double* U, int ncomp, double* G, double* A);
+
+/* chorin::grad, Chorin.cpp:336-449 (scalar U; accumulates G columns 0..2) */
+void orc_chorin_grad(const int64_t* nsup, const int64_t* sup0, const double* int0,
+                     const int64_t* sup1, const double* int1, const int64_t* sup2,
+                     const double* int2, const double* x, const double* y,
+                     const double* z, int64_t ntri, const int64_t* tri,
+                     const double* U, double* G, int ldg, double* A);
+
+/* chorin::flux, Chorin.cpp:451-638 (accumulates F(npoin, 3)) */
+void orc_chorin_flux(const int64_t* nsup, const int64_t* sup0, const double* int0,
+                     const int64_t* sup1, const double* int1, const int64_t* sup2,
+                     const double* int2, const double* x, const double* y,
+                     const double* z, int64_t ntri, const int64_t* tri,
+                     int64_t npoin, const double* U, int ldu, const double* G,
+                     int ldg, double mu, double* F, double* A);
+
+/* chorin::rhs, Chorin.cpp:1010-1044 (adv only: no source); R zeroed.
+ * Returns -1 for an unknown flux, -2 for ncomp outside 3..16. */
+int orc_chorin_rhs(const orc_chocfg* cfg, int64_t npoin, int ncomp,
+                   const int64_t* nsup, const int64_t* sup0, const double* int0,
+                   const int64_t* sup1, const double* int1, const int64_t* sup2,
+                   const double* int2, const double* x, const double* y,
+                   const double* z, int64_t ntri, const int64_t* tri,
+                   const double* P, const double* U, const double* G, double* R,
+                   double* A);
+
 /* ---- linear solver (cg.c) ------------------------------------------------ */
 
 /* tk::CSR constructor (CSR.cpp:19-84) from tets: ia[npoin*ncomp+1],
