@@ -165,6 +165,57 @@ int xg_kozak_rhs(xg_ctx* ctx, int ncomp, double dt, const double* U, double* R);
 int xg_kozak_rhs_dev(xg_ctx* ctx, int ncomp, double dt, const double* U,
                      double* R);
 
+/* ---- chorin:: drop-ins (ChoCG) ------------------------------------------------ */
+
+/* ChoCG's g_cfg reads of chorin::rhs (InciterConfig.cpp:1724-1757). */
+typedef struct {
+  int    flux;        /* 0 "damp2", 1 "damp4" (Chorin.cpp:862-866) */
+  int    stab;        /* stab: |vn_L + vn_R|/2 */
+  int    stab2;
+  double stab2coef;
+  double mu;          /* mat_dyn_viscosity */
+} xg_chocfg;
+
+/* The mesh arguments of the chorin:: operators (Chorin.hpp:19-70): the
+ * superedges of ChoCG::domsuped with 5 values per edge integral
+ * (ChoCG.cpp:398-445, 687-813: normal components, J/120,
+ * dot(grad_p,grad_q)/J/6), coordinates and boundary triangles. */
+int xg_chocg_upload(xg_ctx* ctx, int64_t npoin, const double* x,
+                    const double* y, const double* z, const int64_t nsup[3],
+                    const size_t* dsupedge0, const double* dsupint0,
+                    const size_t* dsupedge1, const double* dsupint1,
+                    const size_t* dsupedge2, const double* dsupint2,
+                    int64_t ntri, const size_t* triinpoel);
+
+/* In the chorin:: entry points `dev` = 1 means every field pointer is a
+ * device pointer (enqueued on the context stream), 0 host pointers (copied
+ * through device staging, synchronous). */
+
+/* chorin::div (Chorin.hpp, Chorin.cpp:82-209): D += div of U (ldu columns,
+ * 0..2 used); with stab the 4th-order pressure damping from P and its
+ * gradient G (ldg columns, 0..2 used) with time step dt. */
+int xg_chorin_div(xg_ctx* ctx, int dev, double dt, const double* P,
+                  const double* G, int ldg, const double* U, int ldu,
+                  double* D, int stab);
+
+/* chorin::grad (Chorin.cpp:336-449): G(:, 0..2) += gradient of scalar U. */
+int xg_chorin_grad(xg_ctx* ctx, int dev, const double* U, double* G, int ldg);
+
+/* chorin::vgrad (Chorin.cpp:211-334): G(npoin, 9) += velocity gradients of
+ * U(npoin, ncomp); ncomp = 3. */
+int xg_chorin_vgrad(xg_ctx* ctx, int dev, int ncomp, const double* U, double* G);
+
+/* chorin::flux (Chorin.cpp:451-638): F(npoin, 3) += momentum flux
+ * divergence from U (ldu columns) and velocity gradients G (ldg >= 9). */
+int xg_chorin_flux(xg_ctx* ctx, int dev, double mu, const double* U, int ldu,
+                   const double* G, int ldg, double* F);
+
+/* chorin::rhs (Chorin.cpp:1010-1044): R(npoin, ncomp) = advection rhs
+ * (damp2 / damp4) of U with pressure P and velocity gradients G (npoin, 9,
+ * damp4 only); no source term; ncomp = 3. */
+int xg_chorin_rhs(xg_ctx* ctx, int dev, const xg_chocfg* cfg, int ncomp,
+                  const double* P, const double* U, const double* G, double* R);
+
 /* ---- pressure-Poisson system (ChoCG) and conjugate gradients ---------------- */
 
 /* tk::CSR(ncomp, psup) (CSR.hpp:29-31, CSR.cpp:19-84): the sparsity pattern

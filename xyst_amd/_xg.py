@@ -43,6 +43,14 @@ SIG = {
     "xg_kozak_upload": (C.c_int, [VP, I64, VP, VP, VP, I64, VP]),
     "xg_kozak_rhs": (C.c_int, [VP, C.c_int, D, VP, VP]),
     "xg_kozak_rhs_dev": (C.c_int, [VP, C.c_int, D, VP, VP]),
+    "xg_chocg_upload": (C.c_int, [VP, I64, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP,
+                                  I64, VP]),
+    "xg_chorin_div": (C.c_int, [VP, C.c_int, D, VP, VP, C.c_int, VP, C.c_int, VP,
+                                C.c_int]),
+    "xg_chorin_grad": (C.c_int, [VP, C.c_int, VP, VP, C.c_int]),
+    "xg_chorin_vgrad": (C.c_int, [VP, C.c_int, C.c_int, VP, VP]),
+    "xg_chorin_flux": (C.c_int, [VP, C.c_int, D, VP, C.c_int, VP, C.c_int, VP]),
+    "xg_chorin_rhs": (C.c_int, [VP, C.c_int, VP, C.c_int, VP, VP, VP, VP]),
     "xg_csr_pattern": (C.c_int, [I64, I64, VP, C.c_int, I64, C.POINTER(I64), VP, VP]),
     "xg_poisson_matrix": (C.c_int, [I64, VP, VP, VP, I64, VP, VP, VP, VP]),
     "xg_cg_matrix": (C.c_int, [VP, C.c_int, I64, VP, VP, VP, C.c_int]),
@@ -67,6 +75,11 @@ class XgError(RuntimeError):
 class Config(C.Structure):
     _fields_ = [("gamma", C.c_double), ("flux", C.c_int), ("stab2", C.c_int),
                 ("stab2coef", C.c_double)]
+
+
+class ChoConfig(C.Structure):
+    _fields_ = [("flux", C.c_int), ("stab", C.c_int), ("stab2", C.c_int),
+                ("stab2coef", C.c_double), ("mu", C.c_double)]
 
 
 _LIB = None

@@ -1,0 +1,96 @@
+"""Python mirror of the chorin:: edge operators (Chorin.hpp:19-70) over the
+C-ABI.
+
+The mesh arguments (ChoCG superedges with 5 values per edge integral,
+coordinates, boundary triangles) are uploaded once per context with
+`upload`.  `div`, `grad`, `vgrad` and `flux` add to their output as the
+reference's do; `rhs` returns a new R.  Numpy arrays run through the host
+entry points; torch device tensors through the device ones (enqueued on the
+context stream).
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._xg import ChoConfig, lib, ptr
+
+__all__ = ["upload", "div", "grad", "vgrad", "flux", "rhs", "config"]
+
+FLUX = {"damp2": 0, "damp4": 1}
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _idx(a):
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def _dev(*arrs):
+    kinds = {not isinstance(a, np.ndarray) for a in arrs if a is not None}
+    if len(kinds) > 1:
+        raise ValueError("mix of host and device arrays")
+    return int(kinds.pop()) if kinds else 0
+
+
+def config(flux="damp2", stab=True, stab2=False, stab2coef=0.0, mu=0.0):
+    if flux not in FLUX:
+        raise ValueError("Flux not correctly configured")
+    return ChoConfig(FLUX[flux], int(bool(stab)), int(bool(stab2)), stab2coef, mu)
+
+
+def upload(ctx, coord, dsupedge, dsupint, triinpoel):
+    x, y, z = (_f64(coord[i]) for i in range(3))
+    s = [_idx(a) for a in dsupedge]
+    d = [_f64(a) for a in dsupint]
+    nsup = np.array([s[0].size // 4, s[1].size // 3, s[2].size // 2], np.int64)
+    tri = _idx(triinpoel)
+    ctx._check(lib().xg_chocg_upload(ctx._h, x.size, ptr(x), ptr(y), ptr(z), ptr(nsup),
+                                     ptr(s[0]), ptr(d[0]), ptr(s[1]), ptr(d[1]),
+                                     ptr(s[2]), ptr(d[2]), tri.size // 3, ptr(tri)))
+    ctx.cho_npoin = x.size
+
+
+def _cols(a):
+    return int(a.shape[1]) if a.ndim == 2 else 1
+
+
+def div(ctx, dt, P, G, U, D, stab=False):
+    """chorin::div: D += div U (+ pressure damping when stab)."""
+    dev = _dev(P if stab else None, G if stab else None, U, D)
+    ctx._check(lib().xg_chorin_div(ctx._h, dev, float(dt), ptr(P) if stab else None,
+                                   ptr(G) if stab else None, _cols(G) if stab else 3,
+                                   ptr(U), _cols(U), ptr(D), int(bool(stab))))
+    return D
+
+
+def grad(ctx, U, G):
+    """chorin::grad: G(:, 0..2) += gradient of the scalar U."""
+    ctx._check(lib().xg_chorin_grad(ctx._h, _dev(U, G), ptr(U), ptr(G), _cols(G)))
+    return G
+
+
+def vgrad(ctx, U, G):
+    """chorin::vgrad: G(npoin, 9) += gradients of U(npoin, 3)."""
+    ctx._check(lib().xg_chorin_vgrad(ctx._h, _dev(U, G), _cols(U), ptr(U), ptr(G)))
+    return G
+
+
+def flux(ctx, mu, U, G, F):
+    """chorin::flux: F(npoin, 3) += momentum flux divergence."""
+    ctx._check(lib().xg_chorin_flux(ctx._h, _dev(U, G, F), float(mu), ptr(U), _cols(U),
+                                    ptr(G), _cols(G), ptr(F)))
+    return F
+
+
+def rhs(ctx, cfg, P, U, G, R=None):
+    """chorin::rhs: R = advection rhs (R allocated for host inputs)."""
+    dev = _dev(P, U, G, R)
+    if R is None:
+        if dev:
+            raise ValueError("pass R for device inputs")
+        R = np.empty_like(_f64(U))
+    ctx._check(lib().xg_chorin_rhs(ctx._h, dev, C.byref(cfg), _cols(U), ptr(P), ptr(U),
+                                   ptr(G), ptr(R)))
+    return R

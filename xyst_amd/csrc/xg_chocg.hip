@@ -1,0 +1,727 @@
+// MI355X device path of the ChoCG edge operators: chorin::div, grad, vgrad,
+// flux and rhs (advection with damp2/damp4), Chorin.cpp:34-1044, for one
+// GPU-resident partition.
+//
+// All five are sums over mesh edges of an edge value scattered -/+ to the
+// edge's ends, plus a boundary-face term with (6,1,1)/8 corner weights.  They
+// share one node-block kernel (the layout of xg_layout.cpp: node blocks of
+// consecutive ids, edges/faces owned by the block of their smallest node):
+// the block's nodes are staged in LDS with the NV values the operator reads;
+// per chunk each thread computes one edge's (or one face corner's) NF values
+// into LDS; the thread owning local node t sums the chunk's slots touching it
+// in slot order; halo partials go to slots that k_fixup adds in block order.
+// No atomics: every result is bitwise reproducible.
+//
+// div/grad/vgrad/flux add to their output, as the reference's do (the caller
+// zeroes it); rhs overwrites R.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "xg_blocks.hpp"
+#include "xg_ctx.hpp"
+#include "xg_flux.hpp"
+#include "xg_layout.hpp"
+
+using namespace xgd;
+
+struct ChoMesh {
+  int npoin = 0;
+  int nblk = 0;
+  std::vector<DevArr> allocs;
+  double *x = nullptr, *y = nullptr, *z = nullptr;
+  int4 *blk0 = nullptr, *blk1 = nullptr;
+  int* halo = nullptr;
+  int4* chk = nullptr;
+  int* chk_ent = nullptr;
+  uint16_t *tend = nullptr, *entries = nullptr;
+  uint32_t* elpq = nullptr;
+  double *dx = nullptr, *dy = nullptr, *dz = nullptr, *dw = nullptr;
+  uint32_t *fab = nullptr, *fcs = nullptr;
+  double *fnx = nullptr, *fny = nullptr, *fnz = nullptr;
+  int nslot = 0;
+  double* slots = nullptr;     // halo partials, up to 9 per slot
+  int nfx = 0;
+  int *fx_node = nullptr, *fx_off = nullptr, *fx_slot = nullptr;
+  // staging of the host-pointer calls (grow only)
+  std::vector<DevArr> stage;
+};
+
+namespace {
+
+constexpr int kT = xg::kChunkSlots;
+constexpr int kCap = xg::kBlockCap;
+
+enum { OP_GRAD = 0, OP_VGRAD, OP_DIV, OP_DIVS, OP_FLUX, OP_ADV2, OP_ADV4 };
+
+template <int OP> struct OpShape;
+template <> struct OpShape<OP_GRAD>  { static constexpr int NV = 1,  NF = 3; };
+template <> struct OpShape<OP_VGRAD> { static constexpr int NV = 3,  NF = 9; };
+template <> struct OpShape<OP_DIV>   { static constexpr int NV = 3,  NF = 1; };
+template <> struct OpShape<OP_DIVS>  { static constexpr int NV = 10, NF = 1; };
+template <> struct OpShape<OP_FLUX>  { static constexpr int NV = 12, NF = 3; };
+template <> struct OpShape<OP_ADV2>  { static constexpr int NV = 4,  NF = 3; };
+template <> struct OpShape<OP_ADV4>  { static constexpr int NV = 16, NF = 3; };
+
+struct ChoArgs {
+  int nblk;
+  const int4 *blk0, *blk1;
+  const int* halo;
+  const int4* chk;
+  const int* chk_ent;
+  const uint16_t *tend, *entries;
+  const uint32_t* elpq;
+  const double *dx, *dy, *dz, *dw;
+  const uint32_t *fab, *fcs;
+  const double *fnx, *fny, *fnz;
+  const double *x, *y, *z;
+  const double* U;   // velocity (columns 0..2) or the scalar (GRAD)
+  int ldu;
+  const double* P;   // pressure
+  const double* G;   // pressure gradient (DIVS, 3 of ldg) or velocity gradients (9 of ldg)
+  int ldg;
+  double* out;
+  double* S;
+  int accum;
+  double dt, mu, stab2coef;
+  int stab, stab2;
+};
+
+// staged value c of global node g; layout per operator:
+//   GRAD: u | VGRAD, DIV: u0 u1 u2 | DIVS: u0..2 P G0..2 x y z
+//   FLUX: u0..2 G0..8 | ADV2: u0..2 P | ADV4: u0..2 P G0..8 x y z
+template <int OP>
+__device__ inline double node_value(const ChoArgs& a, int g, int c) {
+  if (OP == OP_GRAD || OP == OP_VGRAD || OP == OP_DIV) return a.U[(size_t)g * a.ldu + c];
+  if (c < 3) return a.U[(size_t)g * a.ldu + c];
+  if (OP == OP_FLUX) return a.G[(size_t)g * a.ldg + (c - 3)];
+  if (c == 3) return a.P[g];
+  if (OP == OP_ADV2) return 0.0;
+  if (OP == OP_DIVS) {
+    if (c < 7) return a.G[(size_t)g * a.ldg + (c - 4)];
+    return c == 7 ? a.x[g] : c == 8 ? a.y[g] : a.z[g];
+  }
+  // ADV4
+  if (c < 13) return a.G[(size_t)g * a.ldg + (c - 4)];
+  return c == 13 ? a.x[g] : c == 14 ? a.y[g] : a.z[g];
+}
+
+// momentum flux of chorin::flux for edge nodes (P, Q) (Chorin.cpp:451-479),
+// from staged u (0..2) and gradients (3..11)
+__device__ inline double mflux(const double* sN, int P, int Q, int i, int j, double mu) {
+  const double inv = sN[i * kCap + P] * sN[j * kCap + P] + sN[i * kCap + Q] * sN[j * kCap + Q];
+  if (mu < 2.220446049250313e-16) return -inv;
+  double vis = sN[(3 + i * 3 + j) * kCap + P] + sN[(3 + j * 3 + i) * kCap + P] +
+               sN[(3 + i * 3 + j) * kCap + Q] + sN[(3 + j * 3 + i) * kCap + Q];
+  if (i == j)
+    vis -= 2.0 / 3.0 * (sN[3 * kCap + P] + sN[7 * kCap + P] + sN[11 * kCap + P] +
+                        sN[3 * kCap + Q] + sN[7 * kCap + Q] + sN[11 * kCap + Q]);
+  return mu * vis - inv;
+}
+
+__device__ inline double mflux_point(const double* sN, int P, int i, int j, double mu) {
+  const double inv = sN[i * kCap + P] * sN[j * kCap + P];
+  if (mu < 2.220446049250313e-16) return -inv;
+  double vis = sN[(3 + i * 3 + j) * kCap + P] + sN[(3 + j * 3 + i) * kCap + P];
+  if (i == j) vis -= 2.0 / 3.0 * (sN[3 * kCap + P] + sN[7 * kCap + P] + sN[11 * kCap + P]);
+  return mu * vis - inv;
+}
+
+// advective flux of adv_damp2/adv_damp4 (Chorin.cpp:640-829) on edge (p, q)
+template <int OP>
+__device__ inline void adv_edge(const ChoArgs& a, const double* sN, int p, int q,
+                                const double d[3], double dwv, double f[3]) {
+  double uL[3], uR[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    uL[c] = sN[c * kCap + p];
+    uR[c] = sN[c * kCap + q];
+  }
+  double vdL[3];   // unreconstructed differences for the viscous term
+#pragma unroll
+  for (int c = 0; c < 3; ++c) vdL[c] = uR[c] - uL[c];
+  if (OP == OP_ADV4) {
+    const double vx = sN[13 * kCap + q] - sN[13 * kCap + p];
+    const double vy = sN[14 * kCap + q] - sN[14 * kCap + p];
+    const double vz = sN[15 * kCap + q] - sN[15 * kCap + p];
+    constexpr double eps = 1.0e-9, c1 = 1.0 - 1.0 / 3.0, c2 = 1.0 + 1.0 / 3.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double g1 = sN[(4 + c * 3) * kCap + p] * vx + sN[(5 + c * 3) * kCap + p] * vy +
+                        sN[(6 + c * 3) * kCap + p] * vz;
+      const double g2 = sN[(4 + c * 3) * kCap + q] * vx + sN[(5 + c * 3) * kCap + q] * vy +
+                        sN[(6 + c * 3) * kCap + q] * vz;
+      const double delta2 = uR[c] - uL[c];
+      const double delta1 = 2.0 * g1 - delta2;
+      const double delta3 = 2.0 * g2 - delta2;
+      // van Leer: phi(r) = (|r|+r)/(|r|+1) with r = s/t is 2s/(s+t) when s
+      // and t have the same sign and 0 otherwise; phi(1/r) likewise
+      const double A = delta2 + eps, BL = delta1 + eps, BR = delta3 + eps;
+      const double il = xg::same_sign(A, BL) ? xg::rcp(A + BL) : 0.0;
+      const double ir = xg::same_sign(A, BR) ? xg::rcp(A + BR) : 0.0;
+      uL[c] += 0.5 * il * (delta1 * c1 * A + delta2 * c2 * BL);
+      uR[c] -= 0.5 * ir * (delta3 * c1 * A + delta2 * c2 * BR);
+    }
+  }
+  const double vnL = uL[0] * d[0] + uL[1] * d[1] + uL[2] * d[2];
+  const double vnR = uR[0] * d[0] + uR[1] * d[1] + uR[2] * d[2];
+  double aw = 0.0;
+  if (a.stab) aw = fabs(vnL + vnR) * 0.5;
+  if (a.stab2) aw += a.stab2coef * fmax(fabs(vnL), fabs(vnR));
+  const double v = dwv * a.mu;
+  const double pf = sN[3 * kCap + p] + sN[3 * kCap + q];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (OP == OP_ADV2)
+      f[c] = uL[c] * vnL + uR[c] * vnR + pf * d[c] + (aw - v) * (uR[c] - uL[c]);
+    else
+      f[c] = uL[c] * vnL + uR[c] * vnR + pf * d[c] + aw * (uR[c] - uL[c]) - v * vdL[c];
+  }
+}
+
+template <int OP>
+__device__ inline void edge_values(const ChoArgs& a, const double* sN, int p, int q,
+                                   const double d[3], double dwv,
+                                   double f[OpShape<OP>::NF]) {
+  if (OP == OP_GRAD) {
+    const double s = sN[q] + sN[p];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) f[j] = d[j] * s;
+  } else if (OP == OP_VGRAD) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double s = sN[i * kCap + q] + sN[i * kCap + p];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) f[i * 3 + j] = d[j] * s;
+    }
+  } else if (OP == OP_DIV || OP == OP_DIVS) {
+    double dv = d[0] * (sN[p] + sN[q]) + d[1] * (sN[kCap + p] + sN[kCap + q]) +
+                d[2] * (sN[2 * kCap + p] + sN[2 * kCap + q]);
+    if (OP == OP_DIVS) {
+      const double ex = sN[7 * kCap + p] - sN[7 * kCap + q];
+      const double ey = sN[8 * kCap + p] - sN[8 * kCap + q];
+      const double ez = sN[9 * kCap + p] - sN[9 * kCap + q];
+      const double dl = sqrt(ex * ex + ey * ey + ez * ez);
+      const double p2 = sN[3 * kCap + q] - sN[3 * kCap + p];
+      const double D = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      const double p4 = 0.5 * (ex * (sN[4 * kCap + p] + sN[4 * kCap + q]) +
+                               ey * (sN[5 * kCap + p] + sN[5 * kCap + q]) +
+                               ez * (sN[6 * kCap + p] + sN[6 * kCap + q]));
+      dv += D * a.dt * xg::rcp(dl) * (p2 + p4);
+    }
+    f[0] = dv;
+  } else if (OP == OP_FLUX) {
+    // the reference evaluates flux(U,G,i,j,N[1],N[0]) for edge N[0]-N[1]
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) s += d[j] * mflux(sN, q, p, i, j, a.mu);
+      f[i] = s;
+    }
+  } else {
+    adv_edge<OP>(a, sN, p, q, d, dwv, f);
+  }
+}
+
+// corner `k` (weight 6 on it, 1 on the others, /8) of boundary face (A,B,C)
+template <int OP>
+__device__ inline void face_values(const ChoArgs& a, const double* sN, const int N[3],
+                                   int k, const double n[3],
+                                   double f[OpShape<OP>::NF]) {
+  const double w0 = k == 0 ? 6.0 : 1.0, w1 = k == 1 ? 6.0 : 1.0, w2 = k == 2 ? 6.0 : 1.0;
+  if (OP == OP_GRAD) {
+    const double u = (w0 * sN[N[0]] + w1 * sN[N[1]] + w2 * sN[N[2]]) / 8.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) f[j] = u * n[j];
+  } else if (OP == OP_VGRAD) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double u = (w0 * sN[i * kCap + N[0]] + w1 * sN[i * kCap + N[1]] +
+                        w2 * sN[i * kCap + N[2]]) / 8.0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) f[i * 3 + j] = u * n[j];
+    }
+  } else if (OP == OP_DIV || OP == OP_DIVS) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      s += (w0 * sN[j * kCap + N[0]] + w1 * sN[j * kCap + N[1]] + w2 * sN[j * kCap + N[2]]) /
+           8.0 * n[j];
+    f[0] = s;
+  } else if (OP == OP_FLUX) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        s += (w0 * mflux_point(sN, N[0], i, j, a.mu) + w1 * mflux_point(sN, N[1], i, j, a.mu) +
+              w2 * mflux_point(sN, N[2], i, j, a.mu)) / 8.0 * n[j];
+      f[i] = s;
+    }
+  } else {
+    double fn[3][3];   // [node][component]
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int P = N[m];
+      const double u = sN[P], v = sN[kCap + P], w = sN[2 * kCap + P], pr = sN[3 * kCap + P];
+      const double vn = n[0] * u + n[1] * v + n[2] * w;
+      fn[m][0] = u * vn + pr * n[0];
+      fn[m][1] = v * vn + pr * n[1];
+      fn[m][2] = w * vn + pr * n[2];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[c] = (w0 * fn[0][c] + w1 * fn[1][c] + w2 * fn[2][c]) / 8.0;
+  }
+}
+
+template <int OP>
+__global__ __launch_bounds__(kT) void k_cho_blocks(ChoArgs a) {
+  constexpr int NV = OpShape<OP>::NV, NF = OpShape<OP>::NF;
+  __shared__ double sN[NV * kCap];
+  __shared__ double F[kT * NF];
+  __shared__ uint16_t sE[2 * kT];
+  __shared__ uint16_t sT[kCap];
+  const int t = threadIdx.x;
+  const int b = xcd_block(blockIdx.x, a.nblk);
+  const int4 B0 = a.blk0[b];   // own0, nown, nloc, halo_off
+  const int4 B1 = a.blk1[b];   // slot0, chunk0, chunk1
+  const int own0 = B0.x, nown = B0.y, nloc = B0.z;
+  // stage (node, value) pairs over all threads, 4 loads in flight each
+  for (int base = 0; base < nloc * NV; base += 4 * kT) {
+    double v[4];
+    int dst[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k = base + t + r * kT;
+      dst[r] = -1;
+      v[r] = 0.0;
+      if (k < nloc * NV) {
+        const int i = k / NV, c = k - NV * i;
+        const int g = i < nown ? own0 + i : a.halo[B0.w + i - nown];
+        v[r] = node_value<OP>(a, g, c);
+        dst[r] = c * kCap + i;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (dst[r] >= 0) sN[dst[r]] = v[r];
+  }
+  double acc[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) acc[k] = 0.0;
+  for (int ch = B1.y; ch < B1.z; ++ch) {
+    const int4 C = a.chk[ch];   // kind, first, count, tend_off
+    const int eoff = a.chk_ent[ch];
+    const int nent = C.z * (C.x == 0 ? 2 : 3);
+    uint16_t e0 = 0, e1 = 0, te = 0;
+    if (t < nent) e0 = a.entries[eoff + t];
+    if (t + kT < nent) e1 = a.entries[eoff + kT + t];
+    if (t < nloc) te = a.tend[C.w + t];
+    __syncthreads();          // staging done / previous chunk consumed
+    if (t < nent) sE[t] = e0;
+    if (t + kT < nent) sE[kT + t] = e1;
+    if (t < nloc) sT[t] = te;
+    if (t < C.z) {
+      if (C.x == 0) {
+        const int e = C.y + t;
+        const uint32_t pq = a.elpq[e];
+        const double d[3] = {a.dx[e], a.dy[e], a.dz[e]};
+        const double dwv = (OP == OP_ADV2 || OP == OP_ADV4) ? a.dw[e] : 0.0;
+        double f[NF];
+        edge_values<OP>(a, sN, (int)(pq & 0xffff), (int)(pq >> 16), d, dwv, f);
+#pragma unroll
+        for (int k = 0; k < NF; ++k) F[t * NF + k] = f[k];
+      } else {
+        const int fi = C.y + t;
+        const uint32_t ab = a.fab[fi], cs = a.fcs[fi];
+        const int N[3] = {(int)(ab & 0xffff), (int)(ab >> 16), (int)(cs & 0xffff)};
+        const double n[3] = {a.fnx[fi], a.fny[fi], a.fnz[fi]};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          double f[NF];
+          face_values<OP>(a, sN, N, k, n, f);
+#pragma unroll
+          for (int m = 0; m < NF; ++m) F[(3 * t + k) * NF + m] = f[m];
+        }
+      }
+    }
+    __syncthreads();          // F, sE, sT complete
+    if (t < nloc) {
+      const int j0 = t ? sT[t - 1] : 0, j1 = sT[t];
+      for (int j = j0; j < j1; ++j) {
+        const int v = sE[j];
+        const double* fs = F + (v & (kT - 1)) * NF;
+        const double sg = (v & 0x8000) ? -1.0 : 1.0;
+#pragma unroll
+        for (int k = 0; k < NF; ++k) acc[k] = fma(sg, fs[k], acc[k]);
+      }
+    }
+  }
+  if (t < nown) {
+    double* o = a.out + (size_t)(own0 + t) * NF;
+#pragma unroll
+    for (int k = 0; k < NF; ++k) o[k] = a.accum ? o[k] + acc[k] : acc[k];
+  } else if (t < nloc) {
+    double* o = a.S + (size_t)(B1.x + t - nown) * NF;
+#pragma unroll
+    for (int k = 0; k < NF; ++k) o[k] = acc[k];
+  }
+}
+
+// Adds halo partials to an output with ldo columns of which NF are written.
+template <int NF>
+__global__ __launch_bounds__(kBlock) void k_fixup_ld(
+    int nfx, const int* __restrict__ fx_node, const int* __restrict__ fx_off,
+    const int* __restrict__ fx_slot, const double* __restrict__ S,
+    double* __restrict__ R, int ldo) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nfx) return;
+  double* r = R + (size_t)fx_node[i] * ldo;
+  double acc[NF];
+#pragma unroll
+  for (int c = 0; c < NF; ++c) acc[c] = r[c];
+  for (int k = fx_off[i]; k < fx_off[i + 1]; ++k) {
+    const double* s = S + (size_t)fx_slot[k] * NF;
+#pragma unroll
+    for (int c = 0; c < NF; ++c) acc[c] += s[c];
+  }
+#pragma unroll
+  for (int c = 0; c < NF; ++c) r[c] = acc[c];
+}
+
+// Writes NF-wide node sums from a dense (npoin, NF) buffer into an output of
+// ldo columns (outputs whose leading dimension is wider than NF).
+__global__ void k_scatter_cols(int npoin, int nf, const double* __restrict__ src,
+                               double* __restrict__ dst, int ldo, int accum) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)npoin * nf) return;
+  const int64_t n = i / nf, c = i - n * nf;
+  dst[n * ldo + c] = accum ? dst[n * ldo + c] + src[i] : src[i];
+}
+
+void free_cho(ChoMesh* m) {
+  if (!m) return;
+  dev_free_all(m->allocs);
+  dev_free_all(m->stage);
+  delete m;
+}
+
+ChoArgs base_args(const ChoMesh* m) {
+  ChoArgs a{};
+  a.nblk = m->nblk;
+  a.blk0 = m->blk0; a.blk1 = m->blk1; a.halo = m->halo;
+  a.chk = m->chk; a.chk_ent = m->chk_ent; a.tend = m->tend; a.entries = m->entries;
+  a.elpq = m->elpq; a.dx = m->dx; a.dy = m->dy; a.dz = m->dz; a.dw = m->dw;
+  a.fab = m->fab; a.fcs = m->fcs; a.fnx = m->fnx; a.fny = m->fny; a.fnz = m->fnz;
+  a.x = m->x; a.y = m->y; a.z = m->z;
+  a.S = m->slots;
+  return a;
+}
+
+// Runs operator OP into `out` (leading dimension ldo >= NF).  Outputs with
+// ldo == NF are written in place; wider ones go through a dense temporary.
+template <int OP>
+int run_op(xg_ctx* c, ChoArgs a, double* out, int ldo, int accum, double* tmp) {
+  constexpr int NF = OpShape<OP>::NF;
+  ChoMesh* m = c->cho;
+  const bool dense = ldo == NF;
+  a.out = dense ? out : tmp;
+  a.accum = dense ? accum : 0;
+  if (m->nblk)
+    hipLaunchKernelGGL(k_cho_blocks<OP>, dim3(m->nblk), dim3(kT), 0, c->stream, a);
+  if (m->nfx)
+    hipLaunchKernelGGL(k_fixup_ld<NF>, dim3(nblocks(m->nfx)), dim3(kBlock), 0, c->stream,
+                       m->nfx, m->fx_node, m->fx_off, m->fx_slot, m->slots, a.out, NF);
+  if (!dense)
+    hipLaunchKernelGGL(k_scatter_cols, dim3(nblocks((int64_t)m->npoin * NF)), dim3(kBlock),
+                       0, c->stream, m->npoin, NF, tmp, out, ldo, accum);
+  XG_HIP(c, hipGetLastError());
+  return XG_OK;
+}
+
+int check_cho(xg_ctx* c) {
+  if (!c) return XG_EINVAL;
+  if (!c->cho) return fail(c, XG_ENOMESH, "no ChoCG mesh (xg_chocg_upload)");
+  return XG_OK;
+}
+
+// Device buffer k of the host-pointer entry points (grow only).
+int stage_buf(xg_ctx* c, int k, size_t bytes, double** p) {
+  ChoMesh* m = c->cho;
+  if ((int)m->stage.size() <= k) m->stage.resize(k + 1);
+  DevArr& d = m->stage[k];
+  if (d.bytes < bytes) {
+    if (d.p) (void)hipFree(d.p);
+    d.p = nullptr;
+    d.bytes = 0;
+    if (hipMalloc(&d.p, bytes) != hipSuccess) return fail(c, XG_EHIP, "hipMalloc (staging)");
+    d.bytes = bytes;
+  }
+  *p = (double*)d.p;
+  return XG_OK;
+}
+
+// in: host or device input -> device pointer (copied when host)
+int dev_in(xg_ctx* c, int dev, int k, const double* src, size_t n, const double** out) {
+  if (dev || !src) { *out = src; return XG_OK; }
+  double* d;
+  int rc = stage_buf(c, k, n * 8, &d);
+  if (rc) return rc;
+  XG_HIP(c, hipMemcpyAsync(d, src, n * 8, hipMemcpyHostToDevice, c->stream));
+  *out = d;
+  return XG_OK;
+}
+
+}  // namespace
+
+void xg_chocg_free(xg_ctx* c) {
+  free_cho(c->cho);
+  c->cho = nullptr;
+}
+
+extern "C" {
+
+int xg_chocg_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
+                    const double* z, const int64_t nsup[3], const size_t* s0,
+                    const double* i0, const size_t* s1, const double* i1,
+                    const size_t* s2, const double* i2, int64_t ntri,
+                    const size_t* tri) {
+  if (!c) return XG_EINVAL;
+  if (npoin <= 0 || npoin >= (1ll << 31) || !x || !y || !z || !nsup)
+    return fail(c, XG_EINVAL, "xg_chocg_upload: bad mesh size or coordinates");
+  if (nsup[0] < 0 || nsup[1] < 0 || nsup[2] < 0 || ntri < 0 ||
+      (nsup[0] && (!s0 || !i0)) || (nsup[1] && (!s1 || !i1)) ||
+      (nsup[2] && (!s2 || !i2)) || (ntri && !tri))
+    return fail(c, XG_EINVAL, "xg_chocg_upload: missing superedge/face arrays");
+  XG_HIP(c, hipSetDevice(c->device));
+  static const int lpoed[6][2] = {{0,1},{1,2},{2,0},{0,3},{1,3},{2,3}};
+  static const int lpoet[3][2] = {{0,1},{1,2},{2,0}};
+  const int64_t ne = nsup[0] * 6 + nsup[1] * 3 + nsup[2];
+  if (ne >= (1ll << 31)) return fail(c, XG_EINVAL, "xg_chocg_upload: too many edges");
+  std::vector<int> p(ne), q(ne);
+  std::vector<double> d(ne * 3), w(ne);
+  int64_t k = 0;
+  auto push = [&](size_t a, size_t b, const double* D) {
+    p[k] = (int)std::min<size_t>(a, (size_t)INT32_MAX);
+    q[k] = (int)std::min<size_t>(b, (size_t)INT32_MAX);
+    d[k * 3 + 0] = D[0]; d[k * 3 + 1] = D[1]; d[k * 3 + 2] = D[2];
+    w[k] = D[4];
+    ++k;
+  };
+  for (int64_t e = 0; e < nsup[0]; ++e)
+    for (int j = 0; j < 6; ++j)
+      push(s0[e * 4 + lpoed[j][0]], s0[e * 4 + lpoed[j][1]], i0 + (e * 6 + j) * 5);
+  for (int64_t e = 0; e < nsup[1]; ++e)
+    for (int j = 0; j < 3; ++j)
+      push(s1[e * 3 + lpoet[j][0]], s1[e * 3 + lpoet[j][1]], i1 + (e * 3 + j) * 5);
+  for (int64_t e = 0; e < nsup[2]; ++e) push(s2[e * 2], s2[e * 2 + 1], i2 + e * 5);
+  for (int64_t e = 0; e < ne; ++e)
+    if (p[e] < 0 || q[e] < 0 || p[e] >= npoin || q[e] >= npoin || p[e] == q[e])
+      return fail(c, XG_EINVAL, "xg_chocg_upload: superedge node id out of range");
+  for (int64_t f = 0; f < ntri * 3; ++f)
+    if (tri[f] >= (size_t)npoin)
+      return fail(c, XG_EINVAL, "xg_chocg_upload: boundary face node out of range");
+  std::vector<int> tri32(ntri * 3);
+  for (int64_t f = 0; f < ntri * 3; ++f) tri32[f] = (int)tri[f];
+  std::vector<uint8_t> nosym(ntri * 3 + 1, 0);
+  xg::Layout L;
+  int rc = xg::build_layout((int)npoin, p, q, d, ntri, tri32.data(), nosym.data(),
+                            xg::kBlockNodes, xg::kBlockCap, L, c->err);
+  if (rc) return rc;
+  std::vector<double> dwl(L.eidx.size());
+  for (size_t e = 0; e < L.eidx.size(); ++e) dwl[e] = w[L.eidx[e]];
+  // boundary normals n = crossdiv(b - a, c - a, 6) (Vector.hpp:68-77)
+  std::vector<double> fnx(L.fidx.size()), fny(L.fidx.size()), fnz(L.fidx.size());
+  for (size_t i = 0; i < L.fidx.size(); ++i) {
+    const size_t* N = tri + L.fidx[i] * 3;
+    const double ax = x[N[1]] - x[N[0]], ay = y[N[1]] - y[N[0]], az = z[N[1]] - z[N[0]];
+    const double bx = x[N[2]] - x[N[0]], by = y[N[2]] - y[N[0]], bz = z[N[2]] - z[N[0]];
+    fnx[i] = (ay * bz - by * az) / 6.0;
+    fny[i] = (az * bx - bz * ax) / 6.0;
+    fnz[i] = (ax * by - bx * ay) / 6.0;
+  }
+  std::vector<int4> b0(L.blk.size()), b1(L.blk.size());
+  for (size_t b = 0; b < L.blk.size(); ++b) {
+    const auto& B = L.blk[b];
+    b0[b] = make_int4(B.own0, B.nown, B.nloc, B.halo_off);
+    b1[b] = make_int4(B.slot0, B.chunk0, B.chunk1, 0);
+  }
+  std::vector<int4> c0(L.chunks.size());
+  std::vector<int> c1(L.chunks.size());
+  for (size_t i = 0; i < L.chunks.size(); ++i) {
+    const auto& C = L.chunks[i];
+    c0[i] = make_int4(C.kind, C.first, C.count, C.tend_off);
+    c1[i] = C.ent_off;
+  }
+  auto* m = new ChoMesh;
+  m->npoin = (int)npoin;
+  m->nblk = (int)L.blk.size();
+  m->nslot = L.nslot;
+  m->nfx = (int)L.fx_node.size();
+  auto& M = m->allocs;
+  if ((rc = upload(c, M, &m->x, x, npoin)) || (rc = upload(c, M, &m->y, y, npoin)) ||
+      (rc = upload(c, M, &m->z, z, npoin)) || (rc = upload(c, M, &m->elpq, L.elpq)) ||
+      (rc = upload(c, M, &m->dx, L.dx)) || (rc = upload(c, M, &m->dy, L.dy)) ||
+      (rc = upload(c, M, &m->dz, L.dz)) || (rc = upload(c, M, &m->dw, dwl)) ||
+      (rc = upload(c, M, &m->blk0, b0)) || (rc = upload(c, M, &m->blk1, b1)) ||
+      (rc = upload(c, M, &m->halo, L.halo)) || (rc = upload(c, M, &m->chk, c0)) ||
+      (rc = upload(c, M, &m->chk_ent, c1)) || (rc = upload(c, M, &m->tend, L.tend)) ||
+      (rc = upload(c, M, &m->entries, L.entries)) || (rc = upload(c, M, &m->fab, L.fab)) ||
+      (rc = upload(c, M, &m->fcs, L.fcs)) || (rc = upload(c, M, &m->fnx, fnx)) ||
+      (rc = upload(c, M, &m->fny, fny)) || (rc = upload(c, M, &m->fnz, fnz)) ||
+      (rc = dev_alloc(c, M, (void**)&m->slots, (size_t)L.nslot * 9 * sizeof(double))) ||
+      (rc = upload(c, M, &m->fx_node, L.fx_node)) ||
+      (rc = upload(c, M, &m->fx_off, L.fx_off)) ||
+      (rc = upload(c, M, &m->fx_slot, L.fx_slot))) {
+    free_cho(m);
+    return rc;
+  }
+  free_cho(c->cho);
+  c->cho = m;
+  return XG_OK;
+}
+
+int xg_chorin_div(xg_ctx* c, int dev, double dt, const double* P, const double* G,
+                  int ldg, const double* U, int ldu, double* D, int stab) {
+  int rc = check_cho(c);
+  if (rc) return rc;
+  if (!U || !D || ldu < 3 || (stab && (!P || !G || ldg < 3)))
+    return fail(c, XG_EINVAL, "xg_chorin_div: bad arguments");
+  XG_HIP(c, hipSetDevice(c->device));
+  ChoMesh* m = c->cho;
+  const size_t n = (size_t)m->npoin;
+  ChoArgs a = base_args(m);
+  a.ldu = ldu; a.ldg = ldg; a.dt = dt;
+  if ((rc = dev_in(c, dev, 0, U, n * ldu, &a.U)) ||
+      (stab && (rc = dev_in(c, dev, 1, P, n, &a.P))) ||
+      (stab && (rc = dev_in(c, dev, 2, G, n * ldg, &a.G))))
+    return rc;
+  double* out = D;
+  if (!dev) {
+    double* d;
+    if ((rc = stage_buf(c, 3, n * 8, &d))) return rc;
+    XG_HIP(c, hipMemcpyAsync(d, D, n * 8, hipMemcpyHostToDevice, c->stream));
+    out = d;
+  }
+  rc = stab ? run_op<OP_DIVS>(c, a, out, 1, 1, nullptr) : run_op<OP_DIV>(c, a, out, 1, 1, nullptr);
+  if (rc) return rc;
+  if (!dev) {
+    XG_HIP(c, hipMemcpyAsync(D, out, n * 8, hipMemcpyDeviceToHost, c->stream));
+    XG_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return XG_OK;
+}
+
+int xg_chorin_grad(xg_ctx* c, int dev, const double* U, double* G, int ldg) {
+  int rc = check_cho(c);
+  if (rc) return rc;
+  if (!U || !G || ldg < 3) return fail(c, XG_EINVAL, "xg_chorin_grad: bad arguments");
+  XG_HIP(c, hipSetDevice(c->device));
+  ChoMesh* m = c->cho;
+  const size_t n = (size_t)m->npoin;
+  ChoArgs a = base_args(m);
+  a.ldu = 1;
+  if ((rc = dev_in(c, dev, 0, U, n, &a.U))) return rc;
+  double* out = G;
+  if (!dev) {
+    if ((rc = stage_buf(c, 3, n * ldg * 8, &out))) return rc;
+    XG_HIP(c, hipMemcpyAsync(out, G, n * ldg * 8, hipMemcpyHostToDevice, c->stream));
+  }
+  double* tmp = nullptr;
+  if (ldg != 3 && (rc = stage_buf(c, 4, n * 3 * 8, &tmp))) return rc;
+  if ((rc = run_op<OP_GRAD>(c, a, out, ldg, 1, tmp))) return rc;
+  if (!dev) {
+    XG_HIP(c, hipMemcpyAsync(G, out, n * ldg * 8, hipMemcpyDeviceToHost, c->stream));
+    XG_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return XG_OK;
+}
+
+int xg_chorin_vgrad(xg_ctx* c, int dev, int ncomp, const double* U, double* G) {
+  int rc = check_cho(c);
+  if (rc) return rc;
+  if (ncomp != 3)
+    return fail(c, XG_EUNSUPPORTED, "chorin::vgrad on the device: ncomp must be 3");
+  if (!U || !G) return fail(c, XG_EINVAL, "xg_chorin_vgrad: null field");
+  XG_HIP(c, hipSetDevice(c->device));
+  ChoMesh* m = c->cho;
+  const size_t n = (size_t)m->npoin;
+  ChoArgs a = base_args(m);
+  a.ldu = 3;
+  if ((rc = dev_in(c, dev, 0, U, n * 3, &a.U))) return rc;
+  double* out = G;
+  if (!dev) {
+    if ((rc = stage_buf(c, 3, n * 9 * 8, &out))) return rc;
+    XG_HIP(c, hipMemcpyAsync(out, G, n * 9 * 8, hipMemcpyHostToDevice, c->stream));
+  }
+  if ((rc = run_op<OP_VGRAD>(c, a, out, 9, 1, nullptr))) return rc;
+  if (!dev) {
+    XG_HIP(c, hipMemcpyAsync(G, out, n * 9 * 8, hipMemcpyDeviceToHost, c->stream));
+    XG_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return XG_OK;
+}
+
+int xg_chorin_flux(xg_ctx* c, int dev, double mu, const double* U, int ldu,
+                   const double* G, int ldg, double* F) {
+  int rc = check_cho(c);
+  if (rc) return rc;
+  if (!U || !G || !F || ldu < 3 || ldg < 9)
+    return fail(c, XG_EINVAL, "xg_chorin_flux: bad arguments");
+  XG_HIP(c, hipSetDevice(c->device));
+  ChoMesh* m = c->cho;
+  const size_t n = (size_t)m->npoin;
+  ChoArgs a = base_args(m);
+  a.ldu = ldu; a.ldg = ldg; a.mu = mu;
+  if ((rc = dev_in(c, dev, 0, U, n * ldu, &a.U)) || (rc = dev_in(c, dev, 2, G, n * ldg, &a.G)))
+    return rc;
+  double* out = F;
+  if (!dev) {
+    if ((rc = stage_buf(c, 3, n * 3 * 8, &out))) return rc;
+    XG_HIP(c, hipMemcpyAsync(out, F, n * 3 * 8, hipMemcpyHostToDevice, c->stream));
+  }
+  if ((rc = run_op<OP_FLUX>(c, a, out, 3, 1, nullptr))) return rc;
+  if (!dev) {
+    XG_HIP(c, hipMemcpyAsync(F, out, n * 3 * 8, hipMemcpyDeviceToHost, c->stream));
+    XG_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return XG_OK;
+}
+
+int xg_chorin_rhs(xg_ctx* c, int dev, const xg_chocfg* cfg, int ncomp,
+                  const double* P, const double* U, const double* G, double* R) {
+  int rc = check_cho(c);
+  if (rc) return rc;
+  if (!cfg) return fail(c, XG_EINVAL, "xg_chorin_rhs: null config");
+  if (cfg->flux != 0 && cfg->flux != 1) return fail(c, XG_EINVAL, "Flux not correctly configured");
+  if (ncomp != 3)
+    return fail(c, XG_EUNSUPPORTED, "chorin::rhs on the device: ncomp must be 3");
+  if (!P || !U || !R || (cfg->flux == 1 && !G))
+    return fail(c, XG_EINVAL, "xg_chorin_rhs: null field");
+  XG_HIP(c, hipSetDevice(c->device));
+  ChoMesh* m = c->cho;
+  const size_t n = (size_t)m->npoin;
+  ChoArgs a = base_args(m);
+  a.ldu = 3; a.ldg = 9; a.mu = cfg->mu; a.stab = cfg->stab; a.stab2 = cfg->stab2;
+  a.stab2coef = cfg->stab2coef;
+  if ((rc = dev_in(c, dev, 0, U, n * 3, &a.U)) || (rc = dev_in(c, dev, 1, P, n, &a.P)) ||
+      (cfg->flux == 1 && (rc = dev_in(c, dev, 2, G, n * 9, &a.G))))
+    return rc;
+  double* out = R;
+  if (!dev && (rc = stage_buf(c, 3, n * 3 * 8, &out))) return rc;
+  rc = cfg->flux == 0 ? run_op<OP_ADV2>(c, a, out, 3, 0, nullptr)
+                      : run_op<OP_ADV4>(c, a, out, 3, 0, nullptr);
+  if (rc) return rc;
+  if (!dev) {
+    XG_HIP(c, hipMemcpyAsync(R, out, n * 3 * 8, hipMemcpyDeviceToHost, c->stream));
+    XG_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return XG_OK;
+}
+
+}  // extern "C"

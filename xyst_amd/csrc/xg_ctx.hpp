@@ -15,6 +15,7 @@ struct DevArr {
 
 struct CgSolver;   // xg_cg.hip
 struct KozakMesh;  // xg_kozak.hip
+struct ChoMesh;    // xg_chocg.hip
 
 struct xg_ctx {
   int device = 0;
@@ -61,9 +62,12 @@ struct xg_ctx {
   CgSolver* cg = nullptr;
   // KozCG tet layout (xg_kozak.hip)
   KozakMesh* koz = nullptr;
+  // ChoCG edge layout (xg_chocg.hip)
+  ChoMesh* cho = nullptr;
 };
 
 void xg_kozak_free(xg_ctx* c);   // xg_kozak.hip
+void xg_chocg_free(xg_ctx* c);   // xg_chocg.hip
 
 namespace xgd {
 

@@ -182,6 +182,7 @@ int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q
         L.elpq.push_back((uint32_t)local[p[e]] | ((uint32_t)local[q[e]] << 16));
         L.ep.push_back(p[e]);
         L.eq.push_back(q[e]);
+        L.eidx.push_back(e);
         L.dx.push_back(d[e * 3 + 0]);
         L.dy.push_back(d[e * 3 + 1]);
         L.dz.push_back(d[e * 3 + 2]);
@@ -205,6 +206,7 @@ int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q
         L.fab.push_back((uint32_t)local[tri[f * 3]] | ((uint32_t)local[tri[f * 3 + 1]] << 16));
         L.fcs.push_back((uint32_t)local[tri[f * 3 + 2]] | (sym << 16));
         for (int a = 0; a < 3; ++a) L.fglob.push_back(tri[f * 3 + a]);
+        L.fidx.push_back(f);
       }
     }
     B.chunk1 = (int)L.chunks.size();

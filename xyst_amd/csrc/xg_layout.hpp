@@ -39,6 +39,8 @@ struct Layout {
   std::vector<uint32_t> elpq;      // local p | local q << 16
   std::vector<double> dx, dy, dz;
   std::vector<int> ep, eq;         // global end nodes (grad stream, checks)
+  std::vector<int64_t> eidx;       // input index of each layout edge
+  std::vector<int64_t> fidx;       // input index of each layout face
   std::vector<uint32_t> fab;       // local a | local b << 16
   std::vector<uint32_t> fcs;       // local c | symmetry bits << 16
   std::vector<int> fglob;          // global face nodes (checks)

@@ -623,6 +623,7 @@ int xg_ctx_destroy(xg_ctx* c) {
   if (c->sU) { (void)hipFree(c->sU); (void)hipFree(c->sG); (void)hipFree(c->sR); }
   if (c->cg) (void)xg_cg_destroy(c);
   if (c->koz) xg_kozak_free(c);
+  if (c->cho) xg_chocg_free(c);
   delete c;
   return XG_OK;
 }
