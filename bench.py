@@ -133,6 +133,18 @@ def poisson_bench(n, stream, reps=2):
     return out
 
 
+def read_traffic(path):
+    """HBM bytes per rhs launch measured by rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE per dispatch, each scaled by what the counter reads for a copy
+    of known size), as recorded in the profile summary; None if absent."""
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_rhs:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def cpu_baseline(n_sample=64, reps=40):
     """The oracle's riemann::rhs (one core, the reference's loop structure and
     superedge grouping) on an n_sample^3 box of the same workload family."""
@@ -168,6 +180,9 @@ def main():
     ap.add_argument("--order", default="rcm+block", choices=["rcm", "rcm+block"])
     ap.add_argument("--poisson-n", type=int, default=128,
                     help="box size of the pressure-Poisson solve (0: skip)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed region, copy a 512 MiB buffer 3 times "
+                         "(the known-size kernel that calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this path runs on one GPU; --gpus %d is not supported "
@@ -245,6 +260,16 @@ def main():
         "riecg_step_ms": step_ms,
         "wall_s_timed": wall,
     }
+    traffic = read_traffic(os.path.join(ROOT, "profiles", "rhs_box160_traffic.txt"))
+    if traffic is not None and args.n == 160:
+        out["roofline"]["traffic"] = traffic
+    if args.copy_probe:
+        a = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     if args.poisson_n > 0:
         ctx.close()
         del dG, dR, dU, dUn

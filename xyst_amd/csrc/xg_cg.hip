@@ -1,19 +1,21 @@
 // MI355X device path of tk::ConjugateGradients (ConjugateGradients.cpp:
 // 105-823) for one GPU-resident partition, plus a mixed-precision variant.
 //
-// One CG iteration of the reference is two kernels:
-//   k_cg_spmv:   p = z + beta p (into the other p buffer), q = A p, partial
-//                sums of (p,q); the last workgroup reduces them in index order
-//                and sets alpha = rho/(p,q) (or flags the (p,q) ~ 0 breakdown).
-//   k_cg_update: r -= alpha q, z = r/d, x += alpha p, partial sums of (r,z)
-//                and (r,r); the last workgroup reduces them, advances the
-//                iteration count, evaluates the stop test and beta.
-// Scalars live on the device (CgState); the host reads them back only every
-// few iterations; a finished solve turns the remaining launches into no-ops,
-// so the iteration count and result are those of the reference recurrence.
-// Reductions are deterministic: fixed per-workgroup partials summed in index
-// order by whichever workgroup arrives last (an integer arrival counter; no
-// floating-point atomics).
+// Iteration j of the reference recurrence is two kernels:
+//   k_cg_spmv:   p_j = z + beta p_{j-1} (into the other p buffer), q = A p_j,
+//                per-workgroup partial sums of (p_j, q);
+//   k_cg_update: alpha = rho/(p,q) (0 and "finished" when |(p,q)| is below
+//                the breakdown threshold), r -= alpha q, z = r/d,
+//                x += alpha p_j, per-workgroup partials of (r,z) and (r,r).
+// Each kernel starts by summing the partials the previous kernel left, in
+// index order with a fixed tree: every workgroup obtains bitwise the same
+// global value, so no workgroup has to publish it and there is no arrival
+// counter, atomic or cross-workgroup fence.  The (r,z)/(r,r) partials
+// alternate between two buffers by iteration parity, so iteration j still
+// sees rho_{j-2} for beta.  The stop test of ConjugateGradients::x runs at
+// the start of both kernels of the next iteration; once it holds, the
+// remaining launches are no-ops and workgroup 0 records the final state,
+// which the host reads every few iterations.  Results are deterministic.
 //
 // SpMV: 8 lanes per CSR row (rows hold ~15 entries on tet meshes), lane
 // partial sums combined by a fixed butterfly.
@@ -35,15 +37,13 @@ constexpr int kCgThreads = 256;
 constexpr int kLanesPerRow = 8;
 constexpr int kMaxParts = 2048;
 
+constexpr int kVecParts = 512;    // workgroups of the vector kernels (max)
+constexpr int kBufLen = 2 * kVecParts + 8;   // rz[.], rr[.], finished flag
+
 struct CgState {
-  double rho, beta, alpha, rr, normb;
+  double normb, rho0, rr;
   long long it;
-  int finished, done, converged, pad;
-  unsigned int count;            // arrival counter of the last-block reductions
-  unsigned int pad2;
-  double tol_nb;                 // tol * max(normb, 1 if normb <= 1e-14)
-  long long maxit;
-  double brk;                    // |(p,q)| below this ends the solve
+  int done, converged;
 };
 
 }  // namespace
@@ -66,10 +66,10 @@ struct CgSolver {
   double *p0 = nullptr, *p1 = nullptr, *q = nullptr;
   float *r32 = nullptr, *z32 = nullptr, *p32a = nullptr, *p32b = nullptr,
         *q32 = nullptr, *d32 = nullptr, *e32 = nullptr;
-  double* part = nullptr;        // 2 x kMaxParts partial sums
+  double* pq = nullptr;          // kMaxParts partials of (p,q) (or scratch)
+  double* rzb = nullptr;         // 2 x kBufLen partials of (r,z), (r,r)
   CgState* st = nullptr;
   CgState* hst = nullptr;        // pinned host mirror
-  int pbuf = 0;                  // which p buffer holds the current p
   int pc = 0;
   bool ready = false;            // setup done
 };
@@ -79,7 +79,7 @@ namespace {
 // ---------------------------------------------------------------- kernels --
 
 template <class T>
-__device__ inline T warp_row_sum(T v) {
+__device__ inline T row_sum8(T v) {
   // butterfly over the 8 lanes of a row: fixed order, same result every run
   v += __shfl_xor(v, 1, 64);
   v += __shfl_xor(v, 2, 64);
@@ -87,45 +87,64 @@ __device__ inline T warp_row_sum(T v) {
   return v;
 }
 
-__device__ inline double block_sum(double v, double* sm) {
+// Workgroup sum returned to every thread (fixed tree, so every workgroup that
+// sums the same values gets bitwise the same result).
+__device__ inline double block_sum_all(double v, double* sm) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) sm[w] = v;
+  __syncthreads();                       // sm free from a previous use
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
   __syncthreads();
   double s = 0.0;
-  if (threadIdx.x == 0)
-    for (int k = 0; k < kCgThreads / 64; ++k) s += sm[k];
-  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kCgThreads / 64; ++k) s += sm[k];
   return s;
 }
 
-// Publishes this workgroup's partial sums; returns true in the workgroup that
-// arrives last, which then sees every partial.
-__device__ inline bool arrive(CgState* st, double* part, int nparts,
-                              const double* vals, int nv) {
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < nv; ++k) part[k * kMaxParts + blockIdx.x] = vals[k];
-    __threadfence();
-    unsigned int prev = atomicAdd(&st->count, 1u);
-    last = (prev == (unsigned int)nparts - 1);
-  }
-  __syncthreads();
-  if (last) __threadfence();
-  return last;
-}
-
-// Sum of partials [0, n) in index order, by the whole workgroup (fixed tree).
-__device__ inline double sum_parts(const double* part, int n, double* sm) {
+// Sum of part[0, n) in index order (n <= 4 * kCgThreads... any n): thread t
+// adds part[t], part[t + 256], ... then the fixed tree.
+__device__ inline double sum_parts(const double* __restrict__ part, int n, double* sm) {
   double v = 0.0;
-  for (int i = threadIdx.x; i < n; i += kCgThreads)
-    v += __builtin_nontemporal_load(part + i);
-  return block_sum(v, sm);
+  for (int i = threadIdx.x; i < n; i += kCgThreads) v += part[i];
+  return block_sum_all(v, sm);
 }
 
-// r = b - A x (setup, ConjugateGradients::residual/initres) and the diagonal
-// preconditioner d (pc: 1 = jacobi diag(A), 0 = none -> 1).
+// Writes a workgroup partial: slot `k` of `part`.
+__device__ inline void put_part(double* part, int k, double v, double* sm) {
+  double s = block_sum_all(v, sm);
+  if (threadIdx.x == 0) part[k] = s;
+}
+
+// The stop test of ConjugateGradients::x (ConjugateGradients.cpp:772-823)
+// after iteration it = j-1, from that iteration's (r,r) and finished flag.
+struct StopTest {
+  double rz, rr;
+  bool stop, converged;
+};
+
+__device__ inline StopTest stop_test(const double* __restrict__ buf, int nvp, int j,
+                                     long long maxit, double tol_nb, double* sm) {
+  StopTest s;
+  s.rz = sum_parts(buf, nvp, sm);
+  s.rr = sum_parts(buf + kVecParts, nvp, sm);
+  const bool finished = buf[2 * kVecParts] != 0.0;
+  const double normr = sqrt(s.rr);
+  s.stop = j >= 2 && (finished || normr < tol_nb || (long long)(j - 1) >= maxit);
+  s.converged = !(normr > tol_nb);
+  return s;
+}
+
+__device__ inline void record_stop(CgState* st, int j, const StopTest& s) {
+  if (blockIdx.x == 0 && threadIdx.x == 0 && !st->done) {
+    st->it = j - 1;
+    st->rr = s.rr;
+    st->converged = s.converged ? 1 : 0;
+    st->done = 1;
+  }
+}
+
+// r = b - A x (ConjugateGradients::residual) and the diagonal preconditioner
+// d (pc 1 = jacobi: diag(A); 0 = none: 1), ConjugateGradients.cpp:160-260.
 __global__ __launch_bounds__(kCgThreads) void k_cg_residual(
     int64_t nrow, const int* __restrict__ ia, const int* __restrict__ ja,
     const double* __restrict__ a, const double* __restrict__ x,
@@ -140,8 +159,8 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_residual(
       s += a[j] * x[c];
       if (c == row) dg = a[j];
     }
-    s = warp_row_sum(s);
-    dg = warp_row_sum(dg);
+    s = row_sum8(s);
+    dg = row_sum8(dg);
     if (lane == 0) {
       r[row] = b[row] - s;
       d[row] = pc ? dg : 1.0;
@@ -149,119 +168,115 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_residual(
   }
 }
 
-// z = r/d, p = r (initres); partial sums of (b,b) and (r,z)
+// initres (ConjugateGradients.cpp:300-333): z = r/d, p_0 = r; partials of
+// (r,z) and (r,r) into buffer 0 and of (b,b) into `bb`.
 __global__ __launch_bounds__(kCgThreads) void k_cg_init(
     int64_t n, const double* __restrict__ b, const double* __restrict__ r,
-    const double* __restrict__ d, double* __restrict__ z,
-    double* __restrict__ p, CgState* st, double* part, double tol) {
+    const double* __restrict__ d, double* __restrict__ z, double* __restrict__ p,
+    double* __restrict__ buf0, double* __restrict__ bb) {
   __shared__ double sm[kCgThreads / 64];
-  double bb = 0.0, rz = 0.0;
+  double sbb = 0.0, rz = 0.0, rr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kCgThreads + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kCgThreads) {
     double zi = r[i] / d[i];
     z[i] = zi;
     p[i] = r[i];
-    bb += b[i] * b[i];
+    sbb += b[i] * b[i];
     rz += r[i] * zi;
+    rr += r[i] * r[i];
   }
-  double v[2] = {block_sum(bb, sm), block_sum(rz, sm)};
-  if (arrive(st, part, gridDim.x, v, 2)) {
-    double BB = sum_parts(part, gridDim.x, sm);
-    double RZ = sum_parts(part + kMaxParts, gridDim.x, sm);
-    if (threadIdx.x == 0) {
-      st->normb = sqrt(BB);
-      st->rho = RZ;
-      st->beta = 0.0;
-      st->it = 0;
-      st->finished = st->done = st->converged = 0;
-      double nb = st->normb > 1.0e-14 ? st->normb : 1.0;
-      st->tol_nb = tol * nb;
-      st->brk = 2.220446049250313e-16;   // DBL_EPSILON, absolute
-      st->count = 0;
-    }
+  put_part(bb, blockIdx.x, sbb, sm);
+  put_part(buf0, blockIdx.x, rz, sm);
+  put_part(buf0 + kVecParts, blockIdx.x, rr, sm);
+  if (blockIdx.x == 0 && threadIdx.x == 0) buf0[2 * kVecParts] = 0.0;
+}
+
+// normb = ||b|| and rho_0 = (r,z) into the state (one workgroup)
+__global__ __launch_bounds__(kCgThreads) void k_cg_setup_final(
+    const double* __restrict__ bb, const double* __restrict__ buf0, int nvp,
+    CgState* st) {
+  __shared__ double sm[kCgThreads / 64];
+  const double BB = sum_parts(bb, nvp, sm);
+  const double RZ = sum_parts(buf0, nvp, sm);
+  const double RR = sum_parts(buf0 + kVecParts, nvp, sm);
+  if (threadIdx.x == 0) {
+    st->normb = sqrt(BB);
+    st->rho0 = RZ;
+    st->rr = RR;
+    st->it = 0;
+    st->done = 0;
+    st->converged = 0;
   }
 }
 
-// p' = z + beta p, q = A p', partial (p',q); last block: alpha
+// iteration j, first half: p_j = z + beta p_{j-1}, q = A p_j, partials of
+// (p_j, q) (ConjugateGradients::next/qAp, :585-635)
 template <class T>
 __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(
-    int64_t nrow, const int* __restrict__ ia, const int* __restrict__ ja,
+    int j, int64_t nrow, const int* __restrict__ ia, const int* __restrict__ ja,
     const T* __restrict__ a, const T* __restrict__ z, const T* __restrict__ pold,
-    T* __restrict__ pnew, T* __restrict__ q, CgState* st, double* part) {
+    T* __restrict__ pnew, T* __restrict__ q, const double* __restrict__ rzb,
+    int nvp, double* __restrict__ pq, CgState* st, long long maxit, double tol_nb) {
   __shared__ double sm[kCgThreads / 64];
-  if (st->done) return;
-  const T beta = (T)st->beta;
+  const StopTest s = stop_test(rzb + ((j - 1) & 1) * kBufLen, nvp, j, maxit, tol_nb, sm);
+  if (s.stop) {
+    record_stop(st, j, s);
+    return;
+  }
+  double beta = 0.0;   // the reference's m_alpha = rho/rho0 (:590)
+  if (j >= 2) beta = s.rz / sum_parts(rzb + (j & 1) * kBufLen, nvp, sm);
+  const T bt = (T)beta;
   const int lane = threadIdx.x & (kLanesPerRow - 1);
-  double pq = 0.0;
+  double spq = 0.0;
   for (int64_t row = ((int64_t)blockIdx.x * kCgThreads + threadIdx.x) / kLanesPerRow;
        row < nrow; row += (int64_t)gridDim.x * kCgThreads / kLanesPerRow) {
-    T s = 0;
-    for (int j = ia[row] + lane; j < ia[row + 1]; j += kLanesPerRow) {
-      int c = ja[j];
-      s += a[j] * (z[c] + beta * pold[c]);
+    T acc = 0;
+    for (int k = ia[row] + lane; k < ia[row + 1]; k += kLanesPerRow) {
+      const int c = ja[k];
+      acc += a[k] * (z[c] + bt * pold[c]);
     }
-    s = warp_row_sum(s);
+    acc = row_sum8(acc);
     if (lane == 0) {
-      T pr = z[row] + beta * pold[row];
+      const T pr = z[row] + bt * pold[row];
       pnew[row] = pr;
-      q[row] = s;
-      pq += (double)pr * (double)s;
+      q[row] = acc;
+      spq += (double)pr * (double)acc;
     }
   }
-  double v = block_sum(pq, sm);
-  if (arrive(st, part, gridDim.x, &v, 1)) {
-    double PQ = sum_parts(part, gridDim.x, sm);
-    if (threadIdx.x == 0) {
-      if (fabs(PQ) < st->brk) {      // ConjugateGradients.cpp:680-687
-        st->finished = 1;
-        st->alpha = 0.0;
-      } else {
-        st->alpha = st->rho / PQ;
-      }
-      st->count = 0;
-    }
-  }
+  put_part(pq, blockIdx.x, spq, sm);
 }
 
-// r -= alpha q, z = r/d, x += alpha p; partial (r,z), (r,r); last block: the
-// stop test of ConjugateGradients::x (ConjugateGradients.cpp:772-823) and beta
+// iteration j, second half (ConjugateGradients::pq/rz, :672-750): alpha,
+// r -= alpha q, z = r/d, x += alpha p_j; partials of (r,z), (r,r) and the
+// finished flag into buffer j & 1
 template <class T>
 __global__ __launch_bounds__(kCgThreads) void k_cg_update(
-    int64_t n, const T* __restrict__ q, const T* __restrict__ d,
-    const T* __restrict__ p, T* __restrict__ r, T* __restrict__ z,
-    T* __restrict__ x, CgState* st, double* part) {
+    int j, int64_t n, const T* __restrict__ q, const T* __restrict__ d,
+    const T* __restrict__ p, T* __restrict__ r, T* __restrict__ z, T* __restrict__ x,
+    double* __restrict__ rzb, int nvp, const double* __restrict__ pq, int nsp,
+    long long maxit, double tol_nb, double brk) {
   __shared__ double sm[kCgThreads / 64];
-  if (st->done) return;
-  const T alpha = (T)st->alpha;
+  const StopTest s = stop_test(rzb + ((j - 1) & 1) * kBufLen, nvp, j, maxit, tol_nb, sm);
+  if (s.stop) return;
+  const double PQ = sum_parts(pq, nsp, sm);
+  const bool finished = fabs(PQ) < brk;        // ConjugateGradients.cpp:680-687
+  const double alpha = finished ? 0.0 : s.rz / PQ;
+  const T al = (T)alpha;
   double rz = 0.0, rr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kCgThreads + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kCgThreads) {
-    T ri = r[i] - alpha * q[i];
-    T zi = ri / d[i];
+    const T ri = r[i] - al * q[i];
+    const T zi = ri / d[i];
     r[i] = ri;
     z[i] = zi;
-    x[i] += alpha * p[i];
+    x[i] += al * p[i];
     rz += (double)ri * (double)zi;
     rr += (double)ri * (double)ri;
   }
-  double v[2] = {block_sum(rz, sm), block_sum(rr, sm)};
-  if (arrive(st, part, gridDim.x, v, 2)) {
-    double RZ = sum_parts(part, gridDim.x, sm);
-    double RR = sum_parts(part + kMaxParts, gridDim.x, sm);
-    if (threadIdx.x == 0) {
-      double rho0 = st->rho;
-      st->rho = RZ;
-      st->rr = RR;
-      st->beta = RZ / rho0;
-      st->it += 1;
-      double normr = sqrt(RR);
-      if (st->finished || normr < st->tol_nb || st->it >= st->maxit) {
-        st->done = 1;
-        st->converged = normr > st->tol_nb ? 0 : 1;
-      }
-      st->count = 0;
-    }
-  }
+  double* out = rzb + (j & 1) * kBufLen;
+  put_part(out, blockIdx.x, rz, sm);
+  put_part(out + kVecParts, blockIdx.x, rr, sm);
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[2 * kVecParts] = finished ? 1.0 : 0.0;
 }
 
 // ---- mixed precision helpers ----
@@ -274,12 +289,11 @@ __global__ __launch_bounds__(kCgThreads) void k_add_f32(
     x[i] += (double)e[i];
 }
 
-// FP32 inner setup: r32 = rin, e = 0, z = r/d, p = r; (r,z), (r,r) -> state
+// FP32 inner setup: r32 = rin, e = 0, z = r/d, p_0 = r; (r,z), (r,r) -> buffer 0
 __global__ __launch_bounds__(kCgThreads) void k_cg32_init(
     int64_t n, const double* __restrict__ rin, const float* __restrict__ d,
     float* __restrict__ r, float* __restrict__ z, float* __restrict__ p,
-    float* __restrict__ e, CgState* st, double* part, double tol_abs,
-    long long maxit) {
+    float* __restrict__ e, double* __restrict__ buf0) {
   __shared__ double sm[kCgThreads / 64];
   double rz = 0.0, rr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kCgThreads + threadIdx.x; i < n;
@@ -293,31 +307,16 @@ __global__ __launch_bounds__(kCgThreads) void k_cg32_init(
     rz += (double)ri * (double)zi;
     rr += (double)ri * (double)ri;
   }
-  double v[2] = {block_sum(rz, sm), block_sum(rr, sm)};
-  if (arrive(st, part, gridDim.x, v, 2)) {
-    double RZ = sum_parts(part, gridDim.x, sm);
-    double RR = sum_parts(part + kMaxParts, gridDim.x, sm);
-    if (threadIdx.x == 0) {
-      st->rho = RZ;
-      st->rr = RR;
-      st->beta = 0.0;
-      st->it = 0;
-      st->finished = st->converged = 0;
-      st->done = sqrt(RR) < tol_abs ? 1 : 0;
-      st->tol_nb = tol_abs;
-      st->maxit = maxit;
-      st->brk = 1.0e-300;           // inner solve: stop only at (p,q) ~ 0
-      st->count = 0;
-    }
-  }
+  put_part(buf0, blockIdx.x, rz, sm);
+  put_part(buf0 + kVecParts, blockIdx.x, rr, sm);
+  if (blockIdx.x == 0 && threadIdx.x == 0) buf0[2 * kVecParts] = 0.0;
 }
 
-// r = b - A x in FP64 with partial (r,r); last block: rr, normb test
+// r = b - A x in FP64 with partials of (r,r)
 __global__ __launch_bounds__(kCgThreads) void k_residual_norm(
     int64_t nrow, const int* __restrict__ ia, const int* __restrict__ ja,
     const double* __restrict__ a, const double* __restrict__ x,
-    const double* __restrict__ b, double* __restrict__ r, CgState* st,
-    double* part) {
+    const double* __restrict__ b, double* __restrict__ r, double* __restrict__ part) {
   __shared__ double sm[kCgThreads / 64];
   const int lane = threadIdx.x & (kLanesPerRow - 1);
   double rr = 0.0;
@@ -325,21 +324,31 @@ __global__ __launch_bounds__(kCgThreads) void k_residual_norm(
        row < nrow; row += (int64_t)gridDim.x * kCgThreads / kLanesPerRow) {
     double s = 0.0;
     for (int j = ia[row] + lane; j < ia[row + 1]; j += kLanesPerRow) s += a[j] * x[ja[j]];
-    s = warp_row_sum(s);
+    s = row_sum8(s);
     if (lane == 0) {
       double ri = b[row] - s;
       r[row] = ri;
       rr += ri * ri;
     }
   }
-  double v = block_sum(rr, sm);
-  if (arrive(st, part, gridDim.x, &v, 1)) {
-    double RR = sum_parts(part, gridDim.x, sm);
-    if (threadIdx.x == 0) {
-      st->rr = RR;
-      st->count = 0;
-    }
+  put_part(part, blockIdx.x, rr, sm);
+}
+
+// a new solve on the state: iteration count and flags cleared
+__global__ void k_state_reset(CgState* st) {
+  if (threadIdx.x == 0) {
+    st->it = 0;
+    st->done = 0;
+    st->converged = 0;
   }
+}
+
+// st->rr = sum of part[0, n) (one workgroup)
+__global__ __launch_bounds__(kCgThreads) void k_sum_to_state(
+    const double* __restrict__ part, int n, CgState* st) {
+  __shared__ double sm[kCgThreads / 64];
+  const double v = sum_parts(part, n, sm);
+  if (threadIdx.x == 0) st->rr = v;
 }
 
 __global__ void k_f32_from_f64(int64_t n, const double* __restrict__ s,
@@ -357,7 +366,7 @@ int grid_rows(int64_t nrow) {
 
 int grid_vec(int64_t n) {
   int64_t g = (n + kCgThreads - 1) / kCgThreads;
-  return (int)std::max<int64_t>(1, std::min<int64_t>(g, 1024));
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, kVecParts));
 }
 
 void free_solver(CgSolver* s) {
@@ -392,36 +401,34 @@ int read_state(xg_ctx* c) {
   return XG_OK;
 }
 
-// FP64 CG iterations on the current state until done (host checks every
-// `batch` iterations).
-int run_iterations(xg_ctx* c, int64_t maxit) {
+// Iterations j = 1, 2, ... of the recurrence set up in buffer 0 of rzb, until
+// the stop test holds (at the latest at j = maxit + 1); the host checks the
+// state every `batch` iterations.  T: FP64 solve or the FP32 inner solve.
+template <class T>
+int run_cg(xg_ctx* c, const T* a, T* z, T* p0, T* p1, T* q, const T* d, T* r, T* x,
+           int64_t maxit, double tol_nb, double brk) {
   CgSolver* s = c->cg;
   const int gr = grid_rows(s->nrow), gv = grid_vec(s->nrow);
   const int batch = 8;
-  for (int64_t k = 0; k < maxit; k += batch) {
-    for (int i = 0; i < batch && k + i < maxit; ++i) {
-      double* pold = s->pbuf ? s->p1 : s->p0;
-      double* pnew = s->pbuf ? s->p0 : s->p1;
-      hipLaunchKernelGGL(k_cg_spmv<double>, dim3(gr), dim3(kCgThreads), 0,
-                         c->stream, s->nrow, s->dia, s->dja, s->da, s->z, pold,
-                         pnew, s->q, s->st, s->part);
-      hipLaunchKernelGGL(k_cg_update<double>, dim3(gv), dim3(kCgThreads), 0,
-                         c->stream, s->nrow, s->q, s->d, pnew, s->r, s->z, s->x,
-                         s->st, s->part);
-      s->pbuf ^= 1;
+  for (int64_t j0 = 1;; j0 += batch) {
+    if (j0 > maxit + 1 + batch)
+      return fail(c, XG_EHIP, "CG: stop test not reached on the device");
+    for (int i = 0; i < batch; ++i) {
+      const int j = (int)(j0 + i);
+      T* pold = (j - 1) & 1 ? p1 : p0;
+      T* pnew = j & 1 ? p1 : p0;
+      hipLaunchKernelGGL(k_cg_spmv<T>, dim3(gr), dim3(kCgThreads), 0, c->stream, j,
+                         s->nrow, s->dia, s->dja, a, z, pold, pnew, q, s->rzb, gv,
+                         s->pq, s->st, (long long)maxit, tol_nb);
+      hipLaunchKernelGGL(k_cg_update<T>, dim3(gv), dim3(kCgThreads), 0, c->stream, j,
+                         s->nrow, q, d, pnew, r, z, x, s->rzb, gv, s->pq, gr,
+                         (long long)maxit, tol_nb, brk);
     }
     XG_HIP(c, hipGetLastError());
     int rc = read_state(c);
     if (rc) return rc;
-    if (s->hst->done) {
-      // the no-op launches after the stop did not advance p
-      int extra = (int)(std::min<int64_t>(batch, maxit - k) -
-                        (s->hst->it - k));
-      if (extra & 1) s->pbuf ^= 1;
-      return XG_OK;
-    }
+    if (s->hst->done) return XG_OK;
   }
-  return XG_OK;
 }
 
 }  // namespace
@@ -480,7 +487,8 @@ int xg_cg_matrix(xg_ctx* c, int ncomp, int64_t npoin, const size_t* ia,
       (rc = dev_alloc(c, L, (void**)&s->q32, n * 4)) ||
       (rc = dev_alloc(c, L, (void**)&s->d32, n * 4)) ||
       (rc = dev_alloc(c, L, (void**)&s->e32, n * 4)) ||
-      (rc = dev_alloc(c, L, (void**)&s->part, sizeof(double) * 2 * kMaxParts)) ||
+      (rc = dev_alloc(c, L, (void**)&s->pq, sizeof(double) * kMaxParts)) ||
+      (rc = dev_alloc(c, L, (void**)&s->rzb, sizeof(double) * 2 * kBufLen)) ||
       (rc = dev_alloc(c, L, (void**)&s->st, sizeof(CgState)))) {
     free_solver(s);
     return rc;
@@ -599,14 +607,15 @@ int xg_cg_setup(xg_ctx* c, double* normb) {
   CgSolver* s = c->cg;
   XG_HIP(c, hipSetDevice(c->device));
   if (s->restore && (rc = restore_values(c))) return rc;
+  const int gv = grid_vec(s->nrow);
   hipLaunchKernelGGL(k_cg_residual, dim3(grid_rows(s->nrow)), dim3(kCgThreads), 0,
                      c->stream, s->nrow, s->dia, s->dja, s->da, s->x, s->b, s->r,
                      s->d, s->pc);
-  hipLaunchKernelGGL(k_cg_init, dim3(grid_vec(s->nrow)), dim3(kCgThreads), 0,
-                     c->stream, s->nrow, s->b, s->r, s->d, s->z, s->p0, s->st,
-                     s->part, 0.0);
+  hipLaunchKernelGGL(k_cg_init, dim3(gv), dim3(kCgThreads), 0, c->stream, s->nrow,
+                     s->b, s->r, s->d, s->z, s->p0, s->rzb, s->pq);
+  hipLaunchKernelGGL(k_cg_setup_final, dim3(1), dim3(kCgThreads), 0, c->stream, s->pq,
+                     s->rzb, gv, s->st);
   XG_HIP(c, hipGetLastError());
-  s->pbuf = 0;
   if ((rc = read_state(c))) return rc;
   s->ready = true;
   if (normb) *normb = s->hst->normb;
@@ -622,19 +631,11 @@ int xg_cg_solve(xg_ctx* c, int64_t maxit, double tol, double* normr,
   if (maxit < 1 || !(tol >= 0.0)) return fail(c, XG_EINVAL, "xg_cg_solve: bad maxit/tol");
   XG_HIP(c, hipSetDevice(c->device));
   // stop test parameters (ConjugateGradients.cpp:785-787)
-  CgState* h = s->hst;
-  double nb = h->normb > 1.0e-14 ? h->normb : 1.0;
-  h->tol_nb = tol * nb;
-  h->maxit = maxit;
-  h->it = 0;
-  h->done = 0;
-  h->finished = 0;
-  h->converged = 0;
-  h->beta = 0.0;
-  h->count = 0;
-  XG_HIP(c, hipMemcpyAsync(s->st, h, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
-  if ((rc = run_iterations(c, maxit))) return rc;
-  h = s->hst;
+  const double nb = s->hst->normb > 1.0e-14 ? s->hst->normb : 1.0;
+  rc = run_cg<double>(c, s->da, s->z, s->p0, s->p1, s->q, s->d, s->r, s->x, maxit,
+                      tol * nb, 2.220446049250313e-16);
+  if (rc) return rc;
+  const CgState* h = s->hst;
   if (normr) *normr = sqrt(h->rr);
   if (it) *it = h->it;
   if (converged) *converged = h->converged;
@@ -673,6 +674,7 @@ int xg_cg_solve_mixed(xg_ctx* c, int64_t maxit, double tol, double inner_tol,
   if (maxit < 1 || inner_maxit < 1 || !(tol > 0.0) || !(inner_tol > 0.0))
     return fail(c, XG_EINVAL, "xg_cg_solve_mixed: bad arguments");
   XG_HIP(c, hipSetDevice(c->device));
+  if (s->restore && (rc = restore_values(c))) return rc;
   const int64_t n = s->nrow;
   const int gr = grid_rows(n), gv = grid_vec(n);
   // preconditioner (FP64 diagonal, then FP32 copy) and ||b||
@@ -681,7 +683,9 @@ int xg_cg_solve_mixed(xg_ctx* c, int64_t maxit, double tol, double inner_tol,
   hipLaunchKernelGGL(k_f32_from_f64, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n,
                      s->d, s->d32);
   hipLaunchKernelGGL(k_cg_init, dim3(gv), dim3(kCgThreads), 0, c->stream, n, s->b,
-                     s->r, s->d, s->z, s->p0, s->st, s->part, tol);
+                     s->r, s->d, s->z, s->p0, s->rzb, s->pq);
+  hipLaunchKernelGGL(k_cg_setup_final, dim3(1), dim3(kCgThreads), 0, c->stream, s->pq,
+                     s->rzb, gv, s->st);
   if ((rc = read_state(c))) return rc;
   const double normb = s->hst->normb;
   const double target = tol * (normb > 1.0e-14 ? normb : 1.0);
@@ -690,34 +694,23 @@ int xg_cg_solve_mixed(xg_ctx* c, int64_t maxit, double tol, double inner_tol,
   int conv = 0;
   for (;;) {
     hipLaunchKernelGGL(k_residual_norm, dim3(gr), dim3(kCgThreads), 0, c->stream, n,
-                       s->dia, s->dja, s->da, s->x, s->b, s->r, s->st, s->part);
+                       s->dia, s->dja, s->da, s->x, s->b, s->r, s->pq);
+    hipLaunchKernelGGL(k_sum_to_state, dim3(1), dim3(kCgThreads), 0, c->stream, s->pq,
+                       gr, s->st);
     if ((rc = read_state(c))) return rc;
     nr = sqrt(s->hst->rr);
     if (nr < target) { conv = 1; break; }
     if (total >= maxit) break;
     ++outer;
-    // inner FP32 solve of A e = r to |r_e| < inner_tol * |r|
-    hipLaunchKernelGGL(k_cg32_init, dim3(gv), dim3(kCgThreads), 0, c->stream, n,
-                       s->r, s->d32, s->r32, s->z32, s->p32a, s->e32, s->st, s->part,
-                       inner_tol * nr, (long long)inner_maxit);
-    int pb = 0;
-    const int batch = 8;
-    for (int64_t k = 0; k < inner_maxit; k += batch) {
-      for (int i = 0; i < batch; ++i) {
-        float* pold = pb ? s->p32b : s->p32a;
-        float* pnew = pb ? s->p32a : s->p32b;
-        hipLaunchKernelGGL(k_cg_spmv<float>, dim3(gr), dim3(kCgThreads), 0,
-                           c->stream, n, s->dia, s->dja, s->da32, s->z32, pold,
-                           pnew, s->q32, s->st, s->part);
-        hipLaunchKernelGGL(k_cg_update<float>, dim3(gv), dim3(kCgThreads), 0,
-                           c->stream, n, s->q32, s->d32, pnew, s->r32, s->z32,
-                           s->e32, s->st, s->part);
-        pb ^= 1;
-      }
-      XG_HIP(c, hipGetLastError());
-      if ((rc = read_state(c))) return rc;
-      if (s->hst->done) break;
-    }
+    // inner FP32 solve of A e = r to ||r_e|| < inner_tol ||r|| (a relative
+    // stop only: the breakdown threshold is a denormal-scale guard)
+    hipLaunchKernelGGL(k_cg32_init, dim3(gv), dim3(kCgThreads), 0, c->stream, n, s->r,
+                       s->d32, s->r32, s->z32, s->p32a, s->e32, s->rzb);
+    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, c->stream, s->st);
+    rc = run_cg<float>(c, s->da32, s->z32, s->p32a, s->p32b, s->q32, s->d32, s->r32,
+                       s->e32, std::min<int64_t>(inner_maxit, maxit - total),
+                       inner_tol * nr, 1.0e-300);
+    if (rc) return rc;
     total += s->hst->it;
     hipLaunchKernelGGL(k_add_f32, dim3(gv), dim3(kCgThreads), 0, c->stream, n,
                        s->e32, s->x);

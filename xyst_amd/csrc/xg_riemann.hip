@@ -55,107 +55,6 @@ struct ChunkPrefetch {
   }
 };
 
-// Stages NV values per local node into sN[c*CAP + i]: the 5 conserved
-// components of U, and for NV = 23 also the 15 gradient components and the 3
-// coordinates.  The owned nodes are a contiguous id range, so their rows are
-// streamed with flat coalesced loads spread over all threads; halo values are
-// gathered as (node, value) pairs, also over all threads.  Loads of a round
-// are all issued before the first LDS store.
-template <int CAP, int NV>
-__device__ inline void stage_nodes(int t, int own0, int nown, int nloc,
-                                   const int* __restrict__ hl,
-                                   const double* __restrict__ U,
-                                   const double* __restrict__ G,
-                                   const double* __restrict__ x,
-                                   const double* __restrict__ y,
-                                   const double* __restrict__ z, double* sN) {
-  constexpr int T = kRhsThreads;
-  {
-    constexpr int R = (5 * CAP + T - 1) / T;
-    const double* src = U + (size_t)own0 * 5;
-    const int n = nown * 5;
-    double v[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      int k = t + r * T;
-      v[r] = k < n ? src[k] : 0.0;
-    }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      int k = t + r * T;
-      if (k < n) {
-        int i = k / 5;
-        sN[(k - 5 * i) * CAP + i] = v[r];
-      }
-    }
-  }
-  if constexpr (NV == 23) {
-    constexpr int R = (15 * CAP + T - 1) / T;
-    const double* src = G + (size_t)own0 * 15;
-    const int n = nown * 15;
-    double v[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      int k = t + r * T;
-      v[r] = k < n ? src[k] : 0.0;
-    }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      int k = t + r * T;
-      if (k < n) {
-        int i = k / 15;
-        sN[(5 + k - 15 * i) * CAP + i] = v[r];
-      }
-    }
-    if (t < nown) {
-      double a = x[own0 + t], b = y[own0 + t], c = z[own0 + t];
-      sN[20 * CAP + t] = a;
-      sN[21 * CAP + t] = b;
-      sN[22 * CAP + t] = c;
-    }
-  }
-  const int nh = nloc - nown, tot = nh * NV;
-  for (int base = 0; base < tot; base += 4 * T) {
-    double v[4];
-    int d[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int k = base + t + r * T;
-      d[r] = -1;
-      v[r] = 0.0;
-      if (k < tot) {
-        int h = k / NV, c = k - NV * h;
-        int g = hl[h];
-        if constexpr (NV == 5) {
-          v[r] = U[(size_t)g * 5 + c];
-        } else {
-          const double* p = c < 5 ? U + (size_t)g * 5 + c
-                          : c < 20 ? G + (size_t)g * 15 + (c - 5)
-                          : c == 20 ? x + g : c == 21 ? y + g : z + g;
-          v[r] = *p;
-        }
-        d[r] = c * CAP + nown + h;
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-      if (d[r] >= 0) sN[d[r]] = v[r];
-  }
-}
-
-// Conserved -> primitive variables of the staged nodes, in place.
-template <int CAP>
-__device__ inline void stage_primitive(int t, int nloc, double* sN) {
-  if (t < nloc) {
-    double s[5], u[5];
-#pragma unroll
-    for (int c = 0; c < 5; ++c) s[c] = sN[c * CAP + t];
-    xg::primitive_fast(s, u);
-#pragma unroll
-    for (int c = 0; c < 5; ++c) sN[c * CAP + t] = u[c];
-  }
-}
-
 // riemann::grad domain term (Riemann.cpp:263-328) on the rhs block layout.
 // The block's primitive states are staged in LDS.  Per chunk: each thread
 // writes one edge's (u_p + u_q, D) to LDS; then the thread owning local node
@@ -178,10 +77,15 @@ __global__ __launch_bounds__(kRhsThreads) void k_grad_blocks(
   const int4 B0 = blk0[b];   // own0, nown, nloc, halo_off
   const int4 B1 = blk1[b];   // slot0, chunk0, chunk1
   const int own0 = B0.x, nown = B0.y, nloc = B0.z;
-  stage_nodes<CAP, 5>(t, own0, nown, nloc, halo + B0.w, U, nullptr, nullptr,
-                      nullptr, nullptr, sP);
-  __syncthreads();
-  stage_primitive<CAP>(t, nloc, sP);
+  if (t < nloc) {
+    int g = t < nown ? own0 + t : halo[B0.w + t - nown];
+    double s[5], u[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) s[c] = U[(size_t)g * 5 + c];
+    xg::primitive_fast(s, u);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) sP[c * CAP + t] = u[c];
+  }
   // prefetch of the next edge chunk (boundary-face chunks are skipped:
   // k_grad_bnd adds those terms)
   int ch = B1.y;
@@ -364,10 +268,25 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_blocks(
   const int4 B0 = blk0[b];   // own0, nown, nloc, halo_off
   const int4 B1 = blk1[b];   // slot0, chunk0, chunk1
   const int own0 = B0.x, nown = B0.y, nloc = B0.z;
-  // stage the block's nodes
-  stage_nodes<CAP, 23>(t, own0, nown, nloc, halo + B0.w, U, G, x, y, z, sN);
-  __syncthreads();
-  stage_primitive<CAP>(t, nloc, sN);
+  // stage the block's nodes (one per thread, nloc <= threads): all 23 loads of
+  // a node are issued before the first is used
+  if (t < nloc) {
+    int g = t < nown ? own0 + t : halo[B0.w + t - nown];
+    double s[5], u[5], gg[15];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) s[c] = U[(size_t)g * 5 + c];
+#pragma unroll
+    for (int c = 0; c < 15; ++c) gg[c] = G[(size_t)g * 15 + c];
+    double xg_ = x[g], yg = y[g], zg = z[g];
+    xg::primitive_fast(s, u);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) sP[c * CAP + t] = u[c];
+#pragma unroll
+    for (int c = 0; c < 15; ++c) sG[c * CAP + t] = gg[c];
+    sX[0 * CAP + t] = xg_;
+    sX[1 * CAP + t] = yg;
+    sX[2 * CAP + t] = zg;
+  }
   // the next chunk's edge record and entry lists are loaded while the
   // current chunk computes
   ChunkPrefetch pf;
