@@ -10,6 +10,8 @@ struct Cfg {
   double g;
   int stab2;
   double stab2coef;
+  double gm1;   // g - 1, computed on the host (keeps it a scalar operand)
+  double gg1;   // g * (g - 1)
 };
 
 // riemann::primitive, Riemann.cpp:211-227
@@ -238,10 +240,9 @@ __device__ inline void muscl_fast(const double g1[5], const double g2[5],
 // rho, u, v, w, e), divide-free.
 __device__ inline void rusanov_fast(const Cfg& cfg, const double n[3],
                                     double l[5], double r[5], double f[5]) {
-  const double g = cfg.g;
-  const double gg1 = g * (g - 1.0);
-  double pL = (l[0] * l[4]) * (g - 1.0);
-  double pR = (r[0] * r[4]) * (g - 1.0);
+  const double gg1 = cfg.gg1;
+  double pL = (l[0] * l[4]) * cfg.gm1;
+  double pR = (r[0] * r[4]) * cfg.gm1;
   double cL = sqrt(gg1 * l[4]);
   double cR = sqrt(gg1 * r[4]);
   double vnL = l[1] * n[0] + l[2] * n[1] + l[3] * n[2];
@@ -268,15 +269,14 @@ __device__ inline void rusanov_fast(const Cfg& cfg, const double n[3],
 // divides several terms by the same quantity.
 __device__ inline void hllc_fast(const Cfg& cfg, const double d[3], double l[5],
                                  double r[5], double f[5]) {
-  const double g = cfg.g;
-  const double gg1 = g * (g - 1.0);
+  const double gg1 = cfg.gg1;
   double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
   double il = rcp(len);
   double nx = -d[0] * il, ny = -d[1] * il, nz = -d[2] * il;
   double qL = l[1] * nx + l[2] * ny + l[3] * nz;
   double qR = r[1] * nx + r[2] * ny + r[3] * nz;
-  double pL = (l[0] * l[4]) * (g - 1.0);
-  double pR = (r[0] * r[4]) * (g - 1.0);
+  double pL = (l[0] * l[4]) * cfg.gm1;
+  double pR = (r[0] * r[4]) * cfg.gm1;
   double cL = sqrt(gg1 * l[4]);
   double cR = sqrt(gg1 * r[4]);
   l[4] = (l[4] + 0.5 * (l[1] * l[1] + l[2] * l[2] + l[3] * l[3])) * l[0];

@@ -200,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_bnd(
 __device__ inline void face_flux_lds(int la, int lb, int lc, int sym,
                                      const double* __restrict__ sP,
                                      const double* __restrict__ sX,
-                                     double gamma, double out[3][5]) {
+                                     double gm1, double out[3][5]) {
   constexpr int CAP = xg::kBlockCap;
   int N[3] = {la, lb, lc};
   double ba[3], ca[3];
@@ -217,7 +217,7 @@ __device__ inline void face_flux_lds(int la, int lb, int lc, int sym,
   for (int a = 0; a < 3; ++a) {
     double r = sP[0 * CAP + N[a]], u = sP[1 * CAP + N[a]], v = sP[2 * CAP + N[a]];
     double w = sP[3 * CAP + N[a]], e = sP[4 * CAP + N[a]];
-    double p = (r * e) * (gamma - 1.0);
+    double p = (r * e) * gm1;
     double vn = ((sym >> a) & 1) ? 0.0 : (nx * u + ny * v + nz * w);
     double re = r * (e + 0.5 * (u * u + v * v + w * w));
     fl[0][a] = r * vn;
@@ -292,12 +292,12 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_blocks(
   ChunkPrefetch pf;
   uint32_t pq_n = 0;
   double dx_n = 0.0, dy_n = 0.0, dz_n = 0.0;
-  auto fetch = [&](int c) {
+  auto fetch = [&](int c, int tl) {
     if (c < B1.z) {
       const int4 C = chk[c];
-      pf.load(C, chk_ent[c], nloc, t, tend, entries);
-      if (C.x == 0 && t < C.z) {
-        int e = C.y + t;
+      pf.load(C, chk_ent[c], nloc, tl, tend, entries);
+      if (C.x == 0 && tl < C.z) {
+        int e = C.y + tl;
         pq_n = elpq[e];
         dx_n = dx[e];
         dy_n = dy[e];
@@ -305,15 +305,20 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_blocks(
       }
     }
   };
-  fetch(B1.y);
+  fetch(B1.y, t);
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int ch = B1.y; ch < B1.z; ++ch) {
+    // the thread index, opaque to the optimizer: thread-dependent LDS
+    // addresses are recomputed per chunk instead of being held (and spilled)
+    // across the whole loop
+    int tl = t;
+    asm volatile("" : "+v"(tl));
     const int4 C = chk[ch];   // kind, first, count, tend_off
     const uint32_t pq = pq_n;
     double d[3] = {dx_n, dy_n, dz_n};
     __syncthreads();          // staging done / previous chunk's F consumed
-    pf.store(C, nloc, t, sE, sT);
-    if (t < C.z) {
+    pf.store(C, nloc, tl, sE, sT);
+    if (tl < C.z) {
       if (C.x == 0) {
         int lp = pq & 0xffff, lq = pq >> 16;
         double l[5], r[5], f[5], g1[5], g2[5];
@@ -335,22 +340,22 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_blocks(
         else
           xg::hllc_fast(cfg, d, l, r, f);
 #pragma unroll
-        for (int c = 0; c < 5; ++c) F[t * 5 + c] = f[c];
+        for (int c = 0; c < 5; ++c) F[tl * 5 + c] = f[c];
       } else {
-        int f = C.y + t;
+        int f = C.y + tl;
         uint32_t ab = fab[f], cs = fcs[f];
         double o[3][5];
-        face_flux_lds(ab & 0xffff, ab >> 16, cs & 0xffff, cs >> 16, sP, sX, cfg.g, o);
+        face_flux_lds(ab & 0xffff, ab >> 16, cs & 0xffff, cs >> 16, sP, sX, cfg.gm1, o);
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
-          for (int c = 0; c < 5; ++c) F[(3 * t + a) * 5 + c] = o[a][c];
+          for (int c = 0; c < 5; ++c) F[(3 * tl + a) * 5 + c] = o[a][c];
       }
     }
-    fetch(ch + 1);            // in flight during the accumulation below
+    fetch(ch + 1, tl);        // in flight during the accumulation below
     __syncthreads();          // F, sE, sT complete
-    if (t < nloc) {
-      int e0 = t ? sT[t - 1] : 0, e1 = sT[t];
+    if (tl < nloc) {
+      int e0 = tl ? sT[tl - 1] : 0, e1 = sT[tl];
       for (int j = e0; j < e1; ++j) {
         int v = sE[j];
         const double* fs = F + (v & 0x3ff) * 5;
@@ -488,7 +493,8 @@ int launch_grad(xg_ctx* c, const double* U, double* G) {
 int launch_rhs(xg_ctx* c, const double* G, const double* U, double* R) {
   if (c->cfg.flux != 0 && c->cfg.flux != 1)
     return fail(c, XG_EINVAL, "Flux not configured");
-  Cfg cfg{c->cfg.gamma, c->cfg.stab2, c->cfg.stab2coef};
+  const double g = c->cfg.gamma;
+  Cfg cfg{g, c->cfg.stab2, c->cfg.stab2coef, g - 1.0, g * (g - 1.0)};
   auto kern = c->cfg.flux == 0 ? k_rhs_blocks<0> : k_rhs_blocks<1>;
   hipLaunchKernelGGL(kern, dim3(c->nblk), dim3(kRhsThreads), 0, c->stream,
                      c->nblk, c->blk0, c->blk1, c->halo, c->chk, c->chk_ent,
