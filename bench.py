@@ -109,7 +109,7 @@ def poisson_bench(n, stream, reps=2):
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     out = {"n": n, "nodes": int(A.nrow), "nnz": int(A.ja.size), "tol": 1e-10}
-    for name in ("fp64_jacobi_pcg", "mixed_fp32_fp64"):
+    for name in ("fp64_jacobi_pcg", "mixed_fp32_fp64", "fcg_fp64_cg32_inner"):
         ms = []
         for _ in range(reps):
             cg.init(x0, b, dirbc=dirbc, apply=True, pc="jacobi")
@@ -118,17 +118,22 @@ def poisson_bench(n, stream, reps=2):
             if name == "fp64_jacobi_pcg":
                 cg.setup()
                 cg.solve(100000, 1e-10)
-            else:
+            elif name == "mixed_fp32_fp64":
                 cg.solve_mixed(100000, 1e-10, inner_tol=1e-5, inner_maxit=5000)
+            else:
+                cg.solve_flexible(100000, 1e-10, pc="cg32", inner=30)
             ev1.record(stream)
             torch.cuda.synchronize()
             ms.append(ev0.elapsed_time(ev1))
         out[name] = {"ms": min(ms), "iterations": int(cg.it),
                      "converged": bool(cg.converged), "normr_over_normb": cg.normr / cg.normb}
-        if name != "fp64_jacobi_pcg":
+        if name == "mixed_fp32_fp64":
             out[name]["outer"] = int(cg.nouter)
+        if name == "fcg_fp64_cg32_inner":
+            out[name]["inner_iterations"] = int(cg.inner_total)
         log("%s: %.2f ms, %d it, %s" % (name, min(ms), cg.it, cg.converged))
     out["speedup_mixed"] = out["fp64_jacobi_pcg"]["ms"] / out["mixed_fp32_fp64"]["ms"]
+    out["speedup_fcg_cg32"] = out["fp64_jacobi_pcg"]["ms"] / out["fcg_fp64_cg32_inner"]["ms"]
     ctx.close()
     return out
 

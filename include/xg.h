@@ -276,6 +276,16 @@ int xg_cg_solve_mixed(xg_ctx* ctx, int64_t maxit, double tol,
                       double inner_tol, int64_t inner_maxit, double* normr,
                       int64_t* iters, int64_t* nouter, int* converged);
 
+/* Flexible CG (no counterpart in the reference): Notay's FCG(1) in FP64 on
+ * the system set up by xg_cg_init, with preconditioner pc_kind 0 (Jacobi) or
+ * 1 (`inner` iterations of FP32 Jacobi-PCG from a zero guess, i.e. FP32
+ * inner solves inside an FP64 outer Krylov recurrence).  Stops when
+ * ||r|| < tol*normb or after maxit outer iterations; *iters outer,
+ * *inner_total inner iterations. */
+int xg_cg_solve_flexible(xg_ctx* ctx, int pc_kind, int64_t inner, int64_t maxit,
+                         double tol, double* normr, int64_t* iters,
+                         int64_t* inner_total, int* converged);
+
 /* Releases the solver's device memory. */
 int xg_cg_destroy(xg_ctx* ctx);
 

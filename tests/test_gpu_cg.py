@@ -231,3 +231,45 @@ def test_cg_apply_same_dirichlet_set_new_values():
         assert cvo and cg.converged and abs(cg.it - ito) <= 1
         np.testing.assert_allclose(cg.solution(), xo, rtol=0,
                                    atol=1e-7 * np.abs(xo).max())
+
+
+@pytest.mark.parametrize("pc,inner", [("jacobi", 1), ("cg32", 8), ("cg32", 30)])
+def test_flexible_cg_to_1e10(pc, inner):
+    """FP64 flexible CG with Jacobi or FP32 inner-CG preconditioning reaches a
+    true relative residual of 1e-10 on the pressure-Poisson system; with
+    Jacobi it takes the iterations of Jacobi-PCG to within a few."""
+    X = _X()
+    _say("fcg %s inner=%d" % (pc, inner))
+    m = box_mesh(32, seed=9)
+    coord, inpoel = m["coord"], m["inpoel"]
+    A = X.CSR.poisson(coord, inpoel)
+    npoin = coord.shape[1]
+    Ao = O.CSR(npoin, 1, A.ia.astype(np.int64), A.ja.astype(np.int64), A.a.copy())
+    x0 = coord[0]
+    f = np.sin(np.pi * x0) * np.cos(2 * np.pi * coord[1]) + coord[2] ** 2
+    b = Ao.mult(f) * 1e5
+    dirbc = {int(i): [(1, 0.0)] for i in np.where(x0 < 1e-12)[0]}
+    Ab = O.CSR(npoin, 1, Ao.ia, Ao.ja, Ao.a.copy())
+    rr = np.zeros(npoin)
+    for i in sorted(dirbc, reverse=True):
+        Ab.dirichlet(i, 0.0, rr)
+    bb = b - rr
+    for i in dirbc:
+        bb[i] = 0.0
+    nb = np.linalg.norm(bb)
+    cg = X.ConjugateGradients(A)
+    cg.init(np.zeros(npoin), b, dirbc=dirbc, apply=True, pc="jacobi")
+    cg.solve(20000, 1e-10)
+    it_pcg = cg.it
+    cg.init(np.zeros(npoin), b, dirbc=dirbc, apply=True, pc="jacobi")
+    cg.solve_flexible(20000, 1e-10, pc=pc, inner=inner)
+    assert cg.converged
+    x = cg.solution()
+    true = np.linalg.norm(bb - Ab.mult(x))
+    _say("  pcg it=%d fcg it=%d inner=%d true/normb=%.3e"
+         % (it_pcg, cg.it, cg.inner_total, true / nb))
+    assert true < 2e-10 * nb
+    if pc == "jacobi":
+        assert abs(cg.it - it_pcg) <= max(5, it_pcg // 50)
+    else:
+        assert cg.it < it_pcg

@@ -157,6 +157,18 @@ class ConjugateGradients:
         self.nouter = no.value
         return self.normr
 
+    def solve_flexible(self, maxit, tol, pc="jacobi", inner=20):
+        """FP64 flexible CG with a Jacobi or an FP32 inner-CG ("cg32")
+        preconditioner (no reference counterpart); returns ||r||."""
+        kinds = {"jacobi": 0, "cg32": 1}
+        nr, it, tin, cv = C.c_double(), C.c_int64(), C.c_int64(), C.c_int()
+        self.ctx._check(lib().xg_cg_solve_flexible(
+            self.ctx._h, kinds[pc], inner, maxit, tol, C.byref(nr), C.byref(it),
+            C.byref(tin), C.byref(cv)))
+        self.normr, self.it, self.converged = nr.value, it.value, bool(cv.value)
+        self.inner_total = tin.value
+        return self.normr
+
     def solution(self):
         x = np.empty(self.n)
         self.ctx._check(lib().xg_cg_solution(self.ctx._h, ptr(x)))

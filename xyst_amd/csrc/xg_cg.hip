@@ -64,6 +64,8 @@ struct CgSolver {
   float* da32 = nullptr;         // FP32 copy for the mixed-precision inner solve
   double *x = nullptr, *b = nullptr, *r = nullptr, *z = nullptr, *d = nullptr;
   double *p0 = nullptr, *p1 = nullptr, *q = nullptr;
+  double *q2 = nullptr;          // FCG: q of the previous iteration
+  double* fb = nullptr;          // FCG partial sums (2 x 4 x kMaxParts)
   float *r32 = nullptr, *z32 = nullptr, *p32a = nullptr, *p32b = nullptr,
         *q32 = nullptr, *d32 = nullptr, *e32 = nullptr;
   double* pq = nullptr;          // kMaxParts partials of (p,q) (or scratch)
@@ -357,6 +359,100 @@ __global__ void k_f32_from_f64(int64_t n, const double* __restrict__ s,
   if (i < n) d[i] = (float)s[i];
 }
 
+// ---- flexible CG (Notay's FCG(1)) ----
+//   z_k = M(r_k);  beta = -(z_k, q_{k-1}) / (p_{k-1}, q_{k-1});
+//   p_k = z_k + beta p_{k-1};  q_k = A p_k;  alpha = (p_k, r_k) / (p_k, q_k);
+//   x += alpha p_k;  r -= alpha q_k.
+// Partial-sum slots of the buffer of iteration k (fb + (k & 1) * 4 kMaxParts):
+// [0] (z,q_prev)  [1] (p,q)  [2] (p,r)  [3] (r,r), kMaxParts each.
+
+// partials of (z, q_prev)
+__global__ __launch_bounds__(kCgThreads) void k_fcg_dot(
+    int64_t n, const double* __restrict__ z, const double* __restrict__ qp,
+    double* __restrict__ fb) {
+  __shared__ double sm[kCgThreads / 64];
+  double v = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kCgThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kCgThreads)
+    v += z[i] * qp[i];
+  put_part(fb, blockIdx.x, v, sm);
+}
+
+// p_k = z + beta p_{k-1}, q_k = A p_k; partials of (p,q) and (p,r) into
+// the buffer of iteration k.  first != 0: beta = 0.  `cur` holds the
+// (z_k, q_{k-1}) partials, `prev` the (p,q) partials of iteration k-1 (the
+// buffers alternate by iteration parity, so nothing read is overwritten).
+__global__ __launch_bounds__(kCgThreads) void k_fcg_spmv(
+    int first, int64_t nrow, const int* __restrict__ ia, const int* __restrict__ ja,
+    const double* __restrict__ a, const double* __restrict__ z,
+    const double* __restrict__ pold, double* __restrict__ pnew,
+    double* __restrict__ q, const double* __restrict__ r,
+    const double* __restrict__ prev, double* __restrict__ cur, int nvp, int nsp) {
+  __shared__ double sm[kCgThreads / 64];
+  double beta = 0.0;
+  if (!first) {
+    const double zq = sum_parts(cur, nvp, sm);                 // (z, q_prev)
+    const double pq = sum_parts(prev + kMaxParts, nsp, sm);    // (p, q)_prev
+    beta = pq != 0.0 ? -zq / pq : 0.0;
+  }
+  const int lane = threadIdx.x & (kLanesPerRow - 1);
+  double spq = 0.0, spr = 0.0;
+  for (int64_t row = ((int64_t)blockIdx.x * kCgThreads + threadIdx.x) / kLanesPerRow;
+       row < nrow; row += (int64_t)gridDim.x * kCgThreads / kLanesPerRow) {
+    double acc = 0.0;
+    for (int k = ia[row] + lane; k < ia[row + 1]; k += kLanesPerRow) {
+      const int c = ja[k];
+      acc += a[k] * (z[c] + beta * pold[c]);
+    }
+    acc = row_sum8(acc);
+    if (lane == 0) {
+      const double pr = z[row] + beta * pold[row];
+      pnew[row] = pr;
+      q[row] = acc;
+      spq += pr * acc;
+      spr += pr * r[row];
+    }
+  }
+  put_part(cur + kMaxParts, blockIdx.x, spq, sm);
+  put_part(cur + 2 * kMaxParts, blockIdx.x, spr, sm);
+}
+
+// alpha = (p,r)/(p,q); x += alpha p; r -= alpha q; partials of (r,r)
+__global__ __launch_bounds__(kCgThreads) void k_fcg_update(
+    int64_t n, const double* __restrict__ p, const double* __restrict__ q,
+    double* __restrict__ r, double* __restrict__ x, double* __restrict__ fb, int nsp) {
+  __shared__ double sm[kCgThreads / 64];
+  const double PQ = sum_parts(fb + kMaxParts, nsp, sm);
+  const double PR = sum_parts(fb + 2 * kMaxParts, nsp, sm);
+  const double alpha = PQ != 0.0 ? PR / PQ : 0.0;
+  double rr = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kCgThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kCgThreads) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    rr += ri * ri;
+  }
+  put_part(fb + 3 * kMaxParts, blockIdx.x, rr, sm);
+}
+
+// z = r / d (Jacobi preconditioner of the flexible solve)
+__global__ __launch_bounds__(kCgThreads) void k_jacobi_apply(
+    int64_t n, const double* __restrict__ r, const double* __restrict__ d,
+    double* __restrict__ z) {
+  for (int64_t i = (int64_t)blockIdx.x * kCgThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kCgThreads)
+    z[i] = r[i] / d[i];
+}
+
+// z = (double)e
+__global__ __launch_bounds__(kCgThreads) void k_f64_from_f32(
+    int64_t n, const float* __restrict__ e, double* __restrict__ z) {
+  for (int64_t i = (int64_t)blockIdx.x * kCgThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kCgThreads)
+    z[i] = (double)e[i];
+}
+
 // --------------------------------------------------------------- host side --
 
 int grid_rows(int64_t nrow) {
@@ -480,6 +576,8 @@ int xg_cg_matrix(xg_ctx* c, int ncomp, int64_t npoin, const size_t* ia,
       (rc = dev_alloc(c, L, (void**)&s->p0, n * 8)) ||
       (rc = dev_alloc(c, L, (void**)&s->p1, n * 8)) ||
       (rc = dev_alloc(c, L, (void**)&s->q, n * 8)) ||
+      (rc = dev_alloc(c, L, (void**)&s->q2, n * 8)) ||
+      (rc = dev_alloc(c, L, (void**)&s->fb, sizeof(double) * 8 * kMaxParts)) ||
       (rc = dev_alloc(c, L, (void**)&s->r32, n * 4)) ||
       (rc = dev_alloc(c, L, (void**)&s->z32, n * 4)) ||
       (rc = dev_alloc(c, L, (void**)&s->p32a, n * 4)) ||
@@ -720,6 +818,81 @@ int xg_cg_solve_mixed(xg_ctx* c, int64_t maxit, double tol, double inner_tol,
   if (iters) *iters = total;
   if (nouter) *nouter = outer;
   if (converged) *converged = conv;
+  s->ready = false;
+  s->restore = s->dev_bc;
+  return XG_OK;
+}
+
+// Flexible CG (no counterpart in the reference): Notay's FCG(1) in FP64 with
+// a preconditioner that may change from one application to the next.
+// pc_kind 0: Jacobi; 1: `inner` iterations of FP32 Jacobi-PCG on A e = r
+// from e = 0 (nested Krylov: the FP64 outer recurrence keeps its search
+// directions, so the inner solves are not restarts of the outer one).  Stops
+// when the true-recurrence residual ||r|| < tol * ||b|| or after maxit outer
+// iterations.
+int xg_cg_solve_flexible(xg_ctx* c, int pc_kind, int64_t inner, int64_t maxit,
+                         double tol, double* normr, int64_t* iters,
+                         int64_t* inner_total, int* converged) {
+  int rc = check_cg(c);
+  if (rc) return rc;
+  CgSolver* s = c->cg;
+  if (maxit < 1 || !(tol > 0.0) || (pc_kind != 0 && pc_kind != 1) ||
+      (pc_kind == 1 && inner < 1))
+    return fail(c, XG_EINVAL, "xg_cg_solve_flexible: bad arguments");
+  XG_HIP(c, hipSetDevice(c->device));
+  if (s->restore && (rc = restore_values(c))) return rc;
+  const int64_t n = s->nrow;
+  const int gr = grid_rows(n), gv = grid_vec(n);
+  hipLaunchKernelGGL(k_cg_residual, dim3(gr), dim3(kCgThreads), 0, c->stream, n,
+                     s->dia, s->dja, s->da, s->x, s->b, s->r, s->d, s->pc);
+  hipLaunchKernelGGL(k_f32_from_f64, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n,
+                     s->d, s->d32);
+  hipLaunchKernelGGL(k_cg_init, dim3(gv), dim3(kCgThreads), 0, c->stream, n, s->b,
+                     s->r, s->d, s->z, s->p0, s->rzb, s->pq);
+  hipLaunchKernelGGL(k_cg_setup_final, dim3(1), dim3(kCgThreads), 0, c->stream, s->pq,
+                     s->rzb, gv, s->st);
+  if ((rc = read_state(c))) return rc;
+  const double normb = s->hst->normb;
+  const double target = tol * (normb > 1.0e-14 ? normb : 1.0);
+  double nr = sqrt(s->hst->rr);
+  int64_t k = 0, tin = 0;
+  double* P[2] = {s->p0, s->p1};
+  double* Q[2] = {s->q, s->q2};
+  for (; k < maxit && !(nr < target); ++k) {
+    double* Bk = s->fb + (k & 1) * 4 * kMaxParts;
+    double* Bp = s->fb + ((k + 1) & 1) * 4 * kMaxParts;
+    if (pc_kind == 0) {
+      hipLaunchKernelGGL(k_jacobi_apply, dim3(gv), dim3(kCgThreads), 0, c->stream, n,
+                         s->r, s->d, s->z);
+    } else {
+      hipLaunchKernelGGL(k_cg32_init, dim3(gv), dim3(kCgThreads), 0, c->stream, n,
+                         s->r, s->d32, s->r32, s->z32, s->p32a, s->e32, s->rzb);
+      hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, c->stream, s->st);
+      if ((rc = run_cg<float>(c, s->da32, s->z32, s->p32a, s->p32b, s->q32, s->d32,
+                              s->r32, s->e32, inner, 0.0, 1.0e-300)))
+        return rc;
+      tin += s->hst->it;
+      hipLaunchKernelGGL(k_f64_from_f32, dim3(gv), dim3(kCgThreads), 0, c->stream, n,
+                         s->e32, s->z);
+    }
+    if (k > 0)
+      hipLaunchKernelGGL(k_fcg_dot, dim3(gv), dim3(kCgThreads), 0, c->stream, n, s->z,
+                         Q[(k + 1) & 1], Bk);
+    hipLaunchKernelGGL(k_fcg_spmv, dim3(gr), dim3(kCgThreads), 0, c->stream, k == 0 ? 1 : 0,
+                       n, s->dia, s->dja, s->da, s->z, P[(k + 1) & 1], P[k & 1],
+                       Q[k & 1], s->r, Bp, Bk, gv, gr);
+    hipLaunchKernelGGL(k_fcg_update, dim3(gv), dim3(kCgThreads), 0, c->stream, n,
+                       P[k & 1], Q[k & 1], s->r, s->x, Bk, gr);
+    hipLaunchKernelGGL(k_sum_to_state, dim3(1), dim3(kCgThreads), 0, c->stream,
+                       Bk + 3 * kMaxParts, gv, s->st);
+    XG_HIP(c, hipGetLastError());
+    if ((rc = read_state(c))) return rc;
+    nr = sqrt(s->hst->rr);
+  }
+  if (normr) *normr = nr;
+  if (iters) *iters = k;
+  if (inner_total) *inner_total = tin;
+  if (converged) *converged = nr < target ? 1 : 0;
   s->ready = false;
   s->restore = s->dev_bc;
   return XG_OK;
