@@ -277,14 +277,47 @@ int xg_cg_solve_mixed(xg_ctx* ctx, int64_t maxit, double tol,
                       int64_t* iters, int64_t* nouter, int* converged);
 
 /* Flexible CG (no counterpart in the reference): Notay's FCG(1) in FP64 on
- * the system set up by xg_cg_init, with preconditioner pc_kind 0 (Jacobi) or
+ * the system set up by xg_cg_init, with preconditioner pc_kind 0 (Jacobi),
  * 1 (`inner` iterations of FP32 Jacobi-PCG from a zero guess, i.e. FP32
- * inner solves inside an FP64 outer Krylov recurrence).  Stops when
- * ||r|| < tol*normb or after maxit outer iterations; *iters outer,
- * *inner_total inner iterations. */
+ * inner solves inside an FP64 outer Krylov recurrence) or 2 (the learned
+ * preconditioner, xg_gnn_*).  Stops when ||r|| < tol*normb or after maxit
+ * outer iterations; *iters outer iterations, *inner_total the inner
+ * iterations (pc 1) or the Jacobi fallbacks (pc 2). */
 int xg_cg_solve_flexible(xg_ctx* ctx, int pc_kind, int64_t inner, int64_t maxit,
                          double tol, double* normr, int64_t* iters,
                          int64_t* inner_total, int* converged);
+
+/* ---- learned graph preconditioner (no counterpart in the reference) -------- */
+
+/* A 3-layer message-passing network over the graph of the CG matrix (32
+ * channels, per-node 64->32 layers on BF16 MFMA with FP32 accumulation) that
+ * maps a residual r to an approximate error z; used by xg_cg_solve_flexible
+ * with pc_kind 2, with the Jacobi z wherever (z, r) <= 0.  Parameters are a
+ * flat array of XG_GNN_NPARAM = 6337 values: we[32] be[32], for each layer
+ * W[32][64] b[32], wo[32] bo[1].  n, nnz: rows and nonzeros of the system
+ * matrix it will precondition. */
+int xg_gnn_create(xg_ctx* ctx, int64_t n, int64_t nnz);
+int xg_gnn_set_params(xg_ctx* ctx, const double* params, int64_t nparam);
+/* params and/or the accumulated gradient (either may be null) */
+int xg_gnn_get_params(xg_ctx* ctx, double* params, double* grad, int64_t nparam);
+/* Adam step with the gradient accumulated since the last step (averaged over
+ * the pairs it holds), which is then cleared. */
+int xg_gnn_adam(xg_ctx* ctx, double lr, double beta1, double beta2, double eps);
+/* Normalised matrix of the current CG system (after xg_cg_init/xg_cg_setup
+ * with pc = jacobi). */
+int xg_gnn_prepare(xg_ctx* ctx);
+/* z = network(r), device vectors of the CG system's size. */
+int xg_gnn_apply_dev(xg_ctx* ctx, const double* r, double* z);
+/* Loss of the pair (r, e), A e = r: the energy-norm error of z = network(r)
+ * relative to that of e, (z-e)^T A (z-e) / (e^T r); its gradient is added to
+ * the accumulated gradient; z receives the network output. */
+int xg_gnn_loss_grad_dev(xg_ctx* ctx, const double* r, const double* e, double* z,
+                         double* loss);
+/* Trains on residual/error pairs of solves of the current CG system (see
+ * xg_cg.hip): restarted Jacobi-PCG from x = 0 in runs of `stride`
+ * iterations, one pair per restart, an Adam step every `batch` pairs. */
+int xg_gnn_train_system(xg_ctx* ctx, int64_t stride, int64_t max_pairs, int batch,
+                        double lr, double* mean_loss, int64_t* npairs);
 
 /* Releases the solver's device memory. */
 int xg_cg_destroy(xg_ctx* ctx);

@@ -64,6 +64,15 @@ SIG = {
                                     C.POINTER(I64), C.POINTER(C.c_int)]),
     "xg_cg_solve_flexible": (C.c_int, [VP, C.c_int, I64, I64, D, C.POINTER(D),
                                        C.POINTER(I64), C.POINTER(I64), C.POINTER(C.c_int)]),
+    "xg_gnn_create": (C.c_int, [VP, I64, I64]),
+    "xg_gnn_set_params": (C.c_int, [VP, VP, I64]),
+    "xg_gnn_get_params": (C.c_int, [VP, VP, VP, I64]),
+    "xg_gnn_adam": (C.c_int, [VP, D, D, D, D]),
+    "xg_gnn_prepare": (C.c_int, [VP]),
+    "xg_gnn_apply_dev": (C.c_int, [VP, VP, VP]),
+    "xg_gnn_loss_grad_dev": (C.c_int, [VP, VP, VP, VP, C.POINTER(D)]),
+    "xg_gnn_train_system": (C.c_int, [VP, I64, I64, C.c_int, D, C.POINTER(D),
+                                      C.POINTER(I64)]),
     "xg_cg_destroy": (C.c_int, [VP]),
 }
 

@@ -158,9 +158,11 @@ class ConjugateGradients:
         return self.normr
 
     def solve_flexible(self, maxit, tol, pc="jacobi", inner=20):
-        """FP64 flexible CG with a Jacobi or an FP32 inner-CG ("cg32")
-        preconditioner (no reference counterpart); returns ||r||."""
-        kinds = {"jacobi": 0, "cg32": 1}
+        """FP64 flexible CG with a Jacobi, an FP32 inner-CG ("cg32") or the
+        learned ("gnn", xyst_amd.gnn) preconditioner (no reference
+        counterpart); returns ||r||.  inner_total: inner iterations (cg32) or
+        Jacobi fallbacks (gnn)."""
+        kinds = {"jacobi": 0, "cg32": 1, "gnn": 2}
         nr, it, tin, cv = C.c_double(), C.c_int64(), C.c_int64(), C.c_int()
         self.ctx._check(lib().xg_cg_solve_flexible(
             self.ctx._h, kinds[pc], inner, maxit, tol, C.byref(nr), C.byref(it),

@@ -28,6 +28,7 @@
 
 #include "xg_csr.hpp"
 #include "xg_ctx.hpp"
+#include "xg_gnn.hpp"
 
 using namespace xgd;
 
@@ -66,6 +67,7 @@ struct CgSolver {
   double *p0 = nullptr, *p1 = nullptr, *q = nullptr;
   double *q2 = nullptr;          // FCG: q of the previous iteration
   double* fb = nullptr;          // FCG partial sums (2 x 4 x kMaxParts)
+  double *xs = nullptr, *ebuf = nullptr;   // reference solution, error (training)
   float *r32 = nullptr, *z32 = nullptr, *p32a = nullptr, *p32b = nullptr,
         *q32 = nullptr, *d32 = nullptr, *e32 = nullptr;
   double* pq = nullptr;          // kMaxParts partials of (p,q) (or scratch)
@@ -453,6 +455,15 @@ __global__ __launch_bounds__(kCgThreads) void k_f64_from_f32(
     z[i] = (double)e[i];
 }
 
+// e = a - b
+__global__ __launch_bounds__(kCgThreads) void k_sub(
+    int64_t n, const double* __restrict__ a, const double* __restrict__ b,
+    double* __restrict__ e) {
+  for (int64_t i = (int64_t)blockIdx.x * kCgThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kCgThreads)
+    e[i] = a[i] - b[i];
+}
+
 // --------------------------------------------------------------- host side --
 
 int grid_rows(int64_t nrow) {
@@ -578,6 +589,8 @@ int xg_cg_matrix(xg_ctx* c, int ncomp, int64_t npoin, const size_t* ia,
       (rc = dev_alloc(c, L, (void**)&s->q, n * 8)) ||
       (rc = dev_alloc(c, L, (void**)&s->q2, n * 8)) ||
       (rc = dev_alloc(c, L, (void**)&s->fb, sizeof(double) * 8 * kMaxParts)) ||
+      (rc = dev_alloc(c, L, (void**)&s->xs, n * 8)) ||
+      (rc = dev_alloc(c, L, (void**)&s->ebuf, n * 8)) ||
       (rc = dev_alloc(c, L, (void**)&s->r32, n * 4)) ||
       (rc = dev_alloc(c, L, (void**)&s->z32, n * 4)) ||
       (rc = dev_alloc(c, L, (void**)&s->p32a, n * 4)) ||
@@ -836,7 +849,7 @@ int xg_cg_solve_flexible(xg_ctx* c, int pc_kind, int64_t inner, int64_t maxit,
   int rc = check_cg(c);
   if (rc) return rc;
   CgSolver* s = c->cg;
-  if (maxit < 1 || !(tol > 0.0) || (pc_kind != 0 && pc_kind != 1) ||
+  if (maxit < 1 || !(tol > 0.0) || pc_kind < 0 || pc_kind > 2 ||
       (pc_kind == 1 && inner < 1))
     return fail(c, XG_EINVAL, "xg_cg_solve_flexible: bad arguments");
   XG_HIP(c, hipSetDevice(c->device));
@@ -851,6 +864,9 @@ int xg_cg_solve_flexible(xg_ctx* c, int pc_kind, int64_t inner, int64_t maxit,
                      s->r, s->d, s->z, s->p0, s->rzb, s->pq);
   hipLaunchKernelGGL(k_cg_setup_final, dim3(1), dim3(kCgThreads), 0, c->stream, s->pq,
                      s->rzb, gv, s->st);
+  if (pc_kind == 2 &&
+      (rc = xg_gnn_prepare_internal(c, s->dia, s->dja, s->da, s->d, n)))
+    return rc;
   if ((rc = read_state(c))) return rc;
   const double normb = s->hst->normb;
   const double target = tol * (normb > 1.0e-14 ? normb : 1.0);
@@ -864,6 +880,11 @@ int xg_cg_solve_flexible(xg_ctx* c, int pc_kind, int64_t inner, int64_t maxit,
     if (pc_kind == 0) {
       hipLaunchKernelGGL(k_jacobi_apply, dim3(gv), dim3(kCgThreads), 0, c->stream, n,
                          s->r, s->d, s->z);
+    } else if (pc_kind == 2) {
+      if ((rc = xg_gnn_apply_internal(c, s->dia, s->dja, s->r, s->d, s->z))) return rc;
+      int used = 0;
+      if ((rc = xg_gnn_fallback_flag(c, &used))) return rc;
+      tin += used;
     } else {
       hipLaunchKernelGGL(k_cg32_init, dim3(gv), dim3(kCgThreads), 0, c->stream, n,
                          s->r, s->d32, s->r32, s->z32, s->p32a, s->e32, s->rzb);
@@ -895,6 +916,116 @@ int xg_cg_solve_flexible(xg_ctx* c, int pc_kind, int64_t inner, int64_t maxit,
   if (converged) *converged = nr < target ? 1 : 0;
   s->ready = false;
   s->restore = s->dev_bc;
+  return XG_OK;
+}
+
+// ---- learned preconditioner entry points that need the CG system ----
+
+static int check_gnn(xg_ctx* c) {
+  int rc = check_cg(c);
+  if (rc) return rc;
+  if (!c->gnn) return fail(c, XG_ENOMESH, "no learned preconditioner (xg_gnn_create)");
+  return XG_OK;
+}
+
+// e.r on the device, read back
+static int dot_dev(xg_ctx* c, const double* a, const double* b, double* out) {
+  CgSolver* s = c->cg;
+  const int gv = grid_vec(s->nrow);
+  hipLaunchKernelGGL(k_fcg_dot, dim3(gv), dim3(kCgThreads), 0, c->stream, s->nrow, a, b,
+                     s->fb);
+  hipLaunchKernelGGL(k_sum_to_state, dim3(1), dim3(kCgThreads), 0, c->stream, s->fb, gv,
+                     s->st);
+  int rc = read_state(c);
+  if (rc) return rc;
+  *out = s->hst->rr;
+  return XG_OK;
+}
+
+int xg_gnn_prepare(xg_ctx* c) {
+  int rc = check_gnn(c);
+  if (rc) return rc;
+  CgSolver* s = c->cg;
+  XG_HIP(c, hipSetDevice(c->device));
+  return xg_gnn_prepare_internal(c, s->dia, s->dja, s->da, s->d, s->nrow);
+}
+
+int xg_gnn_apply_dev(xg_ctx* c, const double* r, double* z) {
+  int rc = check_gnn(c);
+  if (rc) return rc;
+  if (!r || !z) return fail(c, XG_EINVAL, "xg_gnn_apply_dev: null vector");
+  CgSolver* s = c->cg;
+  XG_HIP(c, hipSetDevice(c->device));
+  return xg_gnn_forward_internal(c, s->dia, s->dja, r, s->d, z);
+}
+
+int xg_gnn_loss_grad_dev(xg_ctx* c, const double* r, const double* e, double* z,
+                         double* loss) {
+  int rc = check_gnn(c);
+  if (rc) return rc;
+  if (!r || !e || !z) return fail(c, XG_EINVAL, "xg_gnn_loss_grad_dev: null vector");
+  CgSolver* s = c->cg;
+  XG_HIP(c, hipSetDevice(c->device));
+  double etr;
+  if ((rc = dot_dev(c, e, r, &etr))) return rc;
+  if (!(etr > 0.0)) return fail(c, XG_EINVAL, "xg_gnn_loss_grad_dev: e.r must be positive");
+  if ((rc = xg_gnn_forward_internal(c, s->dia, s->dja, r, s->d, z))) return rc;
+  return xg_gnn_backward_internal(c, s->dia, s->dja, s->da, r, etr, z, loss);
+}
+
+// Training pairs from the system set up by xg_cg_init (with its Dirichlet
+// conditions applied): the reference solution x* by Jacobi-PCG to 1e-12,
+// then restarted PCG from x = 0 in runs of `stride` iterations; at every
+// restart (r = b - A x, e = x* - x) is one pair: forward, backward, and an
+// Adam step every `batch` pairs.  Returns the mean pair loss and the count.
+int xg_gnn_train_system(xg_ctx* c, int64_t stride, int64_t max_pairs, int batch,
+                        double lr, double* mean_loss, int64_t* npairs) {
+  int rc = check_gnn(c);
+  if (rc) return rc;
+  if (stride < 1 || max_pairs < 1 || batch < 1 || !(lr > 0.0))
+    return fail(c, XG_EINVAL, "xg_gnn_train_system: bad arguments");
+  CgSolver* s = c->cg;
+  XG_HIP(c, hipSetDevice(c->device));
+  const size_t n = (size_t)s->nrow;
+  const int gv = grid_vec(s->nrow);
+  XG_HIP(c, hipMemsetAsync(s->x, 0, n * 8, c->stream));
+  if ((rc = xg_cg_setup(c, nullptr))) return rc;
+  const double normb = s->hst->normb;
+  const double nb = normb > 1.0e-14 ? normb : 1.0;
+  if ((rc = run_cg<double>(c, s->da, s->z, s->p0, s->p1, s->q, s->d, s->r, s->x,
+                           20 * (int64_t)std::cbrt((double)n) * 10, 1.0e-12 * nb,
+                           2.220446049250313e-16)))
+    return rc;
+  XG_HIP(c, hipMemcpyAsync(s->xs, s->x, n * 8, hipMemcpyDeviceToDevice, c->stream));
+  XG_HIP(c, hipMemsetAsync(s->x, 0, n * 8, c->stream));
+  if ((rc = xg_gnn_prepare_internal(c, s->dia, s->dja, s->da, s->d, s->nrow))) return rc;
+  double lsum = 0.0;
+  int64_t np = 0;
+  double* z = xg_gnn_zbuf(c);
+  while (np < max_pairs) {
+    if ((rc = xg_cg_setup(c, nullptr))) return rc;
+    if (sqrt(s->hst->rr) < 1.0e-10 * nb) break;
+    hipLaunchKernelGGL(k_sub, dim3(gv), dim3(kCgThreads), 0, c->stream, s->nrow, s->xs,
+                       s->x, s->ebuf);
+    double etr;
+    if ((rc = dot_dev(c, s->ebuf, s->r, &etr))) return rc;
+    if (!(etr > 0.0)) break;
+    double L = 0.0;
+    if ((rc = xg_gnn_forward_internal(c, s->dia, s->dja, s->r, s->d, z)) ||
+        (rc = xg_gnn_backward_internal(c, s->dia, s->dja, s->da, s->r, etr, z, &L)))
+      return rc;
+    lsum += L;
+    ++np;
+    if (np % batch == 0 && (rc = xg_gnn_adam(c, lr, 0.9, 0.999, 1.0e-8))) return rc;
+    if ((rc = run_cg<double>(c, s->da, s->z, s->p0, s->p1, s->q, s->d, s->r, s->x,
+                             stride, 0.0, 2.220446049250313e-16)))
+      return rc;
+  }
+  if ((rc = xg_gnn_adam(c, lr, 0.9, 0.999, 1.0e-8))) return rc;
+  s->ready = false;
+  s->restore = s->dev_bc;
+  if (mean_loss) *mean_loss = np ? lsum / np : 0.0;
+  if (npairs) *npairs = np;
   return XG_OK;
 }
 

@@ -16,6 +16,7 @@ struct DevArr {
 struct CgSolver;   // xg_cg.hip
 struct KozakMesh;  // xg_kozak.hip
 struct ChoMesh;    // xg_chocg.hip
+struct GnnNet;     // xg_gnn.hip
 
 struct xg_ctx {
   int device = 0;
@@ -64,6 +65,8 @@ struct xg_ctx {
   KozakMesh* koz = nullptr;
   // ChoCG edge layout (xg_chocg.hip)
   ChoMesh* cho = nullptr;
+  // learned preconditioner (xg_gnn.hip)
+  GnnNet* gnn = nullptr;
 };
 
 void xg_kozak_free(xg_ctx* c);   // xg_kozak.hip

@@ -24,6 +24,7 @@ using xg::Cfg;
 
 #include "xg_blocks.hpp"
 #include "xg_ctx.hpp"
+#include "xg_gnn.hpp"
 
 namespace {
 
@@ -549,6 +550,7 @@ int xg_ctx_destroy(xg_ctx* c) {
   if (c->cg) (void)xg_cg_destroy(c);
   if (c->koz) xg_kozak_free(c);
   if (c->cho) xg_chocg_free(c);
+  if (c->gnn) xg_gnn_free(c);
   delete c;
   return XG_OK;
 }
