@@ -3,16 +3,18 @@
 The network has no counterpart in the reference; this numpy restatement
 (float64) is the checker of the device forward and backward kernels.
 
-Network (per node i, 32 channels, A the system matrix with diagonal d):
+Network (per node i, 32 channels, A the system matrix with diagonal d; |d|
+is used where a square root is taken, as the ChoCG matrix is -K, negative
+definite):
   s    = ||D^-1/2 r|| / sqrt(N)                     (scale, one scalar)
-  x_i  = r_i / (sqrt(d_i) s)
+  x_i  = r_i / (sqrt|d_i| s)
   h0_i = relu(we x_i + be)                          (encoder, 1 -> 32)
   for l = 1..3:
-    m_i  = sum_j Ahat_ij h_{l-1,j},  Ahat = D^-1/2 A D^-1/2
+    m_i  = sum_j Ahat_ij h_{l-1,j},  Ahat = |D|^-1/2 A |D|^-1/2
     a_i  = W_l [h_{l-1,i}; m_i] + b_l               (64 -> 32)
     h_l  = h_{l-1} + relu(a_i)
   y_i  = wo . h3_i + bo                             (decoder, 32 -> 1)
-  z_i  = s y_i / sqrt(d_i)
+  z_i  = s y_i / sqrt|d_i|
 
 Loss of a pair (r, e) with A e = r, the energy-norm error of z relative to
 that of e:  L = (z - e)^T A (z - e) / (e^T r) = (z^T A z - 2 z^T r) / (e^T r) + 1.
@@ -36,15 +38,15 @@ def init_params(seed=0, c=C):
 
 
 def _ahat(A_dense, d):
-    s = 1.0 / np.sqrt(d)
+    s = 1.0 / np.sqrt(np.abs(d))
     return A_dense * s[:, None] * s[None, :]
 
 
 def forward(p, A_dense, d, r):
     """Returns z and the cache needed by backward."""
     n = r.size
-    sq = np.sqrt(d)
-    s = np.sqrt(np.sum(r * r / d) / n)
+    sq = np.sqrt(np.abs(d))
+    s = np.sqrt(np.sum(r * r / np.abs(d)) / n)
     x = r / (sq * s)
     Ah = _ahat(A_dense, d)
     pre0 = np.outer(x, p["we"]) + p["be"]

@@ -968,7 +968,8 @@ int xg_gnn_loss_grad_dev(xg_ctx* c, const double* r, const double* e, double* z,
   XG_HIP(c, hipSetDevice(c->device));
   double etr;
   if ((rc = dot_dev(c, e, r, &etr))) return rc;
-  if (!(etr > 0.0)) return fail(c, XG_EINVAL, "xg_gnn_loss_grad_dev: e.r must be positive");
+  if (etr == 0.0 || !std::isfinite(etr))
+    return fail(c, XG_EINVAL, "xg_gnn_loss_grad_dev: e.r must be finite and nonzero");
   if ((rc = xg_gnn_forward_internal(c, s->dia, s->dja, r, s->d, z))) return rc;
   return xg_gnn_backward_internal(c, s->dia, s->dja, s->da, r, etr, z, loss);
 }
@@ -1009,7 +1010,7 @@ int xg_gnn_train_system(xg_ctx* c, int64_t stride, int64_t max_pairs, int batch,
                        s->x, s->ebuf);
     double etr;
     if ((rc = dot_dev(c, s->ebuf, s->r, &etr))) return rc;
-    if (!(etr > 0.0)) break;
+    if (etr == 0.0 || !std::isfinite(etr)) break;
     double L = 0.0;
     if ((rc = xg_gnn_forward_internal(c, s->dia, s->dja, s->r, s->d, z)) ||
         (rc = xg_gnn_backward_internal(c, s->dia, s->dja, s->da, s->r, etr, z, &L)))

@@ -3,12 +3,13 @@
 // 32 channels, applied inside the flexible CG of xg_cg.hip, trained on the
 // device by the forward and backward kernels below.
 //
-//   s = ||D^-1/2 r|| / sqrt(N),  x_i = r_i / (sqrt(d_i) s)
+//   s = ||D^-1/2 r|| / sqrt(N),  x_i = r_i / (sqrt|d_i| s)
 //   h0 = relu(we x + be)                                   (1 -> 32)
-//   l = 1..3:  m = Ahat h,  Ahat = D^-1/2 A D^-1/2          (SpMM, 32 channels)
+//   l = 1..3:  m = Ahat h,  Ahat = |D|^-1/2 A |D|^-1/2      (SpMM, 32 channels)
 //              h += relu(W_l [h; m] + b_l)                 (64 -> 32, BF16 MFMA,
 //                                                           FP32 accumulation)
-//   y = wo . h3 + bo,  z_i = s y_i / sqrt(d_i)
+//   y = wo . h3 + bo,  z_i = s y_i / sqrt|d_i|
+// (|d| because the ChoCG matrix -K is negative definite.)
 // Activations are stored in BF16, sums are FP32.  Training minimises the
 // energy-norm error of z relative to that of the true error e (A e = r):
 //   L = (z^T A z - 2 z^T r) / (e^T r) + 1,  dL/dz = 2 (A z - r) / (e^T r).
@@ -69,19 +70,19 @@ __device__ inline double sum_parts(const double* __restrict__ p, int n, double* 
   return block_sum(v, sm);
 }
 
-// Ahat_ik = a_ik / sqrt(d_i d_k) (FP32) and sqrt(d)
+// Ahat_ik = a_ik / sqrt|d_i d_k| (FP32) and sqrt|d|
 __global__ __launch_bounds__(kThreads) void k_gnn_prep(
     int64_t nrow, const int* __restrict__ ia, const int* __restrict__ ja,
     const double* __restrict__ a, const double* __restrict__ d,
     float* __restrict__ ah, float* __restrict__ sqd) {
   int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (i >= nrow) return;
-  const double di = d[i];
+  const double di = fabs(d[i]);
   sqd[i] = (float)sqrt(di);
-  for (int k = ia[i]; k < ia[i + 1]; ++k) ah[k] = (float)(a[k] / sqrt(di * d[ja[k]]));
+  for (int k = ia[i]; k < ia[i + 1]; ++k) ah[k] = (float)(a[k] / sqrt(di * fabs(d[ja[k]])));
 }
 
-// partials of sum r_i^2 / d_i
+// partials of sum r_i^2 / |d_i|
 __global__ __launch_bounds__(kThreads) void k_gnn_scale_part(
     int64_t n, const double* __restrict__ r, const double* __restrict__ d,
     double* __restrict__ part) {
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(kThreads) void k_gnn_scale_part(
   double v = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kThreads)
-    v += r[i] * r[i] / d[i];
+    v += r[i] * r[i] / fabs(d[i]);
   v = block_sum(v, sm);
   if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
@@ -392,18 +393,25 @@ __global__ void k_gnn_weights_bf16(const float* __restrict__ P, uint16_t* __rest
   Wb[o] = f2bf(P[kOffW0 + l * kLayerSz + k]);
 }
 
-// z = r / d where the learned z is no descent direction ((z, r) <= 0 or not
-// finite): the Jacobi fallback; every workgroup decides alike from the partials
+// z = r / d where the learned z is no descent direction: (z, r) must have
+// the sign of the Jacobi direction's (r/d, r) (negative for the ChoCG matrix
+// -K) and be finite; every workgroup decides alike from the partials
 __global__ __launch_bounds__(kThreads) void k_gnn_zr_part(
     int64_t n, const double* __restrict__ z, const double* __restrict__ r,
-    double* __restrict__ part) {
+    const double* __restrict__ d, double* __restrict__ part) {
   __shared__ double sm[kThreads / 64];
-  double v = 0.0;
+  double v = 0.0, w = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kThreads)
+       i += (int64_t)gridDim.x * kThreads) {
     v += z[i] * r[i];
+    w += r[i] * r[i] / d[i];
+  }
   v = block_sum(v, sm);
-  if (threadIdx.x == 0) part[blockIdx.x] = v;
+  w = block_sum(w, sm);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = v;
+    part[kParts + blockIdx.x] = w;
+  }
 }
 
 __global__ __launch_bounds__(kThreads) void k_gnn_fallback(
@@ -412,7 +420,8 @@ __global__ __launch_bounds__(kThreads) void k_gnn_fallback(
     double* __restrict__ sc) {
   __shared__ double sm[kThreads / 64];
   const double zr = sum_parts(part, nparts, sm);
-  const bool bad = !(zr > 0.0) || !isfinite(zr);
+  const double jr = sum_parts(part + kParts, nparts, sm);
+  const bool bad = !(zr * jr > 0.0) || !isfinite(zr);
   if (blockIdx.x == 0 && threadIdx.x == 0) sc[2] = bad ? 1.0 : 0.0;
   if (!bad) return;
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
@@ -576,7 +585,7 @@ int xg_gnn_apply_internal(xg_ctx* c, const int* ia, const int* ja, const double*
   if (rc) return rc;
   const int gs = grid_for(g->n);
   hipLaunchKernelGGL(k_gnn_zr_part, dim3(gs), dim3(kThreads), 0, c->stream, g->n, z, r,
-                     g->part);
+                     d, g->part);
   hipLaunchKernelGGL(k_gnn_fallback, dim3(gs), dim3(kThreads), 0, c->stream, g->n, z, r, d,
                      g->part, gs, g->sc);
   XG_HIP(c, hipGetLastError());
