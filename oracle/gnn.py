@@ -14,7 +14,8 @@ definite):
     a_i  = W_l [h_{l-1,i}; m_i] + b_l               (64 -> 32)
     h_l  = h_{l-1} + relu(a_i)
   y_i  = wo . h3_i + bo                             (decoder, 32 -> 1)
-  z_i  = s y_i / sqrt|d_i|
+  z_i  = s y_i / sqrt|d_i|,  or r_i / d_i on decoupled rows (a single
+         nonzero, the diagonal: the Dirichlet rows, where that is exact)
 
 Loss of a pair (r, e) with A e = r, the energy-norm error of z relative to
 that of e:  L = (z - e)^T A (z - e) / (e^T r) = (z^T A z - 2 z^T r) / (e^T r) + 1.
@@ -60,9 +61,17 @@ def forward(p, A_dense, d, r):
         cache["a"].append(a)
         cache["h"].append(h)
     y = h @ p["wo"] + p["bo"][0]
-    z = s * y / sq
+    dec = decoupled(A_dense)
+    z = np.where(dec, r / d, s * y / sq)
     cache["y"] = y
+    cache["dec"] = dec
     return z, cache
+
+
+def decoupled(A_dense):
+    """Rows whose only nonzero is the diagonal."""
+    nz = (A_dense != 0.0).sum(1)
+    return (nz == 1) & (np.diag(A_dense) != 0.0)
 
 
 def loss(A_dense, r, e, z):
@@ -74,7 +83,7 @@ def backward(p, cache, dz):
     """Gradients of the parameters given dL/dz (s is treated as a constant of
     the input, as the device does: it depends on r only)."""
     g = {}
-    dy = dz * cache["s"] / cache["sq"]
+    dy = np.where(cache["dec"], 0.0, dz * cache["s"] / cache["sq"])
     h3 = cache["h"][-1]
     g["wo"] = h3.T @ dy
     g["bo"] = np.array([dy.sum()])

@@ -106,6 +106,7 @@ def test_training_lowers_loss_and_fcg_converges():
         losses.append(L)
         _say("  system %d: mean loss %.4f over %d pairs" % (s, L, npairs))
         assert npairs > 5
+    assert min(losses) > -1e-6          # an energy-norm ratio: never negative
     assert losses[-1] < losses[0]
     b = (np.sin(2 * np.pi * x) * z + y) * 1e4
     cg.init(np.zeros(n), b, dirbc=dirbc, apply=True, pc="jacobi")

@@ -9,7 +9,9 @@
 //              h += relu(W_l [h; m] + b_l)                 (64 -> 32, BF16 MFMA,
 //                                                           FP32 accumulation)
 //   y = wo . h3 + bo,  z_i = s y_i / sqrt|d_i|
-// (|d| because the ChoCG matrix -K is negative definite.)
+// (|d| because the ChoCG matrix -K is negative definite.)  Decoupled rows
+// (the diagonal their only nonzero: the Dirichlet rows) take z_i = r_i / d_i,
+// which is exact there; the network acts on the coupled rows only.
 // Activations are stored in BF16, sums are FP32.  Training minimises the
 // energy-norm error of z relative to that of the true error e (A e = r):
 //   L = (z^T A z - 2 z^T r) / (e^T r) + 1,  dL/dz = 2 (A z - r) / (e^T r).
@@ -70,16 +72,21 @@ __device__ inline double sum_parts(const double* __restrict__ p, int n, double* 
   return block_sum(v, sm);
 }
 
-// Ahat_ik = a_ik / sqrt|d_i d_k| (FP32) and sqrt|d|
+// Ahat_ik = a_ik / sqrt|d_i d_k| (FP32), sqrt|d|, and the decoupled-row flag
 __global__ __launch_bounds__(kThreads) void k_gnn_prep(
     int64_t nrow, const int* __restrict__ ia, const int* __restrict__ ja,
     const double* __restrict__ a, const double* __restrict__ d,
-    float* __restrict__ ah, float* __restrict__ sqd) {
+    float* __restrict__ ah, float* __restrict__ sqd, uint8_t* __restrict__ dec) {
   int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (i >= nrow) return;
   const double di = fabs(d[i]);
   sqd[i] = (float)sqrt(di);
-  for (int k = ia[i]; k < ia[i + 1]; ++k) ah[k] = (float)(a[k] / sqrt(di * fabs(d[ja[k]])));
+  int nz = 0;
+  for (int k = ia[i]; k < ia[i + 1]; ++k) {
+    ah[k] = (float)(a[k] / sqrt(di * fabs(d[ja[k]])));
+    nz += a[k] != 0.0 ? 1 : 0;
+  }
+  dec[i] = (nz == 1 && di != 0.0) ? 1 : 0;
 }
 
 // partials of sum r_i^2 / |d_i|
@@ -167,13 +174,18 @@ __global__ __launch_bounds__(kThreads) void k_gnn_mlp(
   }
 }
 
-// y = wo . h3 + bo; z = s y / sqrt(d) (FP64 out)
+// y = wo . h3 + bo; z = s y / sqrt|d| (FP64 out), r/d on decoupled rows
 __global__ __launch_bounds__(kThreads) void k_gnn_decode(
     int64_t n, const uint16_t* __restrict__ H3, const float* __restrict__ P,
-    const float* __restrict__ sqd, const double* __restrict__ sc,
-    double* __restrict__ z) {
+    const float* __restrict__ sqd, const uint8_t* __restrict__ dec,
+    const double* __restrict__ r, const double* __restrict__ d,
+    const double* __restrict__ sc, double* __restrict__ z) {
   const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
+  if (dec[i]) {
+    z[i] = r[i] / d[i];
+    return;
+  }
   float y = P[kOffBo];
 #pragma unroll
   for (int c = 0; c < kC; ++c) y += P[kOffWo + c] * bf2f(H3[i * kC + c]);
@@ -185,7 +197,8 @@ __global__ __launch_bounds__(kThreads) void k_gnn_dz(
     int64_t nrow, const int* __restrict__ ia, const int* __restrict__ ja,
     const double* __restrict__ a, const double* __restrict__ z,
     const double* __restrict__ r, double etr, const float* __restrict__ sqd,
-    const double* __restrict__ sc, float* __restrict__ dy, double* __restrict__ part) {
+    const uint8_t* __restrict__ dec, const double* __restrict__ sc,
+    float* __restrict__ dy, double* __restrict__ part) {
   __shared__ double sm[kThreads / 64];
   double zaz = 0.0, zr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < nrow;
@@ -193,7 +206,7 @@ __global__ __launch_bounds__(kThreads) void k_gnn_dz(
     double az = 0.0;
     for (int k = ia[i]; k < ia[i + 1]; ++k) az += a[k] * z[ja[k]];
     const double dz = 2.0 * (az - r[i]) / etr;
-    dy[i] = (float)(dz * sc[0] / (double)sqd[i]);
+    dy[i] = dec[i] ? 0.0f : (float)(dz * sc[0] / (double)sqd[i]);
     zaz += z[i] * az;
     zr += z[i] * r[i];
   }
@@ -444,6 +457,7 @@ struct GnnNet {
   float *P = nullptr, *G = nullptr, *m1 = nullptr, *m2 = nullptr;
   uint16_t* Wb = nullptr;
   float *ah = nullptr, *sqd = nullptr, *xin = nullptr, *dy = nullptr;
+  uint8_t* dec = nullptr;     // decoupled (Dirichlet) rows
   uint16_t* H[kL + 1] = {};
   uint16_t* M[kL] = {};
   uint16_t* Apre[kL] = {};
@@ -469,6 +483,7 @@ int gnn_alloc(xg_ctx* c, GnnNet* g, int64_t n, int64_t nnz) {
       (rc = dev_alloc(c, L, (void**)&g->Wb, kL * kWsz * 2)) ||
       (rc = dev_alloc(c, L, (void**)&g->ah, (size_t)nnz * 4)) ||
       (rc = dev_alloc(c, L, (void**)&g->sqd, (size_t)n * 4)) ||
+      (rc = dev_alloc(c, L, (void**)&g->dec, (size_t)n)) ||
       (rc = dev_alloc(c, L, (void**)&g->xin, (size_t)n * 4)) ||
       (rc = dev_alloc(c, L, (void**)&g->dy, (size_t)n * 4)) ||
       (rc = dev_alloc(c, L, (void**)&g->dH, nc * 4)) ||
@@ -513,7 +528,8 @@ int gnn_forward(xg_ctx* c, GnnNet* g, const int* ia, const int* ja, const double
                        g->H[l + 1], g->Apre[l]);
   }
   hipLaunchKernelGGL(k_gnn_decode, dim3(grid_for(n) > 0 ? (unsigned)((n + kThreads - 1) / kThreads) : 1),
-                     dim3(kThreads), 0, c->stream, n, g->H[kL], g->P, g->sqd, g->sc, z);
+                     dim3(kThreads), 0, c->stream, n, g->H[kL], g->P, g->sqd, g->dec, r, d,
+                     g->sc, z);
   XG_HIP(c, hipGetLastError());
   return XG_OK;
 }
@@ -524,7 +540,7 @@ int gnn_backward(xg_ctx* c, GnnNet* g, const int* ia, const int* ja, const doubl
   const int gs = grid_for(n);
   const unsigned gnc = (unsigned)((n * kC + kThreads - 1) / kThreads);
   hipLaunchKernelGGL(k_gnn_dz, dim3(gs), dim3(kThreads), 0, c->stream, n, ia, ja, a, z, r,
-                     etr, g->sqd, g->sc, g->dy, g->part);
+                     etr, g->sqd, g->dec, g->sc, g->dy, g->part);
   hipLaunchKernelGGL(k_gnn_loss, dim3(1), dim3(kThreads), 0, c->stream, g->part, gs, etr,
                      g->sc);
   const int gd = grid_for(n * kC);
@@ -572,7 +588,7 @@ int xg_gnn_prepare_internal(xg_ctx* c, const int* ia, const int* ja, const doubl
   GnnNet* g = c->gnn;
   if (!g || g->n != n) return fail(c, XG_ENOMESH, "no learned preconditioner for this system");
   hipLaunchKernelGGL(k_gnn_prep, dim3((unsigned)((n + kThreads - 1) / kThreads)),
-                     dim3(kThreads), 0, c->stream, n, ia, ja, a, d, g->ah, g->sqd);
+                     dim3(kThreads), 0, c->stream, n, ia, ja, a, d, g->ah, g->sqd, g->dec);
   XG_HIP(c, hipGetLastError());
   g->prepared = true;
   return XG_OK;
