@@ -14,8 +14,9 @@ definite):
     a_i  = W_l [h_{l-1,i}; m_i] + b_l               (64 -> 32)
     h_l  = h_{l-1} + relu(a_i)
   y_i  = wo . h3_i + bo                             (decoder, 32 -> 1)
-  z_i  = s y_i / sqrt|d_i|,  or r_i / d_i on decoupled rows (a single
-         nonzero, the diagonal: the Dirichlet rows, where that is exact)
+  z_i  = r_i / d_i + s y_i / sqrt|d_i|   (a learned correction of Jacobi),
+         r_i / d_i alone on decoupled rows (a single nonzero, the diagonal:
+         the Dirichlet rows, where that is exact)
 
 Loss of a pair (r, e) with A e = r, the energy-norm error of z relative to
 that of e:  L = (z - e)^T A (z - e) / (e^T r) = (z^T A z - 2 z^T r) / (e^T r) + 1.
@@ -62,7 +63,7 @@ def forward(p, A_dense, d, r):
         cache["h"].append(h)
     y = h @ p["wo"] + p["bo"][0]
     dec = decoupled(A_dense)
-    z = np.where(dec, r / d, s * y / sq)
+    z = np.where(dec, r / d, r / d + s * y / sq)
     cache["y"] = y
     cache["dec"] = dec
     return z, cache

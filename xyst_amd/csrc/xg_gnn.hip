@@ -8,8 +8,9 @@
 //   l = 1..3:  m = Ahat h,  Ahat = |D|^-1/2 A |D|^-1/2      (SpMM, 32 channels)
 //              h += relu(W_l [h; m] + b_l)                 (64 -> 32, BF16 MFMA,
 //                                                           FP32 accumulation)
-//   y = wo . h3 + bo,  z_i = s y_i / sqrt|d_i|
-// (|d| because the ChoCG matrix -K is negative definite.)  Decoupled rows
+//   y = wo . h3 + bo,  z_i = r_i / d_i + s y_i / sqrt|d_i|   (Jacobi plus a
+// learned correction; |d| because the ChoCG matrix -K is negative definite.)
+// Decoupled rows
 // (the diagonal their only nonzero: the Dirichlet rows) take z_i = r_i / d_i,
 // which is exact there; the network acts on the coupled rows only.
 // Activations are stored in BF16, sums are FP32.  Training minimises the
@@ -174,7 +175,7 @@ __global__ __launch_bounds__(kThreads) void k_gnn_mlp(
   }
 }
 
-// y = wo . h3 + bo; z = s y / sqrt|d| (FP64 out), r/d on decoupled rows
+// y = wo . h3 + bo; z = r/d + s y / sqrt|d| (FP64 out), r/d on decoupled rows
 __global__ __launch_bounds__(kThreads) void k_gnn_decode(
     int64_t n, const uint16_t* __restrict__ H3, const float* __restrict__ P,
     const float* __restrict__ sqd, const uint8_t* __restrict__ dec,
@@ -189,7 +190,7 @@ __global__ __launch_bounds__(kThreads) void k_gnn_decode(
   float y = P[kOffBo];
 #pragma unroll
   for (int c = 0; c < kC; ++c) y += P[kOffWo + c] * bf2f(H3[i * kC + c]);
-  z[i] = sc[0] * (double)y / (double)sqd[i];
+  z[i] = r[i] / d[i] + sc[0] * (double)y / (double)sqd[i];
 }
 
 // dz = 2 (A z - r) / etr;  dy = dz s / sqrt(d);  partials of z^T A z, z^T r

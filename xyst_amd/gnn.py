@@ -19,14 +19,15 @@ __all__ = ["LearnedPreconditioner", "NPARAM", "init_params", "flatten", "unflatt
 
 
 def init_params(seed=0):
-    """He-style initial weights: encoder N(0, 0.5), layers N(0, 0.5/sqrt(64))
-    with zero biases, decoder N(0, 0.5/sqrt(32))."""
+    """Initial weights: encoder N(0, 0.5), layers N(0, 0.5/sqrt(64)) with zero
+    biases, a small decoder N(0, 0.05/sqrt(32)), so the untrained
+    preconditioner is close to Jacobi."""
     rng = np.random.default_rng(seed)
     p = {"we": rng.standard_normal(CH) * 0.5, "be": rng.standard_normal(CH) * 0.1}
     for l in range(NLAYER):
         p["W%d" % l] = rng.standard_normal((CH, 2 * CH)) * (0.5 / np.sqrt(2 * CH))
         p["b%d" % l] = np.zeros(CH)
-    p["wo"] = rng.standard_normal(CH) * (0.5 / np.sqrt(CH))
+    p["wo"] = rng.standard_normal(CH) * (0.05 / np.sqrt(CH))
     p["bo"] = np.zeros(1)
     return p
 
