@@ -280,9 +280,12 @@ int xg_cg_solve_mixed(xg_ctx* ctx, int64_t maxit, double tol,
  * the system set up by xg_cg_init, with preconditioner pc_kind 0 (Jacobi),
  * 1 (`inner` iterations of FP32 Jacobi-PCG from a zero guess, i.e. FP32
  * inner solves inside an FP64 outer Krylov recurrence) or 2 (the learned
- * preconditioner, xg_gnn_*).  Stops when ||r|| < tol*normb or after maxit
- * outer iterations; *iters outer iterations, *inner_total the inner
- * iterations (pc 1) or the Jacobi fallbacks (pc 2). */
+ * preconditioner, xg_gnn_*, judged by the residual: Jacobi for an
+ * iteration whose learned z is no descent direction, and for the rest of the
+ * solve once ||r|| has not improved on its best value for 20 iterations).
+ * Stops when ||r|| < tol*normb or after maxit outer iterations; *iters outer
+ * iterations, *inner_total the inner iterations (pc 1) or the iterations
+ * that used Jacobi (pc 2). */
 int xg_cg_solve_flexible(xg_ctx* ctx, int pc_kind, int64_t inner, int64_t maxit,
                          double tol, double* normr, int64_t* iters,
                          int64_t* inner_total, int* converged);

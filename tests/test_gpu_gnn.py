@@ -90,24 +90,51 @@ def test_forward_and_gradients_match_oracle():
 
 
 def test_training_lowers_loss_and_fcg_converges():
+    """Training on solves of a 16^3 system lowers the loss of a held-out
+    residual/error pair; flexible CG with the trained network reaches 1e-10
+    (true residual)."""
+    import torch
     from xyst_amd import gnn
     _say("gnn training + fcg")
     cg, M, coord, dirbc = _system(16, seed=5)
     n = coord.shape[1]
     net = gnn.LearnedPreconditioner(cg, seed=2)
-    rng = np.random.default_rng(0)
     x, y, z = coord
-    losses = []
+    # held-out pair from the training distribution: the residual and error
+    # of 12 Jacobi-PCG iterations on a right-hand side not trained on
+    bh = (np.cos(3 * x) * y + z * z) * 1e4
+    bbh = bh.copy()
+    for i in dirbc:
+        bbh[i] = 0.0
+    cg.init(np.zeros(n), bh, dirbc=dirbc, apply=True, pc="jacobi")
+    cg.solve(12, 1e-30)
+    x12 = cg.solution()
+    xs = np.linalg.solve(M, bbh)
+    e = xs - x12
+    r = M @ e
+    dr, de = torch.from_numpy(r).cuda(), torch.from_numpy(e).cuda()
+    dz = torch.empty_like(dr)
+
+    def held_out_loss():
+        net.prepare()
+        L = net.loss_grad_dev(dr, de, dz)
+        net.set_params(net.params())        # drop the accumulated gradient
+        return L
+
+    rng = np.random.default_rng(0)
+    cg.init(np.zeros(n), np.ones(n), dirbc=dirbc, apply=True, pc="jacobi")
+    L0 = held_out_loss()
     for s in range(6):
         b = sum(np.sin((k + 1) * np.pi * x + rng.uniform(0, 6)) *
                 np.cos(rng.integers(1, 4) * np.pi * y) for k in range(3)) * 1e4
         cg.init(np.zeros(n), b, dirbc=dirbc, apply=True, pc="jacobi")
-        L, npairs = net.train_system(stride=4, max_pairs=60, batch=4, lr=3e-3)
-        losses.append(L)
+        L, npairs = net.train_system(stride=4, max_pairs=60, batch=4, lr=2e-3)
         _say("  system %d: mean loss %.4f over %d pairs" % (s, L, npairs))
-        assert npairs > 5
-    assert min(losses) > -1e-6          # an energy-norm ratio: never negative
-    assert losses[-1] < losses[0]
+        assert npairs > 5 and L > -1e-6    # an energy-norm ratio: never negative
+    cg.init(np.zeros(n), np.ones(n), dirbc=dirbc, apply=True, pc="jacobi")
+    L1 = held_out_loss()
+    _say("  held-out loss %.4f -> %.4f" % (L0, L1))
+    assert L1 < L0
     b = (np.sin(2 * np.pi * x) * z + y) * 1e4
     cg.init(np.zeros(n), b, dirbc=dirbc, apply=True, pc="jacobi")
     cg.solve(20000, 1e-10)

@@ -874,10 +874,18 @@ int xg_cg_solve_flexible(xg_ctx* c, int pc_kind, int64_t inner, int64_t maxit,
   int64_t k = 0, tin = 0;
   double* P[2] = {s->p0, s->p1};
   double* Q[2] = {s->q, s->q2};
+  // the learned preconditioner is judged by the residual: once ||r|| has not
+  // improved on its best value for kJudge iterations, the rest of the solve
+  // uses Jacobi (flexible CG allows the switch)
+  constexpr int64_t kJudge = 20;
+  double best = nr;
+  int64_t kbest = 0;
+  bool learned = pc_kind == 2;
   for (; k < maxit && !(nr < target); ++k) {
     double* Bk = s->fb + (k & 1) * 4 * kMaxParts;
     double* Bp = s->fb + ((k + 1) & 1) * 4 * kMaxParts;
-    if (pc_kind == 0) {
+    if (pc_kind == 0 || (pc_kind == 2 && !learned)) {
+      if (pc_kind == 2) ++tin;
       hipLaunchKernelGGL(k_jacobi_apply, dim3(gv), dim3(kCgThreads), 0, c->stream, n,
                          s->r, s->d, s->z);
     } else if (pc_kind == 2) {
@@ -909,6 +917,12 @@ int xg_cg_solve_flexible(xg_ctx* c, int pc_kind, int64_t inner, int64_t maxit,
     XG_HIP(c, hipGetLastError());
     if ((rc = read_state(c))) return rc;
     nr = sqrt(s->hst->rr);
+    if (nr < best) {
+      best = nr;
+      kbest = k;
+    } else if (learned && k - kbest >= kJudge) {
+      learned = false;
+    }
   }
   if (normr) *normr = nr;
   if (iters) *iters = k;
