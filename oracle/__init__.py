@@ -92,6 +92,10 @@ def lib():
         L.orc_chorin_rhs.argtypes = [C.POINTER(ChoCfg), I64, C.c_int] + se + [
             _f64p, _f64p, _f64p, I64, _i64p, _f64p, _f64p, _f64p, _f64p, C.c_void_p]
         L.orc_chorin_rhs.restype = C.c_int
+        L.orc_kozcg_step.argtypes = [cfgp, C.POINTER(FctCfg), I64, C.c_int, _f64p, _f64p,
+                                     _f64p, I64, _i64p, _f64p, D, I64, _i64p, _f64p,
+                                     I64, _i64p, _f64p, _f64p, _f64p]
+        L.orc_kozcg_step.restype = C.c_int
         L.orc_kozak_rhs.argtypes = [cfgp, I64, C.c_int, _f64p, _f64p, _f64p, I64,
                                     _i64p, D, C.c_void_p, _f64p, _f64p]
         L.orc_kozak_rhs.restype = None
@@ -364,6 +368,35 @@ def diag(U, Un, v):
 
 
 # ---- KozCG -------------------------------------------------------------------
+
+class FctCfg(C.Structure):
+    _fields_ = [("fct", C.c_int), ("fctdif", C.c_double), ("fctclip", C.c_int),
+                ("nsys", C.c_int), ("sys", C.c_int * 16)]
+
+
+def fctcfg(fct=True, fctdif=1.0, fctclip=False, fctsys=()):
+    f = FctCfg(int(bool(fct)), fctdif, int(bool(fctclip)), len(fctsys))
+    for k, c in enumerate(fctsys):
+        f.sys[k] = int(c)
+    return f
+
+
+def kozcg_step(c, f, coord, inpoel, vol, dt, symnodes, symnorms, dirnodes, dirvals, U):
+    """One KozCG step; returns (U_new, U_old)."""
+    x, y, z = (_c(coord[i], np.float64) for i in range(3))
+    U = _c(U, np.float64).copy()
+    Un = np.empty_like(U)
+    sn = _c(symnodes, np.int64) if len(symnodes) else np.zeros(1, np.int64)
+    sv = _c(symnorms, np.float64) if len(symnorms) else np.zeros(3)
+    dn = _c(dirnodes, np.int64) if len(dirnodes) else np.zeros(1, np.int64)
+    dv = _c(dirvals, np.float64) if len(dirvals) else np.zeros(U.shape[1])
+    rc = lib().orc_kozcg_step(C.byref(c), C.byref(f), U.shape[0], U.shape[1], x, y, z,
+                              inpoel.shape[0], _c(inpoel, np.int64), _c(vol, np.float64),
+                              dt, len(symnodes), sn, sv, len(dirnodes), dn, dv, U, Un)
+    if rc:
+        raise ValueError("unsupported number of components")
+    return U, Un
+
 
 def kozak_rhs(c, coord, inpoel, dt, U, dtp=None, absacc=False):
     """kozak::rhs.  With absacc also returns the round-off magnitudes A."""

@@ -192,3 +192,145 @@ void orc_kozak_rhs_mag(const orc_cfg* cfg, int64_t npoin, int ncomp,
     }
   }
 }
+
+/* ---- KozCG step with flux-corrected transport ---- */
+
+static double kz_max(double a, double b) { return a > b ? a : b; }
+static double kz_min(double a, double b) { return a < b ? a : b; }
+
+/* element antidiffusive contributions of component c, KozCG.cpp:799-806 */
+static void aec_tet(const double* x, const double* y, const double* z,
+                    const int64_t* N, const double* U, int ncomp, int c,
+                    double ctau, double aec[4]) {
+  const double ba[3] = { x[N[1]]-x[N[0]], y[N[1]]-y[N[0]], z[N[1]]-z[N[0]] };
+  const double ca[3] = { x[N[2]]-x[N[0]], y[N[2]]-y[N[0]], z[N[2]]-z[N[0]] };
+  const double da[3] = { x[N[3]]-x[N[0]], y[N[3]]-y[N[0]], z[N[3]]-z[N[0]] };
+  double t[3];
+  cross3(ca, da, t);
+  const double J = ba[0]*t[0] + ba[1]*t[1] + ba[2]*t[2];
+  for (int a = 0; a < 4; ++a) {
+    aec[a] = 0.0;
+    for (int b = 0; b < 4; ++b) {
+      double m = J/120.0 * ((a == b) ? 3.0 : -1.0);
+      aec[a] += m * ctau * U[N[b]*ncomp + c];
+    }
+  }
+}
+
+int orc_kozcg_step(const orc_cfg* cfg, const orc_fctcfg* f, int64_t npoin,
+                   int ncomp, const double* x, const double* y,
+                   const double* z, int64_t ntet, const int64_t* inpoel,
+                   const double* vol, double dt, int64_t nsym,
+                   const int64_t* symnodes, const double* symnorms,
+                   int64_t ndir, const int64_t* dirnodes,
+                   const double* dirvals, double* U, double* Un) {
+  if (ncomp < 5 || ncomp > 16) return -1;
+  const size_t nn = (size_t)npoin * (size_t)ncomp;
+  double* R = (double*)malloc(sizeof(double) * nn);
+  double* A = (double*)calloc(nn, sizeof(double));
+  orc_kozak_rhs(cfg, npoin, ncomp, x, y, z, ntet, inpoel, dt, NULL, U, R);
+  if (f->fct) {
+    const double ctau = f->fctdif;
+    double* P = (double*)calloc(2*nn, sizeof(double));
+    double* Q = (double*)malloc(sizeof(double) * 2*nn);
+    /* aec: P+/- (KozCG.cpp:774-838) */
+    for (int64_t e = 0; e < ntet; ++e) {
+      const int64_t* N = inpoel + e*4;
+      for (int c = 0; c < ncomp; ++c) {
+        double aec[4];
+        aec_tet(x, y, z, N, U, ncomp, c, ctau, aec);
+        for (int a = 0; a < 4; ++a) {
+          P[N[a]*2*ncomp + 2*c] += kz_max(0.0, aec[a]);
+          P[N[a]*2*ncomp + 2*c + 1] += kz_min(0.0, aec[a]);
+        }
+      }
+    }
+    for (int64_t i = 0; i < nsym; ++i) {           /* :817-830 */
+      double* p = P + symnodes[i]*2*ncomp;
+      const double nx = symnorms[i*3], ny = symnorms[i*3+1], nz = symnorms[i*3+2];
+      double rvnp = p[2]*nx + p[4]*ny + p[6]*nz;
+      double rvnn = p[3]*nx + p[5]*ny + p[7]*nz;
+      p[2] -= rvnp * nx;  p[3] -= rvnn * nx;
+      p[4] -= rvnp * ny;  p[5] -= rvnn * ny;
+      p[6] -= rvnp * nz;  p[7] -= rvnn * nz;
+    }
+    /* alw: low-order solution (into R) and Q+/- (:866-948) */
+    for (int64_t i = 0; i < npoin; ++i)
+      for (int c = 0; c < ncomp; ++c) {
+        double* p = P + i*2*ncomp + 2*c;
+        p[0] /= vol[i];
+        p[1] /= vol[i];
+        R[i*ncomp+c] = U[i*ncomp+c] + dt*R[i*ncomp+c]/vol[i] - p[0] - p[1];
+      }
+    const double large = 1.7976931348623157e308;
+    for (size_t k = 0; k < nn; ++k) { Q[2*k] = -large; Q[2*k+1] = large; }
+    for (int64_t e = 0; e < ntet; ++e) {
+      const int64_t* N = inpoel + e*4;
+      for (int c = 0; c < ncomp; ++c) {
+        double alwp = -large, alwn = large;
+        for (int a = 0; a < 4; ++a) {
+          const double ul = R[N[a]*ncomp+c];
+          if (f->fctclip) {
+            alwp = kz_max(alwp, ul);
+            alwn = kz_min(alwn, ul);
+          } else {
+            alwp = kz_max(alwp, kz_max(ul, U[N[a]*ncomp+c]));
+            alwn = kz_min(alwn, kz_min(ul, U[N[a]*ncomp+c]));
+          }
+        }
+        for (int a = 0; a < 4; ++a) {
+          double* q = Q + N[a]*2*ncomp + 2*c;
+          q[0] = kz_max(q[0], alwp);
+          q[1] = kz_min(q[1], alwn);
+        }
+      }
+    }
+    /* lim: limit coefficients (:979-1023) */
+    const double eps = 2.220446049250313e-16;
+    for (int64_t i = 0; i < npoin; ++i)
+      for (int c = 0; c < ncomp; ++c) {
+        double* q = Q + i*2*ncomp + 2*c;
+        const double* p = P + i*2*ncomp + 2*c;
+        q[0] -= R[i*ncomp+c];
+        q[1] -= R[i*ncomp+c];
+        q[0] = p[0] <  eps ? 0.0 : kz_min(1.0, q[0]/p[0]);
+        q[1] = p[1] > -eps ? 0.0 : kz_min(1.0, q[1]/p[1]);
+      }
+    /* limited antidiffusive contributions (:1025-1070) */
+    for (int64_t e = 0; e < ntet; ++e) {
+      const int64_t* N = inpoel + e*4;
+      double coef[16], aec[16][4];
+      for (int c = 0; c < ncomp; ++c) {
+        coef[c] = 1.0;
+        aec_tet(x, y, z, N, U, ncomp, c, ctau, aec[c]);
+        for (int a = 0; a < 4; ++a)
+          coef[c] = kz_min(coef[c], aec[c][a] > 0.0 ? Q[N[a]*2*ncomp + 2*c]
+                                                  : Q[N[a]*2*ncomp + 2*c + 1]);
+      }
+      double cs = 1.0;
+      for (int k = 0; k < f->nsys; ++k) cs = kz_min(cs, coef[f->sys[k]]);
+      for (int k = 0; k < f->nsys; ++k) coef[f->sys[k]] = cs;
+      for (int c = 0; c < ncomp; ++c)
+        for (int a = 0; a < 4; ++a) A[N[a]*ncomp+c] += coef[c] * aec[c][a];
+    }
+    /* solve: low-order + limited contributions (:1140-1146) */
+    for (int64_t i = 0; i < npoin; ++i)
+      for (int c = 0; c < ncomp; ++c)
+        A[i*ncomp+c] = R[i*ncomp+c] + A[i*ncomp+c]/vol[i];
+    free(P);
+    free(Q);
+  } else {
+    for (int64_t i = 0; i < npoin; ++i)                    /* :1148-1154 */
+      for (int c = 0; c < ncomp; ++c)
+        A[i*ncomp+c] = U[i*ncomp+c] + dt*R[i*ncomp+c]/vol[i];
+  }
+  /* BC: dirbc then symbc on momentum (KozCG.cpp:590-605) */
+  for (int64_t k = 0; k < ndir; ++k)
+    for (int c = 0; c < ncomp; ++c) A[dirnodes[k]*ncomp+c] = dirvals[k*ncomp+c];
+  orc_symbc(npoin, ncomp, A, nsym, symnodes, symnorms);
+  memcpy(Un, U, sizeof(double) * nn);
+  memcpy(U, A, sizeof(double) * nn);
+  free(R);
+  free(A);
+  return 0;
+}

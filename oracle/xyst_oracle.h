@@ -132,6 +132,30 @@ void orc_kozak_rhs_mag(const orc_cfg* cfg, int64_t npoin, int ncomp,
                        int64_t ntet, const int64_t* inpoel, double dt,
                        const double* U, double* R, double* A);
 
+/* KozCG flux-corrected transport settings (InciterConfig.cpp:1751-1754) */
+typedef struct {
+  int    fct;          /* FCT on (default) or off */
+  double fctdif;       /* antidiffusion coefficient, default 1.0 */
+  int    fctclip;      /* bounds from the low-order solution only */
+  int    nsys;         /* components limited together (fctsys, 0-based) */
+  int    sys[16];
+} orc_fctcfg;
+
+/* One KozCG step (KozCG.cpp:709-1197): kozak::rhs, then with FCT the
+ * antidiffusive contributions (aec, with symmetry BCs on them), the low-order
+ * solution and allowed bounds (alw), the limit coefficients and limited
+ * contributions (lim), the new solution (solve); without FCT U + dt R/vol.
+ * Then Dirichlet values (all components of the ndir nodes, dirvals
+ * ndir*ncomp) and symmetry BCs.  U is advanced in place, Un receives the old
+ * U.  Returns 0, or -1 for ncomp outside 5..16. */
+int orc_kozcg_step(const orc_cfg* cfg, const orc_fctcfg* f, int64_t npoin,
+                   int ncomp, const double* x, const double* y,
+                   const double* z, int64_t ntet, const int64_t* inpoel,
+                   const double* vol, double dt, int64_t nsym,
+                   const int64_t* symnodes, const double* symnorms,
+                   int64_t ndir, const int64_t* dirnodes,
+                   const double* dirvals, double* U, double* Un);
+
 /* ---- ChoCG (chocg.c, riecg.c) -------------------------------------------- */
 
 /* ChoCG configuration read from g_cfg (InciterConfig.cpp:1724-1757) */

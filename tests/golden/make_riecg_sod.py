@@ -1,12 +1,18 @@
-"""Extract the reference's RieCG/Sod regression fixture into riecg_sod.npz.
+"""Extract the reference's RieCG/Sod and KozCG/Sod regression fixtures into
+riecg_sod.npz and kozcg_sod.npz.
 
-Source files (reference test data, read as data only):
-  tests/regression/inciter/RieCG/Sod/rectangle_01_1.5k.exo   input mesh
-  tests/regression/inciter/RieCG/Sod/sod.std.exo             fields after 10 steps
-  tests/regression/inciter/RieCG/Sod/diag.std                per-step diagnostics
-Run configuration (sod.q): nstep 10, term 0.2, cfl 0.5, gamma 1.4,
-symmetry BCs on side sets 2, 4, 5, 6.  The output mesh is renumbered by the
-solver, so output nodes are matched to input nodes by coordinates.
+Source files (reference test data, read as data only), per solver directory
+tests/regression/inciter/{RieCG,KozCG}/Sod:
+  rectangle_01_1.5k.exo   input mesh (the same file for both)
+  sod.std.exo             fields after 10 steps
+  diag.std                per-step diagnostics
+Run configurations (sod.q):
+  RieCG: nstep 10, term 0.2, cfl 0.5, gamma 1.4, symmetry on side sets 2,4,5,6.
+  KozCG: nstep 10, cfl 0.5, gamma 1.4, symmetry on 2,4,5,6, Dirichlet (all
+         five components, initial values) on side sets 1 and 3, FCT with
+         fctclip = true and fctsys = {1, 2, 5}, fctdif at its default 1.0.
+The output mesh is renumbered by the solver, so output nodes are matched to
+input nodes by coordinates.
 """
 import os
 import sys
@@ -14,12 +20,12 @@ import sys
 import numpy as np
 from scipy.io import netcdf_file
 
-REF = "/root/reference/tests/regression/inciter/RieCG/Sod"
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "riecg_sod.npz")
+REF = "/root/reference/tests/regression/inciter"
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def main():
-    f = netcdf_file(os.path.join(REF, "rectangle_01_1.5k.exo"), "r", mmap=False)
+def read_mesh(ref):
+    f = netcdf_file(os.path.join(ref, "rectangle_01_1.5k.exo"), "r", mmap=False)
     coord = np.stack([f.variables[c].data.astype(np.float64)
                       for c in ("coordx", "coordy", "coordz")])
     tri_blk = f.variables["connect1"].data.astype(np.int64) - 1
@@ -41,15 +47,17 @@ def main():
     # i.e. outward for positively oriented tets.
     tetface = {}
     for t in tets:
-        for f in ((t[0], t[2], t[1]), (t[0], t[1], t[3]), (t[0], t[3], t[2]),
-                  (t[1], t[2], t[3])):
-            tetface[tuple(sorted(f))] = f
-    tri = np.array([tetface[tuple(sorted(f))] for f in tri], np.int64)
+        for fc in ((t[0], t[2], t[1]), (t[0], t[1], t[3]), (t[0], t[3], t[2]),
+                   (t[1], t[2], t[3])):
+            tetface[tuple(sorted(fc))] = fc
+    tri = np.array([tetface[tuple(sorted(fc))] for fc in tri], np.int64)
+    return coord, tets, tri, faceset
 
-    g = netcdf_file(os.path.join(REF, "sod.std.exo"), "r", mmap=False)
+
+def read_result(ref, coord):
+    g = netcdf_file(os.path.join(ref, "sod.std.exo"), "r", mmap=False)
     Y = np.stack([g.variables[c].data.astype(np.float64)
                   for c in ("coordx", "coordy", "coordz")])
-    # match output nodes to input nodes by coordinates
     key = lambda P: np.lexsort(np.round(P, 12)[::-1])
     ia, ib = key(coord), key(Y)
     assert np.abs(coord[:, ia] - Y[:, ib]).max() < 1e-12
@@ -60,14 +68,25 @@ def main():
     fields = np.stack([g.variables["vals_nod_var%d" % (i + 1)].data[-1][perm]
                        for i in range(len(names))], 1)
     t_final = float(g.variables["time_whole"].data[-1])
+    return fields, names, t_final
 
-    diag = np.loadtxt(os.path.join(REF, "diag.std"))
-    np.savez_compressed(OUT, coord=coord, inpoel=tets, triinpoel=tri,
-                        faceset=faceset, fields=fields,
-                        field_names=np.array(names), t_final=t_final,
-                        diag=diag, symsets=np.array([2, 4, 5, 6], np.int32),
-                        nstep=10, term=0.2, cfl=0.5, gamma=1.4)
-    print("wrote", OUT, fields.shape, names, t_final)
+
+def main():
+    for solver, out, extra in (
+            ("RieCG", "riecg_sod.npz", dict(term=0.2)),
+            ("KozCG", "kozcg_sod.npz",
+             dict(term=1.0e300, dirsets=np.array([1, 3], np.int32), fctclip=1,
+                  fctsys=np.array([0, 1, 4], np.int32), fctdif=1.0))):
+        ref = os.path.join(REF, solver, "Sod")
+        coord, tets, tri, faceset = read_mesh(ref)
+        fields, names, t_final = read_result(ref, coord)
+        diag = np.loadtxt(os.path.join(ref, "diag.std"))
+        np.savez_compressed(os.path.join(HERE, out), coord=coord, inpoel=tets,
+                            triinpoel=tri, faceset=faceset, fields=fields,
+                            field_names=np.array(names), t_final=t_final,
+                            diag=diag, symsets=np.array([2, 4, 5, 6], np.int32),
+                            nstep=10, cfl=0.5, gamma=1.4, **extra)
+        print("wrote", out, fields.shape, names, t_final)
 
 
 if __name__ == "__main__":

@@ -26,10 +26,16 @@ class Case:
             self.coord, self.tri, self.faceset, self.symsets)
 
 
-def sod_fixture():
-    d = np.load(os.path.join(GOLDEN, "riecg_sod.npz"))
+def sod_fixture(name="riecg_sod.npz"):
+    d = np.load(os.path.join(GOLDEN, name))
     return Case(d["coord"], d["inpoel"], d["triinpoel"], d["faceset"],
                 d["symsets"]), d
+
+
+def dirichlet_nodes(case, sets):
+    """Sorted unique nodes of the boundary faces on the given side sets."""
+    on = np.isin(case.faceset, np.asarray(sets))
+    return np.unique(case.tri[on].reshape(-1))
 
 
 def box_case(n, seed=20261202, symsets=(1, 2, 3, 4, 5, 6), jitter=0.1):

@@ -100,3 +100,93 @@ def test_kozak_steady_dtp_equals_uniform_dt():
     R1 = O.kozak_rhs(O.cfg(), coord, inpoel, 2.5e-3, U)
     R2 = O.kozak_rhs(O.cfg(), coord, inpoel, 99.0, U, dtp=dtp)
     assert np.array_equal(R1, R2)
+
+
+def _prim_fields(U, gamma=GAMMA):
+    r = U[:, 0]
+    u, v, w = U[:, 1] / r, U[:, 2] / r, U[:, 3] / r
+    e = U[:, 4] / r
+    p = (gamma - 1.0) * r * (e - 0.5 * (u * u + v * v + w * w))
+    return np.stack([r, u, v, w, e, p], 1)
+
+
+def kozcg_sod_run(step=None):
+    """10 KozCG steps of the reference's KozCG/Sod case; `step` defaults to
+    the oracle's.  Yields (it, t, dt, U_new, U_old) per step."""
+    from tests.meshes import dirichlet_nodes, sod_fixture
+    from xyst_amd.meshgen import sod_state
+    c, d = sod_fixture("kozcg_sod.npz")
+    U = sod_state(c.coord)
+    cfg = O.cfg(float(d["gamma"]))
+    f = O.fctcfg(True, float(d["fctdif"]), bool(d["fctclip"]), list(d["fctsys"]))
+    dn = dirichlet_nodes(c, d["dirsets"])
+    dv = U[dn].copy()                  # Sod's initial state is time-independent
+    step = step or (lambda dt, U: O.kozcg_step(cfg, f, c.coord, c.inpoel, c.vol, dt,
+                                               c.symnodes, c.symnorms, dn, dv, U))
+    t = 0.0
+    for it in range(int(d["nstep"])):
+        dt = O.dt(cfg, U, c.vol, float(d["cfl"]))
+        U, Un = step(dt, U)
+        t += dt
+        yield it, t, dt, U, Un, c, d
+
+
+def test_sod_regression_fixture():
+    """10 KozCG steps (Taylor-Galerkin rhs, FCT with fctclip and fctsys
+    {1,2,5}, Dirichlet on side sets 1 and 3, symmetry on 2,4,5,6) against
+    diag.std and sod.std.exo of tests/regression/inciter/KozCG/Sod."""
+    for it, t, dt, U, Un, c, d in kozcg_sod_run():
+        V = c.vol.sum()
+        dg = O.diag(U, Un, c.vol)
+        mine = np.concatenate([[it + 1, t, dt], np.sqrt(dg[:10] / V), [dg[10]]])
+        np.testing.assert_allclose(mine, d["diag"][it], rtol=2e-8, atol=1e-14)
+    assert abs(t - float(d["t_final"])) < 1e-8
+    mine = _prim_fields(U)
+    ref = d["fields"]
+    for i, name in enumerate(d["field_names"]):
+        diff = np.abs(mine[:, i] - ref[:, i])
+        if name.startswith("velocity"):
+            ok = (diff <= 1e-8) | (np.abs(ref[:, i]) < 1e-7)
+        else:
+            ok = (diff <= 1e-7 * np.abs(ref[:, i])) | (np.abs(ref[:, i]) < 1e-8)
+        assert ok.all(), (name, diff.max())
+
+
+def test_fct_off_is_forward_euler_and_limited_step_is_bounded():
+    """Without FCT the step is U + dt R/vol; with FCT and fctclip each new
+    nodal value lies within the low-order solution's bounds over the tets
+    around the node (the defining property of the limiter), checked on a
+    jittered box with a discontinuous state."""
+    m = box_mesh(3, jitter=0.1, seed=11)
+    coord, inpoel = m["coord"], m["inpoel"]
+    vol = O.vol(coord, inpoel)
+    U = smooth_state(coord)
+    U[coord[0] > 0.5, 0] *= 0.3
+    U[coord[0] > 0.5, 4] *= 0.2
+    cfg = O.cfg(GAMMA)
+    dt = O.dt(cfg, U, vol, 0.5)
+    none = np.zeros(0, np.int64)
+    off = O.fctcfg(False)
+    U1, _ = O.kozcg_step(cfg, off, coord, inpoel, vol, dt, none, np.zeros(0), none,
+                         np.zeros(0), U)
+    R = O.kozak_rhs(cfg, coord, inpoel, dt, U)
+    np.testing.assert_allclose(U1, U + dt * R / vol[:, None], rtol=0, atol=1e-15)
+    on = O.fctcfg(True, 1.0, True, [])
+    U2, _ = O.kozcg_step(cfg, on, coord, inpoel, vol, dt, none, np.zeros(0), none,
+                         np.zeros(0), U)
+    # low-order solution, restated from its definition: mass-lumped minus the
+    # consistent-minus-lumped mass difference of U
+    ML = np.zeros_like(U)
+    for t in inpoel:
+        J = np.linalg.det(np.c_[coord[:, t[1:]].T - coord[:, t[0]]])
+        Mdiff = J / 120.0 * (4.0 * np.eye(4) - np.ones((4, 4)))
+        ML[t] += Mdiff @ U[t]
+    ul = U + dt * R / vol[:, None] - ML / vol[:, None]
+    hi = np.full_like(U, -np.inf)
+    lo = np.full_like(U, np.inf)
+    for t in inpoel:
+        hi[t] = np.maximum(hi[t], ul[t].max(0))
+        lo[t] = np.minimum(lo[t], ul[t].min(0))
+    tol = 1e-12 * np.abs(U).max(0)
+    assert (U2 <= hi + tol).all() and (U2 >= lo - tol).all()
+    assert not np.allclose(U2, ul)        # the limiter did add antidiffusion
