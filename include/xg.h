@@ -165,6 +165,40 @@ int xg_kozak_rhs(xg_ctx* ctx, int ncomp, double dt, const double* U, double* R);
 int xg_kozak_rhs_dev(xg_ctx* ctx, int ncomp, double dt, const double* U,
                      double* R);
 
+/* ---- resident KozCG stepping ------------------------------------------------ */
+
+/* KozCG's flux-corrected transport settings (InciterConfig.cpp:1751-1754). */
+typedef struct {
+  int    fct;         /* FCT on (default) or off */
+  double fctdif;      /* antidiffusion coefficient, default 1.0 */
+  int    fctclip;     /* allowed bounds from the low-order solution only */
+  int    nsys;        /* number of fctsys components */
+  int    sys[5];      /* fctsys, 0-based component ids limited together */
+} xg_fctcfg;
+
+/* Per-mesh data of the KozCG chare on the mesh of xg_kozak_upload: nodal
+ * volumes, the symmetry BC lists symbcnodes/symbcnorms (KozCG.cpp:524-540),
+ * the Dirichlet nodes with a component mask and the values physics::dirbc
+ * sets (KozCG.cpp:120-143, BC.cpp:30-69; values fixed in time, ndir*5 each),
+ * and the FCT settings. */
+int xg_kozcg_setup(xg_ctx* ctx, const double* vol, int64_t nsym,
+                   const size_t* symnodes, const double* symnorms, int64_t ndir,
+                   const size_t* dirnodes, const int* dirmask,
+                   const double* dirvals, const xg_fctcfg* fct);
+
+/* KozCG::dt (KozCG.cpp:614-700), unsteady branch without freezeflow:
+ * *dt = cfl * min_p cbrt(vol_p)/max(|u|+c, 1e-8).  U on the device. */
+int xg_kozcg_dt_dev(xg_ctx* ctx, const double* U, double cfl, double* dt);
+
+/* One KozCG time step (KozCG::rhs/aec/alw/lim/solve, KozCG.cpp:709-1197):
+ * the Taylor-Galerkin rhs, with FCT the antidiffusive contributions (with
+ * symmetry BCs on them), the low-order solution, allowed bounds, limit
+ * coefficients and limited contributions, the new solution; without FCT
+ * U + dt R/vol.  Then Dirichlet and symmetry BCs (KozCG::BC,
+ * KozCG.cpp:590-611).  U (npoin*5, device) is advanced in place; Un
+ * (npoin*5, device) receives the old solution. */
+int xg_kozcg_step_dev(xg_ctx* ctx, double dt, double* U, double* Un);
+
 /* ---- chorin:: drop-ins (ChoCG) ------------------------------------------------ */
 
 /* ChoCG's g_cfg reads of chorin::rhs (InciterConfig.cpp:1724-1757). */

@@ -195,8 +195,9 @@ void orc_kozak_rhs_mag(const orc_cfg* cfg, int64_t npoin, int ncomp,
 
 /* ---- KozCG step with flux-corrected transport ---- */
 
-static double kz_max(double a, double b) { return a > b ? a : b; }
-static double kz_min(double a, double b) { return a < b ? a : b; }
+/* std::max / std::min, including which argument a tie or NaN returns */
+static double kz_max(double a, double b) { return (a < b) ? b : a; }
+static double kz_min(double a, double b) { return (b < a) ? b : a; }
 
 /* element antidiffusive contributions of component c, KozCG.cpp:799-806 */
 static void aec_tet(const double* x, const double* y, const double* z,

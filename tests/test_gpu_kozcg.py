@@ -1,4 +1,5 @@
-"""Parity of the HIP kozak::rhs with the CPU oracle (needs an MI355X).
+"""Parity of the HIP kozak::rhs and KozCG step with the CPU oracle (needs an
+MI355X).
 
 Tolerance: FP64, per node and component |R_gpu - R_oracle| <= 1e-12 * A,
 where A is the oracle's first-order round-off magnitude of that entry (the
@@ -97,4 +98,169 @@ def test_kozak_errors():
     K.upload(ctx, m["coord"], m["inpoel"])
     U = np.zeros((m["coord"].shape[1], 6))
     assert lib().xg_kozak_rhs(ctx._h, 6, 1e-3, U.ctypes.data, U.ctypes.data) == XG_EUNSUPPORTED
+    ctx.close()
+
+
+# ---------------------------------------------------------------- KozCG step --
+# Tolerance of one step from the same state: |U_gpu - U_oracle| <= 1e-11 *
+# max_p |U_oracle[:, c]| per component.  The limiter's branches read the
+# antidiffusive contributions, which both sides compute in the reference's
+# operation order (bit-identical); sums and the rhs differ by round-off only.
+STEP_TOL = 1e-11
+
+
+def _kozcg_case(n, order, dirsets=(1, 2), symsets=(3, 4, 5, 6)):
+    from tests.meshes import Case, dirichlet_nodes
+    m = _case(n, order)
+    c = Case(m["coord"], m["inpoel"], m["triinpoel"], m["faceset"], symsets)
+    return c, dirichlet_nodes(c, dirsets)
+
+
+def _setup(ctx, c, dn, U0, **fct):
+    from xyst_amd import kozak as K
+    K.upload(ctx, c.coord, c.inpoel)
+    K.setup(ctx, c.vol, c.symnodes, c.symnorms, dn, np.ones((dn.size, 5), np.int32),
+            U0[dn], **fct)
+
+
+def _oracle_step(c, dn, U0, dt, U, fct=True, fctdif=1.0, fctclip=False, fctsys=()):
+    f = O.fctcfg(fct, fctdif, fctclip, list(fctsys))
+    return O.kozcg_step(O.cfg(), f, c.coord, c.inpoel, c.vol, dt, c.symnodes,
+                        c.symnorms, dn, U0[dn], U)
+
+
+def _check_step(Ug, Uo, what):
+    scale = np.abs(Uo).max(0) + 1e-300
+    ratio = float((np.abs(Ug - Uo) / scale).max())
+    _say("  %s worst |dU|/max|U| = %.3e" % (what, ratio))
+    assert ratio <= STEP_TOL, what
+
+
+@pytest.mark.parametrize("n,order,fct", [
+    (12, "rcm+block", dict(fct=True, fctclip=True, fctsys=(0, 1, 4))),
+    (14, "rcm+block", dict(fct=True, fctclip=False, fctsys=())),
+    (10, "lex", dict(fct=True, fctdif=0.7, fctclip=False, fctsys=(0, 4))),
+    (10, "rcm+block", dict(fct=False))])
+def test_kozcg_step_parity(n, order, fct):
+    """Five device steps, each compared with the oracle's step from the
+    same state, on a Sod-like discontinuity with Dirichlet x faces and
+    symmetry on the others."""
+    import torch
+    from xyst_amd import kozak as K
+    _say("kozcg step n=%d %s %s" % (n, order, fct))
+    c, dn = _kozcg_case(n, order)
+    U0 = sod_state(c.coord)
+    U0[:, 1] = 0.1 * U0[:, 0]          # a flow across the discontinuity
+    U0[:, 4] += 0.5 * 0.01 * U0[:, 0]
+    ctx = _ctx()
+    _setup(ctx, c, dn, U0, **fct)
+    U = torch.tensor(U0, device="cuda")
+    Un = torch.empty_like(U)
+    Uh = U0.copy()
+    for step in range(5):
+        dt = K.dt_dev(ctx, U, 0.5)
+        dto = O.dt(O.cfg(), Uh, c.vol, 0.5)
+        assert dt == pytest.approx(dto, rel=1e-14)
+        K.step_dev(ctx, dt, U, Un)
+        torch.cuda.synchronize()
+        Ug = U.cpu().numpy()
+        assert np.array_equal(Un.cpu().numpy(), Uh)
+        Uo, _ = _oracle_step(c, dn, U0, dt, Uh, **fct)
+        _check_step(Ug, Uo, "step %d" % step)
+        Uh = Ug
+    ctx.close()
+
+
+def test_kozcg_sod_regression_fixture():
+    """The reference's KozCG/Sod case (tests/regression/inciter/KozCG/Sod)
+    on the device: 10 steps against diag.std and sod.std.exo, with the
+    exodiff/numdiff bands, and each step against the oracle."""
+    import torch
+    from tests.meshes import dirichlet_nodes, sod_fixture
+    from xyst_amd import kozak as K
+    _say("kozcg sod fixture")
+    c, d = sod_fixture("kozcg_sod.npz")
+    dn = dirichlet_nodes(c, d["dirsets"])
+    U0 = sod_state(c.coord)
+    fct = dict(fct=True, fctdif=float(d["fctdif"]), fctclip=bool(d["fctclip"]),
+               fctsys=tuple(int(s) for s in d["fctsys"]))
+    ctx = _ctx()
+    _setup(ctx, c, dn, U0, **fct)
+    U = torch.tensor(U0, device="cuda")
+    Un = torch.empty_like(U)
+    V = c.vol.sum()
+    t = 0.0
+    Uh = U0.copy()
+    for it in range(int(d["nstep"])):
+        dt = K.dt_dev(ctx, U, float(d["cfl"]))
+        K.step_dev(ctx, dt, U, Un)
+        torch.cuda.synchronize()
+        Ug = U.cpu().numpy()
+        Uo, _ = _oracle_step(c, dn, U0, dt, Uh, **fct)
+        _check_step(Ug, Uo, "sod step %d" % it)
+        t += dt
+        dg = O.diag(Ug, Uh, c.vol)
+        mine = np.concatenate([[it + 1, t, dt], np.sqrt(dg[:10] / V), [dg[10]]])
+        np.testing.assert_allclose(mine, d["diag"][it], rtol=2e-8, atol=1e-14)
+        Uh = Ug
+    assert abs(t - float(d["t_final"])) < 1e-8
+    r = Uh[:, 0]
+    vel = Uh[:, 1:4] / r[:, None]
+    e = Uh[:, 4] / r
+    p = 0.4 * r * (e - 0.5 * (vel ** 2).sum(1))
+    mine = np.column_stack([r, vel, e, p])
+    for i, name in enumerate(d["field_names"]):
+        diff = np.abs(mine[:, i] - d["fields"][:, i])
+        ref = np.abs(d["fields"][:, i])
+        if name.startswith("velocity"):
+            ok = (diff <= 1e-8) | (ref < 1e-7)
+        else:
+            ok = (diff <= 1e-7 * ref) | (ref < 1e-8)
+        assert ok.all(), (name, diff.max())
+    ctx.close()
+
+
+def test_kozcg_step_reproducible_and_conservative():
+    """Two runs give bitwise equal states; with symmetry BCs only (no
+    Dirichlet nodes) FCT moves mass between nodes but keeps the total
+    (sum of vol * rho) to round-off."""
+    import torch
+    from xyst_amd import kozak as K
+    _say("kozcg reproducible")
+    c, dn = _kozcg_case(12, "rcm+block", dirsets=(), symsets=(1, 2, 3, 4, 5, 6))
+    assert dn.size == 0
+    U0 = sod_state(c.coord)
+    ctx = _ctx()
+    _setup(ctx, c, dn, U0, fct=True, fctclip=True, fctsys=(0, 1, 4))
+    outs = []
+    for rep in range(2):
+        U = torch.tensor(U0, device="cuda")
+        Un = torch.empty_like(U)
+        for step in range(4):
+            K.step_dev(ctx, 1e-3, U, Un)
+        torch.cuda.synchronize()
+        outs.append(U.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+    m0 = (c.vol * U0[:, 0]).sum()
+    m1 = (c.vol * outs[0][:, 0]).sum()
+    assert abs(m1 - m0) <= 1e-13 * m0
+    ctx.close()
+
+
+def test_kozcg_step_errors():
+    from xyst_amd import kozak as K
+    from xyst_amd._xg import XG_EINVAL, XG_ENOMESH, lib
+    _say("kozcg errors")
+    c, dn = _kozcg_case(4, "lex")
+    ctx = _ctx()
+    K.upload(ctx, c.coord, c.inpoel)
+    assert lib().xg_kozcg_step_dev(ctx._h, 1e-3, 8, 16) == XG_ENOMESH
+    with pytest.raises(Exception):
+        K.setup(ctx, c.vol, c.symnodes, c.symnorms, fctsys=(0, 1, 2, 3, 4, 5))
+    bad = c.vol.copy()
+    bad[3] = 0.0
+    with pytest.raises(Exception):
+        K.setup(ctx, bad, c.symnodes, c.symnorms)
+    K.setup(ctx, c.vol, c.symnodes, c.symnorms)
+    assert lib().xg_kozcg_step_dev(ctx._h, 1e-3, 8, 8) == XG_EINVAL
     ctx.close()

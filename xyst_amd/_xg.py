@@ -43,6 +43,9 @@ SIG = {
     "xg_kozak_upload": (C.c_int, [VP, I64, VP, VP, VP, I64, VP]),
     "xg_kozak_rhs": (C.c_int, [VP, C.c_int, D, VP, VP]),
     "xg_kozak_rhs_dev": (C.c_int, [VP, C.c_int, D, VP, VP]),
+    "xg_kozcg_setup": (C.c_int, [VP, VP, I64, VP, VP, I64, VP, VP, VP, VP]),
+    "xg_kozcg_dt_dev": (C.c_int, [VP, VP, D, VP]),
+    "xg_kozcg_step_dev": (C.c_int, [VP, D, VP, VP]),
     "xg_chocg_upload": (C.c_int, [VP, I64, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP,
                                   I64, VP]),
     "xg_chorin_div": (C.c_int, [VP, C.c_int, D, VP, VP, C.c_int, VP, C.c_int, VP,
@@ -91,6 +94,11 @@ class Config(C.Structure):
 class ChoConfig(C.Structure):
     _fields_ = [("flux", C.c_int), ("stab", C.c_int), ("stab2", C.c_int),
                 ("stab2coef", C.c_double), ("mu", C.c_double)]
+
+
+class FctConfig(C.Structure):
+    _fields_ = [("fct", C.c_int), ("fctdif", C.c_double), ("fctclip", C.c_int),
+                ("nsys", C.c_int), ("sys", C.c_int * 5)]
 
 
 _LIB = None

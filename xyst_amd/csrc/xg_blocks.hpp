@@ -1,5 +1,6 @@
 // Device helpers shared by the node-block kernels (xg_riemann.hip,
-// xg_kozak.hip): the XCD-aware block order and the halo fix-up pass.
+// xg_kozak.hip): the XCD-aware block order, the halo fix-up pass and the
+// CFL time step reduction.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -34,6 +35,77 @@ __global__ __launch_bounds__(xgd::kBlock) void k_fixup(
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) r[c] = acc[c];
+}
+
+__device__ inline double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// per-node CFL limit of the explicit solvers' dt, unsteady branch
+// (RieCG.cpp:827-839, KozCG.cpp:648-660), block minimum
+[[maybe_unused]] __global__ __launch_bounds__(xgd::kBlock) void k_dt_partial(
+    int npoin, const double* __restrict__ U, const double* __restrict__ vol,
+    double g, double* __restrict__ part) {
+  __shared__ double sm[xgd::kBlock / 64];
+  double m = 1.7976931348623157e308;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npoin;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const double* s = U + p * 5;
+    double r = s[0];
+    double u = s[1] / r, v = s[2] / r, w = s[3] / r;
+    double pr = (s[4] - 0.5 * r * (u * u + v * v + w * w)) * (g - 1.0);
+    double c = sqrt(g * fmax(pr, 0.0) / r);
+    double L = cbrt(vol[p]);
+    double vel = sqrt(u * u + v * v + w * w);
+    m = fmin(m, L / fmax(vel + c, 1.0e-8));
+  }
+  m = wave_min(m);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = sm[0];
+    for (int k = 1; k < xgd::kBlock / 64; ++k) b = fmin(b, sm[k]);
+    part[blockIdx.x] = b;
+  }
+}
+
+[[maybe_unused]] __global__ __launch_bounds__(xgd::kRedBlocks) void k_min_final(
+    int n, const double* __restrict__ part, double scale,
+    double* __restrict__ out) {
+  __shared__ double sm[xgd::kRedBlocks / 64];
+  double m = 1.7976931348623157e308;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmin(m, part[i]);
+  m = wave_min(m);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = sm[0];
+    for (int k = 1; k < xgd::kRedBlocks / 64; ++k) b = fmin(b, sm[k]);
+    out[0] = b * scale;
+  }
+}
+
+// Symmetry BCs (physics::symbc, BC.cpp:110-136): removes the normal part of the
+// 3-vector at V[node * stride + off], one (node, normal) pair after the other
+// in list order; the pairs are grouped by node (off[i]..off[i+1]).
+[[maybe_unused]] __global__ void k_symbc(int ns, const int* __restrict__ node,
+                                         const int* __restrict__ off,
+                                         const double* __restrict__ nrm,
+                                         double* __restrict__ V, int stride, int voff) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  double* u = V + (size_t)node[i] * stride + voff;
+  double a = u[0], b = u[1], c = u[2];
+  for (int k = off[i]; k < off[i + 1]; ++k) {
+    const double* n = nrm + (size_t)k * 3;
+    double vn = a * n[0] + b * n[1] + c * n[2];
+    a -= vn * n[0];
+    b -= vn * n[1];
+    c -= vn * n[2];
+  }
+  u[0] = a; u[1] = b; u[2] = c;
 }
 
 }  // namespace

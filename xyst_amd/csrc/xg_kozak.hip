@@ -42,6 +42,18 @@ struct KozakMesh {
   int nfx = 0;
   int *fx_node = nullptr, *fx_off = nullptr, *fx_slot = nullptr;
   double *sU = nullptr, *sR = nullptr;   // staging of the host-pointer entry
+  int nslot = 0;
+  // KozCG stepping (xg_kozcg_setup)
+  bool have_step = false;
+  std::vector<DevArr> step_allocs;
+  double* vol = nullptr;
+  int nsymnode = 0, ndir = 0;
+  int *symnode = nullptr, *symoff = nullptr, *dirnode = nullptr;
+  double *symnorm = nullptr, *dirval = nullptr;
+  uint8_t* dirmask = nullptr;
+  xg_fctcfg fct{};
+  double *W = nullptr, *UL = nullptr, *Q = nullptr, *C = nullptr, *A = nullptr;
+  double *S15 = nullptr, *red = nullptr;
 };
 
 namespace {
@@ -55,6 +67,36 @@ __device__ inline void cross(const double a[3], const double b[3], double c[3]) 
   c[2] = a[0] * b[1] - b[0] * a[1];
 }
 
+// tet Jacobian and corner a's antidiffusive contribution of KozCG::aec
+// (KozCG.cpp:794-812) in the reference's operation order and without fused
+// multiply-adds: the limiter branches on its sign, so it is bit-identical
+__device__ inline void tet_aec(const double X[4][3], const double* sU, int cap,
+                               const int N[4], int a, double ctau, double out[5]) {
+#pragma clang fp contract(off)
+  double ba[3], ca[3], da[3], t[3];
+  for (int j = 0; j < 3; ++j) {
+    ba[j] = X[1][j] - X[0][j];
+    ca[j] = X[2][j] - X[0][j];
+    da[j] = X[3][j] - X[0][j];
+  }
+  t[0] = ca[1] * da[2] - da[1] * ca[2];
+  t[1] = ca[2] * da[0] - da[2] * ca[0];
+  t[2] = ca[0] * da[1] - da[0] * ca[1];
+  const double J = ba[0] * t[0] + ba[1] * t[1] + ba[2] * t[2];
+  const double j120 = J / 120.0;
+  const double mo = j120 * -1.0, md = j120 * 3.0;
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    double s = 0.0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) s += (b == a ? md : mo) * ctau * sU[c * cap + N[b]];
+    out[c] = s;
+  }
+}
+
+// AEC: also the antidiffusive contributions P+/- (KozCG::aec), the output
+// record of a node being then R (5), P+ (5), P- (5)
+template <bool AEC>
 __global__ __launch_bounds__(kKozThreads) void k_kozak_blocks(
     int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
     const int* __restrict__ halo, const int4* __restrict__ chk,
@@ -63,8 +105,9 @@ __global__ __launch_bounds__(kKozThreads) void k_kozak_blocks(
     const uint32_t* __restrict__ tcd, const double* __restrict__ x,
     const double* __restrict__ y, const double* __restrict__ z,
     const double* __restrict__ U, double* __restrict__ R,
-    double* __restrict__ S, double gm1, double dt) {
+    double* __restrict__ S, double gm1, double dt, double ctau) {
   constexpr int CAP = xg::kKozCap;
+  constexpr int NO = AEC ? 15 : 5;
   __shared__ double sU[5 * CAP];
   __shared__ double sX[3 * CAP];
   __shared__ double F[kKozThreads * 5];
@@ -88,7 +131,9 @@ __global__ __launch_bounds__(kKozThreads) void k_kozak_blocks(
     sX[2 * CAP + t] = zg;
   }
   const int i = t >> 2, a = t & 3;
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double acc[NO];
+#pragma unroll
+  for (int c = 0; c < NO; ++c) acc[c] = 0.0;
   for (int ch = B1.y; ch < B1.z; ++ch) {
     const int4 C = chk[ch];   // kind, first, count, tend_off
     const int eoff = chk_ent[ch];
@@ -102,6 +147,7 @@ __global__ __launch_bounds__(kKozThreads) void k_kozak_blocks(
     __syncthreads();          // staging done / previous chunk's F consumed
     if (t < 4 * C.z) sE[t] = ent;
     if (t < nloc) sT[t] = te;
+    double aec[5];
     {
       const int N[4] = {(int)(ab & 0xffff), (int)(ab >> 16), (int)(cd & 0xffff),
                         (int)(cd >> 16)};
@@ -174,6 +220,7 @@ __global__ __launch_bounds__(kKozThreads) void k_kozak_blocks(
 #pragma unroll
         for (int c = 0; c < 5; ++c) F[t * 5 + c] = f[c];
       }
+      if constexpr (AEC) tet_aec(X, sU, CAP, N, a, ctau, aec);
     }
     __syncthreads();          // F, sE, sT complete
     if (t < nloc) {
@@ -184,9 +231,276 @@ __global__ __launch_bounds__(kKozThreads) void k_kozak_blocks(
         for (int c = 0; c < 5; ++c) acc[c] += fs[c];
       }
     }
+    if constexpr (AEC) {
+      __syncthreads();        // fluxes consumed
+      if (i < C.z) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) F[t * 5 + c] = aec[c];
+      }
+      __syncthreads();
+      if (t < nloc) {
+        int e0 = t ? sT[t - 1] : 0, e1 = sT[t];
+        for (int j = e0; j < e1; ++j) {
+          const double* fs = F + (int)sE[j] * 5;
+#pragma unroll
+          for (int c = 0; c < 5; ++c) {
+            acc[5 + c] += fs[c] > 0.0 ? fs[c] : 0.0;     // std::max(0.0, aec)
+            acc[10 + c] += fs[c] < 0.0 ? fs[c] : 0.0;    // std::min(0.0, aec)
+          }
+        }
+      }
+    }
   }
   if (t < nown) {
-    double* o = R + (size_t)(own0 + t) * 5;
+    double* o = R + (size_t)(own0 + t) * NO;
+#pragma unroll
+    for (int c = 0; c < NO; ++c) o[c] = acc[c];
+  } else if (t < nloc) {
+    double* o = S + (size_t)(B1.x + t - nown) * NO;
+#pragma unroll
+    for (int c = 0; c < NO; ++c) o[c] = acc[c];
+  }
+}
+
+// ---------------------------------------------------------------- FCT --
+// The KozCG step after the rhs (KozCG.cpp:866-1197).  W holds per node the
+// summed rhs and P+/- (stride 15); the kernels below follow the reference's
+// arithmetic order without fused multiply-adds wherever a comparison
+// (limiter branch, bound) reads the result.
+
+// std::max / std::min: which argument a tie or a NaN returns
+__device__ inline double smax(double a, double b) { return (a < b) ? b : a; }
+__device__ inline double smin(double a, double b) { return (b < a) ? b : a; }
+
+// low-order solution of node g (KozCG.cpp:889-899)
+__device__ inline void low_order(const double* __restrict__ W,
+                                 const double* __restrict__ U, double vol,
+                                 double dt, int g, double ul[5]) {
+#pragma clang fp contract(off)
+  const double* w = W + (size_t)g * 15;
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    const double pp = w[5 + c] / vol, pn = w[10 + c] / vol;
+    ul[c] = U[(size_t)g * 5 + c] + dt * w[c] / vol - pp - pn;
+  }
+}
+
+__device__ inline int local_to_global(int t, int own0, int nown,
+                                      const int* __restrict__ halo, int hoff) {
+  return t < nown ? own0 + t : halo[hoff + t - nown];
+}
+
+// Allowed limits Q+/- (KozCG.cpp:901-935): per tet the max/min over its
+// corners of the low-order solution (with CLIP = false also of U), taken
+// into every corner's running max/min.  Max and min are exact and
+// order-free, so the block gathers them with LDS atomics; halo partials go
+// to slots that k_fixup_minmax merges.  Owners also store the low-order
+// solution UL.
+template <bool CLIP>
+__global__ __launch_bounds__(kKozThreads) void k_kozcg_alw(
+    int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
+    const int* __restrict__ halo, const int4* __restrict__ chk,
+    const uint32_t* __restrict__ tab, const uint32_t* __restrict__ tcd,
+    const double* __restrict__ W, const double* __restrict__ U,
+    const double* __restrict__ vol, double dt, double* __restrict__ UL,
+    double* __restrict__ Q, double* __restrict__ S) {
+  constexpr int CAP = xg::kKozCap;
+  constexpr double kLarge = 1.7976931348623157e308;
+  __shared__ double sL[5 * CAP];     // low-order solution of the local nodes
+  __shared__ double sQ[10 * CAP];    // running max (5) and min (5)
+  __shared__ int sG[CAP];            // global ids (CLIP = false reads U)
+  const int t = threadIdx.x;
+  const int b = xcd_block(blockIdx.x, nblk);
+  const int4 B0 = blk0[b];
+  const int4 B1 = blk1[b];
+  const int own0 = B0.x, nown = B0.y, nloc = B0.z;
+  if (t < nloc) {
+    const int g = local_to_global(t, own0, nown, halo, B0.w);
+    if (!CLIP) sG[t] = g;
+    double ul[5];
+    low_order(W, U, vol[g], dt, g, ul);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      sL[c * CAP + t] = ul[c];
+      sQ[c * CAP + t] = -kLarge;
+      sQ[(5 + c) * CAP + t] = kLarge;
+      if (t < nown) UL[(size_t)g * 5 + c] = ul[c];
+    }
+  }
+  __syncthreads();
+  const int i = t >> 2, a = t & 3;
+  for (int ch = B1.y; ch < B1.z; ++ch) {
+    const int4 C = chk[ch];
+    const int e = C.y + (i < C.z ? i : 0);
+    const uint32_t w = (a < 2) ? tab[e] : tcd[e];
+    const int na = (a & 1) ? (int)(w >> 16) : (int)(w & 0xffff);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      double hi = sL[c * CAP + na], lo = hi;
+      if (!CLIP) {
+        const double u = U[(size_t)sG[na] * 5 + c];
+        hi = smax(hi, u);
+        lo = smin(lo, u);
+      }
+      hi = fmax(hi, __shfl_xor(hi, 1, 64));
+      hi = fmax(hi, __shfl_xor(hi, 2, 64));
+      lo = fmin(lo, __shfl_xor(lo, 1, 64));
+      lo = fmin(lo, __shfl_xor(lo, 2, 64));
+      if (i < C.z) {
+        __hip_atomic_fetch_max(&sQ[c * CAP + na], hi, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_min(&sQ[(5 + c) * CAP + na], lo, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+    }
+  }
+  __syncthreads();
+  if (t < nloc) {
+    double* o = t < nown ? Q + (size_t)(own0 + t) * 10
+                         : S + (size_t)(B1.x + t - nown) * 10;
+#pragma unroll
+    for (int c = 0; c < 10; ++c) o[c] = sQ[c * CAP + t];
+  }
+}
+
+// merges halo bounds into their owners (max for Q+, min for Q-)
+__global__ __launch_bounds__(kBlock) void k_fixup_minmax(
+    int nfx, const int* __restrict__ fx_node, const int* __restrict__ fx_off,
+    const int* __restrict__ fx_slot, const double* __restrict__ S,
+    double* __restrict__ Q) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nfx) return;
+  double* q = Q + (size_t)fx_node[i] * 10;
+  double v[10];
+#pragma unroll
+  for (int c = 0; c < 10; ++c) v[c] = q[c];
+  for (int k = fx_off[i]; k < fx_off[i + 1]; ++k) {
+    const double* s = S + (size_t)fx_slot[k] * 10;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      v[c] = smax(v[c], s[c]);
+      v[5 + c] = smin(v[5 + c], s[5 + c]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 10; ++c) q[c] = v[c];
+}
+
+// limit coefficients C+/- (KozCG.cpp:1005-1024)
+__global__ __launch_bounds__(kBlock) void k_kozcg_coef(
+    int npoin, const double* __restrict__ W, const double* __restrict__ vol,
+    const double* __restrict__ UL, const double* __restrict__ Q,
+    double* __restrict__ Cf) {
+#pragma clang fp contract(off)
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= npoin) return;
+  constexpr double eps = 2.220446049250313e-16;
+  const double v = vol[g];
+  const double* w = W + (size_t)g * 15;
+  const double* q = Q + (size_t)g * 10;
+  double* o = Cf + (size_t)g * 10;
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    const double pp = w[5 + c] / v, pn = w[10 + c] / v;
+    const double ul = UL[(size_t)g * 5 + c];
+    const double qp = q[c] - ul, qn = q[5 + c] - ul;
+    o[c] = pp < eps ? 0.0 : smin(1.0, qp / pp);
+    o[5 + c] = pn > -eps ? 0.0 : smin(1.0, qn / pn);
+  }
+}
+
+// limited antidiffusive contributions (KozCG.cpp:1049-1078): per tet the
+// coefficient of each component is the min over the corners of C+ or C-
+// (by the sign of the corner's contribution), the fctsys components share
+// their min; the corner adds coef * aec.  Same block/slot scheme as the rhs.
+__global__ __launch_bounds__(kKozThreads) void k_kozcg_lim(
+    int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
+    const int* __restrict__ halo, const int4* __restrict__ chk,
+    const int* __restrict__ chk_ent, const uint16_t* __restrict__ tend,
+    const uint16_t* __restrict__ entries, const uint32_t* __restrict__ tab,
+    const uint32_t* __restrict__ tcd, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ U, const double* __restrict__ Cf,
+    double* __restrict__ A, double* __restrict__ S, double ctau, int sysmask) {
+  constexpr int CAP = xg::kKozCap;
+  __shared__ double sU[5 * CAP];
+  __shared__ double sX[3 * CAP];
+  __shared__ double F[kKozThreads * 5];
+  __shared__ int sG[CAP];
+  __shared__ uint16_t sE[kKozThreads];
+  __shared__ uint16_t sT[CAP];
+  const int t = threadIdx.x;
+  const int b = xcd_block(blockIdx.x, nblk);
+  const int4 B0 = blk0[b];
+  const int4 B1 = blk1[b];
+  const int own0 = B0.x, nown = B0.y, nloc = B0.z;
+  if (t < nloc) {
+    const int g = local_to_global(t, own0, nown, halo, B0.w);
+    sG[t] = g;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) sU[c * CAP + t] = U[(size_t)g * 5 + c];
+    sX[0 * CAP + t] = x[g];
+    sX[1 * CAP + t] = y[g];
+    sX[2 * CAP + t] = z[g];
+  }
+  const int i = t >> 2, a = t & 3;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int ch = B1.y; ch < B1.z; ++ch) {
+    const int4 C = chk[ch];
+    const int eoff = chk_ent[ch];
+    const int e = C.y + (i < C.z ? i : 0);
+    const uint32_t ab = tab[e], cd = tcd[e];
+    uint16_t ent = 0, te = 0;
+    if (t < 4 * C.z) ent = entries[eoff + t];
+    if (t < nloc) te = tend[C.w + t];
+    __syncthreads();          // staging done / previous chunk's F consumed
+    if (t < 4 * C.z) sE[t] = ent;
+    if (t < nloc) sT[t] = te;
+    {
+      const int N[4] = {(int)(ab & 0xffff), (int)(ab >> 16), (int)(cd & 0xffff),
+                        (int)(cd >> 16)};
+      double X[4][3];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) X[k][j] = sX[j * CAP + N[k]];
+      double aec[5], coef[5];
+      tet_aec(X, sU, CAP, N, a, ctau, aec);
+      const double* cf = Cf + (size_t)sG[N[a]] * 10;
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        double q = aec[c] > 0.0 ? cf[c] : cf[5 + c];
+        // min over the four corners, starting from 1 (exact, order-free)
+        q = fmin(q, __shfl_xor(q, 1, 64));
+        q = fmin(q, __shfl_xor(q, 2, 64));
+        coef[c] = smin(1.0, q);
+      }
+      if (sysmask) {
+        double cs = 1.0;
+#pragma unroll
+        for (int c = 0; c < 5; ++c)
+          if (sysmask >> c & 1) cs = smin(cs, coef[c]);
+#pragma unroll
+        for (int c = 0; c < 5; ++c)
+          if (sysmask >> c & 1) coef[c] = cs;
+      }
+      if (i < C.z) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) F[t * 5 + c] = coef[c] * aec[c];
+      }
+    }
+    __syncthreads();
+    if (t < nloc) {
+      int e0 = t ? sT[t - 1] : 0, e1 = sT[t];
+      for (int j = e0; j < e1; ++j) {
+        const double* fs = F + (int)sE[j] * 5;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) acc[c] += fs[c];
+      }
+    }
+  }
+  if (t < nown) {
+    double* o = A + (size_t)(own0 + t) * 5;
 #pragma unroll
     for (int c = 0; c < 5; ++c) o[c] = acc[c];
   } else if (t < nloc) {
@@ -196,8 +510,41 @@ __global__ __launch_bounds__(kKozThreads) void k_kozak_blocks(
   }
 }
 
+// new solution (KozCG.cpp:1144-1160); Un receives the old one
+template <bool FCT>
+__global__ __launch_bounds__(kBlock) void k_kozcg_solve(
+    int npoin, double dt, const double* __restrict__ UL,
+    const double* __restrict__ A, const double* __restrict__ R,
+    const double* __restrict__ vol, double* __restrict__ U,
+    double* __restrict__ Un) {
+#pragma clang fp contract(off)
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (int64_t)npoin * 5) return;
+  const int64_t g = k / 5;
+  const double u = U[k];
+  Un[k] = u;
+  if (FCT)
+    U[k] = UL[k] + A[k] / vol[g];
+  else
+    U[k] = u + dt * R[k] / vol[g];
+}
+
+// Dirichlet BCs (physics::dirbc, BC.cpp:30-69): masked components of the
+// listed nodes take their values
+__global__ __launch_bounds__(kBlock) void k_dirbc(
+    int nd, const int* __restrict__ node, const uint8_t* __restrict__ mask,
+    const double* __restrict__ val, double* __restrict__ U) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nd) return;
+  double* u = U + (size_t)node[i] * 5;
+#pragma unroll
+  for (int c = 0; c < 5; ++c)
+    if (mask[i * 5 + c]) u[c] = val[i * 5 + c];
+}
+
 void free_kozak(KozakMesh* k) {
   if (!k) return;
+  dev_free_all(k->step_allocs);
   dev_free_all(k->allocs);
   delete k;
 }
@@ -205,10 +552,10 @@ void free_kozak(KozakMesh* k) {
 int launch_kozak(xg_ctx* c, double dt, const double* U, double* R) {
   KozakMesh* k = c->koz;
   if (k->nblk)
-    hipLaunchKernelGGL(k_kozak_blocks, dim3(k->nblk), dim3(kKozThreads), 0,
+    hipLaunchKernelGGL(k_kozak_blocks<false>, dim3(k->nblk), dim3(kKozThreads), 0,
                        c->stream, k->nblk, k->blk0, k->blk1, k->halo, k->chk,
                        k->chk_ent, k->tend, k->entries, k->tab, k->tcd, k->x,
-                       k->y, k->z, U, R, k->slots, c->cfg.gamma - 1.0, dt);
+                       k->y, k->z, U, R, k->slots, c->cfg.gamma - 1.0, dt, 0.0);
   if (k->nfx)
     hipLaunchKernelGGL(k_fixup<5>, dim3(nblocks(k->nfx)), dim3(kBlock), 0,
                        c->stream, k->nfx, k->fx_node, k->fx_off, k->fx_slot,
@@ -279,6 +626,7 @@ int xg_kozak_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
     return rc;
   }
   k->nfx = (int)L.fx_node.size();
+  k->nslot = L.nslot;
   free_kozak(c->koz);
   c->koz = k;
   return XG_OK;
@@ -308,6 +656,190 @@ int xg_kozak_rhs(xg_ctx* c, int ncomp, double dt, const double* U, double* R) {
   if ((rc = launch_kozak(c, dt, k->sU, k->sR))) return rc;
   XG_HIP(c, hipMemcpyAsync(R, k->sR, bytes, hipMemcpyDeviceToHost, c->stream));
   XG_HIP(c, hipStreamSynchronize(c->stream));
+  return XG_OK;
+}
+
+int xg_kozcg_setup(xg_ctx* c, const double* vol, int64_t nsym,
+                   const size_t* symnodes, const double* symnorms, int64_t ndir,
+                   const size_t* dirnodes, const int* dirmask,
+                   const double* dirvals, const xg_fctcfg* fct) {
+  if (!c) return XG_EINVAL;
+  if (!c->koz) return fail(c, XG_ENOMESH, "no tet mesh (xg_kozak_upload)");
+  KozakMesh* k = c->koz;
+  if (!vol || nsym < 0 || (nsym && (!symnodes || !symnorms)) || ndir < 0 ||
+      (ndir && (!dirnodes || !dirmask || !dirvals)) || !fct)
+    return fail(c, XG_EINVAL, "xg_kozcg_setup: bad arguments");
+  if (fct->nsys < 0 || fct->nsys > 5)
+    return fail(c, XG_EINVAL, "xg_kozcg_setup: fctsys holds at most 5 components");
+  for (int s = 0; s < fct->nsys; ++s)
+    if (fct->sys[s] < 0 || fct->sys[s] > 4)
+      return fail(c, XG_EINVAL, "xg_kozcg_setup: fctsys component out of range");
+  for (int64_t i = 0; i < k->npoin; ++i)
+    if (!(vol[i] > 0)) return fail(c, XG_EINVAL, "xg_kozcg_setup: non-positive nodal volume");
+  // symmetry (node, normal) pairs grouped by node, list order kept per node
+  std::vector<int> node, off(1, 0);
+  std::vector<double> nrm;
+  {
+    std::vector<int64_t> order(nsym);
+    for (int64_t i = 0; i < nsym; ++i) {
+      if (symnodes[i] >= (size_t)k->npoin)
+        return fail(c, XG_EINVAL, "xg_kozcg_setup: symmetry node out of range");
+      order[i] = i;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+      return symnodes[a] < symnodes[b];
+    });
+    for (int64_t q = 0; q < nsym; ++q) {
+      int64_t i = order[q];
+      if (q == 0 || symnodes[order[q - 1]] != symnodes[i]) {
+        if (q) off.push_back((int)(nrm.size() / 3));
+        node.push_back((int)symnodes[i]);
+      }
+      nrm.insert(nrm.end(), symnorms + i * 3, symnorms + i * 3 + 3);
+    }
+    if (nsym) off.push_back((int)(nrm.size() / 3));
+  }
+  std::vector<int> dn(ndir);
+  std::vector<uint8_t> dm((size_t)ndir * 5);
+  for (int64_t i = 0; i < ndir; ++i) {
+    if (dirnodes[i] >= (size_t)k->npoin)
+      return fail(c, XG_EINVAL, "xg_kozcg_setup: Dirichlet node out of range");
+    dn[i] = (int)dirnodes[i];
+    for (int q = 0; q < 5; ++q) dm[i * 5 + q] = dirmask[i * 5 + q] ? 1 : 0;
+  }
+  XG_HIP(c, hipSetDevice(c->device));
+  (void)hipStreamSynchronize(c->stream);
+  dev_free_all(k->step_allocs);
+  k->have_step = false;
+  const size_t n = (size_t)k->npoin;
+  const size_t nsl = std::max<size_t>(1, (size_t)k->nslot);
+  auto& L = k->step_allocs;
+  int rc;
+  if ((rc = upload(c, L, &k->vol, vol, n)) || (rc = upload(c, L, &k->symnode, node)) ||
+      (rc = upload(c, L, &k->symoff, off)) || (rc = upload(c, L, &k->symnorm, nrm)) ||
+      (rc = upload(c, L, &k->dirnode, dn)) || (rc = upload(c, L, &k->dirmask, dm)) ||
+      (rc = upload(c, L, &k->dirval, dirvals, (size_t)ndir * 5)) ||
+      (rc = dev_alloc(c, L, (void**)&k->W, n * 15 * 8)) ||
+      (rc = dev_alloc(c, L, (void**)&k->UL, n * 5 * 8)) ||
+      (rc = dev_alloc(c, L, (void**)&k->Q, n * 10 * 8)) ||
+      (rc = dev_alloc(c, L, (void**)&k->C, n * 10 * 8)) ||
+      (rc = dev_alloc(c, L, (void**)&k->A, n * 5 * 8)) ||
+      (rc = dev_alloc(c, L, (void**)&k->S15, nsl * 15 * 8)) ||
+      (rc = dev_alloc(c, L, (void**)&k->red, sizeof(double) * (kRedBlocks + 1)))) {
+    dev_free_all(k->step_allocs);
+    return rc;
+  }
+  k->nsymnode = (int)node.size();
+  k->ndir = (int)ndir;
+  k->fct = *fct;
+  k->have_step = true;
+  return XG_OK;
+}
+
+int xg_kozcg_dt_dev(xg_ctx* c, const double* U, double cfl, double* dt) {
+  if (!c) return XG_EINVAL;
+  if (!c->koz || !c->koz->have_step) return fail(c, XG_ENOMESH, "xg_kozcg_setup not called");
+  if (!U || !dt) return fail(c, XG_EINVAL, "xg_kozcg_dt: null argument");
+  XG_HIP(c, hipSetDevice(c->device));
+  KozakMesh* k = c->koz;
+  int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, nblocks(k->npoin)));
+  hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(kBlock), 0, c->stream,
+                     k->npoin, U, k->vol, c->cfg.gamma, k->red);
+  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(kRedBlocks), 0, c->stream, nb,
+                     k->red, cfl, k->red + kRedBlocks);
+  XG_HIP(c, hipGetLastError());
+  XG_HIP(c, hipMemcpyAsync(dt, k->red + kRedBlocks, sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+  XG_HIP(c, hipStreamSynchronize(c->stream));
+  if (*dt < 2.220446049250313e-16)
+    return fail(c, XG_EBLOWUP, "time step below machine epsilon");
+  return XG_OK;
+}
+
+int xg_kozcg_step_dev(xg_ctx* c, double dt, double* U, double* Un) {
+  if (!c) return XG_EINVAL;
+  if (!c->koz || !c->koz->have_step) return fail(c, XG_ENOMESH, "xg_kozcg_setup not called");
+  if (!U || !Un || U == Un) return fail(c, XG_EINVAL, "xg_kozcg_step: bad field pointers");
+  XG_HIP(c, hipSetDevice(c->device));
+  KozakMesh* k = c->koz;
+  const double gm1 = c->cfg.gamma - 1.0;
+  const int n = k->npoin;
+  const int64_t n5 = (int64_t)n * 5;
+  const auto& f = k->fct;
+  if (!f.fct) {
+    if (k->nblk)
+      hipLaunchKernelGGL(k_kozak_blocks<false>, dim3(k->nblk), dim3(kKozThreads), 0,
+                         c->stream, k->nblk, k->blk0, k->blk1, k->halo, k->chk,
+                         k->chk_ent, k->tend, k->entries, k->tab, k->tcd, k->x,
+                         k->y, k->z, U, k->W, k->S15, gm1, dt, 0.0);
+    if (k->nfx)
+      hipLaunchKernelGGL(k_fixup<5>, dim3(nblocks(k->nfx)), dim3(kBlock), 0,
+                         c->stream, k->nfx, k->fx_node, k->fx_off, k->fx_slot,
+                         k->S15, k->W);
+    hipLaunchKernelGGL(k_kozcg_solve<false>, dim3(nblocks(n5)), dim3(kBlock), 0,
+                       c->stream, n, dt, nullptr, nullptr, k->W, k->vol, U, Un);
+  } else {
+    int sysmask = 0;
+    for (int s = 0; s < f.nsys; ++s) sysmask |= 1 << f.sys[s];
+    // rhs and P+/- (KozCG::rhs, aec)
+    if (k->nblk)
+      hipLaunchKernelGGL(k_kozak_blocks<true>, dim3(k->nblk), dim3(kKozThreads), 0,
+                         c->stream, k->nblk, k->blk0, k->blk1, k->halo, k->chk,
+                         k->chk_ent, k->tend, k->entries, k->tab, k->tcd, k->x,
+                         k->y, k->z, U, k->W, k->S15, gm1, dt, f.fctdif);
+    if (k->nfx)
+      hipLaunchKernelGGL(k_fixup<15>, dim3(nblocks(k->nfx)), dim3(kBlock), 0,
+                         c->stream, k->nfx, k->fx_node, k->fx_off, k->fx_slot,
+                         k->S15, k->W);
+    // symmetry BCs on the antidiffusive contributions (KozCG.cpp:816-830)
+    if (k->nsymnode) {
+      hipLaunchKernelGGL(k_symbc, dim3(nblocks(k->nsymnode)), dim3(kBlock), 0,
+                         c->stream, k->nsymnode, k->symnode, k->symoff,
+                         k->symnorm, k->W, 15, 6);
+      hipLaunchKernelGGL(k_symbc, dim3(nblocks(k->nsymnode)), dim3(kBlock), 0,
+                         c->stream, k->nsymnode, k->symnode, k->symoff,
+                         k->symnorm, k->W, 15, 11);
+    }
+    // low-order solution and allowed bounds (KozCG::alw)
+    if (k->nblk) {
+      if (f.fctclip)
+        hipLaunchKernelGGL(k_kozcg_alw<true>, dim3(k->nblk), dim3(kKozThreads), 0,
+                           c->stream, k->nblk, k->blk0, k->blk1, k->halo, k->chk,
+                           k->tab, k->tcd, k->W, U, k->vol, dt, k->UL, k->Q, k->S15);
+      else
+        hipLaunchKernelGGL(k_kozcg_alw<false>, dim3(k->nblk), dim3(kKozThreads), 0,
+                           c->stream, k->nblk, k->blk0, k->blk1, k->halo, k->chk,
+                           k->tab, k->tcd, k->W, U, k->vol, dt, k->UL, k->Q, k->S15);
+    }
+    if (k->nfx)
+      hipLaunchKernelGGL(k_fixup_minmax, dim3(nblocks(k->nfx)), dim3(kBlock), 0,
+                         c->stream, k->nfx, k->fx_node, k->fx_off, k->fx_slot,
+                         k->S15, k->Q);
+    // limit coefficients and limited contributions (KozCG::lim)
+    hipLaunchKernelGGL(k_kozcg_coef, dim3(nblocks(n)), dim3(kBlock), 0, c->stream,
+                       n, k->W, k->vol, k->UL, k->Q, k->C);
+    if (k->nblk)
+      hipLaunchKernelGGL(k_kozcg_lim, dim3(k->nblk), dim3(kKozThreads), 0,
+                         c->stream, k->nblk, k->blk0, k->blk1, k->halo, k->chk,
+                         k->chk_ent, k->tend, k->entries, k->tab, k->tcd, k->x,
+                         k->y, k->z, U, k->C, k->A, k->S15, f.fctdif, sysmask);
+    if (k->nfx)
+      hipLaunchKernelGGL(k_fixup<5>, dim3(nblocks(k->nfx)), dim3(kBlock), 0,
+                         c->stream, k->nfx, k->fx_node, k->fx_off, k->fx_slot,
+                         k->S15, k->A);
+    // new solution (KozCG::solve)
+    hipLaunchKernelGGL(k_kozcg_solve<true>, dim3(nblocks(n5)), dim3(kBlock), 0,
+                       c->stream, n, dt, k->UL, k->A, nullptr, k->vol, U, Un);
+  }
+  // KozCG::BC: Dirichlet, then symmetry
+  if (k->ndir)
+    hipLaunchKernelGGL(k_dirbc, dim3(nblocks(k->ndir)), dim3(kBlock), 0, c->stream,
+                       k->ndir, k->dirnode, k->dirmask, k->dirval, U);
+  if (k->nsymnode)
+    hipLaunchKernelGGL(k_symbc, dim3(nblocks(k->nsymnode)), dim3(kBlock), 0,
+                       c->stream, k->nsymnode, k->symnode, k->symoff, k->symnorm,
+                       U, 5, 1);
+  XG_HIP(c, hipGetLastError());
   return XG_OK;
 }
 

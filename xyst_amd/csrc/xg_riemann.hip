@@ -395,74 +395,6 @@ __global__ void k_update(int npoin, double coef_dt, double* __restrict__ U,
   U[i] = Un[i] - coef_dt * R[i] / vol[i / 5];
 }
 
-// physics::symbc (BC.cpp:110-136): the normals of a node are applied in list
-// order by one thread.
-__global__ void k_symbc(int ns, const int* __restrict__ node,
-                        const int* __restrict__ off,
-                        const double* __restrict__ nrm, double* __restrict__ U) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ns) return;
-  double* u = U + (size_t)node[i] * 5 + 1;
-  double a = u[0], b = u[1], c = u[2];
-  for (int k = off[i]; k < off[i + 1]; ++k) {
-    const double* n = nrm + (size_t)k * 3;
-    double vn = a * n[0] + b * n[1] + c * n[2];
-    a -= vn * n[0];
-    b -= vn * n[1];
-    c -= vn * n[2];
-  }
-  u[0] = a; u[1] = b; u[2] = c;
-}
-
-__device__ inline double wave_min(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-  return v;
-}
-
-// per-node CFL limit of RieCG::dt (RieCG.cpp:827-839), block minimum
-__global__ __launch_bounds__(kBlock) void k_dt_partial(
-    int npoin, const double* __restrict__ U, const double* __restrict__ vol,
-    double g, double* __restrict__ part) {
-  __shared__ double sm[kBlock / 64];
-  double m = 1.7976931348623157e308;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npoin;
-       p += (int64_t)gridDim.x * blockDim.x) {
-    const double* s = U + p * 5;
-    double r = s[0];
-    double u = s[1] / r, v = s[2] / r, w = s[3] / r;
-    double pr = (s[4] - 0.5 * r * (u * u + v * v + w * w)) * (g - 1.0);
-    double c = sqrt(g * fmax(pr, 0.0) / r);
-    double L = cbrt(vol[p]);
-    double vel = sqrt(u * u + v * v + w * w);
-    m = fmin(m, L / fmax(vel + c, 1.0e-8));
-  }
-  m = wave_min(m);
-  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double b = sm[0];
-    for (int k = 1; k < kBlock / 64; ++k) b = fmin(b, sm[k]);
-    part[blockIdx.x] = b;
-  }
-}
-
-__global__ __launch_bounds__(kRedBlocks) void k_min_final(
-    int n, const double* __restrict__ part, double scale,
-    double* __restrict__ out) {
-  __shared__ double sm[kRedBlocks / 64];
-  double m = 1.7976931348623157e308;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmin(m, part[i]);
-  m = wave_min(m);
-  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double b = sm[0];
-    for (int k = 1; k < kRedBlocks / 64; ++k) b = fmin(b, sm[k]);
-    out[0] = b * scale;
-  }
-}
-
 // ------------------------------------------------------------- launchers --
 
 int check_mesh(xg_ctx* c, int ncomp) {
@@ -829,7 +761,7 @@ int xg_riecg_stage_dev(xg_ctx* c, int stage, double dt, double* U, double* Un,
   if (c->nsymnode)
     hipLaunchKernelGGL(k_symbc, dim3(nblocks(c->nsymnode)), dim3(kBlock), 0,
                        c->stream, c->nsymnode, c->symnode, c->symoff,
-                       c->symnorm, U);
+                       c->symnorm, U, 5, 1);
   XG_HIP(c, hipGetLastError());
   return XG_OK;
 }
