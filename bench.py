@@ -138,6 +138,70 @@ def poisson_bench(n, stream, reps=2):
     return out
 
 
+def kozak_rhs_bytes(T, N):
+    """Algorithmic bytes of one kozak::rhs: per tet four int32 ids; per node
+    coordinates (24), U (40) read and R (40) written."""
+    return 16 * T + 104 * N
+
+
+def kozcg_step_bytes(T, N):
+    """Algorithmic bytes of one KozCG FCT step done as the three element
+    sweeps its data dependences require, each reading the tets (16 B) and the
+    node data it needs once: (1) rhs + antidiffusive contributions: U, x in,
+    R and P+/- (120) out; (2) allowed bounds: R, P+/-, U, vol in, Q+/- (80)
+    and the low-order solution (40) out; (3) limiter + solve: U, x, Q+/-,
+    low-order solution, P+/-, vol in, U and Un out."""
+    return 48 * T + (64 + 120) * N + (168 + 120) * N + (312 + 80) * N
+
+
+def kozcg_bench(ctx, m, vol, sn, sv, stream, steps):
+    """kozak::rhs and the KozCG FCT step (reference defaults: fctdif 1,
+    no clipping, no system coupling) on the same Sedov state and mesh."""
+    import torch
+    from xyst_amd import kozak as K
+    from xyst_amd.meshgen import sedov_state
+    t0 = time.time()
+    K.upload(ctx, m["coord"], m["inpoel"])
+    K.setup(ctx, vol, sn, sv, fct=True, fctdif=1.0, fctclip=False, fctsys=())
+    log("kozcg layout: %.1fs" % (time.time() - t0))
+    U = torch.from_numpy(sedov_state(m["coord"], vol)).cuda()
+    R = torch.empty_like(U)
+    Un = torch.empty_like(U)
+    N, T = U.shape[0], m["inpoel"].shape[0]
+    dt = K.dt_dev(ctx, U, 0.5)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    for _ in range(3):
+        K.rhs_dev(ctx, dt, U, R)
+    ev0.record(stream)
+    for _ in range(steps):
+        K.rhs_dev(ctx, dt, U, R)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    rhs_ms = ev0.elapsed_time(ev1) / steps
+    U0 = U.clone()
+    for _ in range(3):
+        K.step_dev(ctx, dt, U, Un)
+    U.copy_(U0)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(steps):
+        K.step_dev(ctx, dt, U, Un)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    step_ms = ev0.elapsed_time(ev1) / steps
+    rb, sb = kozak_rhs_bytes(T, N), kozcg_step_bytes(T, N)
+    out = {"tets": T, "nodes": N, "rhs_ms": rhs_ms, "rhs_tets_per_s": T / (rhs_ms * 1e-3),
+           "rhs_algorithmic_GBs": rb / (rhs_ms * 1e-3) / 1e9,
+           "rhs_frac_hbm": rb / (rhs_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "step_ms": step_ms, "step_algorithmic_GBs": sb / (step_ms * 1e-3) / 1e9,
+           "step_frac_hbm": sb / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "fct": "fctdif 1, fctclip off, fctsys none", "dt": dt}
+    log("kozcg: rhs %.3f ms, step %.3f ms" % (rhs_ms, step_ms))
+    del U, R, Un, U0
+    return out
+
+
 def read_traffic(path):
     """HBM bytes per rhs launch measured by rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE per dispatch, each scaled by what the counter reads for a copy
@@ -185,6 +249,8 @@ def main():
     ap.add_argument("--order", default="rcm+block", choices=["rcm", "rcm+block"])
     ap.add_argument("--poisson-n", type=int, default=128,
                     help="box size of the pressure-Poisson solve (0: skip)")
+    ap.add_argument("--kozcg-steps", type=int, default=20,
+                    help="timed kozak::rhs / KozCG steps on the same mesh (0: skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed region, copy a 512 MiB buffer 3 times "
                          "(the known-size kernel that calibrates HBM byte counters)")
@@ -275,6 +341,8 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if args.kozcg_steps > 0:
+        out["kozcg"] = kozcg_bench(ctx, m, vol, sn, sv, stream, args.kozcg_steps)
     if args.poisson_n > 0:
         ctx.close()
         del dG, dR, dU, dUn
