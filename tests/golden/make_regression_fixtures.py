@@ -1,16 +1,22 @@
-"""Extract the reference's RieCG/Sod and KozCG/Sod regression fixtures into
-riecg_sod.npz and kozcg_sod.npz.
+"""Extract the reference's RieCG/Sod, KozCG/Sod and ChoCG/Lid regression
+fixtures into riecg_sod.npz, kozcg_sod.npz and chocg_lid.npz.
 
-Source files (reference test data, read as data only), per solver directory
+Source files (reference test data, read as data only), per directory
 tests/regression/inciter/{RieCG,KozCG}/Sod:
   rectangle_01_1.5k.exo   input mesh (the same file for both)
   sod.std.exo             fields after 10 steps
   diag.std                per-step diagnostics
-Run configurations (sod.q):
+and tests/regression/inciter/ChoCG/Lid: unitcube_1k.exo, ldc.std.exo,
+diag_ldc.std.
+Run configurations (sod.q, ldc.q):
   RieCG: nstep 10, term 0.2, cfl 0.5, gamma 1.4, symmetry on side sets 2,4,5,6.
   KozCG: nstep 10, cfl 0.5, gamma 1.4, symmetry on 2,4,5,6, Dirichlet (all
          five components, initial values) on side sets 1 and 3, FCT with
          fctclip = true and fctsys = {1, 2, 5}, fctdif at its default 1.0.
+  ChoCG: nstep 10, cfl 0.9, flux damp4, Re 100 (viscosity 0.01), rk 1,
+         noslip on side sets 1,2,3,5,6, velocity (1,0,0) set on side set 4
+         (mask 2), pressure CG: 500 iterations, tol 1e-3, Jacobi, hydrostat
+         node 0; fields written every 2 steps (the last is step 10).
 The output mesh is renumbered by the solver, so output nodes are matched to
 input nodes by coordinates.
 """
@@ -24,8 +30,8 @@ REF = "/root/reference/tests/regression/inciter"
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def read_mesh(ref):
-    f = netcdf_file(os.path.join(ref, "rectangle_01_1.5k.exo"), "r", mmap=False)
+def read_mesh(ref, name="rectangle_01_1.5k.exo"):
+    f = netcdf_file(os.path.join(ref, name), "r", mmap=False)
     coord = np.stack([f.variables[c].data.astype(np.float64)
                       for c in ("coordx", "coordy", "coordz")])
     tri_blk = f.variables["connect1"].data.astype(np.int64) - 1
@@ -54,8 +60,8 @@ def read_mesh(ref):
     return coord, tets, tri, faceset
 
 
-def read_result(ref, coord):
-    g = netcdf_file(os.path.join(ref, "sod.std.exo"), "r", mmap=False)
+def read_result(ref, coord, name="sod.std.exo"):
+    g = netcdf_file(os.path.join(ref, name), "r", mmap=False)
     Y = np.stack([g.variables[c].data.astype(np.float64)
                   for c in ("coordx", "coordy", "coordz")])
     key = lambda P: np.lexsort(np.round(P, 12)[::-1])
@@ -87,6 +93,19 @@ def main():
                             diag=diag, symsets=np.array([2, 4, 5, 6], np.int32),
                             nstep=10, cfl=0.5, gamma=1.4, **extra)
         print("wrote", out, fields.shape, names, t_final)
+    ref = os.path.join(REF, "ChoCG", "Lid")
+    coord, tets, tri, faceset = read_mesh(ref, "unitcube_1k.exo")
+    fields, names, t_final = read_result(ref, coord, "ldc.std.exo")
+    diag = np.loadtxt(os.path.join(ref, "diag_ldc.std"))
+    np.savez_compressed(os.path.join(HERE, "chocg_lid.npz"), coord=coord, inpoel=tets,
+                        triinpoel=tri, faceset=faceset, fields=fields,
+                        field_names=np.array(names), t_final=t_final, diag=diag,
+                        nstep=10, cfl=0.9, mu=0.01, flux="damp4", rk=1,
+                        noslip=np.array([1, 2, 3, 5, 6], np.int32), dirset=4,
+                        dirmask=np.array([2, 2, 2], np.int32),
+                        dirval=np.array([1.0, 0.0, 0.0]), p_iter=500, p_tol=1e-3,
+                        p_hydrostat=0)
+    print("wrote chocg_lid.npz", fields.shape, names, t_final)
 
 
 if __name__ == "__main__":

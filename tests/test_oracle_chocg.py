@@ -136,3 +136,39 @@ def test_rhs_flux_not_configured():
     with pytest.raises(ValueError):
         O.chorin_rhs(O.chocfg(flux="bogus"), se, coord, tri, np.zeros(n),
                      np.zeros((n, 3)), np.zeros((n, 9)))
+
+
+def lid_run():
+    """The reference's ChoCG/Lid case on the oracle (ChoCG.cpp control flow,
+    chorin operators, ConjugateGradients)."""
+    import os
+    from oracle.chocg_step import ChoCGRun
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "chocg_lid.npz"))
+    r = ChoCGRun(d["coord"], d["inpoel"], d["triinpoel"], d["faceset"], float(d["mu"]),
+                 flux=str(d["flux"]), cfl=float(d["cfl"]), rk=int(d["rk"]),
+                 noslip=list(d["noslip"]),
+                 dirbc={int(d["dirset"]): (list(d["dirmask"]), list(d["dirval"]))},
+                 p_hydrostat=int(d["p_hydrostat"]), p_tol=float(d["p_tol"]),
+                 p_iter=int(d["p_iter"]))
+    return r, d
+
+
+def test_lid_regression_fixture():
+    """10 ChoCG steps of the lid-driven cavity (tests/regression/inciter/
+    ChoCG/Lid: damp4 flux with stab, viscosity 0.01, noslip walls, lid
+    velocity, Jacobi-PCG pressure projection to 1e-3) against diag_ldc.std
+    (numdiff 1e-7) and the fields of ldc.std.exo (exodiff bands)."""
+    r, d = lid_run()
+    r.initial()
+    for it in range(int(d["nstep"])):
+        dp = r.step()
+        np.testing.assert_allclose(r.diag(dp), d["diag"][it], rtol=1e-8, atol=1e-14)
+    assert abs(r.t - float(d["t_final"])) < 1e-8
+    mine = {"velocityx": r.U[:, 0], "velocityy": r.U[:, 1], "velocityz": r.U[:, 2],
+            "divergence": r.div / r.vol, "pressure": r.P}
+    rel = {"velocityx": 3e-7, "velocityy": 2e-6, "velocityz": 3e-7}
+    for i, name in enumerate(d["field_names"]):
+        ref = d["fields"][:, i]
+        diff = np.abs(mine[str(name)] - ref)
+        ok = (diff <= rel.get(str(name), 1e-7) * np.abs(ref)) | (np.abs(ref) < 1e-8)
+        assert ok.all(), (name, diff.max())
