@@ -250,6 +250,53 @@ int xg_chorin_flux(xg_ctx* ctx, int dev, double mu, const double* U, int ldu,
 int xg_chorin_rhs(xg_ctx* ctx, int dev, const xg_chocfg* cfg, int ncomp,
                   const double* P, const double* U, const double* G, double* R);
 
+/* ---- resident ChoCG stepping ------------------------------------------------- */
+
+/* ChoCG's g_cfg reads of its time step (InciterConfig.cpp): chorin::rhs
+ * settings, cfl, rk stages (1-4, rkcoef of ChoCG.cpp:43-48) and the pressure
+ * solve's pc (0 none, 1 jacobi), iter and tol. */
+typedef struct {
+  xg_chocfg flux;
+  double    cfl;
+  int       rk;
+  int       p_pc;
+  int64_t   p_maxit;
+  double    p_tol;
+} xg_chostep;
+
+/* Per-mesh data of the ChoCG chare on the mesh of xg_chocg_upload, whose
+ * pressure matrix the caller has given to xg_cg_matrix (ChoCG::prelhs):
+ * nodal volumes; symmetry (node, normal) pairs and noslip nodes
+ * (ChoCG.cpp:637-686); velocity Dirichlet nodes with a 3-component mask and
+ * the values physics::dirbc sets (setupDirBC, ChoCG.cpp:211-301); pressure
+ * Dirichlet nodes (bc_dir of the pressure block and the hydrostat node,
+ * ChoCG.cpp:1051-1112) with their initial-projection values (the pressure
+ * increments of the time steps use 0). */
+int xg_chocg_setup(xg_ctx* ctx, const double* vol, const xg_chostep* cfg,
+                   int64_t nsym, const size_t* symnodes, const double* symnorms,
+                   int64_t nnoslip, const size_t* noslipnodes, int64_t ndir,
+                   const size_t* dirnodes, const int* dirmask,
+                   const double* dirvals, int64_t npdir, const size_t* pdirnodes,
+                   const double* pdirvals);
+
+/* The ChoCG start-up (merge ... finpgrad, ChoCG.cpp:817-1323): BCs, the
+ * divergence-free projection of U, the momentum-flux pressure solve that
+ * sets P, and the pressure gradient.  U (npoin*3), P (npoin) on the device;
+ * pit[0..1] receive the two solves' iteration counts. */
+int xg_chocg_initial_dev(xg_ctx* ctx, double* U, double* P, int64_t* pit);
+
+/* ChoCG::dt (ChoCG.cpp:1357-1413) without a constant dt: cfl * min over
+ * nodes of cbrt(vol)/max(|u|, 1e-8) and cbrt(vol)^2/mu. */
+int xg_chocg_dt_dev(xg_ctx* ctx, const double* U, double* dt);
+
+/* One ChoCG time step (rhs/solve/pred/corr for every RK stage, then
+ * div/pinit/psolve/sgrad/psolved/pgrad, ChoCG.cpp:869-1695) with explicit
+ * momentum stages.  U (npoin*3) advanced in place, Un receives the step's
+ * starting velocity, P += dP with dP (npoin) the pressure increment; *pit
+ * the pressure solve's iterations. */
+int xg_chocg_step_dev(xg_ctx* ctx, double dt, double* U, double* Un, double* P,
+                      double* dP, int64_t* pit);
+
 /* ---- pressure-Poisson system (ChoCG) and conjugate gradients ---------------- */
 
 /* tk::CSR(ncomp, psup) (CSR.hpp:29-31, CSR.cpp:19-84): the sparsity pattern
@@ -284,6 +331,14 @@ int xg_cg_matrix(xg_ctx* ctx, int ncomp, int64_t npoin, const size_t* ia,
 int xg_cg_init(xg_ctx* ctx, const double* x, const double* b,
                const double* neubc, int64_t ndir, const size_t* dirnode,
                const int32_t* dirmask, const double* dirval, int apply, int pc);
+
+/* xg_cg_init with the rhs b on the device (npoin*ncomp) and x kept from the
+ * previous solve (the warm start of ConjugateGradients::init with an empty
+ * x, as ChoCG's pressure solves use it, ChoCG.cpp:1122-1124); no Neumann
+ * term.  Same Dirichlet semantics as xg_cg_init. */
+int xg_cg_init_dev(xg_ctx* ctx, const double* b, int64_t ndir,
+                   const size_t* dirnode, const int32_t* dirmask,
+                   const double* dirval, int apply, int pc);
 
 /* ConjugateGradients::setup (ConjugateGradients.cpp:105-200): r = b - A x,
  * preconditioner, normb = ||b||, rho = (r,z). */

@@ -114,6 +114,31 @@ def test_cg_apply_matches_oracle(pc):
     np.testing.assert_allclose(cg.solution(), x2, rtol=0, atol=1e-9 * np.abs(x2).max())
 
 
+@pytest.mark.parametrize("tol", [1e-2, 3e-3, 1e-3, 3e-4, 1e-4, 3e-5, 1e-5, 3e-6])
+def test_cg_stop_by_tolerance_keeps_the_iterate(tol):
+    """A solve that stops on the tolerance returns the iterate of the
+    recurrence at that count: the launches queued after the stop (the host
+    polls in batches) must not touch x.  Loose tolerances put the stop at
+    every position within a batch; x is compared to the oracle's iterate to
+    round-off (the recurrence itself agrees to ~1e-11 here)."""
+    _say("cg stop by tolerance tol=%g" % tol)
+    coord, inpoel, A = _poisson_case(8)
+    n = coord.shape[1]
+    b = np.cos(2 * coord[0]) + coord[2] * coord[1]
+    dirbc = {0: [(1, 0.0)]}
+    xo, bo = np.zeros(n), b.copy()
+    _, nro, ito, _ = O.cg(A, xo, bo, dirbc=dirbc, apply=True, pc="jacobi", maxit=1000,
+                          tol=tol)
+    cg = _X().ConjugateGradients(_dev_matrix(A))
+    cg.init(np.zeros(n), b, dirbc=dirbc, apply=True, pc="jacobi")
+    cg.solve(1000, tol)
+    assert cg.it == ito
+    x = cg.solution()
+    err = float(np.abs(x - xo).max() / np.abs(xo).max())
+    _say("  it %d, |x - x_oracle| / max|x| = %.3e" % (ito, err))
+    assert err < 1e-9
+
+
 def test_cg_residual_history_matches_oracle():
     """Iterate by iterate: with maxit = k the device and oracle residual norms
     agree to round-off for the first iterations."""

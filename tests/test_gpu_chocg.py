@@ -125,3 +125,126 @@ def test_chorin_device_pointers_match_host_entry():
     assert np.array_equal(dR.cpu().numpy(), R)
     assert np.array_equal(dD.cpu().numpy(), D)
     ctx.close()
+
+
+# ---------------------------------------------------------------- ChoCG step --
+
+def _device_run(r, cfgkw):
+    """The device ChoCG on the mesh and BCs of an oracle ChoCGRun r."""
+    import torch
+    from xyst_amd import chorin as X
+    from xyst_amd import riemann as R
+    from xyst_amd.cg import CSR, ConjugateGradients
+    ctx = R.Context(0)
+    X.upload(ctx, r.coord, r.se.sup, r.se.int, r.tri)
+    cg = ConjugateGradients(CSR.poisson(r.coord, r.inpoel), ctx=ctx)
+    dn = np.array(sorted(r.dir), np.int64)
+    dm = np.array([r.dir[p][0] for p in dn], np.int32).reshape(-1, 3)
+    dv = np.array([r.dir[p][1] for p in dn]).reshape(-1, 3)
+    dm = (dm == 2).astype(np.int32)           # mask 2: the configured value
+    pd = sorted(r.pdir | ({r.hydrostat} if r.hydrostat is not None else set()))
+    X.step_setup(ctx, r.vol, X.config(**cfgkw), cfl=r.cfl, rk=len(r.rkcoef),
+                 p_pc=r.p_pc, p_maxit=r.p_iter, p_tol=r.p_tol,
+                 symnodes=r.symnodes, symnorms=r.symnorms, noslip=r.noslip,
+                 dirnodes=dn, dirmask=dm, dirvals=dv, pdirnodes=np.array(pd, np.int64),
+                 pdirvals=np.zeros(len(pd)))
+    U = torch.tensor(r.U, device="cuda")
+    P = torch.tensor(r.P, device="cuda")
+    return ctx, cg, U, P
+
+
+def _lid_oracle():
+    import os
+    from oracle.chocg_step import ChoCGRun
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "chocg_lid.npz"))
+    r = ChoCGRun(d["coord"], d["inpoel"], d["triinpoel"], d["faceset"], float(d["mu"]),
+                 flux=str(d["flux"]), cfl=float(d["cfl"]), rk=int(d["rk"]),
+                 noslip=list(d["noslip"]),
+                 dirbc={int(d["dirset"]): (list(d["dirmask"]), list(d["dirval"]))},
+                 p_hydrostat=int(d["p_hydrostat"]), p_tol=float(d["p_tol"]),
+                 p_iter=int(d["p_iter"]))
+    return r, d
+
+
+def test_chocg_lid_regression_fixture():
+    """The reference's ChoCG/Lid case on the device: start-up projections and
+    10 steps, diagnostics against diag_ldc.std (numdiff 1e-7) and the fields
+    of ldc.std.exo (exodiff bands); pressure iterations equal the oracle's."""
+    import torch
+    from xyst_amd import chorin as X
+    _say("chocg lid fixture")
+    r, d = _lid_oracle()
+    ctx, cg, U, P = _device_run(r, dict(flux="damp4", stab=True, mu=float(d["mu"])))
+    pit = X.initial_dev(ctx, U, P)
+    r.initial()
+    assert list(pit) == r.pit[:2], (pit, r.pit)
+    Un = torch.empty_like(U)
+    dP = torch.empty_like(P)
+    V = r.vol
+    t = 0.0
+    for it in range(int(d["nstep"])):
+        dt = X.dt_dev(ctx, U)
+        assert dt == pytest.approx(r.dt_cfl(), rel=1e-14)
+        n_it = X.step_dev(ctx, dt, U, Un, P, dP)
+        r.step(dt)
+        assert n_it == r.pit[-1]
+        torch.cuda.synchronize()
+        u, un, p, dp = (a.cpu().numpy() for a in (U, Un, P, dP))
+        t += dt
+        s = [np.sum(p * p * V)] + [np.sum(u[:, c] ** 2 * V) for c in range(3)]
+        s += [np.sum(dp * dp * V)] + [np.sum((u[:, c] - un[:, c]) ** 2 * V) for c in range(3)]
+        row = np.concatenate([[it + 1, t, dt], np.sqrt(np.array(s) / V.sum())])
+        np.testing.assert_allclose(row, d["diag"][it], rtol=1e-7, atol=1e-14)
+        err = float(np.max(np.abs(u - r.U)) / np.max(np.abs(r.U)))
+        _say("  step %d pit %d |dU|/max|U| vs oracle %.3e" % (it + 1, n_it, err))
+        assert err < 1e-9
+    mine = {"velocityx": u[:, 0], "velocityy": u[:, 1], "velocityz": u[:, 2],
+            "pressure": p}
+    rel = {"velocityx": 3e-7, "velocityy": 2e-6, "velocityz": 3e-7}
+    for i, name in enumerate(d["field_names"]):
+        if str(name) not in mine:
+            continue
+        ref = d["fields"][:, i]
+        diff = np.abs(mine[str(name)] - ref)
+        ok = (diff <= rel.get(str(name), 1e-7) * np.abs(ref)) | (np.abs(ref) < 1e-8)
+        assert ok.all(), (name, diff.max())
+    ctx.close()
+
+
+@pytest.mark.parametrize("flux,rk", [("damp2", 2), ("damp4", 3)])
+def test_chocg_step_parity_box(flux, rk):
+    """A 12^3 box (RCM + block order), lid-driven cavity BCs with a
+    symmetry wall, damp2/damp4, RK2/RK3: three steps against the oracle's
+    ChoCG, fields within 1e-9 of their max (the pressure solves stop at
+    1e-6, so round-off moves the iterates only at that order times the
+    recurrence's sensitivity)."""
+    import torch
+    from oracle.chocg_step import ChoCGRun
+    from xyst_amd import chorin as X
+    from xyst_amd import riemann as R
+    _say("chocg box %s rk%d" % (flux, rk))
+    m = box_mesh(12, seed=9)
+    m = renumber_mesh(m, R.mesh_rcm(m["coord"].shape[1], m["inpoel"]))
+    m = renumber_mesh(m, R.mesh_block_order(m["coord"], 2048))
+    r = ChoCGRun(m["coord"], m["inpoel"], m["triinpoel"], m["faceset"], 0.01, flux=flux,
+                 cfl=0.5, rk=rk, noslip=[1, 2, 3, 4], sym=[5],
+                 dirbc={6: ([2, 2, 2], [1.0, 0.0, 0.0])}, p_hydrostat=0, p_tol=1e-6,
+                 p_iter=1000)
+    ctx, cg, U, P = _device_run(r, dict(flux=flux, stab=True, mu=0.01))
+    pit = X.initial_dev(ctx, U, P)
+    r.initial()
+    assert list(pit) == r.pit[:2]
+    Un = torch.empty_like(U)
+    dP = torch.empty_like(P)
+    for it in range(3):
+        dt = X.dt_dev(ctx, U)
+        n_it = X.step_dev(ctx, dt, U, Un, P, dP)
+        r.step(dt)
+        torch.cuda.synchronize()
+        u, p = U.cpu().numpy(), P.cpu().numpy()
+        eu = float(np.max(np.abs(u - r.U)) / np.max(np.abs(r.U)))
+        ep = float(np.max(np.abs(p - r.P)) / np.max(np.abs(r.P)))
+        _say("  step %d pit %d/%d |dU| %.3e |dP| %.3e" % (it + 1, n_it, r.pit[-1], eu, ep))
+        assert abs(n_it - r.pit[-1]) <= 1
+        assert eu < 1e-9 and ep < 1e-9
+    ctx.close()

@@ -44,6 +44,12 @@ SIG = {
     "xg_kozak_rhs": (C.c_int, [VP, C.c_int, D, VP, VP]),
     "xg_kozak_rhs_dev": (C.c_int, [VP, C.c_int, D, VP, VP]),
     "xg_kozcg_setup": (C.c_int, [VP, VP, I64, VP, VP, I64, VP, VP, VP, VP]),
+    "xg_chocg_setup": (C.c_int, [VP, VP, VP, I64, VP, VP, I64, VP, I64, VP, VP, VP,
+                                 I64, VP, VP]),
+    "xg_chocg_initial_dev": (C.c_int, [VP, VP, VP, VP]),
+    "xg_chocg_dt_dev": (C.c_int, [VP, VP, VP]),
+    "xg_chocg_step_dev": (C.c_int, [VP, D, VP, VP, VP, VP, VP]),
+    "xg_cg_init_dev": (C.c_int, [VP, VP, I64, VP, VP, VP, C.c_int, C.c_int]),
     "xg_kozcg_dt_dev": (C.c_int, [VP, VP, D, VP]),
     "xg_kozcg_step_dev": (C.c_int, [VP, D, VP, VP]),
     "xg_chocg_upload": (C.c_int, [VP, I64, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP,
@@ -94,6 +100,11 @@ class Config(C.Structure):
 class ChoConfig(C.Structure):
     _fields_ = [("flux", C.c_int), ("stab", C.c_int), ("stab2", C.c_int),
                 ("stab2coef", C.c_double), ("mu", C.c_double)]
+
+
+class ChoStep(C.Structure):
+    _fields_ = [("flux", ChoConfig), ("cfl", C.c_double), ("rk", C.c_int),
+                ("p_pc", C.c_int), ("p_maxit", C.c_int64), ("p_tol", C.c_double)]
 
 
 class FctConfig(C.Structure):

@@ -1,5 +1,5 @@
-"""Python mirror of the chorin:: edge operators (Chorin.hpp:19-70) over the
-C-ABI.
+"""Python mirror of the chorin:: edge operators (Chorin.hpp:19-70) and of
+the ChoCG time step (ChoCG.cpp:817-1695) over the C-ABI.
 
 The mesh arguments (ChoCG superedges with 5 values per edge integral,
 coordinates, boundary triangles) are uploaded once per context with
@@ -12,9 +12,10 @@ import ctypes as C
 
 import numpy as np
 
-from ._xg import ChoConfig, lib, ptr
+from ._xg import ChoConfig, ChoStep, lib, ptr
 
-__all__ = ["upload", "div", "grad", "vgrad", "flux", "rhs", "config"]
+__all__ = ["upload", "div", "grad", "vgrad", "flux", "rhs", "config", "step_setup",
+           "initial_dev", "dt_dev", "step_dev"]
 
 FLUX = {"damp2": 0, "damp4": 1}
 
@@ -94,3 +95,55 @@ def rhs(ctx, cfg, P, U, G, R=None):
     ctx._check(lib().xg_chorin_rhs(ctx._h, dev, C.byref(cfg), _cols(U), ptr(P), ptr(U),
                                    ptr(G), ptr(R)))
     return R
+
+
+def step_setup(ctx, vol, cfg, cfl=0.9, rk=1, p_pc="jacobi", p_maxit=500, p_tol=1e-3,
+               symnodes=(), symnorms=(), noslip=(), dirnodes=(), dirmask=None,
+               dirvals=None, pdirnodes=(), pdirvals=None):
+    """ChoCG per-mesh data (after `upload` and the pressure matrix given to
+    the context's ConjugateGradients): cfg from `config`; velocity Dirichlet
+    nodes with (ndir, 3) mask and values; pressure Dirichlet nodes (bc_dir of
+    the pressure block and the hydrostat node) with their initial values."""
+    st = ChoStep(cfg, float(cfl), int(rk), {"none": 0, "jacobi": 1}[p_pc], int(p_maxit),
+                 float(p_tol))
+    v = _f64(vol)
+    sn, sv = _idx(symnodes), _f64(symnorms).reshape(-1)
+    ns = _idx(noslip)
+    dn = _idx(dirnodes)
+    nd = dn.size
+    dm = np.ascontiguousarray(np.ones((nd, 3)) if dirmask is None else dirmask,
+                              dtype=np.int32).reshape(-1)
+    dv = _f64(np.zeros((nd, 3)) if dirvals is None else dirvals).reshape(-1)
+    pn = _idx(pdirnodes)
+    pv = _f64(np.zeros(pn.size) if pdirvals is None else pdirvals).reshape(-1)
+    if v.size != ctx.cho_npoin or sv.size != 3 * sn.size or dm.size != 3 * nd or \
+            dv.size != 3 * nd or pv.size != pn.size:
+        raise ValueError("step_setup: inconsistent array sizes")
+    ctx._check(lib().xg_chocg_setup(ctx._h, ptr(v), C.byref(st), sn.size, ptr(sn), ptr(sv),
+                                    ns.size, ptr(ns), nd, ptr(dn), ptr(dm), ptr(dv),
+                                    pn.size, ptr(pn), ptr(pv)))
+
+
+def initial_dev(ctx, U, P):
+    """ChoCG start-up on device tensors U (npoin, 3), P (npoin,); returns the
+    two pressure solves' iteration counts."""
+    it = (C.c_int64 * 2)()
+    ctx._check(lib().xg_chocg_initial_dev(ctx._h, ptr(U), ptr(P), it))
+    return it[0], it[1]
+
+
+def dt_dev(ctx, U):
+    out = C.c_double()
+    ctx._check(lib().xg_chocg_dt_dev(ctx._h, ptr(U), C.byref(out)))
+    return out.value
+
+
+def step_dev(ctx, dt, U, Un, P, dP):
+    """One ChoCG step; returns the pressure solve's iteration count."""
+    for a, nc in ((U, 3), (Un, 3)):
+        if tuple(a.shape) != (ctx.cho_npoin, nc) or not a.is_cuda:
+            raise ValueError("device fields must be (%d, 3) on the GPU" % ctx.cho_npoin)
+    it = C.c_int64()
+    ctx._check(lib().xg_chocg_step_dev(ctx._h, float(dt), ptr(U), ptr(Un), ptr(P), ptr(dP),
+                                       C.byref(it)))
+    return it.value

@@ -74,6 +74,7 @@ struct CgSolver {
   double* rzb = nullptr;         // 2 x kBufLen partials of (r,z), (r,r)
   CgState* st = nullptr;
   CgState* hst = nullptr;        // pinned host mirror
+  DevArr bcbuf;                  // Dirichlet rows/values of xg_cg_init_dev
   int pc = 0;
   bool ready = false;            // setup done
 };
@@ -222,6 +223,9 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(
     T* __restrict__ pnew, T* __restrict__ q, const double* __restrict__ rzb,
     int nvp, double* __restrict__ pq, CgState* st, long long maxit, double tol_nb) {
   __shared__ double sm[kCgThreads / 64];
+  // after the stop the partial sums of the two buffers are one update apart
+  // from each other no more: every later launch of the batch is a no-op
+  if (st->done) return;
   const StopTest s = stop_test(rzb + ((j - 1) & 1) * kBufLen, nvp, j, maxit, tol_nb, sm);
   if (s.stop) {
     record_stop(st, j, s);
@@ -258,8 +262,9 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_update(
     int j, int64_t n, const T* __restrict__ q, const T* __restrict__ d,
     const T* __restrict__ p, T* __restrict__ r, T* __restrict__ z, T* __restrict__ x,
     double* __restrict__ rzb, int nvp, const double* __restrict__ pq, int nsp,
-    long long maxit, double tol_nb, double brk) {
+    const CgState* st, long long maxit, double tol_nb, double brk) {
   __shared__ double sm[kCgThreads / 64];
+  if (st->done) return;
   const StopTest s = stop_test(rzb + ((j - 1) & 1) * kBufLen, nvp, j, maxit, tol_nb, sm);
   if (s.stop) return;
   const double PQ = sum_parts(pq, nsp, sm);
@@ -480,7 +485,32 @@ void free_solver(CgSolver* s) {
   if (!s) return;
   dev_free_all(s->allocs);
   if (s->hst) (void)hipHostFree(s->hst);
+  if (s->bcbuf.p) (void)hipFree(s->bcbuf.p);
   delete s;
+}
+
+// b[rows[i]] -= vals[i] for i < nsub (the rr of CSR::dirichlet), then, in a
+// second launch, b[rows[i]] = vals[i] for the Dirichlet rows
+__global__ void k_rows_sub(int n, const int* __restrict__ rows,
+                           const double* __restrict__ vals, double* __restrict__ b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[rows[i]] -= vals[i];
+}
+
+__global__ void k_rows_set(int n, const int* __restrict__ rows,
+                           const double* __restrict__ vals, double* __restrict__ b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[rows[i]] = vals[i];
+}
+
+int grow(xg_ctx* c, DevArr& d, size_t bytes) {
+  if (d.bytes >= bytes) return XG_OK;
+  if (d.p) (void)hipFree(d.p);
+  d.p = nullptr;
+  d.bytes = 0;
+  if (hipMalloc(&d.p, bytes) != hipSuccess) return fail(c, XG_EHIP, "hipMalloc (CG BC buffer)");
+  d.bytes = bytes;
+  return XG_OK;
 }
 
 // Device values back to the matrix as given to xg_cg_matrix.
@@ -528,7 +558,7 @@ int run_cg(xg_ctx* c, const T* a, T* z, T* p0, T* p1, T* q, const T* d, T* r, T*
                          s->nrow, s->dia, s->dja, a, z, pold, pnew, q, s->rzb, gv,
                          s->pq, s->st, (long long)maxit, tol_nb);
       hipLaunchKernelGGL(k_cg_update<T>, dim3(gv), dim3(kCgThreads), 0, c->stream, j,
-                         s->nrow, q, d, pnew, r, z, x, s->rzb, gv, s->pq, gr,
+                         s->nrow, q, d, pnew, r, z, x, s->rzb, gv, s->pq, gr, s->st,
                          (long long)maxit, tol_nb, brk);
     }
     XG_HIP(c, hipGetLastError());
@@ -538,9 +568,80 @@ int run_cg(xg_ctx* c, const T* a, T* z, T* p0, T* p1, T* q, const T* d, T* r, T*
   }
 }
 
+// ConjugateGradients::apply's Dirichlet rows (CSR::dirichlet in decreasing
+// node order): makes the device matrix carry them, rebuilt from the original
+// values when the (node, component) set differs from the one it holds; when
+// rr is given, rr += A_col * val as CSR::dirichlet accumulates it.
+int cg_dirichlet(xg_ctx* c, CgSolver* s, int64_t ndir, const size_t* dirnode,
+                 const int32_t* dirmask, const double* dirval, std::vector<double>* rr) {
+  const size_t n = (size_t)s->nrow;
+  std::vector<std::pair<size_t, int64_t>> order(ndir);
+  for (int64_t k = 0; k < ndir; ++k) {
+    if (dirnode[k] * (size_t)s->ncomp >= n)
+      return fail(c, XG_EINVAL, "xg_cg_init: Dirichlet node out of range");
+    order[k] = {dirnode[k], k};
+  }
+  std::sort(order.begin(), order.end());
+  // the BC-modified matrix depends only on which (node, component) pairs are
+  // Dirichlet: with the same set as the matrix on the device holds, only b is
+  // rebuilt
+  std::vector<int64_t> key;
+  key.reserve(ndir * (1 + s->ncomp));
+  for (int64_t k = 0; k < ndir; ++k) {
+    int64_t i = order[k].second;
+    key.push_back((int64_t)dirnode[i]);
+    for (int j = 0; j < s->ncomp; ++j) key.push_back(dirmask[i * s->ncomp + j] ? 1 : 0);
+  }
+  const bool same = s->dev_bc && key == s->bc_key;
+  if (same) {
+    if (!rr) return XG_OK;
+    // entries of the original columns equal the progressively zeroed ones in
+    // every row that is not itself a Dirichlet row, and those rows of b are
+    // overwritten with their values afterwards
+    for (int64_t k = ndir - 1; k >= 0; --k) {
+      int64_t i = order[k].second;
+      for (int j = 0; j < s->ncomp; ++j) {
+        if (!dirmask[i * s->ncomp + j]) continue;
+        const int64_t col = (int64_t)dirnode[i] * s->ncomp + j;
+        const double val = dirval[i * s->ncomp + j];
+        for (int64_t e = s->ia[col]; e < s->ia[col + 1]; ++e) {
+          const int64_t r = s->ja[e];
+          const int32_t* b0 = s->ja.data() + s->ia[r];
+          const int32_t* b1 = s->ja.data() + s->ia[r + 1];
+          const int32_t* f = std::lower_bound(b0, b1, (int32_t)col);
+          if (f != b1 && *f == col) (*rr)[r] += s->a0[f - s->ja.data()] * val;
+        }
+      }
+    }
+    return XG_OK;
+  }
+  std::vector<double> a(s->a0);
+  std::vector<double> scratch;
+  std::vector<double>* acc = rr;
+  if (!acc) { scratch.assign(n, 0.0); acc = &scratch; }
+  for (int64_t k = ndir - 1; k >= 0; --k) {
+    int64_t i = order[k].second;
+    for (int j = 0; j < s->ncomp; ++j)
+      if (dirmask[i * s->ncomp + j])
+        xg::csr_dirichlet(s->ncomp, s->ia.data(), s->ja.data(), a.data(),
+                          (int64_t)dirnode[i], dirval[i * s->ncomp + j], acc->data(), j);
+  }
+  std::vector<float> a32(a.begin(), a.end());
+  XG_HIP(c, hipStreamSynchronize(c->stream));
+  XG_HIP(c, hipMemcpy(s->da, a.data(), a.size() * 8, hipMemcpyHostToDevice));
+  XG_HIP(c, hipMemcpy(s->da32, a32.data(), a32.size() * 4, hipMemcpyHostToDevice));
+  s->bc_key.swap(key);
+  return XG_OK;
+}
+
 }  // namespace
 
 // =================================================================== C-ABI ==
+
+int64_t xg_cg_rows(const xg_ctx* c) {
+  if (!c || !c->cg || c->cg->ncomp != 1) return -1;
+  return c->cg->nrow;
+}
 
 extern "C" {
 
@@ -648,65 +749,75 @@ int xg_cg_init(xg_ctx* c, const double* x, const double* b, const double* neubc,
   if (b) std::memcpy(bb.data(), b, n * 8);
   else XG_HIP(c, hipMemcpy(bb.data(), s->b, n * 8, hipMemcpyDeviceToHost));
   if (neubc) for (size_t i = 0; i < n; ++i) bb[i] += neubc[i];
-  std::vector<std::pair<size_t, int64_t>> order(ndir);
-  for (int64_t k = 0; k < ndir; ++k) {
-    if (dirnode[k] * (size_t)s->ncomp >= n)
-      return fail(c, XG_EINVAL, "xg_cg_init: Dirichlet node out of range");
-    order[k] = {dirnode[k], k};
-  }
-  std::sort(order.begin(), order.end());
-  // the BC-modified matrix depends only on which (node, component) pairs are
-  // Dirichlet: with the same set as the matrix on the device holds, only b is
-  // rebuilt
-  std::vector<int64_t> key;
-  key.reserve(ndir * (1 + s->ncomp));
-  for (int64_t k = 0; k < ndir; ++k) {
-    int64_t i = order[k].second;
-    key.push_back((int64_t)dirnode[i]);
-    for (int j = 0; j < s->ncomp; ++j) key.push_back(dirmask[i * s->ncomp + j] ? 1 : 0);
-  }
-  const bool same = s->dev_bc && key == s->bc_key;
   std::vector<double> rr(n, 0.0);
-  if (same) {
-    // b -= A_col * val as CSR::dirichlet accumulates it, in decreasing node
-    // order; entries of the original columns equal the progressively zeroed
-    // ones in every row that is not itself a Dirichlet row, and those rows of
-    // b are overwritten with their values below
-    for (int64_t k = ndir - 1; k >= 0; --k) {
-      int64_t i = order[k].second;
-      for (int j = 0; j < s->ncomp; ++j) {
-        if (!dirmask[i * s->ncomp + j]) continue;
-        const int64_t col = (int64_t)dirnode[i] * s->ncomp + j;
-        const double val = dirval[i * s->ncomp + j];
-        for (int64_t e = s->ia[col]; e < s->ia[col + 1]; ++e) {
-          const int64_t r = s->ja[e];
-          const int32_t* b0 = s->ja.data() + s->ia[r];
-          const int32_t* b1 = s->ja.data() + s->ia[r + 1];
-          const int32_t* f = std::lower_bound(b0, b1, (int32_t)col);
-          if (f != b1 && *f == col) rr[r] += s->a0[f - s->ja.data()] * val;
-        }
-      }
-    }
-  } else {
-    std::vector<double> a(s->a0);
-    for (int64_t k = ndir - 1; k >= 0; --k) {
-      int64_t i = order[k].second;
-      for (int j = 0; j < s->ncomp; ++j)
-        if (dirmask[i * s->ncomp + j])
-          xg::csr_dirichlet(s->ncomp, s->ia.data(), s->ja.data(), a.data(),
-                            (int64_t)dirnode[i], dirval[i * s->ncomp + j], rr.data(), j);
-    }
-    std::vector<float> a32(a.begin(), a.end());
-    XG_HIP(c, hipMemcpy(s->da, a.data(), a.size() * 8, hipMemcpyHostToDevice));
-    XG_HIP(c, hipMemcpy(s->da32, a32.data(), a32.size() * 4, hipMemcpyHostToDevice));
-    s->bc_key.swap(key);
-  }
+  if ((rc = cg_dirichlet(c, s, ndir, dirnode, dirmask, dirval, &rr))) return rc;
   for (size_t i = 0; i < n; ++i) bb[i] -= rr[i];
   for (int64_t k = 0; k < ndir; ++k)
     for (int j = 0; j < s->ncomp; ++j)
       if (dirmask[k * s->ncomp + j])
         bb[dirnode[k] * s->ncomp + j] = dirval[k * s->ncomp + j];
   XG_HIP(c, hipMemcpy(s->b, bb.data(), n * 8, hipMemcpyHostToDevice));
+  s->dev_bc = true;
+  s->restore = false;
+  return XG_OK;
+}
+
+int xg_cg_init_dev(xg_ctx* c, const double* b, int64_t ndir, const size_t* dirnode,
+                   const int32_t* dirmask, const double* dirval, int apply, int pc) {
+  int rc = check_cg(c);
+  if (rc) return rc;
+  CgSolver* s = c->cg;
+  if (pc != 0 && pc != 1) return fail(c, XG_EINVAL, "pc must be 0 (none) or 1 (jacobi)");
+  if (!b) return fail(c, XG_EINVAL, "xg_cg_init_dev: null rhs");
+  if (ndir < 0 || (ndir && (!dirnode || !dirmask || !dirval)))
+    return fail(c, XG_EINVAL, "xg_cg_init_dev: bad Dirichlet arrays");
+  XG_HIP(c, hipSetDevice(c->device));
+  const size_t n = (size_t)s->nrow;
+  s->pc = pc;
+  s->ready = false;
+  if (b != s->b)
+    XG_HIP(c, hipMemcpyAsync(s->b, b, n * 8, hipMemcpyDeviceToDevice, c->stream));
+  if (!apply) {
+    if (s->dev_bc && (rc = restore_values(c))) return rc;
+    return XG_OK;
+  }
+  bool nonzero = false;
+  for (int64_t k = 0; k < ndir; ++k)
+    for (int j = 0; j < s->ncomp; ++j)
+      if (dirmask[k * s->ncomp + j] && dirval[k * s->ncomp + j] != 0.0) nonzero = true;
+  // b -= A_col * val only matters for nonzero values (b - 0.0 == b)
+  std::vector<double> rr;
+  if (nonzero) rr.assign(n, 0.0);
+  if ((rc = cg_dirichlet(c, s, ndir, dirnode, dirmask, dirval, nonzero ? &rr : nullptr)))
+    return rc;
+  std::vector<int> rows;
+  std::vector<double> vals;
+  if (nonzero)
+    for (size_t i = 0; i < n; ++i)
+      if (rr[i] != 0.0) { rows.push_back((int)i); vals.push_back(rr[i]); }
+  const size_t nsub = rows.size();
+  for (int64_t k = 0; k < ndir; ++k)
+    for (int j = 0; j < s->ncomp; ++j)
+      if (dirmask[k * s->ncomp + j]) {
+        rows.push_back((int)(dirnode[k] * s->ncomp + j));
+        vals.push_back(dirval[k * s->ncomp + j]);
+      }
+  if (!rows.empty()) {
+    if ((rc = grow(c, s->bcbuf, rows.size() * 12 + 16))) return rc;
+    int* drow = (int*)s->bcbuf.p;
+    double* dval = (double*)((char*)s->bcbuf.p + ((rows.size() * 4 + 7) / 8) * 8);
+    XG_HIP(c, hipStreamSynchronize(c->stream));
+    XG_HIP(c, hipMemcpy(drow, rows.data(), rows.size() * 4, hipMemcpyHostToDevice));
+    XG_HIP(c, hipMemcpy(dval, vals.data(), vals.size() * 8, hipMemcpyHostToDevice));
+    if (nsub)
+      hipLaunchKernelGGL(k_rows_sub, dim3(nblocks((int64_t)nsub)), dim3(kBlock), 0,
+                         c->stream, (int)nsub, drow, dval, s->b);
+    const int nset = (int)(rows.size() - nsub);
+    if (nset)
+      hipLaunchKernelGGL(k_rows_set, dim3(nblocks((int64_t)nset)), dim3(kBlock), 0,
+                         c->stream, nset, drow + nsub, dval + nsub, s->b);
+    XG_HIP(c, hipGetLastError());
+  }
   s->dev_bc = true;
   s->restore = false;
   return XG_OK;

@@ -49,6 +49,20 @@ struct ChoMesh {
   int *fx_node = nullptr, *fx_off = nullptr, *fx_slot = nullptr;
   // staging of the host-pointer calls (grow only)
   std::vector<DevArr> stage;
+  // ChoCG stepping (xg_chocg_setup)
+  bool have_step = false;
+  std::vector<DevArr> step_allocs;
+  xg_chostep cfg{};
+  double* vol = nullptr;
+  int nsymnode = 0, nnoslip = 0, ndir = 0;
+  int *symnode = nullptr, *symoff = nullptr, *noslip = nullptr, *dirnode = nullptr;
+  double *symnorm = nullptr, *dirval = nullptr;
+  uint8_t* dirmask = nullptr;
+  std::vector<size_t> pdir;                 // pressure Dirichlet nodes (host)
+  std::vector<int32_t> pmask;
+  std::vector<double> pval0, pzero;         // initial values / increments (0)
+  double *pgrad = nullptr, *vgrad = nullptr, *sgrad = nullptr, *R = nullptr;
+  double *div = nullptr, *F = nullptr, *red = nullptr;
 };
 
 namespace {
@@ -406,6 +420,7 @@ __global__ void k_scatter_cols(int npoin, int nf, const double* __restrict__ src
 
 void free_cho(ChoMesh* m) {
   if (!m) return;
+  dev_free_all(m->step_allocs);
   dev_free_all(m->allocs);
   dev_free_all(m->stage);
   delete m;
@@ -474,6 +489,156 @@ int dev_in(xg_ctx* c, int dev, int k, const double* src, size_t n, const double*
   if (rc) return rc;
   XG_HIP(c, hipMemcpyAsync(d, src, n * 8, hipMemcpyHostToDevice, c->stream));
   *out = d;
+  return XG_OK;
+}
+
+// ------------------------------------------------------------ stepping --
+
+__global__ void k_fill0(int64_t n, double* __restrict__ p) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// ChoCG::fingrad (ChoCG.cpp:841-866): divide by the nodal volume
+__global__ void k_divvol_n(int npoin, int nc, double* __restrict__ G,
+                           const double* __restrict__ vol) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)npoin * nc) return;
+  G[i] /= vol[i / nc];
+}
+
+// explicit stage of ChoCG::solve (ChoCG.cpp:1555-1560): U = Un - dt R / vol
+__global__ void k_cho_update(int npoin, double dt, double* __restrict__ U,
+                             const double* __restrict__ Un, const double* __restrict__ R,
+                             const double* __restrict__ vol) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)npoin * 3) return;
+  U[i] = Un[i] - dt * R[i] / vol[i / 3];
+}
+
+// physics::dirbc on the 3 velocity components (BC.cpp:30-69)
+__global__ void k_cho_dirbc(int nd, const int* __restrict__ node,
+                            const uint8_t* __restrict__ mask,
+                            const double* __restrict__ val, double* __restrict__ U) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nd) return;
+  double* u = U + (size_t)node[i] * 3;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    if (mask[i * 3 + c]) u[c] = val[i * 3 + c];
+}
+
+// physics::noslipbc (BC.cpp:139-150)
+__global__ void k_noslip(int n, const int* __restrict__ node, double* __restrict__ U) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double* u = U + (size_t)node[i] * 3;
+  u[0] = u[1] = u[2] = 0.0;
+}
+
+// ChoCG::pinit's div /= dt (ChoCG.cpp:1044)
+__global__ void k_divide(int n, double* __restrict__ x, double dt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] /= dt;
+}
+
+// ChoCG::psolved's projection (ChoCG.cpp:1208-1213): U -= dt grad(dp)
+__global__ void k_project(int npoin, double dt, double* __restrict__ U,
+                          const double* __restrict__ sg) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (int64_t)npoin * 3) U[i] -= dt * sg[i];
+}
+
+__global__ void k_add_to(int n, double* __restrict__ P, const double* __restrict__ dP) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) P[i] += dP[i];
+}
+
+// per-node limits of ChoCG::dt (ChoCG.cpp:1384-1394), block minimum
+__global__ __launch_bounds__(kBlock) void k_chodt_partial(
+    int npoin, const double* __restrict__ U, const double* __restrict__ vol,
+    double dif, double* __restrict__ part) {
+  __shared__ double sm[kBlock / 64];
+  constexpr double eps = 2.220446049250313e-16;
+  double m = 1.7976931348623157e308;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npoin;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const double* s = U + p * 3;
+    const double vel = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    const double L = cbrt(vol[p]);
+    m = fmin(m, L / fmax(vel, 1.0e-8));
+    if (dif > eps) m = fmin(m, L * L / dif);
+  }
+  m = wave_min(m);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = sm[0];
+    for (int k = 1; k < kBlock / 64; ++k) b = fmin(b, sm[k]);
+    part[blockIdx.x] = b;
+  }
+}
+
+int check_step(xg_ctx* c) {
+  int rc = check_cho(c);
+  if (rc) return rc;
+  if (!c->cho->have_step) return fail(c, XG_ENOMESH, "xg_chocg_setup not called");
+  return XG_OK;
+}
+
+void zero(xg_ctx* c, double* p, int64_t n) {
+  hipLaunchKernelGGL(k_fill0, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, p);
+}
+
+// ChoCG::BC (ChoCG.cpp:1341-1355): dirbc, symbc (pos 0), noslip
+int cho_bc(xg_ctx* c, double* U) {
+  ChoMesh* m = c->cho;
+  if (m->ndir)
+    hipLaunchKernelGGL(k_cho_dirbc, dim3(nblocks(m->ndir)), dim3(kBlock), 0, c->stream,
+                       m->ndir, m->dirnode, m->dirmask, m->dirval, U);
+  if (m->nsymnode)
+    hipLaunchKernelGGL(k_symbc, dim3(nblocks(m->nsymnode)), dim3(kBlock), 0, c->stream,
+                       m->nsymnode, m->symnode, m->symoff, m->symnorm, U, 3, 0);
+  if (m->nnoslip)
+    hipLaunchKernelGGL(k_noslip, dim3(nblocks(m->nnoslip)), dim3(kBlock), 0, c->stream,
+                       m->nnoslip, m->noslip, U);
+  XG_HIP(c, hipGetLastError());
+  return XG_OK;
+}
+
+// chorin::grad of a scalar, finalised (zeroed output, divided by vol)
+int cho_grad(xg_ctx* c, const double* s, double* G) {
+  ChoMesh* m = c->cho;
+  zero(c, G, (int64_t)m->npoin * 3);
+  int rc = xg_chorin_grad(c, 1, s, G, 3);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_divvol_n, dim3(nblocks((int64_t)m->npoin * 3)), dim3(kBlock), 0,
+                     c->stream, m->npoin, 3, G, m->vol);
+  return XG_OK;
+}
+
+int cho_vgrad(xg_ctx* c, const double* U) {
+  ChoMesh* m = c->cho;
+  zero(c, m->vgrad, (int64_t)m->npoin * 9);
+  int rc = xg_chorin_vgrad(c, 1, 3, U, m->vgrad);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_divvol_n, dim3(nblocks((int64_t)m->npoin * 9)), dim3(kBlock), 0,
+                     c->stream, m->npoin, 9, m->vgrad, m->vol);
+  return XG_OK;
+}
+
+// ChoCG::pinit + psolve (ChoCG.cpp:1026-1143) on the device rhs m->div
+int cho_psolve(xg_ctx* c, const std::vector<double>& pvals, int64_t* it) {
+  ChoMesh* m = c->cho;
+  const int64_t nd = (int64_t)m->pdir.size();
+  int rc = xg_cg_init_dev(c, m->div, nd, nd ? m->pdir.data() : nullptr,
+                          nd ? m->pmask.data() : nullptr, nd ? pvals.data() : nullptr, 1,
+                          m->cfg.p_pc);
+  if (rc) return rc;
+  double normb = 0.0, normr = 0.0;
+  int conv = 0;
+  if ((rc = xg_cg_setup(c, &normb))) return rc;
+  if ((rc = xg_cg_solve(c, m->cfg.p_maxit, m->cfg.p_tol, &normr, it, &conv))) return rc;
   return XG_OK;
 }
 
@@ -721,6 +886,201 @@ int xg_chorin_rhs(xg_ctx* c, int dev, const xg_chocfg* cfg, int ncomp,
     XG_HIP(c, hipMemcpyAsync(R, out, n * 3 * 8, hipMemcpyDeviceToHost, c->stream));
     XG_HIP(c, hipStreamSynchronize(c->stream));
   }
+  return XG_OK;
+}
+
+int xg_chocg_setup(xg_ctx* c, const double* vol, const xg_chostep* cfg,
+                   int64_t nsym, const size_t* symnodes, const double* symnorms,
+                   int64_t nnoslip, const size_t* noslipnodes, int64_t ndir,
+                   const size_t* dirnodes, const int* dirmask, const double* dirvals,
+                   int64_t npdir, const size_t* pdirnodes, const double* pdirvals) {
+  int rc = check_cho(c);
+  if (rc) return rc;
+  ChoMesh* m = c->cho;
+  if (!vol || !cfg || nsym < 0 || (nsym && (!symnodes || !symnorms)) || nnoslip < 0 ||
+      (nnoslip && !noslipnodes) || ndir < 0 || (ndir && (!dirnodes || !dirmask || !dirvals)) ||
+      npdir < 0 || (npdir && (!pdirnodes || !pdirvals)))
+    return fail(c, XG_EINVAL, "xg_chocg_setup: bad arguments");
+  if (cfg->rk < 1 || cfg->rk > 4) return fail(c, XG_EINVAL, "xg_chocg_setup: rk must be 1..4");
+  if (cfg->flux.flux != 0 && cfg->flux.flux != 1)
+    return fail(c, XG_EINVAL, "Flux not correctly configured");
+  if (cfg->p_pc != 0 && cfg->p_pc != 1) return fail(c, XG_EINVAL, "p_pc must be 0 or 1");
+  if (cfg->p_maxit < 1 || !(cfg->p_tol >= 0.0) || !(cfg->cfl > 0.0))
+    return fail(c, XG_EINVAL, "xg_chocg_setup: bad solver settings");
+  if (!c->cg || xg_cg_rows(c) != m->npoin)
+    return fail(c, XG_ENOMESH, "xg_chocg_setup: the pressure matrix (xg_cg_matrix, "
+                               "one component per node) is not set");
+  const int64_t n = m->npoin;
+  for (int64_t i = 0; i < n; ++i)
+    if (!(vol[i] > 0)) return fail(c, XG_EINVAL, "xg_chocg_setup: non-positive nodal volume");
+  auto bad = [n](const size_t* v, int64_t k) {
+    for (int64_t i = 0; i < k; ++i)
+      if (v[i] >= (size_t)n) return true;
+    return false;
+  };
+  if (bad(symnodes, nsym) || bad(noslipnodes, nnoslip) || bad(dirnodes, ndir) ||
+      bad(pdirnodes, npdir))
+    return fail(c, XG_EINVAL, "xg_chocg_setup: node id out of range");
+  // symmetry (node, normal) pairs grouped by node, list order kept per node
+  std::vector<int> node, off(1, 0);
+  std::vector<double> nrm;
+  {
+    std::vector<int64_t> order(nsym);
+    for (int64_t i = 0; i < nsym; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+      return symnodes[a] < symnodes[b];
+    });
+    for (int64_t q = 0; q < nsym; ++q) {
+      int64_t i = order[q];
+      if (q == 0 || symnodes[order[q - 1]] != symnodes[i]) {
+        if (q) off.push_back((int)(nrm.size() / 3));
+        node.push_back((int)symnodes[i]);
+      }
+      nrm.insert(nrm.end(), symnorms + i * 3, symnorms + i * 3 + 3);
+    }
+    if (nsym) off.push_back((int)(nrm.size() / 3));
+  }
+  std::vector<int> ns(noslipnodes, noslipnodes + nnoslip), dn(ndir);
+  std::vector<uint8_t> dm((size_t)ndir * 3);
+  for (int64_t i = 0; i < ndir; ++i) {
+    dn[i] = (int)dirnodes[i];
+    for (int q = 0; q < 3; ++q) dm[i * 3 + q] = dirmask[i * 3 + q] ? 1 : 0;
+  }
+  XG_HIP(c, hipSetDevice(c->device));
+  (void)hipStreamSynchronize(c->stream);
+  dev_free_all(m->step_allocs);
+  m->have_step = false;
+  auto& L = m->step_allocs;
+  const size_t nn = (size_t)n;
+  if ((rc = upload(c, L, &m->vol, vol, nn)) || (rc = upload(c, L, &m->symnode, node)) ||
+      (rc = upload(c, L, &m->symoff, off)) || (rc = upload(c, L, &m->symnorm, nrm)) ||
+      (rc = upload(c, L, &m->noslip, ns)) || (rc = upload(c, L, &m->dirnode, dn)) ||
+      (rc = upload(c, L, &m->dirmask, dm)) ||
+      (rc = upload(c, L, &m->dirval, dirvals, (size_t)ndir * 3)) ||
+      (rc = dev_alloc(c, L, (void**)&m->pgrad, nn * 3 * 8)) ||
+      (rc = dev_alloc(c, L, (void**)&m->vgrad, nn * 9 * 8)) ||
+      (rc = dev_alloc(c, L, (void**)&m->sgrad, nn * 3 * 8)) ||
+      (rc = dev_alloc(c, L, (void**)&m->R, nn * 3 * 8)) ||
+      (rc = dev_alloc(c, L, (void**)&m->F, nn * 3 * 8)) ||
+      (rc = dev_alloc(c, L, (void**)&m->div, nn * 8)) ||
+      (rc = dev_alloc(c, L, (void**)&m->red, sizeof(double) * (kRedBlocks + 1)))) {
+    dev_free_all(m->step_allocs);
+    return rc;
+  }
+  zero(c, m->pgrad, (int64_t)nn * 3);
+  zero(c, m->vgrad, (int64_t)nn * 9);
+  m->nsymnode = (int)node.size();
+  m->nnoslip = (int)nnoslip;
+  m->ndir = (int)ndir;
+  m->pdir.assign(pdirnodes, pdirnodes + npdir);
+  m->pmask.assign(npdir, 1);
+  m->pval0.assign(pdirvals, pdirvals + npdir);
+  m->pzero.assign(npdir, 0.0);
+  m->cfg = *cfg;
+  XG_HIP(c, hipGetLastError());
+  m->have_step = true;
+  return XG_OK;
+}
+
+int xg_chocg_initial_dev(xg_ctx* c, double* U, double* P, int64_t* pit) {
+  int rc = check_step(c);
+  if (rc) return rc;
+  if (!U || !P) return fail(c, XG_EINVAL, "xg_chocg_initial: null field");
+  XG_HIP(c, hipSetDevice(c->device));
+  ChoMesh* m = c->cho;
+  const int n = m->npoin;
+  int64_t it0 = 0, it1 = 0;
+  double* x = nullptr;
+  if ((rc = xg_cg_device_x(c, &x))) return rc;
+  // m_np 0: project the initial velocity (dt = 1)
+  if ((rc = cho_bc(c, U))) return rc;
+  zero(c, m->div, n);
+  if ((rc = xg_chorin_div(c, 1, 0.0, P, m->pgrad, 3, U, 3, m->div, 0))) return rc;
+  if ((rc = cho_psolve(c, m->pval0, &it0))) return rc;
+  if ((rc = cho_grad(c, x, m->sgrad))) return rc;
+  hipLaunchKernelGGL(k_project, dim3(nblocks((int64_t)n * 3)), dim3(kBlock), 0, c->stream,
+                     n, 1.0, U, m->sgrad);
+  if ((rc = cho_bc(c, U))) return rc;
+  // m_np 1: velocity gradients, momentum flux, its divergence -> pressure
+  if ((rc = cho_vgrad(c, U))) return rc;
+  zero(c, m->F, (int64_t)n * 3);
+  if ((rc = xg_chorin_flux(c, 1, m->cfg.flux.mu, U, 3, m->vgrad, 9, m->F))) return rc;
+  hipLaunchKernelGGL(k_divvol_n, dim3(nblocks((int64_t)n * 3)), dim3(kBlock), 0, c->stream,
+                     n, 3, m->F, m->vol);
+  if (m->nsymnode)
+    hipLaunchKernelGGL(k_symbc, dim3(nblocks(m->nsymnode)), dim3(kBlock), 0, c->stream,
+                       m->nsymnode, m->symnode, m->symoff, m->symnorm, m->F, 3, 0);
+  zero(c, m->div, n);
+  if ((rc = xg_chorin_div(c, 1, 0.0, P, m->pgrad, 3, m->F, 3, m->div, 0))) return rc;
+  if ((rc = cho_psolve(c, m->pval0, &it1))) return rc;
+  XG_HIP(c, hipMemcpyAsync(P, x, (size_t)n * 8, hipMemcpyDeviceToDevice, c->stream));
+  if ((rc = cho_grad(c, P, m->pgrad))) return rc;
+  XG_HIP(c, hipGetLastError());
+  if (pit) { pit[0] = it0; pit[1] = it1; }
+  return XG_OK;
+}
+
+int xg_chocg_dt_dev(xg_ctx* c, const double* U, double* dt) {
+  int rc = check_step(c);
+  if (rc) return rc;
+  if (!U || !dt) return fail(c, XG_EINVAL, "xg_chocg_dt: null argument");
+  XG_HIP(c, hipSetDevice(c->device));
+  ChoMesh* m = c->cho;
+  int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, nblocks(m->npoin)));
+  hipLaunchKernelGGL(k_chodt_partial, dim3(nb), dim3(kBlock), 0, c->stream, m->npoin, U,
+                     m->vol, m->cfg.flux.mu, m->red);
+  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(kRedBlocks), 0, c->stream, nb, m->red,
+                     m->cfg.cfl, m->red + kRedBlocks);
+  XG_HIP(c, hipGetLastError());
+  XG_HIP(c, hipMemcpyAsync(dt, m->red + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  XG_HIP(c, hipStreamSynchronize(c->stream));
+  if (*dt < 2.220446049250313e-16)
+    return fail(c, XG_EBLOWUP, "time step below machine epsilon");
+  return XG_OK;
+}
+
+int xg_chocg_step_dev(xg_ctx* c, double dt, double* U, double* Un, double* P, double* dP,
+                      int64_t* pit) {
+  static const double rkcoef[4][4] = {{1.0}, {1.0 / 2.0, 1.0},
+                                      {1.0 / 3.0, 1.0 / 2.0, 1.0},
+                                      {1.0 / 4.0, 1.0 / 3.0, 1.0 / 2.0, 1.0}};
+  int rc = check_step(c);
+  if (rc) return rc;
+  if (!U || !Un || !P || !dP || U == Un) return fail(c, XG_EINVAL, "xg_chocg_step: bad fields");
+  if (!(dt > 0.0)) return fail(c, XG_EINVAL, "xg_chocg_step: dt must be positive");
+  XG_HIP(c, hipSetDevice(c->device));
+  ChoMesh* m = c->cho;
+  const int n = m->npoin;
+  const int64_t n3 = (int64_t)n * 3;
+  const int nst = m->cfg.rk;
+  for (int st = 0; st < nst; ++st) {
+    // ChoCG::rhs, solve (explicit), pred (BC, velocity gradients for damp4)
+    if ((rc = xg_chorin_rhs(c, 1, &m->cfg.flux, 3, P, U, m->vgrad, m->R))) return rc;
+    if (st == 0)
+      XG_HIP(c, hipMemcpyAsync(Un, U, (size_t)n3 * 8, hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_cho_update, dim3(nblocks(n3)), dim3(kBlock), 0, c->stream, n,
+                       rkcoef[nst - 1][st] * dt, U, Un, m->R, m->vol);
+    if ((rc = cho_bc(c, U))) return rc;
+    if (m->cfg.flux.flux == 1 && (rc = cho_vgrad(c, U))) return rc;
+  }
+  // ChoCG::div, pinit, psolve, sgrad, psolved, pgrad
+  zero(c, m->div, n);
+  if ((rc = xg_chorin_div(c, 1, dt, P, m->pgrad, 3, U, 3, m->div, 1))) return rc;
+  hipLaunchKernelGGL(k_divide, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, m->div, dt);
+  int64_t it = 0;
+  if ((rc = cho_psolve(c, m->pzero, &it))) return rc;
+  double* x = nullptr;
+  if ((rc = xg_cg_device_x(c, &x))) return rc;
+  XG_HIP(c, hipMemcpyAsync(dP, x, (size_t)n * 8, hipMemcpyDeviceToDevice, c->stream));
+  if ((rc = cho_grad(c, dP, m->sgrad))) return rc;
+  hipLaunchKernelGGL(k_project, dim3(nblocks(n3)), dim3(kBlock), 0, c->stream, n, dt, U,
+                     m->sgrad);
+  if ((rc = cho_bc(c, U))) return rc;
+  hipLaunchKernelGGL(k_add_to, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, P, dP);
+  if ((rc = cho_grad(c, P, m->pgrad))) return rc;
+  XG_HIP(c, hipGetLastError());
+  if (pit) *pit = it;
   return XG_OK;
 }
 

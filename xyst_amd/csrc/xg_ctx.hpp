@@ -71,6 +71,8 @@ struct xg_ctx {
 
 void xg_kozak_free(xg_ctx* c);   // xg_kozak.hip
 void xg_chocg_free(xg_ctx* c);   // xg_chocg.hip
+// rows of the context's CG matrix when it has one component per node, else -1
+int64_t xg_cg_rows(const xg_ctx* c);   // xg_cg.hip
 
 namespace xgd {
 
