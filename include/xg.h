@@ -297,6 +297,23 @@ int xg_chocg_dt_dev(xg_ctx* ctx, const double* U, double* dt);
 int xg_chocg_step_dev(xg_ctx* ctx, double dt, double* U, double* Un, double* P,
                       double* dP, int64_t* pit);
 
+/* ---- node diagnostics -------------------------------------------------------- */
+
+/* NodeDiagnostics::rhocompute (NodeDiagnostics.cpp:46-146) without the
+ * analytic-solution error norms: out[c] = sum_i U(i,c)^2 V_i, out[ncomp+c] =
+ * sum_i (U(i,c) - Un(i,c))^2 V_i, out[2 ncomp] = sum_i U(i,4) V_i (0 for
+ * ncomp < 5).  U, Un (npoin*ncomp), V (npoin) on the device; out (host)
+ * 2 ncomp + 1 values.  ncomp 1, 3 or 5.  Deterministic summation order. */
+int xg_diag_rho_dev(xg_ctx* ctx, int64_t npoin, int ncomp, const double* U,
+                    const double* Un, const double* V, double* out);
+
+/* NodeDiagnostics::precompute (NodeDiagnostics.cpp:148-264) without the
+ * analytic-solution error norms: out = {sum p^2 V, sum u_c^2 V (3),
+ * sum dp^2 V, sum (u_c - un_c)^2 V (3)}.  U, Un (npoin*3), P, dP, V (npoin)
+ * on the device; out (host) 8 values. */
+int xg_diag_pre_dev(xg_ctx* ctx, int64_t npoin, const double* U, const double* Un,
+                    const double* P, const double* dP, const double* V, double* out);
+
 /* ---- pressure-Poisson system (ChoCG) and conjugate gradients ---------------- */
 
 /* tk::CSR(ncomp, psup) (CSR.hpp:29-31, CSR.cpp:19-84): the sparsity pattern

@@ -67,6 +67,8 @@ struct xg_ctx {
   ChoMesh* cho = nullptr;
   // learned preconditioner (xg_gnn.hip)
   GnnNet* gnn = nullptr;
+  // diagnostics partial sums (xg_diag.hip, grow only)
+  DevArr diagbuf;
 };
 
 void xg_kozak_free(xg_ctx* c);   // xg_kozak.hip

@@ -483,6 +483,7 @@ int xg_ctx_destroy(xg_ctx* c) {
   if (c->koz) xg_kozak_free(c);
   if (c->cho) xg_chocg_free(c);
   if (c->gnn) xg_gnn_free(c);
+  if (c->diagbuf.p) (void)hipFree(c->diagbuf.p);
   delete c;
   return XG_OK;
 }

@@ -1,0 +1,27 @@
+"""Python mirror of the node diagnostics (NodeDiagnostics.cpp:46-264) over
+the C-ABI: the volume-weighted sums of the solution, its change over the
+step and (density-based solvers) the total energy, computed on the device
+from device tensors."""
+import numpy as np
+
+from ._xg import lib, ptr
+
+__all__ = ["rhocompute", "precompute"]
+
+
+def rhocompute(ctx, U, Un, V):
+    """Sums of U_c^2 V, (U_c - Un_c)^2 V and U_4 V; U, Un (npoin, ncomp),
+    V (npoin,) device tensors, ncomp 1, 3 or 5."""
+    n, nc = int(U.shape[0]), int(U.shape[1])
+    out = np.empty(2 * nc + 1)
+    ctx._check(lib().xg_diag_rho_dev(ctx._h, n, nc, ptr(U), ptr(Un), ptr(V), ptr(out)))
+    return out
+
+
+def precompute(ctx, U, Un, P, dP, V):
+    """Sums of p^2 V, u_c^2 V, dp^2 V, (u_c - un_c)^2 V for the pressure-based
+    solver; U, Un (npoin, 3), P, dP, V (npoin,) device tensors."""
+    out = np.empty(8)
+    ctx._check(lib().xg_diag_pre_dev(ctx._h, int(U.shape[0]), ptr(U), ptr(Un), ptr(P),
+                                     ptr(dP), ptr(V), ptr(out)))
+    return out
