@@ -202,6 +202,67 @@ def kozcg_bench(ctx, m, vol, sn, sv, stream, steps):
     return out
 
 
+def chocg_bench(n, stream, steps):
+    """The ChoCG time step (explicit momentum + pressure projection) on the
+    Taylor-Green vortex in [0, pi]^3 with free-slip (symmetry) walls, which
+    its symmetries make equivalent to the periodic 2 pi box: u = sin x cos y
+    cos z, v = -cos x sin y cos z, w = 0; Re 1600 (mu = 1/1600), damp4 flux
+    with stab, rk 1, cfl 0.5, pressure Jacobi-PCG to 1e-6 (warm-started),
+    hydrostat node 0.  Times `steps` steps after the start-up projections."""
+    import torch
+    from xyst_amd import chorin as X
+    from xyst_amd import riemann as R
+    from xyst_amd.cg import CSR, ConjugateGradients
+    from xyst_amd.meshgen import box_mesh, renumber_mesh
+    t0 = time.time()
+    m = box_mesh(n, seed=20261202, length=(np.pi, np.pi, np.pi))
+    m = renumber_mesh(m, R.mesh_rcm(m["coord"].shape[1], m["inpoel"]))
+    m = renumber_mesh(m, R.mesh_block_order(m["coord"], 2048))
+    coord, inpoel = m["coord"], m["inpoel"]
+    N = coord.shape[1]
+    vol = R.mesh_vol(coord, inpoel)
+    edge, dint = X.mesh_domint(coord, inpoel)
+    sn, sv, _ = R.mesh_symbc(coord, m["triinpoel"], m["faceset"], [1, 2, 3, 4, 5, 6])
+    ctx = R.Context(0, stream=stream)
+    sup, ints = R.edges_as_superedges(edge, dint)
+    X.upload(ctx, coord, sup, ints, m["triinpoel"])
+    A = CSR.poisson(coord, inpoel)
+    cg = ConjugateGradients(A, ctx=ctx)
+    X.step_setup(ctx, vol, X.config("damp4", stab=True, mu=1.0 / 1600.0), cfl=0.5, rk=1,
+                 p_pc="jacobi", p_maxit=5000, p_tol=1e-6, symnodes=sn, symnorms=sv,
+                 pdirnodes=np.array([0], np.int64), pdirvals=np.zeros(1))
+    x, y, z = coord
+    U0 = np.stack([np.sin(x) * np.cos(y) * np.cos(z), -np.cos(x) * np.sin(y) * np.cos(z),
+                   np.zeros(N)], 1)
+    U = torch.tensor(U0, device="cuda")
+    P = torch.zeros(N, dtype=torch.float64, device="cuda")
+    Un, dP = torch.empty_like(U), torch.empty_like(P)
+    pit0 = X.initial_dev(ctx, U, P)
+    torch.cuda.synchronize()
+    log("chocg tgv %d^3 setup + start-up: %.1fs (pressure its %s)"
+        % (n, time.time() - t0, list(pit0)))
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    its, ms = [], []
+    for _ in range(steps):
+        dt = X.dt_dev(ctx, U)
+        ev0.record(stream)
+        its.append(X.step_dev(ctx, dt, U, Un, P, dP))
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(ev0.elapsed_time(ev1))
+    nnz = int(A.ja.size)
+    out = {"n": n, "nodes": N, "edges": int(edge.shape[0]), "nnz": nnz,
+           "ms_per_step": float(np.mean(ms)), "ms_steps": ms, "pressure_its": its,
+           "startup_pressure_its": list(pit0), "dt": dt,
+           "config": "TGV in [0,pi]^3, free-slip walls, Re 1600, damp4+stab, rk1, "
+                     "cfl 0.5, Jacobi-PCG 1e-6"}
+    log("chocg: %.2f ms/step, pressure its %s" % (out["ms_per_step"], its))
+    ctx.close()
+    del U, P, Un, dP
+    return out
+
+
 def read_traffic(path):
     """HBM bytes per rhs launch measured by rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE per dispatch, each scaled by what the counter reads for a copy
@@ -251,6 +312,8 @@ def main():
                     help="box size of the pressure-Poisson solve (0: skip)")
     ap.add_argument("--kozcg-steps", type=int, default=20,
                     help="timed kozak::rhs / KozCG steps on the same mesh (0: skip)")
+    ap.add_argument("--chocg-n", type=int, default=128,
+                    help="box size of the ChoCG Taylor-Green steps (0: skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed region, copy a 512 MiB buffer 3 times "
                          "(the known-size kernel that calibrates HBM byte counters)")
@@ -348,6 +411,9 @@ def main():
         del dG, dR, dU, dUn
         torch.cuda.empty_cache()
         out["poisson"] = poisson_bench(args.poisson_n, stream)
+    if args.chocg_n > 0:
+        torch.cuda.empty_cache()
+        out["chocg"] = chocg_bench(args.chocg_n, stream, 5)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline()

@@ -68,6 +68,13 @@ int xg_mesh_domint(int64_t npoin, const double* x, const double* y,
                    const double* z, int64_t ntet, const size_t* inpoel,
                    int64_t cap, int64_t* nedge, size_t* edge, double* dint);
 
+/* ChoCG::domint (ChoCG.cpp:399-445): as xg_mesh_domint with 5 values per
+ * edge (dint[5*cap]): the dual-face integral, J/120 and the P1 stiffness
+ * grad_p.grad_q/J/6, each accumulated in tet order. */
+int xg_mesh_chocg_domint(int64_t npoin, const double* x, const double* y,
+                         const double* z, int64_t ntet, const size_t* inpoel,
+                         int64_t cap, int64_t* nedge, size_t* edge, double* dint);
+
 /* Locality renumbering of the nodes, the job of the RieCG/ChoCG constructor
  * renumbering (RieCG.cpp:82-100) and tk::renumber (Reorder.cpp:215-260):
  * reverse Cuthill-McKee over the edge graph.  perm[old] = new. */

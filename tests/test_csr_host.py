@@ -61,3 +61,16 @@ def test_pattern_rejects_bad_node_ids():
     inp = np.array([[0, 1, 2, 9]], np.uint64)
     with pytest.raises(XgError):
         X.csr_pattern(4, inp)
+
+
+def test_chocg_domint_bit_exact_vs_oracle():
+    """xg_mesh_chocg_domint (host C++) equals the oracle's ChoCG::domint,
+    edge list and all five integrals, bit for bit (same tet-order sums)."""
+    import oracle as O
+    from xyst_amd import chorin as X
+    from xyst_amd.meshgen import box_mesh
+    m = box_mesh(7, seed=11)
+    e1, d1 = X.mesh_domint(m["coord"], m["inpoel"])
+    e0, d0 = O.chocg_domint(m["coord"], m["inpoel"])
+    assert np.array_equal(e1.astype(np.int64), e0)
+    assert np.array_equal(d1, d0)

@@ -26,6 +26,7 @@ SIG = {
     "xg_set_config": (C.c_int, [VP, VP]),
     "xg_mesh_vol": (C.c_int, [I64, VP, VP, VP, I64, VP, VP]),
     "xg_mesh_domint": (C.c_int, [I64, VP, VP, VP, I64, VP, I64, C.POINTER(I64), VP, VP]),
+    "xg_mesh_chocg_domint": (C.c_int, [I64, VP, VP, VP, I64, VP, I64, C.POINTER(I64), VP, VP]),
     "xg_mesh_rcm": (C.c_int, [I64, I64, VP, VP]),
     "xg_mesh_block_order": (C.c_int, [I64, VP, VP, VP, I64, VP]),
     "xg_mesh_symbc": (C.c_int, [I64, VP, VP, VP, I64, VP, VP, I32, VP, I64,

@@ -14,7 +14,7 @@ import numpy as np
 
 from ._xg import ChoConfig, ChoStep, lib, ptr
 
-__all__ = ["upload", "div", "grad", "vgrad", "flux", "rhs", "config", "step_setup",
+__all__ = ["mesh_domint", "upload", "div", "grad", "vgrad", "flux", "rhs", "config", "step_setup",
            "initial_dev", "dt_dev", "step_dev"]
 
 FLUX = {"damp2": 0, "damp4": 1}
@@ -39,6 +39,26 @@ def config(flux="damp2", stab=True, stab2=False, stab2coef=0.0, mu=0.0):
     if flux not in FLUX:
         raise ValueError("Flux not correctly configured")
     return ChoConfig(FLUX[flux], int(bool(stab)), int(bool(stab2)), stab2coef, mu)
+
+
+def mesh_domint(coord, inpoel):
+    """ChoCG::domint: unique edges (E,2) with p<q and 5 integrals per edge
+    (dual-face normal, J/120, P1 stiffness)."""
+    x, y, z = (_f64(coord[i]) for i in range(3))
+    inp = _idx(inpoel)
+    ne = C.c_int64(0)
+    L = lib()
+    rc = L.xg_mesh_chocg_domint(x.size, ptr(x), ptr(y), ptr(z), inp.size // 4, ptr(inp), 0,
+                                C.byref(ne), None, None)
+    if rc:
+        raise ValueError("xg_mesh_chocg_domint failed (%d)" % rc)
+    edge = np.empty((ne.value, 2), np.uint64)
+    dint = np.empty((ne.value, 5))
+    rc = L.xg_mesh_chocg_domint(x.size, ptr(x), ptr(y), ptr(z), inp.size // 4, ptr(inp),
+                                ne.value, C.byref(ne), ptr(edge), ptr(dint))
+    if rc:
+        raise ValueError("xg_mesh_chocg_domint failed (%d)" % rc)
+    return edge, dint
 
 
 def upload(ctx, coord, dsupedge, dsupint, triinpoel):

@@ -58,14 +58,16 @@ int mesh_vol(int64_t npoin, const double* x, const double* y, const double* z,
   return XG_OK;
 }
 
-// Edge integrals of RieCG::domint.  Records (lower node, upper node, tet-edge
-// id) are bucketed by the lower node with a counting sort, each bucket is
-// sorted by (upper node, tet-edge id) so that contributions are summed in
-// tet order -- the order the reference's hash map sees them.
+// Edge integrals of RieCG::domint (stride 3) or ChoCG::domint (stride 5:
+// also J/120 and grad_p.grad_q/J/6, ChoCG.cpp:399-445).  Records (lower
+// node, upper node, tet-edge id) are bucketed by the lower node with a
+// counting sort, each bucket is sorted by (upper node, tet-edge id) so that
+// contributions are summed in tet order -- the order the reference's hash
+// map sees them.
 int mesh_domint(int64_t npoin, const double* x, const double* y,
                 const double* z, int64_t ntet, const size_t* inpoel,
                 std::vector<size_t>& edge, std::vector<double>& dint,
-                bool count_only, std::string& err) {
+                bool count_only, std::string& err, int stride) {
   if (ntet * 6 >= (1ll << 32)) { err = "xg_mesh_domint: too many tets"; return XG_EINVAL; }
   const int64_t nrec = ntet * 6;
   std::vector<uint32_t> cnt((size_t)npoin + 1, 0);
@@ -95,7 +97,7 @@ int mesh_domint(int64_t npoin, const double* x, const double* y,
     }
   }
   edge.clear(); dint.clear();
-  edge.reserve((size_t)nrec / 3); dint.reserve((size_t)nrec / 2);
+  edge.reserve((size_t)nrec / 3); dint.reserve((size_t)nrec / 6 * stride);
   for (int64_t p = 0; p < npoin; ++p) {
     uint64_t* row = key.data() + off[p];
     size_t nr = (size_t)(off[p+1] - off[p]);
@@ -104,7 +106,7 @@ int mesh_domint(int64_t npoin, const double* x, const double* y,
       uint64_t q = row[i] >> 32;
       if (i == 0 || (row[i-1] >> 32) != q) {
         edge.push_back((size_t)p); edge.push_back((size_t)q);
-        dint.push_back(0.0); dint.push_back(0.0); dint.push_back(0.0);
+        for (int j = 0; j < stride; ++j) dint.push_back(0.0);
       }
       if (count_only) continue;
       uint32_t r = (uint32_t)(row[i] & 0xffffffffu);
@@ -114,8 +116,14 @@ int mesh_domint(int64_t npoin, const double* x, const double* y,
       tet_grads(x, y, z, N, g);
       int a = kLpoed[k][0], b = kLpoed[k][1];
       double sig = N[a] > N[b] ? -1.0 : 1.0;
-      double* d = dint.data() + dint.size() - 3;
+      double* d = dint.data() + dint.size() - stride;
       for (int j = 0; j < 3; ++j) d[j] += sig * (g[a][j] - g[b][j]) / 48.0;
+      if (stride == 5) {
+        const double ba[3] = {x[N[1]]-x[N[0]], y[N[1]]-y[N[0]], z[N[1]]-z[N[0]]};
+        const double J = ba[0]*g[1][0] + ba[1]*g[1][1] + ba[2]*g[1][2];
+        d[3] += J / 120.0;
+        d[4] += (g[a][0]*g[b][0] + g[a][1]*g[b][1] + g[a][2]*g[b][2]) / J / 6.0;
+      }
     }
   }
   return XG_OK;
@@ -329,6 +337,26 @@ extern "C" int xg_mesh_domint(int64_t npoin, const double* x, const double* y,
   std::vector<double> d;
   std::string err;
   int rc = xg::mesh_domint(npoin, x, y, z, ntet, inpoel, e, d, cap == 0, err);
+  if (rc) return rc;
+  *nedge = (int64_t)(e.size() / 2);
+  if (cap == 0) return XG_OK;
+  if (cap < *nedge) return XG_EINVAL;
+  std::memcpy(edge, e.data(), e.size() * sizeof(size_t));
+  std::memcpy(dint, d.data(), d.size() * sizeof(double));
+  return XG_OK;
+}
+
+extern "C" int xg_mesh_chocg_domint(int64_t npoin, const double* x, const double* y,
+                                    const double* z, int64_t ntet,
+                                    const size_t* inpoel, int64_t cap,
+                                    int64_t* nedge, size_t* edge, double* dint) {
+  if (npoin < 0 || npoin >= (1ll << 31) || ntet < 0 || !x || !y || !z ||
+      (ntet && !inpoel) || !nedge || cap < 0 || (cap && (!edge || !dint)))
+    return XG_EINVAL;
+  std::vector<size_t> e;
+  std::vector<double> d;
+  std::string err;
+  int rc = xg::mesh_domint(npoin, x, y, z, ntet, inpoel, e, d, cap == 0, err, 5);
   if (rc) return rc;
   *nedge = (int64_t)(e.size() / 2);
   if (cap == 0) return XG_OK;
