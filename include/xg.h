@@ -261,7 +261,9 @@ int xg_chorin_rhs(xg_ctx* ctx, int dev, const xg_chocfg* cfg, int ncomp,
 
 /* ChoCG's g_cfg reads of its time step (InciterConfig.cpp): chorin::rhs
  * settings, cfl, rk stages (1-4, rkcoef of ChoCG.cpp:43-48) and the pressure
- * solve's pc (0 none, 1 jacobi), iter and tol. */
+ * solve's pc (0 none, 1 jacobi; 2, without a reference counterpart: flexible
+ * CG with the multigrid preconditioner of xg_cg_solve_flexible pc_kind 3),
+ * iter and tol. */
 typedef struct {
   xg_chocfg flux;
   double    cfl;
@@ -392,16 +394,24 @@ int xg_cg_solve_mixed(xg_ctx* ctx, int64_t maxit, double tol,
 /* Flexible CG (no counterpart in the reference): Notay's FCG(1) in FP64 on
  * the system set up by xg_cg_init, with preconditioner pc_kind 0 (Jacobi),
  * 1 (`inner` iterations of FP32 Jacobi-PCG from a zero guess, i.e. FP32
- * inner solves inside an FP64 outer Krylov recurrence) or 2 (the learned
+ * inner solves inside an FP64 outer Krylov recurrence), 2 (the learned
  * preconditioner, xg_gnn_*, judged by the residual: Jacobi for an
  * iteration whose learned z is no descent direction, and for the rest of the
- * solve once ||r|| has not improved on its best value for 20 iterations).
+ * solve once ||r|| has not improved on its best value for 20 iterations) or
+ * 3 (one FP32 V-cycle of smoothed-aggregation algebraic multigrid on
+ * sign(diag A) A, built on the host from the matrix with its Dirichlet rows
+ * when first used or when the values change; one component per node).
  * Stops when ||r|| < tol*normb or after maxit outer iterations; *iters outer
- * iterations, *inner_total the inner iterations (pc 1) or the iterations
- * that used Jacobi (pc 2). */
+ * iterations, *inner_total the inner iterations (pc 1), the iterations
+ * that used Jacobi (pc 2) or the V-cycles (pc 3). */
 int xg_cg_solve_flexible(xg_ctx* ctx, int pc_kind, int64_t inner, int64_t maxit,
                          double tol, double* normr, int64_t* iters,
                          int64_t* inner_total, int* converged);
+
+/* The multigrid hierarchy of pc_kind 3: number of levels, operator
+ * complexity (sum of the levels' nonzeros over the fine level's) and the
+ * host setup time of its last build. */
+int xg_amg_info(xg_ctx* ctx, int* levels, double* op_complexity, double* setup_ms);
 
 /* ---- learned graph preconditioner (no counterpart in the reference) -------- */
 

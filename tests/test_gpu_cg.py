@@ -258,7 +258,7 @@ def test_cg_apply_same_dirichlet_set_new_values():
                                    atol=1e-7 * np.abs(xo).max())
 
 
-@pytest.mark.parametrize("pc,inner", [("jacobi", 1), ("cg32", 8), ("cg32", 30)])
+@pytest.mark.parametrize("pc,inner", [("jacobi", 1), ("cg32", 8), ("cg32", 30), ("amg", 1)])
 def test_flexible_cg_to_1e10(pc, inner):
     """FP64 flexible CG with Jacobi or FP32 inner-CG preconditioning reaches a
     true relative residual of 1e-10 on the pressure-Poisson system; with
@@ -296,5 +296,43 @@ def test_flexible_cg_to_1e10(pc, inner):
     assert true < 2e-10 * nb
     if pc == "jacobi":
         assert abs(cg.it - it_pcg) <= max(5, it_pcg // 50)
+    elif pc == "amg":
+        lv, oc, ms = cg.amg_info()
+        _say("  amg levels=%d complexity=%.3f setup=%.0f ms" % (lv, oc, ms))
+        assert cg.it <= 30 and lv >= 3 and oc < 2.0
     else:
         assert cg.it < it_pcg
+
+
+def test_amg_fcg_hydrostat_and_rebuild():
+    """The ChoCG situation: a single Dirichlet (hydrostat) node, the
+    multigrid-preconditioned FCG to 1e-10; a new Dirichlet set rebuilds the
+    hierarchy, the same set reuses it (same setup time reported)."""
+    X = _X()
+    _say("amg hydrostat")
+    m = box_mesh(24, seed=3)
+    coord, inpoel = m["coord"], m["inpoel"]
+    A = X.CSR.poisson(coord, inpoel)
+    n = coord.shape[1]
+    Ao = O.CSR(n, 1, A.ia.astype(np.int64), A.ja.astype(np.int64), A.a.copy())
+    b = Ao.mult(np.cos(3 * coord[0]) * coord[1]) + 0.01
+    cg = X.ConjugateGradients(A)
+    out = []
+    for dirbc in ({0: [(1, 0.0)]}, {0: [(1, 0.0)]},
+                  {int(i): [(1, 0.0)] for i in np.where(coord[2] < 1e-12)[0]}):
+        cg.init(np.zeros(n), b, dirbc=dirbc, apply=True, pc="jacobi")
+        cg.solve_flexible(200, 1e-10, pc="amg")
+        assert cg.converged
+        Ab = O.CSR(n, 1, Ao.ia, Ao.ja, Ao.a.copy())
+        rr = np.zeros(n)
+        for i in sorted(dirbc, reverse=True):
+            Ab.dirichlet(i, 0.0, rr)
+        bb = b - rr
+        for i in dirbc:
+            bb[i] = 0.0
+        x = cg.solution()
+        assert np.linalg.norm(bb - Ab.mult(x)) < 2e-10 * np.linalg.norm(bb)
+        out.append((cg.it, cg.amg_info()))
+        _say("  it=%d levels/complexity/setup=%s" % out[-1])
+    assert out[0][1] == out[1][1]          # same set: hierarchy reused
+    assert all(it <= 40 for it, _ in out)

@@ -158,11 +158,12 @@ class ConjugateGradients:
         return self.normr
 
     def solve_flexible(self, maxit, tol, pc="jacobi", inner=20):
-        """FP64 flexible CG with a Jacobi, an FP32 inner-CG ("cg32") or the
-        learned ("gnn", xyst_amd.gnn) preconditioner (no reference
-        counterpart); returns ||r||.  inner_total: inner iterations (cg32) or
-        Jacobi fallbacks (gnn)."""
-        kinds = {"jacobi": 0, "cg32": 1, "gnn": 2}
+        """FP64 flexible CG with a Jacobi, an FP32 inner-CG ("cg32"), the
+        learned ("gnn", xyst_amd.gnn) or a smoothed-aggregation multigrid
+        ("amg") preconditioner (no reference counterpart); returns ||r||.
+        inner_total: inner iterations (cg32), Jacobi fallbacks (gnn) or
+        V-cycles (amg)."""
+        kinds = {"jacobi": 0, "cg32": 1, "gnn": 2, "amg": 3}
         nr, it, tin, cv = C.c_double(), C.c_int64(), C.c_int64(), C.c_int()
         self.ctx._check(lib().xg_cg_solve_flexible(
             self.ctx._h, kinds[pc], inner, maxit, tol, C.byref(nr), C.byref(it),
@@ -170,6 +171,13 @@ class ConjugateGradients:
         self.normr, self.it, self.converged = nr.value, it.value, bool(cv.value)
         self.inner_total = tin.value
         return self.normr
+
+    def amg_info(self):
+        """(levels, operator complexity, host setup ms) of the AMG hierarchy
+        built by solve_flexible(pc="amg")."""
+        lv, oc, ms = C.c_int(), C.c_double(), C.c_double()
+        self.ctx._check(lib().xg_amg_info(self.ctx._h, C.byref(lv), C.byref(oc), C.byref(ms)))
+        return lv.value, oc.value, ms.value
 
     def solution(self):
         x = np.empty(self.n)

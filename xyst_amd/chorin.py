@@ -124,7 +124,7 @@ def step_setup(ctx, vol, cfg, cfl=0.9, rk=1, p_pc="jacobi", p_maxit=500, p_tol=1
     the context's ConjugateGradients): cfg from `config`; velocity Dirichlet
     nodes with (ndir, 3) mask and values; pressure Dirichlet nodes (bc_dir of
     the pressure block and the hydrostat node) with their initial values."""
-    st = ChoStep(cfg, float(cfl), int(rk), {"none": 0, "jacobi": 1}[p_pc], int(p_maxit),
+    st = ChoStep(cfg, float(cfl), int(rk), {"none": 0, "jacobi": 1, "amg": 2}[p_pc], int(p_maxit),
                  float(p_tol))
     v = _f64(vol)
     sn, sv = _idx(symnodes), _f64(symnorms).reshape(-1)

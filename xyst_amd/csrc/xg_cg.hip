@@ -28,6 +28,7 @@
 
 #include "xg_csr.hpp"
 #include "xg_ctx.hpp"
+#include "xg_amg.hpp"
 #include "xg_gnn.hpp"
 
 using namespace xgd;
@@ -56,6 +57,8 @@ struct CgSolver {
   std::vector<int64_t> ia;
   std::vector<int32_t> ja;
   std::vector<double> a0;
+  std::vector<double> abc;       // host copy of the values on the device
+  int64_t vals_version = 0;      // changes whenever the device values do
   bool dev_bc = false;           // device values carry Dirichlet rows
   bool restore = false;          // restore the original values before reuse
   std::vector<int64_t> bc_key;   // Dirichlet (node, mask...) set of the device values
@@ -519,6 +522,8 @@ int restore_values(xg_ctx* c) {
   std::vector<float> a32(s->a0.begin(), s->a0.end());
   XG_HIP(c, hipMemcpy(s->da, s->a0.data(), s->a0.size() * 8, hipMemcpyHostToDevice));
   XG_HIP(c, hipMemcpy(s->da32, a32.data(), a32.size() * 4, hipMemcpyHostToDevice));
+  s->abc = s->a0;
+  ++s->vals_version;
   s->dev_bc = false;
   s->restore = false;
   return XG_OK;
@@ -631,6 +636,8 @@ int cg_dirichlet(xg_ctx* c, CgSolver* s, int64_t ndir, const size_t* dirnode,
   XG_HIP(c, hipMemcpy(s->da, a.data(), a.size() * 8, hipMemcpyHostToDevice));
   XG_HIP(c, hipMemcpy(s->da32, a32.data(), a32.size() * 4, hipMemcpyHostToDevice));
   s->bc_key.swap(key);
+  s->abc.swap(a);
+  ++s->vals_version;
   return XG_OK;
 }
 
@@ -672,6 +679,7 @@ int xg_cg_matrix(xg_ctx* c, int ncomp, int64_t npoin, const size_t* ia,
     s->ja[j] = (int32_t)col;
   }
   s->a0.assign(a, a + nnz);
+  s->abc = s->a0;
   std::vector<int32_t> ia32(s->ia.begin(), s->ia.end());
   std::vector<float> a32(nnz);
   for (int64_t j = 0; j < nnz; ++j) a32[j] = (float)a[j];
@@ -960,7 +968,7 @@ int xg_cg_solve_flexible(xg_ctx* c, int pc_kind, int64_t inner, int64_t maxit,
   int rc = check_cg(c);
   if (rc) return rc;
   CgSolver* s = c->cg;
-  if (maxit < 1 || !(tol > 0.0) || pc_kind < 0 || pc_kind > 2 ||
+  if (maxit < 1 || !(tol > 0.0) || pc_kind < 0 || pc_kind > 3 ||
       (pc_kind == 1 && inner < 1))
     return fail(c, XG_EINVAL, "xg_cg_solve_flexible: bad arguments");
   XG_HIP(c, hipSetDevice(c->device));
@@ -978,6 +986,13 @@ int xg_cg_solve_flexible(xg_ctx* c, int pc_kind, int64_t inner, int64_t maxit,
   if (pc_kind == 2 &&
       (rc = xg_gnn_prepare_internal(c, s->dia, s->dja, s->da, s->d, n)))
     return rc;
+  if (pc_kind == 3) {
+    if (s->ncomp != 1)
+      return fail(c, XG_EUNSUPPORTED, "AMG preconditioner: one component per node only");
+    if ((rc = xg_amg_prepare_internal(c, n, s->ia.data(), s->ja.data(), s->abc.data(),
+                                      s->vals_version)))
+      return rc;
+  }
   if ((rc = read_state(c))) return rc;
   const double normb = s->hst->normb;
   const double target = tol * (normb > 1.0e-14 ? normb : 1.0);
@@ -999,6 +1014,9 @@ int xg_cg_solve_flexible(xg_ctx* c, int pc_kind, int64_t inner, int64_t maxit,
       if (pc_kind == 2) ++tin;
       hipLaunchKernelGGL(k_jacobi_apply, dim3(gv), dim3(kCgThreads), 0, c->stream, n,
                          s->r, s->d, s->z);
+    } else if (pc_kind == 3) {
+      if ((rc = xg_amg_apply_internal(c, s->r, s->z))) return rc;
+      ++tin;
     } else if (pc_kind == 2) {
       if ((rc = xg_gnn_apply_internal(c, s->dia, s->dja, s->r, s->d, s->z))) return rc;
       int used = 0;

@@ -631,12 +631,17 @@ int cho_vgrad(xg_ctx* c, const double* U) {
 int cho_psolve(xg_ctx* c, const std::vector<double>& pvals, int64_t* it) {
   ChoMesh* m = c->cho;
   const int64_t nd = (int64_t)m->pdir.size();
+  // p_pc 2: Jacobi diagonal for the setup, multigrid inside flexible CG
+  const int pc = m->cfg.p_pc == 2 ? 1 : m->cfg.p_pc;
   int rc = xg_cg_init_dev(c, m->div, nd, nd ? m->pdir.data() : nullptr,
-                          nd ? m->pmask.data() : nullptr, nd ? pvals.data() : nullptr, 1,
-                          m->cfg.p_pc);
+                          nd ? m->pmask.data() : nullptr, nd ? pvals.data() : nullptr, 1, pc);
   if (rc) return rc;
   double normb = 0.0, normr = 0.0;
   int conv = 0;
+  if (m->cfg.p_pc == 2) {
+    int64_t vc = 0;
+    return xg_cg_solve_flexible(c, 3, 1, m->cfg.p_maxit, m->cfg.p_tol, &normr, it, &vc, &conv);
+  }
   if ((rc = xg_cg_setup(c, &normb))) return rc;
   if ((rc = xg_cg_solve(c, m->cfg.p_maxit, m->cfg.p_tol, &normr, it, &conv))) return rc;
   return XG_OK;
@@ -904,7 +909,7 @@ int xg_chocg_setup(xg_ctx* c, const double* vol, const xg_chostep* cfg,
   if (cfg->rk < 1 || cfg->rk > 4) return fail(c, XG_EINVAL, "xg_chocg_setup: rk must be 1..4");
   if (cfg->flux.flux != 0 && cfg->flux.flux != 1)
     return fail(c, XG_EINVAL, "Flux not correctly configured");
-  if (cfg->p_pc != 0 && cfg->p_pc != 1) return fail(c, XG_EINVAL, "p_pc must be 0 or 1");
+  if (cfg->p_pc < 0 || cfg->p_pc > 2) return fail(c, XG_EINVAL, "p_pc must be 0, 1 or 2");
   if (cfg->p_maxit < 1 || !(cfg->p_tol >= 0.0) || !(cfg->cfl > 0.0))
     return fail(c, XG_EINVAL, "xg_chocg_setup: bad solver settings");
   if (!c->cg || xg_cg_rows(c) != m->npoin)

@@ -17,6 +17,7 @@ struct CgSolver;   // xg_cg.hip
 struct KozakMesh;  // xg_kozak.hip
 struct ChoMesh;    // xg_chocg.hip
 struct GnnNet;     // xg_gnn.hip
+struct AmgDev;     // xg_amg.hip
 
 struct xg_ctx {
   int device = 0;
@@ -69,6 +70,8 @@ struct xg_ctx {
   GnnNet* gnn = nullptr;
   // diagnostics partial sums (xg_diag.hip, grow only)
   DevArr diagbuf;
+  // algebraic multigrid preconditioner of the CG (xg_amg.hip)
+  AmgDev* amg = nullptr;
 };
 
 void xg_kozak_free(xg_ctx* c);   // xg_kozak.hip

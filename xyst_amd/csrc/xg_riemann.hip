@@ -23,6 +23,7 @@
 using xg::Cfg;
 
 #include "xg_blocks.hpp"
+#include "xg_amg.hpp"
 #include "xg_ctx.hpp"
 #include "xg_gnn.hpp"
 
@@ -484,6 +485,7 @@ int xg_ctx_destroy(xg_ctx* c) {
   if (c->cho) xg_chocg_free(c);
   if (c->gnn) xg_gnn_free(c);
   if (c->diagbuf.p) (void)hipFree(c->diagbuf.p);
+  if (c->amg) xg_amg_free(c);
   delete c;
   return XG_OK;
 }
