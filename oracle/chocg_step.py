@@ -20,8 +20,8 @@ pressure solve (ChoCG.cpp), on one partition, with the C oracle's operators
            sums of p^2 V, u_c^2 V, dp^2 V, (u_c - un_c)^2 V
 
 Gradients are finalised as in ChoCG::fingrad (841-866): divided by the nodal
-volume.  BCs (ChoCG::BC, 1341-1355): dirbc (mask 2: configured value), then
-symmetry (pos 0), then noslip.  The theta-implicit momentum solve
+volume.  BCs (ChoCG::BC, 1341-1355): dirbc (mask 1: initial value, mask 2:
+configured value), then symmetry (pos 0), then noslip.  The theta-implicit momentum solve
 (theta > 0) is not restated: the cases served run explicit RK.
 """
 import numpy as np
@@ -34,7 +34,7 @@ class ChoCGRun:
 
     def __init__(self, coord, inpoel, tri, faceset, mu, flux="damp4", stab=True,
                  stab2=False, stab2coef=0.2, cfl=0.9, rk=1, noslip=(), sym=(),
-                 dirbc=None, p_hydrostat=None, p_dirsets=(), p_tol=1e-3, p_iter=500,
+                 dirbc=None, p_hydrostat=None, p_dirbc=None, p_tol=1e-3, p_iter=500,
                  p_pc="jacobi", ic=(0.0, 0.0, 0.0)):
         self.coord = np.ascontiguousarray(coord, dtype=np.float64)
         self.inpoel = np.ascontiguousarray(inpoel, dtype=np.int64)
@@ -68,8 +68,15 @@ class ChoCGRun:
                 m, v = self.dir.get(int(p), ([0, 0, 0], None))
                 m = [m[c] if m[c] else mask[c] for c in range(3)]
                 self.dir[int(p)] = (m, list(val))
-        self.pdir = set(int(p) for p in nodes_of(p_dirsets)) if len(p_dirsets) else set()
+        # pressure Dirichlet nodes and the values of the start-up solves
+        # (mask 1: initial value 0, mask 2: the configured value); the
+        # time-step solves set the increment to 0 there
+        self.pdir = {}
+        for sset, (mask, val) in (p_dirbc or {}).items():
+            for p in nodes_of([sset]):
+                self.pdir[int(p)] = 0.0 if mask == 1 else float(val)
         self.hydrostat = p_hydrostat
+        self.ic = np.asarray(ic, np.float64)
         self.p_tol, self.p_iter, self.p_pc = p_tol, p_iter, p_pc
         self.U = np.tile(np.asarray(ic, np.float64), (n, 1))
         self.Un = self.U.copy()
@@ -86,7 +93,9 @@ class ChoCGRun:
     def BC(self, U):
         for p, (m, v) in self.dir.items():
             for c in range(3):
-                if m[c] == 2:
+                if m[c] == 1:
+                    U[p, c] = self.ic[c]
+                elif m[c] == 2:
                     U[p, c] = v[c]
         if self.symnodes.size:
             for k, p in enumerate(self.symnodes):
@@ -104,11 +113,12 @@ class ChoCGRun:
         return O.chorin_div(self.se, self.coord, self.tri, self.dt, self.P, self.pgrad,
                             u, stab=stab)
 
-    def _psolve(self, b):
-        """pinit + psolve: Dirichlet pressure (increment 0 once stepping) on
-        the configured sets and the hydrostat node, then CG warm-started
-        from the previous solution (ConjugateGradients::init with empty x)."""
-        dirbc = {p: [(1, 0.0)] for p in self.pdir}
+    def _psolve(self, b, initial=False):
+        """pinit + psolve: Dirichlet pressure (the configured values in the
+        start-up solves, increment 0 once stepping) on the configured sets
+        and the hydrostat node, then CG warm-started from the previous
+        solution (ConjugateGradients::init with empty x)."""
+        dirbc = {p: [(1, v if initial else 0.0)] for p, v in self.pdir.items()}
         if self.hydrostat is not None and self.hydrostat not in dirbc:
             dirbc[self.hydrostat] = [(1, 0.0)]
         b = b.copy()
@@ -123,7 +133,7 @@ class ChoCGRun:
         self.BC(self.U)
         # m_np = 0: div(u), solve, project with dt = 1
         self.div = self._divergence(self.U, stab=False)
-        sol = self._psolve(self.div)
+        sol = self._psolve(self.div, initial=True)
         self.U -= self._grad(sol)
         self.BC(self.U)
         self.np_ = 1
@@ -136,7 +146,7 @@ class ChoCGRun:
             vn = F[p, 0] * nx + F[p, 1] * ny + F[p, 2] * nz
             F[p] -= vn * np.array([nx, ny, nz])
         self.div = self._divergence(F, stab=False)
-        sol = self._psolve(self.div)
+        sol = self._psolve(self.div, initial=True)
         self.np_ = 2
         self.P = sol.copy()
         self.pgrad = self._grad(self.P)

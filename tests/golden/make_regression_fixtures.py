@@ -1,13 +1,16 @@
-"""Extract the reference's RieCG/Sod, KozCG/Sod and ChoCG/Lid regression
-fixtures into riecg_sod.npz, kozcg_sod.npz and chocg_lid.npz.
+"""Extract the reference's RieCG/Sod, KozCG/Sod, ChoCG/Lid and
+ChoCG/Poiseuille regression fixtures into riecg_sod.npz, kozcg_sod.npz,
+chocg_lid.npz and chocg_poiseuille.npz.
 
 Source files (reference test data, read as data only), per directory
 tests/regression/inciter/{RieCG,KozCG}/Sod:
   rectangle_01_1.5k.exo   input mesh (the same file for both)
   sod.std.exo             fields after 10 steps
   diag.std                per-step diagnostics
-and tests/regression/inciter/ChoCG/Lid: unitcube_1k.exo, ldc.std.exo,
-diag_ldc.std.
+tests/regression/inciter/ChoCG/Lid: unitcube_1k.exo, ldc.std.exo,
+diag_ldc.std, and tests/regression/inciter/ChoCG/Poiseuille:
+poiseuille1tetz.exo (side sets on tet sides), poiseuille_{damp2,damp4,rk2,
+rk3,rk4}.std.exo and diag_poiseuille_*.std.
 Run configurations (sod.q, ldc.q):
   RieCG: nstep 10, term 0.2, cfl 0.5, gamma 1.4, symmetry on side sets 2,4,5,6.
   KozCG: nstep 10, cfl 0.5, gamma 1.4, symmetry on 2,4,5,6, Dirichlet (all
@@ -17,10 +20,16 @@ Run configurations (sod.q, ldc.q):
          noslip on side sets 1,2,3,5,6, velocity (1,0,0) set on side set 4
          (mask 2), pressure CG: 500 iterations, tol 1e-3, Jacobi, hydrostat
          node 0; fields written every 2 steps (the last is step 10).
+  ChoCG Poiseuille (poiseuille_*.q, one PE): nstep 20, viscosity 0.01,
+         damp2 / damp4 / damp2 with rk 2, 3, 4 (flux, cfl and rk read from
+         each case's .q file); noslip on side sets 3, 4; v = w = initial value (mask 1) on
+         sets 1 and 5; pressure Dirichlet p = 2.4 on set 1 and 0 on set 2
+         (mask 2), CG tol 1e-3 with Jacobi, 500 iterations.
 The output mesh is renumbered by the solver, so output nodes are matched to
 input nodes by coordinates.
 """
 import os
+import re
 import sys
 
 import numpy as np
@@ -30,20 +39,34 @@ REF = "/root/reference/tests/regression/inciter"
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
+# ExodusII TETRA4 sides (1-based side number -> local nodes)
+TET_SIDES = {1: (0, 1, 3), 2: (1, 2, 3), 3: (0, 3, 2), 4: (0, 2, 1)}
+
+
 def read_mesh(ref, name="rectangle_01_1.5k.exo"):
+    """Coordinates, tets and boundary triangles with their side-set ids.  Side
+    sets may name the sides of a triangle block (element block 1 of
+    triangles, tets in block 2) or tet sides (a single tet block)."""
     f = netcdf_file(os.path.join(ref, name), "r", mmap=False)
     coord = np.stack([f.variables[c].data.astype(np.float64)
                       for c in ("coordx", "coordy", "coordz")])
-    tri_blk = f.variables["connect1"].data.astype(np.int64) - 1
-    tets = f.variables["connect2"].data.astype(np.int64) - 1
-    ntri = tri_blk.shape[0]
+    blk1 = f.variables["connect1"].data.astype(np.int64) - 1
+    if blk1.shape[1] == 3:
+        tri_blk, tets = blk1, f.variables["connect2"].data.astype(np.int64) - 1
+    else:
+        tri_blk, tets = None, blk1
     tris, sets = [], []
     nss = f.dimensions["num_side_sets"]
     ids = f.variables["ss_prop1"].data
     for s in range(1, nss + 1):
         elem = f.variables["elem_ss%d" % s].data.astype(np.int64)
-        assert (elem <= ntri).all(), "side set on tet sides not handled"
-        tris.append(tri_blk[elem - 1])
+        if tri_blk is not None:
+            assert (elem <= tri_blk.shape[0]).all(), "side set on tet sides"
+            tris.append(tri_blk[elem - 1])
+        else:
+            side = f.variables["side_ss%d" % s].data.astype(np.int64)
+            tris.append(np.array([tets[e - 1][list(TET_SIDES[k])]
+                                  for e, k in zip(elem, side)], np.int64))
         sets.append(np.full(elem.size, ids[s - 1], np.int32))
     tri = np.concatenate(tris)
     faceset = np.concatenate(sets)
@@ -106,6 +129,30 @@ def main():
                         dirval=np.array([1.0, 0.0, 0.0]), p_iter=500, p_tol=1e-3,
                         p_hydrostat=0)
     print("wrote chocg_lid.npz", fields.shape, names, t_final)
+    ref = os.path.join(REF, "ChoCG", "Poiseuille")
+    coord, tets, tri, faceset = read_mesh(ref, "poiseuille1tetz.exo")
+    cases = {}
+    for case in ("damp2", "damp4", "rk2", "rk3", "rk4"):
+        q = open(os.path.join(ref, "poiseuille_%s.q" % case)).read()
+        cfl = float(re.search(r"^cfl\s*=\s*([0-9.eE+-]+)", q, re.M).group(1))
+        flux = re.search(r'^flux\s*=\s*"(\w+)"', q, re.M).group(1)
+        m = re.search(r"^rk\s*=\s*(\d+)", q, re.M)
+        rk = int(m.group(1)) if m else 1
+        fields, names, t_final = read_result(ref, coord, "poiseuille_%s.std.exo" % case)
+        diag = np.loadtxt(os.path.join(ref, "diag_poiseuille_%s.std" % case))
+        cases.update({case + "_fields": fields, case + "_diag": diag,
+                      case + "_t_final": t_final, case + "_flux": flux,
+                      case + "_cfl": cfl, case + "_rk": rk})
+    np.savez_compressed(os.path.join(HERE, "chocg_poiseuille.npz"), coord=coord,
+                        inpoel=tets, triinpoel=tri, faceset=faceset,
+                        field_names=np.array(names), nstep=20, mu=0.01,
+                        noslip=np.array([3, 4], np.int32),
+                        dirsets=np.array([1, 5], np.int32),
+                        dirmask=np.array([0, 1, 1], np.int32),
+                        p_dirsets=np.array([1, 2], np.int32),
+                        p_dirvals=np.array([2.4, 0.0]), p_iter=500, p_tol=1e-3,
+                        **cases)
+    print("wrote chocg_poiseuille.npz", coord.shape, tets.shape, tri.shape)
 
 
 if __name__ == "__main__":

@@ -139,15 +139,18 @@ def _device_run(r, cfgkw):
     X.upload(ctx, r.coord, r.se.sup, r.se.int, r.tri)
     cg = ConjugateGradients(CSR.poisson(r.coord, r.inpoel), ctx=ctx)
     dn = np.array(sorted(r.dir), np.int64)
-    dm = np.array([r.dir[p][0] for p in dn], np.int32).reshape(-1, 3)
-    dv = np.array([r.dir[p][1] for p in dn]).reshape(-1, 3)
-    dm = (dm == 2).astype(np.int32)           # mask 2: the configured value
-    pd = sorted(r.pdir | ({r.hydrostat} if r.hydrostat is not None else set()))
+    mk = np.array([r.dir[p][0] for p in dn], np.int32).reshape(-1, 3)
+    cv = np.array([r.dir[p][1] for p in dn]).reshape(-1, 3)
+    # the values physics::dirbc sets: mask 1 the initial value, mask 2 the
+    # configured one
+    dv = np.where(mk == 2, cv, r.ic[None, :] * np.ones_like(cv))
+    dm = (mk != 0).astype(np.int32)
+    pd = sorted(set(r.pdir) | ({r.hydrostat} if r.hydrostat is not None else set()))
     X.step_setup(ctx, r.vol, X.config(**cfgkw), cfl=r.cfl, rk=len(r.rkcoef),
                  p_pc=r.p_pc, p_maxit=r.p_iter, p_tol=r.p_tol,
                  symnodes=r.symnodes, symnorms=r.symnorms, noslip=r.noslip,
                  dirnodes=dn, dirmask=dm, dirvals=dv, pdirnodes=np.array(pd, np.int64),
-                 pdirvals=np.zeros(len(pd)))
+                 pdirvals=np.array([r.pdir.get(p, 0.0) for p in pd]))
     U = torch.tensor(r.U, device="cuda")
     P = torch.tensor(r.P, device="cuda")
     return ctx, cg, U, P
@@ -293,4 +296,44 @@ def test_chocg_step_with_multigrid_pressure_solve():
         _say("  step %d: amg its %d vs pcg %d, |dU| %.2e |dP| %.2e"
              % (it + 1, n_it, r.pit[-1], eu, ep))
         assert n_it < r.pit[-1] and eu < 1e-6 and ep < 1e-6
+    ctx.close()
+
+
+@pytest.mark.parametrize("case", ["damp2", "rk4"])
+def test_chocg_poiseuille_regression_fixture(case):
+    """The reference's ChoCG/Poiseuille cases on the device (damp2 with RK1;
+    damp4 with RK4 at cfl 1): nonzero pressure Dirichlet values in the
+    start-up projections, mask-1 velocity BCs, noslip walls; 20 steps
+    against diag_poiseuille_*.std (abs or rel 1e-7) and each step against
+    the oracle."""
+    import torch
+    from tests.test_oracle_chocg import poiseuille_run
+    from xyst_amd import chorin as X
+    _say("chocg poiseuille %s" % case)
+    r, d = poiseuille_run(case)
+    ctx, cg, U, P = _device_run(r, dict(flux=str(d[case + "_flux"]), stab=True,
+                                        mu=float(d["mu"])))
+    pit = X.initial_dev(ctx, U, P)
+    r.initial()
+    assert list(pit) == r.pit[:2]
+    Un, dP = torch.empty_like(U), torch.empty_like(P)
+    V = r.vol
+    t = 0.0
+    for it in range(int(d["nstep"])):
+        dt = X.dt_dev(ctx, U)
+        n_it = X.step_dev(ctx, dt, U, Un, P, dP)
+        r.step(dt)
+        torch.cuda.synchronize()
+        u, un, p, dp = (a.cpu().numpy() for a in (U, Un, P, dP))
+        t += dt
+        s = [np.sum(p * p * V)] + [np.sum(u[:, c] ** 2 * V) for c in range(3)]
+        s += [np.sum(dp * dp * V)] + [np.sum((u[:, c] - un[:, c]) ** 2 * V) for c in range(3)]
+        row = np.concatenate([[it + 1, t, dt], np.sqrt(np.array(s) / V.sum())])
+        ref = d[case + "_diag"][it]
+        diff = np.abs(row - ref)
+        assert ((diff <= 1e-7) | (diff <= 1e-7 * np.abs(ref))).all(), (it, diff.max())
+        eu = float(np.max(np.abs(u - r.U)) / np.max(np.abs(r.U)))
+        if it % 5 == 4:
+            _say("  step %d pit %d/%d |dU| vs oracle %.2e" % (it + 1, n_it, r.pit[-1], eu))
+        assert eu < 1e-8
     ctx.close()

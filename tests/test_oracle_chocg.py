@@ -172,3 +172,42 @@ def test_lid_regression_fixture():
         diff = np.abs(mine[str(name)] - ref)
         ok = (diff <= rel.get(str(name), 1e-7) * np.abs(ref)) | (np.abs(ref) < 1e-8)
         assert ok.all(), (name, diff.max())
+
+
+def poiseuille_run(case):
+    import os
+    from oracle.chocg_step import ChoCGRun
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "chocg_poiseuille.npz"))
+    r = ChoCGRun(d["coord"], d["inpoel"], d["triinpoel"], d["faceset"], float(d["mu"]),
+                 flux=str(d[case + "_flux"]), cfl=float(d[case + "_cfl"]),
+                 rk=int(d[case + "_rk"]), noslip=list(d["noslip"]),
+                 dirbc={int(s): (list(d["dirmask"]), [0.0, 0.0, 0.0]) for s in d["dirsets"]},
+                 p_dirbc={int(s): (2, float(v)) for s, v in zip(d["p_dirsets"], d["p_dirvals"])},
+                 p_tol=float(d["p_tol"]), p_iter=int(d["p_iter"]))
+    return r, d
+
+
+@pytest.mark.parametrize("case", ["damp2", "damp4", "rk2", "rk3", "rk4"])
+def test_poiseuille_regression_fixtures(case):
+    """20 ChoCG steps of the Poiseuille channel (tests/regression/inciter/
+    ChoCG/Poiseuille, one PE): damp2 and damp4 fluxes, RK 1-4, a pressure
+    drop set by Dirichlet values (2.4 and 0) in the start-up projections,
+    mask-1 velocity BCs and noslip walls; diag_poiseuille_*.std with the
+    reference's numdiff rule (abs 1e-7 or rel 1e-7) and the exodiff bands
+    of velocity and pressure."""
+    r, d = poiseuille_run(case)
+    r.initial()
+    for it in range(int(d["nstep"])):
+        dp = r.step()
+        row, ref = r.diag(dp), d[case + "_diag"][it]
+        diff = np.abs(row - ref)
+        assert ((diff <= 1e-7) | (diff <= 1e-7 * np.abs(ref))).all(), (it, diff.max())
+    assert abs(r.t - float(d[case + "_t_final"])) < 1e-8
+    mine = {"velocityx": r.U[:, 0], "velocityy": r.U[:, 1], "velocityz": r.U[:, 2],
+            "pressure": r.P}
+    for i, name in enumerate(d["field_names"]):
+        if str(name) not in mine:
+            continue                 # divergence is not compared (exodiff.cfg)
+        ref = d[case + "_fields"][:, i]
+        diff = np.abs(mine[str(name)] - ref)
+        assert ((diff <= 1e-7 * np.abs(ref)) | (np.abs(ref) < 1e-8)).all(), (name, diff.max())
