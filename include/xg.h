@@ -298,9 +298,23 @@ int xg_chocg_initial_dev(xg_ctx* ctx, double* U, double* P, int64_t* pit);
  * nodes of cbrt(vol)/max(|u|, 1e-8) and cbrt(vol)^2/mu. */
 int xg_chocg_dt_dev(xg_ctx* ctx, const double* U, double* dt);
 
+/* The theta-implicit momentum system of ChoCG (momlhs/lhs, ChoCG.cpp:
+ * 191-209, 1434-1477): the 3-component CSR of the tets (inpoel, the mesh of
+ * xg_chocg_upload) with per-entry mass and stiffness sums, and the momentum
+ * solve's settings (pc 0 none / 1 jacobi, iter, tol).  With theta > 0 the
+ * last RK stage of xg_chocg_step_dev solves A du = R, A = -(M/dt + theta mu
+ * K), du = 0 on the dirbc-masked components and noslip nodes, by CG from the
+ * previous du, and sets U = Un + du (ChoCG.cpp:1570-1646).  Call after
+ * xg_chocg_setup. */
+int xg_chocg_momentum_setup(xg_ctx* ctx, int64_t ntet, const size_t* inpoel,
+                            double theta, int m_pc, int64_t m_maxit, double m_tol);
+/* Iterations of the last momentum solve. */
+int xg_chocg_momentum_its(xg_ctx* ctx, int64_t* its);
+
 /* One ChoCG time step (rhs/solve/pred/corr for every RK stage, then
- * div/pinit/psolve/sgrad/psolved/pgrad, ChoCG.cpp:869-1695) with explicit
- * momentum stages.  U (npoin*3) advanced in place, Un receives the step's
+ * div/pinit/psolve/sgrad/psolved/pgrad, ChoCG.cpp:869-1695), the momentum
+ * stages explicit or, after xg_chocg_momentum_setup with theta > 0, the
+ * last one semi-implicit.  U (npoin*3) advanced in place, Un receives the step's
  * starting velocity, P += dP with dP (npoin) the pressure increment; *pit
  * the pressure solve's iterations. */
 int xg_chocg_step_dev(xg_ctx* ctx, double dt, double* U, double* Un, double* P,

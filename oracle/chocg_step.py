@@ -21,8 +21,9 @@ pressure solve (ChoCG.cpp), on one partition, with the C oracle's operators
 
 Gradients are finalised as in ChoCG::fingrad (841-866): divided by the nodal
 volume.  BCs (ChoCG::BC, 1341-1355): dirbc (mask 1: initial value, mask 2:
-configured value), then symmetry (pos 0), then noslip.  The theta-implicit momentum solve
-(theta > 0) is not restated: the cases served run explicit RK.
+configured value), then symmetry (pos 0), then noslip.  With theta > 0 the
+last RK stage solves the momentum system (ChoCG::lhs, solve, msolve,
+msolved) with the 3-component ConjugateGradients.
 """
 import numpy as np
 
@@ -35,7 +36,8 @@ class ChoCGRun:
     def __init__(self, coord, inpoel, tri, faceset, mu, flux="damp4", stab=True,
                  stab2=False, stab2coef=0.2, cfl=0.9, rk=1, noslip=(), sym=(),
                  dirbc=None, p_hydrostat=None, p_dirbc=None, p_tol=1e-3, p_iter=500,
-                 p_pc="jacobi", ic=(0.0, 0.0, 0.0)):
+                 p_pc="jacobi", ic=(0.0, 0.0, 0.0), theta=0.0, m_tol=1e-3, m_iter=50,
+                 m_pc="jacobi"):
         self.coord = np.ascontiguousarray(coord, dtype=np.float64)
         self.inpoel = np.ascontiguousarray(inpoel, dtype=np.int64)
         self.tri = np.ascontiguousarray(tri, dtype=np.int64)
@@ -78,6 +80,10 @@ class ChoCGRun:
         self.hydrostat = p_hydrostat
         self.ic = np.asarray(ic, np.float64)
         self.p_tol, self.p_iter, self.p_pc = p_tol, p_iter, p_pc
+        self.theta, self.m_tol, self.m_iter, self.m_pc = theta, m_tol, m_iter, m_pc
+        self.mit = []
+        if theta > np.finfo(float).eps:
+            self._momentum_setup()
         self.U = np.tile(np.asarray(ic, np.float64), (n, 1))
         self.Un = self.U.copy()
         self.P = np.zeros(n)
@@ -88,6 +94,65 @@ class ChoCGRun:
         self.t, self.dt, self.it = 0.0, 0.0, 0
         self.np_ = 0
         self.pit = []
+
+    # ---- theta-implicit momentum system ---------------------------------
+    def _momentum_setup(self):
+        """ChoCG::momlhs (ChoCG.cpp:191-209): the 3-component CSR of the
+        momentum system; per tet and node pair the mass and stiffness terms
+        of ChoCG::lhs (1434-1477), with the CSR position of each
+        (N_a, N_b, c) entry so that a step's matrix is assembled in the
+        reference's element order."""
+        n = self.npoin
+        self.Am = O.CSR.from_tets(n, self.inpoel, ncomp=3)
+        X = self.coord
+        N = self.inpoel
+        ba = (X[:, N[:, 1]] - X[:, N[:, 0]]).T
+        ca = (X[:, N[:, 2]] - X[:, N[:, 0]]).T
+        da = (X[:, N[:, 3]] - X[:, N[:, 0]]).T
+        g1, g2, g3 = np.cross(ca, da), np.cross(da, ba), np.cross(ba, ca)
+        J = np.einsum("ij,ij->i", ba, g1)
+        g = np.stack([-g1 - g2 - g3, g1, g2, g3], 1)        # (T, 4, 3)
+        ne = N.shape[0]
+        rows, cols, mass, stiff = [], [], [], []
+        for a in range(4):
+            for b in range(4):
+                dot = g[:, a, 0] * g[:, b, 0] + g[:, a, 1] * g[:, b, 1] + g[:, a, 2] * g[:, b, 2]
+                for c in range(3):
+                    rows.append(N[:, a] * 3 + c)
+                    cols.append(N[:, b] * 3 + c)
+                    mass.append((J, 2.0 if a == b else 1.0))
+                    stiff.append(dot)
+        # element-major order of the reference's loops: e, then a, b, c
+        R = np.stack(rows, 1).reshape(-1)
+        C = np.stack(cols, 1).reshape(-1)
+        pos = np.empty(R.size, np.int64)
+        ia, ja = self.Am.ia, self.Am.ja
+        for k in range(R.size):
+            r0, r1 = ia[R[k]], ia[R[k] + 1]
+            pos[k] = r0 + np.searchsorted(ja[r0:r1], C[k])
+        self._mpos = pos
+        self._mJ = np.repeat(J, 48)
+        self._mfac = np.tile(np.array([m[1] for m in mass]), ne)
+        self._mdot = np.stack(stiff, 1).reshape(-1)
+
+    def _momentum_matrix(self):
+        """ChoCG::lhs: A(N_a,N_b,c) -= J/dt/120 (2 or 1) + theta mu
+        grad_a.grad_b / J / 6, summed in element order."""
+        v = self._mJ / self.dt / 120.0 * self._mfac
+        v = v + self.theta * self.mu * self._mdot / self._mJ / 6.0
+        a = np.zeros(self.Am.ja.size)
+        np.add.at(a, self._mpos, -v)
+        self.Am.a = a
+
+    def _momentum_dirbc(self):
+        """The momentum solve's Dirichlet data (ChoCG.cpp:1577-1600): the
+        dirbc masks with value 0, noslip nodes on all three components."""
+        bc = {}
+        for p, (m, _) in self.dir.items():
+            bc[p] = [(m[c], 0.0) for c in range(3)]
+        for p in self.noslip:
+            bc[int(p)] = [(1, 0.0)] * 3
+        return bc
 
     # ---- helpers --------------------------------------------------------
     def BC(self, U):
@@ -164,12 +229,26 @@ class ChoCGRun:
 
     def step(self, dt=None):
         self.dt = self.dt_cfl() if dt is None else dt
+        implicit = self.theta > np.finfo(float).eps
+        if implicit:
+            self._momentum_matrix()
         for stage, c in enumerate(self.rkcoef):
             R = O.chorin_rhs(self.cfg, self.se, self.coord, self.tri, self.P, self.U,
                              self.vgrad)
             if stage == 0:
                 self.Un = self.U.copy()
-            self.U = self.Un - (c * self.dt) * R / self.vol[:, None]
+            if implicit and stage + 1 == len(self.rkcoef):
+                # semi-implicit solve (ChoCG.cpp:1570-1646): A du = R, u = un + du
+                if not hasattr(self, "xm"):
+                    self.xm = np.zeros(3 * self.npoin)
+                b = R.reshape(-1).copy()
+                _, _, it, _ = O.cg(self.Am, self.xm, b, dirbc=self._momentum_dirbc(),
+                                   apply=True, pc=self.m_pc, maxit=self.m_iter,
+                                   tol=self.m_tol)
+                self.mit.append(it)
+                self.U = self.Un + self.xm.reshape(-1, 3)
+            else:
+                self.U = self.Un - (c * self.dt) * R / self.vol[:, None]
             self.BC(self.U)
             if self.flux == "damp4":
                 self.vgrad = (O.chorin_vgrad(self.se, self.coord, self.tri, self.U)

@@ -10,7 +10,7 @@ tests/regression/inciter/{RieCG,KozCG}/Sod:
 tests/regression/inciter/ChoCG/Lid: unitcube_1k.exo, ldc.std.exo,
 diag_ldc.std, and tests/regression/inciter/ChoCG/Poiseuille:
 poiseuille1tetz.exo (side sets on tet sides), poiseuille_{damp2,damp4,rk2,
-rk3,rk4}.std.exo and diag_poiseuille_*.std.
+rk3,rk4}.std.exo and diag_poiseuille_*.std (also _theta).
 Run configurations (sod.q, ldc.q):
   RieCG: nstep 10, term 0.2, cfl 0.5, gamma 1.4, symmetry on side sets 2,4,5,6.
   KozCG: nstep 10, cfl 0.5, gamma 1.4, symmetry on 2,4,5,6, Dirichlet (all
@@ -24,7 +24,10 @@ Run configurations (sod.q, ldc.q):
          damp2 / damp4 / damp2 with rk 2, 3, 4 (flux, cfl and rk read from
          each case's .q file); noslip on side sets 3, 4; v = w = initial value (mask 1) on
          sets 1 and 5; pressure Dirichlet p = 2.4 on set 1 and 0 on set 2
-         (mask 2), CG tol 1e-3 with Jacobi, 500 iterations.
+         (mask 2), CG tol 1e-3 with Jacobi, 500 iterations; the theta case
+         (theta 0.5, damp2, momentum CG tol 1e-3, 50 iterations, Jacobi)
+         keeps its diagnostics only (its fields are split over two
+         partitions).
 The output mesh is renumbered by the solver, so output nodes are matched to
 input nodes by coordinates.
 """
@@ -132,17 +135,20 @@ def main():
     ref = os.path.join(REF, "ChoCG", "Poiseuille")
     coord, tets, tri, faceset = read_mesh(ref, "poiseuille1tetz.exo")
     cases = {}
-    for case in ("damp2", "damp4", "rk2", "rk3", "rk4"):
+    for case in ("damp2", "damp4", "rk2", "rk3", "rk4", "theta"):
         q = open(os.path.join(ref, "poiseuille_%s.q" % case)).read()
         cfl = float(re.search(r"^cfl\s*=\s*([0-9.eE+-]+)", q, re.M).group(1))
         flux = re.search(r'^flux\s*=\s*"(\w+)"', q, re.M).group(1)
         m = re.search(r"^rk\s*=\s*(\d+)", q, re.M)
         rk = int(m.group(1)) if m else 1
-        fields, names, t_final = read_result(ref, coord, "poiseuille_%s.std.exo" % case)
+        m = re.search(r"^theta\s*=\s*([0-9.eE+-]+)", q, re.M)
+        theta = float(m.group(1)) if m else 0.0
         diag = np.loadtxt(os.path.join(ref, "diag_poiseuille_%s.std" % case))
-        cases.update({case + "_fields": fields, case + "_diag": diag,
-                      case + "_t_final": t_final, case + "_flux": flux,
-                      case + "_cfl": cfl, case + "_rk": rk})
+        cases.update({case + "_diag": diag, case + "_flux": flux, case + "_cfl": cfl,
+                      case + "_rk": rk, case + "_theta": theta})
+        if case != "theta":          # two-partition output files for theta
+            fields, names, t_final = read_result(ref, coord, "poiseuille_%s.std.exo" % case)
+            cases.update({case + "_fields": fields, case + "_t_final": t_final})
     np.savez_compressed(os.path.join(HERE, "chocg_poiseuille.npz"), coord=coord,
                         inpoel=tets, triinpoel=tri, faceset=faceset,
                         field_names=np.array(names), nstep=20, mu=0.01,

@@ -299,13 +299,13 @@ def test_chocg_step_with_multigrid_pressure_solve():
     ctx.close()
 
 
-@pytest.mark.parametrize("case", ["damp2", "rk4"])
+@pytest.mark.parametrize("case", ["damp2", "rk4", "theta"])
 def test_chocg_poiseuille_regression_fixture(case):
     """The reference's ChoCG/Poiseuille cases on the device (damp2 with RK1;
-    damp4 with RK4 at cfl 1): nonzero pressure Dirichlet values in the
-    start-up projections, mask-1 velocity BCs, noslip walls; 20 steps
-    against diag_poiseuille_*.std (abs or rel 1e-7) and each step against
-    the oracle."""
+    damp4 with RK4 at cfl 1; theta = 0.5 implicit momentum): nonzero
+    pressure Dirichlet values in the start-up projections, mask-1 velocity
+    BCs, noslip walls; 20 steps against diag_poiseuille_*.std (abs or rel
+    1e-7) and each step against the oracle (momentum iterations equal)."""
     import torch
     from tests.test_oracle_chocg import poiseuille_run
     from xyst_amd import chorin as X
@@ -313,6 +313,9 @@ def test_chocg_poiseuille_regression_fixture(case):
     r, d = poiseuille_run(case)
     ctx, cg, U, P = _device_run(r, dict(flux=str(d[case + "_flux"]), stab=True,
                                         mu=float(d["mu"])))
+    if r.theta > 0.0:
+        X.momentum_setup(ctx, r.inpoel, r.theta, m_pc=r.m_pc, m_maxit=r.m_iter,
+                         m_tol=r.m_tol)
     pit = X.initial_dev(ctx, U, P)
     r.initial()
     assert list(pit) == r.pit[:2]
@@ -336,4 +339,6 @@ def test_chocg_poiseuille_regression_fixture(case):
         if it % 5 == 4:
             _say("  step %d pit %d/%d |dU| vs oracle %.2e" % (it + 1, n_it, r.pit[-1], eu))
         assert eu < 1e-8
+        if r.theta > 0.0:
+            assert X.momentum_its(ctx) == r.mit[-1]
     ctx.close()

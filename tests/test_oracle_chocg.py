@@ -183,18 +183,21 @@ def poiseuille_run(case):
                  rk=int(d[case + "_rk"]), noslip=list(d["noslip"]),
                  dirbc={int(s): (list(d["dirmask"]), [0.0, 0.0, 0.0]) for s in d["dirsets"]},
                  p_dirbc={int(s): (2, float(v)) for s, v in zip(d["p_dirsets"], d["p_dirvals"])},
-                 p_tol=float(d["p_tol"]), p_iter=int(d["p_iter"]))
+                 p_tol=float(d["p_tol"]), p_iter=int(d["p_iter"]),
+                 theta=float(d[case + "_theta"]))
     return r, d
 
 
-@pytest.mark.parametrize("case", ["damp2", "damp4", "rk2", "rk3", "rk4"])
+@pytest.mark.parametrize("case", ["damp2", "damp4", "rk2", "rk3", "rk4", "theta"])
 def test_poiseuille_regression_fixtures(case):
     """20 ChoCG steps of the Poiseuille channel (tests/regression/inciter/
     ChoCG/Poiseuille, one PE): damp2 and damp4 fluxes, RK 1-4, a pressure
     drop set by Dirichlet values (2.4 and 0) in the start-up projections,
-    mask-1 velocity BCs and noslip walls; diag_poiseuille_*.std with the
-    reference's numdiff rule (abs 1e-7 or rel 1e-7) and the exodiff bands
-    of velocity and pressure."""
+    mask-1 velocity BCs and noslip walls, and (case theta) the theta = 0.5
+    implicit momentum solve with its 3-component CG; diag_poiseuille_*.std
+    with the reference's numdiff rule (abs 1e-7 or rel 1e-7) and the exodiff
+    bands of velocity and pressure (not for theta, whose fields the
+    reference keeps in two partition files)."""
     r, d = poiseuille_run(case)
     r.initial()
     for it in range(int(d["nstep"])):
@@ -202,6 +205,9 @@ def test_poiseuille_regression_fixtures(case):
         row, ref = r.diag(dp), d[case + "_diag"][it]
         diff = np.abs(row - ref)
         assert ((diff <= 1e-7) | (diff <= 1e-7 * np.abs(ref))).all(), (it, diff.max())
+    if case == "theta":
+        assert r.mit and all(0 < m <= 50 for m in r.mit)
+        return
     assert abs(r.t - float(d[case + "_t_final"])) < 1e-8
     mine = {"velocityx": r.U[:, 0], "velocityy": r.U[:, 1], "velocityz": r.U[:, 2],
             "pressure": r.P}

@@ -50,6 +50,8 @@ SIG = {
     "xg_chocg_initial_dev": (C.c_int, [VP, VP, VP, VP]),
     "xg_chocg_dt_dev": (C.c_int, [VP, VP, VP]),
     "xg_chocg_step_dev": (C.c_int, [VP, D, VP, VP, VP, VP, VP]),
+    "xg_chocg_momentum_setup": (C.c_int, [VP, I64, VP, D, C.c_int, I64, D]),
+    "xg_chocg_momentum_its": (C.c_int, [VP, VP]),
     "xg_cg_init_dev": (C.c_int, [VP, VP, I64, VP, VP, VP, C.c_int, C.c_int]),
     "xg_diag_rho_dev": (C.c_int, [VP, I64, C.c_int, VP, VP, VP, VP]),
     "xg_amg_info": (C.c_int, [VP, VP, VP, VP]),

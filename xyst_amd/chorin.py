@@ -15,7 +15,7 @@ import numpy as np
 from ._xg import ChoConfig, ChoStep, lib, ptr
 
 __all__ = ["mesh_domint", "upload", "div", "grad", "vgrad", "flux", "rhs", "config", "step_setup",
-           "initial_dev", "dt_dev", "step_dev"]
+           "initial_dev", "dt_dev", "step_dev", "momentum_setup", "momentum_its"]
 
 FLUX = {"damp2": 0, "damp4": 1}
 
@@ -167,3 +167,18 @@ def step_dev(ctx, dt, U, Un, P, dP):
     ctx._check(lib().xg_chocg_step_dev(ctx._h, float(dt), ptr(U), ptr(Un), ptr(P), ptr(dP),
                                        C.byref(it)))
     return it.value
+
+
+def momentum_setup(ctx, inpoel, theta, m_pc="jacobi", m_maxit=50, m_tol=1e-3):
+    """The theta-implicit momentum system (ChoCG::momlhs/lhs); call after
+    step_setup.  inpoel: the tets of the uploaded mesh."""
+    inp = _idx(inpoel)
+    ctx._check(lib().xg_chocg_momentum_setup(ctx._h, inp.size // 4, ptr(inp), float(theta),
+                                             {"none": 0, "jacobi": 1}[m_pc], int(m_maxit),
+                                             float(m_tol)))
+
+
+def momentum_its(ctx):
+    out = C.c_int64()
+    ctx._check(lib().xg_chocg_momentum_its(ctx._h, C.byref(out)))
+    return out.value

@@ -29,6 +29,7 @@
 #include "xg_csr.hpp"
 #include "xg_ctx.hpp"
 #include "xg_amg.hpp"
+#include "xg_cg.hpp"
 #include "xg_gnn.hpp"
 
 using namespace xgd;
@@ -644,6 +645,37 @@ int cg_dirichlet(xg_ctx* c, CgSolver* s, int64_t ndir, const size_t* dirnode,
 }  // namespace
 
 // =================================================================== C-ABI ==
+
+CgSolver* xg_cg_exchange(xg_ctx* c, CgSolver* s) {
+  CgSolver* old = c->cg;
+  c->cg = s;
+  return old;
+}
+
+void xg_cg_free_solver(CgSolver* s) { free_solver(s); }
+
+double* xg_cg_values_dev(xg_ctx* c) {
+  CgSolver* s = c->cg;
+  s->dev_bc = false;
+  s->restore = false;
+  s->bc_key.clear();
+  ++s->vals_version;
+  return s->da;
+}
+
+double* xg_cg_rhs_dev(xg_ctx* c) { return c->cg->b; }
+
+int xg_cg_raw_init(xg_ctx* c, int pc) {
+  int rc = check_cg(c);
+  if (rc) return rc;
+  if (pc != 0 && pc != 1) return fail(c, XG_EINVAL, "pc must be 0 (none) or 1 (jacobi)");
+  CgSolver* s = c->cg;
+  s->pc = pc;
+  s->ready = false;
+  s->restore = false;
+  s->dev_bc = false;
+  return XG_OK;
+}
 
 int64_t xg_cg_rows(const xg_ctx* c) {
   if (!c || !c->cg || c->cg->ncomp != 1) return -1;

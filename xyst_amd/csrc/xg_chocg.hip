@@ -23,6 +23,8 @@
 #include <vector>
 
 #include "xg_blocks.hpp"
+#include "xg_cg.hpp"
+#include "xg_csr.hpp"
 #include "xg_ctx.hpp"
 #include "xg_flux.hpp"
 #include "xg_layout.hpp"
@@ -63,6 +65,18 @@ struct ChoMesh {
   std::vector<double> pval0, pzero;         // initial values / increments (0)
   double *pgrad = nullptr, *vgrad = nullptr, *sgrad = nullptr, *R = nullptr;
   double *div = nullptr, *F = nullptr, *red = nullptr;
+  std::vector<int> h_dirnode, h_noslip;    // host copies of the BC lists
+  std::vector<uint8_t> h_dirmask;
+  // theta-implicit momentum solve (xg_chocg_momentum_setup)
+  double theta = 0.0;
+  int m_pc = 1;
+  int64_t m_maxit = 50, m_nnz = 0, m_its = 0;
+  double m_tol = 1e-3;
+  CgSolver* cgm = nullptr;
+  std::vector<DevArr> mom_allocs;
+  double *mM = nullptr, *mK = nullptr;    // per-entry mass and stiffness sums
+  int nzp = 0, ndp = 0, ndr = 0;
+  int *zpos = nullptr, *dpos = nullptr, *drow = nullptr;
 };
 
 namespace {
@@ -420,6 +434,8 @@ __global__ void k_scatter_cols(int npoin, int nf, const double* __restrict__ src
 
 void free_cho(ChoMesh* m) {
   if (!m) return;
+  if (m->cgm) xg_cg_free_solver(m->cgm);
+  dev_free_all(m->mom_allocs);
   dev_free_all(m->step_allocs);
   dev_free_all(m->allocs);
   dev_free_all(m->stage);
@@ -577,6 +593,71 @@ __global__ __launch_bounds__(kBlock) void k_chodt_partial(
     for (int k = 1; k < kBlock / 64; ++k) b = fmin(b, sm[k]);
     part[blockIdx.x] = b;
   }
+}
+
+// the momentum matrix of ChoCG::lhs (ChoCG.cpp:1434-1477) for step dt:
+// a = -(M/dt + theta mu K) from the per-entry mass and stiffness sums
+__global__ void k_mom_values(int64_t nnz, const double* __restrict__ M,
+                             const double* __restrict__ K, double dt, double tmu,
+                             double* __restrict__ a) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nnz) a[k] = -(M[k] / dt + tmu * K[k]);
+}
+
+__global__ void k_set_at(int n, const int* __restrict__ pos, double v,
+                         double* __restrict__ a) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[pos[i]] = v;
+}
+
+// u = un + du (ChoCG::msolved, ChoCG.cpp:1636-1641)
+__global__ void k_add_into(int64_t n, const double* __restrict__ un,
+                           const double* __restrict__ du, double* __restrict__ u) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) u[i] = un[i] + du[i];
+}
+
+// the momentum solver made the context's solver for the guard's lifetime
+struct MomentumSolver {
+  xg_ctx* c;
+  CgSolver* saved;
+  MomentumSolver(xg_ctx* c_, CgSolver* s) : c(c_) { saved = xg_cg_exchange(c, s); }
+  ~MomentumSolver() { xg_cg_exchange(c, saved); }
+};
+
+// ChoCG::solve's semi-implicit branch, msolve, msolved (ChoCG.cpp:1570-1646):
+// A du = R with the Dirichlet masks and noslip nodes set to du = 0, CG from
+// the previous du; U = Un + du
+int cho_momentum(xg_ctx* c, double dt, double* U, const double* Un) {
+  ChoMesh* m = c->cho;
+  const int64_t n3 = (int64_t)m->npoin * 3;
+  MomentumSolver g(c, m->cgm);
+  double* a = xg_cg_values_dev(c);
+  hipLaunchKernelGGL(k_mom_values, dim3(nblocks(m->m_nnz)), dim3(kBlock), 0, c->stream,
+                     m->m_nnz, m->mM, m->mK, dt, m->theta * m->cfg.flux.mu, a);
+  if (m->nzp)
+    hipLaunchKernelGGL(k_set_at, dim3(nblocks(m->nzp)), dim3(kBlock), 0, c->stream, m->nzp,
+                       m->zpos, 0.0, a);
+  if (m->ndp)
+    hipLaunchKernelGGL(k_set_at, dim3(nblocks(m->ndp)), dim3(kBlock), 0, c->stream, m->ndp,
+                       m->dpos, 1.0, a);
+  double* b = xg_cg_rhs_dev(c);
+  XG_HIP(c, hipMemcpyAsync(b, m->R, (size_t)n3 * 8, hipMemcpyDeviceToDevice, c->stream));
+  if (m->ndr)
+    hipLaunchKernelGGL(k_set_at, dim3(nblocks(m->ndr)), dim3(kBlock), 0, c->stream, m->ndr,
+                       m->drow, 0.0, b);
+  XG_HIP(c, hipGetLastError());
+  int rc = xg_cg_raw_init(c, m->m_pc);
+  if (rc) return rc;
+  double normb = 0.0, normr = 0.0;
+  int conv = 0;
+  if ((rc = xg_cg_setup(c, &normb))) return rc;
+  if ((rc = xg_cg_solve(c, m->m_maxit, m->m_tol, &normr, &m->m_its, &conv))) return rc;
+  double* x = nullptr;
+  if ((rc = xg_cg_device_x(c, &x))) return rc;
+  hipLaunchKernelGGL(k_add_into, dim3(nblocks(n3)), dim3(kBlock), 0, c->stream, n3, Un, x, U);
+  XG_HIP(c, hipGetLastError());
+  return XG_OK;
 }
 
 int check_step(xg_ctx* c) {
@@ -977,6 +1058,9 @@ int xg_chocg_setup(xg_ctx* c, const double* vol, const xg_chostep* cfg,
   m->nsymnode = (int)node.size();
   m->nnoslip = (int)nnoslip;
   m->ndir = (int)ndir;
+  m->h_dirnode = dn;
+  m->h_dirmask = dm;
+  m->h_noslip = ns;
   m->pdir.assign(pdirnodes, pdirnodes + npdir);
   m->pmask.assign(npdir, 1);
   m->pval0.assign(pdirvals, pdirvals + npdir);
@@ -1059,13 +1143,19 @@ int xg_chocg_step_dev(xg_ctx* c, double dt, double* U, double* Un, double* P, do
   const int n = m->npoin;
   const int64_t n3 = (int64_t)n * 3;
   const int nst = m->cfg.rk;
+  const bool implicit = m->theta > 2.220446049250313e-16;
   for (int st = 0; st < nst; ++st) {
-    // ChoCG::rhs, solve (explicit), pred (BC, velocity gradients for damp4)
+    // ChoCG::rhs, solve (explicit, or semi-implicit at the last stage when
+    // theta > 0), pred (BC, velocity gradients for damp4)
     if ((rc = xg_chorin_rhs(c, 1, &m->cfg.flux, 3, P, U, m->vgrad, m->R))) return rc;
     if (st == 0)
       XG_HIP(c, hipMemcpyAsync(Un, U, (size_t)n3 * 8, hipMemcpyDeviceToDevice, c->stream));
-    hipLaunchKernelGGL(k_cho_update, dim3(nblocks(n3)), dim3(kBlock), 0, c->stream, n,
-                       rkcoef[nst - 1][st] * dt, U, Un, m->R, m->vol);
+    if (implicit && st + 1 == nst) {
+      if ((rc = cho_momentum(c, dt, U, Un))) return rc;
+    } else {
+      hipLaunchKernelGGL(k_cho_update, dim3(nblocks(n3)), dim3(kBlock), 0, c->stream, n,
+                         rkcoef[nst - 1][st] * dt, U, Un, m->R, m->vol);
+    }
     if ((rc = cho_bc(c, U))) return rc;
     if (m->cfg.flux.flux == 1 && (rc = cho_vgrad(c, U))) return rc;
   }
@@ -1086,6 +1176,120 @@ int xg_chocg_step_dev(xg_ctx* c, double dt, double* U, double* Un, double* P, do
   if ((rc = cho_grad(c, P, m->pgrad))) return rc;
   XG_HIP(c, hipGetLastError());
   if (pit) *pit = it;
+  return XG_OK;
+}
+
+int xg_chocg_momentum_setup(xg_ctx* c, int64_t ntet, const size_t* inpoel, double theta,
+                            int m_pc, int64_t m_maxit, double m_tol) {
+  int rc = check_step(c);
+  if (rc) return rc;
+  ChoMesh* m = c->cho;
+  if (ntet <= 0 || !inpoel || !(theta >= 0.0) || (m_pc != 0 && m_pc != 1) || m_maxit < 1 ||
+      !(m_tol >= 0.0))
+    return fail(c, XG_EINVAL, "xg_chocg_momentum_setup: bad arguments");
+  const int64_t n = m->npoin;
+  for (int64_t i = 0; i < 4 * ntet; ++i)
+    if (inpoel[i] >= (size_t)n) return fail(c, XG_EINVAL, "xg_chocg_momentum_setup: node id out of range");
+  // ChoCG::momlhs: the 3-component pattern
+  std::vector<int64_t> ia;
+  std::vector<int32_t> ja;
+  if ((rc = xg::csr_pattern(n, ntet, inpoel, 3, ia, ja, c->err))) return rc;
+  const int64_t nnz = ia[3 * n];
+  auto find = [&](int64_t row, int64_t col) -> int64_t {
+    const int32_t* b0 = ja.data() + ia[row];
+    const int32_t* b1 = ja.data() + ia[row + 1];
+    const int32_t* f = std::lower_bound(b0, b1, (int32_t)col);
+    return (f != b1 && *f == col) ? (int64_t)(f - ja.data()) : -1;
+  };
+  // per-entry sums of the mass and stiffness terms of ChoCG::lhs, tet order
+  std::vector<double> M(nnz, 0.0), K(nnz, 0.0);
+  std::vector<double> hx(n), hy(n), hz(n);
+  XG_HIP(c, hipMemcpy(hx.data(), m->x, n * 8, hipMemcpyDeviceToHost));
+  XG_HIP(c, hipMemcpy(hy.data(), m->y, n * 8, hipMemcpyDeviceToHost));
+  XG_HIP(c, hipMemcpy(hz.data(), m->z, n * 8, hipMemcpyDeviceToHost));
+  for (int64_t e = 0; e < ntet; ++e) {
+    const size_t* N = inpoel + e * 4;
+    const double ba[3] = {hx[N[1]] - hx[N[0]], hy[N[1]] - hy[N[0]], hz[N[1]] - hz[N[0]]};
+    const double ca[3] = {hx[N[2]] - hx[N[0]], hy[N[2]] - hy[N[0]], hz[N[2]] - hz[N[0]]};
+    const double da[3] = {hx[N[3]] - hx[N[0]], hy[N[3]] - hy[N[0]], hz[N[3]] - hz[N[0]]};
+    double g[4][3];
+    g[1][0] = ca[1] * da[2] - da[1] * ca[2];
+    g[1][1] = ca[2] * da[0] - da[2] * ca[0];
+    g[1][2] = ca[0] * da[1] - da[0] * ca[1];
+    g[2][0] = da[1] * ba[2] - ba[1] * da[2];
+    g[2][1] = da[2] * ba[0] - ba[2] * da[0];
+    g[2][2] = da[0] * ba[1] - ba[0] * da[1];
+    g[3][0] = ba[1] * ca[2] - ca[1] * ba[2];
+    g[3][1] = ba[2] * ca[0] - ca[2] * ba[0];
+    g[3][2] = ba[0] * ca[1] - ca[0] * ba[1];
+    for (int i = 0; i < 3; ++i) g[0][i] = -g[1][i] - g[2][i] - g[3][i];
+    const double J = ba[0] * g[1][0] + ba[1] * g[1][1] + ba[2] * g[1][2];
+    if (!(J > 0.0)) return fail(c, XG_EINVAL, "xg_chocg_momentum_setup: non-positive tet Jacobian");
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) {
+        const double vm = J / 120.0 * (a == b ? 2.0 : 1.0);
+        const double vk = (g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2]) / J / 6.0;
+        for (int cc = 0; cc < 3; ++cc) {
+          const int64_t p = find((int64_t)N[a] * 3 + cc, (int64_t)N[b] * 3 + cc);
+          if (p < 0) return fail(c, XG_EINVAL, "xg_chocg_momentum_setup: entry not in pattern");
+          M[p] += vm;
+          K[p] += vk;
+        }
+      }
+  }
+  // the momentum solve's Dirichlet rows (ChoCG.cpp:1577-1600): masked
+  // components of the dirbc nodes and all components of noslip nodes, each
+  // set to du = 0 as CSR::dirichlet does: row and column zeroed, diagonal 1
+  std::vector<char> isdir(3 * n, 0);
+  for (size_t k = 0; k < m->h_dirnode.size(); ++k)
+    for (int cc = 0; cc < 3; ++cc)
+      if (m->h_dirmask[k * 3 + cc]) isdir[(size_t)m->h_dirnode[k] * 3 + cc] = 1;
+  for (int p : m->h_noslip)
+    for (int cc = 0; cc < 3; ++cc) isdir[(size_t)p * 3 + cc] = 1;
+  std::vector<int> zpos, dpos, drow;
+  for (int64_t r = 0; r < 3 * n; ++r) {
+    if (!isdir[r]) continue;
+    drow.push_back((int)r);
+    for (int64_t k = ia[r]; k < ia[r + 1]; ++k) {
+      zpos.push_back((int)k);
+      if (ja[k] == r) dpos.push_back((int)k);
+      const int64_t q = find(ja[k], r);      // the column entry in row ja[k]
+      if (q >= 0) zpos.push_back((int)q);
+    }
+  }
+  XG_HIP(c, hipSetDevice(c->device));
+  (void)hipStreamSynchronize(c->stream);
+  if (m->cgm) { xg_cg_free_solver(m->cgm); m->cgm = nullptr; }
+  dev_free_all(m->mom_allocs);
+  {
+    std::vector<size_t> iz(ia.begin(), ia.end()), jz(ja.begin(), ja.end());
+    std::vector<double> zeros(nnz, 0.0);
+    CgSolver* pressure = xg_cg_exchange(c, nullptr);
+    rc = xg_cg_matrix(c, 3, n, iz.data(), jz.data(), zeros.data(), 0);
+    m->cgm = xg_cg_exchange(c, pressure);
+    if (rc) { m->cgm = nullptr; return rc; }
+  }
+  auto& L = m->mom_allocs;
+  if ((rc = upload(c, L, &m->mM, M)) || (rc = upload(c, L, &m->mK, K)) ||
+      (rc = upload(c, L, &m->zpos, zpos)) || (rc = upload(c, L, &m->dpos, dpos)) ||
+      (rc = upload(c, L, &m->drow, drow)))
+    return rc;
+  m->m_nnz = nnz;
+  m->nzp = (int)zpos.size();
+  m->ndp = (int)dpos.size();
+  m->ndr = (int)drow.size();
+  m->theta = theta;
+  m->m_pc = m_pc;
+  m->m_maxit = m_maxit;
+  m->m_tol = m_tol;
+  return XG_OK;
+}
+
+int xg_chocg_momentum_its(xg_ctx* c, int64_t* its) {
+  int rc = check_step(c);
+  if (rc) return rc;
+  if (!its) return fail(c, XG_EINVAL, "xg_chocg_momentum_its: null");
+  *its = c->cho->m_its;
   return XG_OK;
 }
 
