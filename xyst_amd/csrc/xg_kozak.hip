@@ -108,9 +108,10 @@ __global__ __launch_bounds__(kKozThreads) void k_kozak_blocks(
     double* __restrict__ S, double gm1, double dt, double ctau) {
   constexpr int CAP = xg::kKozCap;
   constexpr int NO = AEC ? 15 : 5;
+  constexpr int NF = AEC ? 10 : 5;   // per corner slot: flux (+ aec)
   __shared__ double sU[5 * CAP];
   __shared__ double sX[3 * CAP];
-  __shared__ double F[kKozThreads * 5];
+  __shared__ double F[kKozThreads * NF];
   __shared__ uint16_t sE[kKozThreads];
   __shared__ uint16_t sT[CAP];
   const int t = threadIdx.x;
@@ -216,36 +217,29 @@ __global__ __launch_bounds__(kKozThreads) void k_kozak_blocks(
         f[j + 1] += cg * pr;
         f[4] += cg * (ue[4] + pr) * uj;
       }
+      if constexpr (AEC) tet_aec(X, sU, CAP, N, a, ctau, aec);
       if (i < C.z) {
 #pragma unroll
-        for (int c = 0; c < 5; ++c) F[t * 5 + c] = f[c];
+        for (int c = 0; c < 5; ++c) F[t * NF + c] = f[c];
+        if constexpr (AEC) {
+#pragma unroll
+          for (int c = 0; c < 5; ++c) F[t * NF + 5 + c] = aec[c];
+        }
       }
-      if constexpr (AEC) tet_aec(X, sU, CAP, N, a, ctau, aec);
     }
     __syncthreads();          // F, sE, sT complete
     if (t < nloc) {
       int e0 = t ? sT[t - 1] : 0, e1 = sT[t];
       for (int j = e0; j < e1; ++j) {
-        const double* fs = F + (int)sE[j] * 5;
+        const double* fs = F + (int)sE[j] * NF;
 #pragma unroll
         for (int c = 0; c < 5; ++c) acc[c] += fs[c];
-      }
-    }
-    if constexpr (AEC) {
-      __syncthreads();        // fluxes consumed
-      if (i < C.z) {
-#pragma unroll
-        for (int c = 0; c < 5; ++c) F[t * 5 + c] = aec[c];
-      }
-      __syncthreads();
-      if (t < nloc) {
-        int e0 = t ? sT[t - 1] : 0, e1 = sT[t];
-        for (int j = e0; j < e1; ++j) {
-          const double* fs = F + (int)sE[j] * 5;
+        if constexpr (AEC) {
 #pragma unroll
           for (int c = 0; c < 5; ++c) {
-            acc[5 + c] += fs[c] > 0.0 ? fs[c] : 0.0;     // std::max(0.0, aec)
-            acc[10 + c] += fs[c] < 0.0 ? fs[c] : 0.0;    // std::min(0.0, aec)
+            const double v = fs[5 + c];
+            acc[5 + c] += v > 0.0 ? v : 0.0;     // std::max(0.0, aec)
+            acc[10 + c] += v < 0.0 ? v : 0.0;    // std::min(0.0, aec)
           }
         }
       }
@@ -386,27 +380,24 @@ __global__ __launch_bounds__(kBlock) void k_fixup_minmax(
   for (int c = 0; c < 10; ++c) q[c] = v[c];
 }
 
-// limit coefficients C+/- (KozCG.cpp:1005-1024)
+// limit coefficients C+/- (KozCG.cpp:1005-1024), one thread per node and
+// component (neighbouring threads read neighbouring words)
 __global__ __launch_bounds__(kBlock) void k_kozcg_coef(
     int npoin, const double* __restrict__ W, const double* __restrict__ vol,
     const double* __restrict__ UL, const double* __restrict__ Q,
     double* __restrict__ Cf) {
 #pragma clang fp contract(off)
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= npoin) return;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (int64_t)npoin * 5) return;
+  const int64_t g = k / 5;
+  const int c = (int)(k - g * 5);
   constexpr double eps = 2.220446049250313e-16;
   const double v = vol[g];
-  const double* w = W + (size_t)g * 15;
-  const double* q = Q + (size_t)g * 10;
-  double* o = Cf + (size_t)g * 10;
-#pragma unroll
-  for (int c = 0; c < 5; ++c) {
-    const double pp = w[5 + c] / v, pn = w[10 + c] / v;
-    const double ul = UL[(size_t)g * 5 + c];
-    const double qp = q[c] - ul, qn = q[5 + c] - ul;
-    o[c] = pp < eps ? 0.0 : smin(1.0, qp / pp);
-    o[5 + c] = pn > -eps ? 0.0 : smin(1.0, qn / pn);
-  }
+  const double pp = W[g * 15 + 5 + c] / v, pn = W[g * 15 + 10 + c] / v;
+  const double ul = UL[k];
+  const double qp = Q[g * 10 + c] - ul, qn = Q[g * 10 + 5 + c] - ul;
+  Cf[g * 10 + c] = pp < eps ? 0.0 : smin(1.0, qp / pp);
+  Cf[g * 10 + 5 + c] = pn > -eps ? 0.0 : smin(1.0, qn / pn);
 }
 
 // limited antidiffusive contributions (KozCG.cpp:1049-1078): per tet the
@@ -816,8 +807,8 @@ int xg_kozcg_step_dev(xg_ctx* c, double dt, double* U, double* Un) {
                          c->stream, k->nfx, k->fx_node, k->fx_off, k->fx_slot,
                          k->S15, k->Q);
     // limit coefficients and limited contributions (KozCG::lim)
-    hipLaunchKernelGGL(k_kozcg_coef, dim3(nblocks(n)), dim3(kBlock), 0, c->stream,
-                       n, k->W, k->vol, k->UL, k->Q, k->C);
+    hipLaunchKernelGGL(k_kozcg_coef, dim3(nblocks((int64_t)n * 5)), dim3(kBlock), 0,
+                       c->stream, n, k->W, k->vol, k->UL, k->Q, k->C);
     if (k->nblk)
       hipLaunchKernelGGL(k_kozcg_lim, dim3(k->nblk), dim3(kKozThreads), 0,
                          c->stream, k->nblk, k->blk0, k->blk1, k->halo, k->chk,
