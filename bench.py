@@ -277,6 +277,29 @@ def chocg_bench(n, stream, steps, p_pc="jacobi"):
     return out
 
 
+def copy_bandwidth(stream, nbytes=1 << 30, reps=10):
+    """Measured device-to-device copy rate (read + write bytes per second) of
+    an nbytes buffer (torch copy_, i.e. the HIP runtime's device copy),
+    timed with HIP events: the attainable HBM bandwidth the kernels are also
+    compared with."""
+    import torch
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    b.copy_(a)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(reps):
+        b.copy_(a)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    del a, b
+    return {"GBs": 2.0 * nbytes / (ms * 1e-3) / 1e9, "bytes_copied": nbytes,
+            "ms": ms, "frac_of_peak": 2.0 * nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
 def read_traffic(path):
     """HBM bytes per rhs launch measured by rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE per dispatch, each scaled by what the counter reads for a copy
@@ -328,6 +351,10 @@ def main():
                     help="timed kozak::rhs / KozCG steps on the same mesh (0: skip)")
     ap.add_argument("--chocg-n", type=int, default=128,
                     help="box size of the ChoCG Taylor-Green steps (0: skip)")
+    ap.add_argument("--chocg-steps", type=int, default=5,
+                    help="timed ChoCG Taylor-Green steps per pressure solver")
+    ap.add_argument("--chocg-pc", default="both", choices=["both", "jacobi", "amg"],
+                    help="pressure solver(s) of the ChoCG steps")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed region, copy a 512 MiB buffer 3 times "
                          "(the known-size kernel that calibrates HBM byte counters)")
@@ -408,6 +435,8 @@ def main():
         "riecg_step_ms": step_ms,
         "wall_s_timed": wall,
     }
+    out["hbm_copy"] = copy_bandwidth(stream)
+    out["hbm_copy"]["rhs_frac_of_copy"] = achieved / out["hbm_copy"]["GBs"]
     traffic = read_traffic(os.path.join(ROOT, "profiles", "rhs_box160_traffic.txt"))
     if traffic is not None and args.n == 160:
         out["roofline"]["traffic"] = traffic
@@ -425,11 +454,12 @@ def main():
         del dG, dR, dU, dUn
         torch.cuda.empty_cache()
         out["poisson"] = poisson_bench(args.poisson_n, stream)
-    if args.chocg_n > 0:
+    if args.chocg_n > 0 and args.chocg_pc in ("both", "jacobi"):
         torch.cuda.empty_cache()
-        out["chocg"] = chocg_bench(args.chocg_n, stream, 5)
+        out["chocg"] = chocg_bench(args.chocg_n, stream, args.chocg_steps)
+    if args.chocg_n > 0 and args.chocg_pc in ("both", "amg"):
         torch.cuda.empty_cache()
-        out["chocg_amg"] = chocg_bench(args.chocg_n, stream, 5, p_pc="amg")
+        out["chocg_amg"] = chocg_bench(args.chocg_n, stream, args.chocg_steps, p_pc="amg")
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline()
