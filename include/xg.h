@@ -29,7 +29,8 @@ typedef enum {
   XG_ENOMESH = 2,       /* no mesh uploaded / setup not done */
   XG_EHIP = 3,          /* a HIP runtime call failed */
   XG_EUNSUPPORTED = 4,  /* configuration not supported by the device path */
-  XG_EBLOWUP = 5        /* time step below machine epsilon (RieCG.cpp:861) */
+  XG_EBLOWUP = 5,       /* time step below machine epsilon (RieCG.cpp:861) */
+  XG_EIO = 6            /* a file could not be read or written (xg_exo_*) */
 } xg_status;
 
 typedef struct xg_ctx xg_ctx;
@@ -461,6 +462,55 @@ int xg_gnn_train_system(xg_ctx* ctx, int64_t stride, int64_t max_pairs, int batc
 
 /* Releases the solver's device memory. */
 int xg_cg_destroy(xg_ctx* ctx);
+
+/* ---- ExodusII files (NetCDF classic, CDF-1/CDF-2), host only ------------
+ * The mesh input and field output of the path without the SEACAS/NetCDF
+ * libraries.  A handle is returned by xg_exo_open/xg_exo_create even when
+ * they fail (then holding the message for xg_exo_error); xg_exo_close
+ * releases it.  Node ids are 0-based; steps are 0-based record indices. */
+typedef struct xg_exo xg_exo;
+const char* xg_exo_error(const xg_exo* exo);
+void xg_exo_close(xg_exo* exo);
+/* ExodusIIMeshReader(filename) (ExodusIIMeshReader.hpp:53-58); append != 0
+ * opens a file written by xg_exo_create for further steps (ExoWriter::OPEN,
+ * ExodusIIMeshWriter.hpp:32-35). */
+int xg_exo_open(const char* path, int append, xg_exo** exo);
+/* readHeader/nelem (ExodusIIMeshReader.hpp:136,245): nodes, tetrahedra and
+ * triangles over all element blocks, side sets, node variables, steps. */
+int xg_exo_sizes(xg_exo* exo, int64_t* npoin, int64_t* ntet, int64_t* ntri,
+                 int* nsideset, int* nnodevar, int64_t* nstep);
+/* readNodes (ExodusIIMeshReader.hpp:103): coordx/y/z (or coord). */
+int xg_exo_coords(xg_exo* exo, double* x, double* y, double* z);
+/* readElements of the tetrahedron blocks (ExodusIIMeshReader.hpp:109),
+ * 4 ids per element, blocks in file order. */
+int xg_exo_tets(xg_exo* exo, int64_t* inpoel);
+/* readFaces (ExodusIIMeshReader.hpp:86): the triangle blocks, 3 ids each. */
+int xg_exo_tris(xg_exo* exo, int64_t* triinpoel);
+/* Side set k (0-based position in the file): its id (ss_prop1) and size. */
+int xg_exo_sideset(xg_exo* exo, int k, int* id, int64_t* nface);
+/* readSidesetFaces (ExodusIIMeshReader.hpp:82-86) resolved to node triples:
+ * a side naming a triangle element gives that triangle, a side of a
+ * tetrahedron gives that side's nodes in ExodusII side order. */
+int xg_exo_sideset_faces(xg_exo* exo, int k, int64_t* tri);
+/* readNodeVarNames / readTimeValues / readNodeScalars
+ * (ExodusIIMeshReader.hpp:124-133). */
+int xg_exo_node_var_name(xg_exo* exo, int var, char* buf, int len);
+int xg_exo_times(xg_exo* exo, double* t);
+int xg_exo_read_node_var(xg_exo* exo, int64_t step, int var, double* out);
+/* writeMesh + writeNodeVarNames (ExodusIIMeshWriter.hpp:41,64): creates the
+ * file with coordinates, a triangle block (ntri > 0, element ids 1..ntri), a
+ * tetrahedron block, side sets listing triangles (ss_face[ss_off[s] ..
+ * ss_off[s+1]) index triinpoel, like UnsMesh::bface) and the node-variable
+ * names, which a NetCDF classic header must hold before any step. */
+int xg_exo_create(const char* path, int64_t npoin, const double* x, const double* y,
+                  const double* z, int64_t ntet, const int64_t* inpoel, int64_t ntri,
+                  const int64_t* triinpoel, int nsideset, const int* ss_id,
+                  const int64_t* ss_off, const int64_t* ss_face, int nnodevar,
+                  const char* const* names, xg_exo** exo);
+/* writeTimeStamp / writeNodeScalar (ExodusIIMeshWriter.hpp:58,75); steps are
+ * written in order: step <= the current count (a new step is appended). */
+int xg_exo_write_time(xg_exo* exo, int64_t step, double time);
+int xg_exo_write_node_var(xg_exo* exo, int64_t step, int var, const double* vals);
 
 #ifdef __cplusplus
 }

@@ -33,6 +33,7 @@ input nodes by coordinates.
 """
 import os
 import re
+import shutil
 import sys
 
 import numpy as np
@@ -159,6 +160,17 @@ def main():
                         p_dirvals=np.array([2.4, 0.0]), p_iter=500, p_tol=1e-3,
                         **cases)
     print("wrote chocg_poiseuille.npz", coord.shape, tets.shape, tri.shape)
+
+    # three ExodusII files kept as they are, for the file reader's tests: a
+    # mesh with a triangle block and side sets on it (RieCG/Sod), a mesh with
+    # untyped TRI/TETRA blocks (ChoCG/Lid) and a result file with node sets,
+    # node variables and time steps (ChoCG/Lid output)
+    dst = os.path.join(HERE, "exo")
+    os.makedirs(dst, exist_ok=True)
+    for sub, name in (("RieCG/Sod", "rectangle_01_1.5k.exo"),
+                      ("ChoCG/Lid", "unitcube_1k.exo"), ("ChoCG/Lid", "ldc.std.exo")):
+        shutil.copyfile(os.path.join(REF, sub, name), os.path.join(dst, name))
+    print("copied 3 ExodusII files to", dst)
 
 
 if __name__ == "__main__":

@@ -12,9 +12,9 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libxg.so")
 
-XG_OK, XG_EINVAL, XG_ENOMESH, XG_EHIP, XG_EUNSUPPORTED, XG_EBLOWUP = range(6)
+XG_OK, XG_EINVAL, XG_ENOMESH, XG_EHIP, XG_EUNSUPPORTED, XG_EBLOWUP, XG_EIO = range(7)
 _NAMES = {1: "XG_EINVAL", 2: "XG_ENOMESH", 3: "XG_EHIP", 4: "XG_EUNSUPPORTED",
-          5: "XG_EBLOWUP"}
+          5: "XG_EBLOWUP", 6: "XG_EIO"}
 
 # every symbol include/xg.h declares, with (restype, argtypes)
 I64, I32, D, VP = C.c_int64, C.c_int32, C.c_double, C.c_void_p
@@ -58,6 +58,23 @@ SIG = {
     "xg_diag_pre_dev": (C.c_int, [VP, I64, VP, VP, VP, VP, VP, VP]),
     "xg_kozcg_dt_dev": (C.c_int, [VP, VP, D, VP]),
     "xg_kozcg_step_dev": (C.c_int, [VP, D, VP, VP]),
+    "xg_exo_error": (C.c_char_p, [VP]),
+    "xg_exo_close": (None, [VP]),
+    "xg_exo_open": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(VP)]),
+    "xg_exo_sizes": (C.c_int, [VP, C.POINTER(I64), C.POINTER(I64), C.POINTER(I64),
+                               C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(I64)]),
+    "xg_exo_coords": (C.c_int, [VP, VP, VP, VP]),
+    "xg_exo_tets": (C.c_int, [VP, VP]),
+    "xg_exo_tris": (C.c_int, [VP, VP]),
+    "xg_exo_sideset": (C.c_int, [VP, C.c_int, C.POINTER(C.c_int), C.POINTER(I64)]),
+    "xg_exo_sideset_faces": (C.c_int, [VP, C.c_int, VP]),
+    "xg_exo_node_var_name": (C.c_int, [VP, C.c_int, C.c_char_p, C.c_int]),
+    "xg_exo_times": (C.c_int, [VP, VP]),
+    "xg_exo_read_node_var": (C.c_int, [VP, I64, C.c_int, VP]),
+    "xg_exo_create": (C.c_int, [C.c_char_p, I64, VP, VP, VP, I64, VP, I64, VP, C.c_int, VP,
+                                VP, VP, C.c_int, VP, C.POINTER(VP)]),
+    "xg_exo_write_time": (C.c_int, [VP, I64, D]),
+    "xg_exo_write_node_var": (C.c_int, [VP, I64, C.c_int, VP]),
     "xg_chocg_upload": (C.c_int, [VP, I64, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP,
                                   I64, VP]),
     "xg_chorin_div": (C.c_int, [VP, C.c_int, D, VP, VP, C.c_int, VP, C.c_int, VP,
