@@ -161,16 +161,18 @@ def main():
                         **cases)
     print("wrote chocg_poiseuille.npz", coord.shape, tets.shape, tri.shape)
 
-    # three ExodusII files kept as they are, for the file reader's tests: a
-    # mesh with a triangle block and side sets on it (RieCG/Sod), a mesh with
-    # untyped TRI/TETRA blocks (ChoCG/Lid) and a result file with node sets,
-    # node variables and time steps (ChoCG/Lid output)
+    # ExodusII files kept as they are, for the file reader's tests: a mesh
+    # with a triangle block and side sets on it (RieCG/Sod), a mesh with
+    # untyped TRI/TETRA blocks (ChoCG/Lid), result files with node sets, node
+    # variables and time steps (ChoCG/Lid and RieCG/Sod output; the latter is
+    # also the end-to-end RieCG test's reference)
     dst = os.path.join(HERE, "exo")
     os.makedirs(dst, exist_ok=True)
     for sub, name in (("RieCG/Sod", "rectangle_01_1.5k.exo"),
-                      ("ChoCG/Lid", "unitcube_1k.exo"), ("ChoCG/Lid", "ldc.std.exo")):
+                      ("ChoCG/Lid", "unitcube_1k.exo"), ("ChoCG/Lid", "ldc.std.exo"),
+                      ("RieCG/Sod", "sod.std.exo")):
         shutil.copyfile(os.path.join(REF, sub, name), os.path.join(dst, name))
-    print("copied 3 ExodusII files to", dst)
+    print("copied 4 ExodusII files to", dst)
 
 
 if __name__ == "__main__":
