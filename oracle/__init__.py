@@ -58,6 +58,10 @@ def lib():
                               _f64p, _i64p, _f64p, _f64p, _f64p, _f64p, I64,
                               _i64p, _u8p, _f64p, _f64p, _f64p, C.c_void_p]
         L.orc_rhs.restype = C.c_int
+        L.orc_rhs_mt.argtypes = [C.c_int, cfgp, I64, C.c_int, _i64p, _i64p, _f64p, _i64p,
+                                 _f64p, _i64p, _f64p, _f64p, _f64p, _f64p, I64,
+                                 _i64p, _u8p, _f64p, _f64p, _f64p]
+        L.orc_rhs_mt.restype = C.c_int
         L.orc_dt.argtypes = [cfgp, I64, C.c_int, _f64p, _f64p, D]
         L.orc_dt.restype = D
         L.orc_symbc.argtypes = [I64, C.c_int, _f64p, I64, _i64p, _f64p]
@@ -333,6 +337,24 @@ def rhs(c, se, coord, tri, besym, G, U, absacc=False):
     if rc:
         raise ValueError("unsupported number of components")
     return (R, A) if absacc else R
+
+
+def rhs_mt(c, se, coord, tri, besym, G, U, nthread):
+    """riemann::rhs over nthread threads (per-partition sums added per node
+    in partition order, as the reference's chares and comrhs do)."""
+    x, y, z = (_c(coord[i], np.float64) for i in range(3))
+    U = _c(U, np.float64)
+    npoin, ncomp = U.shape
+    R = np.empty((npoin, ncomp))
+    t = _c(tri, np.int64) if tri.size else np.zeros(3, np.int64)
+    b = _c(besym, np.uint8) if besym.size else np.zeros(3, np.uint8)
+    rc = lib().orc_rhs_mt(int(nthread), C.byref(c), npoin, ncomp, *se.args(), x, y, z,
+                          tri.shape[0], t, b, _c(G, np.float64), U, R)
+    if rc == -1:
+        raise ValueError("Flux not configured")
+    if rc:
+        raise ValueError("orc_rhs_mt failed (%d)" % rc)
+    return R
 
 
 def dt(c, U, v, cfl):

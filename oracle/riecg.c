@@ -540,8 +540,9 @@ static void muscl_scalar(int ncomp, int64_t p, int64_t q, const double* x,
 }
 
 /* Magnitude of the products each flux component of the last flux call was
- * built from: the scale of its round-off, used as the parity yardstick. */
-static double fsc[MAXC];
+ * built from: the scale of its round-off, used as the parity yardstick (one
+ * per thread: orc_rhs_mt runs orc_rhs on several threads at once). */
+static _Thread_local double fsc[MAXC];
 
 /* Rusanov flux, Riemann.cpp:369-478 (symL = symR = 0 in the domain) */
 static void rusanov(const orc_cfg* cfg, int ncomp, const double* x,

@@ -85,6 +85,17 @@ int orc_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp, const int64_t* nsup,
             int64_t nbtri, const int64_t* triinpoel, const uint8_t* besym,
             const double* G, const double* U, double* R, double* A);
 
+/* orc_rhs over nthread OpenMP threads, each on a contiguous share of the
+ * superedges and boundary triangles, partial sums added per node in thread
+ * order (the reference's per-chare rhs plus comrhs, RieCG.cpp:948-965). */
+int orc_rhs_mt(int nthread, const orc_cfg* cfg, int64_t npoin, int ncomp,
+               const int64_t* nsup, const int64_t* sup0, const double* int0,
+               const int64_t* sup1, const double* int1,
+               const int64_t* sup2, const double* int2,
+               const double* x, const double* y, const double* z,
+               int64_t nbtri, const int64_t* triinpoel, const uint8_t* besym,
+               const double* G, const double* U, double* R);
+
 /* RieCG::dt, RieCG.cpp:787-852 (unsteady branch): cfl * min over nodes of
  * cbrt(vol)/max(|u|+c, 1e-8). */
 double orc_dt(const orc_cfg* cfg, int64_t npoin, int ncomp, const double* U,

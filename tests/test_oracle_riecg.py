@@ -180,3 +180,19 @@ def test_flux_not_configured():
     G = O.grad(c.se, c.coord, c.tri, U)
     with pytest.raises(ValueError, match="Flux not configured"):
         O.rhs(O.cfg(flux="roe"), c.se, c.coord, c.tri, c.besym, G, U)
+
+
+def test_threaded_rhs_matches_serial():
+    """The multi-threaded CPU baseline (per-partition sums added per node in
+    partition order) equals the serial rhs to round-off, bit for bit with one
+    thread, and repeats bit for bit at a fixed thread count."""
+    c = box_case(10, seed=4)
+    U = smooth_state(c.coord)
+    G = O.grad(c.se, c.coord, c.tri, U) / c.vol[:, None]
+    cfg = O.cfg()
+    R1, A = O.rhs(cfg, c.se, c.coord, c.tri, c.besym, G, U, absacc=True)
+    assert np.array_equal(O.rhs_mt(cfg, c.se, c.coord, c.tri, c.besym, G, U, 1), R1)
+    for T in (2, 3, 8):
+        Rt = O.rhs_mt(cfg, c.se, c.coord, c.tri, c.besym, G, U, T)
+        assert np.all(np.abs(Rt - R1) <= 1e-15 * A + 1e-300), T
+        assert np.array_equal(Rt, O.rhs_mt(cfg, c.se, c.coord, c.tri, c.besym, G, U, T))
