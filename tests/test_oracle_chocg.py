@@ -217,3 +217,53 @@ def test_poiseuille_regression_fixtures(case):
         ref = d[case + "_fields"][:, i]
         diff = np.abs(mine[str(name)] - ref)
         assert ((diff <= 1e-7 * np.abs(ref)) | (np.abs(ref) < 1e-8)).all(), (name, diff.max())
+
+
+def test_stab2_weights_from_the_operator_response():
+    """chorin's stab2 artificial viscosity (Chorin.cpp:682-687, 797-802) read
+    back from the operator: with u_new = u_n - dt R/vol (ChoCG.cpp:1561) the
+    term is a graph Laplacian with weights stab2coef*max(|vn_p|, |vn_q|).  A
+    transported scalar probes it exactly (its flux is linear in the scalar
+    for a fixed velocity): for s = delta_j, R(stab2) - R(no stab2) is
+    -w_jq at each neighbour q and +sum_q w_jq at j.  On the velocity
+    components the added term conserves (sums to zero) and dissipates
+    (u . dR >= 0).  A min for the max, a lost abs, a wrong sign or a swapped
+    end fails one of these."""
+    coord, inpoel, tri, edge, dint, se, _ = _case(4, seed=12)
+    n = coord.shape[1]
+    rng = np.random.default_rng(5)
+    x, y, z = coord
+    U = np.zeros((n, 4))
+    U[:, 0] = np.sin(2 * x + 1) + 0.3 * rng.standard_normal(n)
+    U[:, 1] = np.cos(3 * y) * x + 0.3 * rng.standard_normal(n)
+    U[:, 2] = 0.5 - z * y + 0.3 * rng.standard_normal(n)
+    P = rng.standard_normal(n)
+    G = np.zeros((n, 12))
+    coef = 0.7
+    on = O.chocfg("damp2", stab=False, stab2=True, stab2coef=coef)
+    off = O.chocfg("damp2", stab=False, stab2=False)
+    vn = lambda e, i: abs(U[i, :3] @ dint[e, :3])
+    w = {}
+    for e, (p, q) in enumerate(edge):
+        w[(p, q)] = w[(q, p)] = coef * max(vn(e, p), vn(e, q))
+    nbrs = {}
+    for p, q in edge:
+        nbrs.setdefault(p, []).append(q)
+        nbrs.setdefault(q, []).append(p)
+    for j in rng.choice(n, 8, replace=False):
+        Us = U.copy()
+        Us[:, 3] = 0.0
+        Us[j, 3] = 1.0
+        dR = (O.chorin_rhs(on, se, coord, tri, P, Us, G)
+              - O.chorin_rhs(off, se, coord, tri, P, Us, G))[:, 3]
+        want = np.zeros(n)
+        for q in nbrs[j]:
+            want[q] = -w[(j, q)]
+        want[j] = sum(w[(j, q)] for q in nbrs[j])
+        np.testing.assert_allclose(dR, want, rtol=1e-12, atol=1e-15)
+    dR = (O.chorin_rhs(on, se, coord, tri, P, U, G)
+          - O.chorin_rhs(off, se, coord, tri, P, U, G))[:, :3]
+    assert np.abs(dR.sum(0)).max() < 1e-13 * np.abs(dR).sum()
+    diss = (U[:, :3] * dR).sum()
+    want = sum(w[(p, q)] * ((U[q, :3] - U[p, :3]) ** 2).sum() for p, q in edge)
+    assert diss > 0 and abs(diss - want) < 1e-12 * want
