@@ -59,19 +59,25 @@ def _tet_oriented(tri, inpoel):
     """Boundary triangles oriented as the reference's partitioner rebuilds
     them from their tetrahedra: one of {t0,t2,t1}, {t0,t1,t3}, {t0,t3,t2},
     {t1,t2,t3} (Partitioner.cpp:578-587), i.e. outward for positively
-    oriented tetrahedra."""
-    faces = {}
-    for t in inpoel:
-        for fc in ((t[0], t[2], t[1]), (t[0], t[1], t[3]), (t[0], t[3], t[2]),
-                   (t[1], t[2], t[3])):
-            faces[tuple(sorted(fc))] = fc
-    out = np.empty_like(tri)
-    for i, fc in enumerate(tri):
-        key = tuple(sorted(int(v) for v in fc))
-        if key not in faces:
-            raise ExodusError("side-set triangle %s is not a tetrahedron face" % (key,))
-        out[i] = faces[key]
-    return out
+    oriented tetrahedra.  Faces are matched by their sorted node triples."""
+    inpoel = np.asarray(inpoel, np.int64)
+    faces = np.concatenate([inpoel[:, [0, 2, 1]], inpoel[:, [0, 1, 3]],
+                            inpoel[:, [0, 3, 2]], inpoel[:, [1, 2, 3]]])
+
+    def keys(f):
+        k = np.ascontiguousarray(np.sort(f, axis=1))
+        return k.view(np.dtype((np.void, 24))).ravel()
+
+    fk, tk = keys(faces), keys(np.asarray(tri, np.int64))
+    order = np.argsort(fk, kind="stable")
+    pos = np.searchsorted(fk[order], tk)
+    pos = np.minimum(pos, fk.size - 1)
+    hit = order[pos]
+    bad = fk[hit] != tk
+    if bad.any():
+        raise ExodusError("side-set triangle %s is not a tetrahedron face"
+                          % (tuple(np.sort(tri[np.argmax(bad)])),))
+    return faces[hit]
 
 
 def read_mesh(path, orient=True):
