@@ -53,6 +53,9 @@ struct xg_ctx {
   // RieCG stepping
   bool have_riecg = false;
   double* vol = nullptr;
+  // per node: 1 = receives halo partials in the fix-up, 2 = boundary node;
+  // the node's last gradient writer divides it by vol (RK stages)
+  uint8_t* gflag = nullptr;
   int nsymnode = 0;
   int *symnode = nullptr, *symoff = nullptr;
   double* symnorm = nullptr;

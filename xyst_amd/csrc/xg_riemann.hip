@@ -61,6 +61,9 @@ struct ChunkPrefetch {
 // The block's primitive states are staged in LDS.  Per chunk: each thread
 // writes one edge's (u_p + u_q, D) to LDS; then the thread owning local node
 // t adds -/+ D (u_p + u_q) for the chunk's edges touching it, in slot order.
+// DIV: owned nodes without halo partials or boundary terms (gflag 0) are
+// final here and are divided by their volume (RieCG.cpp:935-939)
+template <bool DIV>
 __global__ __launch_bounds__(kRhsThreads) void k_grad_blocks(
     int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
     const int* __restrict__ halo, const int4* __restrict__ chk,
@@ -68,7 +71,8 @@ __global__ __launch_bounds__(kRhsThreads) void k_grad_blocks(
     const uint16_t* __restrict__ entries, const uint32_t* __restrict__ elpq,
     const double* __restrict__ dx, const double* __restrict__ dy,
     const double* __restrict__ dz, const double* __restrict__ U,
-    double* __restrict__ G, double* __restrict__ S) {
+    double* __restrict__ G, double* __restrict__ S,
+    const uint8_t* __restrict__ gflag, const double* __restrict__ vol) {
   constexpr int CAP = xg::kBlockCap;
   __shared__ double sP[5 * CAP];
   __shared__ double F[xg::kChunkSlots * 8];
@@ -150,20 +154,67 @@ __global__ __launch_bounds__(kRhsThreads) void k_grad_blocks(
   }
   if (t < nloc) {
     double* o = t < nown ? G + (size_t)(own0 + t) * 15 : S + (size_t)(B1.x + t - nown) * 15;
+    if (DIV && t < nown && gflag[own0 + t] == 0) {
+      const double v = vol[own0 + t];
 #pragma unroll
-    for (int k = 0; k < 15; ++k) o[k] = acc[k];
+      for (int k = 0; k < 15; ++k) o[k] = acc[k] / v;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 15; ++k) o[k] = acc[k];
+    }
   }
+}
+
+// grad's halo fix-up with the division of nodes that have no boundary term
+__global__ __launch_bounds__(kBlock) void k_fixup15_div(
+    int nfx, const int* __restrict__ fx_node, const int* __restrict__ fx_off,
+    const int* __restrict__ fx_slot, const double* __restrict__ S,
+    double* __restrict__ G, const uint8_t* __restrict__ gflag,
+    const double* __restrict__ vol) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nfx) return;
+  const int n = fx_node[i];
+  double* r = G + (size_t)n * 15;
+  double acc[15];
+#pragma unroll
+  for (int c = 0; c < 15; ++c) acc[c] = r[c];
+  for (int k = fx_off[i]; k < fx_off[i + 1]; ++k) {
+    const double* s = S + (size_t)fx_slot[k] * 15;
+#pragma unroll
+    for (int c = 0; c < 15; ++c) acc[c] += s[c];
+  }
+  if ((gflag[n] & 2) == 0) {
+    const double v = vol[n];
+#pragma unroll
+    for (int c = 0; c < 15; ++c) acc[c] /= v;
+  }
+#pragma unroll
+  for (int c = 0; c < 15; ++c) r[c] = acc[c];
+}
+
+// gflag: bit 0 for the nodes of the fix-up list, bit 1 for boundary nodes
+__global__ void k_gflag(int n, const int* __restrict__ node, uint8_t bit,
+                        uint8_t* __restrict__ gflag) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) gflag[node[i]] |= bit;
+}
+
+__global__ void k_fill_u8(int64_t n, uint8_t* __restrict__ a) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
 }
 
 // riemann::grad boundary integral (Riemann.cpp:332-359) gathered per node.
 // As in the reference, every node of a face receives g[j]*n[j] for direction
 // j, with g[j] the 4:1:1-weighted value belonging to the face's j-th node.
+// DIV: boundary nodes are final here and are divided by their volume
+template <bool DIV>
 __global__ __launch_bounds__(kBlock) void k_grad_bnd(
     int nb, const int* __restrict__ bnode, const int* __restrict__ boff,
     const int* __restrict__ bface, const int* __restrict__ tri,
     const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, const double* __restrict__ U,
-    double* __restrict__ G) {
+    double* __restrict__ G, const double* __restrict__ vol) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nb) return;
   int n = bnode[i];
@@ -191,6 +242,11 @@ __global__ __launch_bounds__(kBlock) void k_grad_bnd(
 #pragma unroll
       for (int j = 0; j < 3; ++j) acc[c * 3 + j] += g[j] * nr[j];
     }
+  }
+  if (DIV) {
+    const double v = vol[n];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) acc[k] /= v;
   }
 #pragma unroll
   for (int k = 0; k < 15; ++k) G[(size_t)n * 15 + k] = acc[k];
@@ -378,22 +434,22 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_blocks(
   }
 }
 
-// G /= vol (RieCG.cpp:935-939)
-__global__ void k_divvol(int npoin, double* __restrict__ G,
-                         const double* __restrict__ vol) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)npoin * 15) return;
-  G[i] /= vol[i / 15];
-}
-
-// U = Un - rkcoef*dt*R/vol (RieCG.cpp:1016-1021)
+// U = Un - rkcoef*dt*R/vol (RieCG.cpp:1016-1021); STAGE0 first saves
+// Un = U (RieCG.cpp:1000-1003)
+template <bool STAGE0>
 __global__ void k_update(int npoin, double coef_dt, double* __restrict__ U,
-                         const double* __restrict__ Un,
-                         const double* __restrict__ R,
+                         double* __restrict__ Un, const double* __restrict__ R,
                          const double* __restrict__ vol) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)npoin * 5) return;
-  U[i] = Un[i] - coef_dt * R[i] / vol[i / 5];
+  double un;
+  if (STAGE0) {
+    un = U[i];
+    Un[i] = un;
+  } else {
+    un = Un[i];
+  }
+  U[i] = un - coef_dt * R[i] / vol[i / 5];
 }
 
 // ------------------------------------------------------------- launchers --
@@ -407,19 +463,39 @@ int check_mesh(xg_ctx* c, int ncomp) {
   return XG_OK;
 }
 
-int launch_grad(xg_ctx* c, const double* U, double* G) {
-  hipLaunchKernelGGL(k_grad_blocks, dim3(c->nblk), dim3(kRhsThreads), 0,
-                     c->stream, c->nblk, c->blk0, c->blk1, c->halo, c->chk,
-                     c->chk_ent, c->tend, c->entries, c->elpq, c->dx, c->dy,
-                     c->dz, U, G, c->gslots);
-  if (c->nfx)
-    hipLaunchKernelGGL(k_fixup<15>, dim3(nblocks(c->nfx)), dim3(kBlock), 0,
-                       c->stream, c->nfx, c->fx_node, c->fx_off, c->fx_slot,
-                       c->gslots, G);
-  if (c->nbnode)
-    hipLaunchKernelGGL(k_grad_bnd, dim3(nblocks(c->nbnode)), dim3(kBlock), 0,
-                       c->stream, c->nbnode, c->bnode, c->boff, c->bface,
-                       c->tri, c->x, c->y, c->z, U, G);
+// div: G = grad(U)/vol (the RK stage; needs xg_riecg_setup), each node
+// divided by the last kernel that adds to it; otherwise grad(U) undivided
+int launch_grad(xg_ctx* c, const double* U, double* G, bool div = false) {
+  if (div)
+    hipLaunchKernelGGL(k_grad_blocks<true>, dim3(c->nblk), dim3(kRhsThreads), 0,
+                       c->stream, c->nblk, c->blk0, c->blk1, c->halo, c->chk,
+                       c->chk_ent, c->tend, c->entries, c->elpq, c->dx, c->dy,
+                       c->dz, U, G, c->gslots, c->gflag, c->vol);
+  else
+    hipLaunchKernelGGL(k_grad_blocks<false>, dim3(c->nblk), dim3(kRhsThreads), 0,
+                       c->stream, c->nblk, c->blk0, c->blk1, c->halo, c->chk,
+                       c->chk_ent, c->tend, c->entries, c->elpq, c->dx, c->dy,
+                       c->dz, U, G, c->gslots, nullptr, nullptr);
+  if (c->nfx) {
+    if (div)
+      hipLaunchKernelGGL(k_fixup15_div, dim3(nblocks(c->nfx)), dim3(kBlock), 0,
+                         c->stream, c->nfx, c->fx_node, c->fx_off, c->fx_slot,
+                         c->gslots, G, c->gflag, c->vol);
+    else
+      hipLaunchKernelGGL(k_fixup<15>, dim3(nblocks(c->nfx)), dim3(kBlock), 0,
+                         c->stream, c->nfx, c->fx_node, c->fx_off, c->fx_slot,
+                         c->gslots, G);
+  }
+  if (c->nbnode) {
+    if (div)
+      hipLaunchKernelGGL(k_grad_bnd<true>, dim3(nblocks(c->nbnode)), dim3(kBlock), 0,
+                         c->stream, c->nbnode, c->bnode, c->boff, c->bface,
+                         c->tri, c->x, c->y, c->z, U, G, c->vol);
+    else
+      hipLaunchKernelGGL(k_grad_bnd<false>, dim3(nblocks(c->nbnode)), dim3(kBlock), 0,
+                         c->stream, c->nbnode, c->bnode, c->boff, c->bface,
+                         c->tri, c->x, c->y, c->z, U, G, nullptr);
+  }
   XG_HIP(c, hipGetLastError());
   return XG_OK;
 }
@@ -717,8 +793,18 @@ int xg_riecg_setup(xg_ctx* c, const double* vol, int64_t nsym,
   if ((rc = upload(c, L, &c->vol, vol, (size_t)c->npoin)) ||
       (rc = upload(c, L, &c->symnode, node)) || (rc = upload(c, L, &c->symoff, off)) ||
       (rc = upload(c, L, &c->symnorm, nrm)) ||
-      (rc = dev_alloc(c, L, (void**)&c->red, sizeof(double) * (kRedBlocks + 1))))
+      (rc = dev_alloc(c, L, (void**)&c->red, sizeof(double) * (kRedBlocks + 1))) ||
+      (rc = dev_alloc(c, L, (void**)&c->gflag, (size_t)c->npoin)))
     return rc;
+  hipLaunchKernelGGL(k_fill_u8, dim3(nblocks(c->npoin)), dim3(kBlock), 0, c->stream,
+                     (int64_t)c->npoin, c->gflag);
+  if (c->nfx)
+    hipLaunchKernelGGL(k_gflag, dim3(nblocks(c->nfx)), dim3(kBlock), 0, c->stream,
+                       c->nfx, c->fx_node, (uint8_t)1, c->gflag);
+  if (c->nbnode)
+    hipLaunchKernelGGL(k_gflag, dim3(nblocks(c->nbnode)), dim3(kBlock), 0, c->stream,
+                       c->nbnode, c->bnode, (uint8_t)2, c->gflag);
+  XG_HIP(c, hipGetLastError());
   c->nsymnode = (int)node.size();
   c->have_riecg = true;
   return XG_OK;
@@ -752,15 +838,15 @@ int xg_riecg_stage_dev(xg_ctx* c, int stage, double dt, double* U, double* Un,
   if (!U || !Un || !G || !R) return fail(c, XG_EINVAL, "xg_riecg_stage: null field");
   XG_HIP(c, hipSetDevice(c->device));
   int rc;
-  if ((rc = launch_grad(c, U, G))) return rc;
-  int64_t n15 = (int64_t)c->npoin * 15, n5 = (int64_t)c->npoin * 5;
-  hipLaunchKernelGGL(k_divvol, dim3(nblocks(n15)), dim3(kBlock), 0, c->stream,
-                     c->npoin, G, c->vol);
+  if ((rc = launch_grad(c, U, G, true))) return rc;
+  const int64_t n5 = (int64_t)c->npoin * 5;
   if ((rc = launch_rhs(c, G, U, R))) return rc;
   if (stage == 0)
-    XG_HIP(c, hipMemcpyAsync(Un, U, (size_t)n5 * 8, hipMemcpyDeviceToDevice, c->stream));
-  hipLaunchKernelGGL(k_update, dim3(nblocks(n5)), dim3(kBlock), 0, c->stream,
-                     c->npoin, rkcoef[stage] * dt, U, Un, R, c->vol);
+    hipLaunchKernelGGL(k_update<true>, dim3(nblocks(n5)), dim3(kBlock), 0, c->stream,
+                       c->npoin, rkcoef[stage] * dt, U, Un, R, c->vol);
+  else
+    hipLaunchKernelGGL(k_update<false>, dim3(nblocks(n5)), dim3(kBlock), 0, c->stream,
+                       c->npoin, rkcoef[stage] * dt, U, Un, R, c->vol);
   if (c->nsymnode)
     hipLaunchKernelGGL(k_symbc, dim3(nblocks(c->nsymnode)), dim3(kBlock), 0,
                        c->stream, c->nsymnode, c->symnode, c->symoff,
