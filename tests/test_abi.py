@@ -29,6 +29,32 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(_xg.SIG), "binding table out of sync with xg.h"
 
 
+def test_null_handles_are_rejected():
+    """Every entry point that takes a context (or an ExodusII handle) returns
+    XG_EINVAL for a null one, with every other argument null or zero, and
+    touches nothing: the boundary checks before it dereferences."""
+    import ctypes as C
+    lib = _xg.lib()
+    checked = 0
+    for name, (res, args) in _xg.SIG.items():
+        if not args or args[0] is not C.c_void_p or res is not C.c_int:
+            continue
+        if name in ("xg_ctx_destroy",):
+            continue
+        vals = [0.0 if a is C.c_double else 0 if a in (C.c_int, C.c_int32, C.c_int64)
+                else None for a in args]
+        assert getattr(lib, name)(*vals) == _xg.XG_EINVAL, name
+        checked += 1
+    assert checked >= 50
+    assert lib.xg_ctx_destroy(None) == _xg.XG_EINVAL
+    assert lib.xg_last_error(None)          # a message, not a crash
+    assert lib.xg_exo_error(None)
+    lib.xg_exo_close(None)
+    h = C.c_void_p()
+    assert lib.xg_exo_open(None, 0, C.byref(h)) == _xg.XG_EINVAL
+    lib.xg_exo_close(h)
+
+
 def test_context_without_gpu_fails_loudly():
     import torch
     if torch.cuda.is_available():
