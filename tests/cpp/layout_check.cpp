@@ -2,7 +2,9 @@
 // the block and fix-up kernels with integer-valued fluxes and compares with a
 // direct scatter, bounds-checking every index the device kernels use.
 // Input file (raw little-endian): int64 npoin, ne, ntri; int32 p[ne], q[ne];
-// int32 tri[3*ntri]; uint8 besym[3*ntri].  Prints "ok <stats>" or exits 1.
+// int32 tri[3*ntri]; uint8 besym[3*ntri].  Optional arguments: owned nodes,
+// block capacity and chunk slots (default: the grad blocks; the rhs blocks
+// are kRhsNodes kRhsCap kRhsSlots).  Prints "ok <stats>" or exits 1.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -14,7 +16,10 @@
 static double fval(long i, int c) { return (double)((i * 37 + c * 11) % 2001 - 1000); }
 
 int main(int argc, char** argv) {
-  CHECK(argc == 2);
+  CHECK(argc == 2 || argc == 5);
+  const int nodes = argc == 5 ? std::atoi(argv[2]) : xg::kBlockNodes;
+  const int cap = argc == 5 ? std::atoi(argv[3]) : xg::kBlockCap;
+  const int slots = argc == 5 ? std::atoi(argv[4]) : xg::kChunkSlots;
   FILE* f = std::fopen(argv[1], "rb");
   CHECK(f);
   int64_t hdr[3];
@@ -37,8 +42,8 @@ int main(int argc, char** argv) {
   };
   xg::Layout L;
   std::string err;
-  CHECK(xg::build_layout(npoin, p, q, d, ntri, tri.data(), besym.data(),
-                         xg::kBlockNodes, xg::kBlockCap, L, err) == 0);
+  CHECK(xg::build_layout(npoin, p, q, d, ntri, tri.data(), besym.data(), nodes, cap,
+                         slots, L, err) == 0);
   // direct scatter
   std::vector<double> Rref((size_t)npoin * 5, 0.0), R((size_t)npoin * 5, -1e300);
   for (int64_t e = 0; e < ne; ++e)
@@ -57,8 +62,8 @@ int main(int argc, char** argv) {
   long maxloc = 0, halo = 0;
   for (size_t b = 0; b < L.blk.size(); ++b) {
     const auto& B = L.blk[b];
-    CHECK(B.nloc <= xg::kBlockCap && B.nown <= B.nloc && B.nown > 0);
-    CHECK(B.nloc <= xg::kChunkSlots);          // one local node per thread
+    CHECK(B.nloc <= cap && B.nown <= B.nloc && B.nown > 0);
+    CHECK(B.nloc <= slots);                    // one local node per thread
     CHECK(B.own0 >= 0 && B.own0 + B.nown <= npoin);
     CHECK(B.slot0 + (B.nloc - B.nown) <= L.nslot);
     CHECK(B.halo_off + (B.nloc - B.nown) <= (int)L.halo.size());
@@ -70,10 +75,10 @@ int main(int argc, char** argv) {
       glob[i] = i < B.nown ? B.own0 + i : L.halo[B.halo_off + i - B.nown];
       CHECK(glob[i] >= 0 && glob[i] < npoin);
     }
-    std::vector<double> acc((size_t)B.nloc * 5, 0.0), F((size_t)xg::kChunkSlots * 5);
+    std::vector<double> acc((size_t)B.nloc * 5, 0.0), F((size_t)slots * 5);
     for (int ch = B.chunk0; ch < B.chunk1; ++ch) {
       const auto& C = L.chunks[ch];
-      CHECK(C.count > 0 && C.count <= (C.kind ? xg::kChunkSlots / 3 : xg::kChunkSlots));
+      CHECK(C.count > 0 && C.count <= (C.kind ? slots / 3 : slots));
       for (int t = 0; t < C.count; ++t) {
         if (C.kind == 0) {
           int e = C.first + t;

@@ -24,7 +24,18 @@ def _exe():
     return EXE
 
 
-def _check(tmp_path, m, p, q):
+def _rhs_blocks():
+    """(owned nodes, capacity, chunk slots) of the rhs blocks, from the header."""
+    import re
+    h = open(os.path.join(ROOT, "xyst_amd", "csrc", "xg_layout.hpp")).read()
+    return tuple(re.search(r"constexpr int %s = (\d+);" % k, h).group(1)
+                 for k in ("kRhsNodes", "kRhsCap", "kRhsSlots"))
+
+
+BLOCKS = {"grad": (), "rhs": _rhs_blocks()}
+
+
+def _check(tmp_path, m, p, q, blocks=()):
     _, _, besym = X.mesh_symbc(m["coord"], m["triinpoel"], m["faceset"], [1, 2, 3, 4, 5, 6])
     f = tmp_path / "layout.bin"
     with open(f, "wb") as fh:
@@ -33,25 +44,28 @@ def _check(tmp_path, m, p, q):
         q.astype(np.int32).tofile(fh)
         m["triinpoel"].astype(np.int32).tofile(fh)
         besym.astype(np.uint8).tofile(fh)
-    r = subprocess.run([_exe(), str(f)], capture_output=True, text=True)
+    r = subprocess.run([_exe(), str(f), *blocks], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     return dict(kv.split("=") for kv in r.stdout.split()[1:])
 
 
+@pytest.mark.parametrize("blocks", ["grad", "rhs"])
 @pytest.mark.parametrize("order", ["lex", "rcm", "rcm+block"])
-def test_layout_matches_direct_scatter(tmp_path, order):
+def test_layout_matches_direct_scatter(tmp_path, order, blocks):
     m = box_mesh(24, seed=2)
     if order != "lex":
         m = renumber_mesh(m, X.mesh_rcm(m["coord"].shape[1], m["inpoel"]))
     if order == "rcm+block":
         m = renumber_mesh(m, X.mesh_block_order(m["coord"], 2048))
     e, _ = X.mesh_domint(m["coord"], m["inpoel"])
-    st = _check(tmp_path, m, e[:, 0].astype(np.int64), e[:, 1].astype(np.int64))
+    st = _check(tmp_path, m, e[:, 0].astype(np.int64), e[:, 1].astype(np.int64),
+                BLOCKS[blocks])
     if order == "rcm+block":
-        assert float(st["halo/owned"]) < 0.5
+        assert float(st["halo/owned"]) < (0.5 if blocks == "grad" else 1.1)
 
 
-def test_layout_superedge_orientation(tmp_path):
+@pytest.mark.parametrize("blocks", ["grad", "rhs"])
+def test_layout_superedge_orientation(tmp_path, blocks):
     """Edges in the mixed orientations of the reference's superedges."""
     m = box_mesh(10, seed=4)
     edge, dint = O.domint(m["coord"], m["inpoel"])
@@ -61,7 +75,7 @@ def test_layout_superedge_orientation(tmp_path):
     lt = [(0, 1), (1, 2), (2, 0)]
     P = np.concatenate([s0[:, a] for a, b in lp] + [s1[:, a] for a, b in lt] + [s2[:, 0]])
     Q = np.concatenate([s0[:, b] for a, b in lp] + [s1[:, b] for a, b in lt] + [s2[:, 1]])
-    _check(tmp_path, m, P, Q)
+    _check(tmp_path, m, P, Q, BLOCKS[blocks])
 
 
 def test_block_order_is_permutation_of_compact_cells():

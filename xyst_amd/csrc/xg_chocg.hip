@@ -782,7 +782,7 @@ int xg_chocg_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
   std::vector<uint8_t> nosym(ntri * 3 + 1, 0);
   xg::Layout L;
   int rc = xg::build_layout((int)npoin, p, q, d, ntri, tri32.data(), nosym.data(),
-                            xg::kBlockNodes, xg::kBlockCap, L, c->err);
+                            xg::kBlockNodes, xg::kBlockCap, xg::kChunkSlots, L, c->err);
   if (rc) return rc;
   std::vector<double> dwl(L.eidx.size());
   for (size_t e = 0; e < L.eidx.size(); ++e) dwl[e] = w[L.eidx[e]];

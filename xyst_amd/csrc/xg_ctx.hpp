@@ -29,11 +29,22 @@ struct xg_ctx {
   // mesh
   bool have_mesh = false;
   int npoin = 0;
+  // riemann::rhs's own edge blocks (xg_layout.hpp: kRhsCap, kRhsSlots)
+  struct EdgeBlocks {
+    int nblk = 0, nfx = 0;
+    int4 *blk0 = nullptr, *blk1 = nullptr, *chk = nullptr;
+    int *halo = nullptr, *chk_ent = nullptr;
+    int *fx_node = nullptr, *fx_off = nullptr, *fx_slot = nullptr;
+    uint16_t *tend = nullptr, *entries = nullptr;
+    uint32_t *elpq = nullptr, *fab = nullptr, *fcs = nullptr;
+    double *dx = nullptr, *dy = nullptr, *dz = nullptr;
+    double* slots = nullptr;             // halo partials (5 per slot)
+  } rb;
   double *x = nullptr, *y = nullptr, *z = nullptr;
   int64_t nedge = 0;
   uint32_t* elpq = nullptr;
   double *dx = nullptr, *dy = nullptr, *dz = nullptr;
-  // rhs blocks
+  // grad blocks
   int nblk = 0, nchunk = 0;
   int4 *blk0 = nullptr, *blk1 = nullptr;     // (own0,nown,nloc,halo_off), (slot0,chunk0,chunk1,-)
   int *halo = nullptr;
@@ -42,7 +53,6 @@ struct xg_ctx {
   uint16_t *tend = nullptr, *entries = nullptr;
   uint32_t *fab = nullptr, *fcs = nullptr;
   int nslot = 0;
-  double* slots = nullptr;             // rhs halo partials (5 per slot)
   double* gslots = nullptr;            // grad halo partials (15 per slot)
   int nfx = 0;
   int *fx_node = nullptr, *fx_off = nullptr, *fx_slot = nullptr;

@@ -4,7 +4,7 @@
 // edges and boundary faces whose smallest node id lies in it, and all nodes
 // those touch that lie beyond the block are its halo.  One workgroup per
 // block accumulates the block's nodal sums in LDS: the block's work items
-// (edges, or boundary faces) are cut into chunks of at most kChunkSlots
+// (edges, or boundary faces) are cut into chunks of at most chunk_slots
 // flux slots; a chunk's fluxes go to LDS, then every node the chunk touches
 // sums its slots in a fixed order.  Owned nodes are written directly; halo
 // partial sums go to slots that a fix-up pass adds to their owners in block
@@ -15,8 +15,10 @@
 // coordinates), so a chunk's flux evaluations read LDS only; a thread owns one
 // local node and keeps its running sum in registers.
 //
-// grad uses the same blocks: each node thread gathers the D*(u_p+u_q) terms
-// of the chunk edges that touch it.
+// grad and the chorin operators use blocks of up to kBlockCap nodes with
+// 1024-slot chunks (each node thread gathers the terms of the chunk edges
+// that touch it); riemann::rhs builds its own layout of kRhsCap-node blocks
+// and 512-slot chunks, two workgroups per CU.
 //
 // The layout is fastest when node ids are block-contiguous and spatially
 // compact (xg_mesh_block_order), but is correct for any numbering.
@@ -82,9 +84,13 @@ static void upper_lists(int npoin, const std::vector<int>& p,
 int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q,
                  const std::vector<double>& d, int64_t ntri, const int* tri,
                  const uint8_t* besym, int block_nodes, int block_cap,
-                 Layout& L, std::string& err) {
+                 int chunk_slots, Layout& L, std::string& err) {
   const int64_t ne = (int64_t)p.size();
   L = Layout();
+  if (chunk_slots < 3 || chunk_slots > kChunkSlots) {
+    err = "layout: chunk size out of range";
+    return XG_EINVAL;
+  }
   L.npoin = npoin;
   // upper-neighbour lists (CSR by the smaller end) for halo tracking
   std::vector<int64_t> uoff;
@@ -167,8 +173,8 @@ int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q
       }
       L.chunks.push_back(C);
     };
-    for (size_t i0 = 0; i0 < E.size(); i0 += kChunkSlots) {
-      size_t i1 = std::min(E.size(), i0 + (size_t)kChunkSlots);
+    for (size_t i0 = 0; i0 < E.size(); i0 += (size_t)chunk_slots) {
+      size_t i1 = std::min(E.size(), i0 + (size_t)chunk_slots);
       std::vector<std::pair<int, uint16_t>> ent;
       for (size_t i = i0; i < i1; ++i) {
         int64_t e = E[i];
@@ -189,7 +195,7 @@ int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q
       }
     }
     auto& F = bfaces[b];
-    const int fmax = kChunkSlots / 3;
+    const int fmax = chunk_slots / 3;
     for (size_t i0 = 0; i0 < F.size(); i0 += fmax) {
       size_t i1 = std::min(F.size(), i0 + (size_t)fmax);
       std::vector<std::pair<int, uint16_t>> ent;

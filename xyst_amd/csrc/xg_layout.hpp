@@ -17,6 +17,12 @@ constexpr int kChunkSlots = 1024;
 constexpr int kBlockCap = 616;
 // owned nodes per block (a block also stops growing at kBlockCap)
 constexpr int kBlockNodes = 432;
+// riemann::rhs has blocks of its own, small enough that two 512-thread
+// workgroups share a CU: 23 doubles per local node, 512 flux slots of 5
+// doubles and the entry lists take 80,560 of the 81,920 bytes of LDS each
+constexpr int kRhsSlots = 512;
+constexpr int kRhsCap = 312;
+constexpr int kRhsNodes = 312;
 
 struct Block {
   int own0 = 0, nown = 0, nloc = 0;  // owned ids [own0, own0+nown); local ids
@@ -76,10 +82,12 @@ int build_tet_layout(int npoin, int64_t ntet, const size_t* inpoel,
                      int block_nodes, int block_cap, TetLayout& L,
                      std::string& err);
 
+// chunk_slots: flux slots per edge chunk (<= kChunkSlots; boundary-face
+// chunks hold chunk_slots / 3 faces)
 int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q,
                  const std::vector<double>& d, int64_t ntri, const int* tri,
                  const uint8_t* besym, int block_nodes, int block_cap,
-                 Layout& L, std::string& err);
+                 int chunk_slots, Layout& L, std::string& err);
 
 int block_order(int64_t npoin, const double* x, const double* y,
                 const double* z, int leaf, size_t* perm);

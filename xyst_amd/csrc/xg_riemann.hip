@@ -30,7 +30,8 @@ using xg::Cfg;
 namespace {
 
 using namespace xgd;
-constexpr int kRhsThreads = xg::kChunkSlots;
+constexpr int kGradThreads = xg::kChunkSlots;
+constexpr int kRhsThreads = xg::kRhsSlots;
 
 // ----------------------------------------------------------------- kernels --
 
@@ -38,6 +39,8 @@ constexpr int kRhsThreads = xg::kChunkSlots;
 // (slot | sign bit, grouped by local node) and each local node's entry end
 // are prefetched into registers one chunk ahead with coalesced loads, then
 // placed in LDS, so the per-node accumulation loop reads LDS only.
+// (NT threads, chunks of at most NT edges: up to 2 NT entries)
+template <int NT>
 struct ChunkPrefetch {
   uint16_t e0 = 0, e1 = 0, te = 0;
   __device__ inline void load(const int4& C, int ent_off, int nloc, int t,
@@ -45,14 +48,14 @@ struct ChunkPrefetch {
                               const uint16_t* __restrict__ entries) {
     int nent = C.z * (C.x == 0 ? 2 : 3);
     if (t < nent) e0 = entries[ent_off + t];
-    if (t + kRhsThreads < nent) e1 = entries[ent_off + kRhsThreads + t];
+    if (t + NT < nent) e1 = entries[ent_off + NT + t];
     if (t < nloc) te = tend[C.w + t];
   }
   __device__ inline void store(const int4& C, int nloc, int t, uint16_t* sE,
                                uint16_t* sT) const {
     int nent = C.z * (C.x == 0 ? 2 : 3);
     if (t < nent) sE[t] = e0;
-    if (t + kRhsThreads < nent) sE[kRhsThreads + t] = e1;
+    if (t + NT < nent) sE[NT + t] = e1;
     if (t < nloc) sT[t] = te;
   }
 };
@@ -64,7 +67,7 @@ struct ChunkPrefetch {
 // DIV: owned nodes without halo partials or boundary terms (gflag 0) are
 // final here and are divided by their volume (RieCG.cpp:935-939)
 template <bool DIV>
-__global__ __launch_bounds__(kRhsThreads) void k_grad_blocks(
+__global__ __launch_bounds__(kGradThreads) void k_grad_blocks(
     int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
     const int* __restrict__ halo, const int4* __restrict__ chk,
     const int* __restrict__ chk_ent, const uint16_t* __restrict__ tend,
@@ -76,7 +79,7 @@ __global__ __launch_bounds__(kRhsThreads) void k_grad_blocks(
   constexpr int CAP = xg::kBlockCap;
   __shared__ double sP[5 * CAP];
   __shared__ double F[xg::kChunkSlots * 8];
-  __shared__ uint16_t sE[2 * kRhsThreads];
+  __shared__ uint16_t sE[2 * kGradThreads];
   __shared__ uint16_t sT[CAP];
   const int t = threadIdx.x;
   const int b = xcd_block(blockIdx.x, nblk);
@@ -96,7 +99,7 @@ __global__ __launch_bounds__(kRhsThreads) void k_grad_blocks(
   // k_grad_bnd adds those terms)
   int ch = B1.y;
   while (ch < B1.z && chk[ch].x != 0) ++ch;
-  ChunkPrefetch pf;
+  ChunkPrefetch<kGradThreads> pf;
   uint32_t pq_n = 0;
   double dx_n = 0.0, dy_n = 0.0, dz_n = 0.0;
   auto fetch = [&](int c) {
@@ -259,7 +262,7 @@ __device__ inline void face_flux_lds(int la, int lb, int lc, int sym,
                                      const double* __restrict__ sP,
                                      const double* __restrict__ sX,
                                      double gm1, double out[3][5]) {
-  constexpr int CAP = xg::kBlockCap;
+  constexpr int CAP = xg::kRhsCap;   // the rhs blocks' LDS stride
   int N[3] = {la, lb, lc};
   double ba[3], ca[3];
 #pragma unroll
@@ -302,7 +305,8 @@ __device__ inline void face_flux_lds(int la, int lb, int lc, int sym,
 // one face's three corner terms) into LDS, then every node thread adds the
 // chunk's slots that touch it, in slot order.
 template <int FLUX>
-__global__ __launch_bounds__(kRhsThreads) void k_rhs_blocks(
+__global__ __launch_bounds__(kRhsThreads) __attribute__((amdgpu_waves_per_eu(4)))
+void k_rhs_blocks(
     int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
     const int* __restrict__ halo, const int4* __restrict__ chk,
     const int* __restrict__ chk_ent, const uint16_t* __restrict__ tend,
@@ -313,12 +317,12 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_blocks(
     const double* __restrict__ y, const double* __restrict__ z,
     const double* __restrict__ G, const double* __restrict__ U,
     double* __restrict__ R, double* __restrict__ S, Cfg cfg) {
-  constexpr int CAP = xg::kBlockCap;
+  constexpr int CAP = xg::kRhsCap;
   __shared__ double sN[23 * CAP];
   double* const sP = sN;             // primitive state, 5 x CAP
   double* const sG = sN + 5 * CAP;   // gradients, 15 x CAP
   double* const sX = sN + 20 * CAP;  // coordinates, 3 x CAP
-  __shared__ double F[xg::kChunkSlots * 5];
+  __shared__ double F[kRhsThreads * 5];
   __shared__ uint16_t sE[2 * kRhsThreads];
   __shared__ uint16_t sT[CAP];
   const int t = threadIdx.x;
@@ -347,7 +351,7 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_blocks(
   }
   // the next chunk's edge record and entry lists are loaded while the
   // current chunk computes
-  ChunkPrefetch pf;
+  ChunkPrefetch<kRhsThreads> pf;
   uint32_t pq_n = 0;
   double dx_n = 0.0, dy_n = 0.0, dz_n = 0.0;
   auto fetch = [&](int c, int tl) {
@@ -467,12 +471,12 @@ int check_mesh(xg_ctx* c, int ncomp) {
 // divided by the last kernel that adds to it; otherwise grad(U) undivided
 int launch_grad(xg_ctx* c, const double* U, double* G, bool div = false) {
   if (div)
-    hipLaunchKernelGGL(k_grad_blocks<true>, dim3(c->nblk), dim3(kRhsThreads), 0,
+    hipLaunchKernelGGL(k_grad_blocks<true>, dim3(c->nblk), dim3(kGradThreads), 0,
                        c->stream, c->nblk, c->blk0, c->blk1, c->halo, c->chk,
                        c->chk_ent, c->tend, c->entries, c->elpq, c->dx, c->dy,
                        c->dz, U, G, c->gslots, c->gflag, c->vol);
   else
-    hipLaunchKernelGGL(k_grad_blocks<false>, dim3(c->nblk), dim3(kRhsThreads), 0,
+    hipLaunchKernelGGL(k_grad_blocks<false>, dim3(c->nblk), dim3(kGradThreads), 0,
                        c->stream, c->nblk, c->blk0, c->blk1, c->halo, c->chk,
                        c->chk_ent, c->tend, c->entries, c->elpq, c->dx, c->dy,
                        c->dz, U, G, c->gslots, nullptr, nullptr);
@@ -506,15 +510,50 @@ int launch_rhs(xg_ctx* c, const double* G, const double* U, double* R) {
   const double g = c->cfg.gamma;
   Cfg cfg{g, c->cfg.stab2, c->cfg.stab2coef, g - 1.0, g * (g - 1.0)};
   auto kern = c->cfg.flux == 0 ? k_rhs_blocks<0> : k_rhs_blocks<1>;
-  hipLaunchKernelGGL(kern, dim3(c->nblk), dim3(kRhsThreads), 0, c->stream,
-                     c->nblk, c->blk0, c->blk1, c->halo, c->chk, c->chk_ent,
-                     c->tend, c->entries, c->elpq, c->dx, c->dy, c->dz, c->fab,
-                     c->fcs, c->x, c->y, c->z, G, U, R, c->slots, cfg);
-  if (c->nfx)
-    hipLaunchKernelGGL(k_fixup<5>, dim3(nblocks(c->nfx)), dim3(kBlock), 0,
-                       c->stream, c->nfx, c->fx_node, c->fx_off, c->fx_slot,
-                       c->slots, R);
+  const auto& B = c->rb;
+  if (B.nblk)
+    hipLaunchKernelGGL(kern, dim3(B.nblk), dim3(kRhsThreads), 0, c->stream,
+                       B.nblk, B.blk0, B.blk1, B.halo, B.chk, B.chk_ent, B.tend,
+                       B.entries, B.elpq, B.dx, B.dy, B.dz, B.fab, B.fcs, c->x, c->y,
+                       c->z, G, U, R, B.slots, cfg);
+  if (B.nfx)
+    hipLaunchKernelGGL(k_fixup<5>, dim3(nblocks(B.nfx)), dim3(kBlock), 0,
+                       c->stream, B.nfx, B.fx_node, B.fx_off, B.fx_slot, B.slots, R);
   XG_HIP(c, hipGetLastError());
+  return XG_OK;
+}
+
+// uploads the riemann::rhs layout (its block tables, edge records, chunk
+// entry lists, boundary faces, halo slots and fix-up lists)
+int upload_blocks(xg_ctx* c, std::vector<DevArr>& M, const xg::Layout& L,
+                  xg_ctx::EdgeBlocks& B) {
+  std::vector<int4> b0(L.blk.size()), b1(L.blk.size());
+  for (size_t b = 0; b < L.blk.size(); ++b) {
+    const auto& K = L.blk[b];
+    b0[b] = make_int4(K.own0, K.nown, K.nloc, K.halo_off);
+    b1[b] = make_int4(K.slot0, K.chunk0, K.chunk1, 0);
+  }
+  std::vector<int4> c0(L.chunks.size());
+  std::vector<int> c1(L.chunks.size());
+  for (size_t i = 0; i < L.chunks.size(); ++i) {
+    const auto& C = L.chunks[i];
+    c0[i] = make_int4(C.kind, C.first, C.count, C.tend_off);
+    c1[i] = C.ent_off;
+  }
+  int rc;
+  if ((rc = upload(c, M, &B.blk0, b0)) || (rc = upload(c, M, &B.blk1, b1)) ||
+      (rc = upload(c, M, &B.halo, L.halo)) || (rc = upload(c, M, &B.chk, c0)) ||
+      (rc = upload(c, M, &B.chk_ent, c1)) || (rc = upload(c, M, &B.tend, L.tend)) ||
+      (rc = upload(c, M, &B.entries, L.entries)) || (rc = upload(c, M, &B.elpq, L.elpq)) ||
+      (rc = upload(c, M, &B.dx, L.dx)) || (rc = upload(c, M, &B.dy, L.dy)) ||
+      (rc = upload(c, M, &B.dz, L.dz)) || (rc = upload(c, M, &B.fab, L.fab)) ||
+      (rc = upload(c, M, &B.fcs, L.fcs)) ||
+      (rc = dev_alloc(c, M, (void**)&B.slots, (size_t)L.nslot * 5 * sizeof(double))) ||
+      (rc = upload(c, M, &B.fx_node, L.fx_node)) || (rc = upload(c, M, &B.fx_off, L.fx_off)) ||
+      (rc = upload(c, M, &B.fx_slot, L.fx_slot)))
+    return rc;
+  B.nblk = (int)L.blk.size();
+  B.nfx = (int)L.fx_node.size();
   return XG_OK;
 }
 
@@ -628,9 +667,12 @@ int xg_mesh_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
       return fail(c, XG_EINVAL, "xg_mesh_upload: boundary face node out of range");
   std::vector<int> tri32(ntri * 3);
   for (int64_t f = 0; f < ntri * 3; ++f) tri32[f] = (int)tri[f];
-  xg::Layout L;
+  xg::Layout L, LR;
   int rc = xg::build_layout((int)npoin, p, q, d, ntri, tri32.data(), besym,
-                            xg::kBlockNodes, xg::kBlockCap, L, c->err);
+                            xg::kBlockNodes, xg::kBlockCap, xg::kChunkSlots, L, c->err);
+  if (rc) return rc;
+  rc = xg::build_layout((int)npoin, p, q, d, ntri, tri32.data(), besym, xg::kRhsNodes,
+                        xg::kRhsCap, xg::kRhsSlots, LR, c->err);
   if (rc) return rc;
   std::vector<int4> b0(L.blk.size()), b1(L.blk.size());
   for (size_t b = 0; b < L.blk.size(); ++b) {
@@ -678,7 +720,6 @@ int xg_mesh_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
       (rc = upload(c, M, &c->tend, L.tend)) ||
       (rc = upload(c, M, &c->entries, L.entries)) ||
       (rc = upload(c, M, &c->fab, L.fab)) || (rc = upload(c, M, &c->fcs, L.fcs)) ||
-      (rc = dev_alloc(c, M, (void**)&c->slots, (size_t)L.nslot * 5 * sizeof(double))) ||
       (rc = upload(c, M, &c->fx_node, L.fx_node)) ||
       (rc = upload(c, M, &c->fx_off, L.fx_off)) ||
       (rc = upload(c, M, &c->fx_slot, L.fx_slot)) ||
@@ -686,6 +727,7 @@ int xg_mesh_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
       (rc = upload(c, M, &c->tri, tri32)) || (rc = upload(c, M, &c->bnode, bnode)) ||
       (rc = upload(c, M, &c->boff, boff)) || (rc = upload(c, M, &c->bface, bface)))
     return rc;
+  if ((rc = upload_blocks(c, M, LR, c->rb))) return rc;
   c->npoin = (int)npoin;
   c->nedge = ne;
   c->nblk = (int)L.blk.size();
