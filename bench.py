@@ -37,14 +37,14 @@ def rhs_bytes(E, N, Fb):
     return 32 * E + 224 * N + 15 * Fb
 
 
-def setup(n, seed=20261202, order="rcm+block"):
+def setup(n, seed=20261202, order="rcm+block", cell=1024):
     from xyst_amd import riemann as X
     from xyst_amd.meshgen import box_mesh, renumber_mesh, sedov_state
     t0 = time.time()
     m = box_mesh(n, seed=seed)
     m = renumber_mesh(m, X.mesh_rcm(m["coord"].shape[1], m["inpoel"]))
     if order == "rcm+block":
-        m = renumber_mesh(m, X.mesh_block_order(m["coord"], 2048))
+        m = renumber_mesh(m, X.mesh_block_order(m["coord"], cell))
     log("mesh %d^3 + %s: %.1fs" % (n, order, time.time() - t0))
     vol = X.mesh_vol(m["coord"], m["inpoel"])
     edge, dint = X.mesh_domint(m["coord"], m["inpoel"])
@@ -365,6 +365,8 @@ def main():
     ap.add_argument("--n", type=int, default=160)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--order", default="rcm+block", choices=["rcm", "rcm+block"])
+    ap.add_argument("--block-cell", type=int, default=1024,
+                    help="nodes per cell of the RCB block order (rcm+block)")
     ap.add_argument("--poisson-n", type=int, default=128,
                     help="box size of the pressure-Poisson solve (0: skip)")
     ap.add_argument("--kozcg-steps", type=int, default=20,
@@ -387,7 +389,7 @@ def main():
     import torch
     from xyst_amd import riemann as X
 
-    m, vol, edge, dint, sn, sv, besym, U = setup(args.n, order=args.order)
+    m, vol, edge, dint, sn, sv, besym, U = setup(args.n, order=args.order, cell=args.block_cell)
     N, E, Fb = m["coord"].shape[1], edge.shape[0], m["triinpoel"].shape[0]
     stream = torch.cuda.current_stream()
     ctx = X.Context(0, flux="rusanov", stream=stream)
@@ -447,7 +449,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": "riecg_rhs_sedov_box%d" % args.n, "n": args.n,
                    "nodes": N, "edges": E, "tets": int(m["inpoel"].shape[0]),
-                   "boundary_faces": Fb, "flux": "rusanov", "ordering": args.order,
+                   "boundary_faces": Fb, "flux": "rusanov",
+                   "ordering": args.order + ("" if args.order == "rcm" else
+                                             " (cells of %d nodes)" % args.block_cell),
                    "state": "sedov, rho=1, p=1e-6, E=1 within r<0.02"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
