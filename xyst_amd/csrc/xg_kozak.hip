@@ -2,9 +2,10 @@
 // Taylor-Galerkin right-hand side of one GPU-resident partition.
 //
 // Layout (xg_layout.cpp, build_tet_layout): node blocks of consecutive ids;
-// a block owns the tets whose smallest node it holds.  One 1024-thread
-// workgroup per block stages the conserved state and coordinates of the
-// block's nodes (owned + halo) in LDS.  Per chunk of 256 tets, thread t
+// a block owns the tets whose smallest node it holds.  One 512-thread
+// workgroup per block (two share a CU) stages the conserved state and
+// coordinates of the block's nodes (owned + halo) in LDS.  Per chunk of 128
+// tets, thread t
 // handles corner a = t % 4 of tet t / 4: it evaluates its node's share of the
 // half-step element state, the four corners' shares are combined with two
 // lane exchanges (every corner sees the same sum), and the thread writes its
@@ -58,7 +59,7 @@ struct KozakMesh {
 
 namespace {
 
-constexpr int kKozThreads = xg::kChunkSlots;
+constexpr int kKozThreads = xg::kKozSlots;
 static_assert(xg::kKozCap <= kKozThreads, "one local node per thread");
 
 __device__ inline void cross(const double a[3], const double b[3], double c[3]) {
@@ -97,7 +98,7 @@ __device__ inline void tet_aec(const double X[4][3], const double* sU, int cap,
 // AEC: also the antidiffusive contributions P+/- (KozCG::aec), the output
 // record of a node being then R (5), P+ (5), P- (5)
 template <bool AEC>
-__global__ __launch_bounds__(kKozThreads) void k_kozak_blocks(
+__global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_kozak_blocks(
     int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
     const int* __restrict__ halo, const int4* __restrict__ chk,
     const int* __restrict__ chk_ent, const uint16_t* __restrict__ tend,
@@ -291,7 +292,7 @@ __device__ inline int local_to_global(int t, int own0, int nown,
 // to slots that k_fixup_minmax merges.  Owners also store the low-order
 // solution UL.
 template <bool CLIP>
-__global__ __launch_bounds__(kKozThreads) void k_kozcg_alw(
+__global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_kozcg_alw(
     int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
     const int* __restrict__ halo, const int4* __restrict__ chk,
     const uint32_t* __restrict__ tab, const uint32_t* __restrict__ tcd,
@@ -404,7 +405,7 @@ __global__ __launch_bounds__(kBlock) void k_kozcg_coef(
 // coefficient of each component is the min over the corners of C+ or C-
 // (by the sign of the corner's contribution), the fctsys components share
 // their min; the corner adds coef * aec.  Same block/slot scheme as the rhs.
-__global__ __launch_bounds__(kKozThreads) void k_kozcg_lim(
+__global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_kozcg_lim(
     int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
     const int* __restrict__ halo, const int4* __restrict__ chk,
     const int* __restrict__ chk_ent, const uint16_t* __restrict__ tend,

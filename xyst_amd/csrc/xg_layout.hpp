@@ -60,11 +60,14 @@ struct Layout {
 };
 
 // kozak::rhs blocks: a block stages U (5) and coordinates (3) of up to
-// kKozCap nodes in LDS beside one chunk's corner fluxes; a chunk holds
-// kChunkSlots / 4 tets, one thread per tet corner
-constexpr int kKozCap = 1024;
-constexpr int kKozNodes = 768;
-constexpr int kTetsPerChunk = kChunkSlots / 4;
+// kKozCap nodes in LDS beside one chunk's corner fluxes and antidiffusive
+// contributions (10 doubles per slot); a chunk holds kKozSlots / 4 tets, one
+// thread per tet corner.  At 512 slots and 512 nodes the largest of the
+// KozCG block kernels takes 75,776 bytes of LDS, so two share a CU.
+constexpr int kKozSlots = 512;
+constexpr int kKozCap = 512;
+constexpr int kKozNodes = 384;
+constexpr int kTetsPerChunk = kKozSlots / 4;
 
 struct TetLayout {
   int npoin = 0;
