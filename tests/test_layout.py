@@ -18,21 +18,23 @@ EXE = os.path.join(ROOT, "build", "layout_check")
 def _exe():
     src = [os.path.join(ROOT, "tests", "cpp", "layout_check.cpp"),
            os.path.join(ROOT, "xyst_amd", "csrc", "xg_layout.cpp")]
-    if not os.path.exists(EXE) or any(os.path.getmtime(s) > os.path.getmtime(EXE) for s in src):
+    hdr = os.path.join(ROOT, "xyst_amd", "csrc", "xg_layout.hpp")
+    if not os.path.exists(EXE) or any(os.path.getmtime(s) > os.path.getmtime(EXE)
+                                      for s in src + [hdr]):
         os.makedirs(os.path.dirname(EXE), exist_ok=True)
         subprocess.run(["g++", "-O2", "-std=c++17", "-o", EXE] + src, check=True)
     return EXE
 
 
-def _rhs_blocks():
-    """(owned nodes, capacity, chunk slots) of the rhs blocks, from the header."""
+def _blocks(kind):
+    """(owned nodes, capacity, chunk slots) of a block layout, from the header."""
     import re
     h = open(os.path.join(ROOT, "xyst_amd", "csrc", "xg_layout.hpp")).read()
     return tuple(re.search(r"constexpr int %s = (\d+);" % k, h).group(1)
-                 for k in ("kRhsNodes", "kRhsCap", "kRhsSlots"))
+                 for k in ("k%sNodes" % kind, "k%sCap" % kind, "k%sSlots" % kind))
 
 
-BLOCKS = {"grad": (), "rhs": _rhs_blocks()}
+BLOCKS = {"cho": (), "grad": _blocks("Grad"), "rhs": _blocks("Rhs")}
 
 
 def _check(tmp_path, m, p, q, blocks=()):
@@ -49,7 +51,7 @@ def _check(tmp_path, m, p, q, blocks=()):
     return dict(kv.split("=") for kv in r.stdout.split()[1:])
 
 
-@pytest.mark.parametrize("blocks", ["grad", "rhs"])
+@pytest.mark.parametrize("blocks", ["cho", "grad", "rhs"])
 @pytest.mark.parametrize("order", ["lex", "rcm", "rcm+block"])
 def test_layout_matches_direct_scatter(tmp_path, order, blocks):
     m = box_mesh(24, seed=2)
@@ -61,10 +63,10 @@ def test_layout_matches_direct_scatter(tmp_path, order, blocks):
     st = _check(tmp_path, m, e[:, 0].astype(np.int64), e[:, 1].astype(np.int64),
                 BLOCKS[blocks])
     if order == "rcm+block":
-        assert float(st["halo/owned"]) < (0.5 if blocks == "grad" else 1.1)
+        assert float(st["halo/owned"]) < {"cho": 0.5, "grad": 0.6, "rhs": 1.1}[blocks]
 
 
-@pytest.mark.parametrize("blocks", ["grad", "rhs"])
+@pytest.mark.parametrize("blocks", ["cho", "grad", "rhs"])
 def test_layout_superedge_orientation(tmp_path, blocks):
     """Edges in the mixed orientations of the reference's superedges."""
     m = box_mesh(10, seed=4)

@@ -8,15 +8,21 @@
 
 namespace xg {
 
-// flux slots per chunk = threads per rhs workgroup
+// ChoCG edge blocks: slots per chunk = threads per workgroup (also the
+// largest chunk any layout may use: slot ids are 10-bit)
 constexpr int kChunkSlots = 1024;
-// nodes (owned + halo) a block may touch: their primitive state, gradients
-// and coordinates (23 doubles each) are staged in LDS, beside one chunk of
-// fluxes (kChunkSlots x 5 doubles) and its entry lists (2 x kChunkSlots +
-// kBlockCap 16-bit words): 158.3 KiB of the 160 KiB per CU
+// nodes (owned + halo) a ChoCG block may touch; its operators stage their
+// node fields for these in LDS beside one chunk of edge terms
 constexpr int kBlockCap = 616;
 // owned nodes per block (a block also stops growing at kBlockCap)
 constexpr int kBlockNodes = 432;
+// riemann::grad blocks: 5 primitive doubles per local node beside 512 slots
+// of 8 doubles and the entry lists, 56,320 bytes of LDS, so two 512-thread
+// workgroups share a CU; their halo share stays near that of the larger
+// ChoCG blocks
+constexpr int kGradSlots = 512;
+constexpr int kGradCap = 512;
+constexpr int kGradNodes = 384;
 // riemann::rhs has blocks of its own, small enough that two 512-thread
 // workgroups share a CU: 23 doubles per local node, 512 flux slots of 5
 // doubles and the entry lists take 80,560 of the 81,920 bytes of LDS each

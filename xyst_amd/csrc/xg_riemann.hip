@@ -3,12 +3,13 @@
 // minimum.  All FP64.  Deterministic: no atomics, every nodal sum is taken in
 // an order fixed by the layout (xg_layout.cpp).
 //
-//   rhs:  k_rhs_blocks -- one 1024-thread workgroup per node block; the
-//         block's node data staged in LDS, one node sum per thread;
+//   rhs:  k_rhs_blocks -- one 512-thread workgroup per node block (two per
+//         CU); the block's node data staged in LDS, one node sum per thread;
 //         k_fixup<5> adds halo partials.
-//   grad: k_grad_blocks -- the same blocks, each node thread gathering its
-//         edges' terms from LDS-staged primitive states; k_fixup<15> adds
-//         halo partials; k_grad_bnd adds boundary-face terms.
+//   grad: k_grad_blocks -- blocks of its own, also 512 threads and two
+//         workgroups per CU, each node thread gathering its edges' terms
+//         from LDS-staged primitive states; k_fixup<15> adds halo partials;
+//         k_grad_bnd adds boundary-face terms.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,7 +31,7 @@ using xg::Cfg;
 namespace {
 
 using namespace xgd;
-constexpr int kGradThreads = xg::kChunkSlots;
+constexpr int kGradThreads = xg::kGradSlots;
 constexpr int kRhsThreads = xg::kRhsSlots;
 
 // ----------------------------------------------------------------- kernels --
@@ -67,7 +68,7 @@ struct ChunkPrefetch {
 // DIV: owned nodes without halo partials or boundary terms (gflag 0) are
 // final here and are divided by their volume (RieCG.cpp:935-939)
 template <bool DIV>
-__global__ __launch_bounds__(kGradThreads) void k_grad_blocks(
+__global__ __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_grad_blocks(
     int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
     const int* __restrict__ halo, const int4* __restrict__ chk,
     const int* __restrict__ chk_ent, const uint16_t* __restrict__ tend,
@@ -76,9 +77,10 @@ __global__ __launch_bounds__(kGradThreads) void k_grad_blocks(
     const double* __restrict__ dz, const double* __restrict__ U,
     double* __restrict__ G, double* __restrict__ S,
     const uint8_t* __restrict__ gflag, const double* __restrict__ vol) {
-  constexpr int CAP = xg::kBlockCap;
+  constexpr int CAP = xg::kGradCap;
+  static_assert(CAP <= kGradThreads, "one local node per thread");
   __shared__ double sP[5 * CAP];
-  __shared__ double F[xg::kChunkSlots * 8];
+  __shared__ double F[kGradThreads * 8];
   __shared__ uint16_t sE[2 * kGradThreads];
   __shared__ uint16_t sT[CAP];
   const int t = threadIdx.x;
@@ -669,7 +671,7 @@ int xg_mesh_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
   for (int64_t f = 0; f < ntri * 3; ++f) tri32[f] = (int)tri[f];
   xg::Layout L, LR;
   int rc = xg::build_layout((int)npoin, p, q, d, ntri, tri32.data(), besym,
-                            xg::kBlockNodes, xg::kBlockCap, xg::kChunkSlots, L, c->err);
+                            xg::kGradNodes, xg::kGradCap, xg::kGradSlots, L, c->err);
   if (rc) return rc;
   rc = xg::build_layout((int)npoin, p, q, d, ntri, tri32.data(), besym, xg::kRhsNodes,
                         xg::kRhsCap, xg::kRhsSlots, LR, c->err);
