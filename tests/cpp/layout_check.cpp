@@ -16,10 +16,11 @@
 static double fval(long i, int c) { return (double)((i * 37 + c * 11) % 2001 - 1000); }
 
 int main(int argc, char** argv) {
-  CHECK(argc == 2 || argc == 5);
-  const int nodes = argc == 5 ? std::atoi(argv[2]) : xg::kBlockNodes;
-  const int cap = argc == 5 ? std::atoi(argv[3]) : xg::kBlockCap;
-  const int slots = argc == 5 ? std::atoi(argv[4]) : xg::kChunkSlots;
+  CHECK(argc == 2 || argc == 5 || argc == 6);
+  const bool gather = argc == 6 && std::atoi(argv[5]) != 0;
+  const int nodes = argc >= 5 ? std::atoi(argv[2]) : xg::kBlockNodes;
+  const int cap = argc >= 5 ? std::atoi(argv[3]) : xg::kBlockCap;
+  const int slots = argc >= 5 ? std::atoi(argv[4]) : xg::kChunkSlots;
   FILE* f = std::fopen(argv[1], "rb");
   CHECK(f);
   int64_t hdr[3];
@@ -43,7 +44,7 @@ int main(int argc, char** argv) {
   xg::Layout L;
   std::string err;
   CHECK(xg::build_layout(npoin, p, q, d, ntri, tri.data(), besym.data(), nodes, cap,
-                         slots, L, err) == 0);
+                         slots, L, err, gather) == 0);
   // direct scatter
   std::vector<double> Rref((size_t)npoin * 5, 0.0), R((size_t)npoin * 5, -1e300);
   for (int64_t e = 0; e < ne; ++e)
@@ -51,13 +52,17 @@ int main(int argc, char** argv) {
       Rref[p[e] * 5 + c] -= eflux(p[e], q[e], c);
       Rref[q[e] * 5 + c] += eflux(p[e], q[e], c);
     }
-  for (int64_t t = 0; t < ntri; ++t)
+  for (int64_t t = 0; t < (gather ? 0 : ntri); ++t)   // gather layouts carry no faces
     for (int a = 0; a < 3; ++a)
       for (int c = 0; c < 5; ++c)
         Rref[tri[t * 3 + a] * 5 + c] += fflux(tri[t * 3], tri[t * 3 + 1], tri[t * 3 + 2], a, c);
   // emulate k_rhs_blocks: local node data, per-thread node accumulators
   std::vector<double> S((size_t)L.nslot * 5, -1e300);
-  CHECK((int64_t)L.elpq.size() == ne && (int64_t)L.fab.size() == ntri);
+  if (gather)   // edges between blocks twice, no faces, no halo partials
+    CHECK((int64_t)L.elpq.size() >= ne && L.fab.empty() && L.nslot == 0 &&
+          L.fx_node.empty());
+  else
+    CHECK((int64_t)L.elpq.size() == ne && (int64_t)L.fab.size() == ntri);
   std::vector<int> owned(npoin, 0);
   long maxloc = 0, halo = 0;
   for (size_t b = 0; b < L.blk.size(); ++b) {
@@ -65,7 +70,7 @@ int main(int argc, char** argv) {
     CHECK(B.nloc <= cap && B.nown <= B.nloc && B.nown > 0);
     CHECK(B.nloc <= slots);                    // one local node per thread
     CHECK(B.own0 >= 0 && B.own0 + B.nown <= npoin);
-    CHECK(B.slot0 + (B.nloc - B.nown) <= L.nslot);
+    if (!gather) CHECK(B.slot0 + (B.nloc - B.nown) <= L.nslot);
     CHECK(B.halo_off + (B.nloc - B.nown) <= (int)L.halo.size());
     maxloc = std::max<long>(maxloc, B.nloc);
     halo += B.nloc - B.nown;
@@ -82,7 +87,7 @@ int main(int argc, char** argv) {
       for (int t = 0; t < C.count; ++t) {
         if (C.kind == 0) {
           int e = C.first + t;
-          CHECK(e >= 0 && e < ne);
+          CHECK(e >= 0 && e < (int64_t)L.elpq.size());
           int lp = L.elpq[e] & 0xffff, lq = L.elpq[e] >> 16;
           CHECK(lp < B.nloc && lq < B.nloc);
           CHECK(glob[lp] == L.ep[e] && glob[lq] == L.eq[e]);
@@ -112,7 +117,12 @@ int main(int argc, char** argv) {
       }
     }
     for (int i = 0; i < B.nown * 5; ++i) R[(size_t)B.own0 * 5 + i] = acc[i];
-    for (int i = 0; i < (B.nloc - B.nown) * 5; ++i) S[(size_t)B.slot0 * 5 + i] = acc[B.nown * 5 + i];
+    for (int i = 0; i < (B.nloc - B.nown) * 5; ++i) {
+      if (gather)   // halo nodes receive nothing: the kernel stores no halo sums
+        CHECK(acc[B.nown * 5 + i] == 0.0);
+      else
+        S[(size_t)B.slot0 * 5 + i] = acc[B.nown * 5 + i];
+    }
   }
   for (int n = 0; n < npoin; ++n) CHECK(owned[n] == 1);
   // emulate k_rhs_fixup
@@ -162,6 +172,10 @@ int main(int argc, char** argv) {
               acc[k] += (first ? -D[k] : D[k]) * (uval(glob[o]) + uval(glob[n]));
           }
         }
+        if (gather && n >= B.nown) {
+          for (int k = 0; k < 3; ++k) CHECK(acc[k] == 0.0);
+          continue;
+        }
         double* out = n < B.nown ? &gs[(size_t)(B.own0 + n) * 3] : &gsl[(size_t)(B.slot0 + n - B.nown) * 3];
         for (int k = 0; k < 3; ++k) out[k] = acc[k];
       }
@@ -171,8 +185,8 @@ int main(int argc, char** argv) {
         for (int j = 0; j < 3; ++j) gs[L.fx_node[i] * 3 + j] += gsl[(size_t)L.fx_slot[k] * 3 + j];
     for (size_t i = 0; i < gs.size(); ++i) CHECK(gs[i] == gref[i]);
   }
-  std::printf("ok blocks=%zu chunks=%zu maxloc=%ld halo/owned=%.3f nfix=%zu\n",
+  std::printf("ok blocks=%zu chunks=%zu maxloc=%ld halo/owned=%.3f nfix=%zu edges/input=%.3f\n",
               L.blk.size(), L.chunks.size(), maxloc, (double)halo / npoin,
-              L.fx_node.size());
+              L.fx_node.size(), (double)L.elpq.size() / (double)ne);
   return 0;
 }

@@ -44,18 +44,13 @@ struct xg_ctx {
   int64_t nedge = 0;
   uint32_t* elpq = nullptr;
   double *dx = nullptr, *dy = nullptr, *dz = nullptr;
-  // grad blocks
+  // grad blocks (a gather layout: no halo partials, no boundary faces)
   int nblk = 0, nchunk = 0;
   int4 *blk0 = nullptr, *blk1 = nullptr;     // (own0,nown,nloc,halo_off), (slot0,chunk0,chunk1,-)
   int *halo = nullptr;
   int4 *chk = nullptr;                       // (kind,first,count,tend_off)
   int *chk_ent = nullptr;                    // ent_off
   uint16_t *tend = nullptr, *entries = nullptr;
-  uint32_t *fab = nullptr, *fcs = nullptr;
-  int nslot = 0;
-  double* gslots = nullptr;            // grad halo partials (15 per slot)
-  int nfx = 0;
-  int *fx_node = nullptr, *fx_off = nullptr, *fx_slot = nullptr;
   // grad boundary faces
   int nbnode = 0;
   int *bnode = nullptr, *boff = nullptr, *bface = nullptr;
@@ -63,8 +58,8 @@ struct xg_ctx {
   // RieCG stepping
   bool have_riecg = false;
   double* vol = nullptr;
-  // per node: 1 = receives halo partials in the fix-up, 2 = boundary node;
-  // the node's last gradient writer divides it by vol (RK stages)
+  // per node: 1 = boundary node, whose gradient k_grad_bnd completes and
+  // divides by vol; the others are divided by k_grad_blocks (RK stages)
   uint8_t* gflag = nullptr;
   int nsymnode = 0;
   int *symnode = nullptr, *symoff = nullptr;

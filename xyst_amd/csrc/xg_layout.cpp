@@ -15,10 +15,12 @@
 // coordinates), so a chunk's flux evaluations read LDS only; a thread owns one
 // local node and keeps its running sum in registers.
 //
-// grad and the chorin operators use blocks of up to kBlockCap nodes with
-// 1024-slot chunks (each node thread gathers the terms of the chunk edges
-// that touch it); riemann::rhs builds its own layout of kRhsCap-node blocks
-// and 512-slot chunks, two workgroups per CU.
+// The chorin operators use blocks of up to kBlockCap nodes with 1024-slot
+// chunks (each node thread gathers the terms of the chunk edges that touch
+// it); riemann::rhs builds its own layout of kRhsCap-node blocks and 512-slot
+// chunks, two workgroups per CU.  riemann::grad, whose per-edge work is
+// small, uses a gather layout (see build_layout): edges between two blocks
+// are evaluated in both, so no block leaves partial sums behind.
 //
 // The layout is fastest when node ids are block-contiguous and spatially
 // compact (xg_mesh_block_order), but is correct for any numbering.
@@ -30,10 +32,10 @@
 
 namespace xg {
 
-// Node blocks of consecutive ids for the node graph given as upper-neighbour
-// lists (uoff/up: the neighbours with larger ids of every node), grown while
-// owned <= block_nodes and owned + halo <= block_cap; bstart receives the
-// block starts followed by npoin.
+// Node blocks of consecutive ids for the node graph given as neighbour lists
+// (uoff/up: the neighbours with larger ids of every node, or all neighbours
+// for a gather layout), grown while owned <= block_nodes and owned + halo <=
+// block_cap; bstart receives the block starts followed by npoin.
 static int partition_blocks(int npoin, const std::vector<int64_t>& uoff,
                             const std::vector<int>& up, int block_nodes,
                             int block_cap, std::vector<int>& bstart,
@@ -45,13 +47,14 @@ static int partition_blocks(int npoin, const std::vector<int64_t>& uoff,
     int nown = 0, nhalo = 0, end = s;
     while (end < npoin && nown < block_nodes) {
       // adding node `end`: it leaves the halo if it was there
+      // (a neighbour outside [s, end] is halo: beyond end, or before s)
       int add = 0;
       for (int64_t k = uoff[end]; k < uoff[end + 1]; ++k)
-        if (mark[up[k]] != b && up[k] > end) ++add;
+        if (mark[up[k]] != b && (up[k] > end || up[k] < s)) ++add;
       int was_halo = mark[end] == b ? 1 : 0;
       if (nown > 0 && nown + 1 + nhalo - was_halo + add > block_cap) break;
       for (int64_t k = uoff[end]; k < uoff[end + 1]; ++k)
-        if (mark[up[k]] != b && up[k] > end) { mark[up[k]] = b; ++nhalo; }
+        if (mark[up[k]] != b && (up[k] > end || up[k] < s)) { mark[up[k]] = b; ++nhalo; }
       if (was_halo) { --nhalo; mark[end] = -2; }
       ++nown; ++end;
     }
@@ -81,10 +84,28 @@ static void upper_lists(int npoin, const std::vector<int>& p,
     up[pos[std::min(p[e], q[e])]++] = std::max(p[e], q[e]);
 }
 
+// Neighbour lists in both directions (CSR over all nodes).
+static void full_lists(int npoin, const std::vector<int>& p, const std::vector<int>& q,
+                       std::vector<int64_t>& off, std::vector<int>& nb) {
+  const int64_t ne = (int64_t)p.size();
+  off.assign(npoin + 1, 0);
+  for (int64_t e = 0; e < ne; ++e) {
+    ++off[p[e] + 1];
+    ++off[q[e] + 1];
+  }
+  for (int i = 0; i < npoin; ++i) off[i + 1] += off[i];
+  nb.resize(off[npoin]);
+  std::vector<int64_t> pos(off.begin(), off.end() - 1);
+  for (int64_t e = 0; e < ne; ++e) {
+    nb[pos[p[e]]++] = q[e];
+    nb[pos[q[e]]++] = p[e];
+  }
+}
+
 int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q,
                  const std::vector<double>& d, int64_t ntri, const int* tri,
                  const uint8_t* besym, int block_nodes, int block_cap,
-                 int chunk_slots, Layout& L, std::string& err) {
+                 int chunk_slots, Layout& L, std::string& err, bool gather) {
   const int64_t ne = (int64_t)p.size();
   L = Layout();
   if (chunk_slots < 3 || chunk_slots > kChunkSlots) {
@@ -92,10 +113,14 @@ int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q
     return XG_EINVAL;
   }
   L.npoin = npoin;
-  // upper-neighbour lists (CSR by the smaller end) for halo tracking
+  // neighbour lists for halo tracking: by the smaller end (a block's halo
+  // lies beyond it), or both ways for a gather layout
   std::vector<int64_t> uoff;
   std::vector<int> up;
-  upper_lists(npoin, p, q, uoff, up);
+  if (gather)
+    full_lists(npoin, p, q, uoff, up);
+  else
+    upper_lists(npoin, p, q, uoff, up);
   // (boundary faces are tet faces: their nodes are edge neighbours of the
   // face's smallest node, so the edge lists already cover them)
   std::vector<int> bstart;
@@ -105,10 +130,15 @@ int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q
   std::vector<int> blk_of(npoin);
   for (int b = 0; b < nblk; ++b)
     for (int i = bstart[b]; i < bstart[b + 1]; ++i) blk_of[i] = b;
-  // work items per block
+  // work items per block (gather: an edge belongs to the blocks of both its
+  // ends, and there are no boundary faces)
   std::vector<std::vector<int64_t>> bedges(nblk), bfaces(nblk);
-  for (int64_t e = 0; e < ne; ++e) bedges[blk_of[std::min(p[e], q[e])]].push_back(e);
-  for (int64_t f = 0; f < ntri; ++f) {
+  for (int64_t e = 0; e < ne; ++e) {
+    const int b0 = blk_of[std::min(p[e], q[e])], b1 = blk_of[std::max(p[e], q[e])];
+    bedges[b0].push_back(e);
+    if (gather && b1 != b0) bedges[b1].push_back(e);
+  }
+  for (int64_t f = 0; f < (gather ? 0 : ntri); ++f) {
     int m = std::min(tri[f * 3], std::min(tri[f * 3 + 1], tri[f * 3 + 2]));
     bfaces[blk_of[m]].push_back(f);
   }
@@ -123,8 +153,8 @@ int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q
     // halo: nodes touched by owned items outside the block, ascending
     std::vector<int> halo;
     for (int64_t e : bedges[b]) {
-      int m = std::max(p[e], q[e]);
-      if (m >= bstart[b + 1]) halo.push_back(m);
+      for (int m : {p[e], q[e]})
+        if (m >= bstart[b + 1] || m < bstart[b]) halo.push_back(m);
     }
     for (int64_t f : bfaces[b])
       for (int a = 0; a < 3; ++a)
@@ -140,16 +170,16 @@ int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q
     B.slot0 = nslot;
     for (size_t h = 0; h < halo.size(); ++h) {
       L.halo.push_back(halo[h]);
-      fix[halo[h]].emplace_back(b, nslot + (int)h);
+      if (!gather) fix[halo[h]].emplace_back(b, nslot + (int)h);
     }
-    nslot += (int)halo.size();
+    if (!gather) nslot += (int)halo.size();
     for (int i = 0; i < B.nown; ++i) local[B.own0 + i] = i;
     for (size_t h = 0; h < halo.size(); ++h) local[halo[h]] = B.nown + (int)h;
-    // edges ordered by (local smaller end, local larger end) for gather locality
+    // edges ordered by (smaller local end, larger local end) for locality
     auto& E = bedges[b];
     std::sort(E.begin(), E.end(), [&](int64_t a, int64_t c) {
-      int a0 = local[std::min(p[a], q[a])], a1 = local[std::max(p[a], q[a])];
-      int c0 = local[std::min(p[c], q[c])], c1 = local[std::max(p[c], q[c])];
+      int a0 = std::min(local[p[a]], local[q[a]]), a1 = std::max(local[p[a]], local[q[a]]);
+      int c0 = std::min(local[p[c]], local[q[c]]), c1 = std::max(local[p[c]], local[q[c]]);
       return a0 != c0 ? a0 < c0 : a1 < c1;
     });
     B.chunk0 = (int)L.chunks.size();
@@ -179,8 +209,10 @@ int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q
       for (size_t i = i0; i < i1; ++i) {
         int64_t e = E[i];
         uint16_t slot = (uint16_t)(i - i0);
-        ent.emplace_back(local[p[e]], (uint16_t)(slot | 0x8000));   // R(p) -= f
-        ent.emplace_back(local[q[e]], slot);                        // R(q) += f
+        // R(p) -= f, R(q) += f (gather: owned ends only)
+        if (!gather || local[p[e]] < B.nown)
+          ent.emplace_back(local[p[e]], (uint16_t)(slot | 0x8000));
+        if (!gather || local[q[e]] < B.nown) ent.emplace_back(local[q[e]], slot);
       }
       emit(0, (int64_t)L.elpq.size(), (int)(i1 - i0), ent);
       for (size_t i = i0; i < i1; ++i) {

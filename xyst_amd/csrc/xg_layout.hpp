@@ -16,10 +16,9 @@ constexpr int kChunkSlots = 1024;
 constexpr int kBlockCap = 616;
 // owned nodes per block (a block also stops growing at kBlockCap)
 constexpr int kBlockNodes = 432;
-// riemann::grad blocks: 5 primitive doubles per local node beside 512 slots
-// of 8 doubles and the entry lists, 56,320 bytes of LDS, so two 512-thread
-// workgroups share a CU; their halo share stays near that of the larger
-// ChoCG blocks
+// riemann::grad blocks (a gather layout): 5 primitive doubles per local
+// node beside 512 slots of 8 doubles and the entry lists, 56,320 bytes of
+// LDS, so two 512-thread workgroups share a CU
 constexpr int kGradSlots = 512;
 constexpr int kGradCap = 512;
 constexpr int kGradNodes = 384;
@@ -92,11 +91,16 @@ int build_tet_layout(int npoin, int64_t ntet, const size_t* inpoel,
                      std::string& err);
 
 // chunk_slots: flux slots per edge chunk (<= kChunkSlots; boundary-face
-// chunks hold chunk_slots / 3 faces)
+// chunks hold chunk_slots / 3 faces).
+// gather: a block takes every edge that touches one of its nodes (edges
+// between two blocks are in both), its halo is every outside end of those,
+// and its entries add each edge to its owned ends only, so every owned sum
+// is complete in its block: no halo slots, no fix-up lists, and no boundary
+// faces (an operator on such a layout adds boundary terms by itself)
 int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q,
                  const std::vector<double>& d, int64_t ntri, const int* tri,
                  const uint8_t* besym, int block_nodes, int block_cap,
-                 int chunk_slots, Layout& L, std::string& err);
+                 int chunk_slots, Layout& L, std::string& err, bool gather = false);
 
 int block_order(int64_t npoin, const double* x, const double* y,
                 const double* z, int leaf, size_t* perm);

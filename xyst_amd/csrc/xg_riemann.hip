@@ -6,10 +6,11 @@
 //   rhs:  k_rhs_blocks -- one 512-thread workgroup per node block (two per
 //         CU); the block's node data staged in LDS, one node sum per thread;
 //         k_fixup<5> adds halo partials.
-//   grad: k_grad_blocks -- blocks of its own, also 512 threads and two
-//         workgroups per CU, each node thread gathering its edges' terms
-//         from LDS-staged primitive states; k_fixup<15> adds halo partials;
-//         k_grad_bnd adds boundary-face terms.
+//   grad: k_grad_blocks -- a gather layout of its own (edges between two
+//         blocks evaluated in both, so no halo partials and no fix-up), also
+//         512 threads and two workgroups per CU, each node thread gathering
+//         its edges' terms from LDS-staged primitive states; k_grad_bnd adds
+//         boundary-face terms.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -61,12 +62,14 @@ struct ChunkPrefetch {
   }
 };
 
-// riemann::grad domain term (Riemann.cpp:263-328) on the rhs block layout.
-// The block's primitive states are staged in LDS.  Per chunk: each thread
-// writes one edge's (u_p + u_q, D) to LDS; then the thread owning local node
-// t adds -/+ D (u_p + u_q) for the chunk's edges touching it, in slot order.
-// DIV: owned nodes without halo partials or boundary terms (gflag 0) are
-// final here and are divided by their volume (RieCG.cpp:935-939)
+// riemann::grad domain term (Riemann.cpp:263-328) on the grad (gather)
+// layout: a block holds every edge touching one of its nodes, so each owned
+// node's sum is complete here.  The block's primitive states are staged in
+// LDS.  Per chunk: each thread writes one edge's (u_p + u_q, D) to LDS; then
+// the thread owning local node t adds -/+ D (u_p + u_q) for the chunk's edges
+// touching it, in slot order.
+// DIV: owned nodes without boundary terms (gflag 0) are final here and are
+// divided by their volume (RieCG.cpp:935-939)
 template <bool DIV>
 __global__ __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_grad_blocks(
     int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
@@ -75,8 +78,8 @@ __global__ __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_eu(4)
     const uint16_t* __restrict__ entries, const uint32_t* __restrict__ elpq,
     const double* __restrict__ dx, const double* __restrict__ dy,
     const double* __restrict__ dz, const double* __restrict__ U,
-    double* __restrict__ G, double* __restrict__ S,
-    const uint8_t* __restrict__ gflag, const double* __restrict__ vol) {
+    double* __restrict__ G, const uint8_t* __restrict__ gflag,
+    const double* __restrict__ vol) {
   constexpr int CAP = xg::kGradCap;
   static_assert(CAP <= kGradThreads, "one local node per thread");
   __shared__ double sP[5 * CAP];
@@ -140,7 +143,7 @@ __global__ __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_eu(4)
     }
     fetch(nx);                // in flight during the accumulation below
     __syncthreads();          // F, sE, sT complete
-    if (t < nloc) {
+    if (t < nown) {           // halo nodes have no entries here
       int e0 = t ? sT[t - 1] : 0, e1 = sT[t];
       for (int j = e0; j < e1; ++j) {
         int v = sE[j];
@@ -157,9 +160,9 @@ __global__ __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_eu(4)
     }
     ch = nx;
   }
-  if (t < nloc) {
-    double* o = t < nown ? G + (size_t)(own0 + t) * 15 : S + (size_t)(B1.x + t - nown) * 15;
-    if (DIV && t < nown && gflag[own0 + t] == 0) {
+  if (t < nown) {
+    double* o = G + (size_t)(own0 + t) * 15;
+    if (DIV && gflag[own0 + t] == 0) {
       const double v = vol[own0 + t];
 #pragma unroll
       for (int k = 0; k < 15; ++k) o[k] = acc[k] / v;
@@ -170,34 +173,7 @@ __global__ __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_eu(4)
   }
 }
 
-// grad's halo fix-up with the division of nodes that have no boundary term
-__global__ __launch_bounds__(kBlock) void k_fixup15_div(
-    int nfx, const int* __restrict__ fx_node, const int* __restrict__ fx_off,
-    const int* __restrict__ fx_slot, const double* __restrict__ S,
-    double* __restrict__ G, const uint8_t* __restrict__ gflag,
-    const double* __restrict__ vol) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nfx) return;
-  const int n = fx_node[i];
-  double* r = G + (size_t)n * 15;
-  double acc[15];
-#pragma unroll
-  for (int c = 0; c < 15; ++c) acc[c] = r[c];
-  for (int k = fx_off[i]; k < fx_off[i + 1]; ++k) {
-    const double* s = S + (size_t)fx_slot[k] * 15;
-#pragma unroll
-    for (int c = 0; c < 15; ++c) acc[c] += s[c];
-  }
-  if ((gflag[n] & 2) == 0) {
-    const double v = vol[n];
-#pragma unroll
-    for (int c = 0; c < 15; ++c) acc[c] /= v;
-  }
-#pragma unroll
-  for (int c = 0; c < 15; ++c) r[c] = acc[c];
-}
-
-// gflag: bit 0 for the nodes of the fix-up list, bit 1 for boundary nodes
+// gflag: the nodes with boundary-face gradient terms
 __global__ void k_gflag(int n, const int* __restrict__ node, uint8_t bit,
                         uint8_t* __restrict__ gflag) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -476,22 +452,12 @@ int launch_grad(xg_ctx* c, const double* U, double* G, bool div = false) {
     hipLaunchKernelGGL(k_grad_blocks<true>, dim3(c->nblk), dim3(kGradThreads), 0,
                        c->stream, c->nblk, c->blk0, c->blk1, c->halo, c->chk,
                        c->chk_ent, c->tend, c->entries, c->elpq, c->dx, c->dy,
-                       c->dz, U, G, c->gslots, c->gflag, c->vol);
+                       c->dz, U, G, c->gflag, c->vol);
   else
     hipLaunchKernelGGL(k_grad_blocks<false>, dim3(c->nblk), dim3(kGradThreads), 0,
                        c->stream, c->nblk, c->blk0, c->blk1, c->halo, c->chk,
                        c->chk_ent, c->tend, c->entries, c->elpq, c->dx, c->dy,
-                       c->dz, U, G, c->gslots, nullptr, nullptr);
-  if (c->nfx) {
-    if (div)
-      hipLaunchKernelGGL(k_fixup15_div, dim3(nblocks(c->nfx)), dim3(kBlock), 0,
-                         c->stream, c->nfx, c->fx_node, c->fx_off, c->fx_slot,
-                         c->gslots, G, c->gflag, c->vol);
-    else
-      hipLaunchKernelGGL(k_fixup<15>, dim3(nblocks(c->nfx)), dim3(kBlock), 0,
-                         c->stream, c->nfx, c->fx_node, c->fx_off, c->fx_slot,
-                         c->gslots, G);
-  }
+                       c->dz, U, G, nullptr, nullptr);
   if (c->nbnode) {
     if (div)
       hipLaunchKernelGGL(k_grad_bnd<true>, dim3(nblocks(c->nbnode)), dim3(kBlock), 0,
@@ -671,7 +637,8 @@ int xg_mesh_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
   for (int64_t f = 0; f < ntri * 3; ++f) tri32[f] = (int)tri[f];
   xg::Layout L, LR;
   int rc = xg::build_layout((int)npoin, p, q, d, ntri, tri32.data(), besym,
-                            xg::kGradNodes, xg::kGradCap, xg::kGradSlots, L, c->err);
+                            xg::kGradNodes, xg::kGradCap, xg::kGradSlots, L, c->err,
+                            true);
   if (rc) return rc;
   rc = xg::build_layout((int)npoin, p, q, d, ntri, tri32.data(), besym, xg::kRhsNodes,
                         xg::kRhsCap, xg::kRhsSlots, LR, c->err);
@@ -721,11 +688,6 @@ int xg_mesh_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
       (rc = upload(c, M, &c->chk, c0)) || (rc = upload(c, M, &c->chk_ent, c1)) ||
       (rc = upload(c, M, &c->tend, L.tend)) ||
       (rc = upload(c, M, &c->entries, L.entries)) ||
-      (rc = upload(c, M, &c->fab, L.fab)) || (rc = upload(c, M, &c->fcs, L.fcs)) ||
-      (rc = upload(c, M, &c->fx_node, L.fx_node)) ||
-      (rc = upload(c, M, &c->fx_off, L.fx_off)) ||
-      (rc = upload(c, M, &c->fx_slot, L.fx_slot)) ||
-      (rc = dev_alloc(c, M, (void**)&c->gslots, (size_t)L.nslot * 15 * sizeof(double))) ||
       (rc = upload(c, M, &c->tri, tri32)) || (rc = upload(c, M, &c->bnode, bnode)) ||
       (rc = upload(c, M, &c->boff, boff)) || (rc = upload(c, M, &c->bface, bface)))
     return rc;
@@ -734,8 +696,6 @@ int xg_mesh_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
   c->nedge = ne;
   c->nblk = (int)L.blk.size();
   c->nchunk = (int)L.chunks.size();
-  c->nslot = L.nslot;
-  c->nfx = (int)L.fx_node.size();
   c->nbnode = (int)bnode.size();
   c->have_mesh = true;
   c->have_riecg = false;
@@ -842,12 +802,9 @@ int xg_riecg_setup(xg_ctx* c, const double* vol, int64_t nsym,
     return rc;
   hipLaunchKernelGGL(k_fill_u8, dim3(nblocks(c->npoin)), dim3(kBlock), 0, c->stream,
                      (int64_t)c->npoin, c->gflag);
-  if (c->nfx)
-    hipLaunchKernelGGL(k_gflag, dim3(nblocks(c->nfx)), dim3(kBlock), 0, c->stream,
-                       c->nfx, c->fx_node, (uint8_t)1, c->gflag);
   if (c->nbnode)
     hipLaunchKernelGGL(k_gflag, dim3(nblocks(c->nbnode)), dim3(kBlock), 0, c->stream,
-                       c->nbnode, c->bnode, (uint8_t)2, c->gflag);
+                       c->nbnode, c->bnode, (uint8_t)1, c->gflag);
   XG_HIP(c, hipGetLastError());
   c->nsymnode = (int)node.size();
   c->have_riecg = true;
