@@ -96,6 +96,47 @@ __device__ inline T row_sum8(T v) {
   return v;
 }
 
+// Row sums of a[k] * (z[c] + bt * p[c]) for rows r0 and r1 (r1 < 0: none),
+// 8 lanes per row.  The first two passes of both rows (16 entries each, most
+// rows of a P1 tetrahedral Laplacian) are loaded before any is used, so a
+// wave keeps four gathers per lane in flight; longer rows finish in a loop.
+// Each row's terms are added in the order of a plain loop over k = ia[r] +
+// lane, + 8, ...  Returns the lane partials (row_sum8 completes them).
+// (Also tried: 16 lanes per row, 1.4x slower; a contiguous row range per
+// XCD, no faster.)
+template <class T>
+__device__ inline void rows_zp2(const int* __restrict__ ia, const int* __restrict__ ja,
+                                const T* __restrict__ a, const T* __restrict__ z,
+                                const T* __restrict__ p, T bt, int64_t r0, int64_t r1,
+                                int lane, T& s0, T& s1) {
+  constexpr int L = kLanesPerRow;
+  const int b0 = ia[r0], e0 = ia[r0 + 1];
+  const int b1 = r1 >= 0 ? ia[r1] : 0, e1 = r1 >= 0 ? ia[r1 + 1] : 0;
+  const int k00 = b0 + lane, k01 = k00 + L, k10 = b1 + lane, k11 = k10 + L;
+  const bool u00 = k00 < e0, u01 = k01 < e0, u10 = k10 < e1, u11 = k11 < e1;
+  const int c00 = u00 ? ja[k00] : 0, c01 = u01 ? ja[k01] : 0;
+  const int c10 = u10 ? ja[k10] : 0, c11 = u11 ? ja[k11] : 0;
+  const T a00 = u00 ? a[k00] : T(0), a01 = u01 ? a[k01] : T(0);
+  const T a10 = u10 ? a[k10] : T(0), a11 = u11 ? a[k11] : T(0);
+  const T z00 = z[c00], z01 = z[c01], z10 = z[c10], z11 = z[c11];
+  const T p00 = p[c00], p01 = p[c01], p10 = p[c10], p11 = p[c11];
+  T v0 = 0, v1 = 0;
+  if (u00) v0 += a00 * (z00 + bt * p00);
+  if (u01) v0 += a01 * (z01 + bt * p01);
+  for (int k = k01 + L; k < e0; k += L) {
+    const int c = ja[k];
+    v0 += a[k] * (z[c] + bt * p[c]);
+  }
+  if (u10) v1 += a10 * (z10 + bt * p10);
+  if (u11) v1 += a11 * (z11 + bt * p11);
+  for (int k = k11 + L; k < e1; k += L) {
+    const int c = ja[k];
+    v1 += a[k] * (z[c] + bt * p[c]);
+  }
+  s0 = v0;
+  s1 = v1;
+}
+
 // Workgroup sum returned to every thread (fixed tree, so every workgroup that
 // sums the same values gets bitwise the same result).
 __device__ inline double block_sum_all(double v, double* sm) {
@@ -240,19 +281,26 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(
   const T bt = (T)beta;
   const int lane = threadIdx.x & (kLanesPerRow - 1);
   double spq = 0.0;
+  // rows row and row + stride per trip (the partial sums keep row order)
+  const int64_t stride = (int64_t)gridDim.x * kCgThreads / kLanesPerRow;
   for (int64_t row = ((int64_t)blockIdx.x * kCgThreads + threadIdx.x) / kLanesPerRow;
-       row < nrow; row += (int64_t)gridDim.x * kCgThreads / kLanesPerRow) {
-    T acc = 0;
-    for (int k = ia[row] + lane; k < ia[row + 1]; k += kLanesPerRow) {
-      const int c = ja[k];
-      acc += a[k] * (z[c] + bt * pold[c]);
-    }
-    acc = row_sum8(acc);
+       row < nrow; row += 2 * stride) {
+    const int64_t r1 = row + stride < nrow ? row + stride : -1;
+    T acc0, acc1;
+    rows_zp2(ia, ja, a, z, pold, bt, row, r1, lane, acc0, acc1);
+    acc0 = row_sum8(acc0);
+    acc1 = row_sum8(acc1);
     if (lane == 0) {
       const T pr = z[row] + bt * pold[row];
       pnew[row] = pr;
-      q[row] = acc;
-      spq += (double)pr * (double)acc;
+      q[row] = acc0;
+      spq += (double)pr * (double)acc0;
+      if (r1 >= 0) {
+        const T pr1 = z[r1] + bt * pold[r1];
+        pnew[r1] = pr1;
+        q[r1] = acc1;
+        spq += (double)pr1 * (double)acc1;
+      }
     }
   }
   put_part(pq, blockIdx.x, spq, sm);
@@ -408,20 +456,28 @@ __global__ __launch_bounds__(kCgThreads) void k_fcg_spmv(
   }
   const int lane = threadIdx.x & (kLanesPerRow - 1);
   double spq = 0.0, spr = 0.0;
+  // rows row and row + stride per trip (the partial sums keep row order)
+  const int64_t stride = (int64_t)gridDim.x * kCgThreads / kLanesPerRow;
   for (int64_t row = ((int64_t)blockIdx.x * kCgThreads + threadIdx.x) / kLanesPerRow;
-       row < nrow; row += (int64_t)gridDim.x * kCgThreads / kLanesPerRow) {
-    double acc = 0.0;
-    for (int k = ia[row] + lane; k < ia[row + 1]; k += kLanesPerRow) {
-      const int c = ja[k];
-      acc += a[k] * (z[c] + beta * pold[c]);
-    }
-    acc = row_sum8(acc);
+       row < nrow; row += 2 * stride) {
+    const int64_t r1 = row + stride < nrow ? row + stride : -1;
+    double acc0, acc1;
+    rows_zp2(ia, ja, a, z, pold, beta, row, r1, lane, acc0, acc1);
+    acc0 = row_sum8(acc0);
+    acc1 = row_sum8(acc1);
     if (lane == 0) {
       const double pr = z[row] + beta * pold[row];
       pnew[row] = pr;
-      q[row] = acc;
-      spq += pr * acc;
+      q[row] = acc0;
+      spq += pr * acc0;
       spr += pr * r[row];
+      if (r1 >= 0) {
+        const double pr1 = z[r1] + beta * pold[r1];
+        pnew[r1] = pr1;
+        q[r1] = acc1;
+        spq += pr1 * acc1;
+        spr += pr1 * r[r1];
+      }
     }
   }
   put_part(cur + kMaxParts, blockIdx.x, spq, sm);
