@@ -3,8 +3,11 @@
 // direct scatter, bounds-checking every index the device kernels use.
 // Input file (raw little-endian): int64 npoin, ne, ntri; int32 p[ne], q[ne];
 // int32 tri[3*ntri]; uint8 besym[3*ntri].  Optional arguments: owned nodes,
-// block capacity and chunk slots (default: the grad blocks; the rhs blocks
-// are kRhsNodes kRhsCap kRhsSlots).  Prints "ok <stats>" or exits 1.
+// block capacity and chunk slots (default: the ChoCG blocks; the rhs blocks
+// are kRhsNodes kRhsCap kRhsSlots), then 1 for a gather layout (the grad
+// blocks: kGradNodes kGradCap kGradSlots 1), whose halo nodes must receive
+// nothing and whose owned sums must be complete without a fix-up.  Prints
+// "ok <stats>" or exits 1.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
