@@ -100,8 +100,8 @@ __global__ __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_eu(4)
 #pragma unroll
     for (int c = 0; c < 5; ++c) sP[c * CAP + t] = u[c];
   }
-  // prefetch of the next edge chunk (boundary-face chunks are skipped:
-  // k_grad_bnd adds those terms)
+  // prefetch of the next edge chunk (a gather layout has no boundary-face
+  // chunks, and any would be skipped: k_grad_bnd adds those terms)
   int ch = B1.y;
   while (ch < B1.z && chk[ch].x != 0) ++ch;
   ChunkPrefetch<kGradThreads> pf;
