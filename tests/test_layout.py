@@ -14,6 +14,14 @@ from xyst_amd.meshgen import box_mesh, renumber_mesh
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "build", "layout_check")
 
+def _compile(exe, src):
+    """g++ into a private name, then an atomic rename: parallel test workers
+    never run a half-written binary."""
+    os.makedirs(os.path.dirname(exe), exist_ok=True)
+    tmp = "%s.%d" % (exe, os.getpid())
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", tmp] + src, check=True)
+    os.replace(tmp, exe)
+
 
 def _exe():
     src = [os.path.join(ROOT, "tests", "cpp", "layout_check.cpp"),
@@ -21,8 +29,7 @@ def _exe():
     hdr = os.path.join(ROOT, "xyst_amd", "csrc", "xg_layout.hpp")
     if not os.path.exists(EXE) or any(os.path.getmtime(s) > os.path.getmtime(EXE)
                                       for s in src + [hdr]):
-        os.makedirs(os.path.dirname(EXE), exist_ok=True)
-        subprocess.run(["g++", "-O2", "-std=c++17", "-o", EXE] + src, check=True)
+        _compile(EXE, src)
     return EXE
 
 
@@ -101,8 +108,7 @@ def _tet_exe():
     hdr = os.path.join(ROOT, "xyst_amd", "csrc", "xg_layout.hpp")
     if not os.path.exists(TET_EXE) or any(os.path.getmtime(s) > os.path.getmtime(TET_EXE)
                                           for s in src + [hdr]):
-        os.makedirs(os.path.dirname(TET_EXE), exist_ok=True)
-        subprocess.run(["g++", "-O2", "-std=c++17", "-o", TET_EXE] + src, check=True)
+        _compile(TET_EXE, src)
     return TET_EXE
 
 
