@@ -108,6 +108,8 @@ int main(int argc, char** argv) {
       }
       int nslots = C.kind ? 3 * C.count : C.count;
       CHECK(C.tend_off + B.nloc <= (int)L.tend.size());
+      // the kernels' prefetch reads count * (2 or 3) entries from ent_off
+      CHECK(C.ent_off + C.count * (C.kind ? 3 : 2) <= (int)L.entries.size());
       for (int n = 0; n < B.nloc; ++n) {       // thread n owns local node n
         int e0 = n ? L.tend[C.tend_off + n - 1] : 0, e1 = L.tend[C.tend_off + n];
         CHECK(e0 <= e1 && C.ent_off + e1 <= (int)L.entries.size());

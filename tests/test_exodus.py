@@ -213,3 +213,27 @@ def test_errors(tmp_path):
     with exodus.Writer(path, append=True) as w:
         with pytest.raises(exodus.ExodusError, match="expected 2 node variables"):
             w.write_step(1.0, np.zeros((w.npoin, 3)))
+
+
+def test_connect_size_mismatch_is_an_error(tmp_path):
+    """A connect variable whose size disagrees with its block's
+    num_el_in_blk * num_nod_per_el (a malformed file) is rejected instead of
+    overflowing the caller's buffer, which is sized from the block."""
+    path = str(tmp_path / "mismatch.exo")
+    m = box_mesh(2, seed=3)
+    N, T = m["coord"].shape[1], m["inpoel"].shape[0]
+    g = netcdf_file(path, "w", version=1)
+    g.createDimension("num_nodes", N)
+    g.createDimension("num_el_blk", 1)
+    g.createDimension("num_el_in_blk1", T - 4)        # block says T - 4 tets
+    g.createDimension("num_nod_per_el1", 4)
+    g.createDimension("num_tets_stored", T)            # connect holds T
+    for j, c in enumerate("xyz"):
+        v = g.createVariable("coord" + c, "d", ("num_nodes",))
+        v[:] = m["coord"][j]
+    v = g.createVariable("connect1", "i", ("num_tets_stored", "num_nod_per_el1"))
+    v.elem_type = b"TETRA"
+    v[:] = m["inpoel"] + 1
+    g.close()
+    with pytest.raises(exodus.ExodusError, match="disagrees with its block"):
+        exodus.read_mesh(path, orient=False)

@@ -402,6 +402,10 @@ int gather_blocks(xg_exo* h, bool tets, std::vector<int64_t>& conn) {
     std::vector<int64_t> c;
     int rc = read_i64(h, "connect" + std::to_string(k + 1), c);
     if (rc) return rc;
+    // the callers size their buffers from the block dimensions
+    if ((int64_t)c.size() != b[k].n * b[k].nnode)
+      return exo_fail(h, XG_EIO, "exodus: connect" + std::to_string(k + 1) +
+                                     " size disagrees with its block dimensions");
     for (int64_t& x : c) x -= 1;
     conn.insert(conn.end(), c.begin(), c.end());
   }

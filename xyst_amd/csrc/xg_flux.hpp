@@ -1,6 +1,7 @@
 // Device arithmetic of the RieCG edge flux (FP64): primitive variables,
-// MUSCL/van Leer reconstruction, Rusanov and HLLC fluxes.  Each function
-// follows the reference line range it names, term for term.
+// MUSCL/van Leer reconstruction, Rusanov and HLLC fluxes, in divide-free
+// forms of the reference's arithmetic (each function names the reference
+// line range whose result it reproduces).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -25,151 +26,8 @@ __device__ inline void primitive(const double* __restrict__ U, int i,
   u[4] = s[4] / u[0] - 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]);
 }
 
-// MUSCL / van Leer reconstruction of the flow variables, Riemann.cpp:34-143
-__device__ inline void muscl(const double vw[3], const double* __restrict__ Gp,
-                             const double* __restrict__ Gq, double l[5],
-                             double r[5]) {
-  constexpr double eps = 1.0e-9;
-  constexpr double mc = 1.0 / 3.0;
-  double url[5], urr[5], d1[5], d3[5];
-#pragma unroll
-  for (int c = 0; c < 5; ++c) {
-    double g1 = Gp[c * 3 + 0] * vw[0] + Gp[c * 3 + 1] * vw[1] + Gp[c * 3 + 2] * vw[2];
-    double g2 = Gq[c * 3 + 0] * vw[0] + Gq[c * 3 + 1] * vw[1] + Gq[c * 3 + 2] * vw[2];
-    double delta2 = r[c] - l[c];
-    double delta1 = 2.0 * g1 - delta2;
-    double delta3 = 2.0 * g2 - delta2;
-    d1[c] = delta1;
-    d3[c] = delta3;
-    double rcL = (delta2 + eps) / (delta1 + eps);
-    double rcR = (delta2 + eps) / (delta3 + eps);
-    double rLinv = (delta1 + eps) / (delta2 + eps);
-    double rRinv = (delta3 + eps) / (delta2 + eps);
-    double phiL = (fabs(rcL) + rcL) / (fabs(rcL) + 1.0);
-    double phiR = (fabs(rcR) + rcR) / (fabs(rcR) + 1.0);
-    double phiLi = (fabs(rLinv) + rLinv) / (fabs(rLinv) + 1.0);
-    double phiRi = (fabs(rRinv) + rRinv) / (fabs(rRinv) + 1.0);
-    url[c] = l[c] + 0.25 * (delta1 * (1.0 - mc) * phiL + delta2 * (1.0 + mc) * phiLi);
-    urr[c] = r[c] - 0.25 * (delta3 * (1.0 - mc) * phiR + delta2 * (1.0 + mc) * phiRi);
-  }
-  // first order where density or internal energy would turn negative
-  bool keepL = (l[0] < d1[0] || l[4] < d1[4]);
-  bool keepR = (r[0] < -d3[0] || r[4] < -d3[4]);
-#pragma unroll
-  for (int c = 0; c < 5; ++c) {
-    if (!keepL) l[c] = url[c];
-    if (!keepR) r[c] = urr[c];
-  }
-}
-
-// Rusanov flux, Riemann.cpp:369-478 (domain edges: no symmetry masking)
-__device__ inline void rusanov(const Cfg& cfg, const double n[3], double l[5],
-                               double r[5], double f[5]) {
-  const double g = cfg.g;
-  double pL = (l[0] * l[4]) * (g - 1.0);
-  double pR = (r[0] * r[4]) * (g - 1.0);
-  double vnL = l[1] * n[0] + l[2] * n[1] + l[3] * n[2];
-  double vnR = r[1] * n[0] + r[2] * n[1] + r[3] * n[2];
-  l[4] = (l[4] + 0.5 * (l[1] * l[1] + l[2] * l[2] + l[3] * l[3])) * l[0];
-  l[1] *= l[0]; l[2] *= l[0]; l[3] *= l[0];
-  r[4] = (r[4] + 0.5 * (r[1] * r[1] + r[2] * r[2] + r[3] * r[3])) * r[0];
-  r[1] *= r[0]; r[2] *= r[0]; r[3] *= r[0];
-  double len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-  double sl = fabs(vnL) + sqrt(g * pL / l[0]) * len;
-  double sr = fabs(vnR) + sqrt(g * pR / r[0]) * len;
-  double fw = fmax(sl, sr);
-  f[0] = l[0] * vnL + r[0] * vnR + fw * (r[0] - l[0]);
-  f[1] = l[1] * vnL + r[1] * vnR + (pL + pR) * n[0] + fw * (r[1] - l[1]);
-  f[2] = l[2] * vnL + r[2] * vnR + (pL + pR) * n[1] + fw * (r[2] - l[2]);
-  f[3] = l[3] * vnL + r[3] * vnR + (pL + pR) * n[2] + fw * (r[3] - l[3]);
-  f[4] = (l[4] + pL) * vnL + (r[4] + pR) * vnR + fw * (r[4] - l[4]);
-  if (cfg.stab2) {
-    double fws = cfg.stab2coef * fw;
-#pragma unroll
-    for (int c = 0; c < 5; ++c) f[c] -= fws * (l[c] - r[c]);
-  }
-}
-
-// HLLC flux, Riemann.cpp:480-650 (domain edges: no symmetry masking)
-__device__ inline void hllc(const Cfg& cfg, const double d[3], double l[5],
-                            double r[5], double f[5]) {
-  const double g = cfg.g;
-  double nx = -d[0], ny = -d[1], nz = -d[2];
-  double len = sqrt(nx * nx + ny * ny + nz * nz);
-  nx /= len; ny /= len; nz /= len;
-  double qL = l[1] * nx + l[2] * ny + l[3] * nz;
-  double qR = r[1] * nx + r[2] * ny + r[3] * nz;
-  double pL = (l[0] * l[4]) * (g - 1.0);
-  double pR = (r[0] * r[4]) * (g - 1.0);
-  l[4] = (l[4] + 0.5 * (l[1] * l[1] + l[2] * l[2] + l[3] * l[3])) * l[0];
-  l[1] *= l[0]; l[2] *= l[0]; l[3] *= l[0];
-  r[4] = (r[4] + 0.5 * (r[1] * r[1] + r[2] * r[2] + r[3] * r[3])) * r[0];
-  r[1] *= r[0]; r[2] *= r[0]; r[3] *= r[0];
-  double cL = sqrt(g * pL / l[0]);
-  double cR = sqrt(g * pR / r[0]);
-  double sL = fmin(qL - cL, qR - cR);
-  double sR = fmax(qL + cL, qR + cR);
-  double tL = sL - qL;
-  double tR = sR - qR;
-  double sM = (r[0] * qR * tR - l[0] * qL * tL + pL - pR) / (r[0] * tR - l[0] * tL);
-  double pS = pL - l[0] * tL * (qL - sM);
-  double L2 = -2.0 * len;
-  double mx = nx * L2, my = ny * L2, mz = nz * L2;
-  if (sL > 0.0) {
-    double qL2 = qL * L2;
-    f[0] = l[0] * qL2;
-    f[1] = l[1] * qL2 + pL * mx;
-    f[2] = l[2] * qL2 + pL * my;
-    f[3] = l[3] * qL2 + pL * mz;
-    f[4] = (l[4] + pL) * qL2;
-  } else if (sL <= 0.0 && sM > 0.0) {
-    double s = sL - sM;
-    double u0 = tL * l[0] / s;
-    double u1 = (tL * l[1] + (pS - pL) * nx) / s;
-    double u2 = (tL * l[2] + (pS - pL) * ny) / s;
-    double u3 = (tL * l[3] + (pS - pL) * nz) / s;
-    double u4 = (tL * l[4] - pL * qL + pS * sM) / s;
-    double qL2 = qL * L2;
-    double sL2 = sL * L2;
-    f[0] = l[0] * qL2 + sL2 * (u0 - l[0]);
-    f[1] = l[1] * qL2 + pL * mx + sL2 * (u1 - l[1]);
-    f[2] = l[2] * qL2 + pL * my + sL2 * (u2 - l[2]);
-    f[3] = l[3] * qL2 + pL * mz + sL2 * (u3 - l[3]);
-    f[4] = (l[4] + pL) * qL2 + sL2 * (u4 - l[4]);
-  } else if (sM <= 0.0 && sR >= 0.0) {
-    double s = sR - sM;
-    double u0 = tR * r[0] / s;
-    double u1 = (tR * r[1] + (pS - pR) * nx) / s;
-    double u2 = (tR * r[2] + (pS - pR) * ny) / s;
-    double u3 = (tR * r[3] + (pS - pR) * nz) / s;
-    double u4 = (tR * r[4] - pR * qR + pS * sM) / s;
-    double qR2 = qR * L2;
-    double sR2 = sR * L2;
-    f[0] = r[0] * qR2 + sR2 * (u0 - r[0]);
-    f[1] = r[1] * qR2 + pR * mx + sR2 * (u1 - r[1]);
-    f[2] = r[2] * qR2 + pR * my + sR2 * (u2 - r[2]);
-    f[3] = r[3] * qR2 + pR * mz + sR2 * (u3 - r[3]);
-    f[4] = (r[4] + pR) * qR2 + sR2 * (u4 - r[4]);
-  } else {
-    double qR2 = qR * L2;
-    f[0] = r[0] * qR2;
-    f[1] = r[1] * qR2 + pR * mx;
-    f[2] = r[2] * qR2 + pR * my;
-    f[3] = r[3] * qR2 + pR * mz;
-    f[4] = (r[4] + pR) * qR2;
-  }
-  if (cfg.stab2) {
-    double sl = fabs(qL) + cL;
-    double sr = fabs(qR) + cR;
-    double fws = cfg.stab2coef * fmax(sl, sr) * len;
-#pragma unroll
-    for (int c = 0; c < 5; ++c) f[c] -= fws * (l[c] - r[c]);
-  }
-}
-
-
 // ---------------------------------------------------------------------------
-// Divide-free forms of the same arithmetic.  Each rewrite is an identity in
+// Divide-free forms of the reference arithmetic.  Each rewrite is an identity in
 // exact arithmetic; in FP64 the results differ from the reference's by a few
 // units in the last place.
 //

@@ -252,6 +252,11 @@ int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q
     for (size_t h = 0; h < halo.size(); ++h) local[halo[h]] = -1;
   }
   L.nslot = nslot;
+  // The block kernels prefetch a chunk's entries with a bound of 2 (edges)
+  // or 3 (faces) entries per item.  A gather chunk holds fewer (only owned
+  // ends), so that bound can reach past the last chunk: pad the list so the
+  // prefetch stays inside the buffer.  The padding is never summed.
+  L.entries.resize(L.entries.size() + 3 * (size_t)chunk_slots, 0);
   // fix-up lists: nodes receiving halo partials, slots in block order
   L.fx_off.push_back(0);
   for (int n = 0; n < npoin; ++n) {
