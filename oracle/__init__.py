@@ -96,10 +96,40 @@ def lib():
         L.orc_chorin_rhs.argtypes = [C.POINTER(ChoCfg), I64, C.c_int] + se + [
             _f64p, _f64p, _f64p, I64, _i64p, _f64p, _f64p, _f64p, _f64p, C.c_void_p]
         L.orc_chorin_rhs.restype = C.c_int
-        L.orc_kozcg_step.argtypes = [cfgp, C.POINTER(FctCfg), I64, C.c_int, _f64p, _f64p,
+        L.orc_kozcg_step.argtypes = [cfgp, C.POINTER(FctCfg), C.c_void_p, D, I64,
+                                     C.c_int, _f64p, _f64p,
                                      _f64p, I64, _i64p, _f64p, D, I64, _i64p, _f64p,
                                      I64, _i64p, _f64p, _f64p, _f64p]
         L.orc_kozcg_step.restype = C.c_int
+        probp = C.POINTER(Problem)
+        L.orc_problem_ic.argtypes = [probp, D, D, D, D, _f64p]
+        L.orc_problem_ic.restype = C.c_int
+        L.orc_problem_src.argtypes = [probp, D, D, D, D, _f64p]
+        L.orc_problem_src.restype = C.c_int
+        L.orc_src.argtypes = [probp, I64, C.c_int, _f64p, _f64p, _f64p, _f64p, D,
+                              C.c_void_p, _f64p]
+        L.orc_src.restype = None
+        L.orc_dirbc.argtypes = [probp, I64, C.c_int, _f64p, D, _f64p, _f64p, _f64p,
+                                I64, _i64p, _i32p, _f64p]
+        L.orc_dirbc.restype = None
+        L.orc_farbc.argtypes = [D, C.c_int, _f64p, I64, _i64p, _f64p, _f64p]
+        L.orc_farbc.restype = None
+        L.orc_prebc.argtypes = [D, C.c_int, _f64p, I64, _i64p, _f64p]
+        L.orc_prebc.restype = None
+        L.orc_dt_steady.argtypes = [cfgp, I64, C.c_int, _f64p, _f64p, D, _f64p]
+        L.orc_dt_steady.restype = D
+        L.orc_riecg_step2.argtypes = [cfgp, probp, I64, C.c_int, _i64p, _i64p, _f64p,
+                                      _i64p, _f64p, _i64p, _f64p, _f64p, _f64p, _f64p,
+                                      I64, _i64p, _u8p, _f64p, _f64p,
+                                      C.POINTER(RiecgBC), D, D, C.c_void_p,
+                                      C.c_void_p, _f64p, _f64p, _f64p, _f64p]
+        L.orc_riecg_step2.restype = C.c_int
+        L.orc_diag_err.argtypes = [I64, C.c_int, _f64p, _f64p, _f64p, _f64p]
+        L.orc_diag_err.restype = None
+        L.orc_kozak_rhs_src.argtypes = [cfgp, C.c_void_p, I64, C.c_int, _f64p, _f64p,
+                                        _f64p, I64, _i64p, D, D, C.c_void_p,
+                                        C.c_void_p, _f64p, _f64p]
+        L.orc_kozak_rhs_src.restype = None
         L.orc_kozak_rhs.argtypes = [cfgp, I64, C.c_int, _f64p, _f64p, _f64p, I64,
                                     _i64p, D, C.c_void_p, _f64p, _f64p]
         L.orc_kozak_rhs.restype = None
@@ -127,6 +157,30 @@ def lib():
         L.orc_cg.restype = C.c_int
         _LIB = L
     return _LIB
+
+
+class Problem(C.Structure):
+    """An analytic problem of Problems.cpp (kind 0 none, 1 vortical_flow,
+    2 taylor_green) with its parameters."""
+    _fields_ = [("kind", C.c_int), ("gamma", C.c_double), ("alpha", C.c_double),
+                ("kappa", C.c_double), ("p0", C.c_double)]
+
+
+PROBLEM_KINDS = {"none": 0, "vortical_flow": 1, "taylor_green": 2}
+
+
+def problem(name="none", gamma=1.4, alpha=0.1, kappa=1.0, p0=10.0):
+    return Problem(PROBLEM_KINDS[name], gamma, alpha, kappa, p0)
+
+
+class RiecgBC(C.Structure):
+    """Boundary-condition node lists of the RieCG chare (setupBC)."""
+    _fields_ = [("nsym", C.c_int64), ("symnodes", C.c_void_p), ("symnorms", C.c_void_p),
+                ("ndir", C.c_int64), ("dirnodes", C.c_void_p), ("dirmask", C.c_void_p),
+                ("dirvals", C.c_void_p),
+                ("nfar", C.c_int64), ("farnodes", C.c_void_p), ("farnorms", C.c_void_p),
+                ("far", C.c_double * 5),
+                ("npre", C.c_int64), ("prenodes", C.c_void_p), ("prevals", C.c_void_p)]
 
 
 def cfg(gamma=1.4, flux="rusanov", stab2=False, stab2coef=0.2):
@@ -389,6 +443,189 @@ def diag(U, Un, v):
     return out
 
 
+# ---- problems, BCs, steady stepping, error norms (problems.c) ------------------
+
+def problem_ic(P, coord, t=0.0):
+    """IC() of the problem at every node (npoin, 5)."""
+    n = coord.shape[1]
+    out = np.empty((n, 5))
+    s = np.empty(5)
+    for i in range(n):
+        if lib().orc_problem_ic(C.byref(P), float(coord[0, i]), float(coord[1, i]),
+                                float(coord[2, i]), t, s):
+            raise ValueError("problem has no analytic initial condition")
+        out[i] = s
+    return out
+
+
+def problem_src(P, coord, t=0.0):
+    """SRC() of the problem at every node (npoin, 5)."""
+    n = coord.shape[1]
+    out = np.empty((n, 5))
+    s = np.empty(5)
+    for i in range(n):
+        lib().orc_problem_src(C.byref(P), float(coord[0, i]), float(coord[1, i]),
+                              float(coord[2, i]), t, s)
+        out[i] = s
+    return out
+
+
+def analytic_prim(P, coord, t=0.0):
+    """SOL() in the primitive form NodeDiagnostics compares
+    (NodeDiagnostics.cpp:79-90): rho, u, v, w, specific internal energy."""
+    s = problem_ic(P, coord, t)
+    s[:, 1] /= s[:, 0]
+    s[:, 2] /= s[:, 0]
+    s[:, 3] /= s[:, 0]
+    s[:, 4] = s[:, 4] / s[:, 0] - 0.5 * (s[:, 1] * s[:, 1] + s[:, 2] * s[:, 2]
+                                         + s[:, 3] * s[:, 3])
+    return s
+
+
+def src(P, coord, v, R, t=0.0, tp=None):
+    """riemann::src into R (in place)."""
+    x, y, z = _xyz(coord)
+    tpa = _c(tp, np.float64) if tp is not None else None
+    lib().orc_src(C.byref(P), R.shape[0], R.shape[1], x, y, z, _c(v, np.float64),
+                  t, tpa.ctypes.data if tpa is not None else None, R)
+    return R
+
+
+def dt_steady(c, U, v, cfl):
+    """Steady RieCG::dt: (min dtp, dtp)."""
+    U = _c(U, np.float64)
+    dtp = np.empty(U.shape[0])
+    m = lib().orc_dt_steady(C.byref(c), U.shape[0], U.shape[1], U, _c(v, np.float64),
+                            cfl, dtp)
+    return m, dtp
+
+
+def _sets_nodes(tri, faceset, sets):
+    """Nodes of the boundary triangles on the given side sets."""
+    sel = np.isin(faceset, np.asarray(list(sets), np.int32))
+    return np.unique(tri[sel].reshape(-1))
+
+
+def dirbc_lists(tri, faceset, ncomp, bc_dir):
+    """RieCG::setupBC Dirichlet part (RieCG.cpp:115-158): bc_dir is a list of
+    (side set, [mask per component]); per node the first nonzero mask of each
+    component in bc_dir order.  Returns (nodes ascending, masks (n, ncomp))."""
+    masks = {}
+    for entry in bc_dir:
+        sset, m = entry[0], list(entry[1])
+        assert len(m) == ncomp, "Incorrect Dirichlet BC mask ncomp"
+        for p in _sets_nodes(tri, faceset, [sset]):
+            cur = masks.setdefault(int(p), [0] * ncomp)
+            for c in range(ncomp):
+                if not cur[c]:
+                    cur[c] = int(m[c])
+    nodes = np.array(sorted(masks), np.int64)
+    return nodes, np.array([masks[p] for p in nodes], np.int32).reshape(-1, ncomp)
+
+
+def prebc_lists(tri, faceset, bc_pre):
+    """RieCG::setupBC pressure part (RieCG.cpp:160-200): bc_pre is a list of
+    (side sets, density, pressure).  Returns (nodes, vals (n, 2))."""
+    nodes, vals = [], []
+    for sets, rho, p in bc_pre:
+        for s in sets:
+            for q in _sets_nodes(tri, faceset, [s]):
+                nodes.append(int(q))
+                vals.append((rho, p))
+    return np.array(nodes, np.int64), np.array(vals, np.float64).reshape(-1, 2)
+
+
+def riecg_bc_setup(coord, tri, faceset, symsets, farsets=()):
+    """Symmetry and farfield node/normal lists and besym (RieCG.cpp:202-245,
+    527-618): a farfield node is not also a symmetry node."""
+    far_nodes, far_norms, _ = symbc_setup(coord, tri, faceset, list(farsets)) \
+        if len(farsets) else (np.zeros(0, np.int64), np.zeros((0, 3)), None)
+    sym_nodes, sym_norms, besym = symbc_setup(coord, tri, faceset, list(symsets))
+    if len(farsets):
+        farset = set(int(p) for p in far_nodes)
+        keep = np.array([int(p) not in farset for p in sym_nodes], bool)
+        sym_nodes, sym_norms = sym_nodes[keep], sym_norms[keep]
+        flat = tri.reshape(-1)
+        besym = besym.copy()
+        besym[np.isin(flat, far_nodes)] = 0
+    return sym_nodes, sym_norms, far_nodes, far_norms, besym
+
+
+class BCLists:
+    """Holds the arrays of an orc_riecg_bc alive."""
+
+    def __init__(self, ncomp, symnodes=(), symnorms=(), dirnodes=(), dirmask=(),
+                 dirvals=(), farnodes=(), farnorms=(), far=(1.0, 0, 0, 0, 1.0),
+                 prenodes=(), prevals=()):
+        self.a = dict(
+            sn=_c(np.asarray(symnodes, np.int64).reshape(-1), np.int64),
+            sv=_c(np.asarray(symnorms, np.float64).reshape(-1), np.float64),
+            dn=_c(np.asarray(dirnodes, np.int64).reshape(-1), np.int64),
+            dm=_c(np.asarray(dirmask, np.int32).reshape(-1), np.int32),
+            dv=_c(np.asarray(dirvals, np.float64).reshape(-1), np.float64),
+            fn=_c(np.asarray(farnodes, np.int64).reshape(-1), np.int64),
+            fv=_c(np.asarray(farnorms, np.float64).reshape(-1), np.float64),
+            pn=_c(np.asarray(prenodes, np.int64).reshape(-1), np.int64),
+            pv=_c(np.asarray(prevals, np.float64).reshape(-1), np.float64))
+        a = self.a
+        if a["dv"].size == 0 and a["dn"].size:
+            a["dv"] = np.zeros(a["dn"].size * ncomp)
+        self.s = RiecgBC(a["sn"].size, a["sn"].ctypes.data, a["sv"].ctypes.data,
+                         a["dn"].size, a["dn"].ctypes.data, a["dm"].ctypes.data,
+                         a["dv"].ctypes.data, a["fn"].size, a["fn"].ctypes.data,
+                         a["fv"].ctypes.data, (C.c_double * 5)(*far),
+                         a["pn"].size, a["pn"].ctypes.data, a["pv"].ctypes.data)
+
+
+def orc_apply_bcs(c, P, bc, U, t, coord=None):
+    """RieCG::BC (RieCG.cpp:764-785): dirbc, symbc, farbc, prebc in place;
+    coord is needed when the Dirichlet values come from the problem's IC."""
+    a = bc.a
+    npoin, ncomp = U.shape
+    if a["dn"].size:
+        x, y, z = _xyz(coord) if coord is not None else [np.zeros(npoin)] * 3
+        lib().orc_dirbc(C.byref(P), npoin, ncomp, U, t, x, y, z, a["dn"].size,
+                        a["dn"], a["dm"], a["dv"])
+    if a["sn"].size:
+        lib().orc_symbc(npoin, ncomp, U, a["sn"].size, a["sn"], a["sv"])
+    if a["fn"].size:
+        lib().orc_farbc(c.gamma, ncomp, U, a["fn"].size, a["fn"], a["fv"],
+                        np.array(bc.s.far[:], np.float64))
+    if a["pn"].size:
+        lib().orc_prebc(c.gamma, ncomp, U, a["pn"].size, a["pn"], a["pv"])
+    return U
+
+
+def riecg_step2(c, P, se, coord, tri, besym, vol, v, bc, t, dt, U, dtp=None, tp=None):
+    """One RieCG step with source, all BCs and optional local time stepping
+    (dtp, tp: tp is advanced in place).  U in place; returns Un."""
+    x, y, z = _xyz(coord)
+    npoin, ncomp = U.shape
+    Un = np.empty_like(U)
+    G = np.empty((npoin, 3 * ncomp))
+    R = np.empty((npoin, ncomp))
+    if dtp is not None:
+        assert tp is not None and tp.dtype == np.float64 and tp.flags.c_contiguous
+        dtp = _c(dtp, np.float64)
+    rc = lib().orc_riecg_step2(C.byref(c), C.byref(P), npoin, ncomp, *se.args(), x, y, z,
+                               tri.shape[0], _c(tri, np.int64), _c(besym, np.uint8),
+                               _c(vol, np.float64), _c(v, np.float64), C.byref(bc.s),
+                               t, dt, dtp.ctypes.data if dtp is not None else None,
+                               tp.ctypes.data if dtp is not None else None, U, Un, G, R)
+    if rc:
+        raise ValueError("riecg step failed (%d)" % rc)
+    return Un
+
+
+def diag_err(U, v, An):
+    """Analytic-error sums (L2 sums, then L1 sums) of NodeDiagnostics."""
+    U = _c(U, np.float64)
+    out = np.empty(2 * U.shape[1])
+    lib().orc_diag_err(U.shape[0], U.shape[1], U, _c(v, np.float64), _c(An, np.float64),
+                       out)
+    return out
+
+
 # ---- KozCG -------------------------------------------------------------------
 
 class FctCfg(C.Structure):
@@ -403,8 +640,10 @@ def fctcfg(fct=True, fctdif=1.0, fctclip=False, fctsys=()):
     return f
 
 
-def kozcg_step(c, f, coord, inpoel, vol, dt, symnodes, symnorms, dirnodes, dirvals, U):
-    """One KozCG step; returns (U_new, U_old)."""
+def kozcg_step(c, f, coord, inpoel, vol, dt, symnodes, symnorms, dirnodes, dirvals, U,
+               P=None, t=0.0):
+    """One KozCG step (with the source of problem P at time t); returns
+    (U_new, U_old)."""
     x, y, z = (_c(coord[i], np.float64) for i in range(3))
     U = _c(U, np.float64).copy()
     Un = np.empty_like(U)
@@ -412,7 +651,9 @@ def kozcg_step(c, f, coord, inpoel, vol, dt, symnodes, symnorms, dirnodes, dirva
     sv = _c(symnorms, np.float64) if len(symnorms) else np.zeros(3)
     dn = _c(dirnodes, np.int64) if len(dirnodes) else np.zeros(1, np.int64)
     dv = _c(dirvals, np.float64) if len(dirvals) else np.zeros(U.shape[1])
-    rc = lib().orc_kozcg_step(C.byref(c), C.byref(f), U.shape[0], U.shape[1], x, y, z,
+    rc = lib().orc_kozcg_step(C.byref(c), C.byref(f),
+                              C.cast(C.pointer(P), C.c_void_p) if P is not None else None,
+                              t, U.shape[0], U.shape[1], x, y, z,
                               inpoel.shape[0], _c(inpoel, np.int64), _c(vol, np.float64),
                               dt, len(symnodes), sn, sv, len(dirnodes), dn, dv, U, Un)
     if rc:
@@ -420,8 +661,21 @@ def kozcg_step(c, f, coord, inpoel, vol, dt, symnodes, symnorms, dirnodes, dirva
     return U, Un
 
 
-def kozak_rhs(c, coord, inpoel, dt, U, dtp=None, absacc=False):
-    """kozak::rhs.  With absacc also returns the round-off magnitudes A."""
+def kozak_rhs(c, coord, inpoel, dt, U, dtp=None, absacc=False, P=None, t=0.0, tp=None):
+    """kozak::rhs (with the source of problem P at time t, tp per node when
+    steady).  With absacc also returns the round-off magnitudes A."""
+    if P is not None:
+        assert not absacc
+        x, y, z = _xyz(coord)
+        U = _c(U, np.float64)
+        R = np.empty_like(U)
+        d = _c(dtp, np.float64) if dtp is not None else None
+        tpa = _c(tp, np.float64) if tp is not None else None
+        lib().orc_kozak_rhs_src(C.byref(c), C.cast(C.pointer(P), C.c_void_p), U.shape[0],
+                                U.shape[1], x, y, z, inpoel.shape[0], _c(inpoel, np.int64),
+                                t, dt, tpa.ctypes.data if tpa is not None else None,
+                                d.ctypes.data if d is not None else None, U, R)
+        return R
     x, y, z = (_c(coord[i], np.float64) for i in range(3))
     U = _c(U, np.float64)
     inp = _c(inpoel, np.int64)

@@ -29,7 +29,18 @@ void orc_kozak_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp,
                    const double* x, const double* y, const double* z,
                    int64_t ntet, const int64_t* inpoel, double dt,
                    const double* dtp, const double* U, double* R) {
+  orc_kozak_rhs_src(cfg, NULL, npoin, ncomp, x, y, z, ntet, inpoel, 0.0, dt,
+                    NULL, dtp, U, R);
+}
+
+void orc_kozak_rhs_src(const orc_cfg* cfg, const orc_problem* P,
+                       int64_t npoin, int ncomp, const double* x,
+                       const double* y, const double* z, int64_t ntet,
+                       const int64_t* inpoel, double t0, double dt,
+                       const double* tp, const double* dtp, const double* U,
+                       double* R) {
   const double g = cfg->gamma;
+  const int src = P && (P->kind == 1 || P->kind == 2);
   memset(R, 0, sizeof(double) * (size_t)npoin * (size_t)ncomp);   /* :54 */
   double ue[64];
   for (int64_t e = 0; e < ntet; ++e) {
@@ -70,6 +81,16 @@ void orc_kozak_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp,
         ue[4] -= cg * (u[4] + p[a]) * uj;
         for (int c = 5; c < ncomp; ++c) ue[c] -= cg * u[c] * uj;
       }
+    double te = t0;
+    if (src) {                                          /* Kozak.cpp:124-136 */
+      coef = dte/8.0;
+      if (tp) te = (tp[N[0]] + tp[N[1]] + tp[N[2]] + tp[N[3]])/4.0;
+      for (int a = 0; a < 4; ++a) {
+        double s[5];
+        orc_problem_src(P, x[N[a]], y[N[a]], z[N[a]], te, s);
+        for (int c = 0; c < ncomp && c < 5; ++c) ue[c] += coef * s[c];
+      }
+    }
     /* second half step, Kozak.cpp:138-161 */
     double r = ue[0], ru = ue[1], rv = ue[2], rw = ue[3];
     double pr = (ue[4] - 0.5*(ru*ru + rv*rv + rw*rw)/r) * (g - 1.0);
@@ -87,6 +108,16 @@ void orc_kozak_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp,
         o[4] += cg * (ue[4] + pr) * uj;
         for (int c = 5; c < ncomp; ++c) o[c] += cg * ue[c] * uj;
       }
+    }
+    if (src) {                                          /* Kozak.cpp:163-173 */
+      double xe = (x[N[0]] + x[N[1]] + x[N[2]] + x[N[3]])/4.0;
+      double ye = (y[N[0]] + y[N[1]] + y[N[2]] + y[N[3]])/4.0;
+      double ze = (z[N[0]] + z[N[1]] + z[N[2]] + z[N[3]])/4.0;
+      double se[5];
+      orc_problem_src(P, xe, ye, ze, te + dte/2.0, se);
+      coef = J/24.0;
+      for (int a = 0; a < 4; ++a)
+        for (int c = 0; c < ncomp && c < 5; ++c) R[N[a]*ncomp+c] += coef * se[c];
     }
   }
 }
@@ -218,7 +249,8 @@ static void aec_tet(const double* x, const double* y, const double* z,
   }
 }
 
-int orc_kozcg_step(const orc_cfg* cfg, const orc_fctcfg* f, int64_t npoin,
+int orc_kozcg_step(const orc_cfg* cfg, const orc_fctcfg* f,
+                   const orc_problem* P, double t, int64_t npoin,
                    int ncomp, const double* x, const double* y,
                    const double* z, int64_t ntet, const int64_t* inpoel,
                    const double* vol, double dt, int64_t nsym,
@@ -229,7 +261,8 @@ int orc_kozcg_step(const orc_cfg* cfg, const orc_fctcfg* f, int64_t npoin,
   const size_t nn = (size_t)npoin * (size_t)ncomp;
   double* R = (double*)malloc(sizeof(double) * nn);
   double* A = (double*)calloc(nn, sizeof(double));
-  orc_kozak_rhs(cfg, npoin, ncomp, x, y, z, ntet, inpoel, dt, NULL, U, R);
+  orc_kozak_rhs_src(cfg, P, npoin, ncomp, x, y, z, ntet, inpoel, t, dt, NULL,
+                    NULL, U, R);
   if (f->fct) {
     const double ctau = f->fctdif;
     double* P = (double*)calloc(2*nn, sizeof(double));

@@ -127,6 +127,78 @@ int orc_riecg_step(const orc_cfg* cfg, int64_t npoin, int ncomp,
 void orc_diag(int64_t npoin, int ncomp, const double* U, const double* Un,
               const double* v, double* out);
 
+/* ---- problems, BCs, steady stepping, error norms (problems.c) ---------- */
+
+/* An analytic problem of Problems.cpp: kind 0 none (sod, sedov, userdef:
+ * SRC() and SOL() null), 1 vortical_flow (Problems.cpp:452-509), 2
+ * taylor_green (Problems.cpp:407-450). */
+typedef struct {
+  int    kind;
+  double gamma;                 /* mat_spec_heat_ratio */
+  double alpha, kappa, p0;      /* problem_alpha, problem_kappa, problem_p0 */
+} orc_problem;
+
+/* IC() of the problem at (x, y, z, t): 5 conserved values; -1 for kind 0. */
+int orc_problem_ic(const orc_problem* P, double x, double y, double z,
+                   double t, double* s);
+/* SRC() of the problem: 5 values; returns 0 (and zeros) if it has none. */
+int orc_problem_src(const orc_problem* P, double x, double y, double z,
+                    double t, double* s);
+
+/* riemann::src, Riemann.cpp:880-907: R(p,c) -= s_c(x_p, t or tp[p]) v[p]. */
+void orc_src(const orc_problem* P, int64_t npoin, int ncomp, const double* x,
+             const double* y, const double* z, const double* v, double t,
+             const double* tp, double* R);
+
+/* physics::dirbc, BC.cpp:29-72: ncomp mask entries per node (1: IC at t, or
+ * vals where the problem has no analytic IC; 2: vals). */
+void orc_dirbc(const orc_problem* P, int64_t npoin, int ncomp, double* U,
+               double t, const double* x, const double* y, const double* z,
+               int64_t ndir, const int64_t* nodes, const int32_t* mask,
+               const double* vals);
+
+/* physics::farbc, BC.cpp:152-221; far = {density, u, v, w, pressure}. */
+void orc_farbc(double g, int ncomp, double* U, int64_t nfar,
+               const int64_t* nodes, const double* norms, const double* far);
+
+/* physics::prebc, BC.cpp:223-241; vals = (density, pressure) per node. */
+void orc_prebc(double g, int ncomp, double* U, int64_t npre,
+               const int64_t* nodes, const double* vals);
+
+/* RieCG::dt steady branch (RieCG.cpp:810-824): dtp[p] and its minimum. */
+double orc_dt_steady(const orc_cfg* cfg, int64_t npoin, int ncomp,
+                     const double* U, const double* vol, double cfl,
+                     double* dtp);
+
+/* The boundary-condition node lists of the RieCG chare (setupBC and
+ * streamable, RieCG.cpp:109-245, 527-618). */
+typedef struct {
+  int64_t nsym; const int64_t* symnodes; const double* symnorms;
+  int64_t ndir; const int64_t* dirnodes; const int32_t* dirmask;
+  const double* dirvals;
+  int64_t nfar; const int64_t* farnodes; const double* farnorms;
+  double far[5];
+  int64_t npre; const int64_t* prenodes; const double* prevals;
+} orc_riecg_bc;
+
+/* One RieCG time step with source, every BC and optional local time
+ * stepping (dtp non-null: steady, tp advanced by dtp); RieCG.cpp:764-1057. */
+int orc_riecg_step2(const orc_cfg* cfg, const orc_problem* P, int64_t npoin,
+                    int ncomp, const int64_t* nsup, const int64_t* sup0,
+                    const double* int0, const int64_t* sup1, const double* int1,
+                    const int64_t* sup2, const double* int2, const double* x,
+                    const double* y, const double* z, int64_t nbtri,
+                    const int64_t* triinpoel, const uint8_t* besym,
+                    const double* vol, const double* v, const orc_riecg_bc* bc,
+                    double t, double dt, const double* dtp, double* tp,
+                    double* U, double* Un, double* G, double* R);
+
+/* NodeDiagnostics::rhocompute analytic-error sums (NodeDiagnostics.cpp:
+ * 84-128): out[c] L2 sums, out[ncomp+c] L1 sums of the primitive error
+ * against An (the analytic solution in primitive form). */
+void orc_diag_err(int64_t npoin, int ncomp, const double* U, const double* v,
+                  const double* An, double* out);
+
 /* ---- KozCG (kozcg.c) ------------------------------------------------------ */
 
 /* kozak::rhs, Kozak.cpp:29-180: Taylor-Galerkin rhs, R zeroed first.  dtp:
@@ -135,6 +207,16 @@ void orc_kozak_rhs(const orc_cfg* cfg, int64_t npoin, int ncomp,
                    const double* x, const double* y, const double* z,
                    int64_t ntet, const int64_t* inpoel, double dt,
                    const double* dtp, const double* U, double* R);
+
+/* kozak::rhs with the problem's source term (Kozak.cpp:124-136, 163-173):
+ * the half step adds dt/8 s(x_a, t) per corner (t = mean tp of the corners
+ * when tp is given), the second step J/24 s(x_centroid, t + dt/2). */
+void orc_kozak_rhs_src(const orc_cfg* cfg, const orc_problem* P,
+                       int64_t npoin, int ncomp, const double* x,
+                       const double* y, const double* z, int64_t ntet,
+                       const int64_t* inpoel, double t, double dt,
+                       const double* tp, const double* dtp, const double* U,
+                       double* R);
 
 /* The same evaluation with first-order round-off magnitudes: A receives, per
  * R entry, the sum over its added terms of m(term) (kozcg.c). */
@@ -158,8 +240,10 @@ typedef struct {
  * contributions (lim), the new solution (solve); without FCT U + dt R/vol.
  * Then Dirichlet values (all components of the ndir nodes, dirvals
  * ndir*ncomp) and symmetry BCs.  U is advanced in place, Un receives the old
- * U.  Returns 0, or -1 for ncomp outside 5..16. */
-int orc_kozcg_step(const orc_cfg* cfg, const orc_fctcfg* f, int64_t npoin,
+ * U.  P (nullable) adds its source term to kozak::rhs at time t.
+ * Returns 0, or -1 for ncomp outside 5..16. */
+int orc_kozcg_step(const orc_cfg* cfg, const orc_fctcfg* f,
+                   const orc_problem* P, double t, int64_t npoin,
                    int ncomp, const double* x, const double* y,
                    const double* z, int64_t ntet, const int64_t* inpoel,
                    const double* vol, double dt, int64_t nsym,

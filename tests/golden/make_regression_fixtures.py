@@ -161,6 +161,8 @@ def main():
                         **cases)
     print("wrote chocg_poiseuille.npz", coord.shape, tets.shape, tri.shape)
 
+    analytic_fixtures()
+
     # ExodusII files kept as they are, for the file reader's tests: a mesh
     # with a triangle block and side sets on it (RieCG/Sod), a mesh with
     # untyped TRI/TETRA blocks (ChoCG/Lid), result files with node sets, node
@@ -173,6 +175,112 @@ def main():
                       ("RieCG/Sod", "sod.std.exo")):
         shutil.copyfile(os.path.join(REF, sub, name), os.path.join(dst, name))
     print("copied 4 ExodusII files to", dst)
+
+
+def merge_partitions(ref, coord, stem, nfile):
+    """Fields of a partitioned result (stem.0 .. stem.<nfile-1>) matched to
+    the input nodes by coordinates; a node shared by partitions takes the
+    value of the first file that holds it."""
+    key = lambda P: [tuple(np.round(P[:, i], 12)) for i in range(P.shape[1])]
+    index = {k: i for i, k in enumerate(key(coord))}
+    fields = None
+    seen = np.zeros(coord.shape[1], bool)
+    for k in range(nfile):
+        g = netcdf_file(os.path.join(ref, "%s.%d" % (stem, k)), "r", mmap=False)
+        Y = np.stack([g.variables[c].data.astype(np.float64)
+                      for c in ("coordx", "coordy", "coordz")])
+        names = [b"".join(x).decode().strip("\x00 ")
+                 for x in g.variables["name_nod_var"].data]
+        if fields is None:
+            fields = np.zeros((coord.shape[1], len(names)))
+        for j, kk in enumerate(key(Y)):
+            i = index[kk]
+            if not seen[i]:
+                seen[i] = True
+                fields[i] = [g.variables["vals_nod_var%d" % (v + 1)].data[-1][j]
+                             for v in range(len(names))]
+        t_final = float(g.variables["time_whole"].data[-1])
+    assert seen.all()
+    return fields, names, t_final
+
+
+def chare_of_tet(ref, coord, tets, stem, nfile):
+    """Partition k of each input tet: the file stem.k whose connectivity holds
+    the tet (matched by the coordinates of its corners)."""
+    key = lambda P: [tuple(np.round(P[:, i], 12)) for i in range(P.shape[1])]
+    index = {k: i for i, k in enumerate(key(coord))}
+    tet_id = {tuple(sorted(t)): e for e, t in enumerate(tets)}
+    owner = np.full(tets.shape[0], -1, np.int8)
+    for k in range(nfile):
+        g = netcdf_file(os.path.join(ref, "%s.%d" % (stem, k)), "r", mmap=False)
+        Y = np.stack([g.variables[c].data.astype(np.float64)
+                      for c in ("coordx", "coordy", "coordz")])
+        glob = np.array([index[kk] for kk in key(Y)])
+        for t in g.variables["connect1"].data.astype(np.int64) - 1:
+            e = tet_id[tuple(sorted(glob[t]))]
+            assert owner[e] == -1
+            owner[e] = k
+    assert (owner >= 0).all()
+    return owner
+
+
+def analytic_fixtures():
+    """RieCG/VorticalFlow (rusanov, stab2, hllc, hllc+stab2, steady),
+    RieCG/TaylorGreen, KozCG/VorticalFlow, KozCG/TaylorGreen (unitcube_1k,
+    Dirichlet on all six side sets, every component set to the initial
+    condition) and RieCG/Pipe (rectangle_01_1.5k, symmetry on 2,4,5,6,
+    pressure BCs on sets 1 and 3) into riecg_analytic.npz.
+    Run configurations (the .q files): vortical_flow alpha 0.1, kappa 1.0,
+    p0 10.0, gamma 5/3, cfl 0.8, term 1.0 (steady: nstep 10, local time
+    steps); taylor_green gamma 5/3, cfl 0.8, term 1.0, diagnostics every
+    second step; KozCG cases without FCT; pipe gamma 1.4, cfl 0.5, nstep 20,
+    IC rho 1, u 0, p 99950, bc_pre (1.0, 100000) on set 1 and (1.0, 99900) on
+    set 3."""
+    out = {}
+    rv = os.path.join(REF, "RieCG", "VorticalFlow")
+    coord, tets, tri, faceset = read_mesh(rv, "unitcube_1k.exo")
+    out.update(coord=coord, inpoel=tets, triinpoel=tri, faceset=faceset)
+    for case, diagf, exo in (("rusanov", "diag.std", "vortical_flow.std.exo"),
+                             ("stab2", "diag_stab2.std", "vortical_flow_stab2.std.exo"),
+                             ("hllc", "diag_hllc.std", None),
+                             ("hllc_stab2", "diag_hllc_stab2.std", None),
+                             ("steady", "diag_steady.std", "vortical_flow_steady.std.exo")):
+        out["vf_%s_diag" % case] = np.loadtxt(os.path.join(rv, diagf))
+        if exo:
+            f, names, tf = read_result(rv, coord, exo)
+            out["vf_%s_fields" % case] = f[:, :6]
+            out["vf_%s_t_final" % case] = tf
+    for case, stem in (("hllc", "vortical_flow_hllc_pe4_u0.5.std.exo"),
+                       ("hllc_stab2", "vortical_flow_hllc_stab2_pe4_u0.5.std.exo")):
+        f, names, tf = merge_partitions(rv, coord, stem, 8)
+        out["vf_%s_fields" % case] = f[:, :6]
+        out["vf_%s_t_final" % case] = tf
+    # the HLLC runs are partitioned (4 PEs, 8 chares): the chare of every
+    # tet, from the tets each partition file holds
+    out["vf_chare_of_tet"] = chare_of_tet(rv, coord, tets,
+                                          "vortical_flow_hllc_pe4_u0.5.std.exo", 8)
+    rt = os.path.join(REF, "RieCG", "TaylorGreen")
+    out["tg_diag"] = np.loadtxt(os.path.join(rt, "diag.std"))
+    f, names, tf = read_result(rt, coord, "taylor_green.std.exo")
+    out.update(tg_fields=f[:, :6], tg_t_final=tf)
+    kv = os.path.join(REF, "KozCG", "VorticalFlow")
+    out["koz_vf_diag"] = np.loadtxt(os.path.join(kv, "diag.std"))
+    f, names, tf = read_result(kv, coord, "vortical_flow.std.exo")
+    out.update(koz_vf_fields=f[:, :6], koz_vf_t_final=tf)
+    kt = os.path.join(REF, "KozCG", "TaylorGreen")
+    out["koz_tg_diag"] = np.loadtxt(os.path.join(kt, "diag.std"))
+    f, names, tf = read_result(kt, coord, "taylor_green.std.exo")
+    out.update(koz_tg_fields=f[:, :6], koz_tg_t_final=tf)
+    rp = os.path.join(REF, "RieCG", "Pipe")
+    pc, pt, ptri, pfs = read_mesh(rp)
+    f, names, tf = read_result(rp, pc, "pipe.std.exo")
+    out.update(pipe_coord=pc, pipe_inpoel=pt, pipe_triinpoel=ptri, pipe_faceset=pfs,
+               pipe_fields=f, pipe_t_final=tf,
+               pipe_diag=np.loadtxt(os.path.join(rp, "diag.std")))
+    out["field_names"] = np.array(["density", "xvelocity", "yvelocity", "zvelocity",
+                                   "energy", "pressure"])
+    np.savez_compressed(os.path.join(HERE, "riecg_analytic.npz"), **out)
+    print("wrote riecg_analytic.npz", sorted(out))
 
 
 if __name__ == "__main__":
