@@ -43,6 +43,20 @@ typedef struct {
   double stab2coef;
 } xg_config;
 
+/* The analytic problem whose source term the device adds (problems::SRC(),
+ * Problems.cpp:1299-1325; g_cfg problem, problem_alpha/kappa/p0).  Problems
+ * without a source (sod, sedov, userdef) are XG_PROBLEM_NONE.  The sources
+ * served are independent of time. */
+enum {
+  XG_PROBLEM_NONE = 0,
+  XG_PROBLEM_VORTICAL_FLOW = 1,   /* Problems.cpp:452-509 */
+  XG_PROBLEM_TAYLOR_GREEN = 2     /* Problems.cpp:407-450 */
+};
+typedef struct {
+  int    kind;
+  double alpha, kappa, p0;        /* vortical_flow parameters */
+} xg_problem;
+
 /* ---- context ------------------------------------------------------------ */
 int xg_ctx_create(int device, xg_ctx** ctx);
 int xg_ctx_destroy(xg_ctx* ctx);
@@ -52,6 +66,9 @@ const char* xg_last_error(const xg_ctx* ctx);
  * entry method per chare at a time. */
 int xg_set_stream(xg_ctx* ctx, void* hip_stream);
 int xg_set_config(xg_ctx* ctx, const xg_config* cfg);
+/* The problem of riemann::rhs's and kozak::rhs's source term (default
+ * XG_PROBLEM_NONE); XG_EUNSUPPORTED for a kind the device does not have. */
+int xg_set_problem(xg_ctx* ctx, const xg_problem* problem);
 
 /* ---- host preprocessing (native C++, no device work) --------------------- */
 
@@ -129,12 +146,18 @@ int xg_riemann_grad(xg_ctx* ctx, int ncomp, const double* U, double* G);
 int xg_riemann_grad_dev(xg_ctx* ctx, int ncomp, const double* U, double* G);
 
 /* riemann::rhs (Riemann.hpp:28-39, Riemann.cpp:909-946): R(npoin, ncomp)
- * from G (already divided by the nodal volume) and U.  No source term
- * (problems::SRC() is null for the problems this path serves). */
+ * = advdom + advbnd from G (already divided by the nodal volume) and U, then
+ * the source term of the context's problem (riemann::src, Riemann.cpp:
+ * 880-907): R(p,c) -= s_c(x_p, t) v[p], with v the nodal volumes of this
+ * partition's own elements (may be null when the problem has no source) and
+ * tp the per-node time of steady runs (null: t; the sources served do not
+ * depend on time, so tp changes nothing).  XG_EUNSUPPORTED for ncomp != 5
+ * (no transported scalars). */
 int xg_riemann_rhs(xg_ctx* ctx, int ncomp, const double* G, const double* U,
-                   double* R);
+                   const double* v, double t, const double* tp, double* R);
 int xg_riemann_rhs_dev(xg_ctx* ctx, int ncomp, const double* G,
-                       const double* U, double* R);
+                       const double* U, const double* v, double t,
+                       const double* tp, double* R);
 
 /* ---- resident RieCG stepping ------------------------------------------------ */
 
@@ -143,16 +166,61 @@ int xg_riemann_rhs_dev(xg_ctx* ctx, int ncomp, const double* G,
 int xg_riecg_setup(xg_ctx* ctx, const double* vol, int64_t nsym,
                    const size_t* symnodes, const double* symnorms);
 
+/* The other boundary conditions of the RieCG chare (setupBC and
+ * streamable, RieCG.cpp:109-245, 527-618; applied by RieCG::BC,
+ * RieCG.cpp:764-785, after the symmetry BCs of xg_riecg_setup in the order
+ * dirbc, symbc, farbc, prebc):
+ *   Dirichlet (physics::dirbc, BC.cpp:29-72): ndir nodes with 5 mask entries
+ *     each; a component with mask 1 takes dirvals[k*5+c], the initial
+ *     condition at the node, which the caller evaluates (the problems served
+ *     have initial conditions independent of time); other masks leave it
+ *     (the RieCG chare passes no BC values);
+ *   farfield (physics::farbc, BC.cpp:152-221): nfar (node, unit normal)
+ *     pairs and the free stream far = {density, u, v, w, pressure};
+ *   pressure (physics::prebc, BC.cpp:223-241): npre nodes with (density,
+ *     pressure) each.
+ * Call after xg_riecg_setup; replaces any earlier lists. */
+int xg_riecg_bc(xg_ctx* ctx, int64_t ndir, const size_t* dirnodes,
+                const int32_t* dirmask, const double* dirvals, int64_t nfar,
+                const size_t* farnodes, const double* farnorms,
+                const double far[5], int64_t npre, const size_t* prenodes,
+                const double* prevals);
+
+/* RieCG::BC (RieCG.cpp:764-785) on device-resident U: dirbc, symbc, farbc,
+ * prebc.  The chare applies it to the initial state too (RieCG.cpp:754). */
+int xg_riecg_apply_bc_dev(xg_ctx* ctx, double* U);
+
 /* RieCG::dt (RieCG.cpp:787-852), unsteady branch: *dt = cfl * min_p
  * cbrt(vol_p)/max(|u|+c, 1e-8).  U on the device. */
 int xg_riecg_dt_dev(xg_ctx* ctx, const double* U, double cfl, double* dt);
 
+/* RieCG::dt, steady branch (local time stepping, RieCG.cpp:810-824):
+ * dtp[p] = cfl cbrt(vol_p)/max(|u|+c, 1e-8) (device, npoin) and *dt its
+ * minimum. */
+int xg_riecg_dt_steady_dev(xg_ctx* ctx, const double* U, double cfl,
+                           double* dtp, double* dt);
+
 /* One RK stage (RieCG::grad/rhs/solve, RieCG.cpp:871-1057): G = grad(U)/vol,
- * R = rhs(G,U); at stage 0 Un = U; U = Un - rkcoef[stage]*dt*R/vol with
- * rkcoef {1/3,1/2,1}; symmetry BCs.  All arrays on the device:
+ * R = rhs(G,U) with the problem's source at time t (steady: at tp +
+ * prev_rkcoef dtp); at stage 0 Un = U; U = Un - rkcoef[stage]*dt*R/vol with
+ * rkcoef {1/3,1/2,1} (steady: dtp[i] for dt); then RieCG::BC.  dtp, tp
+ * (device, npoin) null for time-accurate runs; with them, stage 2 also
+ * advances tp += dtp (RieCG.cpp:1047-1050).  All arrays on the device:
  * U, Un (npoin*5), G (npoin*15), R (npoin*5). */
-int xg_riecg_stage_dev(xg_ctx* ctx, int stage, double dt, double* U,
-                       double* Un, double* G, double* R);
+int xg_riecg_stage_dev(xg_ctx* ctx, int stage, double t, double dt,
+                       const double* dtp, double* tp, double* U, double* Un,
+                       double* G, double* R);
+
+/* The pieces of a stage for partitioned runs, where the partial sums of the
+ * partitions are added between them (xg_comm_sum_dev):
+ * xg_riecg_grad_div_dev divides G(npoin, 15) by the summed nodal volume of
+ * xg_riecg_setup (RieCG.cpp:935-939); xg_riecg_solve_dev is RieCG::solve
+ * (RieCG.cpp:992-1057): Un = U at stage 0, the RK update with R, the BCs
+ * at t + rkcoef dt, and tp += dtp after stage 2 when steady. */
+int xg_riecg_grad_div_dev(xg_ctx* ctx, double* G);
+int xg_riecg_solve_dev(xg_ctx* ctx, int stage, double t, double dt,
+                       const double* dtp, double* tp, double* U, double* Un,
+                       const double* R);
 
 /* Number of bytes the layout occupies on the device (for reporting). */
 int xg_mesh_device_bytes(const xg_ctx* ctx, int64_t* bytes);
@@ -166,11 +234,15 @@ int xg_kozak_upload(xg_ctx* ctx, int64_t npoin, const double* x,
                     const double* y, const double* z, int64_t ntet,
                     const size_t* inpoel);
 
-/* kozak::rhs (Kozak.hpp:19-27, Kozak.cpp:29-180), unsteady (dtp empty) and
- * without a source term: R(npoin, ncomp) zeroed and filled with the
- * Taylor-Galerkin rhs of U for time step dt.  ncomp = 5. */
-int xg_kozak_rhs(xg_ctx* ctx, int ncomp, double dt, const double* U, double* R);
-int xg_kozak_rhs_dev(xg_ctx* ctx, int ncomp, double dt, const double* U,
+/* kozak::rhs (Kozak.hpp:19-27, Kozak.cpp:29-180): R(npoin, ncomp) zeroed
+ * and filled with the Taylor-Galerkin rhs of U for time step dt at time t,
+ * with the source term of the context's problem (Kozak.cpp:124-136,
+ * 163-173).  tp, dtp (steady runs) must be null: XG_EUNSUPPORTED otherwise,
+ * and for ncomp != 5. */
+int xg_kozak_rhs(xg_ctx* ctx, int ncomp, double t, double dt, const double* tp,
+                 const double* dtp, const double* U, double* R);
+int xg_kozak_rhs_dev(xg_ctx* ctx, int ncomp, double t, double dt,
+                     const double* tp, const double* dtp, const double* U,
                      double* R);
 
 /* ---- resident KozCG stepping ------------------------------------------------ */
@@ -203,9 +275,10 @@ int xg_kozcg_dt_dev(xg_ctx* ctx, const double* U, double cfl, double* dt);
  * symmetry BCs on them), the low-order solution, allowed bounds, limit
  * coefficients and limited contributions, the new solution; without FCT
  * U + dt R/vol.  Then Dirichlet and symmetry BCs (KozCG::BC,
- * KozCG.cpp:590-611).  U (npoin*5, device) is advanced in place; Un
- * (npoin*5, device) receives the old solution. */
-int xg_kozcg_step_dev(xg_ctx* ctx, double dt, double* U, double* Un);
+ * KozCG.cpp:590-611).  t is the time at the start of the step (the source
+ * term's).  U (npoin*5, device) is advanced in place; Un (npoin*5, device)
+ * receives the old solution. */
+int xg_kozcg_step_dev(xg_ctx* ctx, double t, double dt, double* U, double* Un);
 
 /* ---- chorin:: drop-ins (ChoCG) ------------------------------------------------ */
 
@@ -331,12 +404,46 @@ int xg_chocg_step_dev(xg_ctx* ctx, double dt, double* U, double* Un, double* P,
 int xg_diag_rho_dev(xg_ctx* ctx, int64_t npoin, int ncomp, const double* U,
                     const double* Un, const double* V, double* out);
 
+/* The analytic-error sums of NodeDiagnostics::rhocompute
+ * (NodeDiagnostics.cpp:79-128): with An (npoin*5, device) the analytic
+ * solution in primitive form (rho, u, v, w, specific internal energy; the
+ * caller evaluates SOL()), out[c] = sum_i (nu_c - An_c)^2 V_i and
+ * out[5+c] = sum_i |nu_c - An_c| V_i, nu the same form of U.  ncomp = 5;
+ * out (host) 10 values; deterministic summation order. */
+int xg_diag_err_dev(xg_ctx* ctx, int64_t npoin, int ncomp, const double* U,
+                    const double* V, const double* An, double* out);
+
 /* NodeDiagnostics::precompute (NodeDiagnostics.cpp:148-264) without the
  * analytic-solution error norms: out = {sum p^2 V, sum u_c^2 V (3),
  * sum dp^2 V, sum (u_c - un_c)^2 V (3)}.  U, Un (npoin*3), P, dP, V (npoin)
  * on the device; out (host) 8 values. */
 int xg_diag_pre_dev(xg_ctx* ctx, int64_t npoin, const double* U, const double* Un,
                     const double* P, const double* dP, const double* V, double* out);
+
+/* ---- partitions on one device ------------------------------------------------
+ * The reference runs one chare per mesh partition and adds the partitions'
+ * partial nodal sums at shared nodes by messages (Discretization's
+ * NodeCommMap; RieCG::comgrad/comrhs, RieCG.cpp:871-916, 948-990;
+ * ConjugateGradients::comq, ConjugateGradients.cpp:611-627).  Here the
+ * partitions are contexts on one device and the exchange is one kernel. */
+typedef struct xg_comm xg_comm;
+/* nparts partitions (at most 64); partition k has nlocal[k] nodes whose
+ * global ids are gid[k][0 .. nlocal[k]).  Device memory on `device`.  A
+ * handle is returned even when this fails (holding the message for
+ * xg_comm_error); xg_comm_destroy releases it. */
+int xg_comm_create(int device, int nparts, const int64_t* nlocal,
+                   const size_t* const* gid, xg_comm** comm);
+/* Every node held by more than one partition gets, in every copy, the sum
+ * of the copies' values over the partitions in partition order.
+ * fields[k]: partition k's device array (nlocal[k] x ncomp, AoS).  Enqueued
+ * on `stream` (a hipStream_t, NULL = null stream); the caller orders it after
+ * the work that wrote the partials. */
+int xg_comm_sum_dev(xg_comm* comm, int ncomp, double* const* fields,
+                    void* stream);
+/* Number of nodes held by more than one partition. */
+int xg_comm_shared(const xg_comm* comm, int64_t* nshared);
+const char* xg_comm_error(const xg_comm* comm);
+int xg_comm_destroy(xg_comm* comm);
 
 /* ---- pressure-Poisson system (ChoCG) and conjugate gradients ---------------- */
 

@@ -35,15 +35,27 @@ SIG = {
                                  VP, I64, VP, VP]),
     "xg_riemann_grad": (C.c_int, [VP, C.c_int, VP, VP]),
     "xg_riemann_grad_dev": (C.c_int, [VP, C.c_int, VP, VP]),
-    "xg_riemann_rhs": (C.c_int, [VP, C.c_int, VP, VP, VP]),
-    "xg_riemann_rhs_dev": (C.c_int, [VP, C.c_int, VP, VP, VP]),
+    "xg_riemann_rhs": (C.c_int, [VP, C.c_int, VP, VP, VP, D, VP, VP]),
+    "xg_riemann_rhs_dev": (C.c_int, [VP, C.c_int, VP, VP, VP, D, VP, VP]),
+    "xg_set_problem": (C.c_int, [VP, VP]),
     "xg_riecg_setup": (C.c_int, [VP, VP, I64, VP, VP]),
     "xg_riecg_dt_dev": (C.c_int, [VP, VP, D, C.POINTER(D)]),
-    "xg_riecg_stage_dev": (C.c_int, [VP, C.c_int, D, VP, VP, VP, VP]),
+    "xg_riecg_stage_dev": (C.c_int, [VP, C.c_int, D, D, VP, VP, VP, VP, VP, VP]),
+    "xg_riecg_bc": (C.c_int, [VP, I64, VP, VP, VP, I64, VP, VP, VP, I64, VP, VP]),
+    "xg_riecg_apply_bc_dev": (C.c_int, [VP, VP]),
+    "xg_riecg_dt_steady_dev": (C.c_int, [VP, VP, D, VP, C.POINTER(D)]),
+    "xg_riecg_grad_div_dev": (C.c_int, [VP, VP]),
+    "xg_riecg_solve_dev": (C.c_int, [VP, C.c_int, D, D, VP, VP, VP, VP, VP]),
+    "xg_diag_err_dev": (C.c_int, [VP, I64, C.c_int, VP, VP, VP, VP]),
+    "xg_comm_create": (C.c_int, [C.c_int, C.c_int, VP, VP, C.POINTER(VP)]),
+    "xg_comm_sum_dev": (C.c_int, [VP, C.c_int, VP, VP]),
+    "xg_comm_shared": (C.c_int, [VP, C.POINTER(I64)]),
+    "xg_comm_error": (C.c_char_p, [VP]),
+    "xg_comm_destroy": (C.c_int, [VP]),
     "xg_mesh_device_bytes": (C.c_int, [VP, C.POINTER(I64)]),
     "xg_kozak_upload": (C.c_int, [VP, I64, VP, VP, VP, I64, VP]),
-    "xg_kozak_rhs": (C.c_int, [VP, C.c_int, D, VP, VP]),
-    "xg_kozak_rhs_dev": (C.c_int, [VP, C.c_int, D, VP, VP]),
+    "xg_kozak_rhs": (C.c_int, [VP, C.c_int, D, D, VP, VP, VP, VP]),
+    "xg_kozak_rhs_dev": (C.c_int, [VP, C.c_int, D, D, VP, VP, VP, VP]),
     "xg_kozcg_setup": (C.c_int, [VP, VP, I64, VP, VP, I64, VP, VP, VP, VP]),
     "xg_chocg_setup": (C.c_int, [VP, VP, VP, I64, VP, VP, I64, VP, I64, VP, VP, VP,
                                  I64, VP, VP]),
@@ -57,7 +69,7 @@ SIG = {
     "xg_amg_info": (C.c_int, [VP, VP, VP, VP]),
     "xg_diag_pre_dev": (C.c_int, [VP, I64, VP, VP, VP, VP, VP, VP]),
     "xg_kozcg_dt_dev": (C.c_int, [VP, VP, D, VP]),
-    "xg_kozcg_step_dev": (C.c_int, [VP, D, VP, VP]),
+    "xg_kozcg_step_dev": (C.c_int, [VP, D, D, VP, VP]),
     "xg_exo_error": (C.c_char_p, [VP]),
     "xg_exo_close": (None, [VP]),
     "xg_exo_open": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(VP)]),
@@ -118,6 +130,14 @@ class XgError(RuntimeError):
 class Config(C.Structure):
     _fields_ = [("gamma", C.c_double), ("flux", C.c_int), ("stab2", C.c_int),
                 ("stab2coef", C.c_double)]
+
+
+class Problem(C.Structure):
+    _fields_ = [("kind", C.c_int), ("alpha", C.c_double), ("kappa", C.c_double),
+                ("p0", C.c_double)]
+
+
+PROBLEMS = {"none": 0, "vortical_flow": 1, "taylor_green": 2}
 
 
 class ChoConfig(C.Structure):

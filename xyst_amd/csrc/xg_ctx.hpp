@@ -64,6 +64,21 @@ struct xg_ctx {
   int nsymnode = 0;
   int *symnode = nullptr, *symoff = nullptr;
   double* symnorm = nullptr;
+  // the other RieCG BCs (xg_riecg_bc): Dirichlet nodes with masks and
+  // values, farfield (node, normal) pairs grouped by node, pressure
+  // (node, density, pressure) records grouped by node
+  std::vector<DevArr> bc_allocs;
+  int ndirnode = 0, nfarnode = 0, nprenode = 0;
+  int* dirnode = nullptr;
+  uint8_t* dirmask = nullptr;
+  double* dirval = nullptr;
+  int *farnode = nullptr, *faroff = nullptr;
+  double* farnorm = nullptr;
+  double far[5] = {1.0, 0.0, 0.0, 0.0, 1.0};
+  int *prenode = nullptr, *preoff = nullptr;
+  double* preval = nullptr;
+  // source term of riemann::rhs and kozak::rhs (xg_set_problem)
+  xg_problem problem{XG_PROBLEM_NONE, 0.0, 0.0, 0.0};
   double* red = nullptr;               // reduction partials + result
   // staging for the host-pointer entry points
   double *sU = nullptr, *sG = nullptr, *sR = nullptr;

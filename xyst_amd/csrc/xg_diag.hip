@@ -88,6 +88,41 @@ __global__ __launch_bounds__(kBlock) void k_diag_pre(int64_t n, const double* __
   }
 }
 
+// rhocompute's analytic-error sums (NodeDiagnostics.cpp:112-122): the
+// numerical solution in primitive form against An, (nu_c - An_c)^2 V and
+// |nu_c - An_c| V
+__global__ __launch_bounds__(kBlock) void k_diag_err(int64_t n, const double* __restrict__ U,
+                                                     const double* __restrict__ V,
+                                                     const double* __restrict__ An,
+                                                     double* __restrict__ part) {
+  __shared__ double sm[kBlock / 64];
+  double acc[10];
+#pragma unroll
+  for (int q = 0; q < 10; ++q) acc[q] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    const double v = V[i];
+    double nu[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) nu[c] = U[i * 5 + c];
+    nu[1] /= nu[0];
+    nu[2] /= nu[0];
+    nu[3] /= nu[0];
+    nu[4] = nu[4] / nu[0] - 0.5 * (nu[1] * nu[1] + nu[2] * nu[2] + nu[3] * nu[3]);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      const double du = nu[c] - An[i * 5 + c];
+      acc[c] += du * du * v;
+      acc[5 + c] += fabs(du) * v;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 10; ++q) {
+    const double s = block_sum(acc[q], sm);
+    if (threadIdx.x == 0) part[(size_t)q * gridDim.x + blockIdx.x] = s;
+  }
+}
+
 // out[q] = sum of part[q * nb .. q * nb + nb) in index order
 __global__ __launch_bounds__(kBlock) void k_diag_final(int ns, int nb,
                                                        const double* __restrict__ part,
@@ -148,6 +183,21 @@ int xg_diag_rho_dev(xg_ctx* c, int64_t npoin, int ncomp, const double* U, const 
   else
     hipLaunchKernelGGL(k_diag_rho<1>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, Un, V, part);
   return finish(c, ns, nb, part, dout, out);
+}
+
+int xg_diag_err_dev(xg_ctx* c, int64_t npoin, int ncomp, const double* U, const double* V,
+                    const double* An, double* out) {
+  if (!c) return XG_EINVAL;
+  if (npoin < 0 || !U || !V || !An || !out)
+    return fail(c, XG_EINVAL, "xg_diag_err: bad arguments");
+  if (ncomp != 5) return fail(c, XG_EUNSUPPORTED, "xg_diag_err: ncomp must be 5");
+  XG_HIP(c, hipSetDevice(c->device));
+  const int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, nblocks(npoin)));
+  double *part, *dout;
+  int rc = diag_buffers(c, 10, nb, &part, &dout);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_diag_err, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, V, An, part);
+  return finish(c, 10, nb, part, dout, out);
 }
 
 int xg_diag_pre_dev(xg_ctx* c, int64_t npoin, const double* U, const double* Un,

@@ -24,6 +24,7 @@
 #include "xg_ctx.hpp"
 #include "xg_flux.hpp"
 #include "xg_layout.hpp"
+#include "xg_problems.hpp"
 
 using namespace xgd;
 
@@ -96,8 +97,10 @@ __device__ inline void tet_aec(const double X[4][3], const double* sU, int cap,
 }
 
 // AEC: also the antidiffusive contributions P+/- (KozCG::aec), the output
-// record of a node being then R (5), P+ (5), P- (5)
-template <bool AEC>
+// record of a node being then R (5), P+ (5), P- (5).  SRC: the problem's
+// source term (Kozak.cpp:124-136, 163-173): dt/8 s(x_a) of every corner in
+// the half-step state, J/24 s(centroid) added to every corner.
+template <bool AEC, bool SRC>
 __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_kozak_blocks(
     int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
     const int* __restrict__ halo, const int4* __restrict__ chk,
@@ -106,7 +109,7 @@ __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))
     const uint32_t* __restrict__ tcd, const double* __restrict__ x,
     const double* __restrict__ y, const double* __restrict__ z,
     const double* __restrict__ U, double* __restrict__ R,
-    double* __restrict__ S, double gm1, double dt, double ctau) {
+    double* __restrict__ S, double gm1, double dt, double ctau, xg::Prob P) {
   constexpr int CAP = xg::kKozCap;
   constexpr int NO = AEC ? 15 : 5;
   constexpr int NF = AEC ? 10 : 5;   // per corner slot: flux (+ aec)
@@ -195,6 +198,13 @@ __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))
         h[j + 1] -= cg * p;
         h[4] -= cg * (u[4] + p) * uj;
       }
+      if constexpr (SRC) {
+        double sa[5];
+        xg::source(P, X[a][0], X[a][1], X[a][2], sa);
+        const double c8 = dt / 8.0;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) h[c] += c8 * sa[c];
+      }
       // sum over the tet's four corners (lanes 4i..4i+3); (h0+h1)+(h2+h3) and
       // (h2+h3)+(h0+h1) round identically, so all four lanes agree
       double ue[5];
@@ -217,6 +227,15 @@ __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))
         f[3] += cg * ue[3] * uj;
         f[j + 1] += cg * pr;
         f[4] += cg * (ue[4] + pr) * uj;
+      }
+      if constexpr (SRC) {
+        double se[5];
+        xg::source(P, (X[0][0] + X[1][0] + X[2][0] + X[3][0]) / 4.0,
+                   (X[0][1] + X[1][1] + X[2][1] + X[3][1]) / 4.0,
+                   (X[0][2] + X[1][2] + X[2][2] + X[3][2]) / 4.0, se);
+        const double cj = J / 24.0;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) f[c] += cj * se[c];
       }
       if constexpr (AEC) tet_aec(X, sU, CAP, N, a, ctau, aec);
       if (i < C.z) {
@@ -541,13 +560,28 @@ void free_kozak(KozakMesh* k) {
   delete k;
 }
 
+xg::Prob prob_of(const xg_ctx* c) {
+  return xg::Prob{c->problem.kind, c->problem.alpha, c->problem.kappa, c->problem.p0,
+                  c->cfg.gamma};
+}
+
+// the rhs block kernel with or without the antidiffusive contributions and
+// the problem's source term
+template <bool AEC>
+void launch_kozak_blocks(xg_ctx* c, const double* U, double* out, double* slots,
+                         double dt, double ctau) {
+  KozakMesh* k = c->koz;
+  const xg::Prob P = prob_of(c);
+  auto kern = xg::has_source(P) ? k_kozak_blocks<AEC, true> : k_kozak_blocks<AEC, false>;
+  hipLaunchKernelGGL(kern, dim3(k->nblk), dim3(kKozThreads), 0, c->stream, k->nblk,
+                     k->blk0, k->blk1, k->halo, k->chk, k->chk_ent, k->tend, k->entries,
+                     k->tab, k->tcd, k->x, k->y, k->z, U, out, slots,
+                     c->cfg.gamma - 1.0, dt, ctau, P);
+}
+
 int launch_kozak(xg_ctx* c, double dt, const double* U, double* R) {
   KozakMesh* k = c->koz;
-  if (k->nblk)
-    hipLaunchKernelGGL(k_kozak_blocks<false>, dim3(k->nblk), dim3(kKozThreads), 0,
-                       c->stream, k->nblk, k->blk0, k->blk1, k->halo, k->chk,
-                       k->chk_ent, k->tend, k->entries, k->tab, k->tcd, k->x,
-                       k->y, k->z, U, R, k->slots, c->cfg.gamma - 1.0, dt, 0.0);
+  if (k->nblk) launch_kozak_blocks<false>(c, U, R, k->slots, dt, 0.0);
   if (k->nfx)
     hipLaunchKernelGGL(k_fixup<5>, dim3(nblocks(k->nfx)), dim3(kBlock), 0,
                        c->stream, k->nfx, k->fx_node, k->fx_off, k->fx_slot,
@@ -624,17 +658,25 @@ int xg_kozak_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
   return XG_OK;
 }
 
-int xg_kozak_rhs_dev(xg_ctx* c, int ncomp, double dt, const double* U, double* R) {
+int xg_kozak_rhs_dev(xg_ctx* c, int ncomp, double t, double dt, const double* tp,
+                     const double* dtp, const double* U, double* R) {
+  (void)t;   // the sources served are independent of time
   int rc = check_kozak(c, ncomp);
   if (rc) return rc;
+  if (tp || dtp)
+    return fail(c, XG_EUNSUPPORTED, "kozak::rhs on the device: no steady state (tp, dtp)");
   if (!U || !R) return fail(c, XG_EINVAL, "xg_kozak_rhs: null field");
   XG_HIP(c, hipSetDevice(c->device));
   return launch_kozak(c, dt, U, R);
 }
 
-int xg_kozak_rhs(xg_ctx* c, int ncomp, double dt, const double* U, double* R) {
+int xg_kozak_rhs(xg_ctx* c, int ncomp, double t, double dt, const double* tp,
+                 const double* dtp, const double* U, double* R) {
+  (void)t;
   int rc = check_kozak(c, ncomp);
   if (rc) return rc;
+  if (tp || dtp)
+    return fail(c, XG_EUNSUPPORTED, "kozak::rhs on the device: no steady state (tp, dtp)");
   if (!U || !R) return fail(c, XG_EINVAL, "xg_kozak_rhs: null field");
   XG_HIP(c, hipSetDevice(c->device));
   KozakMesh* k = c->koz;
@@ -748,22 +790,18 @@ int xg_kozcg_dt_dev(xg_ctx* c, const double* U, double cfl, double* dt) {
   return XG_OK;
 }
 
-int xg_kozcg_step_dev(xg_ctx* c, double dt, double* U, double* Un) {
+int xg_kozcg_step_dev(xg_ctx* c, double t, double dt, double* U, double* Un) {
+  (void)t;   // the sources served are independent of time
   if (!c) return XG_EINVAL;
   if (!c->koz || !c->koz->have_step) return fail(c, XG_ENOMESH, "xg_kozcg_setup not called");
   if (!U || !Un || U == Un) return fail(c, XG_EINVAL, "xg_kozcg_step: bad field pointers");
   XG_HIP(c, hipSetDevice(c->device));
   KozakMesh* k = c->koz;
-  const double gm1 = c->cfg.gamma - 1.0;
   const int n = k->npoin;
   const int64_t n5 = (int64_t)n * 5;
   const auto& f = k->fct;
   if (!f.fct) {
-    if (k->nblk)
-      hipLaunchKernelGGL(k_kozak_blocks<false>, dim3(k->nblk), dim3(kKozThreads), 0,
-                         c->stream, k->nblk, k->blk0, k->blk1, k->halo, k->chk,
-                         k->chk_ent, k->tend, k->entries, k->tab, k->tcd, k->x,
-                         k->y, k->z, U, k->W, k->S15, gm1, dt, 0.0);
+    if (k->nblk) launch_kozak_blocks<false>(c, U, k->W, k->S15, dt, 0.0);
     if (k->nfx)
       hipLaunchKernelGGL(k_fixup<5>, dim3(nblocks(k->nfx)), dim3(kBlock), 0,
                          c->stream, k->nfx, k->fx_node, k->fx_off, k->fx_slot,
@@ -774,11 +812,7 @@ int xg_kozcg_step_dev(xg_ctx* c, double dt, double* U, double* Un) {
     int sysmask = 0;
     for (int s = 0; s < f.nsys; ++s) sysmask |= 1 << f.sys[s];
     // rhs and P+/- (KozCG::rhs, aec)
-    if (k->nblk)
-      hipLaunchKernelGGL(k_kozak_blocks<true>, dim3(k->nblk), dim3(kKozThreads), 0,
-                         c->stream, k->nblk, k->blk0, k->blk1, k->halo, k->chk,
-                         k->chk_ent, k->tend, k->entries, k->tab, k->tcd, k->x,
-                         k->y, k->z, U, k->W, k->S15, gm1, dt, f.fctdif);
+    if (k->nblk) launch_kozak_blocks<true>(c, U, k->W, k->S15, dt, f.fctdif);
     if (k->nfx)
       hipLaunchKernelGGL(k_fixup<15>, dim3(nblocks(k->nfx)), dim3(kBlock), 0,
                          c->stream, k->nfx, k->fx_node, k->fx_off, k->fx_slot,

@@ -21,6 +21,7 @@
 
 #include "xg_flux.hpp"
 #include "xg_layout.hpp"
+#include "xg_problems.hpp"
 
 using xg::Cfg;
 
@@ -434,6 +435,135 @@ __global__ void k_update(int npoin, double coef_dt, double* __restrict__ U,
   U[i] = un - coef_dt * R[i] / vol[i / 5];
 }
 
+// U = Un - rkcoef*dtp[i]*R/vol with per-node time steps (steady runs,
+// RieCG.cpp:1017-1018)
+template <bool STAGE0>
+__global__ void k_update_steady(int npoin, double rk, const double* __restrict__ dtp,
+                                double* __restrict__ U, double* __restrict__ Un,
+                                const double* __restrict__ R,
+                                const double* __restrict__ vol) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)npoin * 5) return;
+  double un;
+  if (STAGE0) {
+    un = U[i];
+    Un[i] = un;
+  } else {
+    un = Un[i];
+  }
+  U[i] = un - rk * dtp[i / 5] * R[i] / vol[i / 5];
+}
+
+// tp += dtp (RieCG.cpp:1047-1050)
+__global__ void k_axpy1(int64_t n, const double* __restrict__ d, double* __restrict__ t) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) t[i] += d[i];
+}
+
+// F(i, :) /= vol[i] for NC components (RieCG.cpp:935-939)
+template <int NC>
+__global__ void k_div_vol(int npoin, double* __restrict__ F, const double* __restrict__ vol) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)npoin * NC) return;
+  F[i] /= vol[i / NC];
+}
+
+// riemann::src (Riemann.cpp:880-907): R(p,c) -= s_c(x_p) v[p]
+__global__ void k_src(int npoin, xg::Prob P, const double* __restrict__ x,
+                      const double* __restrict__ y, const double* __restrict__ z,
+                      const double* __restrict__ v, double* __restrict__ R) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npoin) return;
+  double s[5];
+  xg::source(P, x[p], y[p], z[p], s);
+  double* r = R + (size_t)p * 5;
+  const double vp = v[p];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) r[c] -= s[c] * vp;
+}
+
+// physics::dirbc (BC.cpp:29-72) as the RieCG chare calls it: components
+// with mask 1 take the node's initial-condition value
+__global__ void k_dirbc_ic(int nd, const int* __restrict__ node,
+                           const uint8_t* __restrict__ mask,
+                           const double* __restrict__ val, double* __restrict__ U) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nd) return;
+  double* u = U + (size_t)node[i] * 5;
+#pragma unroll
+  for (int c = 0; c < 5; ++c)
+    if (mask[i * 5 + c] == 1) u[c] = val[i * 5 + c];
+}
+
+// physics::farbc (BC.cpp:152-221), the (node, normal) pairs of a node in
+// list order
+__global__ void k_farbc(int nf, const int* __restrict__ node, const int* __restrict__ off,
+                        const double* __restrict__ nrm, double fr, double fu, double fv,
+                        double fw, double fp, double g, double* __restrict__ U) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  double* s = U + (size_t)node[i] * 5;
+  double r = s[0], ru = s[1], rv = s[2], rw = s[3], re = s[4];
+  const double a = sqrt(g * fp / fr);
+  for (int k = off[i]; k < off[i + 1]; ++k) {
+    const double* n = nrm + (size_t)k * 3;
+    const double M = (fu * n[0] + fv * n[1] + fw * n[2]) / a;
+    if (M <= -1.0) {              // supersonic inflow
+      r = fr; ru = fr * fu; rv = fr * fv; rw = fr * fw;
+      re = xg::total_energy(g, fr, fu, fv, fw, fp);
+    } else if (M > -1.0 && M < 0.0) {   // subsonic inflow: inside pressure
+      const double pr = (re - 0.5 * (ru * ru + rv * rv + rw * rw) / r) * (g - 1.0);
+      r = fr; ru = fr * fu; rv = fr * fv; rw = fr * fw;
+      re = xg::total_energy(g, fr, fu, fv, fw, pr);
+    } else if (M >= 0.0 && M < 1.0) {   // subsonic outflow: outside pressure
+      re = xg::total_energy(g, r, ru / r, rv / r, rw / r, fp);
+    }
+  }
+  s[0] = r; s[1] = ru; s[2] = rv; s[3] = rw; s[4] = re;
+}
+
+// physics::prebc (BC.cpp:223-241), a node's (density, pressure) records in
+// list order
+__global__ void k_prebc(int np, const int* __restrict__ node, const int* __restrict__ off,
+                        const double* __restrict__ val, double g, double* __restrict__ U) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  double* s = U + (size_t)node[i] * 5;
+  for (int k = off[i]; k < off[i + 1]; ++k) {
+    s[0] = val[k * 2 + 0];
+    s[4] = xg::total_energy(g, s[0], s[1] / s[0], s[2] / s[0], s[3] / s[0], val[k * 2 + 1]);
+  }
+}
+
+// steady RieCG::dt (RieCG.cpp:810-824): dtp per node and a block minimum
+__global__ __launch_bounds__(kBlock) void k_dt_steady(
+    int npoin, const double* __restrict__ U, const double* __restrict__ vol,
+    double g, double cfl, double* __restrict__ dtp, double* __restrict__ part) {
+  __shared__ double sm[kBlock / 64];
+  double m = 1.7976931348623157e308;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npoin;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const double* s = U + p * 5;
+    double r = s[0];
+    double u = s[1] / r, v = s[2] / r, w = s[3] / r;
+    double pr = (s[4] - 0.5 * r * (u * u + v * v + w * w)) * (g - 1.0);
+    double c = sqrt(g * fmax(pr, 0.0) / r);
+    double L = cbrt(vol[p]);
+    double vel = sqrt(u * u + v * v + w * w);
+    double d = L / fmax(vel + c, 1.0e-8) * cfl;
+    dtp[p] = d;
+    m = fmin(m, d);
+  }
+  m = wave_min(m);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = sm[0];
+    for (int k = 1; k < kBlock / 64; ++k) b = fmin(b, sm[k]);
+    part[blockIdx.x] = b;
+  }
+}
+
 // ------------------------------------------------------------- launchers --
 
 int check_mesh(xg_ctx* c, int ncomp) {
@@ -472,7 +602,13 @@ int launch_grad(xg_ctx* c, const double* U, double* G, bool div = false) {
   return XG_OK;
 }
 
-int launch_rhs(xg_ctx* c, const double* G, const double* U, double* R) {
+xg::Prob prob_of(const xg_ctx* c) {
+  return xg::Prob{c->problem.kind, c->problem.alpha, c->problem.kappa, c->problem.p0,
+                  c->cfg.gamma};
+}
+
+int launch_rhs(xg_ctx* c, const double* G, const double* U, double* R,
+               const double* v) {
   if (c->cfg.flux != 0 && c->cfg.flux != 1)
     return fail(c, XG_EINVAL, "Flux not configured");
   const double g = c->cfg.gamma;
@@ -487,7 +623,89 @@ int launch_rhs(xg_ctx* c, const double* G, const double* U, double* R) {
   if (B.nfx)
     hipLaunchKernelGGL(k_fixup<5>, dim3(nblocks(B.nfx)), dim3(kBlock), 0,
                        c->stream, B.nfx, B.fx_node, B.fx_off, B.fx_slot, B.slots, R);
+  const xg::Prob P = prob_of(c);
+  if (xg::has_source(P))
+    hipLaunchKernelGGL(k_src, dim3(nblocks(c->npoin)), dim3(kBlock), 0, c->stream,
+                       c->npoin, P, c->x, c->y, c->z, v, R);
   XG_HIP(c, hipGetLastError());
+  return XG_OK;
+}
+
+// RieCG::BC (RieCG.cpp:764-785): dirbc, symbc, farbc, prebc
+int launch_bc(xg_ctx* c, double* U) {
+  if (c->ndirnode)
+    hipLaunchKernelGGL(k_dirbc_ic, dim3(nblocks(c->ndirnode)), dim3(kBlock), 0,
+                       c->stream, c->ndirnode, c->dirnode, c->dirmask, c->dirval, U);
+  if (c->nsymnode)
+    hipLaunchKernelGGL(k_symbc, dim3(nblocks(c->nsymnode)), dim3(kBlock), 0,
+                       c->stream, c->nsymnode, c->symnode, c->symoff,
+                       c->symnorm, U, 5, 1);
+  if (c->nfarnode)
+    hipLaunchKernelGGL(k_farbc, dim3(nblocks(c->nfarnode)), dim3(kBlock), 0,
+                       c->stream, c->nfarnode, c->farnode, c->faroff, c->farnorm,
+                       c->far[0], c->far[1], c->far[2], c->far[3], c->far[4],
+                       c->cfg.gamma, U);
+  if (c->nprenode)
+    hipLaunchKernelGGL(k_prebc, dim3(nblocks(c->nprenode)), dim3(kBlock), 0,
+                       c->stream, c->nprenode, c->prenode, c->preoff, c->preval,
+                       c->cfg.gamma, U);
+  XG_HIP(c, hipGetLastError());
+  return XG_OK;
+}
+
+// RieCG::solve (RieCG.cpp:992-1057) after the rhs is complete
+int launch_solve(xg_ctx* c, int stage, double dt, const double* dtp, double* tp,
+                 double* U, double* Un, const double* R) {
+  static const double rkcoef[3] = {1.0 / 3.0, 1.0 / 2.0, 1.0};
+  const int64_t n5 = (int64_t)c->npoin * 5;
+  if (dtp) {
+    if (stage == 0)
+      hipLaunchKernelGGL(k_update_steady<true>, dim3(nblocks(n5)), dim3(kBlock), 0,
+                         c->stream, c->npoin, rkcoef[stage], dtp, U, Un, R, c->vol);
+    else
+      hipLaunchKernelGGL(k_update_steady<false>, dim3(nblocks(n5)), dim3(kBlock), 0,
+                         c->stream, c->npoin, rkcoef[stage], dtp, U, Un, R, c->vol);
+  } else if (stage == 0) {
+    hipLaunchKernelGGL(k_update<true>, dim3(nblocks(n5)), dim3(kBlock), 0, c->stream,
+                       c->npoin, rkcoef[stage] * dt, U, Un, R, c->vol);
+  } else {
+    hipLaunchKernelGGL(k_update<false>, dim3(nblocks(n5)), dim3(kBlock), 0, c->stream,
+                       c->npoin, rkcoef[stage] * dt, U, Un, R, c->vol);
+  }
+  int rc = launch_bc(c, U);
+  if (rc) return rc;
+  if (dtp && stage == 2)
+    hipLaunchKernelGGL(k_axpy1, dim3(nblocks(c->npoin)), dim3(kBlock), 0, c->stream,
+                       (int64_t)c->npoin, dtp, tp);
+  XG_HIP(c, hipGetLastError());
+  return XG_OK;
+}
+
+// (node, record) lists grouped by node, records of a node in list order
+template <int W>
+int group_by_node(xg_ctx* c, int64_t n, const size_t* nodes, const double* rec,
+                  std::vector<int>& node, std::vector<int>& off,
+                  std::vector<double>& out, const char* what) {
+  std::vector<int64_t> order(n);
+  for (int64_t i = 0; i < n; ++i) {
+    if (nodes[i] >= (size_t)c->npoin)
+      return fail(c, XG_EINVAL, std::string("xg_riecg_bc: ") + what + " node out of range");
+    order[i] = i;
+  }
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int64_t a, int64_t b) { return nodes[a] < nodes[b]; });
+  off.assign(1, 0);
+  node.clear();
+  out.clear();
+  for (int64_t k = 0; k < n; ++k) {
+    int64_t i = order[k];
+    if (k == 0 || nodes[order[k - 1]] != nodes[i]) {
+      if (k) off.push_back((int)(out.size() / W));
+      node.push_back((int)nodes[i]);
+    }
+    out.insert(out.end(), rec + i * W, rec + i * W + W);
+  }
+  if (n) off.push_back((int)(out.size() / W));
   return XG_OK;
 }
 
@@ -531,7 +749,7 @@ int stage_buffers(xg_ctx* c) {
   if (c->sU) { (void)hipFree(c->sU); (void)hipFree(c->sG); (void)hipFree(c->sR); }
   c->sU = c->sG = c->sR = nullptr; c->sbytes = 0;
   XG_HIP(c, hipMalloc((void**)&c->sU, need * 5));
-  XG_HIP(c, hipMalloc((void**)&c->sG, need * 15));
+  XG_HIP(c, hipMalloc((void**)&c->sG, need * 16));   // G and, after it, v
   XG_HIP(c, hipMalloc((void**)&c->sR, need * 5));
   c->sbytes = need;
   return XG_OK;
@@ -562,6 +780,7 @@ int xg_ctx_destroy(xg_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   dev_free_all(c->mesh_allocs);
   dev_free_all(c->riecg_allocs);
+  dev_free_all(c->bc_allocs);
   if (c->sU) { (void)hipFree(c->sU); (void)hipFree(c->sG); (void)hipFree(c->sR); }
   if (c->cg) (void)xg_cg_destroy(c);
   if (c->koz) xg_kozak_free(c);
@@ -719,12 +938,16 @@ int xg_riemann_grad_dev(xg_ctx* c, int ncomp, const double* U, double* G) {
 }
 
 int xg_riemann_rhs_dev(xg_ctx* c, int ncomp, const double* G, const double* U,
-                       double* R) {
+                       const double* v, double t, const double* tp, double* R) {
+  (void)t;   // the sources served do not depend on time
+  (void)tp;
   int rc = check_mesh(c, ncomp);
   if (rc) return rc;
   if (!U || !G || !R) return fail(c, XG_EINVAL, "xg_riemann_rhs: null field");
+  if (xg::has_source(prob_of(c)) && !v)
+    return fail(c, XG_EINVAL, "xg_riemann_rhs: the problem's source needs the nodal volumes v");
   XG_HIP(c, hipSetDevice(c->device));
-  return launch_rhs(c, G, U, R);
+  return launch_rhs(c, G, U, R, v);
 }
 
 int xg_riemann_grad(xg_ctx* c, int ncomp, const double* U, double* G) {
@@ -742,16 +965,27 @@ int xg_riemann_grad(xg_ctx* c, int ncomp, const double* U, double* G) {
 }
 
 int xg_riemann_rhs(xg_ctx* c, int ncomp, const double* G, const double* U,
-                   double* R) {
+                   const double* v, double t, const double* tp, double* R) {
+  (void)t;   // the sources served do not depend on time
+  (void)tp;
   int rc = check_mesh(c, ncomp);
   if (rc) return rc;
   if (!U || !G || !R) return fail(c, XG_EINVAL, "xg_riemann_rhs: null field");
+  const bool src = xg::has_source(prob_of(c));
+  if (src && !v)
+    return fail(c, XG_EINVAL, "xg_riemann_rhs: the problem's source needs the nodal volumes v");
   XG_HIP(c, hipSetDevice(c->device));
   if ((rc = stage_buffers(c))) return rc;
   size_t n = (size_t)c->npoin;
   XG_HIP(c, hipMemcpyAsync(c->sU, U, n * 40, hipMemcpyHostToDevice, c->stream));
   XG_HIP(c, hipMemcpyAsync(c->sG, G, n * 120, hipMemcpyHostToDevice, c->stream));
-  if ((rc = launch_rhs(c, c->sG, c->sU, c->sR))) return rc;
+  // the volumes are staged after G in the G staging buffer (16 doubles per node)
+  double* sv = nullptr;
+  if (src) {
+    sv = c->sG + n * 15;
+    XG_HIP(c, hipMemcpyAsync(sv, v, n * 8, hipMemcpyHostToDevice, c->stream));
+  }
+  if ((rc = launch_rhs(c, c->sG, c->sU, c->sR, sv))) return rc;
   XG_HIP(c, hipMemcpyAsync(R, c->sR, n * 40, hipMemcpyDeviceToHost, c->stream));
   XG_HIP(c, hipStreamSynchronize(c->stream));
   return XG_OK;
@@ -791,6 +1025,8 @@ int xg_riecg_setup(xg_ctx* c, const double* vol, int64_t nsym,
   XG_HIP(c, hipSetDevice(c->device));
   (void)hipStreamSynchronize(c->stream);
   dev_free_all(c->riecg_allocs);
+  dev_free_all(c->bc_allocs);
+  c->ndirnode = c->nfarnode = c->nprenode = 0;
   c->have_riecg = false;
   int rc;
   auto& L = c->riecg_allocs;
@@ -830,29 +1066,125 @@ int xg_riecg_dt_dev(xg_ctx* c, const double* U, double cfl, double* dt) {
   return XG_OK;
 }
 
-int xg_riecg_stage_dev(xg_ctx* c, int stage, double dt, double* U, double* Un,
-                       double* G, double* R) {
-  static const double rkcoef[3] = {1.0 / 3.0, 1.0 / 2.0, 1.0};
+int xg_riecg_dt_steady_dev(xg_ctx* c, const double* U, double cfl, double* dtp,
+                           double* dt) {
+  if (!c) return XG_EINVAL;
+  if (!c->have_riecg) return fail(c, XG_ENOMESH, "xg_riecg_setup not called");
+  if (!U || !dtp || !dt) return fail(c, XG_EINVAL, "xg_riecg_dt_steady: null argument");
+  XG_HIP(c, hipSetDevice(c->device));
+  int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, nblocks(c->npoin)));
+  hipLaunchKernelGGL(k_dt_steady, dim3(nb), dim3(kBlock), 0, c->stream, c->npoin, U,
+                     c->vol, c->cfg.gamma, cfl, dtp, c->red);
+  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(kRedBlocks), 0, c->stream, nb,
+                     c->red, 1.0, c->red + kRedBlocks);
+  XG_HIP(c, hipGetLastError());
+  XG_HIP(c, hipMemcpyAsync(dt, c->red + kRedBlocks, sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+  XG_HIP(c, hipStreamSynchronize(c->stream));
+  if (*dt < 2.220446049250313e-16)
+    return fail(c, XG_EBLOWUP, "time step below machine epsilon");
+  return XG_OK;
+}
+
+int xg_riecg_bc(xg_ctx* c, int64_t ndir, const size_t* dirnodes, const int32_t* dirmask,
+                const double* dirvals, int64_t nfar, const size_t* farnodes,
+                const double* farnorms, const double far[5], int64_t npre,
+                const size_t* prenodes, const double* prevals) {
+  if (!c) return XG_EINVAL;
+  if (!c->have_riecg) return fail(c, XG_ENOMESH, "xg_riecg_setup not called");
+  if (ndir < 0 || nfar < 0 || npre < 0 ||
+      (ndir && (!dirnodes || !dirmask || !dirvals)) ||
+      (nfar && (!farnodes || !farnorms || !far)) || (npre && (!prenodes || !prevals)))
+    return fail(c, XG_EINVAL, "xg_riecg_bc: bad arguments");
+  std::vector<int> dn(ndir);
+  std::vector<uint8_t> dm(ndir * 5);
+  for (int64_t i = 0; i < ndir; ++i) {
+    if (dirnodes[i] >= (size_t)c->npoin)
+      return fail(c, XG_EINVAL, "xg_riecg_bc: Dirichlet node out of range");
+    dn[i] = (int)dirnodes[i];
+    for (int k = 0; k < 5; ++k) dm[i * 5 + k] = (uint8_t)(dirmask[i * 5 + k] == 1);
+  }
+  std::vector<int> fnode, foff, pnode, poff;
+  std::vector<double> fnrm, pval;
+  int rc;
+  if ((rc = group_by_node<3>(c, nfar, farnodes, farnorms, fnode, foff, fnrm, "farfield")) ||
+      (rc = group_by_node<2>(c, npre, prenodes, prevals, pnode, poff, pval, "pressure")))
+    return rc;
+  if (nfar && !(far[0] > 0.0 && far[4] > 0.0))
+    return fail(c, XG_EINVAL, "xg_riecg_bc: farfield density and pressure must be positive");
+  XG_HIP(c, hipSetDevice(c->device));
+  (void)hipStreamSynchronize(c->stream);
+  dev_free_all(c->bc_allocs);
+  c->ndirnode = c->nfarnode = c->nprenode = 0;
+  auto& L = c->bc_allocs;
+  if ((rc = upload(c, L, &c->dirnode, dn)) || (rc = upload(c, L, &c->dirmask, dm)) ||
+      (rc = upload(c, L, &c->dirval, dirvals, (size_t)ndir * 5)) ||
+      (rc = upload(c, L, &c->farnode, fnode)) || (rc = upload(c, L, &c->faroff, foff)) ||
+      (rc = upload(c, L, &c->farnorm, fnrm)) || (rc = upload(c, L, &c->prenode, pnode)) ||
+      (rc = upload(c, L, &c->preoff, poff)) || (rc = upload(c, L, &c->preval, pval)))
+    return rc;
+  if (nfar)
+    for (int k = 0; k < 5; ++k) c->far[k] = far[k];
+  c->ndirnode = (int)ndir;
+  c->nfarnode = (int)fnode.size();
+  c->nprenode = (int)pnode.size();
+  return XG_OK;
+}
+
+int xg_riecg_apply_bc_dev(xg_ctx* c, double* U) {
+  if (!c) return XG_EINVAL;
+  if (!c->have_riecg) return fail(c, XG_ENOMESH, "xg_riecg_setup not called");
+  if (!U) return fail(c, XG_EINVAL, "xg_riecg_apply_bc: null field");
+  XG_HIP(c, hipSetDevice(c->device));
+  return launch_bc(c, U);
+}
+
+int xg_riecg_stage_dev(xg_ctx* c, int stage, double t, double dt, const double* dtp,
+                       double* tp, double* U, double* Un, double* G, double* R) {
+  (void)t;   // BC values and sources served are independent of time
   if (!c) return XG_EINVAL;
   if (!c->have_riecg) return fail(c, XG_ENOMESH, "xg_riecg_setup not called");
   if (stage < 0 || stage > 2) return fail(c, XG_EINVAL, "stage must be 0, 1 or 2");
   if (!U || !Un || !G || !R) return fail(c, XG_EINVAL, "xg_riecg_stage: null field");
+  if ((dtp == nullptr) != (tp == nullptr))
+    return fail(c, XG_EINVAL, "xg_riecg_stage: dtp and tp go together");
   XG_HIP(c, hipSetDevice(c->device));
   int rc;
   if ((rc = launch_grad(c, U, G, true))) return rc;
-  const int64_t n5 = (int64_t)c->npoin * 5;
-  if ((rc = launch_rhs(c, G, U, R))) return rc;
-  if (stage == 0)
-    hipLaunchKernelGGL(k_update<true>, dim3(nblocks(n5)), dim3(kBlock), 0, c->stream,
-                       c->npoin, rkcoef[stage] * dt, U, Un, R, c->vol);
-  else
-    hipLaunchKernelGGL(k_update<false>, dim3(nblocks(n5)), dim3(kBlock), 0, c->stream,
-                       c->npoin, rkcoef[stage] * dt, U, Un, R, c->vol);
-  if (c->nsymnode)
-    hipLaunchKernelGGL(k_symbc, dim3(nblocks(c->nsymnode)), dim3(kBlock), 0,
-                       c->stream, c->nsymnode, c->symnode, c->symoff,
-                       c->symnorm, U, 5, 1);
+  if ((rc = launch_rhs(c, G, U, R, c->vol))) return rc;
+  return launch_solve(c, stage, dt, dtp, tp, U, Un, R);
+}
+
+int xg_riecg_grad_div_dev(xg_ctx* c, double* G) {
+  if (!c) return XG_EINVAL;
+  if (!c->have_riecg) return fail(c, XG_ENOMESH, "xg_riecg_setup not called");
+  if (!G) return fail(c, XG_EINVAL, "xg_riecg_grad_div: null field");
+  XG_HIP(c, hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_div_vol<15>, dim3(nblocks((int64_t)c->npoin * 15)), dim3(kBlock), 0,
+                     c->stream, c->npoin, G, c->vol);
   XG_HIP(c, hipGetLastError());
+  return XG_OK;
+}
+
+int xg_riecg_solve_dev(xg_ctx* c, int stage, double t, double dt, const double* dtp,
+                       double* tp, double* U, double* Un, const double* R) {
+  (void)t;
+  if (!c) return XG_EINVAL;
+  if (!c->have_riecg) return fail(c, XG_ENOMESH, "xg_riecg_setup not called");
+  if (stage < 0 || stage > 2) return fail(c, XG_EINVAL, "stage must be 0, 1 or 2");
+  if (!U || !Un || !R) return fail(c, XG_EINVAL, "xg_riecg_solve: null field");
+  if ((dtp == nullptr) != (tp == nullptr))
+    return fail(c, XG_EINVAL, "xg_riecg_solve: dtp and tp go together");
+  XG_HIP(c, hipSetDevice(c->device));
+  return launch_solve(c, stage, dt, dtp, tp, U, Un, R);
+}
+
+int xg_set_problem(xg_ctx* c, const xg_problem* p) {
+  if (!c || !p) return XG_EINVAL;
+  if (p->kind != XG_PROBLEM_NONE && p->kind != XG_PROBLEM_VORTICAL_FLOW &&
+      p->kind != XG_PROBLEM_TAYLOR_GREEN)
+    return fail(c, XG_EUNSUPPORTED, "xg_set_problem: no device source for this problem");
+  c->problem = *p;
   return XG_OK;
 }
 

@@ -6,7 +6,7 @@ import numpy as np
 
 from ._xg import lib, ptr
 
-__all__ = ["rhocompute", "precompute"]
+__all__ = ["rhocompute", "errcompute", "precompute"]
 
 
 def rhocompute(ctx, U, Un, V):
@@ -15,6 +15,17 @@ def rhocompute(ctx, U, Un, V):
     n, nc = int(U.shape[0]), int(U.shape[1])
     out = np.empty(2 * nc + 1)
     ctx._check(lib().xg_diag_rho_dev(ctx._h, n, nc, ptr(U), ptr(Un), ptr(V), ptr(out)))
+    return out
+
+
+def errcompute(ctx, U, V, An):
+    """The analytic-error sums of rhocompute: (nu_c - An_c)^2 V (5 values)
+    and |nu_c - An_c| V (5 values), nu the primitive form of U (rho, u, v,
+    w, specific internal energy) and An the analytic solution in that form;
+    U, An (npoin, 5), V (npoin,) device tensors."""
+    out = np.empty(10)
+    ctx._check(lib().xg_diag_err_dev(ctx._h, int(U.shape[0]), 5, ptr(U), ptr(V), ptr(An),
+                                     ptr(out)))
     return out
 
 

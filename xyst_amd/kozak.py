@@ -24,22 +24,24 @@ def upload(ctx, coord, inpoel):
     ctx.koz_npoin = x.size
 
 
-def rhs(ctx, dt, U):
+def rhs(ctx, dt, U, t=0.0):
     """R = kozak::rhs(U) for time step dt; host arrays (npoin, 5)."""
     U = np.ascontiguousarray(U, dtype=np.float64)
     if U.shape != (ctx.koz_npoin, 5):
         raise ValueError("U must be (%d, 5)" % ctx.koz_npoin)
     R = np.empty_like(U)
-    ctx._check(lib().xg_kozak_rhs(ctx._h, 5, float(dt), ptr(U), ptr(R)))
+    ctx._check(lib().xg_kozak_rhs(ctx._h, 5, float(t), float(dt), None, None, ptr(U),
+                                  ptr(R)))
     return R
 
 
-def rhs_dev(ctx, dt, U, R):
+def rhs_dev(ctx, dt, U, R, t=0.0):
     """Device tensors U, R (npoin, 5) float64; enqueued on the context stream."""
     for a in (U, R):
         if tuple(a.shape) != (ctx.koz_npoin, 5) or not a.is_cuda:
             raise ValueError("device fields must be (%d, 5) on the GPU" % ctx.koz_npoin)
-    ctx._check(lib().xg_kozak_rhs_dev(ctx._h, 5, float(dt), ptr(U), ptr(R)))
+    ctx._check(lib().xg_kozak_rhs_dev(ctx._h, 5, float(t), float(dt), None, None, ptr(U),
+                                      ptr(R)))
 
 
 def setup(ctx, vol, symnodes, symnorms, dirnodes=(), dirmask=None, dirvals=None,
@@ -75,10 +77,10 @@ def dt_dev(ctx, U, cfl):
     return out.value
 
 
-def step_dev(ctx, dt, U, Un):
+def step_dev(ctx, dt, U, Un, t=0.0):
     """One KozCG step on device tensors (npoin, 5): U advanced in place, Un
     receives the old solution; enqueued on the context stream."""
     for a in (U, Un):
         if tuple(a.shape) != (ctx.koz_npoin, 5) or not a.is_cuda:
             raise ValueError("device fields must be (%d, 5) on the GPU" % ctx.koz_npoin)
-    ctx._check(lib().xg_kozcg_step_dev(ctx._h, float(dt), ptr(U), ptr(Un)))
+    ctx._check(lib().xg_kozcg_step_dev(ctx._h, float(t), float(dt), ptr(U), ptr(Un)))

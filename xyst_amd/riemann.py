@@ -186,12 +186,22 @@ class Context:
         self._check(lib().xg_riemann_grad(self._h, U.shape[1], ptr(U), ptr(G)))
         return G
 
-    def rhs(self, G, U):
+    def set_problem(self, name="none", alpha=0.1, kappa=1.0, p0=10.0):
+        """The problem whose source term riemann::rhs adds (problems::SRC())."""
+        p = _xg.Problem(_xg.PROBLEMS[name], alpha, kappa, p0)
+        self._check(lib().xg_set_problem(self._h, C.byref(p)))
+
+    def rhs(self, G, U, v=None, t=0.0, tp=None):
+        """riemann::rhs(..., G, U, v, t, tp, R): v the nodal volumes of this
+        partition's elements (needed when the problem has a source)."""
         U, G = _f64(U), _f64(G)
         if self.npoin:
             self._fields(U, G)
+        va = _f64(v) if v is not None else None
+        ta = _f64(tp) if tp is not None else None
         R = np.empty_like(U)
-        self._check(lib().xg_riemann_rhs(self._h, U.shape[1], ptr(G), ptr(U), ptr(R)))
+        self._check(lib().xg_riemann_rhs(self._h, U.shape[1], ptr(G), ptr(U), ptr(va),
+                                         t, ptr(ta), ptr(R)))
         return R
 
     # HBM-resident variants: torch tensors on the GPU
@@ -199,9 +209,10 @@ class Context:
         self._fields(U, G, dev=True)
         self._check(lib().xg_riemann_grad_dev(self._h, U.shape[1], ptr(U), ptr(G)))
 
-    def rhs_dev(self, G, U, R):
+    def rhs_dev(self, G, U, R, v=None, t=0.0, tp=None):
         self._fields(U, G, R, dev=True)
-        self._check(lib().xg_riemann_rhs_dev(self._h, U.shape[1], ptr(G), ptr(U), ptr(R)))
+        self._check(lib().xg_riemann_rhs_dev(self._h, U.shape[1], ptr(G), ptr(U), ptr(v),
+                                             t, ptr(tp), ptr(R)))
 
     # resident RieCG stepping
     def riecg_setup(self, vol, symnodes, symnorms):
@@ -220,8 +231,54 @@ class Context:
         self._check(lib().xg_riecg_dt_dev(self._h, ptr(U), cfl, C.byref(out)))
         return out.value
 
-    def stage_dev(self, stage, dt, U, Un, G, R):
+    def riecg_bc(self, dirnodes=(), dirmask=(), dirvals=(), farnodes=(), farnorms=(),
+                 far=(1.0, 0.0, 0.0, 0.0, 1.0), prenodes=(), prevals=()):
+        """RieCG's Dirichlet (mask 1: the given initial-condition values),
+        farfield and pressure BC lists (RieCG::setupBC)."""
+        dn = _idx(dirnodes).reshape(-1)
+        dm = np.ascontiguousarray(dirmask, dtype=np.int32).reshape(-1)
+        dv = _f64(dirvals).reshape(-1)
+        fn = _idx(farnodes).reshape(-1)
+        fv = _f64(farnorms).reshape(-1)
+        fa = _f64(far).reshape(-1)
+        pn = _idx(prenodes).reshape(-1)
+        pv = _f64(prevals).reshape(-1)
+        if dm.size != 5 * dn.size or dv.size != 5 * dn.size or fv.size != 3 * fn.size \
+                or pv.size != 2 * pn.size or fa.size != 5:
+            raise ValueError("riecg_bc: list sizes disagree")
+        self._check(lib().xg_riecg_bc(self._h, dn.size, ptr(dn) if dn.size else None,
+                                      ptr(dm) if dm.size else None,
+                                      ptr(dv) if dv.size else None, fn.size,
+                                      ptr(fn) if fn.size else None,
+                                      ptr(fv) if fv.size else None, ptr(fa), pn.size,
+                                      ptr(pn) if pn.size else None,
+                                      ptr(pv) if pv.size else None))
+
+    def apply_bc_dev(self, U):
+        """RieCG::BC on a device state (dirbc, symbc, farbc, prebc)."""
+        self._fields(U, dev=True)
+        self._check(lib().xg_riecg_apply_bc_dev(self._h, ptr(U)))
+
+    def dt_steady_dev(self, U, cfl, dtp):
+        """Steady RieCG::dt: fills dtp (device, npoin), returns its minimum."""
+        self._fields(U, dev=True)
+        out = C.c_double(0.0)
+        self._check(lib().xg_riecg_dt_steady_dev(self._h, ptr(U), cfl, ptr(dtp),
+                                                 C.byref(out)))
+        return out.value
+
+    def stage_dev(self, stage, dt, U, Un, G, R, t=0.0, dtp=None, tp=None):
         self._fields(U, G, R, dev=True)
         self._fields(Un, dev=True)
-        self._check(lib().xg_riecg_stage_dev(self._h, stage, dt, ptr(U), ptr(Un),
-                                             ptr(G), ptr(R)))
+        self._check(lib().xg_riecg_stage_dev(self._h, stage, t, dt, ptr(dtp), ptr(tp),
+                                             ptr(U), ptr(Un), ptr(G), ptr(R)))
+
+    def grad_div_dev(self, G):
+        """G /= the summed nodal volume (after the partitions' exchange)."""
+        self._check(lib().xg_riecg_grad_div_dev(self._h, ptr(G)))
+
+    def solve_dev(self, stage, dt, U, Un, R, t=0.0, dtp=None, tp=None):
+        """RieCG::solve with a complete rhs R: update and BCs."""
+        self._fields(U, R=R, dev=True)
+        self._check(lib().xg_riecg_solve_dev(self._h, stage, t, dt, ptr(dtp), ptr(tp),
+                                             ptr(U), ptr(Un), ptr(R)))
