@@ -3,7 +3,7 @@
 Workload (the throughput configuration): 160^3 hex-split unit box, each hex
 split into 6 tets (24,576,000 tets, 4,173,281 nodes, 28,902,880 edges),
 nodes jittered +-10% of the spacing from a fixed seed, RCM-renumbered and
-then grouped into compact cells of 2048 nodes (RCM order kept inside); FP64
+then grouped into compact cells of 1024 nodes (RCM order kept inside); FP64
 Sedov blast state (energy 1.0 deposited in the nodes within r < 0.02 of the
 centre, rho = 1, p = 1e-6 elsewhere), gamma 1.4, Rusanov flux, symmetry walls.
 
@@ -94,6 +94,29 @@ def poisson_setup(n, seed=20261202, modes=8):
     return A, b, {int(i): [(1, 0.0)] for i in dirnodes}
 
 
+def bc_residual(A, b, dirnodes):
+    """||b' - A' x|| / ||b'|| of the system the solvers solve: the Dirichlet
+    rows and columns of A replaced by the identity and b' = 0 there (the
+    Dirichlet values are 0), recomputed in FP64 on the host with scipy from
+    the solution the device returns -- independent of every solver's own
+    residual."""
+    import scipy.sparse as sp
+    n = A.nrow
+    M = sp.csr_matrix((A.a, A.ja, A.ia), shape=(n, n))
+    dn = np.asarray(dirnodes, np.int64)
+    bb = b.copy()
+    bb[dn] = 0.0
+    nb = np.linalg.norm(bb)
+
+    def residual(x):
+        xm = x.copy()
+        xm[dn] = 0.0
+        r = bb - M @ xm
+        r[dn] = -x[dn]
+        return float(np.linalg.norm(r) / nb)
+    return residual
+
+
 def poisson_bench(n, stream, reps=2):
     """ms per pressure-Poisson solve to 1e-10: FP64 Jacobi-PCG (the
     reference algorithm on the device), the FP32-inner solve with FP64
@@ -103,11 +126,14 @@ def poisson_bench(n, stream, reps=2):
     iterations, with A and b resident on the device; the multigrid
     hierarchy is built on the host once per matrix and Dirichlet set (as
     ChoCG's matrix is), its build time is reported beside, not in, the
-    per-solve time."""
+    per-solve time.  Every solver reports the true relative residual
+    recomputed on the host (bc_residual) and the relative error of its
+    solution against the FP64 Jacobi-PCG solution."""
     import torch
     from xyst_amd import cg as CG
     from xyst_amd import riemann as X
     A, b, dirbc = poisson_setup(n)
+    residual = bc_residual(A, b, sorted(dirbc))
     ctx = X.Context(0, stream=stream)
     cg = CG.ConjugateGradients(A, ctx=ctx)
     x0 = np.zeros(A.nrow)
@@ -133,8 +159,15 @@ def poisson_bench(n, stream, reps=2):
             ev1.record(stream)
             torch.cuda.synchronize()
             ms.append(ev0.elapsed_time(ev1))
+        x = cg.solution()
+        if name == "fp64_jacobi_pcg":
+            x_ref = x
         out[name] = {"ms": min(ms), "iterations": int(cg.it),
-                     "converged": bool(cg.converged), "normr_over_normb": cg.normr / cg.normb}
+                     "converged": bool(cg.converged),
+                     "solver_normr_over_normb": cg.normr / cg.normb,
+                     "true_residual_rel": residual(x),
+                     "err_vs_fp64_pcg_rel": float(np.linalg.norm(x - x_ref) /
+                                                  np.linalg.norm(x_ref))}
         if name == "mixed_fp32_fp64":
             out[name]["outer"] = int(cg.nouter)
         if name == "fcg_fp64_cg32_inner":
@@ -143,7 +176,9 @@ def poisson_bench(n, stream, reps=2):
             lv, oc, sms = cg.amg_info()
             out[name].update({"first_solve_ms_with_setup": ms[0], "levels": lv,
                               "operator_complexity": oc, "host_setup_ms": sms})
-        log("%s: %.2f ms, %d it, %s" % (name, min(ms), cg.it, cg.converged))
+        log("%s: %.2f ms, %d it, %s, true residual %.2e, error vs FP64 PCG %.2e"
+            % (name, min(ms), cg.it, cg.converged, out[name]["true_residual_rel"],
+               out[name]["err_vs_fp64_pcg_rel"]))
     out["speedup_mixed"] = out["fp64_jacobi_pcg"]["ms"] / out["mixed_fp32_fp64"]["ms"]
     out["speedup_fcg_cg32"] = out["fp64_jacobi_pcg"]["ms"] / out["fcg_fp64_cg32_inner"]["ms"]
     out["speedup_fcg_amg"] = out["fp64_jacobi_pcg"]["ms"] / out["fcg_fp64_amg"]["ms"]
@@ -300,16 +335,40 @@ def copy_bandwidth(stream, nbytes=1 << 30, reps=10):
             "ms": ms, "frac_of_peak": 2.0 * nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
+RHS_SOURCES = ("xyst_amd/csrc/xg_riemann.hip", "xyst_amd/csrc/xg_flux.hpp",
+               "xyst_amd/csrc/xg_blocks.hpp", "xyst_amd/csrc/xg_layout.cpp",
+               "xyst_amd/csrc/xg_layout.hpp")
+
+
+def rhs_sources_sha16():
+    """Hash of the sources the rhs kernels and their layout are built from."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in RHS_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def read_traffic(path):
     """HBM bytes per rhs launch measured by rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE per dispatch, each scaled by what the counter reads for a copy
-    of known size), as recorded in the profile summary; None if absent."""
+    of known size), as recorded in the profile summary together with the hash
+    of the rhs sources it was measured on; None if absent or if the sources
+    have changed since (the measurement would be stale)."""
     if not os.path.exists(path):
         return None
+    traffic, sha = None, None
     for line in open(path):
         if line.startswith("traffic_bytes_per_rhs:"):
-            return float(line.split(":", 1)[1])
-    return None
+            traffic = float(line.split(":", 1)[1])
+        if line.startswith("rhs_sources_sha16:"):
+            sha = line.split(":", 1)[1].strip()
+    if sha != rhs_sources_sha16():
+        log("rhs traffic in %s was measured on other rhs sources: not reported"
+            % os.path.basename(path))
+        return None
+    return traffic
 
 
 def cpu_threads():
@@ -322,23 +381,20 @@ def cpu_threads():
     return max(1, n)
 
 
-def cpu_baseline(n_sample=64, reps=40, reps_mt=200):
+def cpu_baseline(m, edge, dint, besym, U, vol, reps=2, reps_mt=10):
     """The oracle's riemann::rhs (the reference's loop structure and
-    superedge grouping) on an n_sample^3 box of the same workload family:
-    one thread, and all of this process's cores with one mesh partition per
+    superedge grouping) on the bench's own mesh and state (the same
+    renumbered 160^3 box, Sedov state, symmetry walls): one thread, and all
+    of this process's cores with one contiguous superedge partition per
     thread and the per-node sum of the partitions (the reference's one chare
-    per core).  `value` is the all-cores rate."""
+    per core).  `value` is the all-cores rate.  The superedges are grouped by
+    the oracle from the bench's edge list."""
     import oracle as O
-    from xyst_amd.meshgen import box_mesh, sedov_state
-    m = box_mesh(n_sample, seed=3)
-    vol = O.vol(m["coord"], m["inpoel"])
-    edge, dint = O.domint(m["coord"], m["inpoel"])
+    t0 = time.time()
     se = O.domsuped(m["inpoel"], edge, dint)
-    _, _, besym = O.symbc_setup(m["coord"], m["triinpoel"], m["faceset"],
-                                [1, 2, 3, 4, 5, 6])
-    U = sedov_state(m["coord"], vol)
     G = O.grad(se, m["coord"], m["triinpoel"], U) / vol[:, None]
     cfg = O.cfg()
+    log("cpu baseline setup: %.1fs" % (time.time() - t0))
     t = time.perf_counter()
     for _ in range(reps):
         O.rhs(cfg, se, m["coord"], m["triinpoel"], besym, G, U)
@@ -351,9 +407,9 @@ def cpu_baseline(n_sample=64, reps=40, reps_mt=200):
     tT = (time.perf_counter() - t) / reps_mt
     E = edge.shape[0]
     return {"value": E / tT, "unit": "edges/s", "cores": T, "kind": "port",
-            "sample": "oracle riemann::rhs on a %d^3 box (%d edges, Sedov state): "
-                      "%d threads, one mesh partition each, mean of %d evaluations; "
-                      "one thread, mean of %d evaluations" % (n_sample, E, T, reps_mt, reps),
+            "sample": "oracle riemann::rhs on the bench's %d-edge mesh and Sedov state: "
+                      "%d threads, one superedge partition each, mean of %d evaluations; "
+                      "one thread, mean of %d evaluations" % (E, T, reps_mt, reps),
             "single_core_value": E / t1}
 
 
@@ -367,8 +423,9 @@ def main():
     ap.add_argument("--order", default="rcm+block", choices=["rcm", "rcm+block"])
     ap.add_argument("--block-cell", type=int, default=1024,
                     help="nodes per cell of the RCB block order (rcm+block)")
-    ap.add_argument("--poisson-n", type=int, default=128,
-                    help="box size of the pressure-Poisson solve (0: skip)")
+    ap.add_argument("--poisson-n", default="128,192",
+                    help="box sizes of the pressure-Poisson solves, comma separated "
+                         "(0: skip)")
     ap.add_argument("--kozcg-steps", type=int, default=20,
                     help="timed kozak::rhs / KozCG steps on the same mesh (0: skip)")
     ap.add_argument("--chocg-n", type=int, default=128,
@@ -476,20 +533,23 @@ def main():
         del a, b
     if args.kozcg_steps > 0:
         out["kozcg"] = kozcg_bench(ctx, m, vol, sn, sv, stream, args.kozcg_steps)
-    if args.poisson_n > 0:
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(m, edge, dint, besym, U, vol)
+    psizes = [int(v) for v in str(args.poisson_n).split(",") if int(v) > 0]
+    if psizes:
         ctx.close()
         del dG, dR, dU, dUn
         torch.cuda.empty_cache()
-        out["poisson"] = poisson_bench(args.poisson_n, stream)
+    for k, pn in enumerate(psizes):
+        torch.cuda.empty_cache()
+        out["poisson" if k == 0 else "poisson_%d" % pn] = poisson_bench(pn, stream)
     if args.chocg_n > 0 and args.chocg_pc in ("both", "jacobi"):
         torch.cuda.empty_cache()
         out["chocg"] = chocg_bench(args.chocg_n, stream, args.chocg_steps)
     if args.chocg_n > 0 and args.chocg_pc in ("both", "amg"):
         torch.cuda.empty_cache()
         out["chocg_amg"] = chocg_bench(args.chocg_n, stream, args.chocg_steps, p_pc="amg")
-    if not args.no_cpu_baseline:
-        log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
     ctx.close()
 

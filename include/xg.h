@@ -69,6 +69,10 @@ int xg_set_config(xg_ctx* ctx, const xg_config* cfg);
 /* The problem of riemann::rhs's and kozak::rhs's source term (default
  * XG_PROBLEM_NONE); XG_EUNSUPPORTED for a kind the device does not have. */
 int xg_set_problem(xg_ctx* ctx, const xg_problem* problem);
+/* The problem set by xg_set_problem (a caller holding a source term checks
+ * that the device adds the same one, as problems::SRC() is queried in
+ * Riemann.cpp:895 and Kozak.cpp:62). */
+int xg_get_problem(const xg_ctx* ctx, xg_problem* problem);
 
 /* ---- host preprocessing (native C++, no device work) --------------------- */
 

@@ -134,38 +134,32 @@ int64_t orc_chocg_domint(int64_t npoin, const double* x, const double* y,
 
 /* ------------------------------------------------------------- domsuped -- */
 
-static int64_t edge_find(int64_t nedge, const int64_t* edge, int64_t a,
+/* Edge id of (a, b), a < b, in the (p, q)-sorted unique edge list, found by
+ * a binary search within row a (row offsets from edge_rows). */
+static int64_t edge_find(const int64_t* row, const int64_t* edge, int64_t a,
                          int64_t b) {
-  int64_t lo = 0, hi = nedge - 1;
+  int64_t lo = row[a], hi = row[a+1] - 1;
   while (lo <= hi) {
     int64_t m = (lo + hi) / 2;
-    int64_t p = edge[m*2], q = edge[m*2+1];
-    if (p == a && q == b) return m;
-    if (p < a || (p == a && q < b)) lo = m + 1; else hi = m - 1;
+    int64_t q = edge[m*2+1];
+    if (q == b) return m;
+    if (q < b) lo = m + 1; else hi = m - 1;
   }
   return -1;
 }
 
-typedef struct { int64_t k[3]; int64_t n[3]; int64_t rec; } facerec;
-
-static int facerec_cmp(const void* a, const void* b) {
-  const facerec* u = (const facerec*)a; const facerec* v = (const facerec*)b;
-  for (int i = 0; i < 3; ++i)
-    if (u->k[i] != v->k[i]) return u->k[i] < v->k[i] ? -1 : 1;
-  return (u->rec > v->rec) - (u->rec < v->rec);
-}
-
-static int facerec_rec_cmp(const void* a, const void* b) {
-  const facerec* u = (const facerec*)a; const facerec* v = (const facerec*)b;
-  return (u->rec > v->rec) - (u->rec < v->rec);
-}
-
-static void sort3(const int64_t* n, int64_t* k) {
-  int64_t a = n[0], b = n[1], c = n[2], t;
-  if (a > b) { t = a; a = b; b = t; }
-  if (b > c) { t = b; b = c; c = t; }
-  if (a > b) { t = a; a = b; b = t; }
-  k[0] = a; k[1] = b; k[2] = c;
+/* row[a] = first edge with p >= a, for a = 0..npoin (edges sorted by p) */
+static int64_t* edge_rows(int64_t nedge, const int64_t* edge, int64_t* npoin) {
+  int64_t np = 0;
+  for (int64_t m = 0; m < nedge; ++m) {
+    if (edge[m*2] + 1 > np) np = edge[m*2] + 1;
+    if (edge[m*2+1] + 1 > np) np = edge[m*2+1] + 1;
+  }
+  int64_t* row = (int64_t*)calloc((size_t)np + 1, sizeof(int64_t));
+  for (int64_t m = 0; m < nedge; ++m) ++row[edge[m*2] + 1];
+  for (int64_t a = 0; a < np; ++a) row[a+1] += row[a];
+  *npoin = np;
+  return row;
 }
 
 /* Superedge grouping for edge integrals of `stride` values per edge, the
@@ -177,6 +171,8 @@ static int domsuped_stride(int stride, int64_t ntet, const int64_t* inpoel,
                            double* int0, int64_t* sup1, double* int1,
                            int64_t* sup2, double* int2) {
   char* used = (char*)calloc((size_t)nedge, 1);
+  int64_t npoin = 0;
+  int64_t* row = edge_rows(nedge, edge, &npoin);
   int64_t n0 = 0, n1 = 0, n2 = 0;
   /* tet superedges, RieCG.cpp:657-680 */
   for (int64_t e = 0; e < ntet; ++e) {
@@ -185,8 +181,7 @@ static int domsuped_stride(int stride, int64_t ntet, const int64_t* inpoel,
     for (int k = 0; k < 6; ++k) {
       int64_t a = N[lpoed[k][0]], b = N[lpoed[k][1]];
       sig[k] = a < b ? 1.0 : -1.0;
-      int64_t m = a < b ? edge_find(nedge, edge, a, b)
-                        : edge_find(nedge, edge, b, a);
+      int64_t m = a < b ? edge_find(row, edge, a, b) : edge_find(row, edge, b, a);
       if (m < 0 || used[m]) break;
       id[k] = m; ++f;
     }
@@ -200,46 +195,35 @@ static int domsuped_stride(int stride, int64_t ntet, const int64_t* inpoel,
       ++n0;
     }
   }
-  /* unique tet faces in first-seen order (the FaceSet of RieCG.cpp:651-656);
-   * faces of tet superedges never qualify as all their edges are taken */
-  int64_t nf = ntet * 4;
-  facerec* fr = (facerec*)malloc(sizeof(facerec) * (size_t)nf);
+  /* triangle superedges over the unique tet faces in first-seen order (the
+   * FaceSet of RieCG.cpp:646-651, node order of the first tet holding the
+   * face), RieCG.cpp:683-702.  Walking every (tet, face) in order visits each
+   * face first at its first-seen place; a later copy of it never qualifies
+   * (taken: its edges are used; not taken: an edge was already used, and
+   * edges are never released), so no deduplication is needed. */
   for (int64_t e = 0; e < ntet; ++e)
-    for (int a = 0; a < 4; ++a) {
-      facerec* r = fr + e*4 + a;
-      for (int i = 0; i < 3; ++i) r->n[i] = inpoel[e*4 + lpofa[a][i]];
-      sort3(r->n, r->k);
-      r->rec = e*4 + a;
-    }
-  qsort(fr, (size_t)nf, sizeof(facerec), facerec_cmp);
-  int64_t nu = 0;
-  for (int64_t i = 0; i < nf; ++i)
-    if (i == 0 || memcmp(fr[i].k, fr[i-1].k, sizeof(fr[i].k)) != 0)
-      fr[nu++] = fr[i];
-  qsort(fr, (size_t)nu, sizeof(facerec), facerec_rec_cmp);
-  /* triangle superedges, RieCG.cpp:683-702 */
-  for (int64_t i = 0; i < nu; ++i) {
-    const int64_t* N = fr[i].n;
-    int64_t id[3]; double sig[3]; int f = 0;
-    for (int k = 0; k < 3; ++k) {
-      int64_t a = N[lpoet[k][0]], b = N[lpoet[k][1]];
-      sig[k] = a < b ? 1.0 : -1.0;
-      int64_t m = a < b ? edge_find(nedge, edge, a, b)
-                        : edge_find(nedge, edge, b, a);
-      if (m < 0 || used[m]) break;
-      id[k] = m; ++f;
-    }
-    if (f == 3) {
-      for (int j = 0; j < 3; ++j) sup1[n1*3+j] = N[j];
+    for (int fa = 0; fa < 4; ++fa) {
+      int64_t N[3];
+      for (int i = 0; i < 3; ++i) N[i] = inpoel[e*4 + lpofa[fa][i]];
+      int64_t id[3]; double sig[3]; int f = 0;
       for (int k = 0; k < 3; ++k) {
-        for (int j = 0; j < stride; ++j)
-          int1[(n1*3+k)*stride+j] = (j < 3 ? sig[k] : 1.0) * dint[id[k]*stride+j];
-        used[id[k]] = 1;
+        int64_t a = N[lpoet[k][0]], b = N[lpoet[k][1]];
+        sig[k] = a < b ? 1.0 : -1.0;
+        int64_t m = a < b ? edge_find(row, edge, a, b) : edge_find(row, edge, b, a);
+        if (m < 0 || used[m]) break;
+        id[k] = m; ++f;
       }
-      ++n1;
+      if (f == 3) {
+        for (int j = 0; j < 3; ++j) sup1[n1*3+j] = N[j];
+        for (int k = 0; k < 3; ++k) {
+          for (int j = 0; j < stride; ++j)
+            int1[(n1*3+k)*stride+j] = (j < 3 ? sig[k] : 1.0) * dint[id[k]*stride+j];
+          used[id[k]] = 1;
+        }
+        ++n1;
+      }
     }
-  }
-  free(fr);
+  free(row);
   /* remaining edges, RieCG.cpp:704-716 */
   for (int64_t m = 0; m < nedge; ++m) {
     if (used[m]) continue;

@@ -1,0 +1,111 @@
+// Test harness around the reference-side shim (xyst_amd/shim): builds
+// tk::Fields from flat arrays, calls riemann::grad/rhs and kozak::rhs with
+// the reference's signatures, and reports a thrown tk::Exception as a
+// message.  Compiled against the stand-in headers of tests/cpp/shim_stub
+// into tests/cpp/libshim_harness.so, loaded by tests/test_shim.py.
+#include <algorithm>
+#include <array>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "Kozak.hpp"
+#include "Problems.hpp"
+#include "Riemann.hpp"
+#include "xg_shim.hpp"
+
+namespace {
+
+bool g_src = false;   // whether problems::SRC() returns a function
+
+int report(const std::exception& e, char* err, int errlen) {
+  if (err && errlen > 0) {
+    std::strncpy(err, e.what(), (size_t)errlen - 1);
+    err[errlen - 1] = '\0';
+  }
+  return 1;
+}
+
+tk::Fields fields(int64_t n, int ncomp, const double* a) {
+  tk::Fields F((size_t)n, (size_t)ncomp);
+  if (a) std::copy(a, a + n * ncomp, F.vec().begin());
+  return F;
+}
+
+std::vector<double> vec(int64_t n, const double* a) {
+  return a ? std::vector<double>(a, a + n) : std::vector<double>();
+}
+
+}  // namespace
+
+namespace problems {
+
+std::function<std::vector<tk::real>(tk::real, tk::real, tk::real, tk::real, std::size_t)>
+SRC() {
+  if (!g_src) return {};
+  return [](tk::real, tk::real, tk::real, tk::real, std::size_t) {
+    return std::vector<tk::real>(5, 0.0);
+  };
+}
+
+}  // namespace problems
+
+extern "C" {
+
+void shim_set_src(int on) { g_src = on != 0; }
+
+int shim_set_problem(void* ctx, const char* name, double alpha, double kappa, double p0,
+                     char* err, int errlen) {
+  try {
+    xg_shim::set_problem(static_cast<xg_ctx*>(ctx), name, alpha, kappa, p0);
+  } catch (const std::exception& e) {
+    return report(e, err, errlen);
+  }
+  return 0;
+}
+
+int shim_riemann_grad(void* ctx, int64_t npoin, int ncomp, const double* U, double* G,
+                      char* err, int errlen) {
+  try {
+    xg_shim::t_ctx = static_cast<xg_ctx*>(ctx);
+    tk::Fields u = fields(npoin, ncomp, U), g = fields(npoin, 3 * ncomp, nullptr);
+    riemann::grad({}, {}, {}, {}, u, g);
+    std::copy(g.vec().begin(), g.vec().end(), G);
+  } catch (const std::exception& e) {
+    return report(e, err, errlen);
+  }
+  return 0;
+}
+
+int shim_riemann_rhs(void* ctx, int64_t npoin, int ncomp, int gprop, const double* G,
+                     const double* U,
+                     const double* v, double t, const double* tp, double* R, char* err,
+                     int errlen) {
+  try {
+    xg_shim::t_ctx = static_cast<xg_ctx*>(ctx);
+    tk::Fields g = fields(npoin, gprop, G), u = fields(npoin, ncomp, U),
+               r = fields(npoin, ncomp, nullptr);
+    riemann::rhs({}, {}, {}, {}, {}, g, u, vec(npoin, v), t, vec(npoin, tp), r);
+    std::copy(r.vec().begin(), r.vec().end(), R);
+  } catch (const std::exception& e) {
+    return report(e, err, errlen);
+  }
+  return 0;
+}
+
+int shim_kozak_rhs(void* ctx, int64_t npoin, int ncomp, double t, double dt,
+                   const double* tp, const double* dtp, const double* U, double* R,
+                   char* err, int errlen) {
+  try {
+    xg_shim::t_ctx = static_cast<xg_ctx*>(ctx);
+    tk::Fields u = fields(npoin, ncomp, U), r = fields(npoin, ncomp, nullptr);
+    kozak::rhs({}, {}, t, dt, vec(npoin, tp), vec(npoin, dtp), u, r);
+    std::copy(r.vec().begin(), r.vec().end(), R);
+  } catch (const std::exception& e) {
+    return report(e, err, errlen);
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -93,11 +93,13 @@ def test_kozak_errors():
     _say("kozak errors")
     ctx = _ctx()
     U = np.zeros((8, 5))
-    assert lib().xg_kozak_rhs(ctx._h, 5, 1e-3, U.ctypes.data, U.ctypes.data) == XG_ENOMESH
+    assert lib().xg_kozak_rhs(ctx._h, 5, 0.0, 1e-3, None, None, U.ctypes.data,
+                              U.ctypes.data) == XG_ENOMESH
     m = _case(2)
     K.upload(ctx, m["coord"], m["inpoel"])
     U = np.zeros((m["coord"].shape[1], 6))
-    assert lib().xg_kozak_rhs(ctx._h, 6, 1e-3, U.ctypes.data, U.ctypes.data) == XG_EUNSUPPORTED
+    assert lib().xg_kozak_rhs(ctx._h, 6, 0.0, 1e-3, None, None, U.ctypes.data,
+                              U.ctypes.data) == XG_EUNSUPPORTED
     ctx.close()
 
 
@@ -254,7 +256,7 @@ def test_kozcg_step_errors():
     c, dn = _kozcg_case(4, "lex")
     ctx = _ctx()
     K.upload(ctx, c.coord, c.inpoel)
-    assert lib().xg_kozcg_step_dev(ctx._h, 1e-3, 8, 16) == XG_ENOMESH
+    assert lib().xg_kozcg_step_dev(ctx._h, 0.0, 1e-3, 8, 16) == XG_ENOMESH
     with pytest.raises(Exception):
         K.setup(ctx, c.vol, c.symnodes, c.symnorms, fctsys=(0, 1, 2, 3, 4, 5))
     bad = c.vol.copy()
@@ -262,5 +264,5 @@ def test_kozcg_step_errors():
     with pytest.raises(Exception):
         K.setup(ctx, bad, c.symnodes, c.symnorms)
     K.setup(ctx, c.vol, c.symnodes, c.symnorms)
-    assert lib().xg_kozcg_step_dev(ctx._h, 1e-3, 8, 8) == XG_EINVAL
+    assert lib().xg_kozcg_step_dev(ctx._h, 0.0, 1e-3, 8, 8) == XG_EINVAL
     ctx.close()

@@ -38,6 +38,7 @@ SIG = {
     "xg_riemann_rhs": (C.c_int, [VP, C.c_int, VP, VP, VP, D, VP, VP]),
     "xg_riemann_rhs_dev": (C.c_int, [VP, C.c_int, VP, VP, VP, D, VP, VP]),
     "xg_set_problem": (C.c_int, [VP, VP]),
+    "xg_get_problem": (C.c_int, [VP, VP]),
     "xg_riecg_setup": (C.c_int, [VP, VP, I64, VP, VP]),
     "xg_riecg_dt_dev": (C.c_int, [VP, VP, D, C.POINTER(D)]),
     "xg_riecg_stage_dev": (C.c_int, [VP, C.c_int, D, D, VP, VP, VP, VP, VP, VP]),

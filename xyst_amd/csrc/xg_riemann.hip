@@ -1188,4 +1188,10 @@ int xg_set_problem(xg_ctx* c, const xg_problem* p) {
   return XG_OK;
 }
 
+int xg_get_problem(const xg_ctx* c, xg_problem* p) {
+  if (!c || !p) return XG_EINVAL;
+  *p = c->problem;
+  return XG_OK;
+}
+
 }  // extern "C"
