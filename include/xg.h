@@ -97,6 +97,22 @@ int xg_mesh_chocg_domint(int64_t npoin, const double* x, const double* y,
                          const double* z, int64_t ntet, const size_t* inpoel,
                          int64_t cap, int64_t* nedge, size_t* edge, double* dint);
 
+/* RieCG::domsuped (RieCG.cpp:620-736) and ChoCG::domsuped (ChoCG.cpp:
+ * 687-813): the unique edges of xg_mesh_domint / xg_mesh_chocg_domint
+ * (edge[2*nedge] sorted by (p, q), p < q; stride 3 or 5 integral values per
+ * edge) grouped into tet superedges (a tet whose six edges are all still
+ * free: 4 ids, 6 x stride values), then triangle superedges over the tets'
+ * faces in first-seen order (3 ids, 3 x stride), then the remaining edges
+ * (2 ids, stride), each edge oriented as its superedge gives it (the first
+ * three values change sign when that is q -> p).  The orientation matters
+ * to the last bits: the MUSCL limiter's epsilon makes the edge flux not
+ * exactly antisymmetric.  With sup0 = NULL only nsup[3] is computed;
+ * otherwise the six arrays are filled (sized from nsup by the caller). */
+int xg_mesh_domsuped(int64_t ntet, const size_t* inpoel, int64_t nedge,
+                     const size_t* edge, const double* dint, int stride,
+                     int64_t nsup[3], size_t* sup0, double* int0, size_t* sup1,
+                     double* int1, size_t* sup2, double* int2);
+
 /* Locality renumbering of the nodes, the job of the RieCG/ChoCG constructor
  * renumbering (RieCG.cpp:82-100) and tk::renumber (Reorder.cpp:215-260):
  * reverse Cuthill-McKee over the edge graph.  perm[old] = new. */
@@ -448,6 +464,36 @@ int xg_comm_sum_dev(xg_comm* comm, int ncomp, double* const* fields,
 int xg_comm_shared(const xg_comm* comm, int64_t* nshared);
 const char* xg_comm_error(const xg_comm* comm);
 int xg_comm_destroy(xg_comm* comm);
+
+/* The chares of one ConjugateGradients array that share a GPU
+ * (ConjugateGradients.hpp:53-152, ConjugateGradients.cpp:105-823): nparts
+ * partitions of one linear system with ncomp unknowns per node.  Partition
+ * k holds npoin[k] nodes with global ids gid[k] (their overlap is the node
+ * communication map), the partial matrix of its own elements in tk::CSR
+ * form (ia[k], ja[k], a[k]; 1-based when one_based, as tk::CSR stores them;
+ * Dirichlet rows set by CSR::dirichlet with 1/count on the diagonal,
+ * CSR.cpp:107-152) and, at setup, b[k] with the full values at shared nodes
+ * and x[k] (null: zero).  The exchanges (comres, comd, comq: partial sums
+ * completed in partition order; comx: the copies averaged) run on the
+ * device; dot products count a node in the highest partition holding it
+ * (tk::slave, Reorder.cpp:393-412).  pc: 0 none (1/count per copy, summed),
+ * 1 Jacobi.  xg_pcg_setup returns normb, xg_pcg_solve the recurrence's
+ * ||r|| with the reference's stop test (||r|| < tol normb, |(p,q)| <
+ * DBL_EPSILON, maxit).  A handle is returned even when create fails
+ * (message in xg_pcg_error); xg_pcg_destroy releases it. */
+typedef struct xg_pcg xg_pcg;
+int xg_pcg_create(int device, int nparts, int ncomp, const int64_t* npoin,
+                  const size_t* const* gid, const size_t* const* ia,
+                  const size_t* const* ja, const double* const* a, int one_based,
+                  xg_pcg** pcg);
+int xg_pcg_set_stream(xg_pcg* pcg, void* hip_stream);
+int xg_pcg_setup(xg_pcg* pcg, const double* const* x, const double* const* b, int pc,
+                 double* normb);
+int xg_pcg_solve(xg_pcg* pcg, int64_t maxit, double tol, double* normr, int64_t* it,
+                 int* converged);
+int xg_pcg_solution(xg_pcg* pcg, int part, double* x);
+const char* xg_pcg_error(const xg_pcg* pcg);
+int xg_pcg_destroy(xg_pcg* pcg);
 
 /* ---- pressure-Poisson system (ChoCG) and conjugate gradients ---------------- */
 

@@ -74,3 +74,22 @@ def test_chocg_domint_bit_exact_vs_oracle():
     e0, d0 = O.chocg_domint(m["coord"], m["inpoel"])
     assert np.array_equal(e1.astype(np.int64), e0)
     assert np.array_equal(d1, d0)
+
+
+@pytest.mark.parametrize("n,seed,stride", [(4, 1, 3), (9, 2, 3), (13, 5, 3), (9, 2, 5)])
+def test_domsuped_bit_exact_vs_oracle(n, seed, stride):
+    """xg_mesh_domsuped (host C++) groups the edges into the oracle's tet,
+    triangle and edge superedges (RieCG::domsuped / ChoCG::domsuped), with
+    the same ids, orientations and signed integrals, bit for bit."""
+    from xyst_amd import riemann as R
+    m = box_mesh(n, seed=seed)
+    dom = O.domint if stride == 3 else O.chocg_domint
+    edge, dint = dom(m["coord"], m["inpoel"])
+    se = O.domsuped(m["inpoel"], edge, dint)
+    sup, val = R.mesh_domsuped(m["inpoel"], edge, dint)
+    for k in range(3):
+        assert np.array_equal(sup[k].astype(np.int64), se.sup[k])
+        assert np.array_equal(val[k], se.int[k])
+    assert sup[0].size > 0 and sup[1].size > 0
+    with pytest.raises(XgError):
+        R.mesh_domsuped(m["inpoel"], edge[::-1].copy(), dint)     # unsorted edges

@@ -4,6 +4,7 @@ shared-node communicator validates its input before touching the device."""
 import ctypes as C
 
 import numpy as np
+import pytest
 
 from oracle import partitioned as PP
 from tests.analytic_cases import fixture, unitcube
@@ -45,3 +46,20 @@ def test_comm_rejects_bad_input():
     assert b"repeats" in lib.xg_comm_error(h)
     lib.xg_comm_destroy(h)
     assert lib.xg_comm_sum_dev(None, 1, None, None) == _xg.XG_EINVAL
+
+
+def test_partitioned_cg_rejects_bad_input():
+    """xg_pcg_create refuses malformed partitions before any device work."""
+    from xyst_amd.cg import CSR
+    from xyst_amd.partition import PartitionedCG
+    inpoel = np.array([[0, 1, 2, 3]], np.int64)
+    A = CSR.from_tets(4, inpoel)
+    with pytest.raises(_xg.XgError, match="repeats"):
+        PartitionedCG([A], [np.array([0, 1, 1, 3])])
+    bad = CSR(1, A.ia, A.ja.copy(), A.a)
+    bad.ja[-1] = 9
+    with pytest.raises(_xg.XgError, match="column"):
+        PartitionedCG([bad], [np.arange(4)])
+    lib = _xg.lib()
+    assert lib.xg_pcg_solve(None, 1, 1e-3, None, None, None) == _xg.XG_EINVAL
+    assert lib.xg_pcg_destroy(None) == _xg.XG_EINVAL

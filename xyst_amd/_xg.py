@@ -27,6 +27,7 @@ SIG = {
     "xg_mesh_vol": (C.c_int, [I64, VP, VP, VP, I64, VP, VP]),
     "xg_mesh_domint": (C.c_int, [I64, VP, VP, VP, I64, VP, I64, C.POINTER(I64), VP, VP]),
     "xg_mesh_chocg_domint": (C.c_int, [I64, VP, VP, VP, I64, VP, I64, C.POINTER(I64), VP, VP]),
+    "xg_mesh_domsuped": (C.c_int, [I64, VP, I64, VP, VP, C.c_int, VP, VP, VP, VP, VP, VP, VP]),
     "xg_mesh_rcm": (C.c_int, [I64, I64, VP, VP]),
     "xg_mesh_block_order": (C.c_int, [I64, VP, VP, VP, I64, VP]),
     "xg_mesh_symbc": (C.c_int, [I64, VP, VP, VP, I64, VP, VP, I32, VP, I64,
@@ -53,6 +54,14 @@ SIG = {
     "xg_comm_shared": (C.c_int, [VP, C.POINTER(I64)]),
     "xg_comm_error": (C.c_char_p, [VP]),
     "xg_comm_destroy": (C.c_int, [VP]),
+    "xg_pcg_create": (C.c_int, [C.c_int, C.c_int, C.c_int, VP, VP, VP, VP, VP, C.c_int,
+                                C.POINTER(VP)]),
+    "xg_pcg_set_stream": (C.c_int, [VP, VP]),
+    "xg_pcg_setup": (C.c_int, [VP, VP, VP, C.c_int, C.POINTER(D)]),
+    "xg_pcg_solve": (C.c_int, [VP, I64, D, C.POINTER(D), C.POINTER(I64), C.POINTER(C.c_int)]),
+    "xg_pcg_solution": (C.c_int, [VP, C.c_int, VP]),
+    "xg_pcg_error": (C.c_char_p, [VP]),
+    "xg_pcg_destroy": (C.c_int, [VP]),
     "xg_mesh_device_bytes": (C.c_int, [VP, C.POINTER(I64)]),
     "xg_kozak_upload": (C.c_int, [VP, I64, VP, VP, VP, I64, VP]),
     "xg_kozak_rhs": (C.c_int, [VP, C.c_int, D, D, VP, VP, VP, VP]),

@@ -289,9 +289,114 @@ int mesh_symbc(int64_t npoin, const double* x, const double* y,
   return XG_OK;
 }
 
+// RieCG::domsuped (RieCG.cpp:620-736), ChoCG::domsuped (ChoCG.cpp:687-813):
+// the unique edges (sorted by (p, q), p < q) grouped into tet superedges,
+// then triangle superedges over the tets' faces in first-seen order, then
+// the remaining edges.  An edge is looked up by a binary search within the
+// row of its smaller node.  Faces are visited as (tet, face) in order: the
+// first visit of a face is its first-seen place, and a later visit never
+// qualifies (if it was taken its edges are used; if not, one of its edges
+// was already used, and edges are never released), so the face set needs no
+// deduplication.  The first three values of an integral flip their sign when
+// the superedge orients the edge q -> p.
+static const int kLpofa[4][3] = {{1, 2, 3}, {2, 0, 3}, {3, 0, 1}, {0, 2, 1}};
+static const int kLpoet[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+
+struct Superedges {
+  std::vector<size_t> sup[3];
+  std::vector<double> val[3];
+};
+
+int mesh_domsuped(int64_t ntet, const size_t* inpoel, int64_t nedge, const size_t* edge,
+                  const double* dint, int stride, Superedges& S, std::string& err) {
+  size_t np = 0;
+  for (int64_t m = 0; m < nedge; ++m) {
+    if (edge[2 * m] >= edge[2 * m + 1] ||
+        (m && (edge[2 * m] < edge[2 * m - 2] ||
+               (edge[2 * m] == edge[2 * m - 2] && edge[2 * m + 1] <= edge[2 * m - 1])))) {
+      err = "xg_mesh_domsuped: edges must be unique, p < q, sorted by (p, q)";
+      return XG_EINVAL;
+    }
+    np = std::max(np, (size_t)edge[2 * m + 1] + 1);
+  }
+  for (int64_t e = 0; e < 4 * ntet; ++e) np = std::max(np, (size_t)inpoel[e] + 1);
+  std::vector<int64_t> row(np + 1, 0);
+  for (int64_t m = 0; m < nedge; ++m) ++row[edge[2 * m] + 1];
+  for (size_t a = 0; a < np; ++a) row[a + 1] += row[a];
+  auto find = [&](size_t a, size_t b) -> int64_t {
+    if (a > b) std::swap(a, b);
+    int64_t lo = row[a], hi = row[a + 1];
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) / 2;
+      if (edge[2 * mid + 1] < b) lo = mid + 1; else hi = mid;
+    }
+    return (lo < row[a + 1] && edge[2 * lo + 1] == b) ? lo : -1;
+  };
+  std::vector<char> used(nedge, 0);
+  auto take = [&](int k, const size_t* N, int nn, const int (*lp)[2], int ne) {
+    int64_t id[6];
+    for (int j = 0; j < ne; ++j) {
+      id[j] = find(N[lp[j][0]], N[lp[j][1]]);
+      if (id[j] < 0 || used[id[j]]) return;
+    }
+    for (int i = 0; i < nn; ++i) S.sup[k].push_back(N[i]);
+    for (int j = 0; j < ne; ++j) {
+      const double sg = N[lp[j][0]] < N[lp[j][1]] ? 1.0 : -1.0;
+      for (int v = 0; v < stride; ++v)
+        S.val[k].push_back((v < 3 ? sg : 1.0) * dint[id[j] * stride + v]);
+      used[id[j]] = 1;
+    }
+  };
+  for (int64_t e = 0; e < ntet; ++e) take(0, inpoel + 4 * e, 4, kLpoed, 6);
+  for (int64_t e = 0; e < ntet; ++e)
+    for (int f = 0; f < 4; ++f) {
+      const size_t N[3] = {inpoel[4 * e + kLpofa[f][0]], inpoel[4 * e + kLpofa[f][1]],
+                           inpoel[4 * e + kLpofa[f][2]]};
+      take(1, N, 3, kLpoet, 3);
+    }
+  for (int64_t m = 0; m < nedge; ++m) {
+    if (used[m]) continue;
+    S.sup[2].push_back(edge[2 * m]);
+    S.sup[2].push_back(edge[2 * m + 1]);
+    for (int v = 0; v < stride; ++v) S.val[2].push_back(dint[m * stride + v]);
+  }
+  if (S.sup[0].size() / 4 * 6 + S.sup[1].size() / 3 * 3 + S.sup[2].size() / 2 !=
+      (size_t)nedge) {
+    err = "xg_mesh_domsuped: the tets do not hold every edge";   // RieCG.cpp:733-735
+    return XG_EINVAL;
+  }
+  return XG_OK;
+}
+
 }  // namespace xg
 
 // ------------------------------------------------------------- C-ABI ------
+
+extern "C" int xg_mesh_domsuped(int64_t ntet, const size_t* inpoel, int64_t nedge,
+                                const size_t* edge, const double* dint, int stride,
+                                int64_t nsup[3], size_t* sup0, double* int0,
+                                size_t* sup1, double* int1, size_t* sup2,
+                                double* int2) {
+  if (ntet < 0 || nedge < 0 || (ntet && !inpoel) || (nedge && (!edge || !dint)) ||
+      (stride != 3 && stride != 5) || !nsup)
+    return XG_EINVAL;
+  xg::Superedges S;
+  std::string err;
+  int rc = xg::mesh_domsuped(ntet, inpoel, nedge, edge, dint, stride, S, err);
+  if (rc) return rc;
+  const int nn[3] = {4, 3, 2};
+  for (int k = 0; k < 3; ++k) nsup[k] = (int64_t)(S.sup[k].size() / nn[k]);
+  if (!sup0) return XG_OK;
+  size_t* so[3] = {sup0, sup1, sup2};
+  double* vo[3] = {int0, int1, int2};
+  for (int k = 0; k < 3; ++k) {
+    if (S.sup[k].empty()) continue;
+    if (!so[k] || !vo[k]) return XG_EINVAL;
+    std::memcpy(so[k], S.sup[k].data(), S.sup[k].size() * sizeof(size_t));
+    std::memcpy(vo[k], S.val[k].data(), S.val[k].size() * sizeof(double));
+  }
+  return XG_OK;
+}
 
 extern "C" int xg_mesh_symbc(int64_t npoin, const double* x, const double* y,
                              const double* z, int64_t ntri, const size_t* tri,

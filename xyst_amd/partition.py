@@ -18,7 +18,7 @@ import numpy as np
 from . import riemann as X
 from ._xg import XgError, lib, ptr
 
-__all__ = ["Comm", "Part", "split_mesh", "PartitionedRieCG"]
+__all__ = ["Comm", "Part", "split_mesh", "PartitionedRieCG", "PartitionedCG"]
 
 
 class Comm:
@@ -120,7 +120,9 @@ class PartitionedRieCG:
                             stab2coef=stab2coef)
             ctx.set_problem(problem, **problem_args)
             edge, dint = X.mesh_domint(p.coord, p.inpoel)
-            sup, ints = X.edges_as_superedges(edge, dint)
+            # the chare's own superedges (RieCG::domsuped), oriented as the
+            # reference orients them
+            sup, ints = X.mesh_domsuped(p.inpoel, edge, dint)
             bs = besym.reshape(-1, 3)[p.face_index].reshape(-1)
             ctx.upload_mesh(p.coord, sup, ints, p.tri, bs)
             p.vol = X.mesh_vol(p.coord, p.inpoel)          # own elements only
@@ -210,3 +212,81 @@ class PartitionedRieCG:
         from .diagnostics import errcompute
         return sum(errcompute(ctx, u, v, a)
                    for ctx, u, v, a in zip(self.ctx, Us, self.v, Ans))
+
+
+class PartitionedCG:
+    """The chares of one ConjugateGradients array on one device (xg_pcg_*,
+    ConjugateGradients.cpp:105-823): partition k's partial matrix As[k] (an
+    object with ia, ja, a, 0-based, as xyst_amd.cg.CSR or the oracle's CSR)
+    over its nodes with global ids gids[k]."""
+
+    def __init__(self, As, gids, ncomp=1, device=0):
+        L = lib()
+        self.ncomp = ncomp
+        self._keep = []
+        npoin = np.array([len(g) for g in gids], np.int64)
+        self.rows = [int(n) * ncomp for n in npoin]
+
+        def arr(lst, dtype):
+            a = [np.ascontiguousarray(v, dtype=dtype) for v in lst]
+            self._keep.extend(a)
+            return (C.c_void_p * len(a))(*[v.ctypes.data for v in a])
+        g = arr(gids, np.uint64)
+        ia = arr([A.ia for A in As], np.uint64)
+        ja = arr([A.ja for A in As], np.uint64)
+        av = arr([A.a for A in As], np.float64)
+        self._npoin = npoin
+        h = C.c_void_p()
+        rc = L.xg_pcg_create(device, len(As), ncomp, npoin.ctypes.data, g, ia, ja, av, 0,
+                             C.byref(h))
+        self._h = h
+        if rc:
+            msg = L.xg_pcg_error(h).decode() if h else ""
+            self.close()
+            raise XgError(rc, msg)
+
+    def _check(self, rc):
+        if rc:
+            raise XgError(rc, lib().xg_pcg_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().xg_pcg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def setup(self, xs, bs, pc="none"):
+        """ConjugateGradients::setup on every partition; returns normb."""
+        bb = [np.ascontiguousarray(b, dtype=np.float64) for b in bs]
+        xx = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in xs]
+        for k, b in enumerate(bb):
+            if b.size != self.rows[k]:
+                raise ValueError("b of partition %d: %d values, need %d" % (k, b.size, self.rows[k]))
+        barr = (C.c_void_p * len(bb))(*[b.ctypes.data for b in bb])
+        xarr = (C.c_void_p * len(xx))(*[None if x is None else x.ctypes.data for x in xx])
+        nb = C.c_double()
+        self._check(lib().xg_pcg_setup(self._h, xarr, barr, {"none": 0, "jacobi": 1}[pc],
+                                       C.byref(nb)))
+        self.normb = nb.value
+        return self.normb
+
+    def solve(self, maxit, tol):
+        """ConjugateGradients::solve; returns ||r|| of the recurrence."""
+        nr, it, cv = C.c_double(), C.c_int64(), C.c_int()
+        self._check(lib().xg_pcg_solve(self._h, maxit, tol, C.byref(nr), C.byref(it),
+                                       C.byref(cv)))
+        self.normr, self.it, self.converged = nr.value, it.value, bool(cv.value)
+        return self.normr
+
+    def solution(self):
+        out = []
+        for k, n in enumerate(self.rows):
+            x = np.empty(n)
+            self._check(lib().xg_pcg_solution(self._h, k, x.ctypes.data))
+            out.append(x)
+        return out

@@ -91,6 +91,27 @@ def mesh_symbc(coord, triinpoel, faceset, symsets):
     return nodes.astype(np.int64), norms, besym
 
 
+def mesh_domsuped(inpoel, edge, dint):
+    """RieCG::domsuped / ChoCG::domsuped: the edges of mesh_domint (or the
+    5-value ChoCG integrals) grouped into the reference's tet, triangle and
+    edge superedges, oriented as the reference orients them; returns the
+    (dsupedge, dsupint) triple of lists."""
+    inp = _idx(inpoel)
+    e = _idx(edge).reshape(-1)
+    d = _f64(dint)
+    st = d.shape[1] if d.ndim == 2 else 3
+    d = d.reshape(-1)
+    ns = np.zeros(3, np.int64)
+    L = lib()
+    args = (inp.size // 4, ptr(inp), e.size // 2, ptr(e), ptr(d), st, ptr(ns))
+    check(L.xg_mesh_domsuped(*args, None, None, None, None, None, None), what="domsuped")
+    sup = [np.empty(n * k, np.uint64) for n, k in zip(ns, (4, 3, 2))]
+    val = [np.empty(n * k * st) for n, k in zip(ns, (6, 3, 1))]
+    check(L.xg_mesh_domsuped(*args, ptr(sup[0]), ptr(val[0]), ptr(sup[1]), ptr(val[1]),
+                             ptr(sup[2]), ptr(val[2])), what="domsuped")
+    return sup, val
+
+
 def edges_as_superedges(edge, dint):
     """A plain edge list as the reference's (dsupedge, dsupint) triple: every
     edge in the third (single-edge) group."""
