@@ -424,6 +424,19 @@ int xg_chocg_step_dev(xg_ctx* ctx, double dt, double* U, double* Un, double* P,
 int xg_diag_rho_dev(xg_ctx* ctx, int64_t npoin, int ncomp, const double* U,
                     const double* Un, const double* V, double* out);
 
+/* xg_diag_rho_dev for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops); it has no counterpart in
+ * the reference.  Same sums in the same order as xg_diag_rho_dev, bit for
+ * bit, but out (2 ncomp + 1 doubles) is a DEVICE pointer: the call only
+ * enqueues two kernels on the context's stream, copies nothing to the host
+ * and does not synchronize.  It allocates nothing: the workgroup partials
+ * live in a fixed device array of 11 x 1024 doubles per GPU, held until the
+ * process ends; larger sizes are refused (npoin never needs more, ncomp is
+ * at most 5).  Calls that overlap on one GPU (two streams or threads) share
+ * that array and race: serialize them. */
+int xg_diag_rho_dev_noalloc(xg_ctx* ctx, int64_t npoin, int ncomp, const double* U,
+                            const double* Un, const double* V, double* out);
+
 /* The analytic-error sums of NodeDiagnostics::rhocompute
  * (NodeDiagnostics.cpp:79-128): with An (npoin*5, device) the analytic
  * solution in primitive form (rho, u, v, w, specific internal energy; the

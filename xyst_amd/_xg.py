@@ -76,6 +76,7 @@ SIG = {
     "xg_chocg_momentum_its": (C.c_int, [VP, VP]),
     "xg_cg_init_dev": (C.c_int, [VP, VP, I64, VP, VP, VP, C.c_int, C.c_int]),
     "xg_diag_rho_dev": (C.c_int, [VP, I64, C.c_int, VP, VP, VP, VP]),
+    "xg_diag_rho_dev_noalloc": (C.c_int, [VP, I64, C.c_int, VP, VP, VP, VP]),
     "xg_amg_info": (C.c_int, [VP, VP, VP, VP]),
     "xg_diag_pre_dev": (C.c_int, [VP, I64, VP, VP, VP, VP, VP, VP]),
     "xg_kozcg_dt_dev": (C.c_int, [VP, VP, D, VP]),

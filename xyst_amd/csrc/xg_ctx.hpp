@@ -99,6 +99,7 @@ struct xg_ctx {
 
 void xg_kozak_free(xg_ctx* c);   // xg_kozak.hip
 void xg_chocg_free(xg_ctx* c);   // xg_chocg.hip
+int xg_diag_scratch_init(int device);   // xg_diag.hip, with `device` current
 // rows of the context's CG matrix when it has one component per node, else -1
 int64_t xg_cg_rows(const xg_ctx* c);   // xg_cg.hip
 

@@ -8,12 +8,48 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "xg_ctx.hpp"
 
 using namespace xgd;
 
+// Scratch memory of xg_diag_rho_dev_noalloc: the workgroup partials of at
+// most kScratchSums sums over at most kRedBlocks workgroups (90,112 bytes),
+// one copy per GPU, held until the process ends.  Host code never passes
+// `scratch` itself to a kernel: its device address on each GPU comes from
+// hipGetSymbolAddress (xg_diag_scratch_init).
+constexpr int kScratchSums = 11;
+constexpr size_t kScratchDoubles = (size_t)kScratchSums * kRedBlocks;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+// Called by xg_ctx_create with `device` current: records the device address
+// of this GPU's `scratch` copy once per device.
+int xg_diag_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return XG_EINVAL;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return XG_OK;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return XG_EHIP;
+  g_scratch_ptr[device] = static_cast<double*>(p);
+  return XG_OK;
+}
+
+namespace {
+
+double* scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
 
 __device__ inline double block_sum(double v, double* sm) {
 #pragma unroll
@@ -183,6 +219,33 @@ int xg_diag_rho_dev(xg_ctx* c, int64_t npoin, int ncomp, const double* U, const 
   else
     hipLaunchKernelGGL(k_diag_rho<1>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, Un, V, part);
   return finish(c, ns, nb, part, dout, out);
+}
+
+int xg_diag_rho_dev_noalloc(xg_ctx* c, int64_t npoin, int ncomp, const double* U,
+                            const double* Un, const double* V, double* out) {
+  if (!c) return XG_EINVAL;
+  if (npoin < 0 || !U || !Un || !V || !out)
+    return fail(c, XG_EINVAL, "xg_diag_rho_noalloc: bad arguments");
+  if (ncomp != 5 && ncomp != 3 && ncomp != 1)
+    return fail(c, XG_EUNSUPPORTED, "xg_diag_rho_noalloc: ncomp must be 1, 3 or 5");
+  const int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, nblocks(npoin)));
+  const int ns = 2 * ncomp + 1;
+  if ((size_t)ns * nb > kScratchDoubles)
+    return fail(c, XG_EINVAL, "xg_diag_rho_noalloc: more partial sums than the "
+                              "scratch array holds (11 x 1024)");
+  double* part = scratch_ptr(c->device);
+  if (!part)
+    return fail(c, XG_EHIP, "xg_diag_rho_noalloc: no scratch address for this device");
+  XG_HIP(c, hipSetDevice(c->device));
+  if (ncomp == 5)
+    hipLaunchKernelGGL(k_diag_rho<5>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, Un, V, part);
+  else if (ncomp == 3)
+    hipLaunchKernelGGL(k_diag_rho<3>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, Un, V, part);
+  else
+    hipLaunchKernelGGL(k_diag_rho<1>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, Un, V, part);
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(kBlock), 0, c->stream, ns, nb, part, out);
+  XG_HIP(c, hipGetLastError());
+  return XG_OK;
 }
 
 int xg_diag_err_dev(xg_ctx* c, int64_t npoin, int ncomp, const double* U, const double* V,

@@ -768,6 +768,7 @@ int xg_ctx_create(int device, xg_ctx** out) {
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return XG_EHIP;
   if (device < 0 || device >= n) return XG_EINVAL;
   if (hipSetDevice(device) != hipSuccess) return XG_EHIP;
+  if (xg_diag_scratch_init(device) != XG_OK) return XG_EHIP;
   xg_ctx* c = new xg_ctx;
   c->device = device;
   *out = c;
