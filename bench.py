@@ -55,7 +55,18 @@ def setup(n, seed=20261202, order="rcm+block", cell=1024):
     return m, vol, edge, dint, sn, sv, besym, U
 
 
+_POISSON = {}
+
+
 def poisson_setup(n, seed=20261202, modes=8):
+    """poisson_system(n) once per size (the Poisson and learned sections use
+    the same systems)."""
+    if (n, seed, modes) not in _POISSON:
+        _POISSON[(n, seed, modes)] = poisson_system(n, seed, modes)
+    return _POISSON[(n, seed, modes)]
+
+
+def poisson_system(n, seed=20261202, modes=8):
     """ChoCG pressure-Poisson system on an n^3 box: A = -K (ChoCG::prelhs),
     b = the edge-based divergence sum_e D_e . (u_p + u_q) (chorin::div's
     domain term) of a smooth velocity field of `modes` seeded Fourier modes of
@@ -120,7 +131,8 @@ def bc_residual(A, b, dirnodes):
 def poisson_bench(n, stream, reps=2):
     """ms per pressure-Poisson solve to 1e-10: FP64 Jacobi-PCG (the
     reference algorithm on the device), the FP32-inner solve with FP64
-    residual refinement, FP64 flexible CG with an FP32 inner CG, and FP64
+    residual refinement (restarted, and as one FP32 recurrence with FP64
+    reliable updates), FP64 flexible CG with an FP32 inner CG, and FP64
     flexible CG with one FP32 smoothed-aggregation multigrid V-cycle per
     iteration.  Timed: setup (r = b - Ax, diagonal, norms) and the
     iterations, with A and b resident on the device; the multigrid
@@ -140,8 +152,8 @@ def poisson_bench(n, stream, reps=2):
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     out = {"n": n, "nodes": int(A.nrow), "nnz": int(A.ja.size), "tol": 1e-10}
-    for name in ("fp64_jacobi_pcg", "mixed_fp32_fp64", "fcg_fp64_cg32_inner",
-                 "fcg_fp64_amg"):
+    for name in ("fp64_jacobi_pcg", "mixed_fp32_fp64", "mixed_fp32_reliable",
+                 "fcg_fp64_cg32_inner", "fcg_fp64_amg"):
         ms = []
         for _ in range(reps):
             cg.init(x0, b, dirbc=dirbc, apply=True, pc="jacobi")
@@ -152,6 +164,8 @@ def poisson_bench(n, stream, reps=2):
                 cg.solve(100000, 1e-10)
             elif name == "mixed_fp32_fp64":
                 cg.solve_mixed(100000, 1e-10, inner_tol=1e-5, inner_maxit=5000)
+            elif name == "mixed_fp32_reliable":
+                cg.solve_reliable(100000, 1e-10, delta=0.1)
             elif name == "fcg_fp64_cg32_inner":
                 cg.solve_flexible(100000, 1e-10, pc="cg32", inner=30)
             else:
@@ -170,6 +184,8 @@ def poisson_bench(n, stream, reps=2):
                                                   np.linalg.norm(x_ref))}
         if name == "mixed_fp32_fp64":
             out[name]["outer"] = int(cg.nouter)
+        if name == "mixed_fp32_reliable":
+            out[name]["reliable_updates"] = int(cg.nupdates)
         if name == "fcg_fp64_cg32_inner":
             out[name]["inner_iterations"] = int(cg.inner_total)
         if name == "fcg_fp64_amg":
@@ -180,9 +196,93 @@ def poisson_bench(n, stream, reps=2):
             % (name, min(ms), cg.it, cg.converged, out[name]["true_residual_rel"],
                out[name]["err_vs_fp64_pcg_rel"]))
     out["speedup_mixed"] = out["fp64_jacobi_pcg"]["ms"] / out["mixed_fp32_fp64"]["ms"]
+    out["speedup_reliable"] = out["fp64_jacobi_pcg"]["ms"] / out["mixed_fp32_reliable"]["ms"]
     out["speedup_fcg_cg32"] = out["fp64_jacobi_pcg"]["ms"] / out["fcg_fp64_cg32_inner"]["ms"]
     out["speedup_fcg_amg"] = out["fp64_jacobi_pcg"]["ms"] / out["fcg_fp64_amg"]["ms"]
     ctx.close()
+    return out
+
+
+def learned_bench(stream, eval_sizes, train_n=64, nsys=36, pairs=600, seed=7):
+    """The learned graph preconditioner: trained on the device on residual/
+    error pairs of Jacobi-PCG solves of ChoCG pressure systems on a
+    train_n^3 box (seeded random smooth right-hand sides, up to `pairs`
+    pairs per system), then applied inside FP64 flexible CG on the
+    pressure-Poisson systems of the Poisson section (eval_sizes), against
+    FP64 Jacobi-PCG on the same system: iterations, ms and the true
+    residual of each.  FCG falls back to Jacobi for the rest of a solve once
+    the residual has not improved for 20 iterations."""
+    import torch
+    from xyst_amd import cg as CG
+    from xyst_amd import gnn
+    from xyst_amd import riemann as X
+    from xyst_amd.meshgen import box_mesh, renumber_mesh
+    t0 = time.time()
+    m = box_mesh(train_n, seed=seed)
+    m = renumber_mesh(m, X.mesh_rcm(m["coord"].shape[1], m["inpoel"]))
+    coord = m["coord"]
+    N = coord.shape[1]
+    A = CG.CSR.poisson(coord, m["inpoel"])
+    dirbc = {int(i): [(1, 0.0)] for i in np.where(coord[0] < 1e-12)[0]}
+    ctx = X.Context(0, stream=stream)
+    cg = CG.ConjugateGradients(A, ctx=ctx)
+    cg.init(np.zeros(N), np.ones(N), dirbc=dirbc, apply=True, pc="jacobi")
+    net = gnn.LearnedPreconditioner(cg, seed=seed)
+    rng = np.random.default_rng(seed)
+    x, y, z = coord
+    total, losses = 0, []
+    for k in range(nsys):
+        b = np.zeros(N)
+        for _ in range(4):
+            kx, ky, kz = rng.integers(1, 4, size=3)
+            b += rng.standard_normal() * np.sin(np.pi * (kx * x + rng.uniform())) * \
+                np.cos(np.pi * ky * y) * np.cos(np.pi * (kz * z + rng.uniform()))
+        b *= 1e4 / np.linalg.norm(b)
+        cg.init(np.zeros(N), b, dirbc=dirbc, apply=True, pc="jacobi")
+        L, npairs = net.train_system(stride=1, max_pairs=pairs, batch=8, lr=1e-3)
+        total += npairs
+        losses.append(L)
+    params = net.params()
+    train_s = time.time() - t0
+    log("learned: trained on %d pairs of %d systems (%d^3) in %.1fs, loss %.4f -> %.4f"
+        % (total, nsys, train_n, train_s, losses[0], losses[-1]))
+    ctx.close()
+    out = {"train_n": train_n, "train_pairs": int(total), "train_systems": nsys,
+           "train_s": train_s, "loss_first_system": losses[0],
+           "loss_last_system": losses[-1]}
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    for n in eval_sizes:
+        A, b, dirbc = poisson_setup(n)
+        residual = bc_residual(A, b, sorted(dirbc))
+        ctx = X.Context(0, stream=stream)
+        cg = CG.ConjugateGradients(A, ctx=ctx)
+        net = gnn.LearnedPreconditioner(cg, params=params)
+        row = {}
+        for name in ("fp64_jacobi_pcg", "fcg_fp64_learned"):
+            cg.init(np.zeros(A.nrow), b, dirbc=dirbc, apply=True, pc="jacobi")
+            torch.cuda.synchronize()
+            ev0.record(stream)
+            if name == "fp64_jacobi_pcg":
+                cg.setup()
+                cg.solve(100000, 1e-10)
+                maxit = 3 * cg.it
+            else:
+                cg.solve_flexible(maxit, 1e-10, pc="gnn")
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            row[name] = {"ms": ev0.elapsed_time(ev1), "iterations": int(cg.it),
+                         "converged": bool(cg.converged),
+                         "true_residual_rel": residual(cg.solution())}
+            if name == "fcg_fp64_learned":
+                row[name]["jacobi_fallback_applications"] = int(cg.inner_total)
+        row["speedup_learned"] = row["fp64_jacobi_pcg"]["ms"] / row["fcg_fp64_learned"]["ms"]
+        log("learned %d^3: PCG %d it %.1f ms; FCG+network %d it %.1f ms"
+            % (n, row["fp64_jacobi_pcg"]["iterations"], row["fp64_jacobi_pcg"]["ms"],
+               row["fcg_fp64_learned"]["iterations"], row["fcg_fp64_learned"]["ms"]))
+        out["eval_%d" % n] = row
+        ctx.close()
+        torch.cuda.empty_cache()
     return out
 
 
@@ -434,6 +534,9 @@ def main():
                     help="timed ChoCG Taylor-Green steps per pressure solver")
     ap.add_argument("--chocg-pc", default="both", choices=["both", "jacobi", "amg"],
                     help="pressure solver(s) of the ChoCG steps")
+    ap.add_argument("--learned-n", type=int, default=64,
+                    help="box size the learned preconditioner is trained on; it is "
+                         "evaluated on the Poisson sizes (0: skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed region, copy a 512 MiB buffer 3 times "
                          "(the known-size kernel that calibrates HBM byte counters)")
@@ -544,6 +647,9 @@ def main():
     for k, pn in enumerate(psizes):
         torch.cuda.empty_cache()
         out["poisson" if k == 0 else "poisson_%d" % pn] = poisson_bench(pn, stream)
+    if args.learned_n > 0 and psizes:
+        torch.cuda.empty_cache()
+        out["learned"] = learned_bench(stream, psizes, train_n=args.learned_n)
     if args.chocg_n > 0 and args.chocg_pc in ("both", "jacobi"):
         torch.cuda.empty_cache()
         out["chocg"] = chocg_bench(args.chocg_n, stream, args.chocg_steps)

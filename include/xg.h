@@ -567,6 +567,19 @@ int xg_cg_solve(xg_ctx* ctx, int64_t maxit, double tol, double* normr,
 int xg_cg_solution(xg_ctx* ctx, double* x);
 int xg_cg_device_x(xg_ctx* ctx, double** x);
 
+/* Mixed precision with reliable updates (no counterpart in the reference):
+ * the Jacobi-PCG recurrence of the system set up by xg_cg_init runs in FP32
+ * (FP32 matrix values, vectors and SpMV, the solution increment kept in
+ * FP32); whenever its residual has fallen by the factor delta (0 < delta <
+ * 1) since the last update, the increment is added to the FP64 x, the FP64
+ * residual b - A x replaces the FP32 one and the recurrence continues with
+ * its search direction.  Stops when ||b - A x|| < tol*normb (FP64, checked
+ * at every update) or after maxit iterations; *normr is that FP64 residual
+ * norm, *nupdates the number of reliable updates. */
+int xg_cg_solve_reliable(xg_ctx* ctx, int64_t maxit, double tol, double delta,
+                         double* normr, int64_t* iters, int64_t* nupdates,
+                         int* converged);
+
 /* Mixed precision solve of the system set up by xg_cg_init (no counterpart
  * in the reference): FP64 residuals r = b - A x, FP32 preconditioned CG on
  * A e = r to ||r_e|| < inner_tol*||r|| (or inner_maxit), x += e, until the

@@ -2,7 +2,9 @@
 // build_tet_layout) with integer corner fluxes: every tet corner's slot must
 // reach exactly its node once (owned nodes directly, halo nodes through the
 // fix-up slots), and every index the kernel forms must stay in bounds.
-// Input file: int64 npoin, ntet; then int32 inpoel[4*ntet].
+// Input file: int64 npoin, ntet; then int32 inpoel[4*ntet].  Optional second
+// argument: tets per chunk (the four-lane kernels' 128, the two-threads-per-
+// tet kozak::rhs's 256).
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -19,7 +21,9 @@
   } while (0)
 
 int main(int argc, char** argv) {
-  CHECK(argc == 2);
+  CHECK(argc == 2 || argc == 3);
+  const int tpc = argc == 3 ? std::atoi(argv[2]) : xg::kTetsPerChunk;
+  CHECK(tpc == xg::kTetsPerChunk || tpc == xg::kTetsPerPairChunk);
   FILE* f = std::fopen(argv[1], "rb");
   CHECK(f);
   int64_t hdr[2];
@@ -32,7 +36,8 @@ int main(int argc, char** argv) {
   std::vector<size_t> inp(t32.begin(), t32.end());
   xg::TetLayout L;
   std::string err;
-  CHECK(xg::build_tet_layout(npoin, ntet, inp.data(), xg::kKozNodes, xg::kKozCap, L, err) == 0);
+  CHECK(xg::build_tet_layout(npoin, ntet, inp.data(), xg::kKozNodes, xg::kKozCap, L, err,
+                              tpc) == 0);
   // corner "flux" of tet e at corner a: a distinct integer
   auto flux = [&](int64_t e, int a) { return (int64_t)(e * 4 + a + 1); };
   std::vector<int64_t> R(npoin, 0), S(L.nslot, 0), expect(npoin, 0);
@@ -65,15 +70,17 @@ int main(int argc, char** argv) {
   int maxloc = 0;
   for (size_t b = 0; b < L.blk.size(); ++b) {
     const auto& B = L.blk[b];
-    CHECK(B.nloc <= xg::kKozCap && B.nloc <= xg::kChunkSlots && B.nown >= 1);
+    CHECK(B.nloc <= xg::kKozCap && B.nown >= 1);
     maxloc = std::max(maxloc, B.nloc);
     std::vector<int> glob(B.nloc);
     for (int i = 0; i < B.nloc; ++i)
       glob[i] = i < B.nown ? B.own0 + i : L.halo[B.halo_off + i - B.nown];
-    std::vector<int64_t> acc(B.nloc, 0), F(xg::kChunkSlots, 0);
+    std::vector<int64_t> acc(B.nloc, 0), F(4 * tpc, 0);
     for (int ch = B.chunk0; ch < B.chunk1; ++ch) {
       const auto& C = L.chunks[ch];
-      CHECK(C.kind == 2 && C.count >= 1 && C.count <= xg::kTetsPerChunk);
+      CHECK(C.kind == 2 && C.count >= 1 && C.count <= tpc);
+      // the kernels load at most two entries per thread of 512
+      CHECK(4 * C.count <= 2 * 512);
       for (int t = 0; t < 4 * C.count; ++t) {
         int i = t >> 2, a = t & 3;
         int64_t pos = C.first + i;

@@ -3,7 +3,8 @@ through size-independent properties and, where it finishes in seconds, the
 oracle:
 
 * the 128^3 pressure-Poisson system: every solver (FP64 Jacobi-PCG, FP32
-  inner CG with FP64 refinement, FCG with an FP32 inner CG, FCG with the
+  inner CG with FP64 refinement, the FP32 recurrence with FP64 reliable
+  updates, FCG with an FP32 inner CG, FCG with the
   multigrid V-cycle) reaches a true relative residual below 1e-10,
   recomputed in FP64 on the host from the returned solution, and the
   solutions agree with the FP64 Jacobi-PCG one to the conditioning times
@@ -96,12 +97,14 @@ def test_poisson_128_every_solver_to_a_true_1e10():
     x0 = np.zeros(A.nrow)
     _say("  setup %.1fs, N=%d" % (time.time() - t0, A.nrow))
     sols = {}
-    for name in ("fp64_pcg", "mixed", "fcg_cg32", "fcg_amg"):
+    for name in ("fp64_pcg", "mixed", "reliable", "fcg_cg32", "fcg_amg"):
         cg.init(x0, b, dirbc=dirbc, apply=True, pc="jacobi")
         if name == "fp64_pcg":
             cg.solve(100000, 1e-10)
         elif name == "mixed":
             cg.solve_mixed(100000, 1e-10, inner_tol=1e-5, inner_maxit=5000)
+        elif name == "reliable":
+            cg.solve_reliable(100000, 1e-10, delta=0.1)
         elif name == "fcg_cg32":
             cg.solve_flexible(100000, 1e-10, pc="cg32", inner=30)
         else:

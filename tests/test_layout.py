@@ -112,8 +112,9 @@ def _tet_exe():
     return TET_EXE
 
 
-@pytest.mark.parametrize("order", ["lex", "rcm+block"])
-def test_tet_layout_matches_direct_scatter(tmp_path, order):
+@pytest.mark.parametrize("order,tpc", [("lex", 128), ("rcm+block", 128), ("lex", 256),
+                                       ("rcm+block", 256)])
+def test_tet_layout_matches_direct_scatter(tmp_path, order, tpc):
     """kozak::rhs layout: each tet corner reaches its node exactly once, every
     index the kernel forms is in bounds (CPU emulation, integer fluxes)."""
     m = box_mesh(20, seed=9)
@@ -124,6 +125,6 @@ def test_tet_layout_matches_direct_scatter(tmp_path, order):
     with open(f, "wb") as fh:
         np.array([m["coord"].shape[1], m["inpoel"].shape[0]], np.int64).tofile(fh)
         m["inpoel"].astype(np.int32).tofile(fh)
-    r = subprocess.run([_tet_exe(), str(f)], capture_output=True, text=True)
+    r = subprocess.run([_tet_exe(), str(f), str(tpc)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     assert r.stdout.startswith("ok")

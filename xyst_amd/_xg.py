@@ -117,6 +117,8 @@ SIG = {
     "xg_cg_device_x": (C.c_int, [VP, C.POINTER(VP)]),
     "xg_cg_solve_mixed": (C.c_int, [VP, I64, D, D, I64, C.POINTER(D), C.POINTER(I64),
                                     C.POINTER(I64), C.POINTER(C.c_int)]),
+    "xg_cg_solve_reliable": (C.c_int, [VP, I64, D, D, C.POINTER(D), C.POINTER(I64),
+                                       C.POINTER(I64), C.POINTER(C.c_int)]),
     "xg_cg_solve_flexible": (C.c_int, [VP, C.c_int, I64, I64, D, C.POINTER(D),
                                        C.POINTER(I64), C.POINTER(I64), C.POINTER(C.c_int)]),
     "xg_gnn_create": (C.c_int, [VP, I64, I64]),

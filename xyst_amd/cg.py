@@ -157,6 +157,18 @@ class ConjugateGradients:
         self.nouter = no.value
         return self.normr
 
+    def solve_reliable(self, maxit, tol, delta=0.1):
+        """FP32 Jacobi-PCG recurrence with FP64 reliable updates whenever its
+        residual has fallen by delta (no reference counterpart); returns the
+        FP64 residual norm ||b - A x||."""
+        nr, it, nu, cv = C.c_double(), C.c_int64(), C.c_int64(), C.c_int()
+        self.ctx._check(lib().xg_cg_solve_reliable(
+            self.ctx._h, maxit, tol, delta, C.byref(nr), C.byref(it), C.byref(nu),
+            C.byref(cv)))
+        self.normr, self.it, self.converged = nr.value, it.value, bool(cv.value)
+        self.nupdates = nu.value
+        return self.normr
+
     def solve_flexible(self, maxit, tol, pc="jacobi", inner=20):
         """FP64 flexible CG with a Jacobi, an FP32 inner-CG ("cg32"), the
         learned ("gnn", xyst_amd.gnn) or a smoothed-aggregation multigrid

@@ -342,6 +342,31 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_update(
 
 // ---- mixed precision helpers ----
 
+// reliable update of the FP32 recurrence (the search direction is kept):
+// r32 = (float) r64 (the FP64 residual b - A x), z32 = r32 / d32, e32 = 0;
+// partials of (r,z) and (r,r) into `buf`, the slot the next iteration's
+// stop test and beta read
+__global__ __launch_bounds__(kCgThreads) void k_cg32_replace(
+    int64_t n, const double* __restrict__ rin, const float* __restrict__ d,
+    float* __restrict__ r, float* __restrict__ z, float* __restrict__ e,
+    double* __restrict__ buf) {
+  __shared__ double sm[kCgThreads / 64];
+  double rz = 0.0, rr = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kCgThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kCgThreads) {
+    const float ri = (float)rin[i];
+    const float zi = ri / d[i];
+    r[i] = ri;
+    z[i] = zi;
+    e[i] = 0.0f;
+    rz += (double)ri * (double)zi;
+    rr += (double)ri * (double)ri;
+  }
+  put_part(buf, blockIdx.x, rz, sm);
+  put_part(buf + kVecParts, blockIdx.x, rr, sm);
+  if (blockIdx.x == 0 && threadIdx.x == 0) buf[2 * kVecParts] = 0.0;
+}
+
 // x += (double)e
 __global__ __launch_bounds__(kCgThreads) void k_add_f32(
     int64_t n, const float* __restrict__ e, double* __restrict__ x) {
@@ -605,11 +630,11 @@ int read_state(xg_ctx* c) {
 // state every `batch` iterations.  T: FP64 solve or the FP32 inner solve.
 template <class T>
 int run_cg(xg_ctx* c, const T* a, T* z, T* p0, T* p1, T* q, const T* d, T* r, T* x,
-           int64_t maxit, double tol_nb, double brk) {
+           int64_t maxit, double tol_nb, double brk, int64_t jstart = 1) {
   CgSolver* s = c->cg;
   const int gr = grid_rows(s->nrow), gv = grid_vec(s->nrow);
   const int batch = 8;
-  for (int64_t j0 = 1;; j0 += batch) {
+  for (int64_t j0 = jstart;; j0 += batch) {
     if (j0 > maxit + 1 + batch)
       return fail(c, XG_EHIP, "CG: stop test not reached on the device");
     for (int i = 0; i < batch; ++i) {
@@ -1037,6 +1062,79 @@ int xg_cg_solve_mixed(xg_ctx* c, int64_t maxit, double tol, double inner_tol,
   if (normr) *normr = nr;
   if (iters) *iters = total;
   if (nouter) *nouter = outer;
+  if (converged) *converged = conv;
+  s->ready = false;
+  s->restore = s->dev_bc;
+  return XG_OK;
+}
+
+// Mixed precision with reliable updates (no counterpart in the reference):
+// the Jacobi-PCG recurrence runs in FP32 (FP32 matrix values, vectors and
+// SpMV; the solution increment accumulated in FP32 and added to the FP64
+// x).  Whenever the FP32 residual has fallen by `delta` since the last
+// update, x takes the increment, the FP64 residual r = b - A x is computed
+// and replaces the FP32 one, and the recurrence continues with its search
+// direction (Sleijpen and van der Vorst's reliable updates).  Stops when
+// the FP64 residual ||b - A x|| < tol ||b||.
+int xg_cg_solve_reliable(xg_ctx* c, int64_t maxit, double tol, double delta,
+                         double* normr, int64_t* iters, int64_t* nupdates,
+                         int* converged) {
+  int rc = check_cg(c);
+  if (rc) return rc;
+  CgSolver* s = c->cg;
+  if (maxit < 1 || !(tol > 0.0) || !(delta > 0.0 && delta < 1.0))
+    return fail(c, XG_EINVAL, "xg_cg_solve_reliable: bad arguments");
+  XG_HIP(c, hipSetDevice(c->device));
+  if (s->restore && (rc = restore_values(c))) return rc;
+  const int64_t n = s->nrow;
+  const int gr = grid_rows(n), gv = grid_vec(n);
+  hipLaunchKernelGGL(k_cg_residual, dim3(gr), dim3(kCgThreads), 0, c->stream, n,
+                     s->dia, s->dja, s->da, s->x, s->b, s->r, s->d, s->pc);
+  hipLaunchKernelGGL(k_f32_from_f64, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n,
+                     s->d, s->d32);
+  hipLaunchKernelGGL(k_cg_init, dim3(gv), dim3(kCgThreads), 0, c->stream, n, s->b,
+                     s->r, s->d, s->z, s->p0, s->rzb, s->pq);
+  hipLaunchKernelGGL(k_cg_setup_final, dim3(1), dim3(kCgThreads), 0, c->stream, s->pq,
+                     s->rzb, gv, s->st);
+  if ((rc = read_state(c))) return rc;
+  const double normb = s->hst->normb;
+  const double target = tol * (normb > 1.0e-14 ? normb : 1.0);
+  int64_t done = 0, nup = 0, j = 1;
+  double nr = 0.0;
+  int conv = 0;
+  for (;;) {
+    // the FP64 residual of x
+    hipLaunchKernelGGL(k_residual_norm, dim3(gr), dim3(kCgThreads), 0, c->stream, n,
+                       s->dia, s->dja, s->da, s->x, s->b, s->r, s->pq);
+    hipLaunchKernelGGL(k_sum_to_state, dim3(1), dim3(kCgThreads), 0, c->stream, s->pq,
+                       gr, s->st);
+    if ((rc = read_state(c))) return rc;
+    nr = sqrt(s->hst->rr);
+    if (nr < target) { conv = 1; break; }
+    if (done >= maxit) break;
+    if (j == 1) {   // first pass: p_0 = r
+      hipLaunchKernelGGL(k_cg32_init, dim3(gv), dim3(kCgThreads), 0, c->stream, n, s->r,
+                         s->d32, s->r32, s->z32, s->p32a, s->e32, s->rzb);
+    } else {        // reliable update: the recurrence goes on from the true residual
+      ++nup;
+      hipLaunchKernelGGL(k_cg32_replace, dim3(gv), dim3(kCgThreads), 0, c->stream, n,
+                         s->r, s->d32, s->r32, s->z32, s->e32,
+                         s->rzb + ((j - 1) & 1) * kBufLen);
+    }
+    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, c->stream, s->st);
+    rc = run_cg<float>(c, s->da32, s->z32, s->p32a, s->p32b, s->q32, s->d32, s->r32,
+                       s->e32, maxit, delta * nr, 1.0e-300, j);
+    if (rc) return rc;
+    const int64_t it = s->hst->it;       // iterations done in total
+    hipLaunchKernelGGL(k_add_f32, dim3(gv), dim3(kCgThreads), 0, c->stream, n, s->e32,
+                       s->x);
+    if (it < j) break;                   // no progress possible in FP32
+    done = it;
+    j = it + 1;
+  }
+  if (normr) *normr = nr;
+  if (iters) *iters = done;
+  if (nupdates) *nupdates = nup;
   if (converged) *converged = conv;
   s->ready = false;
   s->restore = s->dev_bc;

@@ -45,7 +45,8 @@ constexpr int kChunk = 64;             // nodes per backward chunk
 
 static_assert(kOffBo + 1 == XG_GNN_NPARAM, "parameter layout");
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 __device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
@@ -136,9 +137,10 @@ __global__ __launch_bounds__(kThreads) void k_gnn_agg(
   M[i * kC + c] = f2bf(acc);
 }
 
-// h_out = h + relu(W [h; m] + b) for 32 nodes per wave on the 32x32x8 BF16
-// MFMA: lane l supplies A[l%32][k0..k0+3] (node row) and B[k0..k0+3][l%32]
-// (= W[l%32][k0..]) with k0 = 8 kb + 4 (l/32); it receives D[i][l%32] for
+// h_out = h + relu(W [h; m] + b) for 32 nodes per wave on gfx950's
+// 32x32x16 BF16 MFMA (FP32 accumulation), four steps over the 64 inputs:
+// lane l supplies A[l%32][k0..k0+7] (node row) and B[k0..k0+7][l%32]
+// (= W[l%32][k0..]) with k0 = 16 kb + 8 (l/32); it receives D[i][l%32] for
 // i = 8 (v/4) + 4 (l/32) + v%4, v = 0..15.
 __global__ __launch_bounds__(kThreads) void k_gnn_mlp(
     int64_t n, const uint16_t* __restrict__ Hin, const uint16_t* __restrict__ Min,
@@ -152,15 +154,17 @@ __global__ __launch_bounds__(kThreads) void k_gnn_mlp(
   const bool valid = node < n;
   floatx16 acc = {};
 #pragma unroll
-  for (int kb = 0; kb < 8; ++kb) {
-    const int k0 = kb * 8 + kq * 4;
-    bf16x4 av = {0, 0, 0, 0};
+  for (int kb = 0; kb < 4; ++kb) {
+    const int k0 = kb * 16 + kq * 8;     // eight inputs, all from h or all from m
+    u16x8 av = {0, 0, 0, 0, 0, 0, 0, 0};
     if (valid) {
       const uint16_t* src = k0 < kC ? Hin + node * kC + k0 : Min + node * kC + (k0 - kC);
-      av = *(const bf16x4*)src;
+      av = *(const u16x8*)src;
     }
-    const bf16x4 bv = *(const bf16x4*)(Wb + ri * (2 * kC) + k0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+    const u16x8 bv = *(const u16x8*)(Wb + ri * (2 * kC) + k0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av),
+                                                  __builtin_bit_cast(bf16x8, bv), acc, 0, 0,
+                                                  0);
   }
   const float bj = b[ri];
 #pragma unroll

@@ -4,15 +4,23 @@
 // Layout (xg_layout.cpp, build_tet_layout): node blocks of consecutive ids;
 // a block owns the tets whose smallest node it holds.  One 512-thread
 // workgroup per block (two share a CU) stages the conserved state and
-// coordinates of the block's nodes (owned + halo) in LDS.  Per chunk of 128
-// tets, thread t
-// handles corner a = t % 4 of tet t / 4: it evaluates its node's share of the
-// half-step element state, the four corners' shares are combined with two
-// lane exchanges (every corner sees the same sum), and the thread writes its
-// corner's second-step flux to LDS.  Then the thread owning local node t
-// adds the chunk's corner fluxes that touch it, in slot order; halo partials
-// go to slots that k_fixup adds to their owners in block order.  No atomics:
-// every sum has a fixed order and the result is bitwise reproducible.
+// coordinates of the block's nodes (owned + halo) in LDS and walks the
+// block's tets chunk by chunk, the next chunk's tet records and entry lists
+// loaded while the current one computes.
+//  * kozak::rhs (k_kozak_pairs): chunks of 256 tets, two threads per tet;
+//    each thread evaluates the tet geometry and two corners' shares of the
+//    half-step state, one lane exchange with its partner completes the
+//    state, and the thread writes its two corners' second-step fluxes.
+//  * with the antidiffusive contributions of the FCT step (k_kozak_blocks)
+//    and in the FCT kernels: chunks of 128 tets, thread t handles corner
+//    a = t % 4 of tet t / 4; the four corners combine their shares with two
+//    lane exchanges within the quad (DPP), so every corner sees the same sum.
+// Then the thread owning local node t adds the chunk's corner values that
+// touch it, in slot order; halo partials go to slots that k_fixup adds to
+// their owners in block order.  No floating-point atomic sums: every sum has
+// a fixed order and the result is bitwise reproducible (the FCT bounds
+// kernel merges max/min with LDS ds_max_f64/ds_min_f64, which are exact and
+// independent of order).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,6 +48,12 @@ struct KozakMesh {
   int* chk_ent = nullptr;
   uint16_t *tend = nullptr, *entries = nullptr;
   uint32_t *tab = nullptr, *tcd = nullptr;
+  // the same blocks cut into chunks of kTetsPerPairChunk tets (kozak::rhs
+  // without the antidiffusive contributions, two threads per tet)
+  int4 *blk1p = nullptr, *chkp = nullptr;
+  int* chk_entp = nullptr;
+  uint16_t *tendp = nullptr, *entriesp = nullptr;
+  uint32_t *tabp = nullptr, *tcdp = nullptr;
   double* slots = nullptr;
   int nfx = 0;
   int *fx_node = nullptr, *fx_off = nullptr, *fx_slot = nullptr;
@@ -62,6 +76,43 @@ namespace {
 
 constexpr int kKozThreads = xg::kKozSlots;
 static_assert(xg::kKozCap <= kKozThreads, "one local node per thread");
+
+// v from lane (lane ^ 1) or (lane ^ 2) of the same quad: two v_mov_b32_dpp
+// with quad_perm [1,0,3,2] (0xB1) or [2,3,0,1] (0x4E).  Every lane of the
+// wave is active where these are used.
+template <int CTRL>
+__device__ inline double quad_swap(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline double quad_xor1(double v) { return quad_swap<0xB1>(v); }
+__device__ inline double quad_xor2(double v) { return quad_swap<0x4E>(v); }
+
+// One chunk's tet record (corner a of tet i reads its tet's four local ids)
+// and entry lists, loaded one chunk ahead so that their global-memory
+// latency overlaps the current chunk's work.
+struct TetPrefetch {
+  int4 C = make_int4(0, 0, 0, 0);   // kind, first, count, tend_off
+  uint32_t ab = 0, cd = 0;
+  uint16_t ent = 0, te = 0;
+  __device__ inline void load(int ch, int chunk1, const int4* __restrict__ chk,
+                              const int* __restrict__ chk_ent,
+                              const uint16_t* __restrict__ tend,
+                              const uint16_t* __restrict__ entries,
+                              const uint32_t* __restrict__ tab,
+                              const uint32_t* __restrict__ tcd, int t, int nloc) {
+    if (ch >= chunk1) return;
+    C = chk[ch];
+    const int i = t >> 2;
+    // a valid tet for idle lanes, so the quad exchanges run with every lane
+    const int e = C.y + (i < C.z ? i : 0);
+    ab = tab[e];
+    cd = tcd[e];
+    ent = t < 4 * C.z ? entries[chk_ent[ch] + t] : 0;
+    te = t < nloc ? tend[C.w + t] : 0;
+  }
+};
 
 __device__ inline void cross(const double a[3], const double b[3], double c[3]) {
   c[0] = a[1] * b[2] - b[1] * a[2];
@@ -139,19 +190,15 @@ __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))
   double acc[NO];
 #pragma unroll
   for (int c = 0; c < NO; ++c) acc[c] = 0.0;
+  TetPrefetch pf;
+  pf.load(B1.y, B1.z, chk, chk_ent, tend, entries, tab, tcd, t, nloc);
   for (int ch = B1.y; ch < B1.z; ++ch) {
-    const int4 C = chk[ch];   // kind, first, count, tend_off
-    const int eoff = chk_ent[ch];
-    // this chunk's tet (a valid one for idle lanes, so the lane exchanges
-    // below run with every lane active)
-    const int e = C.y + (i < C.z ? i : 0);
-    const uint32_t ab = tab[e], cd = tcd[e];
-    uint16_t ent = 0, te = 0;
-    if (t < 4 * C.z) ent = entries[eoff + t];
-    if (t < nloc) te = tend[C.w + t];
+    const int4 C = pf.C;
+    const uint32_t ab = pf.ab, cd = pf.cd;
     __syncthreads();          // staging done / previous chunk's F consumed
-    if (t < 4 * C.z) sE[t] = ent;
-    if (t < nloc) sT[t] = te;
+    if (t < 4 * C.z) sE[t] = pf.ent;
+    if (t < nloc) sT[t] = pf.te;
+    pf.load(ch + 1, B1.z, chk, chk_ent, tend, entries, tab, tcd, t, nloc);
     double aec[5];
     {
       const int N[4] = {(int)(ab & 0xffff), (int)(ab >> 16), (int)(cd & 0xffff),
@@ -210,8 +257,8 @@ __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))
       double ue[5];
 #pragma unroll
       for (int c = 0; c < 5; ++c) {
-        double v = h[c] + __shfl_xor(h[c], 1, 64);
-        ue[c] = v + __shfl_xor(v, 2, 64);
+        double v = h[c] + quad_xor1(h[c]);
+        ue[c] = v + quad_xor2(v);
       }
       // second half step (Kozak.cpp:138-161): corner a's flux
       const double ie = xg::rcp(ue[0]);
@@ -276,6 +323,204 @@ __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))
   }
 }
 
+// kozak::rhs without the antidiffusive contributions, two threads per tet:
+// thread t handles corners 2h and 2h + 1 (h = t & 1) of chunk tet t >> 1 on
+// chunks of kTetsPerPairChunk tets.  Each thread evaluates the tet geometry
+// and its two corners' shares of the half-step state; one lane exchange with
+// its partner completes the state ((h0 + h1) + (h2 + h3) in both lanes, the
+// sum the four-lane kernel forms), and each thread writes its two corners'
+// second-step fluxes.  Accumulation, halo slots and fix-up as in
+// k_kozak_blocks.  SRC: the problem's source term (Kozak.cpp:124-136,
+// 163-173).
+template <bool SRC>
+__global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_kozak_pairs(
+    int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
+    const int* __restrict__ halo, const int4* __restrict__ chk,
+    const int* __restrict__ chk_ent, const uint16_t* __restrict__ tend,
+    const uint16_t* __restrict__ entries, const uint32_t* __restrict__ tab,
+    const uint32_t* __restrict__ tcd, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ U, double* __restrict__ R,
+    double* __restrict__ S, double gm1, double dt, xg::Prob P) {
+  constexpr int CAP = xg::kKozCap;
+  constexpr int NS = xg::kKozPairSlots;
+  static_assert(NS == 2 * kKozThreads, "two corner slots per thread");
+  __shared__ double sU[5 * CAP];
+  __shared__ double sX[3 * CAP];
+  __shared__ double F[NS * 5];
+  __shared__ uint16_t sE[NS];
+  __shared__ uint16_t sT[CAP];
+  const int t = threadIdx.x;
+  const int b = xcd_block(blockIdx.x, nblk);
+  const int4 B0 = blk0[b];   // own0, nown, nloc, halo_off
+  const int4 B1 = blk1[b];   // slot0, chunk0, chunk1
+  const int own0 = B0.x, nown = B0.y, nloc = B0.z;
+  if (t < nloc) {
+    int g = t < nown ? own0 + t : halo[B0.w + t - nown];
+    double s[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) s[c] = U[(size_t)g * 5 + c];
+    double xg_ = x[g], yg = y[g], zg = z[g];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) sU[c * CAP + t] = s[c];
+    sX[0 * CAP + t] = xg_;
+    sX[1 * CAP + t] = yg;
+    sX[2 * CAP + t] = zg;
+  }
+  const int i = t >> 1, h = t & 1;
+  // the next chunk's tet record and entry lists (two entries per thread)
+  int4 Cn = make_int4(0, 0, 0, 0);
+  uint32_t abn = 0, cdn = 0;
+  uint16_t en0 = 0, en1 = 0, ten = 0;
+  auto fetch = [&](int ch) {
+    if (ch >= B1.z) return;
+    Cn = chk[ch];
+    const int e = Cn.y + (i < Cn.z ? i : 0);   // a valid tet for idle lanes
+    abn = tab[e];
+    cdn = tcd[e];
+    const int off = chk_ent[ch], ne = 4 * Cn.z;
+    en0 = t < ne ? entries[off + t] : 0;
+    en1 = kKozThreads + t < ne ? entries[off + kKozThreads + t] : 0;
+    ten = t < nloc ? tend[Cn.w + t] : 0;
+  };
+  fetch(B1.y);
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int ch = B1.y; ch < B1.z; ++ch) {
+    const int4 C = Cn;
+    const uint32_t ab = abn, cd = cdn;
+    __syncthreads();          // staging done / previous chunk's F consumed
+    if (t < 4 * C.z) sE[t] = en0;
+    if (kKozThreads + t < 4 * C.z) sE[kKozThreads + t] = en1;
+    if (t < nloc) sT[t] = ten;
+    fetch(ch + 1);
+    {
+      const int N[4] = {(int)(ab & 0xffff), (int)(ab >> 16), (int)(cd & 0xffff),
+                        (int)(cd >> 16)};
+      double X[4][3];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) X[k][j] = sX[j * CAP + N[k]];
+      double ba[3], ca[3], da[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        ba[j] = X[1][j] - X[0][j];
+        ca[j] = X[2][j] - X[0][j];
+        da[j] = X[3][j] - X[0][j];
+      }
+      double g1[3], g2[3], g3[3];
+      cross(ca, da, g1);
+      cross(da, ba, g2);
+      cross(ba, ca, g3);
+      const double J = ba[0] * g1[0] + ba[1] * g1[1] + ba[2] * g1[2];
+      // this thread's corners: (0, 1) or (2, 3)
+      double gA[3], gB[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        gA[j] = h ? g2[j] : -g1[j] - g2[j] - g3[j];
+        gB[j] = h ? g3[j] : g1[j];
+      }
+      const int nA = h ? N[2] : N[0], nB = h ? N[3] : N[1];
+      const double coef = dt * xg::rcp(J) * 0.5;
+      // the two corners' shares of the half-step state (Kozak.cpp:92-124)
+      double hs[5];
+#pragma unroll
+      for (int c = 0; c < 5; ++c) hs[c] = 0.0;
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int na = k ? nB : nA;
+        const double* ga = k ? gB : gA;
+        double u[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) u[c] = sU[c * CAP + na];
+        const double ir = xg::rcp(u[0]);
+        const double p = (u[4] - 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) * ir) * gm1;
+        double hh[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) hh[c] = 0.25 * u[c];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const double cg = coef * ga[j];
+          const double uj = u[j + 1] * ir;
+          hh[0] -= cg * u[j + 1];
+          hh[1] -= cg * u[1] * uj;
+          hh[2] -= cg * u[2] * uj;
+          hh[3] -= cg * u[3] * uj;
+          hh[j + 1] -= cg * p;
+          hh[4] -= cg * (u[4] + p) * uj;
+        }
+        if constexpr (SRC) {
+          double sa[5];
+          xg::source(P, sX[0 * CAP + na], sX[1 * CAP + na], sX[2 * CAP + na], sa);
+          const double c8 = dt / 8.0;
+#pragma unroll
+          for (int c = 0; c < 5; ++c) hh[c] += c8 * sa[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 5; ++c) hs[c] = k ? hs[c] + hh[c] : hh[c];
+      }
+      // the partner holds the other two corners' sum
+      double ue[5];
+#pragma unroll
+      for (int c = 0; c < 5; ++c) ue[c] = hs[c] + quad_xor1(hs[c]);
+      // second half step (Kozak.cpp:138-161): the two corners' fluxes
+      const double ie = xg::rcp(ue[0]);
+      const double pr = (ue[4] - 0.5 * (ue[1] * ue[1] + ue[2] * ue[2] + ue[3] * ue[3]) * ie) * gm1;
+      double fl[3][5];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double uj = ue[j + 1] * ie;
+        fl[j][0] = ue[j + 1];
+        fl[j][1] = ue[1] * uj;
+        fl[j][2] = ue[2] * uj;
+        fl[j][3] = ue[3] * uj;
+        fl[j][j + 1] += pr;
+        fl[j][4] = (ue[4] + pr) * uj;
+      }
+      double se[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      double cj = 0.0;
+      if constexpr (SRC) {
+        xg::source(P, (X[0][0] + X[1][0] + X[2][0] + X[3][0]) / 4.0,
+                   (X[0][1] + X[1][1] + X[2][1] + X[3][1]) / 4.0,
+                   (X[0][2] + X[1][2] + X[2][2] + X[3][2]) / 4.0, se);
+        cj = J / 24.0;
+      }
+      if (i < C.z) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          const double* ga = k ? gB : gA;
+          const int slot = 4 * i + 2 * h + k;
+#pragma unroll
+          for (int c = 0; c < 5; ++c) {
+            double f = (ga[0] * (1.0 / 6.0)) * fl[0][c] + (ga[1] * (1.0 / 6.0)) * fl[1][c] +
+                       (ga[2] * (1.0 / 6.0)) * fl[2][c];
+            if constexpr (SRC) f += cj * se[c];
+            F[slot * 5 + c] = f;
+          }
+        }
+      }
+    }
+    __syncthreads();          // F, sE, sT complete
+    if (t < nloc) {
+      int e0 = t ? sT[t - 1] : 0, e1 = sT[t];
+      for (int j = e0; j < e1; ++j) {
+        const double* fs = F + (int)sE[j] * 5;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) acc[c] += fs[c];
+      }
+    }
+  }
+  if (t < nown) {
+    double* o = R + (size_t)(own0 + t) * 5;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) o[c] = acc[c];
+  } else if (t < nloc) {
+    double* o = S + (size_t)(B1.x + t - nown) * 5;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) o[c] = acc[c];
+  }
+}
+
 // ---------------------------------------------------------------- FCT --
 // The KozCG step after the rhs (KozCG.cpp:866-1197).  W holds per node the
 // summed rhs and P+/- (stride 15); the kernels below follow the reference's
@@ -306,10 +551,14 @@ __device__ inline int local_to_global(int t, int own0, int nown,
 
 // Allowed limits Q+/- (KozCG.cpp:901-935): per tet the max/min over its
 // corners of the low-order solution (with CLIP = false also of U), taken
-// into every corner's running max/min.  Max and min are exact and
-// order-free, so the block gathers them with LDS atomics; halo partials go
+// into every corner's running max/min.  Each local node's own bound pair
+// (max and min of its low-order solution and, without clipping, its U) is
+// staged in LDS once; per chunk the four corners of a tet exchange theirs in
+// the quad, and the corner merges the tet's bounds into its node's running
+// max/min with LDS ds_max_f64/ds_min_f64 -- max and min are exact and
+// independent of order, so the result is deterministic.  Halo partials go
 // to slots that k_fixup_minmax merges.  Owners also store the low-order
-// solution UL.
+// solution UL.  LDS: 80 KiB, two workgroups per CU.
 template <bool CLIP>
 __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_kozcg_alw(
     int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
@@ -320,46 +569,57 @@ __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))
     double* __restrict__ Q, double* __restrict__ S) {
   constexpr int CAP = xg::kKozCap;
   constexpr double kLarge = 1.7976931348623157e308;
-  __shared__ double sL[5 * CAP];     // low-order solution of the local nodes
+  __shared__ double sB[10 * CAP];    // each node's own max (5) and min (5)
   __shared__ double sQ[10 * CAP];    // running max (5) and min (5)
-  __shared__ int sG[CAP];            // global ids (CLIP = false reads U)
   const int t = threadIdx.x;
   const int b = xcd_block(blockIdx.x, nblk);
   const int4 B0 = blk0[b];
   const int4 B1 = blk1[b];
   const int own0 = B0.x, nown = B0.y, nloc = B0.z;
+  const int i = t >> 2, a = t & 3;
+  // the first chunk's tet, loaded while the block stages
+  int4 Cn = make_int4(0, 0, 0, 0);
+  uint32_t wn = 0;
+  auto fetch = [&](int ch) {
+    if (ch < B1.z) {
+      Cn = chk[ch];
+      const int e = Cn.y + (i < Cn.z ? i : 0);
+      wn = (a < 2) ? tab[e] : tcd[e];
+    }
+  };
+  fetch(B1.y);
   if (t < nloc) {
     const int g = local_to_global(t, own0, nown, halo, B0.w);
-    if (!CLIP) sG[t] = g;
     double ul[5];
     low_order(W, U, vol[g], dt, g, ul);
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
-      sL[c * CAP + t] = ul[c];
+      double hi = ul[c], lo = ul[c];
+      if (!CLIP) {
+        const double u = U[(size_t)g * 5 + c];
+        hi = smax(hi, u);
+        lo = smin(lo, u);
+      }
+      sB[c * CAP + t] = hi;
+      sB[(5 + c) * CAP + t] = lo;
       sQ[c * CAP + t] = -kLarge;
       sQ[(5 + c) * CAP + t] = kLarge;
       if (t < nown) UL[(size_t)g * 5 + c] = ul[c];
     }
   }
   __syncthreads();
-  const int i = t >> 2, a = t & 3;
   for (int ch = B1.y; ch < B1.z; ++ch) {
-    const int4 C = chk[ch];
-    const int e = C.y + (i < C.z ? i : 0);
-    const uint32_t w = (a < 2) ? tab[e] : tcd[e];
+    const int4 C = Cn;
+    const uint32_t w = wn;
+    fetch(ch + 1);
     const int na = (a & 1) ? (int)(w >> 16) : (int)(w & 0xffff);
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
-      double hi = sL[c * CAP + na], lo = hi;
-      if (!CLIP) {
-        const double u = U[(size_t)sG[na] * 5 + c];
-        hi = smax(hi, u);
-        lo = smin(lo, u);
-      }
-      hi = fmax(hi, __shfl_xor(hi, 1, 64));
-      hi = fmax(hi, __shfl_xor(hi, 2, 64));
-      lo = fmin(lo, __shfl_xor(lo, 1, 64));
-      lo = fmin(lo, __shfl_xor(lo, 2, 64));
+      double hi = sB[c * CAP + na], lo = sB[(5 + c) * CAP + na];
+      hi = fmax(hi, quad_xor1(hi));
+      hi = fmax(hi, quad_xor2(hi));
+      lo = fmin(lo, quad_xor1(lo));
+      lo = fmin(lo, quad_xor2(lo));
       if (i < C.z) {
         __hip_atomic_fetch_max(&sQ[c * CAP + na], hi, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -456,17 +716,31 @@ __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))
   }
   const int i = t >> 2, a = t & 3;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  TetPrefetch pf;
+  pf.load(B1.y, B1.z, chk, chk_ent, tend, entries, tab, tcd, t, nloc);
+  __syncthreads();            // staging done
+  // the corner node's limit coefficients C+ (5) and C- (5), loaded one
+  // chunk ahead like the tets
+  double cfn[10];
+  auto fetch_cf = [&](const TetPrefetch& P) {
+    const uint32_t w = (a < 2) ? P.ab : P.cd;
+    const int na = (a & 1) ? (int)(w >> 16) : (int)(w & 0xffff);
+    const double* cf = Cf + (size_t)sG[na] * 10;
+#pragma unroll
+    for (int c = 0; c < 10; ++c) cfn[c] = cf[c];
+  };
+  if (B1.y < B1.z) fetch_cf(pf);
   for (int ch = B1.y; ch < B1.z; ++ch) {
-    const int4 C = chk[ch];
-    const int eoff = chk_ent[ch];
-    const int e = C.y + (i < C.z ? i : 0);
-    const uint32_t ab = tab[e], cd = tcd[e];
-    uint16_t ent = 0, te = 0;
-    if (t < 4 * C.z) ent = entries[eoff + t];
-    if (t < nloc) te = tend[C.w + t];
-    __syncthreads();          // staging done / previous chunk's F consumed
-    if (t < 4 * C.z) sE[t] = ent;
-    if (t < nloc) sT[t] = te;
+    const int4 C = pf.C;
+    const uint32_t ab = pf.ab, cd = pf.cd;
+    double cf[10];
+#pragma unroll
+    for (int c = 0; c < 10; ++c) cf[c] = cfn[c];
+    __syncthreads();          // previous chunk's F consumed
+    if (t < 4 * C.z) sE[t] = pf.ent;
+    if (t < nloc) sT[t] = pf.te;
+    pf.load(ch + 1, B1.z, chk, chk_ent, tend, entries, tab, tcd, t, nloc);
+    if (ch + 1 < B1.z) fetch_cf(pf);
     {
       const int N[4] = {(int)(ab & 0xffff), (int)(ab >> 16), (int)(cd & 0xffff),
                         (int)(cd >> 16)};
@@ -477,13 +751,12 @@ __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))
         for (int j = 0; j < 3; ++j) X[k][j] = sX[j * CAP + N[k]];
       double aec[5], coef[5];
       tet_aec(X, sU, CAP, N, a, ctau, aec);
-      const double* cf = Cf + (size_t)sG[N[a]] * 10;
 #pragma unroll
       for (int c = 0; c < 5; ++c) {
         double q = aec[c] > 0.0 ? cf[c] : cf[5 + c];
         // min over the four corners, starting from 1 (exact, order-free)
-        q = fmin(q, __shfl_xor(q, 1, 64));
-        q = fmin(q, __shfl_xor(q, 2, 64));
+        q = fmin(q, quad_xor1(q));
+        q = fmin(q, quad_xor2(q));
         coef[c] = smin(1.0, q);
       }
       if (sysmask) {
@@ -581,7 +854,14 @@ void launch_kozak_blocks(xg_ctx* c, const double* U, double* out, double* slots,
 
 int launch_kozak(xg_ctx* c, double dt, const double* U, double* R) {
   KozakMesh* k = c->koz;
-  if (k->nblk) launch_kozak_blocks<false>(c, U, R, k->slots, dt, 0.0);
+  if (k->nblk) {
+    const xg::Prob P = prob_of(c);
+    auto kern = xg::has_source(P) ? k_kozak_pairs<true> : k_kozak_pairs<false>;
+    hipLaunchKernelGGL(kern, dim3(k->nblk), dim3(kKozThreads), 0, c->stream, k->nblk,
+                       k->blk0, k->blk1p, k->halo, k->chkp, k->chk_entp, k->tendp,
+                       k->entriesp, k->tabp, k->tcdp, k->x, k->y, k->z, U, R, k->slots,
+                       c->cfg.gamma - 1.0, dt, P);
+  }
   if (k->nfx)
     hipLaunchKernelGGL(k_fixup<5>, dim3(nblocks(k->nfx)), dim3(kBlock), 0,
                        c->stream, k->nfx, k->fx_node, k->fx_off, k->fx_slot,
@@ -614,10 +894,16 @@ int xg_kozak_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
       (ntet && !inpoel))
     return fail(c, XG_EINVAL, "xg_kozak_upload: bad arguments");
   XG_HIP(c, hipSetDevice(c->device));
-  xg::TetLayout L;
+  xg::TetLayout L, L2;
   int rc = xg::build_tet_layout((int)npoin, ntet, inpoel, xg::kKozNodes, xg::kKozCap,
                                 L, c->err);
   if (rc) return rc;
+  // the same node blocks, chunks of kTetsPerPairChunk tets
+  rc = xg::build_tet_layout((int)npoin, ntet, inpoel, xg::kKozNodes, xg::kKozCap, L2,
+                            c->err, xg::kTetsPerPairChunk);
+  if (rc) return rc;
+  if (L2.blk.size() != L.blk.size() || L2.nslot != L.nslot)
+    return fail(c, XG_EINVAL, "kozak layout: the two chunkings disagree on the blocks");
   auto* k = new KozakMesh;
   k->npoin = (int)npoin;
   k->ntet = ntet;
@@ -635,6 +921,22 @@ int xg_kozak_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
     ch[i] = make_int4(C.kind, C.first, C.count, C.tend_off);
     ce[i] = C.ent_off;
   }
+  std::vector<int4> b1p(k->nblk), chp(L2.chunks.size());
+  std::vector<int> cep(L2.chunks.size());
+  for (int b = 0; b < k->nblk; ++b) {
+    const auto& B = L.blk[b];
+    const auto& B2 = L2.blk[b];
+    if (B2.own0 != B.own0 || B2.nown != B.nown || B2.nloc != B.nloc || B2.slot0 != B.slot0) {
+      free_kozak(k);
+      return fail(c, XG_EINVAL, "kozak layout: the two chunkings disagree on the blocks");
+    }
+    b1p[b] = make_int4(B2.slot0, B2.chunk0, B2.chunk1, 0);
+  }
+  for (size_t i = 0; i < L2.chunks.size(); ++i) {
+    const auto& C = L2.chunks[i];
+    chp[i] = make_int4(C.kind, C.first, C.count, C.tend_off);
+    cep[i] = C.ent_off;
+  }
   auto& A = k->allocs;
   const size_t n = (size_t)npoin;
   if ((rc = upload(c, A, &k->x, x, n)) || (rc = upload(c, A, &k->y, y, n)) ||
@@ -644,6 +946,10 @@ int xg_kozak_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
       (rc = upload(c, A, &k->tend, L.tend)) ||
       (rc = upload(c, A, &k->entries, L.entries)) ||
       (rc = upload(c, A, &k->tab, L.tab)) || (rc = upload(c, A, &k->tcd, L.tcd)) ||
+      (rc = upload(c, A, &k->blk1p, b1p)) || (rc = upload(c, A, &k->chkp, chp)) ||
+      (rc = upload(c, A, &k->chk_entp, cep)) || (rc = upload(c, A, &k->tendp, L2.tend)) ||
+      (rc = upload(c, A, &k->entriesp, L2.entries)) ||
+      (rc = upload(c, A, &k->tabp, L2.tab)) || (rc = upload(c, A, &k->tcdp, L2.tcd)) ||
       (rc = dev_alloc(c, A, (void**)&k->slots, (size_t)L.nslot * 40)) ||
       (rc = upload(c, A, &k->fx_node, L.fx_node)) ||
       (rc = upload(c, A, &k->fx_off, L.fx_off)) ||

@@ -270,11 +270,15 @@ int build_layout(int npoin, const std::vector<int>& p, const std::vector<int>& q
 
 // kozak::rhs layout: blocks as above on the tets' node graph; a block owns the
 // tets whose smallest node lies in it.  Tets are ordered by their local node
-// ids and cut into chunks of kTetsPerChunk; slot 4*i + a holds corner a of
+// ids and cut into chunks of tets_per_chunk; slot 4*i + a holds corner a of
 // chunk tet i, and every local node sums its slots in slot order.
 int build_tet_layout(int npoin, int64_t ntet, const size_t* inpoel,
                      int block_nodes, int block_cap, TetLayout& L,
-                     std::string& err) {
+                     std::string& err, int tets_per_chunk) {
+  if (tets_per_chunk < 1 || 4 * tets_per_chunk > 32768) {
+    err = "kozak layout: bad chunk size";
+    return XG_EINVAL;
+  }
   static const int kTetEdge[6][2] = {{0,1},{1,2},{2,0},{0,3},{1,3},{2,3}};
   L = TetLayout();
   L.npoin = npoin;
@@ -365,8 +369,8 @@ int build_tet_layout(int npoin, int64_t ntet, const size_t* inpoel,
     std::stable_sort(T.begin(), T.end(),
                      [&](int64_t a, int64_t c) { return key(a) < key(c); });
     B.chunk0 = (int)L.chunks.size();
-    for (size_t i0 = 0; i0 < T.size(); i0 += kTetsPerChunk) {
-      size_t i1 = std::min(T.size(), i0 + (size_t)kTetsPerChunk);
+    for (size_t i0 = 0; i0 < T.size(); i0 += (size_t)tets_per_chunk) {
+      size_t i1 = std::min(T.size(), i0 + (size_t)tets_per_chunk);
       std::vector<std::pair<int, uint16_t>> ent;
       for (size_t i = i0; i < i1; ++i)
         for (int a = 0; a < 4; ++a)

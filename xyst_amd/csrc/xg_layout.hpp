@@ -73,6 +73,10 @@ constexpr int kKozSlots = 512;
 constexpr int kKozCap = 512;
 constexpr int kKozNodes = 384;
 constexpr int kTetsPerChunk = kKozSlots / 4;
+// kozak::rhs without the antidiffusive contributions runs two threads per
+// tet on chunks of twice as many tets (1024 corner slots of 5 doubles)
+constexpr int kKozPairSlots = 1024;
+constexpr int kTetsPerPairChunk = kKozPairSlots / 4;
 
 struct TetLayout {
   int npoin = 0;
@@ -88,7 +92,8 @@ struct TetLayout {
 
 int build_tet_layout(int npoin, int64_t ntet, const size_t* inpoel,
                      int block_nodes, int block_cap, TetLayout& L,
-                     std::string& err);
+                     std::string& err,
+                     int tets_per_chunk = kTetsPerChunk);
 
 // chunk_slots: flux slots per edge chunk (<= kChunkSlots; boundary-face
 // chunks hold chunk_slots / 3 faces).
