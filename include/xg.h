@@ -50,7 +50,9 @@ typedef struct {
 enum {
   XG_PROBLEM_NONE = 0,
   XG_PROBLEM_VORTICAL_FLOW = 1,   /* Problems.cpp:452-509 */
-  XG_PROBLEM_TAYLOR_GREEN = 2     /* Problems.cpp:407-450 */
+  XG_PROBLEM_TAYLOR_GREEN = 2,    /* Problems.cpp:407-450 */
+  XG_PROBLEM_SLOT_CYL = 3         /* Problems.cpp:511-672: momentum source of
+                                     the rotation carrying one scalar */
 };
 typedef struct {
   int    kind;
@@ -161,7 +163,10 @@ int xg_mesh_upload(xg_ctx* ctx, int64_t npoin,
 /* ---- riemann:: drop-ins ---------------------------------------------------- */
 
 /* riemann::grad (Riemann.hpp:19-25, Riemann.cpp:229-367): G(npoin, 3*ncomp)
- * = undivided edge-based gradients of the primitive variables.  ncomp = 5. */
+ * = undivided edge-based gradients of the primitive variables (the
+ * transported scalars as they are).  ncomp = 5 + scalars, 5 <= ncomp <=
+ * XG_MAX_NCOMP; XG_EUNSUPPORTED beyond. */
+#define XG_MAX_NCOMP 7
 int xg_riemann_grad(xg_ctx* ctx, int ncomp, const double* U, double* G);
 int xg_riemann_grad_dev(xg_ctx* ctx, int ncomp, const double* U, double* G);
 
@@ -171,8 +176,10 @@ int xg_riemann_grad_dev(xg_ctx* ctx, int ncomp, const double* U, double* G);
  * 880-907): R(p,c) -= s_c(x_p, t) v[p], with v the nodal volumes of this
  * partition's own elements (may be null when the problem has no source) and
  * tp the per-node time of steady runs (null: t; the sources served do not
- * depend on time, so tp changes nothing).  XG_EUNSUPPORTED for ncomp != 5
- * (no transported scalars). */
+ * depend on time, so tp changes nothing).  Transported scalars (ncomp > 5)
+ * get the MUSCL reconstruction and scalar fluxes of Riemann.cpp:145-209,
+ * 455-466, 635-643 and no source.  XG_EUNSUPPORTED for ncomp outside
+ * 5..XG_MAX_NCOMP. */
 int xg_riemann_rhs(xg_ctx* ctx, int ncomp, const double* G, const double* U,
                    const double* v, double t, const double* tp, double* R);
 int xg_riemann_rhs_dev(xg_ctx* ctx, int ncomp, const double* G,
@@ -186,15 +193,26 @@ int xg_riemann_rhs_dev(xg_ctx* ctx, int ncomp, const double* G,
 int xg_riecg_setup(xg_ctx* ctx, const double* vol, int64_t nsym,
                    const size_t* symnodes, const double* symnorms);
 
+/* Components per node of the RieCG chare's fields (U.nprop(): 5 + the
+ * problem's transported scalars, at most XG_MAX_NCOMP); default 5.  The
+ * stepping entries below read and write U, Un, R with ncomp and G with
+ * 3 ncomp components per node.  Call after xg_riecg_setup and before
+ * xg_riecg_bc (it drops the Dirichlet lists, whose masks are per
+ * component). */
+int xg_riecg_set_ncomp(xg_ctx* ctx, int ncomp);
+
 /* The other boundary conditions of the RieCG chare (setupBC and
  * streamable, RieCG.cpp:109-245, 527-618; applied by RieCG::BC,
  * RieCG.cpp:764-785, after the symmetry BCs of xg_riecg_setup in the order
  * dirbc, symbc, farbc, prebc):
- *   Dirichlet (physics::dirbc, BC.cpp:29-72): ndir nodes with 5 mask entries
- *     each; a component with mask 1 takes dirvals[k*5+c], the initial
- *     condition at the node, which the caller evaluates (the problems served
- *     have initial conditions independent of time); other masks leave it
- *     (the RieCG chare passes no BC values);
+ *   Dirichlet (physics::dirbc, BC.cpp:29-72): ndir nodes with ncomp mask
+ *     entries each (xg_riecg_set_ncomp); a component with mask 1 takes
+ *     dirvals[k*ncomp+c], the initial condition at the node, which the
+ *     caller evaluates once: the values stay fixed in time (exact for the
+ *     problems served: their flow fields do not depend on time, and
+ *     slot_cyl's scalar changes only inside shapes that stay at distance
+ *     <= 0.4 from the rotation axis, off the unit square's boundary);
+ *     other masks leave it (the RieCG chare passes no BC values);
  *   farfield (physics::farbc, BC.cpp:152-221): nfar (node, unit normal)
  *     pairs and the free stream far = {density, u, v, w, pressure};
  *   pressure (physics::prebc, BC.cpp:223-241): npre nodes with (density,
@@ -226,14 +244,14 @@ int xg_riecg_dt_steady_dev(xg_ctx* ctx, const double* U, double cfl,
  * rkcoef {1/3,1/2,1} (steady: dtp[i] for dt); then RieCG::BC.  dtp, tp
  * (device, npoin) null for time-accurate runs; with them, stage 2 also
  * advances tp += dtp (RieCG.cpp:1047-1050).  All arrays on the device:
- * U, Un (npoin*5), G (npoin*15), R (npoin*5). */
+ * U, Un (npoin*ncomp), G (npoin*3*ncomp), R (npoin*ncomp). */
 int xg_riecg_stage_dev(xg_ctx* ctx, int stage, double t, double dt,
                        const double* dtp, double* tp, double* U, double* Un,
                        double* G, double* R);
 
 /* The pieces of a stage for partitioned runs, where the partial sums of the
  * partitions are added between them (xg_comm_sum_dev):
- * xg_riecg_grad_div_dev divides G(npoin, 15) by the summed nodal volume of
+ * xg_riecg_grad_div_dev divides G(npoin, 3 ncomp) by the summed nodal volume of
  * xg_riecg_setup (RieCG.cpp:935-939); xg_riecg_solve_dev is RieCG::solve
  * (RieCG.cpp:992-1057): Un = U at stage 0, the RK update with R, the BCs
  * at t + rkcoef dt, and tp += dtp after stage 2 when steady. */
@@ -420,7 +438,8 @@ int xg_chocg_step_dev(xg_ctx* ctx, double dt, double* U, double* Un, double* P,
  * analytic-solution error norms: out[c] = sum_i U(i,c)^2 V_i, out[ncomp+c] =
  * sum_i (U(i,c) - Un(i,c))^2 V_i, out[2 ncomp] = sum_i U(i,4) V_i (0 for
  * ncomp < 5).  U, Un (npoin*ncomp), V (npoin) on the device; out (host)
- * 2 ncomp + 1 values.  ncomp 1, 3 or 5.  Deterministic summation order. */
+ * 2 ncomp + 1 values.  ncomp 1, 3 or 5..XG_MAX_NCOMP.  Deterministic
+ * summation order. */
 int xg_diag_rho_dev(xg_ctx* ctx, int64_t npoin, int ncomp, const double* U,
                     const double* Un, const double* V, double* out);
 
@@ -438,11 +457,12 @@ int xg_diag_rho_dev_noalloc(xg_ctx* ctx, int64_t npoin, int ncomp, const double*
                             const double* Un, const double* V, double* out);
 
 /* The analytic-error sums of NodeDiagnostics::rhocompute
- * (NodeDiagnostics.cpp:79-128): with An (npoin*5, device) the analytic
- * solution in primitive form (rho, u, v, w, specific internal energy; the
- * caller evaluates SOL()), out[c] = sum_i (nu_c - An_c)^2 V_i and
- * out[5+c] = sum_i |nu_c - An_c| V_i, nu the same form of U.  ncomp = 5;
- * out (host) 10 values; deterministic summation order. */
+ * (NodeDiagnostics.cpp:79-128): with An (npoin*ncomp, device) the analytic
+ * solution in primitive form (rho, u, v, w, specific internal energy, then
+ * the scalars; the caller evaluates SOL()), out[c] = sum_i (nu_c - An_c)^2
+ * V_i and out[ncomp+c] = sum_i |nu_c - An_c| V_i, nu the same form of U.
+ * 5 <= ncomp <= XG_MAX_NCOMP; out (host) 2 ncomp values; deterministic
+ * summation order. */
 int xg_diag_err_dev(xg_ctx* ctx, int64_t npoin, int ncomp, const double* U,
                     const double* V, const double* An, double* out);
 

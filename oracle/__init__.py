@@ -161,12 +161,12 @@ def lib():
 
 class Problem(C.Structure):
     """An analytic problem of Problems.cpp (kind 0 none, 1 vortical_flow,
-    2 taylor_green) with its parameters."""
+    2 taylor_green, 3 slot_cyl) with its parameters."""
     _fields_ = [("kind", C.c_int), ("gamma", C.c_double), ("alpha", C.c_double),
                 ("kappa", C.c_double), ("p0", C.c_double)]
 
 
-PROBLEM_KINDS = {"none": 0, "vortical_flow": 1, "taylor_green": 2}
+PROBLEM_KINDS = {"none": 0, "vortical_flow": 1, "taylor_green": 2, "slot_cyl": 3}
 
 
 def problem(name="none", gamma=1.4, alpha=0.1, kappa=1.0, p0=10.0):
@@ -446,15 +446,19 @@ def diag(U, Un, v):
 # ---- problems, BCs, steady stepping, error norms (problems.c) ------------------
 
 def problem_ic(P, coord, t=0.0):
-    """IC() of the problem at every node (npoin, 5)."""
+    """IC() of the problem at every node (npoin, ncomp): 5 conserved flow
+    values, then the problem's transported scalars (slot_cyl: one)."""
     n = coord.shape[1]
-    out = np.empty((n, 5))
-    s = np.empty(5)
+    s = np.empty(16)
+    out = None
     for i in range(n):
-        if lib().orc_problem_ic(C.byref(P), float(coord[0, i]), float(coord[1, i]),
-                                float(coord[2, i]), t, s):
+        k = lib().orc_problem_ic(C.byref(P), float(coord[0, i]), float(coord[1, i]),
+                                 float(coord[2, i]), t, s)
+        if k <= 0:
             raise ValueError("problem has no analytic initial condition")
-        out[i] = s
+        if out is None:
+            out = np.empty((n, k))
+        out[i] = s[:k]
     return out
 
 
@@ -472,7 +476,8 @@ def problem_src(P, coord, t=0.0):
 
 def analytic_prim(P, coord, t=0.0):
     """SOL() in the primitive form NodeDiagnostics compares
-    (NodeDiagnostics.cpp:79-90): rho, u, v, w, specific internal energy."""
+    (NodeDiagnostics.cpp:79-90): rho, u, v, w, specific internal energy,
+    then the scalars as they are."""
     s = problem_ic(P, coord, t)
     s[:, 1] /= s[:, 0]
     s[:, 2] /= s[:, 0]

@@ -40,7 +40,7 @@ void orc_kozak_rhs_src(const orc_cfg* cfg, const orc_problem* P,
                        const double* tp, const double* dtp, const double* U,
                        double* R) {
   const double g = cfg->gamma;
-  const int src = P && (P->kind == 1 || P->kind == 2);
+  const int src = P && P->kind >= 1 && P->kind <= 3;
   memset(R, 0, sizeof(double) * (size_t)npoin * (size_t)ncomp);   /* :54 */
   double ue[64];
   for (int64_t e = 0; e < ntet; ++e) {

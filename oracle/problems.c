@@ -44,17 +44,58 @@ static void tg_ic(const orc_problem* P, double x, double y, double* s) {
   s[0] = r; s[1] = r*u; s[2] = r*v; s[3] = r*w; s[4] = rE;
 }
 
+/* slot_cyl::ic, Problems.cpp:513-623, RieCG/KozCG form (ncomp 6): unit
+ * density, the rigid rotation u = 0.5 - y, v = x - 0.5 about (0.5, 0.5),
+ * pressure 1, and one scalar c0 holding a cone, a hump and a slotted
+ * cylinder (radius 0.15) rotated by the angle t from their initial centres
+ * (0.5, 0.25), (0.25, 0.5), (0.5, 0.75); later shapes overwrite earlier
+ * ones where they overlap. */
+static void sc_ic(const orc_problem* P, double x, double y, double t,
+                  double* s) {
+  const double R0 = 0.15;
+  double r = 1.0, u = 0.5 - y, v = x - 0.5;
+  s[0] = r; s[1] = r*u; s[2] = r*v; s[3] = 0.0;
+  s[4] = totalenergy(P->gamma, r, u, v, 0.0, 1.0);
+  s[5] = 0.0;
+  /* a centre (x0, y0) at angle t: distance rc from the rotation axis */
+  double rc = sqrt((0.5-0.5)*(0.5-0.5) + (0.25-0.5)*(0.25-0.5));
+  double kx = 0.5 + rc*sin(t), ky = 0.5 - rc*cos(t);               /* cone */
+  rc = sqrt((0.25-0.5)*(0.25-0.5) + (0.5-0.5)*(0.5-0.5));
+  double hx = 0.5 + rc*sin(t - kPi/2.0), hy = 0.5 - rc*cos(t - kPi/2.0);   /* hump */
+  rc = sqrt((0.5-0.5)*(0.5-0.5) + (0.75-0.5)*(0.75-0.5));
+  double cx = 0.5 + rc*sin(t + kPi), cy = 0.5 - rc*cos(t + kPi);   /* cylinder */
+  /* the slot: three corner points, rotated by t about (0.5, 0.5) */
+  double px[3] = {0.525, 0.525, 0.475};
+  double py[3] = {cy - rc*cos(asin(0.025/rc)), 0.8, 0.8};
+  double qx[3], qy[3];
+  for (int k = 0; k < 3; ++k) {
+    qx[k] = 0.5 + cos(t)*(px[k]-0.5) - sin(t)*(py[k]-0.5);
+    qy[k] = 0.5 + sin(t)*(px[k]-0.5) + cos(t)*(py[k]-0.5);
+  }
+  double e1x = qx[1]-qx[0], e1y = qy[1]-qy[0];
+  double e2x = qx[2]-qx[1], e2y = qy[2]-qy[1];
+  double l1 = sqrt(e1x*e1x + e1y*e1y), l2 = sqrt(e2x*e2x + e2y*e2y);
+  double d = sqrt((x-kx)*(x-kx) + (y-ky)*(y-ky)) / R0;
+  if (d < 1.0) s[5] = 0.6*(1.0 - d);
+  d = sqrt((x-hx)*(x-hx) + (y-hy)*(y-hy)) / R0;
+  if (d < 1.0) s[5] = 0.2*(1.0 + cos(kPi*(d < 1.0 ? d : 1.0)));
+  d = sqrt((x-cx)*(x-cx) + (y-cy)*(y-cy)) / R0;
+  /* signed distances from the slot's long side and its end */
+  double s1 = (e1x*(y-qy[0]) - (x-qx[0])*e1y) / l1;
+  double s2 = (e2x*(y-qy[1]) - (x-qx[1])*e2y) / l2;
+  if (d < 1.0 && (s1 > 0.05 || s1 < 0.0 || s2 < 0.0)) s[5] = 0.6;
+}
+
 int orc_problem_ic(const orc_problem* P, double x, double y, double z,
                    double t, double* s) {
-  (void)t;
-  if (P->kind == 1) { vf_ic(P, x, y, z, s); return 0; }
-  if (P->kind == 2) { tg_ic(P, x, y, s); return 0; }
+  if (P->kind == 1) { vf_ic(P, x, y, z, s); return 5; }
+  if (P->kind == 2) { tg_ic(P, x, y, s); return 5; }
+  if (P->kind == 3) { sc_ic(P, x, y, t, s); return 6; }
   return -1;
 }
 
 int orc_problem_src(const orc_problem* P, double x, double y, double z,
                     double t, double* s) {
-  (void)t;
   for (int c = 0; c < 5; ++c) s[c] = 0.0;
   if (P->kind == 1) {                 /* vortical_flow::src, Problems.cpp:481-507 */
     const double a = P->alpha, k = P->kappa, g = P->gamma;
@@ -70,6 +111,13 @@ int orc_problem_src(const orc_problem* P, double x, double y, double z,
                        - cos(3.0*kPi*y)*cos(kPi*x) );
     return 1;
   }
+  if (P->kind == 3) {                 /* slot_cyl::src, Problems.cpp:625-670 */
+    double u[6];
+    sc_ic(P, x, y, t, u);
+    s[1] = -u[2];
+    s[2] = u[1];
+    return 1;
+  }
   return 0;
 }
 
@@ -78,8 +126,8 @@ int orc_problem_src(const orc_problem* P, double x, double y, double z,
 void orc_src(const orc_problem* P, int64_t npoin, int ncomp, const double* x,
              const double* y, const double* z, const double* v, double t,
              const double* tp, double* R) {
-  if (P->kind != 1 && P->kind != 2) return;
-  double s[5];
+  if (P->kind < 1 || P->kind > 3) return;
+  double s[6];
   for (int64_t p = 0; p < npoin; ++p) {
     double tt = tp ? tp[p] : t;
     orc_problem_src(P, x[p], y[p], z[p], tt, s);
@@ -98,10 +146,10 @@ void orc_dirbc(const orc_problem* P, int64_t npoin, int ncomp, double* U,
   double u[16];
   for (int64_t i = 0; i < ndir; ++i) {
     int64_t p = nodes[i];
-    int analytic = orc_problem_ic(P, x[p], y[p], z[p], t, u) == 0;
+    int nic = orc_problem_ic(P, x[p], y[p], z[p], t, u);
     for (int c = 0; c < ncomp; ++c) {
       int m = mask[i*ncomp+c];
-      if (m == 1) U[p*ncomp+c] = analytic && c < 5 ? u[c] : vals[i*ncomp+c];
+      if (m == 1) U[p*ncomp+c] = c < nic ? u[c] : vals[i*ncomp+c];
       else if (m == 2) U[p*ncomp+c] = vals[i*ncomp+c];
     }
   }

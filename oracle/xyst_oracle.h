@@ -138,10 +138,12 @@ typedef struct {
   double alpha, kappa, p0;      /* problem_alpha, problem_kappa, problem_p0 */
 } orc_problem;
 
-/* IC() of the problem at (x, y, z, t): 5 conserved values; -1 for kind 0. */
+/* IC() of the problem at (x, y, z, t): the conserved values (5, or 6 with
+ * slot_cyl's scalar); returns how many, -1 for kind 0. */
 int orc_problem_ic(const orc_problem* P, double x, double y, double z,
                    double t, double* s);
-/* SRC() of the problem: 5 values; returns 0 (and zeros) if it has none. */
+/* SRC() of the problem: the 5 flow values (the scalars have no source);
+ * returns 0 (and zeros) if it has none. */
 int orc_problem_src(const orc_problem* P, double x, double y, double z,
                     double t, double* s);
 

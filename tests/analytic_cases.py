@@ -95,3 +95,56 @@ TG_BANDS = [("rel", 1e-7, 1e-8), ("rel", 2e-7, 1e-8), ("rel", 4e-7, 1e-8),
 # exodiff.cfg of RieCG/Pipe
 PIPE_BANDS = [("rel", 1e-6, 1e-7), ("abs", 1e-6, 0.0), ("abs", 2e-7, 1e-6),
               ("abs", 2e-7, 1e-6), ("rel", 1e-6, 1e-7), ("rel", 1e-6, 1e-7)]
+
+
+# ---- RieCG/SlotCyl: one transported scalar -----------------------------------
+
+def slotcyl_fixture():
+    return np.load(os.path.join(GOLDEN, "riecg_slotcyl.npz"))
+
+
+def slotcyl_case(d):
+    return Case(d["coord"], d["inpoel"], d["triinpoel"], d["faceset"], [])
+
+
+def slotcyl_dirichlet(c):
+    """bc_dir of slot_cyl.q: side set 1 all six components, side set 2 the
+    five flow components (c0 left free)."""
+    return O.dirbc_lists(c.tri, c.faceset, 6, [(1, [1] * 6), (2, [1] * 5 + [0])])
+
+
+def slotcyl_diag_row(it, t, dt, dg, V, de):
+    """it, t, dt, L2(U_c) (6), L2(U_c - Un_c) (6), total energy, L2 and L1
+    of the error (6 each)."""
+    return np.array([it, t, dt] + list(np.sqrt(dg[:12] / V)) + [dg[12]]
+                    + list(np.sqrt(de[:6] / V)) + list(de[6:] / V))
+
+
+def check_slotcyl_diag(mine, ref, tight):
+    """diag.ndiff.cfg of RieCG/SlotCyl: cols 2-9 any abs 1e-4 rel 1e-7,
+    cols 10-15 any abs 6e-5 rel 1e-7, col 16 rel 1e-7, the error columns
+    ignored.  tight: agreement with the printed 12 digits (t, dt, the
+    conserved and residual norms and the total energy to 1e-9; the error
+    norms to 1e-7)."""
+    rel = np.abs(mine - ref) / np.maximum(np.abs(ref), 1e-300)
+    ab = np.abs(mine - ref)
+    assert ((ab[1:9] <= 1e-4) | (rel[1:9] <= 1e-7)).all(), ("cols 2-9", ab[1:9])
+    assert ((ab[9:15] <= 6e-5) | (rel[9:15] <= 1e-7)).all(), ("cols 10-15", ab[9:15])
+    assert rel[15] <= 1e-7, ("total energy", rel[15])
+    if tight:
+        nz = np.abs(ref) > 0
+        assert (rel[1:16][nz[1:16]] <= 1e-9).all(), ("tight", rel[1:16])
+        assert (ab[1:16][~nz[1:16]] <= 1e-15).all(), ("tight zeros", ab[1:16])
+        assert (rel[16:][nz[16:]] <= 1e-7).all(), ("errors", rel[16:])
+
+
+def slotcyl_fields(U, g):
+    """The seven fields of slot_cyl.std.exo: primitive flow fields and c0."""
+    return np.concatenate([primitive_fields(U[:, :5], g), U[:, 5:6]], 1)
+
+
+# exodiff.cfg of RieCG/SlotCyl: relative 1e-7 floor 1e-8, zvelocity absolute
+# 1e-7 floor 1e-7, c0 absolute 1e-8
+SC_BANDS = [("rel", 1e-7, 1e-8), ("rel", 1e-7, 1e-8), ("rel", 1e-7, 1e-8),
+            ("abs", 1e-7, 1e-7), ("rel", 1e-7, 1e-8), ("rel", 1e-7, 1e-8),
+            ("abs", 1e-8, 0.0)]

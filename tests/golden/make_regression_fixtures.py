@@ -1,6 +1,8 @@
-"""Extract the reference's RieCG/Sod, KozCG/Sod, ChoCG/Lid and
-ChoCG/Poiseuille regression fixtures into riecg_sod.npz, kozcg_sod.npz,
-chocg_lid.npz and chocg_poiseuille.npz.
+"""Extract the reference's RieCG/Sod, KozCG/Sod, ChoCG/Lid,
+ChoCG/Poiseuille, analytic-problem (riecg_analytic.npz, see
+analytic_fixtures) and RieCG/SlotCyl regression fixtures into
+riecg_sod.npz, kozcg_sod.npz, chocg_lid.npz, chocg_poiseuille.npz,
+riecg_analytic.npz and riecg_slotcyl.npz.
 
 Source files (reference test data, read as data only), per directory
 tests/regression/inciter/{RieCG,KozCG}/Sod:
@@ -162,6 +164,7 @@ def main():
     print("wrote chocg_poiseuille.npz", coord.shape, tets.shape, tri.shape)
 
     analytic_fixtures()
+    slotcyl_fixtures()
 
     # ExodusII files kept as they are, for the file reader's tests: a mesh
     # with a triangle block and side sets on it (RieCG/Sod), a mesh with
@@ -281,6 +284,34 @@ def analytic_fixtures():
                                    "energy", "pressure"])
     np.savez_compressed(os.path.join(HERE, "riecg_analytic.npz"), **out)
     print("wrote riecg_analytic.npz", sorted(out))
+
+
+def slotcyl_fixtures():
+    """RieCG/SlotCyl (one transported scalar c0: slotted cylinder, cone and
+    hump rotated by a prescribed flow; Rusanov and HLLC runs, both compared
+    with the same diag.std) into riecg_slotcyl.npz.  Run configuration
+    (slot_cyl.q, slot_cyl_hllc.q): unitsquare_01_3.6k.exo, nstep 20, cfl
+    0.9, gamma 5/3, Dirichlet to the initial condition on side set 1 (all
+    six components) and side set 2 (the five flow components, c0 free);
+    fields (incl. the analytic ones) written after the last step; the
+    Rusanov run also as 8 chares (slot_cyl_pe4_u0.5.std.exo.*)."""
+    rs = os.path.join(REF, "RieCG", "SlotCyl")
+    coord, tets, tri, faceset = read_mesh(rs, "unitsquare_01_3.6k.exo")
+    out = dict(coord=coord, inpoel=tets, triinpoel=tri, faceset=faceset,
+               diag=np.loadtxt(os.path.join(rs, "diag.std")), nstep=20, cfl=0.9,
+               gamma=5.0 / 3.0)
+    for case, exo in (("rusanov", "slot_cyl.std.exo"), ("hllc", "slot_cyl_hllc.std.exo")):
+        f, names, tf = read_result(rs, coord, exo)
+        out["%s_fields" % case] = f
+        out["%s_t_final" % case] = tf
+    out["field_names"] = np.array(names)
+    # the Rusanov run on 4 PEs with 8 chares (-u 0.5): fields and the chare
+    # of every tet, from the partition files
+    f, _, tf = merge_partitions(rs, coord, "slot_cyl_pe4_u0.5.std.exo", 8)
+    out.update(pe8_fields=f, pe8_t_final=tf,
+               chare_of_tet=chare_of_tet(rs, coord, tets, "slot_cyl_pe4_u0.5.std.exo", 8))
+    np.savez_compressed(os.path.join(HERE, "riecg_slotcyl.npz"), **out)
+    print("wrote riecg_slotcyl.npz", sorted(out), names)
 
 
 if __name__ == "__main__":

@@ -152,7 +152,7 @@ def test_errors_surface_as_codes():
     c = box_case(2)
     ctx.upload_mesh(c.coord, c.se.sup, c.se.int, c.tri, c.besym)
     with pytest.raises(XgError, match="EUNSUPPORTED"):
-        ctx.grad(np.ones((c.npoin, 6)))
+        ctx.grad(np.ones((c.npoin, 8)))   # more than two transported scalars
     with pytest.raises(XgError, match="Flux not configured"):
         ctx.set_config(flux=7)
     bad = [s.copy() for s in c.se.sup]

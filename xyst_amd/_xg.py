@@ -41,6 +41,7 @@ SIG = {
     "xg_set_problem": (C.c_int, [VP, VP]),
     "xg_get_problem": (C.c_int, [VP, VP]),
     "xg_riecg_setup": (C.c_int, [VP, VP, I64, VP, VP]),
+    "xg_riecg_set_ncomp": (C.c_int, [VP, C.c_int]),
     "xg_riecg_dt_dev": (C.c_int, [VP, VP, D, C.POINTER(D)]),
     "xg_riecg_stage_dev": (C.c_int, [VP, C.c_int, D, D, VP, VP, VP, VP, VP, VP]),
     "xg_riecg_bc": (C.c_int, [VP, I64, VP, VP, VP, I64, VP, VP, VP, I64, VP, VP]),
@@ -150,7 +151,7 @@ class Problem(C.Structure):
                 ("p0", C.c_double)]
 
 
-PROBLEMS = {"none": 0, "vortical_flow": 1, "taylor_green": 2}
+PROBLEMS = {"none": 0, "vortical_flow": 1, "taylor_green": 2, "slot_cyl": 3}
 
 
 class ChoConfig(C.Structure):

@@ -44,15 +44,16 @@ __device__ inline double wave_min(double v) {
 }
 
 // per-node CFL limit of the explicit solvers' dt, unsteady branch
-// (RieCG.cpp:827-839, KozCG.cpp:648-660), block minimum
+// (RieCG.cpp:827-839, KozCG.cpp:648-660), block minimum; nc components per
+// node, the flow variables first
 [[maybe_unused]] __global__ __launch_bounds__(xgd::kBlock) void k_dt_partial(
-    int npoin, const double* __restrict__ U, const double* __restrict__ vol,
+    int npoin, int nc, const double* __restrict__ U, const double* __restrict__ vol,
     double g, double* __restrict__ part) {
   __shared__ double sm[xgd::kBlock / 64];
   double m = 1.7976931348623157e308;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npoin;
        p += (int64_t)gridDim.x * blockDim.x) {
-    const double* s = U + p * 5;
+    const double* s = U + p * nc;
     double r = s[0];
     double u = s[1] / r, v = s[2] / r, w = s[3] / r;
     double pr = (s[4] - 0.5 * r * (u * u + v * v + w * w)) * (g - 1.0);

@@ -38,7 +38,7 @@ struct xg_ctx {
     uint16_t *tend = nullptr, *entries = nullptr;
     uint32_t *elpq = nullptr, *fab = nullptr, *fcs = nullptr;
     double *dx = nullptr, *dy = nullptr, *dz = nullptr;
-    double* slots = nullptr;             // halo partials (5 per slot)
+    double* slots = nullptr;             // halo partials (XG_MAX_NCOMP per slot)
   } rb;
   double *x = nullptr, *y = nullptr, *z = nullptr;
   int64_t nedge = 0;
@@ -57,6 +57,7 @@ struct xg_ctx {
   int* tri = nullptr;
   // RieCG stepping
   bool have_riecg = false;
+  int ncomp = 5;                         // components per node (xg_riecg_set_ncomp)
   double* vol = nullptr;
   // per node: 1 = boundary node, whose gradient k_grad_bnd completes and
   // divides by vol; the others are divided by k_grad_blocks (RK stages)
@@ -82,7 +83,8 @@ struct xg_ctx {
   double* red = nullptr;               // reduction partials + result
   // staging for the host-pointer entry points
   double *sU = nullptr, *sG = nullptr, *sR = nullptr;
-  size_t sbytes = 0;
+  size_t sbytes = 0;                     // bytes of one component over all nodes
+  int snc = 0;                           // components the buffers hold
   // pressure-Poisson solver (xg_cg.hip)
   CgSolver* cg = nullptr;
   // KozCG tet layout (xg_kozak.hip)

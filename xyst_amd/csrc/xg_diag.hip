@@ -124,36 +124,37 @@ __global__ __launch_bounds__(kBlock) void k_diag_pre(int64_t n, const double* __
   }
 }
 
-// rhocompute's analytic-error sums (NodeDiagnostics.cpp:112-122): the
-// numerical solution in primitive form against An, (nu_c - An_c)^2 V and
-// |nu_c - An_c| V
+// rhocompute's analytic-error sums (NodeDiagnostics.cpp:112-128): the
+// numerical solution in primitive form (the scalars as they are) against
+// An, (nu_c - An_c)^2 V and |nu_c - An_c| V, NC components per node
+template <int NC>
 __global__ __launch_bounds__(kBlock) void k_diag_err(int64_t n, const double* __restrict__ U,
                                                      const double* __restrict__ V,
                                                      const double* __restrict__ An,
                                                      double* __restrict__ part) {
   __shared__ double sm[kBlock / 64];
-  double acc[10];
+  double acc[2 * NC];
 #pragma unroll
-  for (int q = 0; q < 10; ++q) acc[q] = 0.0;
+  for (int q = 0; q < 2 * NC; ++q) acc[q] = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kBlock) {
     const double v = V[i];
-    double nu[5];
+    double nu[NC];
 #pragma unroll
-    for (int c = 0; c < 5; ++c) nu[c] = U[i * 5 + c];
+    for (int c = 0; c < NC; ++c) nu[c] = U[i * NC + c];
     nu[1] /= nu[0];
     nu[2] /= nu[0];
     nu[3] /= nu[0];
     nu[4] = nu[4] / nu[0] - 0.5 * (nu[1] * nu[1] + nu[2] * nu[2] + nu[3] * nu[3]);
 #pragma unroll
-    for (int c = 0; c < 5; ++c) {
-      const double du = nu[c] - An[i * 5 + c];
+    for (int c = 0; c < NC; ++c) {
+      const double du = nu[c] - An[i * NC + c];
       acc[c] += du * du * v;
-      acc[5 + c] += fabs(du) * v;
+      acc[NC + c] += fabs(du) * v;
     }
   }
 #pragma unroll
-  for (int q = 0; q < 10; ++q) {
+  for (int q = 0; q < 2 * NC; ++q) {
     const double s = block_sum(acc[q], sm);
     if (threadIdx.x == 0) part[(size_t)q * gridDim.x + blockIdx.x] = s;
   }
@@ -204,15 +205,20 @@ int xg_diag_rho_dev(xg_ctx* c, int64_t npoin, int ncomp, const double* U, const 
   if (!c) return XG_EINVAL;
   if (npoin < 0 || !U || !Un || !V || !out)
     return fail(c, XG_EINVAL, "xg_diag_rho: bad arguments");
-  if (ncomp != 5 && ncomp != 3 && ncomp != 1)
-    return fail(c, XG_EUNSUPPORTED, "xg_diag_rho: ncomp must be 1, 3 or 5");
+  if (ncomp != 1 && ncomp != 3 && (ncomp < 5 || ncomp > XG_MAX_NCOMP))
+    return fail(c, XG_EUNSUPPORTED, "xg_diag_rho: ncomp must be 1, 3 or 5..XG_MAX_NCOMP");
   XG_HIP(c, hipSetDevice(c->device));
   const int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, nblocks(npoin)));
   const int ns = 2 * ncomp + 1;
   double *part, *dout;
   int rc = diag_buffers(c, ns, nb, &part, &dout);
   if (rc) return rc;
-  if (ncomp == 5)
+  static_assert(XG_MAX_NCOMP == 7, "one instantiation per component count");
+  if (ncomp == 7)
+    hipLaunchKernelGGL(k_diag_rho<7>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, Un, V, part);
+  else if (ncomp == 6)
+    hipLaunchKernelGGL(k_diag_rho<6>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, Un, V, part);
+  else if (ncomp == 5)
     hipLaunchKernelGGL(k_diag_rho<5>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, Un, V, part);
   else if (ncomp == 3)
     hipLaunchKernelGGL(k_diag_rho<3>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, Un, V, part);
@@ -253,14 +259,21 @@ int xg_diag_err_dev(xg_ctx* c, int64_t npoin, int ncomp, const double* U, const 
   if (!c) return XG_EINVAL;
   if (npoin < 0 || !U || !V || !An || !out)
     return fail(c, XG_EINVAL, "xg_diag_err: bad arguments");
-  if (ncomp != 5) return fail(c, XG_EUNSUPPORTED, "xg_diag_err: ncomp must be 5");
+  if (ncomp < 5 || ncomp > XG_MAX_NCOMP)
+    return fail(c, XG_EUNSUPPORTED, "xg_diag_err: ncomp must be 5..XG_MAX_NCOMP");
   XG_HIP(c, hipSetDevice(c->device));
   const int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, nblocks(npoin)));
+  const int ns = 2 * ncomp;
   double *part, *dout;
-  int rc = diag_buffers(c, 10, nb, &part, &dout);
+  int rc = diag_buffers(c, ns, nb, &part, &dout);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_diag_err, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, V, An, part);
-  return finish(c, 10, nb, part, dout, out);
+  if (ncomp == 5)
+    hipLaunchKernelGGL(k_diag_err<5>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, V, An, part);
+  else if (ncomp == 6)
+    hipLaunchKernelGGL(k_diag_err<6>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, V, An, part);
+  else
+    hipLaunchKernelGGL(k_diag_err<7>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, V, An, part);
+  return finish(c, ns, nb, part, dout, out);
 }
 
 int xg_diag_pre_dev(xg_ctx* c, int64_t npoin, const double* U, const double* Un,

@@ -15,16 +15,28 @@ struct Cfg {
   double gg1;   // g * (g - 1)
 };
 
-// riemann::primitive, Riemann.cpp:211-227
+// riemann::primitive, Riemann.cpp:211-227: NC components per node, the
+// transported scalars (c >= 5) as they are
+template <int NC>
 __device__ inline void primitive(const double* __restrict__ U, int i,
-                                 double u[5]) {
-  const double* s = U + (size_t)i * 5;
+                                 double u[NC]) {
+  const double* s = U + (size_t)i * NC;
   u[0] = s[0];
   u[1] = s[1] / u[0];
   u[2] = s[2] / u[0];
   u[3] = s[3] / u[0];
   u[4] = s[4] / u[0] - 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]);
+#pragma unroll
+  for (int c = 5; c < NC; ++c) u[c] = s[c];
 }
+
+// The transported scalars' flux of an edge, f_c = (l_c a + r_c b) m +
+// sw (r_c - l_c), in the reference's association (Riemann.cpp:460-466 for
+// Rusanov: a, b the dual-face normal velocities, m = 1; :637-643 for HLLC:
+// a, b the unit-normal velocities, m the dual-face area).
+struct ScalarFlux {
+  double a, b, m, sw;
+};
 
 // ---------------------------------------------------------------------------
 // Divide-free forms of the reference arithmetic.  Each rewrite is an identity in
@@ -94,10 +106,33 @@ __device__ inline void muscl_fast(const double g1[5], const double g2[5],
   }
 }
 
+// MUSCL / van Leer reconstruction of the transported scalars c = 5..NC-1
+// (Riemann.cpp:145-209): the flow variables' limiter, without the
+// positivity fallback.
+template <int NC>
+__device__ inline void muscl_scalars(const double g1[NC], const double g2[NC],
+                                     double l[NC], double r[NC]) {
+  constexpr double eps = 1.0e-9;
+  constexpr double c1 = 2.0 / 3.0;
+  constexpr double c2 = 4.0 / 3.0;
+#pragma unroll
+  for (int c = 5; c < NC; ++c) {
+    double delta2 = r[c] - l[c];
+    double delta1 = 2.0 * g1[c] - delta2;
+    double delta3 = 2.0 * g2[c] - delta2;
+    double a = delta2 + eps, bl = delta1 + eps, br = delta3 + eps;
+    double il = same_sign(a, bl) ? rcp(a + bl) : 0.0;
+    double ir = same_sign(a, br) ? rcp(a + br) : 0.0;
+    l[c] += 0.5 * il * (delta1 * c1 * a + delta2 * c2 * bl);
+    r[c] -= 0.5 * ir * (delta3 * c1 * a + delta2 * c2 * br);
+  }
+}
+
 // Rusanov flux (Riemann.cpp:369-478) on primitive end states (l, r hold
 // rho, u, v, w, e), divide-free.
 __device__ inline void rusanov_fast(const Cfg& cfg, const double n[3],
-                                    double l[5], double r[5], double f[5]) {
+                                    double l[5], double r[5], double f[5],
+                                    ScalarFlux& sf) {
   const double gg1 = cfg.gg1;
   double pL = (l[0] * l[4]) * cfg.gm1;
   double pR = (r[0] * r[4]) * cfg.gm1;
@@ -121,12 +156,13 @@ __device__ inline void rusanov_fast(const Cfg& cfg, const double n[3],
 #pragma unroll
     for (int c = 0; c < 5; ++c) f[c] -= fws * (l[c] - r[c]);
   }
+  sf = ScalarFlux{vnL, vnR, 1.0, fmax(fabs(vnL), fabs(vnR))};
 }
 
 // HLLC flux (Riemann.cpp:480-650), reciprocals shared where the reference
 // divides several terms by the same quantity.
 __device__ inline void hllc_fast(const Cfg& cfg, const double d[3], double l[5],
-                                 double r[5], double f[5]) {
+                                 double r[5], double f[5], ScalarFlux& sf) {
   const double gg1 = cfg.gg1;
   double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
   double il = rcp(len);
@@ -197,6 +233,7 @@ __device__ inline void hllc_fast(const Cfg& cfg, const double d[3], double l[5],
 #pragma unroll
     for (int c = 0; c < 5; ++c) f[c] -= fws * (l[c] - r[c]);
   }
+  sf = ScalarFlux{qL, qR, len, fmax(fabs(qL), fabs(qR)) * len};
 }
 
 }  // namespace xg

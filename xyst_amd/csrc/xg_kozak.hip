@@ -1084,7 +1084,7 @@ int xg_kozcg_dt_dev(xg_ctx* c, const double* U, double cfl, double* dt) {
   KozakMesh* k = c->koz;
   int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, nblocks(k->npoin)));
   hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(kBlock), 0, c->stream,
-                     k->npoin, U, k->vol, c->cfg.gamma, k->red);
+                     k->npoin, 5, U, k->vol, c->cfg.gamma, k->red);
   hipLaunchKernelGGL(k_min_final, dim3(1), dim3(kRedBlocks), 0, c->stream, nb,
                      k->red, cfl, k->red + kRedBlocks);
   XG_HIP(c, hipGetLastError());

@@ -17,13 +17,17 @@ struct Prob {
 };
 
 __host__ __device__ inline bool has_source(const Prob& P) {
-  return P.kind == XG_PROBLEM_VORTICAL_FLOW || P.kind == XG_PROBLEM_TAYLOR_GREEN;
+  return P.kind == XG_PROBLEM_VORTICAL_FLOW || P.kind == XG_PROBLEM_TAYLOR_GREEN ||
+         P.kind == XG_PROBLEM_SLOT_CYL;
 }
 
-// SRC() of the problem at (x, y, z); both sources are independent of time.
+// SRC() of the problem at (x, y, z); the sources are independent of time.
 //  vortical_flow::src (Problems.cpp:481-507): momentum (a u - k v, k u + a v)
 //    of the initial state's velocity, energy s.u + 8 a^3 z^2/(g-1);
-//  taylor_green::src (Problems.cpp:433-448): energy only.
+//  taylor_green::src (Problems.cpp:433-448): energy only;
+//  slot_cyl::src (Problems.cpp:625-670): the momentum (-rho v, rho u) of the
+//    rotation u = 0.5 - y, v = x - 0.5 at unit density (the centripetal
+//    force); the transported scalars have no source.
 __device__ inline void source(const Prob& P, double x, double y, double z,
                               double s[5]) {
   s[0] = s[1] = s[2] = s[3] = s[4] = 0.0;
@@ -38,6 +42,9 @@ __device__ inline void source(const Prob& P, double x, double y, double z,
     constexpr double pi = 3.14159265358979323846;
     s[4] = 3.0 * pi / 8.0 * (cos(3.0 * pi * x) * cos(pi * y) -
                              cos(3.0 * pi * y) * cos(pi * x));
+  } else if (P.kind == XG_PROBLEM_SLOT_CYL) {
+    s[1] = -(x - 0.5);
+    s[2] = 0.5 - y;
   }
 }
 

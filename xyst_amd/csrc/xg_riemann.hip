@@ -70,8 +70,9 @@ struct ChunkPrefetch {
 // the thread owning local node t adds -/+ D (u_p + u_q) for the chunk's edges
 // touching it, in slot order.
 // DIV: owned nodes without boundary terms (gflag 0) are final here and are
-// divided by their volume (RieCG.cpp:935-939)
-template <bool DIV>
+// divided by their volume (RieCG.cpp:935-939).  NC components per node
+// (5 + transported scalars).
+template <bool DIV, int NC>
 __global__ __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_grad_blocks(
     int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
     const int* __restrict__ halo, const int4* __restrict__ chk,
@@ -82,9 +83,10 @@ __global__ __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_eu(4)
     double* __restrict__ G, const uint8_t* __restrict__ gflag,
     const double* __restrict__ vol) {
   constexpr int CAP = xg::kGradCap;
+  constexpr int FW = NC + 3;   // an edge's record: u_p + u_q, then D
   static_assert(CAP <= kGradThreads, "one local node per thread");
-  __shared__ double sP[5 * CAP];
-  __shared__ double F[kGradThreads * 8];
+  __shared__ double sP[NC * CAP];
+  __shared__ double F[kGradThreads * FW];
   __shared__ uint16_t sE[2 * kGradThreads];
   __shared__ uint16_t sT[CAP];
   const int t = threadIdx.x;
@@ -94,12 +96,14 @@ __global__ __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_eu(4)
   const int own0 = B0.x, nown = B0.y, nloc = B0.z;
   if (t < nloc) {
     int g = t < nown ? own0 + t : halo[B0.w + t - nown];
-    double s[5], u[5];
+    double s[NC], u[NC];
 #pragma unroll
-    for (int c = 0; c < 5; ++c) s[c] = U[(size_t)g * 5 + c];
+    for (int c = 0; c < NC; ++c) s[c] = U[(size_t)g * NC + c];
     xg::primitive_fast(s, u);
 #pragma unroll
-    for (int c = 0; c < 5; ++c) sP[c * CAP + t] = u[c];
+    for (int c = 5; c < NC; ++c) u[c] = s[c];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) sP[c * CAP + t] = u[c];
   }
   // prefetch of the next edge chunk (a gather layout has no boundary-face
   // chunks, and any would be skipped: k_grad_bnd adds those terms)
@@ -122,9 +126,9 @@ __global__ __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_eu(4)
     }
   };
   fetch(ch);
-  double acc[15];
+  double acc[3 * NC];
 #pragma unroll
-  for (int k = 0; k < 15; ++k) acc[k] = 0.0;
+  for (int k = 0; k < 3 * NC; ++k) acc[k] = 0.0;
   while (ch < B1.z) {
     const int4 C = chk[ch];
     int nx = ch + 1;
@@ -135,12 +139,12 @@ __global__ __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_eu(4)
     pf.store(C, nloc, t, sE, sT);
     if (t < C.z) {
       int lp = pq & 0xffff, lq = pq >> 16;
-      double* f = F + t * 8;
+      double* f = F + t * FW;
 #pragma unroll
-      for (int c = 0; c < 5; ++c) f[c] = sP[c * CAP + lq] + sP[c * CAP + lp];
-      f[5] = d0;
-      f[6] = d1;
-      f[7] = d2;
+      for (int c = 0; c < NC; ++c) f[c] = sP[c * CAP + lq] + sP[c * CAP + lp];
+      f[NC] = d0;
+      f[NC + 1] = d1;
+      f[NC + 2] = d2;
     }
     fetch(nx);                // in flight during the accumulation below
     __syncthreads();          // F, sE, sT complete
@@ -148,11 +152,11 @@ __global__ __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_eu(4)
       int e0 = t ? sT[t - 1] : 0, e1 = sT[t];
       for (int j = e0; j < e1; ++j) {
         int v = sE[j];
-        const double* f = F + (v & 0x3ff) * 8;
+        const double* f = F + (v & 0x3ff) * FW;
         double sg = (v & 0x8000) ? -1.0 : 1.0;
-        double D0 = sg * f[5], D1 = sg * f[6], D2 = sg * f[7];
+        double D0 = sg * f[NC], D1 = sg * f[NC + 1], D2 = sg * f[NC + 2];
 #pragma unroll
-        for (int c = 0; c < 5; ++c) {
+        for (int c = 0; c < NC; ++c) {
           acc[c * 3 + 0] += D0 * f[c];
           acc[c * 3 + 1] += D1 * f[c];
           acc[c * 3 + 2] += D2 * f[c];
@@ -162,14 +166,14 @@ __global__ __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_eu(4)
     ch = nx;
   }
   if (t < nown) {
-    double* o = G + (size_t)(own0 + t) * 15;
+    double* o = G + (size_t)(own0 + t) * (3 * NC);
     if (DIV && gflag[own0 + t] == 0) {
       const double v = vol[own0 + t];
 #pragma unroll
-      for (int k = 0; k < 15; ++k) o[k] = acc[k] / v;
+      for (int k = 0; k < 3 * NC; ++k) o[k] = acc[k] / v;
     } else {
 #pragma unroll
-      for (int k = 0; k < 15; ++k) o[k] = acc[k];
+      for (int k = 0; k < 3 * NC; ++k) o[k] = acc[k];
     }
   }
 }
@@ -190,7 +194,7 @@ __global__ void k_fill_u8(int64_t n, uint8_t* __restrict__ a) {
 // As in the reference, every node of a face receives g[j]*n[j] for direction
 // j, with g[j] the 4:1:1-weighted value belonging to the face's j-th node.
 // DIV: boundary nodes are final here and are divided by their volume
-template <bool DIV>
+template <bool DIV, int NC>
 __global__ __launch_bounds__(kBlock) void k_grad_bnd(
     int nb, const int* __restrict__ bnode, const int* __restrict__ boff,
     const int* __restrict__ bface, const int* __restrict__ tri,
@@ -200,23 +204,23 @@ __global__ __launch_bounds__(kBlock) void k_grad_bnd(
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nb) return;
   int n = bnode[i];
-  double acc[15];
+  double acc[3 * NC];
 #pragma unroll
-  for (int k = 0; k < 15; ++k) acc[k] = G[(size_t)n * 15 + k];
+  for (int k = 0; k < 3 * NC; ++k) acc[k] = G[(size_t)n * (3 * NC) + k];
   for (int k = boff[i]; k < boff[i + 1]; ++k) {
     int f = bface[k] >> 2;
     int A = tri[f * 3 + 0], B = tri[f * 3 + 1], C = tri[f * 3 + 2];
-    double u[3][5];
-    xg::primitive(U, A, u[0]);
-    xg::primitive(U, B, u[1]);
-    xg::primitive(U, C, u[2]);
+    double u[3][NC];
+    xg::primitive<NC>(U, A, u[0]);
+    xg::primitive<NC>(U, B, u[1]);
+    xg::primitive<NC>(U, C, u[2]);
     double ba[3] = {x[B] - x[A], y[B] - y[A], z[B] - z[A]};
     double ca[3] = {x[C] - x[A], y[C] - y[A], z[C] - z[A]};
     double nr[3] = {(ba[1] * ca[2] - ca[1] * ba[2]) / 12.0,
                     (ba[2] * ca[0] - ca[2] * ba[0]) / 12.0,
                     (ba[0] * ca[1] - ca[0] * ba[1]) / 12.0};
 #pragma unroll
-    for (int c = 0; c < 5; ++c) {
+    for (int c = 0; c < NC; ++c) {
       double uab = (u[0][c] + u[1][c]) / 4.0;
       double ubc = (u[1][c] + u[2][c]) / 4.0;
       double uca = (u[2][c] + u[0][c]) / 4.0;
@@ -228,19 +232,21 @@ __global__ __launch_bounds__(kBlock) void k_grad_bnd(
   if (DIV) {
     const double v = vol[n];
 #pragma unroll
-    for (int k = 0; k < 15; ++k) acc[k] /= v;
+    for (int k = 0; k < 3 * NC; ++k) acc[k] /= v;
   }
 #pragma unroll
-  for (int k = 0; k < 15; ++k) G[(size_t)n * 15 + k] = acc[k];
+  for (int k = 0; k < 3 * NC; ++k) G[(size_t)n * (3 * NC) + k] = acc[k];
 }
 
 // advbnd (Riemann.cpp:768-878) for one boundary face from the staged
 // primitive states: the three corner contributions (fab + fca + f_a style
-// weights).  The conserved values are rebuilt from the primitive ones.
+// weights).  The conserved values are rebuilt from the primitive ones; a
+// transported scalar's corner flux is its value times the normal velocity.
+template <int NC>
 __device__ inline void face_flux_lds(int la, int lb, int lc, int sym,
                                      const double* __restrict__ sP,
                                      const double* __restrict__ sX,
-                                     double gm1, double out[3][5]) {
+                                     double gm1, double out[3][NC]) {
   constexpr int CAP = xg::kRhsCap;   // the rhs blocks' LDS stride
   int N[3] = {la, lb, lc};
   double ba[3], ca[3];
@@ -252,7 +258,7 @@ __device__ inline void face_flux_lds(int la, int lb, int lc, int sym,
   double nx = (ba[1] * ca[2] - ca[1] * ba[2]) / 12.0;
   double ny = (ba[2] * ca[0] - ca[2] * ba[0]) / 12.0;
   double nz = (ba[0] * ca[1] - ca[0] * ba[1]) / 12.0;
-  double fl[5][3];
+  double fl[NC][3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     double r = sP[0 * CAP + N[a]], u = sP[1 * CAP + N[a]], v = sP[2 * CAP + N[a]];
@@ -265,9 +271,11 @@ __device__ inline void face_flux_lds(int la, int lb, int lc, int sym,
     fl[2][a] = r * v * vn + p * ny;
     fl[3][a] = r * w * vn + p * nz;
     fl[4][a] = (re + p) * vn;
+#pragma unroll
+    for (int c = 5; c < NC; ++c) fl[c][a] = sP[c * CAP + N[a]] * vn;
   }
 #pragma unroll
-  for (int c = 0; c < 5; ++c) {
+  for (int c = 0; c < NC; ++c) {
     double fab = (fl[c][0] + fl[c][1]) / 4.0;
     double fbc = (fl[c][1] + fl[c][2]) / 4.0;
     double fca = (fl[c][2] + fl[c][0]) / 4.0;
@@ -282,8 +290,10 @@ __device__ inline void face_flux_lds(int la, int lb, int lc, int sym,
 // state, gradient and coordinates.  Thread t owns local node t and keeps its
 // nodal sum in registers.  Per chunk: each thread evaluates one edge flux (or
 // one face's three corner terms) into LDS, then every node thread adds the
-// chunk's slots that touch it, in slot order.
-template <int FLUX>
+// chunk's slots that touch it, in slot order.  NC components per node
+// (5 + transported scalars; with scalars the LDS of one workgroup exceeds
+// half a CU's, so one workgroup runs per CU).
+template <int FLUX, int NC>
 __global__ __launch_bounds__(kRhsThreads) __attribute__((amdgpu_waves_per_eu(4)))
 void k_rhs_blocks(
     int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
@@ -297,11 +307,11 @@ void k_rhs_blocks(
     const double* __restrict__ G, const double* __restrict__ U,
     double* __restrict__ R, double* __restrict__ S, Cfg cfg) {
   constexpr int CAP = xg::kRhsCap;
-  __shared__ double sN[23 * CAP];
-  double* const sP = sN;             // primitive state, 5 x CAP
-  double* const sG = sN + 5 * CAP;   // gradients, 15 x CAP
-  double* const sX = sN + 20 * CAP;  // coordinates, 3 x CAP
-  __shared__ double F[kRhsThreads * 5];
+  __shared__ double sN[(4 * NC + 3) * CAP];
+  double* const sP = sN;                  // primitive state, NC x CAP
+  double* const sG = sN + NC * CAP;       // gradients, 3 NC x CAP
+  double* const sX = sN + 4 * NC * CAP;   // coordinates, 3 x CAP
+  __shared__ double F[kRhsThreads * NC];
   __shared__ uint16_t sE[2 * kRhsThreads];
   __shared__ uint16_t sT[CAP];
   const int t = threadIdx.x;
@@ -309,21 +319,23 @@ void k_rhs_blocks(
   const int4 B0 = blk0[b];   // own0, nown, nloc, halo_off
   const int4 B1 = blk1[b];   // slot0, chunk0, chunk1
   const int own0 = B0.x, nown = B0.y, nloc = B0.z;
-  // stage the block's nodes (one per thread, nloc <= threads): all 23 loads of
-  // a node are issued before the first is used
+  // stage the block's nodes (one per thread, nloc <= threads): all 4 NC + 3
+  // loads of a node are issued before the first is used
   if (t < nloc) {
     int g = t < nown ? own0 + t : halo[B0.w + t - nown];
-    double s[5], u[5], gg[15];
+    double s[NC], u[NC], gg[3 * NC];
 #pragma unroll
-    for (int c = 0; c < 5; ++c) s[c] = U[(size_t)g * 5 + c];
+    for (int c = 0; c < NC; ++c) s[c] = U[(size_t)g * NC + c];
 #pragma unroll
-    for (int c = 0; c < 15; ++c) gg[c] = G[(size_t)g * 15 + c];
+    for (int c = 0; c < 3 * NC; ++c) gg[c] = G[(size_t)g * (3 * NC) + c];
     double xg_ = x[g], yg = y[g], zg = z[g];
     xg::primitive_fast(s, u);
 #pragma unroll
-    for (int c = 0; c < 5; ++c) sP[c * CAP + t] = u[c];
+    for (int c = 5; c < NC; ++c) u[c] = s[c];
 #pragma unroll
-    for (int c = 0; c < 15; ++c) sG[c * CAP + t] = gg[c];
+    for (int c = 0; c < NC; ++c) sP[c * CAP + t] = u[c];
+#pragma unroll
+    for (int c = 0; c < 3 * NC; ++c) sG[c * CAP + t] = gg[c];
     sX[0 * CAP + t] = xg_;
     sX[1 * CAP + t] = yg;
     sX[2 * CAP + t] = zg;
@@ -347,7 +359,9 @@ void k_rhs_blocks(
     }
   };
   fetch(B1.y, t);
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = 0.0;
   for (int ch = B1.y; ch < B1.z; ++ch) {
     // the thread index, opaque to the optimizer: thread-dependent LDS
     // addresses are recomputed per chunk instead of being held (and spilled)
@@ -362,12 +376,12 @@ void k_rhs_blocks(
     if (tl < C.z) {
       if (C.x == 0) {
         int lp = pq & 0xffff, lq = pq >> 16;
-        double l[5], r[5], f[5], g1[5], g2[5];
+        double l[NC], r[NC], f[NC], g1[NC], g2[NC];
         double vw[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j) vw[j] = sX[j * CAP + lq] - sX[j * CAP + lp];
 #pragma unroll
-        for (int c = 0; c < 5; ++c) {
+        for (int c = 0; c < NC; ++c) {
           l[c] = sP[c * CAP + lp];
           r[c] = sP[c * CAP + lq];
           g1[c] = sG[(c * 3 + 0) * CAP + lp] * vw[0] + sG[(c * 3 + 1) * CAP + lp] * vw[1] +
@@ -376,21 +390,28 @@ void k_rhs_blocks(
                   sG[(c * 3 + 2) * CAP + lq] * vw[2];
         }
         xg::muscl_fast(g1, g2, l, r);
+        xg::ScalarFlux sf;
         if (FLUX == 0)
-          xg::rusanov_fast(cfg, d, l, r, f);
+          xg::rusanov_fast(cfg, d, l, r, f, sf);
         else
-          xg::hllc_fast(cfg, d, l, r, f);
+          xg::hllc_fast(cfg, d, l, r, f, sf);
+        if constexpr (NC > 5) {
+          xg::muscl_scalars<NC>(g1, g2, l, r);
 #pragma unroll
-        for (int c = 0; c < 5; ++c) F[tl * 5 + c] = f[c];
+          for (int c = 5; c < NC; ++c)
+            f[c] = (l[c] * sf.a + r[c] * sf.b) * sf.m + sf.sw * (r[c] - l[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) F[tl * NC + c] = f[c];
       } else {
         int f = C.y + tl;
         uint32_t ab = fab[f], cs = fcs[f];
-        double o[3][5];
-        face_flux_lds(ab & 0xffff, ab >> 16, cs & 0xffff, cs >> 16, sP, sX, cfg.gm1, o);
+        double o[3][NC];
+        face_flux_lds<NC>(ab & 0xffff, ab >> 16, cs & 0xffff, cs >> 16, sP, sX, cfg.gm1, o);
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
-          for (int c = 0; c < 5; ++c) F[(3 * tl + a) * 5 + c] = o[a][c];
+          for (int c = 0; c < NC; ++c) F[(3 * tl + a) * NC + c] = o[a][c];
       }
     }
     fetch(ch + 1, tl);        // in flight during the accumulation below
@@ -399,32 +420,32 @@ void k_rhs_blocks(
       int e0 = tl ? sT[tl - 1] : 0, e1 = sT[tl];
       for (int j = e0; j < e1; ++j) {
         int v = sE[j];
-        const double* fs = F + (v & 0x3ff) * 5;
+        const double* fs = F + (v & 0x3ff) * NC;
         double sg = (v & 0x8000) ? -1.0 : 1.0;
 #pragma unroll
-        for (int c = 0; c < 5; ++c) acc[c] = fma(sg, fs[c], acc[c]);
+        for (int c = 0; c < NC; ++c) acc[c] = fma(sg, fs[c], acc[c]);
       }
     }
   }
   if (t < nown) {
-    double* o = R + (size_t)(own0 + t) * 5;
+    double* o = R + (size_t)(own0 + t) * NC;
 #pragma unroll
-    for (int c = 0; c < 5; ++c) o[c] = acc[c];
+    for (int c = 0; c < NC; ++c) o[c] = acc[c];
   } else if (t < nloc) {
-    double* o = S + (size_t)(B1.x + t - nown) * 5;
+    double* o = S + (size_t)(B1.x + t - nown) * NC;
 #pragma unroll
-    for (int c = 0; c < 5; ++c) o[c] = acc[c];
+    for (int c = 0; c < NC; ++c) o[c] = acc[c];
   }
 }
 
 // U = Un - rkcoef*dt*R/vol (RieCG.cpp:1016-1021); STAGE0 first saves
-// Un = U (RieCG.cpp:1000-1003)
-template <bool STAGE0>
+// Un = U (RieCG.cpp:1000-1003); NC components per node
+template <bool STAGE0, int NC>
 __global__ void k_update(int npoin, double coef_dt, double* __restrict__ U,
                          double* __restrict__ Un, const double* __restrict__ R,
                          const double* __restrict__ vol) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)npoin * 5) return;
+  if (i >= (int64_t)npoin * NC) return;
   double un;
   if (STAGE0) {
     un = U[i];
@@ -432,18 +453,18 @@ __global__ void k_update(int npoin, double coef_dt, double* __restrict__ U,
   } else {
     un = Un[i];
   }
-  U[i] = un - coef_dt * R[i] / vol[i / 5];
+  U[i] = un - coef_dt * R[i] / vol[i / NC];
 }
 
 // U = Un - rkcoef*dtp[i]*R/vol with per-node time steps (steady runs,
 // RieCG.cpp:1017-1018)
-template <bool STAGE0>
+template <bool STAGE0, int NC>
 __global__ void k_update_steady(int npoin, double rk, const double* __restrict__ dtp,
                                 double* __restrict__ U, double* __restrict__ Un,
                                 const double* __restrict__ R,
                                 const double* __restrict__ vol) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)npoin * 5) return;
+  if (i >= (int64_t)npoin * NC) return;
   double un;
   if (STAGE0) {
     un = U[i];
@@ -451,7 +472,7 @@ __global__ void k_update_steady(int npoin, double rk, const double* __restrict__
   } else {
     un = Un[i];
   }
-  U[i] = un - rk * dtp[i / 5] * R[i] / vol[i / 5];
+  U[i] = un - rk * dtp[i / NC] * R[i] / vol[i / NC];
 }
 
 // tp += dtp (RieCG.cpp:1047-1050)
@@ -468,41 +489,42 @@ __global__ void k_div_vol(int npoin, double* __restrict__ F, const double* __res
   F[i] /= vol[i / NC];
 }
 
-// riemann::src (Riemann.cpp:880-907): R(p,c) -= s_c(x_p) v[p]
-__global__ void k_src(int npoin, xg::Prob P, const double* __restrict__ x,
+// riemann::src (Riemann.cpp:880-907): R(p,c) -= s_c(x_p) v[p] for the flow
+// components of the nc per node (the scalars have no source)
+__global__ void k_src(int npoin, int nc, xg::Prob P, const double* __restrict__ x,
                       const double* __restrict__ y, const double* __restrict__ z,
                       const double* __restrict__ v, double* __restrict__ R) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npoin) return;
   double s[5];
   xg::source(P, x[p], y[p], z[p], s);
-  double* r = R + (size_t)p * 5;
+  double* r = R + (size_t)p * nc;
   const double vp = v[p];
 #pragma unroll
   for (int c = 0; c < 5; ++c) r[c] -= s[c] * vp;
 }
 
 // physics::dirbc (BC.cpp:29-72) as the RieCG chare calls it: components
-// with mask 1 take the node's initial-condition value
-__global__ void k_dirbc_ic(int nd, const int* __restrict__ node,
+// with mask 1 take the node's initial-condition value (nc per node)
+__global__ void k_dirbc_ic(int nd, int nc, const int* __restrict__ node,
                            const uint8_t* __restrict__ mask,
                            const double* __restrict__ val, double* __restrict__ U) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nd) return;
-  double* u = U + (size_t)node[i] * 5;
-#pragma unroll
-  for (int c = 0; c < 5; ++c)
-    if (mask[i * 5 + c] == 1) u[c] = val[i * 5 + c];
+  double* u = U + (size_t)node[i] * nc;
+  for (int c = 0; c < nc; ++c)
+    if (mask[i * nc + c] == 1) u[c] = val[i * nc + c];
 }
 
 // physics::farbc (BC.cpp:152-221), the (node, normal) pairs of a node in
 // list order
-__global__ void k_farbc(int nf, const int* __restrict__ node, const int* __restrict__ off,
-                        const double* __restrict__ nrm, double fr, double fu, double fv,
-                        double fw, double fp, double g, double* __restrict__ U) {
+__global__ void k_farbc(int nf, int nc, const int* __restrict__ node,
+                        const int* __restrict__ off, const double* __restrict__ nrm,
+                        double fr, double fu, double fv, double fw, double fp, double g,
+                        double* __restrict__ U) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nf) return;
-  double* s = U + (size_t)node[i] * 5;
+  double* s = U + (size_t)node[i] * nc;
   double r = s[0], ru = s[1], rv = s[2], rw = s[3], re = s[4];
   const double a = sqrt(g * fp / fr);
   for (int k = off[i]; k < off[i + 1]; ++k) {
@@ -524,11 +546,12 @@ __global__ void k_farbc(int nf, const int* __restrict__ node, const int* __restr
 
 // physics::prebc (BC.cpp:223-241), a node's (density, pressure) records in
 // list order
-__global__ void k_prebc(int np, const int* __restrict__ node, const int* __restrict__ off,
-                        const double* __restrict__ val, double g, double* __restrict__ U) {
+__global__ void k_prebc(int np, int nc, const int* __restrict__ node,
+                        const int* __restrict__ off, const double* __restrict__ val,
+                        double g, double* __restrict__ U) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  double* s = U + (size_t)node[i] * 5;
+  double* s = U + (size_t)node[i] * nc;
   for (int k = off[i]; k < off[i + 1]; ++k) {
     s[0] = val[k * 2 + 0];
     s[4] = xg::total_energy(g, s[0], s[1] / s[0], s[2] / s[0], s[3] / s[0], val[k * 2 + 1]);
@@ -537,13 +560,13 @@ __global__ void k_prebc(int np, const int* __restrict__ node, const int* __restr
 
 // steady RieCG::dt (RieCG.cpp:810-824): dtp per node and a block minimum
 __global__ __launch_bounds__(kBlock) void k_dt_steady(
-    int npoin, const double* __restrict__ U, const double* __restrict__ vol,
+    int npoin, int nc, const double* __restrict__ U, const double* __restrict__ vol,
     double g, double cfl, double* __restrict__ dtp, double* __restrict__ part) {
   __shared__ double sm[kBlock / 64];
   double m = 1.7976931348623157e308;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npoin;
        p += (int64_t)gridDim.x * blockDim.x) {
-    const double* s = U + p * 5;
+    const double* s = U + p * nc;
     double r = s[0];
     double u = s[1] / r, v = s[2] / r, w = s[3] / r;
     double pr = (s[4] - 0.5 * r * (u * u + v * v + w * w)) * (g - 1.0);
@@ -566,38 +589,52 @@ __global__ __launch_bounds__(kBlock) void k_dt_steady(
 
 // ------------------------------------------------------------- launchers --
 
-int check_mesh(xg_ctx* c, int ncomp) {
-  if (!c) return XG_EINVAL;
-  if (!c->have_mesh) return fail(c, XG_ENOMESH, "no mesh uploaded");
-  if (ncomp != 5)
+int check_ncomp(xg_ctx* c, int ncomp) {
+  if (ncomp < 5 || ncomp > XG_MAX_NCOMP)
     return fail(c, XG_EUNSUPPORTED,
-                "device path supports ncomp = 5 (no transported scalars)");
+                "device path supports ncomp = 5 + at most " +
+                    std::to_string(XG_MAX_NCOMP - 5) + " transported scalars");
   return XG_OK;
 }
 
-// div: G = grad(U)/vol (the RK stage; needs xg_riecg_setup), each node
-// divided by the last kernel that adds to it; otherwise grad(U) undivided
-int launch_grad(xg_ctx* c, const double* U, double* G, bool div = false) {
+int check_mesh(xg_ctx* c, int ncomp) {
+  if (!c) return XG_EINVAL;
+  if (!c->have_mesh) return fail(c, XG_ENOMESH, "no mesh uploaded");
+  return check_ncomp(c, ncomp);
+}
+
+template <int NC>
+void launch_grad_nc(xg_ctx* c, const double* U, double* G, bool div) {
   if (div)
-    hipLaunchKernelGGL(k_grad_blocks<true>, dim3(c->nblk), dim3(kGradThreads), 0,
+    hipLaunchKernelGGL((k_grad_blocks<true, NC>), dim3(c->nblk), dim3(kGradThreads), 0,
                        c->stream, c->nblk, c->blk0, c->blk1, c->halo, c->chk,
                        c->chk_ent, c->tend, c->entries, c->elpq, c->dx, c->dy,
                        c->dz, U, G, c->gflag, c->vol);
   else
-    hipLaunchKernelGGL(k_grad_blocks<false>, dim3(c->nblk), dim3(kGradThreads), 0,
+    hipLaunchKernelGGL((k_grad_blocks<false, NC>), dim3(c->nblk), dim3(kGradThreads), 0,
                        c->stream, c->nblk, c->blk0, c->blk1, c->halo, c->chk,
                        c->chk_ent, c->tend, c->entries, c->elpq, c->dx, c->dy,
                        c->dz, U, G, nullptr, nullptr);
   if (c->nbnode) {
     if (div)
-      hipLaunchKernelGGL(k_grad_bnd<true>, dim3(nblocks(c->nbnode)), dim3(kBlock), 0,
+      hipLaunchKernelGGL((k_grad_bnd<true, NC>), dim3(nblocks(c->nbnode)), dim3(kBlock), 0,
                          c->stream, c->nbnode, c->bnode, c->boff, c->bface,
                          c->tri, c->x, c->y, c->z, U, G, c->vol);
     else
-      hipLaunchKernelGGL(k_grad_bnd<false>, dim3(nblocks(c->nbnode)), dim3(kBlock), 0,
+      hipLaunchKernelGGL((k_grad_bnd<false, NC>), dim3(nblocks(c->nbnode)), dim3(kBlock), 0,
                          c->stream, c->nbnode, c->bnode, c->boff, c->bface,
                          c->tri, c->x, c->y, c->z, U, G, nullptr);
   }
+}
+
+// div: G = grad(U)/vol (the RK stage; needs xg_riecg_setup), each node
+// divided by the last kernel that adds to it; otherwise grad(U) undivided.
+// nc components per node (checked by the caller).
+int launch_grad(xg_ctx* c, int nc, const double* U, double* G, bool div = false) {
+  static_assert(XG_MAX_NCOMP == 7, "one instantiation per component count");
+  if (nc == 5) launch_grad_nc<5>(c, U, G, div);
+  else if (nc == 6) launch_grad_nc<6>(c, U, G, div);
+  else launch_grad_nc<7>(c, U, G, div);
   XG_HIP(c, hipGetLastError());
   return XG_OK;
 }
@@ -607,13 +644,11 @@ xg::Prob prob_of(const xg_ctx* c) {
                   c->cfg.gamma};
 }
 
-int launch_rhs(xg_ctx* c, const double* G, const double* U, double* R,
-               const double* v) {
-  if (c->cfg.flux != 0 && c->cfg.flux != 1)
-    return fail(c, XG_EINVAL, "Flux not configured");
+template <int NC>
+void launch_rhs_nc(xg_ctx* c, const double* G, const double* U, double* R) {
   const double g = c->cfg.gamma;
   Cfg cfg{g, c->cfg.stab2, c->cfg.stab2coef, g - 1.0, g * (g - 1.0)};
-  auto kern = c->cfg.flux == 0 ? k_rhs_blocks<0> : k_rhs_blocks<1>;
+  auto kern = c->cfg.flux == 0 ? k_rhs_blocks<0, NC> : k_rhs_blocks<1, NC>;
   const auto& B = c->rb;
   if (B.nblk)
     hipLaunchKernelGGL(kern, dim3(B.nblk), dim3(kRhsThreads), 0, c->stream,
@@ -621,57 +656,78 @@ int launch_rhs(xg_ctx* c, const double* G, const double* U, double* R,
                        B.entries, B.elpq, B.dx, B.dy, B.dz, B.fab, B.fcs, c->x, c->y,
                        c->z, G, U, R, B.slots, cfg);
   if (B.nfx)
-    hipLaunchKernelGGL(k_fixup<5>, dim3(nblocks(B.nfx)), dim3(kBlock), 0,
+    hipLaunchKernelGGL(k_fixup<NC>, dim3(nblocks(B.nfx)), dim3(kBlock), 0,
                        c->stream, B.nfx, B.fx_node, B.fx_off, B.fx_slot, B.slots, R);
+}
+
+// riemann::rhs with nc components per node (checked by the caller), then
+// the problem's source on the flow components
+int launch_rhs(xg_ctx* c, int nc, const double* G, const double* U, double* R,
+               const double* v) {
+  if (c->cfg.flux != 0 && c->cfg.flux != 1)
+    return fail(c, XG_EINVAL, "Flux not configured");
+  if (nc == 5) launch_rhs_nc<5>(c, G, U, R);
+  else if (nc == 6) launch_rhs_nc<6>(c, G, U, R);
+  else launch_rhs_nc<7>(c, G, U, R);
   const xg::Prob P = prob_of(c);
   if (xg::has_source(P))
     hipLaunchKernelGGL(k_src, dim3(nblocks(c->npoin)), dim3(kBlock), 0, c->stream,
-                       c->npoin, P, c->x, c->y, c->z, v, R);
+                       c->npoin, nc, P, c->x, c->y, c->z, v, R);
   XG_HIP(c, hipGetLastError());
   return XG_OK;
 }
 
 // RieCG::BC (RieCG.cpp:764-785): dirbc, symbc, farbc, prebc
 int launch_bc(xg_ctx* c, double* U) {
+  const int nc = c->ncomp;
   if (c->ndirnode)
     hipLaunchKernelGGL(k_dirbc_ic, dim3(nblocks(c->ndirnode)), dim3(kBlock), 0,
-                       c->stream, c->ndirnode, c->dirnode, c->dirmask, c->dirval, U);
+                       c->stream, c->ndirnode, nc, c->dirnode, c->dirmask, c->dirval, U);
   if (c->nsymnode)
     hipLaunchKernelGGL(k_symbc, dim3(nblocks(c->nsymnode)), dim3(kBlock), 0,
                        c->stream, c->nsymnode, c->symnode, c->symoff,
-                       c->symnorm, U, 5, 1);
+                       c->symnorm, U, nc, 1);
   if (c->nfarnode)
     hipLaunchKernelGGL(k_farbc, dim3(nblocks(c->nfarnode)), dim3(kBlock), 0,
-                       c->stream, c->nfarnode, c->farnode, c->faroff, c->farnorm,
+                       c->stream, c->nfarnode, nc, c->farnode, c->faroff, c->farnorm,
                        c->far[0], c->far[1], c->far[2], c->far[3], c->far[4],
                        c->cfg.gamma, U);
   if (c->nprenode)
     hipLaunchKernelGGL(k_prebc, dim3(nblocks(c->nprenode)), dim3(kBlock), 0,
-                       c->stream, c->nprenode, c->prenode, c->preoff, c->preval,
+                       c->stream, c->nprenode, nc, c->prenode, c->preoff, c->preval,
                        c->cfg.gamma, U);
   XG_HIP(c, hipGetLastError());
   return XG_OK;
+}
+
+template <int NC>
+void launch_update_nc(xg_ctx* c, int stage, double rk, double dt, const double* dtp,
+                      double* U, double* Un, const double* R) {
+  const int64_t n = (int64_t)c->npoin * NC;
+  if (dtp) {
+    if (stage == 0)
+      hipLaunchKernelGGL((k_update_steady<true, NC>), dim3(nblocks(n)), dim3(kBlock), 0,
+                         c->stream, c->npoin, rk, dtp, U, Un, R, c->vol);
+    else
+      hipLaunchKernelGGL((k_update_steady<false, NC>), dim3(nblocks(n)), dim3(kBlock), 0,
+                         c->stream, c->npoin, rk, dtp, U, Un, R, c->vol);
+  } else if (stage == 0) {
+    hipLaunchKernelGGL((k_update<true, NC>), dim3(nblocks(n)), dim3(kBlock), 0, c->stream,
+                       c->npoin, rk * dt, U, Un, R, c->vol);
+  } else {
+    hipLaunchKernelGGL((k_update<false, NC>), dim3(nblocks(n)), dim3(kBlock), 0, c->stream,
+                       c->npoin, rk * dt, U, Un, R, c->vol);
+  }
 }
 
 // RieCG::solve (RieCG.cpp:992-1057) after the rhs is complete
 int launch_solve(xg_ctx* c, int stage, double dt, const double* dtp, double* tp,
                  double* U, double* Un, const double* R) {
   static const double rkcoef[3] = {1.0 / 3.0, 1.0 / 2.0, 1.0};
-  const int64_t n5 = (int64_t)c->npoin * 5;
-  if (dtp) {
-    if (stage == 0)
-      hipLaunchKernelGGL(k_update_steady<true>, dim3(nblocks(n5)), dim3(kBlock), 0,
-                         c->stream, c->npoin, rkcoef[stage], dtp, U, Un, R, c->vol);
-    else
-      hipLaunchKernelGGL(k_update_steady<false>, dim3(nblocks(n5)), dim3(kBlock), 0,
-                         c->stream, c->npoin, rkcoef[stage], dtp, U, Un, R, c->vol);
-  } else if (stage == 0) {
-    hipLaunchKernelGGL(k_update<true>, dim3(nblocks(n5)), dim3(kBlock), 0, c->stream,
-                       c->npoin, rkcoef[stage] * dt, U, Un, R, c->vol);
-  } else {
-    hipLaunchKernelGGL(k_update<false>, dim3(nblocks(n5)), dim3(kBlock), 0, c->stream,
-                       c->npoin, rkcoef[stage] * dt, U, Un, R, c->vol);
-  }
+  const double rk = rkcoef[stage];
+  if (c->ncomp == 5) launch_update_nc<5>(c, stage, rk, dt, dtp, U, Un, R);
+  else if (c->ncomp == 6) launch_update_nc<6>(c, stage, rk, dt, dtp, U, Un, R);
+  else launch_update_nc<7>(c, stage, rk, dt, dtp, U, Un, R);
   int rc = launch_bc(c, U);
   if (rc) return rc;
   if (dtp && stage == 2)
@@ -734,7 +790,8 @@ int upload_blocks(xg_ctx* c, std::vector<DevArr>& M, const xg::Layout& L,
       (rc = upload(c, M, &B.dx, L.dx)) || (rc = upload(c, M, &B.dy, L.dy)) ||
       (rc = upload(c, M, &B.dz, L.dz)) || (rc = upload(c, M, &B.fab, L.fab)) ||
       (rc = upload(c, M, &B.fcs, L.fcs)) ||
-      (rc = dev_alloc(c, M, (void**)&B.slots, (size_t)L.nslot * 5 * sizeof(double))) ||
+      (rc = dev_alloc(c, M, (void**)&B.slots,
+                      (size_t)L.nslot * XG_MAX_NCOMP * sizeof(double))) ||
       (rc = upload(c, M, &B.fx_node, L.fx_node)) || (rc = upload(c, M, &B.fx_off, L.fx_off)) ||
       (rc = upload(c, M, &B.fx_slot, L.fx_slot)))
     return rc;
@@ -743,15 +800,17 @@ int upload_blocks(xg_ctx* c, std::vector<DevArr>& M, const xg::Layout& L,
   return XG_OK;
 }
 
-int stage_buffers(xg_ctx* c) {
+// device staging of the host-pointer entries for nc components per node
+int stage_buffers(xg_ctx* c, int nc) {
   size_t need = (size_t)c->npoin * 8;
-  if (c->sbytes >= need) return XG_OK;
+  if (c->sbytes >= need && c->snc >= nc) return XG_OK;
   if (c->sU) { (void)hipFree(c->sU); (void)hipFree(c->sG); (void)hipFree(c->sR); }
-  c->sU = c->sG = c->sR = nullptr; c->sbytes = 0;
-  XG_HIP(c, hipMalloc((void**)&c->sU, need * 5));
-  XG_HIP(c, hipMalloc((void**)&c->sG, need * 16));   // G and, after it, v
-  XG_HIP(c, hipMalloc((void**)&c->sR, need * 5));
+  c->sU = c->sG = c->sR = nullptr; c->sbytes = 0; c->snc = 0;
+  XG_HIP(c, hipMalloc((void**)&c->sU, need * nc));
+  XG_HIP(c, hipMalloc((void**)&c->sG, need * (3 * nc + 1)));   // G and, after it, v
+  XG_HIP(c, hipMalloc((void**)&c->sR, need * nc));
   c->sbytes = need;
+  c->snc = nc;
   return XG_OK;
 }
 
@@ -935,7 +994,7 @@ int xg_riemann_grad_dev(xg_ctx* c, int ncomp, const double* U, double* G) {
   if (rc) return rc;
   if (!U || !G) return fail(c, XG_EINVAL, "xg_riemann_grad: null field");
   XG_HIP(c, hipSetDevice(c->device));
-  return launch_grad(c, U, G);
+  return launch_grad(c, ncomp, U, G);
 }
 
 int xg_riemann_rhs_dev(xg_ctx* c, int ncomp, const double* G, const double* U,
@@ -948,7 +1007,7 @@ int xg_riemann_rhs_dev(xg_ctx* c, int ncomp, const double* G, const double* U,
   if (xg::has_source(prob_of(c)) && !v)
     return fail(c, XG_EINVAL, "xg_riemann_rhs: the problem's source needs the nodal volumes v");
   XG_HIP(c, hipSetDevice(c->device));
-  return launch_rhs(c, G, U, R, v);
+  return launch_rhs(c, ncomp, G, U, R, v);
 }
 
 int xg_riemann_grad(xg_ctx* c, int ncomp, const double* U, double* G) {
@@ -956,11 +1015,11 @@ int xg_riemann_grad(xg_ctx* c, int ncomp, const double* U, double* G) {
   if (rc) return rc;
   if (!U || !G) return fail(c, XG_EINVAL, "xg_riemann_grad: null field");
   XG_HIP(c, hipSetDevice(c->device));
-  if ((rc = stage_buffers(c))) return rc;
-  size_t n = (size_t)c->npoin;
-  XG_HIP(c, hipMemcpyAsync(c->sU, U, n * 40, hipMemcpyHostToDevice, c->stream));
-  if ((rc = launch_grad(c, c->sU, c->sG))) return rc;
-  XG_HIP(c, hipMemcpyAsync(G, c->sG, n * 120, hipMemcpyDeviceToHost, c->stream));
+  if ((rc = stage_buffers(c, ncomp))) return rc;
+  size_t n = (size_t)c->npoin * 8;
+  XG_HIP(c, hipMemcpyAsync(c->sU, U, n * ncomp, hipMemcpyHostToDevice, c->stream));
+  if ((rc = launch_grad(c, ncomp, c->sU, c->sG))) return rc;
+  XG_HIP(c, hipMemcpyAsync(G, c->sG, n * 3 * ncomp, hipMemcpyDeviceToHost, c->stream));
   XG_HIP(c, hipStreamSynchronize(c->stream));
   return XG_OK;
 }
@@ -976,18 +1035,19 @@ int xg_riemann_rhs(xg_ctx* c, int ncomp, const double* G, const double* U,
   if (src && !v)
     return fail(c, XG_EINVAL, "xg_riemann_rhs: the problem's source needs the nodal volumes v");
   XG_HIP(c, hipSetDevice(c->device));
-  if ((rc = stage_buffers(c))) return rc;
+  if ((rc = stage_buffers(c, ncomp))) return rc;
   size_t n = (size_t)c->npoin;
-  XG_HIP(c, hipMemcpyAsync(c->sU, U, n * 40, hipMemcpyHostToDevice, c->stream));
-  XG_HIP(c, hipMemcpyAsync(c->sG, G, n * 120, hipMemcpyHostToDevice, c->stream));
-  // the volumes are staged after G in the G staging buffer (16 doubles per node)
+  XG_HIP(c, hipMemcpyAsync(c->sU, U, n * 8 * ncomp, hipMemcpyHostToDevice, c->stream));
+  XG_HIP(c, hipMemcpyAsync(c->sG, G, n * 24 * ncomp, hipMemcpyHostToDevice, c->stream));
+  // the volumes are staged after G in the G staging buffer (3 ncomp + 1
+  // doubles per node)
   double* sv = nullptr;
   if (src) {
-    sv = c->sG + n * 15;
+    sv = c->sG + n * 3 * ncomp;
     XG_HIP(c, hipMemcpyAsync(sv, v, n * 8, hipMemcpyHostToDevice, c->stream));
   }
-  if ((rc = launch_rhs(c, c->sG, c->sU, c->sR, sv))) return rc;
-  XG_HIP(c, hipMemcpyAsync(R, c->sR, n * 40, hipMemcpyDeviceToHost, c->stream));
+  if ((rc = launch_rhs(c, ncomp, c->sG, c->sU, c->sR, sv))) return rc;
+  XG_HIP(c, hipMemcpyAsync(R, c->sR, n * 8 * ncomp, hipMemcpyDeviceToHost, c->stream));
   XG_HIP(c, hipStreamSynchronize(c->stream));
   return XG_OK;
 }
@@ -1029,6 +1089,7 @@ int xg_riecg_setup(xg_ctx* c, const double* vol, int64_t nsym,
   dev_free_all(c->bc_allocs);
   c->ndirnode = c->nfarnode = c->nprenode = 0;
   c->have_riecg = false;
+  c->ncomp = 5;
   int rc;
   auto& L = c->riecg_allocs;
   if ((rc = upload(c, L, &c->vol, vol, (size_t)c->npoin)) ||
@@ -1048,6 +1109,22 @@ int xg_riecg_setup(xg_ctx* c, const double* vol, int64_t nsym,
   return XG_OK;
 }
 
+int xg_riecg_set_ncomp(xg_ctx* c, int ncomp) {
+  if (!c) return XG_EINVAL;
+  if (!c->have_riecg) return fail(c, XG_ENOMESH, "xg_riecg_setup not called");
+  int rc = check_ncomp(c, ncomp);
+  if (rc) return rc;
+  if (ncomp != c->ncomp) {
+    // the Dirichlet masks and values are per component: drop the lists
+    XG_HIP(c, hipSetDevice(c->device));
+    (void)hipStreamSynchronize(c->stream);
+    dev_free_all(c->bc_allocs);
+    c->ndirnode = c->nfarnode = c->nprenode = 0;
+  }
+  c->ncomp = ncomp;
+  return XG_OK;
+}
+
 int xg_riecg_dt_dev(xg_ctx* c, const double* U, double cfl, double* dt) {
   if (!c) return XG_EINVAL;
   if (!c->have_riecg) return fail(c, XG_ENOMESH, "xg_riecg_setup not called");
@@ -1055,7 +1132,7 @@ int xg_riecg_dt_dev(xg_ctx* c, const double* U, double cfl, double* dt) {
   XG_HIP(c, hipSetDevice(c->device));
   int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, nblocks(c->npoin)));
   hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(kBlock), 0, c->stream,
-                     c->npoin, U, c->vol, c->cfg.gamma, c->red);
+                     c->npoin, c->ncomp, U, c->vol, c->cfg.gamma, c->red);
   hipLaunchKernelGGL(k_min_final, dim3(1), dim3(kRedBlocks), 0, c->stream, nb,
                      c->red, cfl, c->red + kRedBlocks);
   XG_HIP(c, hipGetLastError());
@@ -1074,8 +1151,8 @@ int xg_riecg_dt_steady_dev(xg_ctx* c, const double* U, double cfl, double* dtp,
   if (!U || !dtp || !dt) return fail(c, XG_EINVAL, "xg_riecg_dt_steady: null argument");
   XG_HIP(c, hipSetDevice(c->device));
   int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, nblocks(c->npoin)));
-  hipLaunchKernelGGL(k_dt_steady, dim3(nb), dim3(kBlock), 0, c->stream, c->npoin, U,
-                     c->vol, c->cfg.gamma, cfl, dtp, c->red);
+  hipLaunchKernelGGL(k_dt_steady, dim3(nb), dim3(kBlock), 0, c->stream, c->npoin,
+                     c->ncomp, U, c->vol, c->cfg.gamma, cfl, dtp, c->red);
   hipLaunchKernelGGL(k_min_final, dim3(1), dim3(kRedBlocks), 0, c->stream, nb,
                      c->red, 1.0, c->red + kRedBlocks);
   XG_HIP(c, hipGetLastError());
@@ -1097,13 +1174,14 @@ int xg_riecg_bc(xg_ctx* c, int64_t ndir, const size_t* dirnodes, const int32_t* 
       (ndir && (!dirnodes || !dirmask || !dirvals)) ||
       (nfar && (!farnodes || !farnorms || !far)) || (npre && (!prenodes || !prevals)))
     return fail(c, XG_EINVAL, "xg_riecg_bc: bad arguments");
+  const int nc = c->ncomp;
   std::vector<int> dn(ndir);
-  std::vector<uint8_t> dm(ndir * 5);
+  std::vector<uint8_t> dm(ndir * nc);
   for (int64_t i = 0; i < ndir; ++i) {
     if (dirnodes[i] >= (size_t)c->npoin)
       return fail(c, XG_EINVAL, "xg_riecg_bc: Dirichlet node out of range");
     dn[i] = (int)dirnodes[i];
-    for (int k = 0; k < 5; ++k) dm[i * 5 + k] = (uint8_t)(dirmask[i * 5 + k] == 1);
+    for (int k = 0; k < nc; ++k) dm[i * nc + k] = (uint8_t)(dirmask[i * nc + k] == 1);
   }
   std::vector<int> fnode, foff, pnode, poff;
   std::vector<double> fnrm, pval;
@@ -1119,7 +1197,7 @@ int xg_riecg_bc(xg_ctx* c, int64_t ndir, const size_t* dirnodes, const int32_t* 
   c->ndirnode = c->nfarnode = c->nprenode = 0;
   auto& L = c->bc_allocs;
   if ((rc = upload(c, L, &c->dirnode, dn)) || (rc = upload(c, L, &c->dirmask, dm)) ||
-      (rc = upload(c, L, &c->dirval, dirvals, (size_t)ndir * 5)) ||
+      (rc = upload(c, L, &c->dirval, dirvals, (size_t)ndir * nc)) ||
       (rc = upload(c, L, &c->farnode, fnode)) || (rc = upload(c, L, &c->faroff, foff)) ||
       (rc = upload(c, L, &c->farnorm, fnrm)) || (rc = upload(c, L, &c->prenode, pnode)) ||
       (rc = upload(c, L, &c->preoff, poff)) || (rc = upload(c, L, &c->preval, pval)))
@@ -1151,8 +1229,8 @@ int xg_riecg_stage_dev(xg_ctx* c, int stage, double t, double dt, const double* 
     return fail(c, XG_EINVAL, "xg_riecg_stage: dtp and tp go together");
   XG_HIP(c, hipSetDevice(c->device));
   int rc;
-  if ((rc = launch_grad(c, U, G, true))) return rc;
-  if ((rc = launch_rhs(c, G, U, R, c->vol))) return rc;
+  if ((rc = launch_grad(c, c->ncomp, U, G, true))) return rc;
+  if ((rc = launch_rhs(c, c->ncomp, G, U, R, c->vol))) return rc;
   return launch_solve(c, stage, dt, dtp, tp, U, Un, R);
 }
 
@@ -1161,8 +1239,16 @@ int xg_riecg_grad_div_dev(xg_ctx* c, double* G) {
   if (!c->have_riecg) return fail(c, XG_ENOMESH, "xg_riecg_setup not called");
   if (!G) return fail(c, XG_EINVAL, "xg_riecg_grad_div: null field");
   XG_HIP(c, hipSetDevice(c->device));
-  hipLaunchKernelGGL(k_div_vol<15>, dim3(nblocks((int64_t)c->npoin * 15)), dim3(kBlock), 0,
-                     c->stream, c->npoin, G, c->vol);
+  const int64_t n = (int64_t)c->npoin * 3 * c->ncomp;
+  if (c->ncomp == 5)
+    hipLaunchKernelGGL(k_div_vol<15>, dim3(nblocks(n)), dim3(kBlock), 0, c->stream,
+                       c->npoin, G, c->vol);
+  else if (c->ncomp == 6)
+    hipLaunchKernelGGL(k_div_vol<18>, dim3(nblocks(n)), dim3(kBlock), 0, c->stream,
+                       c->npoin, G, c->vol);
+  else
+    hipLaunchKernelGGL(k_div_vol<21>, dim3(nblocks(n)), dim3(kBlock), 0, c->stream,
+                       c->npoin, G, c->vol);
   XG_HIP(c, hipGetLastError());
   return XG_OK;
 }
@@ -1183,7 +1269,7 @@ int xg_riecg_solve_dev(xg_ctx* c, int stage, double t, double dt, const double* 
 int xg_set_problem(xg_ctx* c, const xg_problem* p) {
   if (!c || !p) return XG_EINVAL;
   if (p->kind != XG_PROBLEM_NONE && p->kind != XG_PROBLEM_VORTICAL_FLOW &&
-      p->kind != XG_PROBLEM_TAYLOR_GREEN)
+      p->kind != XG_PROBLEM_TAYLOR_GREEN && p->kind != XG_PROBLEM_SLOT_CYL)
     return fail(c, XG_EUNSUPPORTED, "xg_set_problem: no device source for this problem");
   c->problem = *p;
   return XG_OK;

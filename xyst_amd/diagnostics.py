@@ -11,7 +11,8 @@ __all__ = ["rhocompute", "errcompute", "precompute"]
 
 def rhocompute(ctx, U, Un, V):
     """Sums of U_c^2 V, (U_c - Un_c)^2 V and U_4 V; U, Un (npoin, ncomp),
-    V (npoin,) device tensors, ncomp 1, 3 or 5."""
+    V (npoin,) device tensors, ncomp 1, 3 or 5..7 (5 + transported
+    scalars)."""
     n, nc = int(U.shape[0]), int(U.shape[1])
     out = np.empty(2 * nc + 1)
     ctx._check(lib().xg_diag_rho_dev(ctx._h, n, nc, ptr(U), ptr(Un), ptr(V), ptr(out)))
@@ -19,12 +20,16 @@ def rhocompute(ctx, U, Un, V):
 
 
 def errcompute(ctx, U, V, An):
-    """The analytic-error sums of rhocompute: (nu_c - An_c)^2 V (5 values)
-    and |nu_c - An_c| V (5 values), nu the primitive form of U (rho, u, v,
-    w, specific internal energy) and An the analytic solution in that form;
-    U, An (npoin, 5), V (npoin,) device tensors."""
-    out = np.empty(10)
-    ctx._check(lib().xg_diag_err_dev(ctx._h, int(U.shape[0]), 5, ptr(U), ptr(V), ptr(An),
+    """The analytic-error sums of rhocompute: (nu_c - An_c)^2 V (ncomp
+    values) and |nu_c - An_c| V (ncomp values), nu the primitive form of U
+    (rho, u, v, w, specific internal energy, then the scalars) and An the
+    analytic solution in that form; U, An (npoin, ncomp), V (npoin,) device
+    tensors, ncomp 5..7."""
+    nc = int(U.shape[1])
+    if tuple(An.shape) != tuple(U.shape):
+        raise ValueError("An must have the shape of U")
+    out = np.empty(2 * nc)
+    ctx._check(lib().xg_diag_err_dev(ctx._h, int(U.shape[0]), nc, ptr(U), ptr(V), ptr(An),
                                      ptr(out)))
     return out
 

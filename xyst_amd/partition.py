@@ -102,8 +102,10 @@ class PartitionedRieCG:
     and `gather` move a global host array in and out."""
 
     def __init__(self, coord, inpoel, tri, owner, device=0, gamma=1.4, flux="rusanov",
-                 stab2=False, stab2coef=0.2, problem="none", besym=None, **problem_args):
+                 stab2=False, stab2coef=0.2, problem="none", besym=None, ncomp=5,
+                 **problem_args):
         import torch
+        self.ncomp = ncomp
         self.torch = torch
         self.dev = torch.device("cuda", device)
         coord = np.ascontiguousarray(coord, dtype=np.float64)
@@ -132,7 +134,7 @@ class PartitionedRieCG:
         self.Vol = Vol
         self.comm = Comm([p.gid for p in self.parts], device)
         for p, ctx in zip(self.parts, self.ctx):
-            ctx.riecg_setup(Vol[p.gid], np.zeros(0, np.int64), np.zeros((0, 3)))
+            ctx.riecg_setup(Vol[p.gid], np.zeros(0, np.int64), np.zeros((0, 3)), ncomp)
 
     def close(self):
         for c in self.ctx:
@@ -156,9 +158,10 @@ class PartitionedRieCG:
         holds the node."""
         for p, ctx in zip(self.parts, self.ctx):
             sn, sv = self._local(p, symnodes, np.reshape(symnorms, (-1, 3)))
-            ctx.riecg_setup(self.Vol[p.gid], sn, sv)
-            dn, dm, dv = self._local(p, dirnodes, np.reshape(dirmask, (-1, 5)),
-                                     np.reshape(dirvals, (-1, 5)))
+            ctx.riecg_setup(self.Vol[p.gid], sn, sv, self.ncomp)
+            nc = self.ncomp
+            dn, dm, dv = self._local(p, dirnodes, np.reshape(dirmask, (-1, nc)),
+                                     np.reshape(dirvals, (-1, nc)))
             fn, fv = self._local(p, farnodes, np.reshape(farnorms, (-1, 3)))
             pn, pv = self._local(p, prenodes, np.reshape(prevals, (-1, 2)))
             ctx.riecg_bc(dn, dm, dv, fn, fv, far, pn, pv)
@@ -186,8 +189,8 @@ class PartitionedRieCG:
         exchange, G/Vol, the partial rhs with the source on each partition's
         own volumes, the rhs exchange, the update and the BCs."""
         torch = self.torch
-        G = [torch.empty((u.shape[0], 15), dtype=torch.float64, device=self.dev)
-             for u in Us]
+        G = [torch.empty((u.shape[0], 3 * self.ncomp), dtype=torch.float64,
+                         device=self.dev) for u in Us]
         R = [torch.empty_like(u) for u in Us]
         for stage in range(3):
             for ctx, u, g in zip(self.ctx, Us, G):

@@ -130,6 +130,7 @@ class Context:
         check(L.xg_ctx_create(device, C.byref(h)), what="xg_ctx_create (no GPU?)")
         self._h = h
         self.npoin = 0
+        self.ncomp = 5
         self.set_config(gamma, flux, stab2, stab2coef)
         if stream is not None:
             self.set_stream(stream)
@@ -236,7 +237,9 @@ class Context:
                                              t, ptr(tp), ptr(R)))
 
     # resident RieCG stepping
-    def riecg_setup(self, vol, symnodes, symnorms):
+    def riecg_setup(self, vol, symnodes, symnorms, ncomp=5):
+        """The chare's nodal volumes and symmetry lists; ncomp = 5 + the
+        problem's transported scalars (the fields' components per node)."""
         v = _f64(vol)
         if v.shape != (self.npoin,):
             raise ValueError("vol must have npoin entries")
@@ -245,9 +248,19 @@ class Context:
         self._check(lib().xg_riecg_setup(self._h, ptr(v), sn.size,
                                          ptr(sn) if sn.size else None,
                                          ptr(sv) if sv.size else None))
+        self._check(lib().xg_riecg_set_ncomp(self._h, ncomp))
+        self.ncomp = ncomp
+
+    def _stepped(self, *fields):
+        """The stepping entries read the chare's ncomp components per node."""
+        for a in fields:
+            if a is not None and a.shape[1] != self.ncomp:
+                raise ValueError("fields have %d components, the chare %d"
+                                 % (a.shape[1], self.ncomp))
 
     def dt_dev(self, U, cfl):
         self._fields(U, dev=True)
+        self._stepped(U)
         out = C.c_double(0.0)
         self._check(lib().xg_riecg_dt_dev(self._h, ptr(U), cfl, C.byref(out)))
         return out.value
@@ -264,7 +277,8 @@ class Context:
         fa = _f64(far).reshape(-1)
         pn = _idx(prenodes).reshape(-1)
         pv = _f64(prevals).reshape(-1)
-        if dm.size != 5 * dn.size or dv.size != 5 * dn.size or fv.size != 3 * fn.size \
+        nc = self.ncomp
+        if dm.size != nc * dn.size or dv.size != nc * dn.size or fv.size != 3 * fn.size \
                 or pv.size != 2 * pn.size or fa.size != 5:
             raise ValueError("riecg_bc: list sizes disagree")
         self._check(lib().xg_riecg_bc(self._h, dn.size, ptr(dn) if dn.size else None,
@@ -278,11 +292,13 @@ class Context:
     def apply_bc_dev(self, U):
         """RieCG::BC on a device state (dirbc, symbc, farbc, prebc)."""
         self._fields(U, dev=True)
+        self._stepped(U)
         self._check(lib().xg_riecg_apply_bc_dev(self._h, ptr(U)))
 
     def dt_steady_dev(self, U, cfl, dtp):
         """Steady RieCG::dt: fills dtp (device, npoin), returns its minimum."""
         self._fields(U, dev=True)
+        self._stepped(U)
         out = C.c_double(0.0)
         self._check(lib().xg_riecg_dt_steady_dev(self._h, ptr(U), cfl, ptr(dtp),
                                                  C.byref(out)))
@@ -291,15 +307,19 @@ class Context:
     def stage_dev(self, stage, dt, U, Un, G, R, t=0.0, dtp=None, tp=None):
         self._fields(U, G, R, dev=True)
         self._fields(Un, dev=True)
+        self._stepped(U, Un)
         self._check(lib().xg_riecg_stage_dev(self._h, stage, t, dt, ptr(dtp), ptr(tp),
                                              ptr(U), ptr(Un), ptr(G), ptr(R)))
 
     def grad_div_dev(self, G):
         """G /= the summed nodal volume (after the partitions' exchange)."""
+        if tuple(G.shape) != (self.npoin, 3 * self.ncomp):
+            raise ValueError("G must be (npoin, 3 ncomp)")
         self._check(lib().xg_riecg_grad_div_dev(self._h, ptr(G)))
 
     def solve_dev(self, stage, dt, U, Un, R, t=0.0, dtp=None, tp=None):
         """RieCG::solve with a complete rhs R: update and BCs."""
         self._fields(U, R=R, dev=True)
+        self._stepped(U, Un)
         self._check(lib().xg_riecg_solve_dev(self._h, stage, t, dt, ptr(dtp), ptr(tp),
                                              ptr(U), ptr(Un), ptr(R)))

@@ -21,6 +21,8 @@ void set_problem(xg_ctx* c, const std::string& name, double alpha,
     p.kind = XG_PROBLEM_VORTICAL_FLOW;
   else if (name == "taylor_green")
     p.kind = XG_PROBLEM_TAYLOR_GREEN;
+  else if (name == "slot_cyl")
+    p.kind = XG_PROBLEM_SLOT_CYL;
   else if (problems::SRC())
     Throw("xg: the source term of problem '" + name +
           "' is not evaluated on the device");
