@@ -22,8 +22,8 @@
 
 int main(int argc, char** argv) {
   CHECK(argc == 2 || argc == 3);
-  const int tpc = argc == 3 ? std::atoi(argv[2]) : xg::kTetsPerChunk;
-  CHECK(tpc == xg::kTetsPerChunk || tpc == xg::kTetsPerPairChunk);
+  const int tpc = argc == 3 ? std::atoi(argv[2]) : xg::kTetsPerPairChunk;
+  CHECK(tpc >= 1 && tpc <= xg::kTetsPerPairChunk);
   FILE* f = std::fopen(argv[1], "rb");
   CHECK(f);
   int64_t hdr[2];

@@ -2,25 +2,25 @@
 // Taylor-Galerkin right-hand side of one GPU-resident partition.
 //
 // Layout (xg_layout.cpp, build_tet_layout): node blocks of consecutive ids;
-// a block owns the tets whose smallest node it holds.  One 512-thread
-// workgroup per block (two share a CU) stages the conserved state and
-// coordinates of the block's nodes (owned + halo) in LDS and walks the
-// block's tets chunk by chunk, the next chunk's tet records and entry lists
-// loaded while the current one computes.
-//  * kozak::rhs (k_kozak_pairs): chunks of 256 tets, two threads per tet;
-//    each thread evaluates the tet geometry and two corners' shares of the
-//    half-step state, one lane exchange with its partner completes the
-//    state, and the thread writes its two corners' second-step fluxes.
-//  * with the antidiffusive contributions of the FCT step (k_kozak_blocks)
-//    and in the FCT kernels: chunks of 128 tets, thread t handles corner
-//    a = t % 4 of tet t / 4; the four corners combine their shares with two
-//    lane exchanges within the quad (DPP), so every corner sees the same sum.
-// Then the thread owning local node t adds the chunk's corner values that
-// touch it, in slot order; halo partials go to slots that k_fixup adds to
-// their owners in block order.  No floating-point atomic sums: every sum has
-// a fixed order and the result is bitwise reproducible (the FCT bounds
-// kernel merges max/min with LDS ds_max_f64/ds_min_f64, which are exact and
-// independent of order).
+// a block owns the tets whose smallest node it holds, in chunks of 256 tets.
+// One 512-thread workgroup per block (two share a CU) stages the conserved
+// state and coordinates of the block's nodes (owned + halo) in LDS and walks
+// the block's tets chunk by chunk, the next chunk's tet records and entry
+// lists loaded while the current one computes.  Thread t handles corners 2h
+// and 2h + 1 (h = t % 2) of chunk tet t / 2; the two threads of a tet are
+// neighbouring lanes and combine their partial values with one DPP lane
+// exchange.
+//  * kozak::rhs (k_kozak_pairs): each thread evaluates the tet geometry and
+//    two corners' shares of the half-step state, the exchange completes the
+//    state, and the thread writes its two corners' second-step fluxes; in
+//    the FCT step also their antidiffusive contributions (second pass).
+//  * the FCT bounds (k_kozcg_alw) and limiter (k_kozcg_lim) on the same
+//    chunks.
+// Then the thread owning local node t adds (or, for the bounds, takes the
+// max/min of) the chunk's corner values that touch it, in slot order; halo
+// partials go to slots that k_fixup adds to their owners in block order.
+// No atomics: every sum has a fixed order and the result is bitwise
+// reproducible.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,18 +42,13 @@ struct KozakMesh {
   int nblk = 0;
   std::vector<DevArr> allocs;
   double *x = nullptr, *y = nullptr, *z = nullptr;
+  // node blocks, their tets in chunks of kTetsPerPairChunk
   int4 *blk0 = nullptr, *blk1 = nullptr;
   int* halo = nullptr;
   int4* chk = nullptr;
   int* chk_ent = nullptr;
   uint16_t *tend = nullptr, *entries = nullptr;
   uint32_t *tab = nullptr, *tcd = nullptr;
-  // the same blocks cut into chunks of kTetsPerPairChunk tets (kozak::rhs
-  // without the antidiffusive contributions, two threads per tet)
-  int4 *blk1p = nullptr, *chkp = nullptr;
-  int* chk_entp = nullptr;
-  uint16_t *tendp = nullptr, *entriesp = nullptr;
-  uint32_t *tabp = nullptr, *tcdp = nullptr;
   double* slots = nullptr;
   int nfx = 0;
   int *fx_node = nullptr, *fx_off = nullptr, *fx_slot = nullptr;
@@ -74,12 +69,12 @@ struct KozakMesh {
 
 namespace {
 
-constexpr int kKozThreads = xg::kKozSlots;
+constexpr int kKozThreads = xg::kKozPairSlots / 2;   // two corners per thread
 static_assert(xg::kKozCap <= kKozThreads, "one local node per thread");
 
-// v from lane (lane ^ 1) or (lane ^ 2) of the same quad: two v_mov_b32_dpp
-// with quad_perm [1,0,3,2] (0xB1) or [2,3,0,1] (0x4E).  Every lane of the
-// wave is active where these are used.
+// v from lane (lane ^ 1) of the same quad: two v_mov_b32_dpp with
+// quad_perm [1,0,3,2] (0xB1).  Every lane of the wave is active where this
+// is used.
 template <int CTRL>
 __device__ inline double quad_swap(double v) {
   const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
@@ -87,15 +82,16 @@ __device__ inline double quad_swap(double v) {
   return __hiloint2double(hi, lo);
 }
 __device__ inline double quad_xor1(double v) { return quad_swap<0xB1>(v); }
-__device__ inline double quad_xor2(double v) { return quad_swap<0x4E>(v); }
 
-// One chunk's tet record (corner a of tet i reads its tet's four local ids)
-// and entry lists, loaded one chunk ahead so that their global-memory
-// latency overlaps the current chunk's work.
-struct TetPrefetch {
+// One chunk of the pair layout, loaded one chunk ahead so that its
+// global-memory latency overlaps the current chunk's work: the chunk record,
+// thread t's tet (t >> 1; idle threads read the chunk's first tet, so the
+// lane exchanges run with every lane), two entries of the chunk's entry
+// list (4 per tet) and local node t's entry end.
+struct PairPrefetch {
   int4 C = make_int4(0, 0, 0, 0);   // kind, first, count, tend_off
   uint32_t ab = 0, cd = 0;
-  uint16_t ent = 0, te = 0;
+  uint16_t e0 = 0, e1 = 0, te = 0;
   __device__ inline void load(int ch, int chunk1, const int4* __restrict__ chk,
                               const int* __restrict__ chk_ent,
                               const uint16_t* __restrict__ tend,
@@ -104,13 +100,20 @@ struct TetPrefetch {
                               const uint32_t* __restrict__ tcd, int t, int nloc) {
     if (ch >= chunk1) return;
     C = chk[ch];
-    const int i = t >> 2;
-    // a valid tet for idle lanes, so the quad exchanges run with every lane
+    const int i = t >> 1;
     const int e = C.y + (i < C.z ? i : 0);
     ab = tab[e];
     cd = tcd[e];
-    ent = t < 4 * C.z ? entries[chk_ent[ch] + t] : 0;
+    const int off = chk_ent[ch], ne = 4 * C.z;
+    e0 = t < ne ? entries[off + t] : 0;
+    e1 = kKozThreads + t < ne ? entries[off + kKozThreads + t] : 0;
     te = t < nloc ? tend[C.w + t] : 0;
+  }
+  // the chunk's entry list and entry ends into LDS
+  __device__ inline void store(int t, int nloc, uint16_t* sE, uint16_t* sT) const {
+    if (t < 4 * C.z) sE[t] = e0;
+    if (kKozThreads + t < 4 * C.z) sE[kKozThreads + t] = e1;
+    if (t < nloc) sT[t] = te;
   }
 };
 
@@ -120,11 +123,12 @@ __device__ inline void cross(const double a[3], const double b[3], double c[3]) 
   c[2] = a[0] * b[1] - b[0] * a[1];
 }
 
-// tet Jacobian and corner a's antidiffusive contribution of KozCG::aec
-// (KozCG.cpp:794-812) in the reference's operation order and without fused
-// multiply-adds: the limiter branches on its sign, so it is bit-identical
-__device__ inline void tet_aec(const double X[4][3], const double* sU, int cap,
-                               const int N[4], int a, double ctau, double out[5]) {
+// The tet's Jacobian and the antidiffusive contributions of its corners
+// 2h and 2h + 1 (KozCG::aec, KozCG.cpp:794-812), in the reference's
+// operation order and without fused multiply-adds: the limiter branches on
+// their signs, so they are bit-identical.
+__device__ inline void pair_aec(const double X[4][3], const double* sU, int cap,
+                                const int N[4], int h, double ctau, double out[2][5]) {
 #pragma clang fp contract(off)
   double ba[3], ca[3], da[3], t[3];
   for (int j = 0; j < 3; ++j) {
@@ -140,19 +144,35 @@ __device__ inline void tet_aec(const double X[4][3], const double* sU, int cap,
   const double mo = j120 * -1.0, md = j120 * 3.0;
 #pragma unroll
   for (int c = 0; c < 5; ++c) {
-    double s = 0.0;
+    double u[4];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) s += (b == a ? md : mo) * ctau * sU[c * cap + N[b]];
-    out[c] = s;
+    for (int b = 0; b < 4; ++b) u[b] = sU[c * cap + N[b]];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int a = 2 * h + k;
+      double s = 0.0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) s += (b == a ? md : mo) * ctau * u[b];
+      out[k][c] = s;
+    }
   }
 }
 
-// AEC: also the antidiffusive contributions P+/- (KozCG::aec), the output
-// record of a node being then R (5), P+ (5), P- (5).  SRC: the problem's
-// source term (Kozak.cpp:124-136, 163-173): dt/8 s(x_a) of every corner in
-// the half-step state, J/24 s(centroid) added to every corner.
-template <bool AEC, bool SRC>
-__global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_kozak_blocks(
+// kozak::rhs (Kozak.cpp:29-180) on the pair layout: chunks of
+// kTetsPerPairChunk tets, thread t handles corners 2h and 2h + 1 (h = t & 1)
+// of chunk tet t >> 1.  Each thread evaluates the tet geometry and its two
+// corners' shares of the half-step state; one lane exchange with its partner
+// completes the state ((h0 + h1) + (h2 + h3) in both lanes), and each thread
+// writes its two corners' second-step fluxes into the chunk's corner slots
+// (4 i + a).  The thread owning local node t then adds the chunk's slots of
+// its node, in slot order; halo partials go to slots that k_fixup adds to
+// their owners.
+// SRC: the problem's source term (Kozak.cpp:124-136, 163-173).
+// AEC: also the antidiffusive contributions of KozCG::aec, written into the
+// same corner slots after the fluxes are summed and summed per node by sign
+// (P+, P-), the output record of a node being R (5), P+ (5), P- (5).
+template <bool SRC, bool AEC>
+__global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_kozak_pairs(
     int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
     const int* __restrict__ halo, const int4* __restrict__ chk,
     const int* __restrict__ chk_ent, const uint16_t* __restrict__ tend,
@@ -162,188 +182,8 @@ __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))
     const double* __restrict__ U, double* __restrict__ R,
     double* __restrict__ S, double gm1, double dt, double ctau, xg::Prob P) {
   constexpr int CAP = xg::kKozCap;
-  constexpr int NO = AEC ? 15 : 5;
-  constexpr int NF = AEC ? 10 : 5;   // per corner slot: flux (+ aec)
-  __shared__ double sU[5 * CAP];
-  __shared__ double sX[3 * CAP];
-  __shared__ double F[kKozThreads * NF];
-  __shared__ uint16_t sE[kKozThreads];
-  __shared__ uint16_t sT[CAP];
-  const int t = threadIdx.x;
-  const int b = xcd_block(blockIdx.x, nblk);
-  const int4 B0 = blk0[b];   // own0, nown, nloc, halo_off
-  const int4 B1 = blk1[b];   // slot0, chunk0, chunk1
-  const int own0 = B0.x, nown = B0.y, nloc = B0.z;
-  if (t < nloc) {
-    int g = t < nown ? own0 + t : halo[B0.w + t - nown];
-    double s[5];
-#pragma unroll
-    for (int c = 0; c < 5; ++c) s[c] = U[(size_t)g * 5 + c];
-    double xg_ = x[g], yg = y[g], zg = z[g];
-#pragma unroll
-    for (int c = 0; c < 5; ++c) sU[c * CAP + t] = s[c];
-    sX[0 * CAP + t] = xg_;
-    sX[1 * CAP + t] = yg;
-    sX[2 * CAP + t] = zg;
-  }
-  const int i = t >> 2, a = t & 3;
-  double acc[NO];
-#pragma unroll
-  for (int c = 0; c < NO; ++c) acc[c] = 0.0;
-  TetPrefetch pf;
-  pf.load(B1.y, B1.z, chk, chk_ent, tend, entries, tab, tcd, t, nloc);
-  for (int ch = B1.y; ch < B1.z; ++ch) {
-    const int4 C = pf.C;
-    const uint32_t ab = pf.ab, cd = pf.cd;
-    __syncthreads();          // staging done / previous chunk's F consumed
-    if (t < 4 * C.z) sE[t] = pf.ent;
-    if (t < nloc) sT[t] = pf.te;
-    pf.load(ch + 1, B1.z, chk, chk_ent, tend, entries, tab, tcd, t, nloc);
-    double aec[5];
-    {
-      const int N[4] = {(int)(ab & 0xffff), (int)(ab >> 16), (int)(cd & 0xffff),
-                        (int)(cd >> 16)};
-      double X[4][3];
-#pragma unroll
-      for (int k = 0; k < 4; ++k)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) X[k][j] = sX[j * CAP + N[k]];
-      double ba[3], ca[3], da[3];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        ba[j] = X[1][j] - X[0][j];
-        ca[j] = X[2][j] - X[0][j];
-        da[j] = X[3][j] - X[0][j];
-      }
-      double g1[3], g2[3], g3[3], ga[3];
-      cross(ca, da, g1);
-      cross(da, ba, g2);
-      cross(ba, ca, g3);
-      const double J = ba[0] * g1[0] + ba[1] * g1[1] + ba[2] * g1[2];
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-        ga[j] = a == 0 ? -g1[j] - g2[j] - g3[j] : a == 1 ? g1[j] : a == 2 ? g2[j] : g3[j];
-      const int na = N[a];
-      double u[5];
-#pragma unroll
-      for (int c = 0; c < 5; ++c) u[c] = sU[c * CAP + na];
-      // this corner's share of the half-step state (Kozak.cpp:92-124)
-      const double ir = xg::rcp(u[0]);
-      const double p = (u[4] - 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) * ir) * gm1;
-      const double coef = dt * xg::rcp(J) * 0.5;
-      double h[5];
-#pragma unroll
-      for (int c = 0; c < 5; ++c) h[c] = 0.25 * u[c];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        const double cg = coef * ga[j];
-        const double uj = u[j + 1] * ir;
-        h[0] -= cg * u[j + 1];
-        h[1] -= cg * u[1] * uj;
-        h[2] -= cg * u[2] * uj;
-        h[3] -= cg * u[3] * uj;
-        h[j + 1] -= cg * p;
-        h[4] -= cg * (u[4] + p) * uj;
-      }
-      if constexpr (SRC) {
-        double sa[5];
-        xg::source(P, X[a][0], X[a][1], X[a][2], sa);
-        const double c8 = dt / 8.0;
-#pragma unroll
-        for (int c = 0; c < 5; ++c) h[c] += c8 * sa[c];
-      }
-      // sum over the tet's four corners (lanes 4i..4i+3); (h0+h1)+(h2+h3) and
-      // (h2+h3)+(h0+h1) round identically, so all four lanes agree
-      double ue[5];
-#pragma unroll
-      for (int c = 0; c < 5; ++c) {
-        double v = h[c] + quad_xor1(h[c]);
-        ue[c] = v + quad_xor2(v);
-      }
-      // second half step (Kozak.cpp:138-161): corner a's flux
-      const double ie = xg::rcp(ue[0]);
-      const double pr = (ue[4] - 0.5 * (ue[1] * ue[1] + ue[2] * ue[2] + ue[3] * ue[3]) * ie) * gm1;
-      double f[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        const double uj = ue[j + 1] * ie;
-        const double cg = ga[j] * (1.0 / 6.0);
-        f[0] += cg * ue[j + 1];
-        f[1] += cg * ue[1] * uj;
-        f[2] += cg * ue[2] * uj;
-        f[3] += cg * ue[3] * uj;
-        f[j + 1] += cg * pr;
-        f[4] += cg * (ue[4] + pr) * uj;
-      }
-      if constexpr (SRC) {
-        double se[5];
-        xg::source(P, (X[0][0] + X[1][0] + X[2][0] + X[3][0]) / 4.0,
-                   (X[0][1] + X[1][1] + X[2][1] + X[3][1]) / 4.0,
-                   (X[0][2] + X[1][2] + X[2][2] + X[3][2]) / 4.0, se);
-        const double cj = J / 24.0;
-#pragma unroll
-        for (int c = 0; c < 5; ++c) f[c] += cj * se[c];
-      }
-      if constexpr (AEC) tet_aec(X, sU, CAP, N, a, ctau, aec);
-      if (i < C.z) {
-#pragma unroll
-        for (int c = 0; c < 5; ++c) F[t * NF + c] = f[c];
-        if constexpr (AEC) {
-#pragma unroll
-          for (int c = 0; c < 5; ++c) F[t * NF + 5 + c] = aec[c];
-        }
-      }
-    }
-    __syncthreads();          // F, sE, sT complete
-    if (t < nloc) {
-      int e0 = t ? sT[t - 1] : 0, e1 = sT[t];
-      for (int j = e0; j < e1; ++j) {
-        const double* fs = F + (int)sE[j] * NF;
-#pragma unroll
-        for (int c = 0; c < 5; ++c) acc[c] += fs[c];
-        if constexpr (AEC) {
-#pragma unroll
-          for (int c = 0; c < 5; ++c) {
-            const double v = fs[5 + c];
-            acc[5 + c] += v > 0.0 ? v : 0.0;     // std::max(0.0, aec)
-            acc[10 + c] += v < 0.0 ? v : 0.0;    // std::min(0.0, aec)
-          }
-        }
-      }
-    }
-  }
-  if (t < nown) {
-    double* o = R + (size_t)(own0 + t) * NO;
-#pragma unroll
-    for (int c = 0; c < NO; ++c) o[c] = acc[c];
-  } else if (t < nloc) {
-    double* o = S + (size_t)(B1.x + t - nown) * NO;
-#pragma unroll
-    for (int c = 0; c < NO; ++c) o[c] = acc[c];
-  }
-}
-
-// kozak::rhs without the antidiffusive contributions, two threads per tet:
-// thread t handles corners 2h and 2h + 1 (h = t & 1) of chunk tet t >> 1 on
-// chunks of kTetsPerPairChunk tets.  Each thread evaluates the tet geometry
-// and its two corners' shares of the half-step state; one lane exchange with
-// its partner completes the state ((h0 + h1) + (h2 + h3) in both lanes, the
-// sum the four-lane kernel forms), and each thread writes its two corners'
-// second-step fluxes.  Accumulation, halo slots and fix-up as in
-// k_kozak_blocks.  SRC: the problem's source term (Kozak.cpp:124-136,
-// 163-173).
-template <bool SRC>
-__global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_kozak_pairs(
-    int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
-    const int* __restrict__ halo, const int4* __restrict__ chk,
-    const int* __restrict__ chk_ent, const uint16_t* __restrict__ tend,
-    const uint16_t* __restrict__ entries, const uint32_t* __restrict__ tab,
-    const uint32_t* __restrict__ tcd, const double* __restrict__ x,
-    const double* __restrict__ y, const double* __restrict__ z,
-    const double* __restrict__ U, double* __restrict__ R,
-    double* __restrict__ S, double gm1, double dt, xg::Prob P) {
-  constexpr int CAP = xg::kKozCap;
   constexpr int NS = xg::kKozPairSlots;
+  constexpr int NO = AEC ? 15 : 5;
   static_assert(NS == 2 * kKozThreads, "two corner slots per thread");
   __shared__ double sU[5 * CAP];
   __shared__ double sX[3 * CAP];
@@ -368,31 +208,18 @@ __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))
     sX[2 * CAP + t] = zg;
   }
   const int i = t >> 1, h = t & 1;
-  // the next chunk's tet record and entry lists (two entries per thread)
-  int4 Cn = make_int4(0, 0, 0, 0);
-  uint32_t abn = 0, cdn = 0;
-  uint16_t en0 = 0, en1 = 0, ten = 0;
-  auto fetch = [&](int ch) {
-    if (ch >= B1.z) return;
-    Cn = chk[ch];
-    const int e = Cn.y + (i < Cn.z ? i : 0);   // a valid tet for idle lanes
-    abn = tab[e];
-    cdn = tcd[e];
-    const int off = chk_ent[ch], ne = 4 * Cn.z;
-    en0 = t < ne ? entries[off + t] : 0;
-    en1 = kKozThreads + t < ne ? entries[off + kKozThreads + t] : 0;
-    ten = t < nloc ? tend[Cn.w + t] : 0;
-  };
-  fetch(B1.y);
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  PairPrefetch pf;
+  pf.load(B1.y, B1.z, chk, chk_ent, tend, entries, tab, tcd, t, nloc);
+  double acc[NO];
+#pragma unroll
+  for (int c = 0; c < NO; ++c) acc[c] = 0.0;
   for (int ch = B1.y; ch < B1.z; ++ch) {
-    const int4 C = Cn;
-    const uint32_t ab = abn, cd = cdn;
+    const int4 C = pf.C;
+    const uint32_t ab = pf.ab, cd = pf.cd;
     __syncthreads();          // staging done / previous chunk's F consumed
-    if (t < 4 * C.z) sE[t] = en0;
-    if (kKozThreads + t < 4 * C.z) sE[kKozThreads + t] = en1;
-    if (t < nloc) sT[t] = ten;
-    fetch(ch + 1);
+    pf.store(t, nloc, sE, sT);
+    pf.load(ch + 1, B1.z, chk, chk_ent, tend, entries, tab, tcd, t, nloc);
+    double aec[2][5];
     {
       const int N[4] = {(int)(ab & 0xffff), (int)(ab >> 16), (int)(cd & 0xffff),
                         (int)(cd >> 16)};
@@ -499,6 +326,7 @@ __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))
           }
         }
       }
+      if constexpr (AEC) pair_aec(X, sU, CAP, N, h, ctau, aec);
     }
     __syncthreads();          // F, sE, sT complete
     if (t < nloc) {
@@ -509,15 +337,37 @@ __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))
         for (int c = 0; c < 5; ++c) acc[c] += fs[c];
       }
     }
+    if constexpr (AEC) {
+      __syncthreads();        // the fluxes are summed: the slots take the aec
+      if (i < C.z) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+          for (int c = 0; c < 5; ++c) F[(4 * i + 2 * h + k) * 5 + c] = aec[k][c];
+      }
+      __syncthreads();
+      if (t < nloc) {
+        int e0 = t ? sT[t - 1] : 0, e1 = sT[t];
+        for (int j = e0; j < e1; ++j) {
+          const double* fs = F + (int)sE[j] * 5;
+#pragma unroll
+          for (int c = 0; c < 5; ++c) {
+            const double v = fs[c];
+            acc[5 + c] += v > 0.0 ? v : 0.0;     // std::max(0.0, aec)
+            acc[10 + c] += v < 0.0 ? v : 0.0;    // std::min(0.0, aec)
+          }
+        }
+      }
+    }
   }
   if (t < nown) {
-    double* o = R + (size_t)(own0 + t) * 5;
+    double* o = R + (size_t)(own0 + t) * NO;
 #pragma unroll
-    for (int c = 0; c < 5; ++c) o[c] = acc[c];
+    for (int c = 0; c < NO; ++c) o[c] = acc[c];
   } else if (t < nloc) {
-    double* o = S + (size_t)(B1.x + t - nown) * 5;
+    double* o = S + (size_t)(B1.x + t - nown) * NO;
 #pragma unroll
-    for (int c = 0; c < 5; ++c) o[c] = acc[c];
+    for (int c = 0; c < NO; ++c) o[c] = acc[c];
   }
 }
 
@@ -553,41 +403,38 @@ __device__ inline int local_to_global(int t, int own0, int nown,
 // corners of the low-order solution (with CLIP = false also of U), taken
 // into every corner's running max/min.  Each local node's own bound pair
 // (max and min of its low-order solution and, without clipping, its U) is
-// staged in LDS once; per chunk the four corners of a tet exchange theirs in
-// the quad, and the corner merges the tet's bounds into its node's running
-// max/min with LDS ds_max_f64/ds_min_f64 -- max and min are exact and
-// independent of order, so the result is deterministic.  Halo partials go
-// to slots that k_fixup_minmax merges.  Owners also store the low-order
-// solution UL.  LDS: 80 KiB, two workgroups per CU.
+// staged in LDS once.  Per chunk of the pair layout, thread t takes the
+// max/min over its two corners, one lane exchange with its partner gives the
+// tet's, and the pair writes the tet's bounds (10 doubles) to LDS; then the
+// thread owning local node t merges the bounds of the chunk's tets that
+// touch it (its entry list) into its running max/min in registers -- max
+// and min are exact and independent of order, so the result is
+// deterministic.  Halo partials go to slots that k_fixup_minmax merges.
+// Owners also store the low-order solution UL.
 template <bool CLIP>
 __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_kozcg_alw(
     int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
     const int* __restrict__ halo, const int4* __restrict__ chk,
-    const uint32_t* __restrict__ tab, const uint32_t* __restrict__ tcd,
-    const double* __restrict__ W, const double* __restrict__ U,
-    const double* __restrict__ vol, double dt, double* __restrict__ UL,
-    double* __restrict__ Q, double* __restrict__ S) {
+    const int* __restrict__ chk_ent, const uint16_t* __restrict__ tend,
+    const uint16_t* __restrict__ entries, const uint32_t* __restrict__ tab,
+    const uint32_t* __restrict__ tcd, const double* __restrict__ W,
+    const double* __restrict__ U, const double* __restrict__ vol, double dt,
+    double* __restrict__ UL, double* __restrict__ Q, double* __restrict__ S) {
   constexpr int CAP = xg::kKozCap;
+  constexpr int NT = xg::kTetsPerPairChunk;
   constexpr double kLarge = 1.7976931348623157e308;
   __shared__ double sB[10 * CAP];    // each node's own max (5) and min (5)
-  __shared__ double sQ[10 * CAP];    // running max (5) and min (5)
+  __shared__ double sQ[NT * 10];     // the chunk's tets' max (5) and min (5)
+  __shared__ uint16_t sE[xg::kKozPairSlots];
+  __shared__ uint16_t sT[CAP];
   const int t = threadIdx.x;
   const int b = xcd_block(blockIdx.x, nblk);
   const int4 B0 = blk0[b];
   const int4 B1 = blk1[b];
   const int own0 = B0.x, nown = B0.y, nloc = B0.z;
-  const int i = t >> 2, a = t & 3;
-  // the first chunk's tet, loaded while the block stages
-  int4 Cn = make_int4(0, 0, 0, 0);
-  uint32_t wn = 0;
-  auto fetch = [&](int ch) {
-    if (ch < B1.z) {
-      Cn = chk[ch];
-      const int e = Cn.y + (i < Cn.z ? i : 0);
-      wn = (a < 2) ? tab[e] : tcd[e];
-    }
-  };
-  fetch(B1.y);
+  const int i = t >> 1, h = t & 1;
+  PairPrefetch pf;
+  pf.load(B1.y, B1.z, chk, chk_ent, tend, entries, tab, tcd, t, nloc);
   if (t < nloc) {
     const int g = local_to_global(t, own0, nown, halo, B0.w);
     double ul[5];
@@ -602,38 +449,51 @@ __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))
       }
       sB[c * CAP + t] = hi;
       sB[(5 + c) * CAP + t] = lo;
-      sQ[c * CAP + t] = -kLarge;
-      sQ[(5 + c) * CAP + t] = kLarge;
       if (t < nown) UL[(size_t)g * 5 + c] = ul[c];
     }
   }
-  __syncthreads();
+  double qh[5], ql[5];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    qh[c] = -kLarge;
+    ql[c] = kLarge;
+  }
   for (int ch = B1.y; ch < B1.z; ++ch) {
-    const int4 C = Cn;
-    const uint32_t w = wn;
-    fetch(ch + 1);
-    const int na = (a & 1) ? (int)(w >> 16) : (int)(w & 0xffff);
+    const int4 C = pf.C;
+    const uint32_t w = h ? pf.cd : pf.ab;
+    __syncthreads();          // staging done / previous chunk's sQ consumed
+    pf.store(t, nloc, sE, sT);
+    pf.load(ch + 1, B1.z, chk, chk_ent, tend, entries, tab, tcd, t, nloc);
+    const int nA = (int)(w & 0xffff), nB = (int)(w >> 16);
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
-      double hi = sB[c * CAP + na], lo = sB[(5 + c) * CAP + na];
+      double hi = fmax(sB[c * CAP + nA], sB[c * CAP + nB]);
+      double lo = fmin(sB[(5 + c) * CAP + nA], sB[(5 + c) * CAP + nB]);
       hi = fmax(hi, quad_xor1(hi));
-      hi = fmax(hi, quad_xor2(hi));
       lo = fmin(lo, quad_xor1(lo));
-      lo = fmin(lo, quad_xor2(lo));
-      if (i < C.z) {
-        __hip_atomic_fetch_max(&sQ[c * CAP + na], hi, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_min(&sQ[(5 + c) * CAP + na], lo, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (i < C.z) sQ[i * 10 + 5 * h + c] = h ? lo : hi;
+    }
+    __syncthreads();          // sQ, sE, sT complete
+    if (t < nloc) {
+      int e0 = t ? sT[t - 1] : 0, e1 = sT[t];
+      for (int j = e0; j < e1; ++j) {
+        const double* q = sQ + ((int)sE[j] >> 2) * 10;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+          qh[c] = fmax(qh[c], q[c]);
+          ql[c] = fmin(ql[c], q[5 + c]);
+        }
       }
     }
   }
-  __syncthreads();
   if (t < nloc) {
     double* o = t < nown ? Q + (size_t)(own0 + t) * 10
                          : S + (size_t)(B1.x + t - nown) * 10;
 #pragma unroll
-    for (int c = 0; c < 10; ++c) o[c] = sQ[c * CAP + t];
+    for (int c = 0; c < 5; ++c) {
+      o[c] = qh[c];
+      o[5 + c] = ql[c];
+    }
   }
 }
 
@@ -683,7 +543,10 @@ __global__ __launch_bounds__(kBlock) void k_kozcg_coef(
 // limited antidiffusive contributions (KozCG.cpp:1049-1078): per tet the
 // coefficient of each component is the min over the corners of C+ or C-
 // (by the sign of the corner's contribution), the fctsys components share
-// their min; the corner adds coef * aec.  Same block/slot scheme as the rhs.
+// their min; the corner adds coef * aec.  Pair layout as kozak::rhs: thread
+// t recomputes the antidiffusive contributions of its two corners, takes
+// the min over them and, with one lane exchange, over its partner's; the
+// corner slots are summed per node as in the rhs.
 __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_kozcg_lim(
     int nblk, const int4* __restrict__ blk0, const int4* __restrict__ blk1,
     const int* __restrict__ halo, const int4* __restrict__ chk,
@@ -694,11 +557,12 @@ __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))
     const double* __restrict__ U, const double* __restrict__ Cf,
     double* __restrict__ A, double* __restrict__ S, double ctau, int sysmask) {
   constexpr int CAP = xg::kKozCap;
+  constexpr int NS = xg::kKozPairSlots;
   __shared__ double sU[5 * CAP];
   __shared__ double sX[3 * CAP];
-  __shared__ double F[kKozThreads * 5];
+  __shared__ double F[NS * 5];
   __shared__ int sG[CAP];
-  __shared__ uint16_t sE[kKozThreads];
+  __shared__ uint16_t sE[NS];
   __shared__ uint16_t sT[CAP];
   const int t = threadIdx.x;
   const int b = xcd_block(blockIdx.x, nblk);
@@ -714,33 +578,32 @@ __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))
     sX[1 * CAP + t] = y[g];
     sX[2 * CAP + t] = z[g];
   }
-  const int i = t >> 2, a = t & 3;
+  const int i = t >> 1, h = t & 1;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  TetPrefetch pf;
+  PairPrefetch pf;
   pf.load(B1.y, B1.z, chk, chk_ent, tend, entries, tab, tcd, t, nloc);
   __syncthreads();            // staging done
-  // the corner node's limit coefficients C+ (5) and C- (5), loaded one
-  // chunk ahead like the tets
-  double cfn[10];
-  auto fetch_cf = [&](const TetPrefetch& P) {
-    const uint32_t w = (a < 2) ? P.ab : P.cd;
-    const int na = (a & 1) ? (int)(w >> 16) : (int)(w & 0xffff);
-    const double* cf = Cf + (size_t)sG[na] * 10;
+  // the two corner nodes' limit coefficients C+ (5) and C- (5): the next
+  // chunk's are loaded as soon as the current chunk's coefficients are
+  // formed, so the loads overlap the slot writes and the accumulation
+  double cfn[2][10];
+  auto fetch_cf = [&](const PairPrefetch& Pf) {
+    const uint32_t w = h ? Pf.cd : Pf.ab;
 #pragma unroll
-    for (int c = 0; c < 10; ++c) cfn[c] = cf[c];
+    for (int k = 0; k < 2; ++k) {
+      const int na = k ? (int)(w >> 16) : (int)(w & 0xffff);
+      const double* cf = Cf + (size_t)sG[na] * 10;
+#pragma unroll
+      for (int c = 0; c < 10; ++c) cfn[k][c] = cf[c];
+    }
   };
   if (B1.y < B1.z) fetch_cf(pf);
   for (int ch = B1.y; ch < B1.z; ++ch) {
     const int4 C = pf.C;
     const uint32_t ab = pf.ab, cd = pf.cd;
-    double cf[10];
-#pragma unroll
-    for (int c = 0; c < 10; ++c) cf[c] = cfn[c];
     __syncthreads();          // previous chunk's F consumed
-    if (t < 4 * C.z) sE[t] = pf.ent;
-    if (t < nloc) sT[t] = pf.te;
+    pf.store(t, nloc, sE, sT);
     pf.load(ch + 1, B1.z, chk, chk_ent, tend, entries, tab, tcd, t, nloc);
-    if (ch + 1 < B1.z) fetch_cf(pf);
     {
       const int N[4] = {(int)(ab & 0xffff), (int)(ab >> 16), (int)(cd & 0xffff),
                         (int)(cd >> 16)};
@@ -749,16 +612,18 @@ __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))
       for (int k = 0; k < 4; ++k)
 #pragma unroll
         for (int j = 0; j < 3; ++j) X[k][j] = sX[j * CAP + N[k]];
-      double aec[5], coef[5];
-      tet_aec(X, sU, CAP, N, a, ctau, aec);
+      double aec[2][5], coef[5];
+      pair_aec(X, sU, CAP, N, h, ctau, aec);
 #pragma unroll
       for (int c = 0; c < 5; ++c) {
-        double q = aec[c] > 0.0 ? cf[c] : cf[5 + c];
+        const double qa = aec[0][c] > 0.0 ? cfn[0][c] : cfn[0][5 + c];
+        const double qb = aec[1][c] > 0.0 ? cfn[1][c] : cfn[1][5 + c];
         // min over the four corners, starting from 1 (exact, order-free)
+        double q = fmin(qa, qb);
         q = fmin(q, quad_xor1(q));
-        q = fmin(q, quad_xor2(q));
         coef[c] = smin(1.0, q);
       }
+      if (ch + 1 < B1.z) fetch_cf(pf);
       if (sysmask) {
         double cs = 1.0;
 #pragma unroll
@@ -770,7 +635,9 @@ __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))
       }
       if (i < C.z) {
 #pragma unroll
-        for (int c = 0; c < 5; ++c) F[t * 5 + c] = coef[c] * aec[c];
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+          for (int c = 0; c < 5; ++c) F[(4 * i + 2 * h + k) * 5 + c] = coef[c] * aec[k][c];
       }
     }
     __syncthreads();
@@ -838,14 +705,15 @@ xg::Prob prob_of(const xg_ctx* c) {
                   c->cfg.gamma};
 }
 
-// the rhs block kernel with or without the antidiffusive contributions and
-// the problem's source term
+// the pair kernel with or without the antidiffusive contributions (ctau
+// their coefficient) and the problem's source term; out and slots take 15
+// or 5 values per node
 template <bool AEC>
-void launch_kozak_blocks(xg_ctx* c, const double* U, double* out, double* slots,
-                         double dt, double ctau) {
+void launch_kozak_pairs(xg_ctx* c, const double* U, double* out, double* slots,
+                        double dt, double ctau) {
   KozakMesh* k = c->koz;
   const xg::Prob P = prob_of(c);
-  auto kern = xg::has_source(P) ? k_kozak_blocks<AEC, true> : k_kozak_blocks<AEC, false>;
+  auto kern = xg::has_source(P) ? k_kozak_pairs<true, AEC> : k_kozak_pairs<false, AEC>;
   hipLaunchKernelGGL(kern, dim3(k->nblk), dim3(kKozThreads), 0, c->stream, k->nblk,
                      k->blk0, k->blk1, k->halo, k->chk, k->chk_ent, k->tend, k->entries,
                      k->tab, k->tcd, k->x, k->y, k->z, U, out, slots,
@@ -854,14 +722,7 @@ void launch_kozak_blocks(xg_ctx* c, const double* U, double* out, double* slots,
 
 int launch_kozak(xg_ctx* c, double dt, const double* U, double* R) {
   KozakMesh* k = c->koz;
-  if (k->nblk) {
-    const xg::Prob P = prob_of(c);
-    auto kern = xg::has_source(P) ? k_kozak_pairs<true> : k_kozak_pairs<false>;
-    hipLaunchKernelGGL(kern, dim3(k->nblk), dim3(kKozThreads), 0, c->stream, k->nblk,
-                       k->blk0, k->blk1p, k->halo, k->chkp, k->chk_entp, k->tendp,
-                       k->entriesp, k->tabp, k->tcdp, k->x, k->y, k->z, U, R, k->slots,
-                       c->cfg.gamma - 1.0, dt, P);
-  }
+  if (k->nblk) launch_kozak_pairs<false>(c, U, R, k->slots, dt, 0.0);
   if (k->nfx)
     hipLaunchKernelGGL(k_fixup<5>, dim3(nblocks(k->nfx)), dim3(kBlock), 0,
                        c->stream, k->nfx, k->fx_node, k->fx_off, k->fx_slot,
@@ -894,16 +755,10 @@ int xg_kozak_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
       (ntet && !inpoel))
     return fail(c, XG_EINVAL, "xg_kozak_upload: bad arguments");
   XG_HIP(c, hipSetDevice(c->device));
-  xg::TetLayout L, L2;
+  xg::TetLayout L;
   int rc = xg::build_tet_layout((int)npoin, ntet, inpoel, xg::kKozNodes, xg::kKozCap,
-                                L, c->err);
+                                L, c->err, xg::kTetsPerPairChunk);
   if (rc) return rc;
-  // the same node blocks, chunks of kTetsPerPairChunk tets
-  rc = xg::build_tet_layout((int)npoin, ntet, inpoel, xg::kKozNodes, xg::kKozCap, L2,
-                            c->err, xg::kTetsPerPairChunk);
-  if (rc) return rc;
-  if (L2.blk.size() != L.blk.size() || L2.nslot != L.nslot)
-    return fail(c, XG_EINVAL, "kozak layout: the two chunkings disagree on the blocks");
   auto* k = new KozakMesh;
   k->npoin = (int)npoin;
   k->ntet = ntet;
@@ -921,22 +776,6 @@ int xg_kozak_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
     ch[i] = make_int4(C.kind, C.first, C.count, C.tend_off);
     ce[i] = C.ent_off;
   }
-  std::vector<int4> b1p(k->nblk), chp(L2.chunks.size());
-  std::vector<int> cep(L2.chunks.size());
-  for (int b = 0; b < k->nblk; ++b) {
-    const auto& B = L.blk[b];
-    const auto& B2 = L2.blk[b];
-    if (B2.own0 != B.own0 || B2.nown != B.nown || B2.nloc != B.nloc || B2.slot0 != B.slot0) {
-      free_kozak(k);
-      return fail(c, XG_EINVAL, "kozak layout: the two chunkings disagree on the blocks");
-    }
-    b1p[b] = make_int4(B2.slot0, B2.chunk0, B2.chunk1, 0);
-  }
-  for (size_t i = 0; i < L2.chunks.size(); ++i) {
-    const auto& C = L2.chunks[i];
-    chp[i] = make_int4(C.kind, C.first, C.count, C.tend_off);
-    cep[i] = C.ent_off;
-  }
   auto& A = k->allocs;
   const size_t n = (size_t)npoin;
   if ((rc = upload(c, A, &k->x, x, n)) || (rc = upload(c, A, &k->y, y, n)) ||
@@ -946,10 +785,6 @@ int xg_kozak_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
       (rc = upload(c, A, &k->tend, L.tend)) ||
       (rc = upload(c, A, &k->entries, L.entries)) ||
       (rc = upload(c, A, &k->tab, L.tab)) || (rc = upload(c, A, &k->tcd, L.tcd)) ||
-      (rc = upload(c, A, &k->blk1p, b1p)) || (rc = upload(c, A, &k->chkp, chp)) ||
-      (rc = upload(c, A, &k->chk_entp, cep)) || (rc = upload(c, A, &k->tendp, L2.tend)) ||
-      (rc = upload(c, A, &k->entriesp, L2.entries)) ||
-      (rc = upload(c, A, &k->tabp, L2.tab)) || (rc = upload(c, A, &k->tcdp, L2.tcd)) ||
       (rc = dev_alloc(c, A, (void**)&k->slots, (size_t)L.nslot * 40)) ||
       (rc = upload(c, A, &k->fx_node, L.fx_node)) ||
       (rc = upload(c, A, &k->fx_off, L.fx_off)) ||
@@ -1107,7 +942,7 @@ int xg_kozcg_step_dev(xg_ctx* c, double t, double dt, double* U, double* Un) {
   const int64_t n5 = (int64_t)n * 5;
   const auto& f = k->fct;
   if (!f.fct) {
-    if (k->nblk) launch_kozak_blocks<false>(c, U, k->W, k->S15, dt, 0.0);
+    if (k->nblk) launch_kozak_pairs<false>(c, U, k->W, k->S15, dt, 0.0);
     if (k->nfx)
       hipLaunchKernelGGL(k_fixup<5>, dim3(nblocks(k->nfx)), dim3(kBlock), 0,
                          c->stream, k->nfx, k->fx_node, k->fx_off, k->fx_slot,
@@ -1118,7 +953,7 @@ int xg_kozcg_step_dev(xg_ctx* c, double t, double dt, double* U, double* Un) {
     int sysmask = 0;
     for (int s = 0; s < f.nsys; ++s) sysmask |= 1 << f.sys[s];
     // rhs and P+/- (KozCG::rhs, aec)
-    if (k->nblk) launch_kozak_blocks<true>(c, U, k->W, k->S15, dt, f.fctdif);
+    if (k->nblk) launch_kozak_pairs<true>(c, U, k->W, k->S15, dt, f.fctdif);
     if (k->nfx)
       hipLaunchKernelGGL(k_fixup<15>, dim3(nblocks(k->nfx)), dim3(kBlock), 0,
                          c->stream, k->nfx, k->fx_node, k->fx_off, k->fx_slot,
@@ -1134,14 +969,11 @@ int xg_kozcg_step_dev(xg_ctx* c, double t, double dt, double* U, double* Un) {
     }
     // low-order solution and allowed bounds (KozCG::alw)
     if (k->nblk) {
-      if (f.fctclip)
-        hipLaunchKernelGGL(k_kozcg_alw<true>, dim3(k->nblk), dim3(kKozThreads), 0,
-                           c->stream, k->nblk, k->blk0, k->blk1, k->halo, k->chk,
-                           k->tab, k->tcd, k->W, U, k->vol, dt, k->UL, k->Q, k->S15);
-      else
-        hipLaunchKernelGGL(k_kozcg_alw<false>, dim3(k->nblk), dim3(kKozThreads), 0,
-                           c->stream, k->nblk, k->blk0, k->blk1, k->halo, k->chk,
-                           k->tab, k->tcd, k->W, U, k->vol, dt, k->UL, k->Q, k->S15);
+      auto alw = f.fctclip ? k_kozcg_alw<true> : k_kozcg_alw<false>;
+      hipLaunchKernelGGL(alw, dim3(k->nblk), dim3(kKozThreads), 0, c->stream, k->nblk,
+                         k->blk0, k->blk1, k->halo, k->chk, k->chk_ent, k->tend,
+                         k->entries, k->tab, k->tcd, k->W, U, k->vol, dt, k->UL, k->Q,
+                         k->S15);
     }
     if (k->nfx)
       hipLaunchKernelGGL(k_fixup_minmax, dim3(nblocks(k->nfx)), dim3(kBlock), 0,

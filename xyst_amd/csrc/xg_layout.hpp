@@ -64,17 +64,13 @@ struct Layout {
   std::vector<int> fx_node, fx_off, fx_slot;
 };
 
-// kozak::rhs blocks: a block stages U (5) and coordinates (3) of up to
-// kKozCap nodes in LDS beside one chunk's corner fluxes and antidiffusive
-// contributions (10 doubles per slot); a chunk holds kKozSlots / 4 tets, one
-// thread per tet corner.  At 512 slots and 512 nodes the largest of the
-// KozCG block kernels takes 75,776 bytes of LDS, so two share a CU.
-constexpr int kKozSlots = 512;
+// KozCG blocks: a block stages U (5) and coordinates (3) of up to kKozCap
+// nodes in LDS beside one chunk's corner values (5 doubles per slot); a
+// chunk holds kKozPairSlots / 4 tets, two threads per tet.  At 1024 slots
+// and 512 nodes the largest of the KozCG block kernels takes 78,848 bytes of
+// LDS, so two share a CU.
 constexpr int kKozCap = 512;
 constexpr int kKozNodes = 384;
-constexpr int kTetsPerChunk = kKozSlots / 4;
-// kozak::rhs without the antidiffusive contributions runs two threads per
-// tet on chunks of twice as many tets (1024 corner slots of 5 doubles)
 constexpr int kKozPairSlots = 1024;
 constexpr int kTetsPerPairChunk = kKozPairSlots / 4;
 
@@ -93,7 +89,7 @@ struct TetLayout {
 int build_tet_layout(int npoin, int64_t ntet, const size_t* inpoel,
                      int block_nodes, int block_cap, TetLayout& L,
                      std::string& err,
-                     int tets_per_chunk = kTetsPerChunk);
+                     int tets_per_chunk = kTetsPerPairChunk);
 
 // chunk_slots: flux slots per edge chunk (<= kChunkSlots; boundary-face
 // chunks hold chunk_slots / 3 faces).
