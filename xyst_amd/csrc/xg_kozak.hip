@@ -352,9 +352,11 @@ __global__ __launch_bounds__(kKozThreads) __attribute__((amdgpu_waves_per_eu(4))
           const double* fs = F + (int)sE[j] * 5;
 #pragma unroll
           for (int c = 0; c < 5; ++c) {
+            // std::max(0.0, aec), std::min(0.0, aec) up to the sign of a
+            // zero, which no later sum or comparison can tell apart
             const double v = fs[c];
-            acc[5 + c] += v > 0.0 ? v : 0.0;     // std::max(0.0, aec)
-            acc[10 + c] += v < 0.0 ? v : 0.0;    // std::min(0.0, aec)
+            acc[5 + c] += fmax(v, 0.0);
+            acc[10 + c] += fmin(v, 0.0);
           }
         }
       }
