@@ -275,8 +275,11 @@ int xg_kozak_upload(xg_ctx* ctx, int64_t npoin, const double* x,
 /* kozak::rhs (Kozak.hpp:19-27, Kozak.cpp:29-180): R(npoin, ncomp) zeroed
  * and filled with the Taylor-Galerkin rhs of U for time step dt at time t,
  * with the source term of the context's problem (Kozak.cpp:124-136,
- * 163-173).  tp, dtp (steady runs) must be null: XG_EUNSUPPORTED otherwise,
- * and for ncomp != 5. */
+ * 163-173).  ncomp = 5 + transported scalars (advected with the flow
+ * velocity, no source), at most XG_MAX_NCOMP.  Steady runs pass dtp
+ * (npoin; device memory for _dev): each tet then uses the mean of its
+ * corners' dtp (Kozak.cpp:105); tp is accepted and unused (the sources
+ * served do not depend on time).  Null dtp: time-accurate with dt. */
 int xg_kozak_rhs(xg_ctx* ctx, int ncomp, double t, double dt, const double* tp,
                  const double* dtp, const double* U, double* R);
 int xg_kozak_rhs_dev(xg_ctx* ctx, int ncomp, double t, double dt,
@@ -294,19 +297,33 @@ typedef struct {
   int    sys[5];      /* fctsys, 0-based component ids limited together */
 } xg_fctcfg;
 
+/* Components per node of the KozCG chare's fields (5 + transported
+ * scalars, at most XG_MAX_NCOMP; default 5).  Call before xg_kozcg_setup,
+ * whose Dirichlet lists and buffers hold ncomp values per node (a change
+ * drops an earlier setup). */
+int xg_kozcg_set_ncomp(xg_ctx* ctx, int ncomp);
+
 /* Per-mesh data of the KozCG chare on the mesh of xg_kozak_upload: nodal
  * volumes, the symmetry BC lists symbcnodes/symbcnorms (KozCG.cpp:524-540),
  * the Dirichlet nodes with a component mask and the values physics::dirbc
- * sets (KozCG.cpp:120-143, BC.cpp:30-69; values fixed in time, ndir*5 each),
- * and the FCT settings. */
+ * sets (KozCG.cpp:120-143, BC.cpp:30-69; values fixed in time, ndir*ncomp
+ * each), and the FCT settings (fctsys ids below ncomp). */
 int xg_kozcg_setup(xg_ctx* ctx, const double* vol, int64_t nsym,
                    const size_t* symnodes, const double* symnorms, int64_t ndir,
                    const size_t* dirnodes, const int* dirmask,
                    const double* dirvals, const xg_fctcfg* fct);
 
-/* KozCG::dt (KozCG.cpp:614-700), unsteady branch without freezeflow:
- * *dt = cfl * min_p cbrt(vol_p)/max(|u|+c, 1e-8).  U on the device. */
+/* KozCG::dt (KozCG.cpp:614-700), unsteady branch:
+ * *dt = cfl * min_p cbrt(vol_p)/max(|u|+c, 1e-8).  U on the device.  The
+ * chare multiplies it by freezeflow once its time exceeds freezetime
+ * (KozCG.cpp:669-674). */
 int xg_kozcg_dt_dev(xg_ctx* ctx, const double* U, double cfl, double* dt);
+
+/* KozCG::dt, steady branch (local time stepping, KozCG.cpp:636-649):
+ * dtp[p] = cfl cbrt(vol_p)/max(|u|+c, 1e-8) (device, npoin) and *dt its
+ * minimum. */
+int xg_kozcg_dt_steady_dev(xg_ctx* ctx, const double* U, double cfl, double* dtp,
+                           double* dt);
 
 /* One KozCG time step (KozCG::rhs/aec/alw/lim/solve, KozCG.cpp:709-1197):
  * the Taylor-Galerkin rhs, with FCT the antidiffusive contributions (with
@@ -314,9 +331,19 @@ int xg_kozcg_dt_dev(xg_ctx* ctx, const double* U, double cfl, double* dt);
  * coefficients and limited contributions, the new solution; without FCT
  * U + dt R/vol.  Then Dirichlet and symmetry BCs (KozCG::BC,
  * KozCG.cpp:590-611).  t is the time at the start of the step (the source
- * term's).  U (npoin*5, device) is advanced in place; Un (npoin*5, device)
- * receives the old solution. */
+ * term's).  U (npoin*ncomp, device) is advanced in place; Un
+ * (npoin*ncomp, device) receives the old solution. */
 int xg_kozcg_step_dev(xg_ctx* ctx, double t, double dt, double* U, double* Un);
+
+/* xg_kozcg_step_dev with local time stepping and freezeflow: dtp, tp
+ * (device, npoin; both null for time-accurate runs) make kozak::rhs use
+ * each tet's mean dtp, the low-order solution (or, without FCT, the update)
+ * each node's own dtp (KozCG.cpp:889-890, 1155), and advance tp += dtp after
+ * the step (:1189-1193); freeze (the chare's freezeflow > 1, KozCG.cpp:1141)
+ * gives the flow components back their values of the step's start after
+ * the BCs (:1168-1175), so only the scalars advance. */
+int xg_kozcg_step2_dev(xg_ctx* ctx, double t, double dt, const double* dtp,
+                       double* tp, int freeze, double* U, double* Un);
 
 /* ---- chorin:: drop-ins (ChoCG) ------------------------------------------------ */
 

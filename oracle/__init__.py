@@ -98,8 +98,9 @@ def lib():
         L.orc_chorin_rhs.restype = C.c_int
         L.orc_kozcg_step.argtypes = [cfgp, C.POINTER(FctCfg), C.c_void_p, D, I64,
                                      C.c_int, _f64p, _f64p,
-                                     _f64p, I64, _i64p, _f64p, D, I64, _i64p, _f64p,
-                                     I64, _i64p, _f64p, _f64p, _f64p]
+                                     _f64p, I64, _i64p, _f64p, D, C.c_void_p, C.c_void_p,
+                                     C.c_int, I64, _i64p, _f64p,
+                                     I64, _i64p, C.c_void_p, _f64p, _f64p, _f64p]
         L.orc_kozcg_step.restype = C.c_int
         probp = C.POINTER(Problem)
         L.orc_problem_ic.argtypes = [probp, D, D, D, D, _f64p]
@@ -646,9 +647,11 @@ def fctcfg(fct=True, fctdif=1.0, fctclip=False, fctsys=()):
 
 
 def kozcg_step(c, f, coord, inpoel, vol, dt, symnodes, symnorms, dirnodes, dirvals, U,
-               P=None, t=0.0):
+               P=None, t=0.0, dtp=None, tp=None, freeze=False, dirmask=None):
     """One KozCG step (with the source of problem P at time t); returns
-    (U_new, U_old)."""
+    (U_new, U_old).  dtp, tp (npoin): local time stepping, tp advanced in
+    place; freeze: the flow components keep their values (freezeflow);
+    dirmask (ndir, ncomp): the Dirichlet components (default all)."""
     x, y, z = (_c(coord[i], np.float64) for i in range(3))
     U = _c(U, np.float64).copy()
     Un = np.empty_like(U)
@@ -656,11 +659,18 @@ def kozcg_step(c, f, coord, inpoel, vol, dt, symnodes, symnorms, dirnodes, dirva
     sv = _c(symnorms, np.float64) if len(symnorms) else np.zeros(3)
     dn = _c(dirnodes, np.int64) if len(dirnodes) else np.zeros(1, np.int64)
     dv = _c(dirvals, np.float64) if len(dirvals) else np.zeros(U.shape[1])
+    if dtp is not None:
+        assert tp is not None and tp.dtype == np.float64 and tp.flags.c_contiguous
+        dtp = _c(dtp, np.float64)
+    dm = _c(dirmask, np.int32).reshape(-1) if dirmask is not None else None
     rc = lib().orc_kozcg_step(C.byref(c), C.byref(f),
                               C.cast(C.pointer(P), C.c_void_p) if P is not None else None,
                               t, U.shape[0], U.shape[1], x, y, z,
                               inpoel.shape[0], _c(inpoel, np.int64), _c(vol, np.float64),
-                              dt, len(symnodes), sn, sv, len(dirnodes), dn, dv, U, Un)
+                              dt, dtp.ctypes.data if dtp is not None else None,
+                              tp.ctypes.data if dtp is not None else None, int(bool(freeze)),
+                              len(symnodes), sn, sv, len(dirnodes), dn,
+                              dm.ctypes.data if dm is not None else None, dv, U, Un)
     if rc:
         raise ValueError("unsupported number of components")
     return U, Un

@@ -253,16 +253,17 @@ int orc_kozcg_step(const orc_cfg* cfg, const orc_fctcfg* f,
                    const orc_problem* P, double t, int64_t npoin,
                    int ncomp, const double* x, const double* y,
                    const double* z, int64_t ntet, const int64_t* inpoel,
-                   const double* vol, double dt, int64_t nsym,
+                   const double* vol, double dt, const double* dtp, double* tp,
+                   int freeze, int64_t nsym,
                    const int64_t* symnodes, const double* symnorms,
-                   int64_t ndir, const int64_t* dirnodes,
+                   int64_t ndir, const int64_t* dirnodes, const int32_t* dirmask,
                    const double* dirvals, double* U, double* Un) {
   if (ncomp < 5 || ncomp > 16) return -1;
   const size_t nn = (size_t)npoin * (size_t)ncomp;
   double* R = (double*)malloc(sizeof(double) * nn);
   double* A = (double*)calloc(nn, sizeof(double));
-  orc_kozak_rhs_src(cfg, P, npoin, ncomp, x, y, z, ntet, inpoel, t, dt, NULL,
-                    NULL, U, R);
+  orc_kozak_rhs_src(cfg, P, npoin, ncomp, x, y, z, ntet, inpoel, t, dt, tp,
+                    dtp, U, R);
   if (f->fct) {
     const double ctau = f->fctdif;
     double* P = (double*)calloc(2*nn, sizeof(double));
@@ -288,14 +289,17 @@ int orc_kozcg_step(const orc_cfg* cfg, const orc_fctcfg* f,
       p[4] -= rvnp * ny;  p[5] -= rvnn * ny;
       p[6] -= rvnp * nz;  p[7] -= rvnn * nz;
     }
-    /* alw: low-order solution (into R) and Q+/- (:866-948) */
-    for (int64_t i = 0; i < npoin; ++i)
+    /* alw: low-order solution (into R) and Q+/- (:866-948); steady: the
+     * node's own time step (:889-890) */
+    for (int64_t i = 0; i < npoin; ++i) {
+      const double dti = dtp ? dtp[i] : dt;
       for (int c = 0; c < ncomp; ++c) {
         double* p = P + i*2*ncomp + 2*c;
         p[0] /= vol[i];
         p[1] /= vol[i];
-        R[i*ncomp+c] = U[i*ncomp+c] + dt*R[i*ncomp+c]/vol[i] - p[0] - p[1];
+        R[i*ncomp+c] = U[i*ncomp+c] + dti*R[i*ncomp+c]/vol[i] - p[0] - p[1];
       }
+    }
     const double large = 1.7976931348623157e308;
     for (size_t k = 0; k < nn; ++k) { Q[2*k] = -large; Q[2*k+1] = large; }
     for (int64_t e = 0; e < ntet; ++e) {
@@ -354,16 +358,27 @@ int orc_kozcg_step(const orc_cfg* cfg, const orc_fctcfg* f,
     free(P);
     free(Q);
   } else {
-    for (int64_t i = 0; i < npoin; ++i)                    /* :1148-1154 */
+    for (int64_t i = 0; i < npoin; ++i) {                  /* :1148-1154 */
+      const double dti = dtp ? dtp[i] : dt;
       for (int c = 0; c < ncomp; ++c)
-        A[i*ncomp+c] = U[i*ncomp+c] + dt*R[i*ncomp+c]/vol[i];
+        A[i*ncomp+c] = U[i*ncomp+c] + dti*R[i*ncomp+c]/vol[i];
+    }
   }
-  /* BC: dirbc then symbc on momentum (KozCG.cpp:590-605) */
+  /* BC: dirbc (components with a nonzero mask; all without masks) then
+   * symbc on momentum (KozCG.cpp:590-605) */
   for (int64_t k = 0; k < ndir; ++k)
-    for (int c = 0; c < ncomp; ++c) A[dirnodes[k]*ncomp+c] = dirvals[k*ncomp+c];
+    for (int c = 0; c < ncomp; ++c)
+      if (!dirmask || dirmask[k*ncomp+c]) A[dirnodes[k]*ncomp+c] = dirvals[k*ncomp+c];
   orc_symbc(npoin, ncomp, A, nsym, symnodes, symnorms);
+  /* freezeflow: the flow components keep their old values (:1168-1175) */
+  if (freeze)
+    for (int64_t i = 0; i < npoin; ++i)
+      for (int c = 0; c < 5; ++c) A[i*ncomp+c] = U[i*ncomp+c];
   memcpy(Un, U, sizeof(double) * nn);
   memcpy(U, A, sizeof(double) * nn);
+  /* local time stepping: tp += dtp (:1189-1193) */
+  if (dtp && tp)
+    for (int64_t i = 0; i < npoin; ++i) tp[i] += dtp[i];
   free(R);
   free(A);
   return 0;

@@ -243,14 +243,20 @@ typedef struct {
  * Then Dirichlet values (all components of the ndir nodes, dirvals
  * ndir*ncomp) and symmetry BCs.  U is advanced in place, Un receives the old
  * U.  P (nullable) adds its source term to kozak::rhs at time t.
+ * Steady (local time stepping, dtp and tp nonnull): kozak::rhs with dtp/tp,
+ * the low-order solution (or U + dt R/vol) with each node's dtp, and tp +=
+ * dtp at the end.  dirmask (nullable: every component) selects the
+ * Dirichlet components (nonzero).  freeze: the flow components keep their
+ * old values after the BCs (freezeflow > 1, KozCG.cpp:1141, 1168-1175).
  * Returns 0, or -1 for ncomp outside 5..16. */
 int orc_kozcg_step(const orc_cfg* cfg, const orc_fctcfg* f,
                    const orc_problem* P, double t, int64_t npoin,
                    int ncomp, const double* x, const double* y,
                    const double* z, int64_t ntet, const int64_t* inpoel,
-                   const double* vol, double dt, int64_t nsym,
+                   const double* vol, double dt, const double* dtp, double* tp,
+                   int freeze, int64_t nsym,
                    const int64_t* symnodes, const double* symnorms,
-                   int64_t ndir, const int64_t* dirnodes,
+                   int64_t ndir, const int64_t* dirnodes, const int32_t* dirmask,
                    const double* dirvals, double* U, double* Un);
 
 /* ---- ChoCG (chocg.c, riecg.c) -------------------------------------------- */

@@ -294,7 +294,8 @@ def slotcyl_fixtures():
     0.9, gamma 5/3, Dirichlet to the initial condition on side set 1 (all
     six components) and side set 2 (the five flow components, c0 free);
     fields (incl. the analytic ones) written after the last step; the
-    Rusanov run also as 8 chares (slot_cyl_pe4_u0.5.std.exo.*)."""
+    Rusanov run also as 8 chares (slot_cyl_pe4_u0.5.std.exo.*); and KozCG/SlotCyl
+    (koz_*: cfl 0.5, FCT defaults, freezeflow 3 from time 0)."""
     rs = os.path.join(REF, "RieCG", "SlotCyl")
     coord, tets, tri, faceset = read_mesh(rs, "unitsquare_01_3.6k.exo")
     out = dict(coord=coord, inpoel=tets, triinpoel=tri, faceset=faceset,
@@ -310,6 +311,12 @@ def slotcyl_fixtures():
     f, _, tf = merge_partitions(rs, coord, "slot_cyl_pe4_u0.5.std.exo", 8)
     out.update(pe8_fields=f, pe8_t_final=tf,
                chare_of_tet=chare_of_tet(rs, coord, tets, "slot_cyl_pe4_u0.5.std.exo", 8))
+    # KozCG/SlotCyl (serial): the same mesh and BCs, cfl 0.5, FCT with the
+    # defaults, freezeflow 3 from freezetime 0
+    ks = os.path.join(REF, "KozCG", "SlotCyl")
+    f, _, tf = read_result(ks, coord, "slot_cyl.std.exo")
+    out.update(koz_fields=f, koz_t_final=tf, koz_diag=np.loadtxt(os.path.join(ks, "diag.std")),
+               koz_cfl=0.5, koz_freezeflow=3.0, koz_freezetime=0.0)
     np.savez_compressed(os.path.join(HERE, "riecg_slotcyl.npz"), **out)
     print("wrote riecg_slotcyl.npz", sorted(out), names)
 

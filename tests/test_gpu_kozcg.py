@@ -87,7 +87,8 @@ def test_kozak_rhs_dev_matches_host_entry():
 
 
 def test_kozak_errors():
-    """No tet mesh uploaded -> XG_ENOMESH; ncomp other than 5 -> unsupported."""
+    """No tet mesh uploaded -> XG_ENOMESH; more than two transported scalars
+    (ncomp > 7) -> unsupported."""
     from xyst_amd._xg import XG_ENOMESH, XG_EUNSUPPORTED, lib
     from xyst_amd import kozak as K
     _say("kozak errors")
@@ -97,8 +98,8 @@ def test_kozak_errors():
                               U.ctypes.data) == XG_ENOMESH
     m = _case(2)
     K.upload(ctx, m["coord"], m["inpoel"])
-    U = np.zeros((m["coord"].shape[1], 6))
-    assert lib().xg_kozak_rhs(ctx._h, 6, 0.0, 1e-3, None, None, U.ctypes.data,
+    U = np.zeros((m["coord"].shape[1], 8))
+    assert lib().xg_kozak_rhs(ctx._h, 8, 0.0, 1e-3, None, None, U.ctypes.data,
                               U.ctypes.data) == XG_EUNSUPPORTED
     ctx.close()
 

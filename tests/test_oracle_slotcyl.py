@@ -123,3 +123,47 @@ def test_slot_cyl_eight_chares():
     # and the serial fields do not pass for it: the fixture discriminates
     with pytest.raises(AssertionError):
         check_fields(d["rusanov_fields"][:, :7], d["pe8_fields"][:, :7], SC_BANDS)
+
+
+def run_kozcg_slotcyl(check=True):
+    """KozCG/SlotCyl: kozak::rhs with the scalar's advection and the
+    momentum source, FCT (reference defaults) over six components, Dirichlet
+    masks, and freezeflow 3 from freezetime 0: KozCG::dt multiplies the time
+    step once the time exceeds freezetime (from the second step on), and
+    while it does the flow components keep their values (KozCG.cpp:669-674,
+    1141, 1168-1175)."""
+    d = slotcyl_fixture()
+    c = slotcyl_case(d)
+    g = float(d["gamma"])
+    P = O.problem("slot_cyl", gamma=g)
+    cfg = O.cfg(g)
+    fct = O.fctcfg(fct=True, fctdif=1.0, fctclip=False, fctsys=())
+    U = O.problem_ic(P, c.coord)
+    dn, dm = slotcyl_dirichlet(c)
+    dirvals = U[dn].copy()
+    V = c.vol.sum()
+    ref = d["koz_diag"]
+    ff, ft = float(d["koz_freezeflow"]), float(d["koz_freezetime"])
+    t = 0.0
+    rows = []
+    for it in range(int(d["nstep"])):
+        dt = O.dt(cfg, U, c.vol, float(d["koz_cfl"]))
+        freeze = t > ft
+        if freeze:
+            dt *= ff
+        U, Un = O.kozcg_step(cfg, fct, c.coord, c.inpoel, c.vol, dt, [], [], dn, dirvals, U,
+                             P=P, t=t, freeze=freeze, dirmask=dm)
+        t += dt
+        An = O.analytic_prim(P, c.coord, t)
+        mine = slotcyl_diag_row(it + 1, t, dt, O.diag(U, Un, c.vol), V,
+                                O.diag_err(U, c.vol, An))
+        rows.append(mine)
+        if check:
+            check_slotcyl_diag(mine, ref[it], tight=True)
+    return U, t, d, np.array(rows)
+
+
+def test_kozcg_slot_cyl():
+    U, t, d, _ = run_kozcg_slotcyl()
+    assert abs(t - float(d["koz_t_final"])) < 1e-8
+    check_fields(slotcyl_fields(U, float(d["gamma"])), d["koz_fields"][:, :7], SC_BANDS)

@@ -7,8 +7,8 @@ CPU: the shim turns every refused call into a tk::Exception -- a null
 context, a host problem with a source term the device was not told about, a
 problem whose source the device lacks, mismatched field sizes.  GPU: through
 the shim, riemann::grad/rhs and kozak::rhs return exactly what the C-ABI
-returns, with and without the vortical-flow source, and kozak::rhs refuses
-steady local time steps.
+returns, with and without the vortical-flow source and (kozak::rhs) with
+steady local time steps; more scalars than the device serves are refused.
 """
 import ctypes as C
 import os
@@ -137,9 +137,14 @@ def test_shim_on_the_device_matches_the_c_abi():
     Rk, e = s.kozak(h, 0.0, 1e-3, U)
     assert e is None
     assert np.array_equal(Rk, K.rhs(ctx, 1e-3, U))
-    # steady local time steps are not evaluated by the device kozak::rhs
-    _, e = s.kozak(h, 0.0, 1e-3, U, tp=np.zeros(U.shape[0]),
-                   dtp=np.full(U.shape[0], 1e-3))
-    assert e is not None and "steady" in e
+    # steady local time steps: each tet takes its corners' mean
+    dtp = 1e-3 * (1.0 + 0.5 * np.cos(5 * coord[1]))
+    Rs, e = s.kozak(h, 0.0, 1e-3, U, tp=np.zeros(U.shape[0]), dtp=dtp)
+    assert e is None
+    assert np.array_equal(Rs, K.rhs(ctx, 1e-3, U, dtp=dtp))
+    assert not np.array_equal(Rs, Rk)
+    # more transported scalars than the device serves: refused
+    _, e = s.kozak(h, 0.0, 1e-3, np.ones((U.shape[0], 8)))
+    assert e is not None and "scalars" in e
     s.set_src(False)
     ctx.close()

@@ -82,6 +82,9 @@ SIG = {
     "xg_diag_pre_dev": (C.c_int, [VP, I64, VP, VP, VP, VP, VP, VP]),
     "xg_kozcg_dt_dev": (C.c_int, [VP, VP, D, VP]),
     "xg_kozcg_step_dev": (C.c_int, [VP, D, D, VP, VP]),
+    "xg_kozcg_set_ncomp": (C.c_int, [VP, C.c_int]),
+    "xg_kozcg_dt_steady_dev": (C.c_int, [VP, VP, D, VP, VP]),
+    "xg_kozcg_step2_dev": (C.c_int, [VP, D, D, VP, VP, C.c_int, VP, VP]),
     "xg_exo_error": (C.c_char_p, [VP]),
     "xg_exo_close": (None, [VP]),
     "xg_exo_open": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(VP)]),

@@ -88,6 +88,43 @@ __device__ inline double wave_min(double v) {
   }
 }
 
+// tp += dtp, the local times of steady runs (RieCG.cpp:1047-1050,
+// KozCG.cpp:1189-1193)
+[[maybe_unused]] __global__ void k_axpy1(int64_t n, const double* __restrict__ d, double* __restrict__ t) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) t[i] += d[i];
+}
+
+// steady dt of the explicit solvers (RieCG.cpp:810-824, KozCG.cpp:636-649):
+// dtp per node and a block minimum; nc components per node
+[[maybe_unused]] __global__ __launch_bounds__(xgd::kBlock) void k_dt_steady(
+    int npoin, int nc, const double* __restrict__ U, const double* __restrict__ vol,
+    double g, double cfl, double* __restrict__ dtp, double* __restrict__ part) {
+  __shared__ double sm[xgd::kBlock / 64];
+  double m = 1.7976931348623157e308;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npoin;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const double* s = U + p * nc;
+    double r = s[0];
+    double u = s[1] / r, v = s[2] / r, w = s[3] / r;
+    double pr = (s[4] - 0.5 * r * (u * u + v * v + w * w)) * (g - 1.0);
+    double c = sqrt(g * fmax(pr, 0.0) / r);
+    double L = cbrt(vol[p]);
+    double vel = sqrt(u * u + v * v + w * w);
+    double d = L / fmax(vel + c, 1.0e-8) * cfl;
+    dtp[p] = d;
+    m = fmin(m, d);
+  }
+  m = wave_min(m);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = sm[0];
+    for (int k = 1; k < xgd::kBlock / 64; ++k) b = fmin(b, sm[k]);
+    part[blockIdx.x] = b;
+  }
+}
+
 // Symmetry BCs (physics::symbc, BC.cpp:110-136): removes the normal part of the
 // 3-vector at V[node * stride + off], one (node, normal) pair after the other
 // in list order; the pairs are grouped by node (off[i]..off[i+1]).

@@ -475,12 +475,6 @@ __global__ void k_update_steady(int npoin, double rk, const double* __restrict__
   U[i] = un - rk * dtp[i / NC] * R[i] / vol[i / NC];
 }
 
-// tp += dtp (RieCG.cpp:1047-1050)
-__global__ void k_axpy1(int64_t n, const double* __restrict__ d, double* __restrict__ t) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) t[i] += d[i];
-}
-
 // F(i, :) /= vol[i] for NC components (RieCG.cpp:935-939)
 template <int NC>
 __global__ void k_div_vol(int npoin, double* __restrict__ F, const double* __restrict__ vol) {
@@ -555,35 +549,6 @@ __global__ void k_prebc(int np, int nc, const int* __restrict__ node,
   for (int k = off[i]; k < off[i + 1]; ++k) {
     s[0] = val[k * 2 + 0];
     s[4] = xg::total_energy(g, s[0], s[1] / s[0], s[2] / s[0], s[3] / s[0], val[k * 2 + 1]);
-  }
-}
-
-// steady RieCG::dt (RieCG.cpp:810-824): dtp per node and a block minimum
-__global__ __launch_bounds__(kBlock) void k_dt_steady(
-    int npoin, int nc, const double* __restrict__ U, const double* __restrict__ vol,
-    double g, double cfl, double* __restrict__ dtp, double* __restrict__ part) {
-  __shared__ double sm[kBlock / 64];
-  double m = 1.7976931348623157e308;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npoin;
-       p += (int64_t)gridDim.x * blockDim.x) {
-    const double* s = U + p * nc;
-    double r = s[0];
-    double u = s[1] / r, v = s[2] / r, w = s[3] / r;
-    double pr = (s[4] - 0.5 * r * (u * u + v * v + w * w)) * (g - 1.0);
-    double c = sqrt(g * fmax(pr, 0.0) / r);
-    double L = cbrt(vol[p]);
-    double vel = sqrt(u * u + v * v + w * w);
-    double d = L / fmax(vel + c, 1.0e-8) * cfl;
-    dtp[p] = d;
-    m = fmin(m, d);
-  }
-  m = wave_min(m);
-  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double b = sm[0];
-    for (int k = 1; k < kBlock / 64; ++k) b = fmin(b, sm[k]);
-    part[blockIdx.x] = b;
   }
 }
 

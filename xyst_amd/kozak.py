@@ -2,10 +2,12 @@
 (KozCG.cpp:614-1197) over the C-ABI.
 
 The mesh arguments (inpoel, coord) are uploaded once per context with
-`upload`; `rhs(ctx, dt, U)` then has the argument meaning of kozak::rhs for
-an unsteady run without a source term (t, tp and dtp unused).  `setup`,
-`dt_dev` and `step_dev` advance a GPU-resident KozCG solution: rhs, FCT
-(aec, alw, lim), solve and the Dirichlet and symmetry BCs.
+`upload`; `rhs(ctx, dt, U)` then has the argument meaning of kozak::rhs
+(ncomp = U's columns: 5 + transported scalars; dtp, the nodes' local time
+steps of a steady run, optional).  `setup`, `dt_dev`, `dt_steady_dev` and
+`step_dev` advance a GPU-resident KozCG solution: rhs, FCT (aec, alw, lim),
+solve and the Dirichlet and symmetry BCs, with local time stepping and
+freezeflow when asked.
 """
 import ctypes as C
 
@@ -13,7 +15,7 @@ import numpy as np
 
 from ._xg import FctConfig, lib, ptr
 
-__all__ = ["upload", "rhs", "rhs_dev", "setup", "dt_dev", "step_dev"]
+__all__ = ["upload", "rhs", "rhs_dev", "setup", "dt_dev", "dt_steady_dev", "step_dev"]
 
 
 def upload(ctx, coord, inpoel):
@@ -22,44 +24,67 @@ def upload(ctx, coord, inpoel):
     ctx._check(lib().xg_kozak_upload(ctx._h, x.size, ptr(x), ptr(y), ptr(z),
                                      inp.size // 4, ptr(inp)))
     ctx.koz_npoin = x.size
+    ctx.koz_ncomp = 5
 
 
-def rhs(ctx, dt, U, t=0.0):
-    """R = kozak::rhs(U) for time step dt; host arrays (npoin, 5)."""
+def _shape(ctx, U):
+    if U.ndim != 2 or U.shape[0] != ctx.koz_npoin:
+        raise ValueError("U must be (%d, ncomp)" % ctx.koz_npoin)
+    return U.shape[1]
+
+
+def rhs(ctx, dt, U, t=0.0, dtp=None):
+    """R = kozak::rhs(U) for time step dt (or the local steps dtp); host
+    arrays (npoin, ncomp)."""
     U = np.ascontiguousarray(U, dtype=np.float64)
-    if U.shape != (ctx.koz_npoin, 5):
-        raise ValueError("U must be (%d, 5)" % ctx.koz_npoin)
+    nc = _shape(ctx, U)
+    d = None
+    if dtp is not None:
+        d = np.ascontiguousarray(dtp, dtype=np.float64)
+        if d.shape != (ctx.koz_npoin,):
+            raise ValueError("dtp must have npoin entries")
     R = np.empty_like(U)
-    ctx._check(lib().xg_kozak_rhs(ctx._h, 5, float(t), float(dt), None, None, ptr(U),
+    ctx._check(lib().xg_kozak_rhs(ctx._h, nc, float(t), float(dt), None, ptr(d), ptr(U),
                                   ptr(R)))
     return R
 
 
-def rhs_dev(ctx, dt, U, R, t=0.0):
-    """Device tensors U, R (npoin, 5) float64; enqueued on the context stream."""
-    for a in (U, R):
-        if tuple(a.shape) != (ctx.koz_npoin, 5) or not a.is_cuda:
-            raise ValueError("device fields must be (%d, 5) on the GPU" % ctx.koz_npoin)
-    ctx._check(lib().xg_kozak_rhs_dev(ctx._h, 5, float(t), float(dt), None, None, ptr(U),
-                                      ptr(R)))
+def _dev(ctx, *fields, nc=None):
+    for a in fields:
+        if a is None:
+            continue
+        if a.ndim != 2 or a.shape[0] != ctx.koz_npoin or not a.is_cuda:
+            raise ValueError("device fields must be (%d, ncomp) on the GPU" % ctx.koz_npoin)
+        if nc is not None and a.shape[1] != nc:
+            raise ValueError("fields have %d components, the chare %d" % (a.shape[1], nc))
+
+
+def rhs_dev(ctx, dt, U, R, t=0.0, dtp=None):
+    """Device tensors U, R (npoin, ncomp) float64, dtp (npoin) optional;
+    enqueued on the context stream."""
+    _dev(ctx, U, R, nc=U.shape[1])
+    ctx._check(lib().xg_kozak_rhs_dev(ctx._h, int(U.shape[1]), float(t), float(dt), None,
+                                      ptr(dtp), ptr(U), ptr(R)))
 
 
 def setup(ctx, vol, symnodes, symnorms, dirnodes=(), dirmask=None, dirvals=None,
-          fct=True, fctdif=1.0, fctclip=False, fctsys=()):
+          fct=True, fctdif=1.0, fctclip=False, fctsys=(), ncomp=5):
     """KozCG per-mesh data: nodal volumes, symmetry (node, normal) pairs,
-    Dirichlet nodes with (ndir, 5) mask and values, FCT settings (fctsys as
-    0-based component ids)."""
+    Dirichlet nodes with (ndir, ncomp) mask and values, FCT settings (fctsys
+    as 0-based component ids); ncomp = 5 + transported scalars."""
+    ctx._check(lib().xg_kozcg_set_ncomp(ctx._h, int(ncomp)))
+    ctx.koz_ncomp = ncomp
     v = np.ascontiguousarray(vol, dtype=np.float64)
     sn = np.ascontiguousarray(symnodes, dtype=np.uint64)
     sv = np.ascontiguousarray(symnorms, dtype=np.float64).reshape(-1)
     dn = np.ascontiguousarray(dirnodes, dtype=np.uint64)
     nd = dn.size
-    dm = np.ascontiguousarray(np.ones((nd, 5)) if dirmask is None else dirmask,
+    dm = np.ascontiguousarray(np.ones((nd, ncomp)) if dirmask is None else dirmask,
                               dtype=np.int32).reshape(-1)
-    dv = np.ascontiguousarray(np.zeros((nd, 5)) if dirvals is None else dirvals,
+    dv = np.ascontiguousarray(np.zeros((nd, ncomp)) if dirvals is None else dirvals,
                               dtype=np.float64).reshape(-1)
-    if v.size != ctx.koz_npoin or sv.size != 3 * sn.size or dm.size != 5 * nd \
-            or dv.size != 5 * nd:
+    if v.size != ctx.koz_npoin or sv.size != 3 * sn.size or dm.size != ncomp * nd \
+            or dv.size != ncomp * nd:
         raise ValueError("setup: inconsistent array sizes")
     if len(fctsys) > 5:
         raise ValueError("fctsys holds at most 5 components")
@@ -71,16 +96,30 @@ def setup(ctx, vol, symnodes, symnorms, dirnodes=(), dirmask=None, dirvals=None,
 
 
 def dt_dev(ctx, U, cfl):
-    """KozCG::dt, unsteady: cfl * min cbrt(vol)/max(|u|+c, 1e-8)."""
+    """KozCG::dt, unsteady: cfl * min cbrt(vol)/max(|u|+c, 1e-8) (the caller
+    applies freezeflow)."""
+    _dev(ctx, U, nc=ctx.koz_ncomp)
     out = C.c_double()
     ctx._check(lib().xg_kozcg_dt_dev(ctx._h, ptr(U), float(cfl), C.byref(out)))
     return out.value
 
 
-def step_dev(ctx, dt, U, Un, t=0.0):
-    """One KozCG step on device tensors (npoin, 5): U advanced in place, Un
-    receives the old solution; enqueued on the context stream."""
-    for a in (U, Un):
-        if tuple(a.shape) != (ctx.koz_npoin, 5) or not a.is_cuda:
-            raise ValueError("device fields must be (%d, 5) on the GPU" % ctx.koz_npoin)
-    ctx._check(lib().xg_kozcg_step_dev(ctx._h, float(t), float(dt), ptr(U), ptr(Un)))
+def dt_steady_dev(ctx, U, cfl, dtp):
+    """KozCG::dt, steady: fills dtp (device, npoin), returns its minimum."""
+    _dev(ctx, U, nc=ctx.koz_ncomp)
+    out = C.c_double()
+    ctx._check(lib().xg_kozcg_dt_steady_dev(ctx._h, ptr(U), float(cfl), ptr(dtp),
+                                            C.byref(out)))
+    return out.value
+
+
+def step_dev(ctx, dt, U, Un, t=0.0, dtp=None, tp=None, freeze=False):
+    """One KozCG step on device tensors (npoin, ncomp): U advanced in place,
+    Un receives the old solution; dtp, tp (device, npoin) for local time
+    stepping (tp advanced), freeze keeps the flow components (freezeflow);
+    enqueued on the context stream."""
+    _dev(ctx, U, Un, nc=ctx.koz_ncomp)
+    if (dtp is None) != (tp is None):
+        raise ValueError("dtp and tp go together")
+    ctx._check(lib().xg_kozcg_step2_dev(ctx._h, float(t), float(dt), ptr(dtp), ptr(tp),
+                                        int(bool(freeze)), ptr(U), ptr(Un)))

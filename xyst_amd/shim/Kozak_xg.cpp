@@ -1,7 +1,7 @@
 // kozak::rhs (Kozak.hpp:19-27) on libxg.so.  The tet mesh was uploaded once
 // per mesh by xg_kozak_upload (KozCG::setup); the body passes the fields, the
 // time, the time step and the steady-state arrays.  The library refuses what
-// it does not evaluate (steady local time steps, transported scalars) with
+// it does not evaluate (more than two transported scalars) with
 // XG_EUNSUPPORTED, which becomes a tk::Exception here.
 #include <string>
 #include <vector>
