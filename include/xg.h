@@ -144,6 +144,16 @@ int xg_mesh_symbc(int64_t npoin, const double* x, const double* y,
                   int64_t cap, int64_t* nout, size_t* nodes, double* norms,
                   uint8_t* besym);
 
+/* The Neumann part of the pressure rhs of ChoCG::pinit (ChoCG.cpp:1070-1095):
+ * neubc (npoin) zeroed, then for every boundary triangle e with besym[e]
+ * (the faces of the pressure block's bc_sym side sets), n = (x1 - x0) x
+ * (x2 - x0) / 6 and neubc[N_k] -= n . pg[N_k] for its three nodes, pg
+ * (npoin*3) the problem's pressure gradient (PRESSURE_GRAD) at the nodes.
+ * tri as in xg_mesh_symbc (outward); face order as given. */
+int xg_mesh_neumann(int64_t npoin, const double* x, const double* y,
+                    const double* z, int64_t ntri, const size_t* tri,
+                    const uint8_t* besym, const double* pg, double* neubc);
+
 /* ---- mesh on the device --------------------------------------------------- */
 
 /* The mesh arguments of riemann::grad/rhs (Riemann.hpp:19-39): coordinates,
@@ -433,6 +443,17 @@ int xg_chocg_setup(xg_ctx* ctx, const double* vol, const xg_chostep* cfg,
  * pit[0..1] receive the two solves' iteration counts. */
 int xg_chocg_initial_dev(xg_ctx* ctx, double* U, double* P, int64_t* pit);
 
+/* ChoCG's run of the first pressure solve alone (nstep = 1 with a problem
+ * that has a pressure rhs, the Poisson tests; ChoCG.cpp:1026-1124,
+ * 1195-1225): BCs on U, b = div(U) -- replaced by pr * vol where pr (npoin,
+ * PRESSURE_RHS at the nodes) is given -- plus the Neumann vector neubc
+ * (npoin, from xg_mesh_neumann, or null), the CG solve with the pressure
+ * Dirichlet nodes and values of xg_chocg_setup (mask 1: PRESSURE_IC at the
+ * node), the projection U -= grad(p) with its BCs, and P = the solution.
+ * pr, neubc, U (npoin*3), P (npoin) on the device; *pit the iterations. */
+int xg_chocg_poisson_dev(xg_ctx* ctx, const double* pr, const double* neubc, double* U,
+                         double* P, int64_t* pit);
+
 /* ChoCG::dt (ChoCG.cpp:1357-1413) without a constant dt: cfl * min over
  * nodes of cbrt(vol)/max(|u|, 1e-8) and cbrt(vol)^2/mu. */
 int xg_chocg_dt_dev(xg_ctx* ctx, const double* U, double* dt);
@@ -488,8 +509,10 @@ int xg_diag_rho_dev_noalloc(xg_ctx* ctx, int64_t npoin, int ncomp, const double*
  * solution in primitive form (rho, u, v, w, specific internal energy, then
  * the scalars; the caller evaluates SOL()), out[c] = sum_i (nu_c - An_c)^2
  * V_i and out[ncomp+c] = sum_i |nu_c - An_c| V_i, nu the same form of U.
- * 5 <= ncomp <= XG_MAX_NCOMP; out (host) 2 ncomp values; deterministic
- * summation order. */
+ * 5 <= ncomp <= XG_MAX_NCOMP, or ncomp = 1: the pressure error of
+ * NodeDiagnostics::precompute (NodeDiagnostics.cpp:186-199, 237-241) with U
+ * the pressure and An PRESSURE_SOL() at the nodes, no conversion; out (host)
+ * 2 ncomp values; deterministic summation order. */
 int xg_diag_err_dev(xg_ctx* ctx, int64_t npoin, int ncomp, const double* U,
                     const double* V, const double* An, double* out);
 

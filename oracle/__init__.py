@@ -89,6 +89,9 @@ def lib():
         L.orc_chorin_grad.argtypes = se + [_f64p, _f64p, _f64p, I64, _i64p, _f64p,
                                            _f64p, C.c_int, C.c_void_p]
         L.orc_chorin_grad.restype = None
+        L.orc_neumann.argtypes = [I64, _f64p, _f64p, _f64p, I64, _i64p, C.c_void_p,
+                                  _f64p, _f64p]
+        L.orc_neumann.restype = None
         L.orc_chorin_flux.argtypes = se + [_f64p, _f64p, _f64p, I64, _i64p, I64,
                                            _f64p, C.c_int, _f64p, C.c_int, D, _f64p,
                                            C.c_void_p]
@@ -317,6 +320,18 @@ def chorin_grad(se, coord, tri, U, absacc=False):
     lib().orc_chorin_grad(*se.args(), *_xyz(coord), tri.shape[0], _tri(tri), U, G, 3,
                           A.ctypes.data if absacc else None)
     return (G, A) if absacc else G
+
+
+def neumann(coord, tri, besym, pg):
+    """ChoCG::pinit's Neumann vector (ChoCG.cpp:1070-1095) from the boundary
+    faces tri (ntri, 3) flagged in besym (ntri,) and the pressure gradient
+    pg (npoin, 3) at the nodes."""
+    x, y, z = _xyz(coord)
+    b = np.ascontiguousarray(besym, dtype=np.uint8)
+    out = np.empty(x.size)
+    lib().orc_neumann(x.size, x, y, z, tri.shape[0], _tri(tri), b.ctypes.data,
+                      _c(pg, np.float64), out)
+    return out
 
 
 def chorin_flux(se, coord, tri, U, G, mu, absacc=False):

@@ -534,3 +534,24 @@ int orc_chorin_rhs(const orc_chocfg* cfg, int64_t npoin, int ncomp,
   }
   return 0;
 }
+
+/* ChoCG::pinit's Neumann vector (ChoCG.cpp:1070-1095): every boundary face
+ * flagged in besym subtracts, at each of its three nodes, the dot product of
+ * its third of the area vector, crossdiv(x1 - x0, x2 - x0, 6), with the
+ * pressure gradient pg (npoin x 3) at that node.  neubc is zeroed first. */
+void orc_neumann(int64_t npoin, const double* x, const double* y, const double* z,
+                 int64_t ntri, const int64_t* tri, const unsigned char* besym,
+                 const double* pg, double* neubc) {
+  for (int64_t i = 0; i < npoin; ++i) neubc[i] = 0.0;
+  for (int64_t e = 0; e < ntri; ++e) {
+    if (!besym[e]) continue;
+    const int64_t* N = tri + 3*e;
+    double n[3];
+    crossdiv(x[N[1]]-x[N[0]], y[N[1]]-y[N[0]], z[N[1]]-z[N[0]],
+             x[N[2]]-x[N[0]], y[N[2]]-y[N[0]], z[N[2]]-z[N[0]], 6.0, n);
+    for (int k = 0; k < 3; ++k) {
+      const double* g = pg + 3*N[k];
+      neubc[N[k]] -= n[0]*g[0] + n[1]*g[1] + n[2]*g[2];
+    }
+  }
+}

@@ -18,6 +18,10 @@ pressure solve (ChoCG.cpp), on one partition, with the C oracle's operators
            psolved (u -= dt grad(dp), BC, p += dp) -> pgrad -> diag
   diag     NodeDiagnostics::precompute (NodeDiagnostics.cpp:148-264):
            sums of p^2 V, u_c^2 V, dp^2 V, (u_c - un_c)^2 V
+  poisson  the run of the first pressure solve alone (nstep = 1, the
+           Poisson problems): b = pr * vol + Neumann vector, Dirichlet
+           values from the problem's pressure, p = solution; its
+           diagnostics add the analytic-pressure error norms
 
 Gradients are finalised as in ChoCG::fingrad (841-866): divided by the nodal
 volume.  BCs (ChoCG::BC, 1341-1355): dirbc (mask 1: initial value, mask 2:
@@ -30,6 +34,38 @@ import numpy as np
 import oracle as O
 
 
+def pressure_problem(name, coord, beta=(0.0, 0.0, 0.0)):
+    """The pressure functions of the Poisson problems at the nodes
+    (Problems.cpp:841-996, PRESSURE_RHS/IC/GRAD at 1172-1264): the rhs pr,
+    the initial/analytic pressure (the Dirichlet value of mask 1 and
+    PRESSURE_SOL) and the gradient of the Neumann BCs (None where the
+    problem has none)."""
+    x, y, z = (np.asarray(coord[i], np.float64) for i in range(3))
+    pi = np.pi
+    pg = None
+    if name == "poisson_const":
+        pr = np.full(x.size, 6.0)
+        ic = x * x + y * y + z * z
+    elif name == "poisson_harmonic":
+        pr = np.zeros(x.size)
+        dx, dy, dz = x - beta[0], y - beta[1], z - beta[2]
+        ic = 1.0 / np.sqrt(dx * dx + dy * dy + dz * dz)
+    elif name == "poisson_sine":
+        pr = -pi * pi * x * y * np.sin(pi * z)
+        ic = x * y * np.sin(pi * z)
+    elif name == "poisson_sine3":
+        pr = -3.0 * pi * pi * np.sin(pi * x) * np.sin(pi * y) * np.sin(pi * z)
+        ic = np.sin(pi * x) * np.sin(pi * y) * np.sin(pi * z)
+    elif name == "poisson_neumann":
+        pr = -3.0 * np.cos(2.0 * x) * np.exp(y)
+        ic = np.cos(2.0 * x) * np.exp(y)
+        pg = np.stack([-2.0 * np.sin(2.0 * x) * np.exp(y), np.cos(2.0 * x) * np.exp(y),
+                       np.zeros(x.size)], 1)
+    else:
+        raise ValueError("unknown Poisson problem %r" % name)
+    return pr, ic, pg
+
+
 class ChoCGRun:
     """One ChoCG partition.  Velocity U (npoin, 3), pressure P (npoin,)."""
 
@@ -37,7 +73,7 @@ class ChoCGRun:
                  stab2=False, stab2coef=0.2, cfl=0.9, rk=1, noslip=(), sym=(),
                  dirbc=None, p_hydrostat=None, p_dirbc=None, p_tol=1e-3, p_iter=500,
                  p_pc="jacobi", ic=(0.0, 0.0, 0.0), theta=0.0, m_tol=1e-3, m_iter=50,
-                 m_pc="jacobi"):
+                 m_pc="jacobi", p_ic=None):
         self.coord = np.ascontiguousarray(coord, dtype=np.float64)
         self.inpoel = np.ascontiguousarray(inpoel, dtype=np.int64)
         self.tri = np.ascontiguousarray(tri, dtype=np.int64)
@@ -71,12 +107,16 @@ class ChoCGRun:
                 m = [m[c] if m[c] else mask[c] for c in range(3)]
                 self.dir[int(p)] = (m, list(val))
         # pressure Dirichlet nodes and the values of the start-up solves
-        # (mask 1: initial value 0, mask 2: the configured value); the
-        # time-step solves set the increment to 0 there
+        # (mask 1: the initial pressure p_ic at the node, 0 without one; mask
+        # 2: the configured value); the time-step solves set the increment to
+        # 0 there
         self.pdir = {}
         for sset, (mask, val) in (p_dirbc or {}).items():
             for p in nodes_of([sset]):
-                self.pdir[int(p)] = 0.0 if mask == 1 else float(val)
+                if mask == 1:
+                    self.pdir[int(p)] = 0.0 if p_ic is None else float(p_ic[p])
+                else:
+                    self.pdir[int(p)] = float(val)
         self.hydrostat = p_hydrostat
         self.ic = np.asarray(ic, np.float64)
         self.p_tol, self.p_iter, self.p_pc = p_tol, p_iter, p_pc
@@ -178,7 +218,7 @@ class ChoCGRun:
         return O.chorin_div(self.se, self.coord, self.tri, self.dt, self.P, self.pgrad,
                             u, stab=stab)
 
-    def _psolve(self, b, initial=False):
+    def _psolve(self, b, initial=False, neubc=None):
         """pinit + psolve: Dirichlet pressure (the configured values in the
         start-up solves, increment 0 once stepping) on the configured sets
         and the hydrostat node, then CG warm-started from the previous
@@ -187,8 +227,9 @@ class ChoCGRun:
         if self.hydrostat is not None and self.hydrostat not in dirbc:
             dirbc[self.hydrostat] = [(1, 0.0)]
         b = b.copy()
-        normb, normr, it, conv = O.cg(self.A, self.x, b, dirbc=dirbc, apply=True,
-                                      pc=self.p_pc, maxit=self.p_iter, tol=self.p_tol)
+        normb, normr, it, conv = O.cg(self.A, self.x, b, neubc=neubc, dirbc=dirbc,
+                                      apply=True, pc=self.p_pc, maxit=self.p_iter,
+                                      tol=self.p_tol)
         self.pit.append(it)
         return self.x.copy()
 
@@ -215,6 +256,33 @@ class ChoCGRun:
         self.np_ = 2
         self.P = sol.copy()
         self.pgrad = self._grad(self.P)
+
+    def poisson(self, pr=None, neubc=None):
+        """The first pressure solve alone, as ChoCG runs it with nstep = 1
+        (ChoCG.cpp:1026-1124, 1195-1225): b = div(u), replaced by pr * vol
+        where the problem has a pressure rhs (pr at the nodes), plus the
+        Neumann vector neubc; the solution projects u (dt = 1) and becomes
+        p.  Returns the solution."""
+        self.BC(self.U)
+        self.div = self._divergence(self.U, stab=False)
+        if pr is not None:
+            self.div = np.asarray(pr, np.float64) * self.vol
+        sol = self._psolve(self.div, initial=True, neubc=neubc)
+        self.U -= self._grad(sol)
+        self.BC(self.U)
+        self.P = sol.copy()
+        return sol
+
+    def pressure_diag(self, dp, ap):
+        """NodeDiagnostics::precompute with an analytic pressure ap
+        (NodeDiagnostics.cpp:186-241, Transporter.cpp:1570-1586): L2(p),
+        L2(dp), L2(p - ap), L1(p - ap) over the total volume."""
+        V = self.vol
+        tot = V.sum()
+        e = self.P - ap
+        return np.array([np.sqrt(np.sum(self.P * self.P * V) / tot),
+                         np.sqrt(np.sum(dp * dp * V) / tot),
+                         np.sqrt(np.sum(e * e * V) / tot), np.sum(np.abs(e) * V) / tot])
 
     def dt_cfl(self):
         """ChoCG::dt (1357-1413): cfl * min(L/max(|u|,1e-8), L^2/mu)."""

@@ -306,6 +306,13 @@ void orc_chorin_grad(const int64_t* nsup, const int64_t* sup0, const double* int
                      const double* int2, const double* x, const double* y,
                      const double* z, int64_t ntri, const int64_t* tri,
                      const double* U, double* G, int ldg, double* A);
+/* ChoCG::pinit's Neumann vector of the pressure rhs (ChoCG.cpp:1070-1095):
+ * neubc (npoin) from the flagged boundary faces and the pressure gradient
+ * pg (npoin x 3) at the nodes. */
+void orc_neumann(int64_t npoin, const double* x, const double* y, const double* z,
+                 int64_t ntri, const int64_t* tri, const unsigned char* besym,
+                 const double* pg, double* neubc);
+
 
 /* chorin::flux, Chorin.cpp:451-638 (accumulates F(npoin, 3)) */
 void orc_chorin_flux(const int64_t* nsup, const int64_t* sup0, const double* int0,

@@ -1,8 +1,9 @@
 """Extract the reference's RieCG/Sod, KozCG/Sod, ChoCG/Lid,
 ChoCG/Poiseuille, analytic-problem (riecg_analytic.npz, see
-analytic_fixtures) and RieCG/SlotCyl regression fixtures into
+analytic_fixtures), RieCG/SlotCyl and ChoCG/Poisson regression fixtures into
 riecg_sod.npz, kozcg_sod.npz, chocg_lid.npz, chocg_poiseuille.npz,
-riecg_analytic.npz and riecg_slotcyl.npz.
+riecg_analytic.npz, riecg_slotcyl.npz and chocg_poisson.npz (run with the
+argument "poisson" to write only the last).
 
 Source files (reference test data, read as data only), per directory
 tests/regression/inciter/{RieCG,KozCG}/Sod:
@@ -165,6 +166,7 @@ def main():
 
     analytic_fixtures()
     slotcyl_fixtures()
+    poisson_fixtures()
 
     # ExodusII files kept as they are, for the file reader's tests: a mesh
     # with a triangle block and side sets on it (RieCG/Sod), a mesh with
@@ -321,5 +323,43 @@ def slotcyl_fixtures():
     print("wrote riecg_slotcyl.npz", sorted(out), names)
 
 
+def poisson_fixtures():
+    """ChoCG/Poisson (one pressure solve, nstep 1, dt 0.5; CG tol 1e-6, 100
+    iterations, no preconditioner unless noted) into chocg_poisson.npz.
+    Run configurations (poisson_*.q): poisson_const, poisson_sine and
+    poisson_sine3 on unitcube_01_1k.exo with pressure Dirichlet (mask 1: the
+    problem's pressure) on side sets 1-6; poisson_neumann on
+    unitcube_0pidiv4_1k.exo with Dirichlet on sets 1, 2, 3 and Neumann
+    (bc_sym) on 4, 5, 6; poisson_sine3 also with the Jacobi preconditioner
+    on 4 PEs, compared with the same diag file.  Fields (pressure, analytic)
+    from the serial run where the reference holds one (poisson_const) and
+    from the partition files otherwise."""
+    ref = os.path.join(REF, "ChoCG", "Poisson")
+    out = {}
+    for key, mesh in (("cube", "unitcube_01_1k.exo"), ("cube4", "unitcube_0pidiv4_1k.exo")):
+        coord, tets, tri, faceset = read_mesh(ref, mesh)
+        out.update({key + "_coord": coord, key + "_inpoel": tets, key + "_triinpoel": tri,
+                    key + "_faceset": faceset})
+    for case, mesh, files, ndiag in (
+            ("const", "cube", ("poisson_const.std.exo", 0), "const"),
+            ("sine", "cube", ("poisson_sine_pe2.std.exo", 2), "sine"),
+            ("sine3", "cube", ("poisson_sine3_pe2.std.exo", 2), "sine3"),
+            ("sine3_jacobi", "cube", ("poisson_sine3_jacobi_pe4.std.exo", 4), "sine3"),
+            ("neumann", "cube4", ("poisson_neumann_pe2.std.exo", 2), "neumann")):
+        coord = out[mesh + "_coord"]
+        if files[1] == 0:
+            fields, names, _ = read_result(ref, coord, files[0])
+        else:
+            fields, names, _ = merge_partitions(ref, coord, files[0], files[1])
+        out[case + "_fields"] = fields
+        out[case + "_names"] = np.array(names)
+        out[case + "_diag"] = np.loadtxt(os.path.join(ref, "diag_poisson_%s.std" % ndiag))
+    out.update(p_tol=1e-6, p_iter=100, dt=0.5)
+    np.savez_compressed(os.path.join(HERE, "chocg_poisson.npz"), **out)
+    print("wrote chocg_poisson.npz", sorted(out))
+
+
 if __name__ == "__main__":
+    if sys.argv[1:] == ["poisson"]:
+        sys.exit(poisson_fixtures())
     sys.exit(main())

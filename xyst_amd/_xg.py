@@ -32,6 +32,7 @@ SIG = {
     "xg_mesh_block_order": (C.c_int, [I64, VP, VP, VP, I64, VP]),
     "xg_mesh_symbc": (C.c_int, [I64, VP, VP, VP, I64, VP, VP, I32, VP, I64,
                                 C.POINTER(I64), VP, VP, VP]),
+    "xg_mesh_neumann": (C.c_int, [I64, VP, VP, VP, I64, VP, VP, VP, VP]),
     "xg_mesh_upload": (C.c_int, [VP, I64, VP, VP, VP, VP, VP, VP, VP, VP, VP,
                                  VP, I64, VP, VP]),
     "xg_riemann_grad": (C.c_int, [VP, C.c_int, VP, VP]),
@@ -71,6 +72,7 @@ SIG = {
     "xg_chocg_setup": (C.c_int, [VP, VP, VP, I64, VP, VP, I64, VP, I64, VP, VP, VP,
                                  I64, VP, VP]),
     "xg_chocg_initial_dev": (C.c_int, [VP, VP, VP, VP]),
+    "xg_chocg_poisson_dev": (C.c_int, [VP, VP, VP, VP, VP, VP]),
     "xg_chocg_dt_dev": (C.c_int, [VP, VP, VP]),
     "xg_chocg_step_dev": (C.c_int, [VP, D, VP, VP, VP, VP, VP]),
     "xg_chocg_momentum_setup": (C.c_int, [VP, I64, VP, D, C.c_int, I64, D]),

@@ -15,7 +15,8 @@ import numpy as np
 from ._xg import ChoConfig, ChoStep, lib, ptr
 
 __all__ = ["mesh_domint", "upload", "div", "grad", "vgrad", "flux", "rhs", "config", "step_setup",
-           "initial_dev", "dt_dev", "step_dev", "momentum_setup", "momentum_its"]
+           "initial_dev", "poisson_dev", "neumann", "dt_dev", "step_dev", "momentum_setup",
+           "momentum_its"]
 
 FLUX = {"damp2": 0, "damp4": 1}
 
@@ -150,6 +151,42 @@ def initial_dev(ctx, U, P):
     it = (C.c_int64 * 2)()
     ctx._check(lib().xg_chocg_initial_dev(ctx._h, ptr(U), ptr(P), it))
     return it[0], it[1]
+
+
+def neumann(coord, tri, besym, pg):
+    """The Neumann vector of ChoCG::pinit (ChoCG.cpp:1070-1095): boundary
+    faces tri (ntri, 3, outward) flagged in besym (ntri,), pressure gradient
+    pg (npoin, 3) at the nodes; host arrays, returns (npoin,)."""
+    x, y, z = (_f64(coord[i]) for i in range(3))
+    t = _idx(tri)
+    b = np.ascontiguousarray(besym, dtype=np.uint8).reshape(-1)
+    g = _f64(pg).reshape(-1)
+    if t.size != 3 * b.size or g.size != 3 * x.size:
+        raise ValueError("neumann: inconsistent array sizes")
+    out = np.empty(x.size)
+    rc = lib().xg_mesh_neumann(x.size, ptr(x), ptr(y), ptr(z), b.size, ptr(t), ptr(b),
+                               ptr(g), ptr(out))
+    if rc:
+        raise ValueError("xg_mesh_neumann failed (%d)" % rc)
+    return out
+
+
+def poisson_dev(ctx, U, P, pr=None, neubc=None):
+    """ChoCG's first pressure solve alone (nstep = 1, the Poisson problems):
+    pr (npoin,) the problem's pressure rhs at the nodes, neubc (npoin,) the
+    Neumann vector, device tensors or None; U (npoin, 3) is projected and P
+    (npoin,) receives the solution.  Returns the CG iterations."""
+    n = ctx.cho_npoin
+    if tuple(U.shape) != (n, 3) or tuple(P.shape) != (n,) or not U.is_cuda:
+        raise ValueError("device fields must be (%d, 3) and (%d,)" % (n, n))
+    for a in (pr, neubc):
+        if a is not None and (tuple(a.shape) != (n,) or not a.is_cuda):
+            raise ValueError("pr and neubc must be (%d,) device tensors" % n)
+    it = C.c_int64()
+    ctx._check(lib().xg_chocg_poisson_dev(ctx._h, ptr(pr) if pr is not None else None,
+                                          ptr(neubc) if neubc is not None else None,
+                                          ptr(U), ptr(P), C.byref(it)))
+    return it.value
 
 
 def dt_dev(ctx, U):

@@ -570,6 +570,19 @@ __global__ void k_add_to(int n, double* __restrict__ P, const double* __restrict
   if (i < n) P[i] += dP[i];
 }
 
+// ChoCG::pinit's problem rhs (ChoCG.cpp:1112-1119, div = pr * vol) and
+// ConjugateGradients' Neumann term (ConjugateGradients.cpp:382, 473: b +=
+// neubc); either may be absent
+__global__ void k_poisson_rhs(int n, const double* __restrict__ pr,
+                              const double* __restrict__ vol,
+                              const double* __restrict__ neubc, double* __restrict__ div) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = pr ? pr[i] * vol[i] : div[i];
+  if (neubc) d += neubc[i];
+  div[i] = d;
+}
+
 // per-node limits of ChoCG::dt (ChoCG.cpp:1384-1394), block minimum
 __global__ __launch_bounds__(kBlock) void k_chodt_partial(
     int npoin, const double* __restrict__ U, const double* __restrict__ vol,
@@ -1106,6 +1119,37 @@ int xg_chocg_initial_dev(xg_ctx* c, double* U, double* P, int64_t* pit) {
   if ((rc = cho_grad(c, P, m->pgrad))) return rc;
   XG_HIP(c, hipGetLastError());
   if (pit) { pit[0] = it0; pit[1] = it1; }
+  return XG_OK;
+}
+
+int xg_chocg_poisson_dev(xg_ctx* c, const double* pr, const double* neubc, double* U,
+                         double* P, int64_t* pit) {
+  int rc = check_step(c);
+  if (rc) return rc;
+  if (!U || !P) return fail(c, XG_EINVAL, "xg_chocg_poisson: null field");
+  XG_HIP(c, hipSetDevice(c->device));
+  ChoMesh* m = c->cho;
+  const int n = m->npoin;
+  int64_t it = 0;
+  double* x = nullptr;
+  if ((rc = xg_cg_device_x(c, &x))) return rc;
+  // m_np 0: div(u), replaced by the problem's rhs where it has one
+  if ((rc = cho_bc(c, U))) return rc;
+  zero(c, m->div, n);
+  if ((rc = xg_chorin_div(c, 1, 0.0, P, m->pgrad, 3, U, 3, m->div, 0))) return rc;
+  if (pr || neubc)
+    hipLaunchKernelGGL(k_poisson_rhs, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, pr,
+                       m->vol, neubc, m->div);
+  if ((rc = cho_psolve(c, m->pval0, &it))) return rc;
+  // psolved with m_np 0: project u with dt = 1, BCs; nstep = 1: p = solution
+  if ((rc = cho_grad(c, x, m->sgrad))) return rc;
+  hipLaunchKernelGGL(k_project, dim3(nblocks((int64_t)n * 3)), dim3(kBlock), 0, c->stream,
+                     n, 1.0, U, m->sgrad);
+  if ((rc = cho_bc(c, U))) return rc;
+  XG_HIP(c, hipMemcpyAsync(P, x, (size_t)n * 8, hipMemcpyDeviceToDevice, c->stream));
+  XG_HIP(c, hipGetLastError());
+  XG_HIP(c, hipStreamSynchronize(c->stream));
+  if (pit) *pit = it;
   return XG_OK;
 }
 

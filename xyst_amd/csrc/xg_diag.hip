@@ -126,7 +126,8 @@ __global__ __launch_bounds__(kBlock) void k_diag_pre(int64_t n, const double* __
 
 // rhocompute's analytic-error sums (NodeDiagnostics.cpp:112-128): the
 // numerical solution in primitive form (the scalars as they are) against
-// An, (nu_c - An_c)^2 V and |nu_c - An_c| V, NC components per node
+// An, (nu_c - An_c)^2 V and |nu_c - An_c| V, NC components per node; NC = 1:
+// precompute's pressure error (NodeDiagnostics.cpp:237-241), p as it is
 template <int NC>
 __global__ __launch_bounds__(kBlock) void k_diag_err(int64_t n, const double* __restrict__ U,
                                                      const double* __restrict__ V,
@@ -142,10 +143,12 @@ __global__ __launch_bounds__(kBlock) void k_diag_err(int64_t n, const double* __
     double nu[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) nu[c] = U[i * NC + c];
-    nu[1] /= nu[0];
-    nu[2] /= nu[0];
-    nu[3] /= nu[0];
-    nu[4] = nu[4] / nu[0] - 0.5 * (nu[1] * nu[1] + nu[2] * nu[2] + nu[3] * nu[3]);
+    if constexpr (NC >= 5) {
+      nu[1] /= nu[0];
+      nu[2] /= nu[0];
+      nu[3] /= nu[0];
+      nu[4] = nu[4] / nu[0] - 0.5 * (nu[1] * nu[1] + nu[2] * nu[2] + nu[3] * nu[3]);
+    }
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const double du = nu[c] - An[i * NC + c];
@@ -259,15 +262,17 @@ int xg_diag_err_dev(xg_ctx* c, int64_t npoin, int ncomp, const double* U, const 
   if (!c) return XG_EINVAL;
   if (npoin < 0 || !U || !V || !An || !out)
     return fail(c, XG_EINVAL, "xg_diag_err: bad arguments");
-  if (ncomp < 5 || ncomp > XG_MAX_NCOMP)
-    return fail(c, XG_EUNSUPPORTED, "xg_diag_err: ncomp must be 5..XG_MAX_NCOMP");
+  if (ncomp != 1 && (ncomp < 5 || ncomp > XG_MAX_NCOMP))
+    return fail(c, XG_EUNSUPPORTED, "xg_diag_err: ncomp must be 1 or 5..XG_MAX_NCOMP");
   XG_HIP(c, hipSetDevice(c->device));
   const int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, nblocks(npoin)));
   const int ns = 2 * ncomp;
   double *part, *dout;
   int rc = diag_buffers(c, ns, nb, &part, &dout);
   if (rc) return rc;
-  if (ncomp == 5)
+  if (ncomp == 1)
+    hipLaunchKernelGGL(k_diag_err<1>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, V, An, part);
+  else if (ncomp == 5)
     hipLaunchKernelGGL(k_diag_err<5>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, V, An, part);
   else if (ncomp == 6)
     hipLaunchKernelGGL(k_diag_err<6>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, V, An, part);

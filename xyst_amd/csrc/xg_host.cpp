@@ -422,6 +422,32 @@ extern "C" int xg_mesh_symbc(int64_t npoin, const double* x, const double* y,
   return XG_OK;
 }
 
+extern "C" int xg_mesh_neumann(int64_t npoin, const double* x, const double* y,
+                               const double* z, int64_t ntri, const size_t* tri,
+                               const uint8_t* besym, const double* pg, double* neubc) {
+  if (npoin <= 0 || ntri < 0 || !x || !y || !z || !pg || !neubc ||
+      (ntri && (!tri || !besym)))
+    return XG_EINVAL;
+  for (int64_t i = 0; i < npoin; ++i) neubc[i] = 0.0;
+  for (int64_t e = 0; e < ntri; ++e) {
+    if (!besym[e]) continue;
+    const size_t* N = tri + 3 * e;
+    if (N[0] >= (size_t)npoin || N[1] >= (size_t)npoin || N[2] >= (size_t)npoin)
+      return XG_EINVAL;
+    // one third of the face's outward area vector: (x1-x0) x (x2-x0) / 6
+    const double ax = x[N[1]] - x[N[0]], ay = y[N[1]] - y[N[0]], az = z[N[1]] - z[N[0]];
+    const double bx = x[N[2]] - x[N[0]], by = y[N[2]] - y[N[0]], bz = z[N[2]] - z[N[0]];
+    const double n0 = (ay * bz - by * az) / 6.0;
+    const double n1 = (az * bx - bz * ax) / 6.0;
+    const double n2 = (ax * by - bx * ay) / 6.0;
+    for (int k = 0; k < 3; ++k) {
+      const double* g = pg + 3 * N[k];
+      neubc[N[k]] -= n0 * g[0] + n1 * g[1] + n2 * g[2];
+    }
+  }
+  return XG_OK;
+}
+
 extern "C" int xg_mesh_vol(int64_t npoin, const double* x, const double* y,
                            const double* z, int64_t ntet, const size_t* inpoel,
                            double* vol) {

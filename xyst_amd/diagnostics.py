@@ -24,8 +24,10 @@ def errcompute(ctx, U, V, An):
     values) and |nu_c - An_c| V (ncomp values), nu the primitive form of U
     (rho, u, v, w, specific internal energy, then the scalars) and An the
     analytic solution in that form; U, An (npoin, ncomp), V (npoin,) device
-    tensors, ncomp 5..7."""
-    nc = int(U.shape[1])
+    tensors, ncomp 5..7; or the pressure-based solvers' pressure error
+    (NodeDiagnostics.cpp:237-241): U the pressure, An the analytic pressure,
+    both (npoin,)."""
+    nc = 1 if U.dim() == 1 else int(U.shape[1])
     if tuple(An.shape) != tuple(U.shape):
         raise ValueError("An must have the shape of U")
     out = np.empty(2 * nc)
