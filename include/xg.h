@@ -364,6 +364,7 @@ typedef struct {
   int    stab2;
   double stab2coef;
   double mu;          /* mat_dyn_viscosity */
+  double diff;        /* mat_dyn_diffusivity: the transported scalars' */
 } xg_chocfg;
 
 /* The mesh arguments of the chorin:: operators (Chorin.hpp:19-70): the
@@ -391,8 +392,9 @@ int xg_chorin_div(xg_ctx* ctx, int dev, double dt, const double* P,
 /* chorin::grad (Chorin.cpp:336-449): G(:, 0..2) += gradient of scalar U. */
 int xg_chorin_grad(xg_ctx* ctx, int dev, const double* U, double* G, int ldg);
 
-/* chorin::vgrad (Chorin.cpp:211-334): G(npoin, 9) += velocity gradients of
- * U(npoin, ncomp); ncomp = 3. */
+/* chorin::vgrad (Chorin.cpp:211-334): G(npoin, 3 ncomp) += gradients of the
+ * velocity and transported scalars U(npoin, ncomp); ncomp 3 or 4 (one
+ * transported scalar, the count of the ChoCG problems, Problems.cpp:531-545). */
 int xg_chorin_vgrad(xg_ctx* ctx, int dev, int ncomp, const double* U, double* G);
 
 /* chorin::flux (Chorin.cpp:451-638): F(npoin, 3) += momentum flux
@@ -401,8 +403,10 @@ int xg_chorin_flux(xg_ctx* ctx, int dev, double mu, const double* U, int ldu,
                    const double* G, int ldg, double* F);
 
 /* chorin::rhs (Chorin.cpp:1010-1044): R(npoin, ncomp) = advection rhs
- * (damp2 / damp4) of U with pressure P and velocity gradients G (npoin, 9,
- * damp4 only); no source term; ncomp = 3. */
+ * (damp2 / damp4) of U (velocity, then the transported scalar: its flux
+ * from the same normal velocities, diffusivity cfg->diff) with pressure P
+ * and gradients G (npoin, 3 ncomp, damp4 only); ncomp 3 or 4.  Without the
+ * source term (the ChoCG step adds it, below). */
 int xg_chorin_rhs(xg_ctx* ctx, int dev, const xg_chocfg* cfg, int ncomp,
                   const double* P, const double* U, const double* G, double* R);
 
@@ -422,11 +426,18 @@ typedef struct {
   double    p_tol;
 } xg_chostep;
 
+/* The number of components of U for the ChoCG stepping of this context:
+ * 3 (velocity) or 4 (velocity and one transported scalar, ChoCG's
+ * Problems.cpp:531-545 count).  After xg_chocg_upload, before
+ * xg_chocg_setup (a change drops the setup; default 3). */
+int xg_chocg_set_ncomp(xg_ctx* ctx, int ncomp);
+
 /* Per-mesh data of the ChoCG chare on the mesh of xg_chocg_upload, whose
  * pressure matrix the caller has given to xg_cg_matrix (ChoCG::prelhs):
  * nodal volumes; symmetry (node, normal) pairs and noslip nodes
- * (ChoCG.cpp:637-686); velocity Dirichlet nodes with a 3-component mask and
- * the values physics::dirbc sets (setupDirBC, ChoCG.cpp:211-301); pressure
+ * (ChoCG.cpp:637-686); Dirichlet nodes with a per-component mask (ncomp
+ * of xg_chocg_set_ncomp per node) and the values physics::dirbc sets
+ * (setupDirBC, ChoCG.cpp:211-301); pressure
  * Dirichlet nodes (bc_dir of the pressure block and the hydrostat node,
  * ChoCG.cpp:1051-1112) with their initial-projection values (the pressure
  * increments of the time steps use 0). */
@@ -439,8 +450,8 @@ int xg_chocg_setup(xg_ctx* ctx, const double* vol, const xg_chostep* cfg,
 
 /* The ChoCG start-up (merge ... finpgrad, ChoCG.cpp:817-1323): BCs, the
  * divergence-free projection of U, the momentum-flux pressure solve that
- * sets P, and the pressure gradient.  U (npoin*3), P (npoin) on the device;
- * pit[0..1] receive the two solves' iteration counts. */
+ * sets P, and the pressure gradient.  U (npoin*ncomp), P (npoin) on the
+ * device; pit[0..1] receive the two solves' iteration counts. */
 int xg_chocg_initial_dev(xg_ctx* ctx, double* U, double* P, int64_t* pit);
 
 /* ChoCG's run of the first pressure solve alone (nstep = 1 with a problem
@@ -450,12 +461,14 @@ int xg_chocg_initial_dev(xg_ctx* ctx, double* U, double* P, int64_t* pit);
  * (npoin, from xg_mesh_neumann, or null), the CG solve with the pressure
  * Dirichlet nodes and values of xg_chocg_setup (mask 1: PRESSURE_IC at the
  * node), the projection U -= grad(p) with its BCs, and P = the solution.
- * pr, neubc, U (npoin*3), P (npoin) on the device; *pit the iterations. */
+ * pr, neubc, U (npoin*ncomp), P (npoin) on the device; *pit the
+ * iterations. */
 int xg_chocg_poisson_dev(xg_ctx* ctx, const double* pr, const double* neubc, double* U,
                          double* P, int64_t* pit);
 
 /* ChoCG::dt (ChoCG.cpp:1357-1413) without a constant dt: cfl * min over
- * nodes of cbrt(vol)/max(|u|, 1e-8) and cbrt(vol)^2/mu. */
+ * nodes of cbrt(vol)/max(|u|, 1e-8) and cbrt(vol)^2/max(mu, diff); the
+ * caller multiplies by freezeflow once t > freezetime (:1396-1399). */
 int xg_chocg_dt_dev(xg_ctx* ctx, const double* U, double* dt);
 
 /* The theta-implicit momentum system of ChoCG (momlhs/lhs, ChoCG.cpp:
@@ -465,7 +478,8 @@ int xg_chocg_dt_dev(xg_ctx* ctx, const double* U, double* dt);
  * last RK stage of xg_chocg_step_dev solves A du = R, A = -(M/dt + theta mu
  * K), du = 0 on the dirbc-masked components and noslip nodes, by CG from the
  * previous du, and sets U = Un + du (ChoCG.cpp:1570-1646).  Call after
- * xg_chocg_setup. */
+ * xg_chocg_setup; 3 components only (XG_EUNSUPPORTED with a transported
+ * scalar). */
 int xg_chocg_momentum_setup(xg_ctx* ctx, int64_t ntet, const size_t* inpoel,
                             double theta, int m_pc, int64_t m_maxit, double m_tol);
 /* Iterations of the last momentum solve. */
@@ -474,11 +488,21 @@ int xg_chocg_momentum_its(xg_ctx* ctx, int64_t* its);
 /* One ChoCG time step (rhs/solve/pred/corr for every RK stage, then
  * div/pinit/psolve/sgrad/psolved/pgrad, ChoCG.cpp:869-1695), the momentum
  * stages explicit or, after xg_chocg_momentum_setup with theta > 0, the
- * last one semi-implicit.  U (npoin*3) advanced in place, Un receives the step's
- * starting velocity, P += dP with dP (npoin) the pressure increment; *pit
- * the pressure solve's iterations. */
+ * last one semi-implicit.  The rhs includes the source of the context's
+ * problem (xg_set_problem: slot_cyl, with ncomp 4; the other device
+ * problems have no ChoCG source and are refused).  U (npoin*ncomp) advanced
+ * in place, Un receives the step's starting state, P += dP with dP (npoin)
+ * the pressure increment; *pit the pressure solve's iterations. */
 int xg_chocg_step_dev(xg_ctx* ctx, double dt, double* U, double* Un, double* P,
                       double* dP, int64_t* pit);
+/* xg_chocg_step_dev with the flow frozen when freeze != 0 (the chare's
+ * freezeflow > 1, ChoCG.cpp:1550-1568): every explicit stage puts the
+ * velocity of its start back after its update, BCs and (damp4) gradients,
+ * so only the transported scalar advances until the projection; with damp2
+ * the last stage's divergence is taken before the velocity is put back,
+ * as the reference's pred/corr order does. */
+int xg_chocg_step2_dev(xg_ctx* ctx, double dt, int freeze, double* U, double* Un,
+                       double* P, double* dP, int64_t* pit);
 
 /* ---- node diagnostics -------------------------------------------------------- */
 
@@ -509,10 +533,11 @@ int xg_diag_rho_dev_noalloc(xg_ctx* ctx, int64_t npoin, int ncomp, const double*
  * solution in primitive form (rho, u, v, w, specific internal energy, then
  * the scalars; the caller evaluates SOL()), out[c] = sum_i (nu_c - An_c)^2
  * V_i and out[ncomp+c] = sum_i |nu_c - An_c| V_i, nu the same form of U.
- * 5 <= ncomp <= XG_MAX_NCOMP, or ncomp = 1: the pressure error of
- * NodeDiagnostics::precompute (NodeDiagnostics.cpp:186-199, 237-241) with U
- * the pressure and An PRESSURE_SOL() at the nodes, no conversion; out (host)
- * 2 ncomp values; deterministic summation order. */
+ * 5 <= ncomp <= XG_MAX_NCOMP; or the error sums of
+ * NodeDiagnostics::precompute (NodeDiagnostics.cpp:186-250) without
+ * conversion: ncomp = 1 with U the pressure and An PRESSURE_SOL() at the
+ * nodes, ncomp 3 or 4 with U the velocity (and transported scalar) and An
+ * SOL(); out (host) 2 ncomp values; deterministic summation order. */
 int xg_diag_err_dev(xg_ctx* ctx, int64_t npoin, int ncomp, const double* U,
                     const double* V, const double* An, double* out);
 
@@ -522,6 +547,12 @@ int xg_diag_err_dev(xg_ctx* ctx, int64_t npoin, int ncomp, const double* U,
  * on the device; out (host) 8 values. */
 int xg_diag_pre_dev(xg_ctx* ctx, int64_t npoin, const double* U, const double* Un,
                     const double* P, const double* dP, const double* V, double* out);
+/* xg_diag_pre_dev for U, Un of ncomp (3 or 4: with a transported scalar)
+ * components per node: out = {sum p^2 V, sum u_c^2 V (ncomp), sum dp^2 V,
+ * sum (u_c - un_c)^2 V (ncomp)}, 2 ncomp + 2 values. */
+int xg_diag_pre_nc_dev(xg_ctx* ctx, int64_t npoin, int ncomp, const double* U,
+                       const double* Un, const double* P, const double* dP,
+                       const double* V, double* out);
 
 /* ---- partitions on one device ------------------------------------------------
  * The reference runs one chare per mesh partition and adds the partitions'

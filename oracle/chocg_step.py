@@ -73,7 +73,8 @@ class ChoCGRun:
                  stab2=False, stab2coef=0.2, cfl=0.9, rk=1, noslip=(), sym=(),
                  dirbc=None, p_hydrostat=None, p_dirbc=None, p_tol=1e-3, p_iter=500,
                  p_pc="jacobi", ic=(0.0, 0.0, 0.0), theta=0.0, m_tol=1e-3, m_iter=50,
-                 m_pc="jacobi", p_ic=None):
+                 m_pc="jacobi", p_ic=None, U0=None, src=None, diff=0.0, freezeflow=1.0,
+                 freezetime=0.0):
         self.coord = np.ascontiguousarray(coord, dtype=np.float64)
         self.inpoel = np.ascontiguousarray(inpoel, dtype=np.int64)
         self.tri = np.ascontiguousarray(tri, dtype=np.int64)
@@ -85,7 +86,7 @@ class ChoCGRun:
         self.vol = O.vol(self.coord, self.inpoel)
         self.A = O.CSR.from_tets(n, self.inpoel)
         O.prelhs(self.coord, self.inpoel, self.A)
-        self.cfg = O.chocfg(flux, stab, stab2, stab2coef, mu)
+        self.cfg = O.chocfg(flux, stab, stab2, stab2coef, mu, diff)
         self.flux = flux
         self.mu = mu
         self.cfl = cfl
@@ -98,13 +99,20 @@ class ChoCGRun:
                                                             self.faceset, list(sym))
         else:
             self.symnodes, self.symnorms = np.zeros(0, np.int64), np.zeros((0, 3))
-        # setupDirBC: per node a mask (0/1/2) and values; masks of several sets
-        # merge (0 -> nonzero), values are the last set's
+        # the initial state: velocity and the transported scalars (ncomp
+        # columns), a constant ic or the field U0
+        if U0 is None:
+            U0 = np.tile(np.asarray(ic, np.float64), (n, 1))
+        self.U0 = np.ascontiguousarray(U0, dtype=np.float64)
+        nc = self.U0.shape[1]
+        self.ncomp = nc
+        # setupDirBC: per node a mask (0/1/2) per component and values; masks
+        # of several sets merge (0 -> nonzero), values are the last set's
         self.dir = {}
         for s, (mask, val) in (dirbc or {}).items():
             for p in nodes_of([s]):
-                m, v = self.dir.get(int(p), ([0, 0, 0], None))
-                m = [m[c] if m[c] else mask[c] for c in range(3)]
+                m, v = self.dir.get(int(p), ([0] * nc, None))
+                m = [m[c] if m[c] else mask[c] for c in range(nc)]
                 self.dir[int(p)] = (m, list(val))
         # pressure Dirichlet nodes and the values of the start-up solves
         # (mask 1: the initial pressure p_ic at the node, 0 without one; mask
@@ -118,17 +126,21 @@ class ChoCGRun:
                 else:
                     self.pdir[int(p)] = float(val)
         self.hydrostat = p_hydrostat
-        self.ic = np.asarray(ic, np.float64)
+        # a time-independent source of chorin::src (Chorin.cpp:985-1008):
+        # s (npoin, ncomp) at the nodes, R -= s vol
+        self.src = None if src is None else np.asarray(src, np.float64)
+        self.freezeflow, self.freezetime, self.ff = freezeflow, freezetime, 1.0
         self.p_tol, self.p_iter, self.p_pc = p_tol, p_iter, p_pc
         self.theta, self.m_tol, self.m_iter, self.m_pc = theta, m_tol, m_iter, m_pc
         self.mit = []
         if theta > np.finfo(float).eps:
+            assert nc == 3, "the theta-implicit momentum system has 3 components"
             self._momentum_setup()
-        self.U = np.tile(np.asarray(ic, np.float64), (n, 1))
+        self.U = self.U0.copy()
         self.Un = self.U.copy()
         self.P = np.zeros(n)
         self.pgrad = np.zeros((n, 3))
-        self.vgrad = np.zeros((n, 9))
+        self.vgrad = np.zeros((n, 3 * nc))
         self.x = np.zeros(n)          # the pressure solver's solution (warm start)
         self.div = np.zeros(n)
         self.t, self.dt, self.it = 0.0, 0.0, 0
@@ -197,9 +209,9 @@ class ChoCGRun:
     # ---- helpers --------------------------------------------------------
     def BC(self, U):
         for p, (m, v) in self.dir.items():
-            for c in range(3):
+            for c in range(self.ncomp):
                 if m[c] == 1:
-                    U[p, c] = self.ic[c]
+                    U[p, c] = self.U0[p, c]
                 elif m[c] == 2:
                     U[p, c] = v[c]
         if self.symnodes.size:
@@ -209,7 +221,7 @@ class ChoCGRun:
                 U[p, 0] -= vn * nx
                 U[p, 1] -= vn * ny
                 U[p, 2] -= vn * nz
-        U[self.noslip, :] = 0.0
+        U[self.noslip, :3] = 0.0
 
     def _grad(self, s):
         return O.chorin_grad(self.se, self.coord, self.tri, s) / self.vol[:, None]
@@ -240,7 +252,7 @@ class ChoCGRun:
         # m_np = 0: div(u), solve, project with dt = 1
         self.div = self._divergence(self.U, stab=False)
         sol = self._psolve(self.div, initial=True)
-        self.U -= self._grad(sol)
+        self.U[:, :3] -= self._grad(sol)
         self.BC(self.U)
         self.np_ = 1
         # velgrad -> flux -> div(flux): initial pressure from the momentum flux
@@ -268,7 +280,7 @@ class ChoCGRun:
         if pr is not None:
             self.div = np.asarray(pr, np.float64) * self.vol
         sol = self._psolve(self.div, initial=True, neubc=neubc)
-        self.U -= self._grad(sol)
+        self.U[:, :3] -= self._grad(sol)
         self.BC(self.U)
         self.P = sol.copy()
         return sol
@@ -285,7 +297,8 @@ class ChoCGRun:
                          np.sqrt(np.sum(e * e * V) / tot), np.sum(np.abs(e) * V) / tot])
 
     def dt_cfl(self):
-        """ChoCG::dt (1357-1413): cfl * min(L/max(|u|,1e-8), L^2/mu)."""
+        """ChoCG::dt (1357-1413): cfl * min(L/max(|u|,1e-8), L^2/mu), times
+        freezeflow once t > freezetime (then for good)."""
         u = self.U
         vel = np.sqrt(u[:, 0] * u[:, 0] + u[:, 1] * u[:, 1] + u[:, 2] * u[:, 2])
         L = np.cbrt(self.vol)
@@ -293,19 +306,39 @@ class ChoCGRun:
         eps = np.finfo(float).eps
         if self.mu > eps:
             m = min(m, np.min(L * L / self.mu))
-        return m * self.cfl
+        m = m * self.cfl
+        if self.t > self.freezetime:
+            self.ff = self.freezeflow
+        return m * self.ff
+
+    def _rhs(self):
+        R = O.chorin_rhs(self.cfg, self.se, self.coord, self.tri, self.P, self.U,
+                         self.vgrad)
+        if self.src is not None:
+            R -= self.src * self.vol[:, None]
+        return R
 
     def step(self, dt=None):
+        """One time step.  With the flow frozen (freezeflow > 1 once set by
+        dt_cfl) every stage keeps the velocity it started with
+        (ChoCG.cpp:1550-1568): the velocity is put back after the stage's
+        update, BCs and -- damp4 -- velocity gradients, which therefore see
+        the updated velocity; with damp2 the last stage's divergence does
+        too (corr runs inside pred there, before the velocity is put
+        back)."""
         self.dt = self.dt_cfl() if dt is None else dt
         implicit = self.theta > np.finfo(float).eps
+        frozen = self.ff > 1.0
         if implicit:
             self._momentum_matrix()
+        udiv = None
         for stage, c in enumerate(self.rkcoef):
-            R = O.chorin_rhs(self.cfg, self.se, self.coord, self.tri, self.P, self.U,
-                             self.vgrad)
+            R = self._rhs()
             if stage == 0:
                 self.Un = self.U.copy()
-            if implicit and stage + 1 == len(self.rkcoef):
+            u_in = self.U.copy() if frozen else None
+            last = stage + 1 == len(self.rkcoef)
+            if implicit and last:
                 # semi-implicit solve (ChoCG.cpp:1570-1646): A du = R, u = un + du
                 if not hasattr(self, "xm"):
                     self.xm = np.zeros(3 * self.npoin)
@@ -321,11 +354,15 @@ class ChoCGRun:
             if self.flux == "damp4":
                 self.vgrad = (O.chorin_vgrad(self.se, self.coord, self.tri, self.U)
                               / self.vol[:, None])
+            elif frozen and last:
+                udiv = self.U.copy()
+            if frozen:
+                self.U[:, :3] = u_in[:, :3]
         # projection
-        div = self._divergence(self.U, stab=True)
+        div = self._divergence(self.U if udiv is None else udiv, stab=True)
         self.div = div / self.dt
         dp = self._psolve(self.div)
-        self.U -= self.dt * self._grad(dp)
+        self.U[:, :3] -= self.dt * self._grad(dp)
         self.BC(self.U)
         self.P += dp
         self.pgrad = self._grad(self.P)
@@ -333,15 +370,24 @@ class ChoCGRun:
         self.t += self.dt
         return dp
 
-    def diag(self, dp):
-        """Columns of the diagnostics file: it, t, dt, L2(p), L2(u,v,w),
-        L2(dp), L2(du,dv,dw); sums over nodes of f^2 V, normalised by the
-        total volume and square-rooted (Transporter::diagnostics)."""
+    def diag(self, dp, An=None):
+        """Columns of the diagnostics file: it, t, dt, L2(p), L2(u_c), L2(dp),
+        L2(du_c), then with an analytic solution An (npoin, ncomp) L2(err_c)
+        and L1(err_c); sums over nodes of f^2 V (|f| V for L1), normalised by
+        the total volume, L2 square-rooted (NodeDiagnostics::precompute,
+        Transporter::diagnostics)."""
         V = self.vol
         tot = V.sum()
+        nc = self.ncomp
         s = [np.sum(self.P * self.P * V)]
-        s += [np.sum(self.U[:, c] * self.U[:, c] * V) for c in range(3)]
+        s += [np.sum(self.U[:, c] * self.U[:, c] * V) for c in range(nc)]
         s += [np.sum(dp * dp * V)]
         d = self.U - self.Un
-        s += [np.sum(d[:, c] * d[:, c] * V) for c in range(3)]
-        return np.concatenate([[self.it, self.t, self.dt], np.sqrt(np.array(s) / tot)])
+        s += [np.sum(d[:, c] * d[:, c] * V) for c in range(nc)]
+        row = [[self.it, self.t, self.dt], np.sqrt(np.array(s) / tot)]
+        if An is not None:
+            e = self.U - An
+            row.append(np.sqrt(np.array([np.sum(e[:, c] * e[:, c] * V)
+                                         for c in range(nc)]) / tot))
+            row.append(np.array([np.sum(np.abs(e[:, c]) * V) for c in range(nc)]) / tot)
+        return np.concatenate(row)

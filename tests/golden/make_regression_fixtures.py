@@ -1,9 +1,10 @@
 """Extract the reference's RieCG/Sod, KozCG/Sod, ChoCG/Lid,
 ChoCG/Poiseuille, analytic-problem (riecg_analytic.npz, see
-analytic_fixtures), RieCG/SlotCyl and ChoCG/Poisson regression fixtures into
-riecg_sod.npz, kozcg_sod.npz, chocg_lid.npz, chocg_poiseuille.npz,
-riecg_analytic.npz, riecg_slotcyl.npz and chocg_poisson.npz (run with the
-argument "poisson" to write only the last).
+analytic_fixtures), RieCG/SlotCyl, ChoCG/Poisson and ChoCG/SlotCyl
+regression fixtures into riecg_sod.npz, kozcg_sod.npz, chocg_lid.npz,
+chocg_poiseuille.npz, riecg_analytic.npz, riecg_slotcyl.npz,
+chocg_poisson.npz and chocg_slotcyl.npz (the argument "poisson" or
+"chocg_slotcyl" writes only that one).
 
 Source files (reference test data, read as data only), per directory
 tests/regression/inciter/{RieCG,KozCG}/Sod:
@@ -167,6 +168,7 @@ def main():
     analytic_fixtures()
     slotcyl_fixtures()
     poisson_fixtures()
+    chocg_slotcyl_fixtures()
 
     # ExodusII files kept as they are, for the file reader's tests: a mesh
     # with a triangle block and side sets on it (RieCG/Sod), a mesh with
@@ -359,7 +361,35 @@ def poisson_fixtures():
     print("wrote chocg_poisson.npz", sorted(out))
 
 
+def chocg_slotcyl_fixtures():
+    """ChoCG/SlotCyl (velocity and one transported scalar c0: the slotted
+    cylinder, cone and hump carried by a prescribed rotation held by a
+    momentum source) into chocg_slotcyl.npz.  Run configuration
+    (slot_cyl*.q): unitsquare_01_3.6k.exo, nstep 20, cfl 0.9, Dirichlet to
+    the initial condition on side set 1 (all four components) and side set 2
+    (velocity only), pressure CG with Jacobi, tol 1e-2, 300 iterations,
+    hydrostat node 0; damp2 with rk 3 (slot_cyl.q), damp4 with rk 4
+    (slot_cyl_damp4.q) and damp4 with rk 4, freezeflow 2 after t = 0.1
+    (slot_cyl_damp4_freeze.q); fields after the last step."""
+    ref = os.path.join(REF, "ChoCG", "SlotCyl")
+    coord, tets, tri, faceset = read_mesh(ref, "unitsquare_01_3.6k.exo")
+    out = dict(coord=coord, inpoel=tets, triinpoel=tri, faceset=faceset, nstep=20, cfl=0.9,
+               p_iter=300, p_tol=1e-2, p_hydrostat=0, freezeflow=2.0, freezetime=0.1)
+    for case, exo, diag in (("damp2", "slot_cyl.std.exo", "diag.std"),
+                            ("damp4", "slot_cyl_damp4.std.exo", "diag_damp4.std"),
+                            ("freeze", "slot_cyl_damp4_freeze.std.exo",
+                             "diag_damp4_freeze.std")):
+        f, names, tf = read_result(ref, coord, exo)
+        out.update({case + "_fields": f, case + "_t_final": tf,
+                    case + "_diag": np.loadtxt(os.path.join(ref, diag))})
+    out["field_names"] = np.array(names)
+    np.savez_compressed(os.path.join(HERE, "chocg_slotcyl.npz"), **out)
+    print("wrote chocg_slotcyl.npz", names)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["poisson"]:
         sys.exit(poisson_fixtures())
+    if sys.argv[1:] == ["chocg_slotcyl"]:
+        sys.exit(chocg_slotcyl_fixtures())
     sys.exit(main())

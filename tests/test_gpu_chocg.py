@@ -129,21 +129,27 @@ def test_chorin_device_pointers_match_host_entry():
 
 # ---------------------------------------------------------------- ChoCG step --
 
-def _device_run(r, cfgkw):
-    """The device ChoCG on the mesh and BCs of an oracle ChoCGRun r."""
+def _device_run(r, cfgkw, problem=None):
+    """The device ChoCG on the mesh and BCs of an oracle ChoCGRun r (its
+    component count; problem: the source's problem name)."""
     import torch
     from xyst_amd import chorin as X
     from xyst_amd import riemann as R
     from xyst_amd.cg import CSR, ConjugateGradients
     ctx = R.Context(0)
     X.upload(ctx, r.coord, r.se.sup, r.se.int, r.tri)
+    if r.ncomp != 3:
+        X.set_ncomp(ctx, r.ncomp)
+    if problem is not None:
+        ctx.set_problem(problem)
     cg = ConjugateGradients(CSR.poisson(r.coord, r.inpoel), ctx=ctx)
+    nc = r.ncomp
     dn = np.array(sorted(r.dir), np.int64)
-    mk = np.array([r.dir[p][0] for p in dn], np.int32).reshape(-1, 3)
-    cv = np.array([r.dir[p][1] for p in dn]).reshape(-1, 3)
+    mk = np.array([r.dir[p][0] for p in dn], np.int32).reshape(-1, nc)
+    cv = np.array([r.dir[p][1] for p in dn]).reshape(-1, nc)
     # the values physics::dirbc sets: mask 1 the initial value, mask 2 the
     # configured one
-    dv = np.where(mk == 2, cv, r.ic[None, :] * np.ones_like(cv))
+    dv = np.where(mk == 2, cv, r.U0[dn].reshape(-1, nc))
     dm = (mk != 0).astype(np.int32)
     pd = sorted(set(r.pdir) | ({r.hydrostat} if r.hydrostat is not None else set()))
     X.step_setup(ctx, r.vol, X.config(**cfgkw), cfl=r.cfl, rk=len(r.rkcoef),

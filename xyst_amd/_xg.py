@@ -51,6 +51,7 @@ SIG = {
     "xg_riecg_grad_div_dev": (C.c_int, [VP, VP]),
     "xg_riecg_solve_dev": (C.c_int, [VP, C.c_int, D, D, VP, VP, VP, VP, VP]),
     "xg_diag_err_dev": (C.c_int, [VP, I64, C.c_int, VP, VP, VP, VP]),
+    "xg_diag_pre_nc_dev": (C.c_int, [VP, I64, C.c_int, VP, VP, VP, VP, VP, VP]),
     "xg_comm_create": (C.c_int, [C.c_int, C.c_int, VP, VP, C.POINTER(VP)]),
     "xg_comm_sum_dev": (C.c_int, [VP, C.c_int, VP, VP]),
     "xg_comm_shared": (C.c_int, [VP, C.POINTER(I64)]),
@@ -72,6 +73,8 @@ SIG = {
     "xg_chocg_setup": (C.c_int, [VP, VP, VP, I64, VP, VP, I64, VP, I64, VP, VP, VP,
                                  I64, VP, VP]),
     "xg_chocg_initial_dev": (C.c_int, [VP, VP, VP, VP]),
+    "xg_chocg_set_ncomp": (C.c_int, [VP, C.c_int]),
+    "xg_chocg_step2_dev": (C.c_int, [VP, D, C.c_int, VP, VP, VP, VP, VP]),
     "xg_chocg_poisson_dev": (C.c_int, [VP, VP, VP, VP, VP, VP]),
     "xg_chocg_dt_dev": (C.c_int, [VP, VP, VP]),
     "xg_chocg_step_dev": (C.c_int, [VP, D, VP, VP, VP, VP, VP]),
@@ -161,7 +164,7 @@ PROBLEMS = {"none": 0, "vortical_flow": 1, "taylor_green": 2, "slot_cyl": 3}
 
 class ChoConfig(C.Structure):
     _fields_ = [("flux", C.c_int), ("stab", C.c_int), ("stab2", C.c_int),
-                ("stab2coef", C.c_double), ("mu", C.c_double)]
+                ("stab2coef", C.c_double), ("mu", C.c_double), ("diff", C.c_double)]
 
 
 class ChoStep(C.Structure):

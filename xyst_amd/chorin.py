@@ -15,8 +15,8 @@ import numpy as np
 from ._xg import ChoConfig, ChoStep, lib, ptr
 
 __all__ = ["mesh_domint", "upload", "div", "grad", "vgrad", "flux", "rhs", "config", "step_setup",
-           "initial_dev", "poisson_dev", "neumann", "dt_dev", "step_dev", "momentum_setup",
-           "momentum_its"]
+           "set_ncomp", "initial_dev", "poisson_dev", "neumann", "dt_dev", "step_dev",
+           "momentum_setup", "momentum_its"]
 
 FLUX = {"damp2": 0, "damp4": 1}
 
@@ -36,10 +36,12 @@ def _dev(*arrs):
     return int(kinds.pop()) if kinds else 0
 
 
-def config(flux="damp2", stab=True, stab2=False, stab2coef=0.0, mu=0.0):
+def config(flux="damp2", stab=True, stab2=False, stab2coef=0.0, mu=0.0, diff=0.0):
+    """chorin::rhs settings: flux, stab, stab2 (coefficient), viscosity mu
+    and the transported scalar's diffusivity diff."""
     if flux not in FLUX:
         raise ValueError("Flux not correctly configured")
-    return ChoConfig(FLUX[flux], int(bool(stab)), int(bool(stab2)), stab2coef, mu)
+    return ChoConfig(FLUX[flux], int(bool(stab)), int(bool(stab2)), stab2coef, mu, diff)
 
 
 def mesh_domint(coord, inpoel):
@@ -72,6 +74,7 @@ def upload(ctx, coord, dsupedge, dsupint, triinpoel):
                                      ptr(s[0]), ptr(d[0]), ptr(s[1]), ptr(d[1]),
                                      ptr(s[2]), ptr(d[2]), tri.size // 3, ptr(tri)))
     ctx.cho_npoin = x.size
+    ctx.cho_ncomp = 3
 
 
 def _cols(a):
@@ -94,7 +97,8 @@ def grad(ctx, U, G):
 
 
 def vgrad(ctx, U, G):
-    """chorin::vgrad: G(npoin, 9) += gradients of U(npoin, 3)."""
+    """chorin::vgrad: G(npoin, 3 ncomp) += gradients of U(npoin, ncomp),
+    ncomp 3 or 4."""
     ctx._check(lib().xg_chorin_vgrad(ctx._h, _dev(U, G), _cols(U), ptr(U), ptr(G)))
     return G
 
@@ -107,7 +111,8 @@ def flux(ctx, mu, U, G, F):
 
 
 def rhs(ctx, cfg, P, U, G, R=None):
-    """chorin::rhs: R = advection rhs (R allocated for host inputs)."""
+    """chorin::rhs: R = advection rhs of U (npoin, ncomp: velocity and a
+    transported scalar, ncomp 3 or 4; R allocated for host inputs)."""
     dev = _dev(P, U, G, R)
     if R is None:
         if dev:
@@ -118,36 +123,53 @@ def rhs(ctx, cfg, P, U, G, R=None):
     return R
 
 
+def set_ncomp(ctx, ncomp):
+    """Components of U for the ChoCG stepping: 3, or 4 with a transported
+    scalar; call after `upload`, before `step_setup`."""
+    ctx._check(lib().xg_chocg_set_ncomp(ctx._h, int(ncomp)))
+    ctx.cho_ncomp = int(ncomp)
+
+
 def step_setup(ctx, vol, cfg, cfl=0.9, rk=1, p_pc="jacobi", p_maxit=500, p_tol=1e-3,
                symnodes=(), symnorms=(), noslip=(), dirnodes=(), dirmask=None,
                dirvals=None, pdirnodes=(), pdirvals=None):
-    """ChoCG per-mesh data (after `upload` and the pressure matrix given to
-    the context's ConjugateGradients): cfg from `config`; velocity Dirichlet
-    nodes with (ndir, 3) mask and values; pressure Dirichlet nodes (bc_dir of
-    the pressure block and the hydrostat node) with their initial values."""
+    """ChoCG per-mesh data (after `upload`, `set_ncomp` and the pressure
+    matrix given to the context's ConjugateGradients): cfg from `config`;
+    Dirichlet nodes with (ndir, ncomp) masks and values; pressure Dirichlet
+    nodes (bc_dir of the pressure block and the hydrostat node) with their
+    initial values."""
     st = ChoStep(cfg, float(cfl), int(rk), {"none": 0, "jacobi": 1, "amg": 2}[p_pc], int(p_maxit),
                  float(p_tol))
+    nc = getattr(ctx, "cho_ncomp", 3)
     v = _f64(vol)
     sn, sv = _idx(symnodes), _f64(symnorms).reshape(-1)
     ns = _idx(noslip)
     dn = _idx(dirnodes)
     nd = dn.size
-    dm = np.ascontiguousarray(np.ones((nd, 3)) if dirmask is None else dirmask,
+    dm = np.ascontiguousarray(np.ones((nd, nc)) if dirmask is None else dirmask,
                               dtype=np.int32).reshape(-1)
-    dv = _f64(np.zeros((nd, 3)) if dirvals is None else dirvals).reshape(-1)
+    dv = _f64(np.zeros((nd, nc)) if dirvals is None else dirvals).reshape(-1)
     pn = _idx(pdirnodes)
     pv = _f64(np.zeros(pn.size) if pdirvals is None else pdirvals).reshape(-1)
-    if v.size != ctx.cho_npoin or sv.size != 3 * sn.size or dm.size != 3 * nd or \
-            dv.size != 3 * nd or pv.size != pn.size:
+    if v.size != ctx.cho_npoin or sv.size != 3 * sn.size or dm.size != nc * nd or \
+            dv.size != nc * nd or pv.size != pn.size:
         raise ValueError("step_setup: inconsistent array sizes")
     ctx._check(lib().xg_chocg_setup(ctx._h, ptr(v), C.byref(st), sn.size, ptr(sn), ptr(sv),
                                     ns.size, ptr(ns), nd, ptr(dn), ptr(dm), ptr(dv),
                                     pn.size, ptr(pn), ptr(pv)))
 
 
+def _fields(ctx, *arrs):
+    n, nc = ctx.cho_npoin, getattr(ctx, "cho_ncomp", 3)
+    for a in arrs:
+        if tuple(a.shape) != (n, nc) or not a.is_cuda:
+            raise ValueError("device fields must be (%d, %d) on the GPU" % (n, nc))
+
+
 def initial_dev(ctx, U, P):
-    """ChoCG start-up on device tensors U (npoin, 3), P (npoin,); returns the
-    two pressure solves' iteration counts."""
+    """ChoCG start-up on device tensors U (npoin, ncomp), P (npoin,); returns
+    the two pressure solves' iteration counts."""
+    _fields(ctx, U)
     it = (C.c_int64 * 2)()
     ctx._check(lib().xg_chocg_initial_dev(ctx._h, ptr(U), ptr(P), it))
     return it[0], it[1]
@@ -174,11 +196,12 @@ def neumann(coord, tri, besym, pg):
 def poisson_dev(ctx, U, P, pr=None, neubc=None):
     """ChoCG's first pressure solve alone (nstep = 1, the Poisson problems):
     pr (npoin,) the problem's pressure rhs at the nodes, neubc (npoin,) the
-    Neumann vector, device tensors or None; U (npoin, 3) is projected and P
-    (npoin,) receives the solution.  Returns the CG iterations."""
+    Neumann vector, device tensors or None; U (npoin, ncomp) is projected and
+    P (npoin,) receives the solution.  Returns the CG iterations."""
     n = ctx.cho_npoin
-    if tuple(U.shape) != (n, 3) or tuple(P.shape) != (n,) or not U.is_cuda:
-        raise ValueError("device fields must be (%d, 3) and (%d,)" % (n, n))
+    _fields(ctx, U)
+    if tuple(P.shape) != (n,):
+        raise ValueError("P must be (%d,)" % n)
     for a in (pr, neubc):
         if a is not None and (tuple(a.shape) != (n,) or not a.is_cuda):
             raise ValueError("pr and neubc must be (%d,) device tensors" % n)
@@ -190,19 +213,19 @@ def poisson_dev(ctx, U, P, pr=None, neubc=None):
 
 
 def dt_dev(ctx, U):
+    _fields(ctx, U)
     out = C.c_double()
     ctx._check(lib().xg_chocg_dt_dev(ctx._h, ptr(U), C.byref(out)))
     return out.value
 
 
-def step_dev(ctx, dt, U, Un, P, dP):
-    """One ChoCG step; returns the pressure solve's iteration count."""
-    for a, nc in ((U, 3), (Un, 3)):
-        if tuple(a.shape) != (ctx.cho_npoin, nc) or not a.is_cuda:
-            raise ValueError("device fields must be (%d, 3) on the GPU" % ctx.cho_npoin)
+def step_dev(ctx, dt, U, Un, P, dP, freeze=False):
+    """One ChoCG step (freeze: the flow frozen, freezeflow > 1); returns the
+    pressure solve's iteration count."""
+    _fields(ctx, U, Un)
     it = C.c_int64()
-    ctx._check(lib().xg_chocg_step_dev(ctx._h, float(dt), ptr(U), ptr(Un), ptr(P), ptr(dP),
-                                       C.byref(it)))
+    ctx._check(lib().xg_chocg_step2_dev(ctx._h, float(dt), int(bool(freeze)), ptr(U), ptr(Un),
+                                        ptr(P), ptr(dP), C.byref(it)))
     return it.value
 
 

@@ -46,12 +46,13 @@ struct ChoMesh {
   uint32_t *fab = nullptr, *fcs = nullptr;
   double *fnx = nullptr, *fny = nullptr, *fnz = nullptr;
   int nslot = 0;
-  double* slots = nullptr;     // halo partials, up to 9 per slot
+  double* slots = nullptr;     // halo partials, kSlotVals per slot
   int nfx = 0;
   int *fx_node = nullptr, *fx_off = nullptr, *fx_slot = nullptr;
   // staging of the host-pointer calls (grow only)
   std::vector<DevArr> stage;
   // ChoCG stepping (xg_chocg_setup)
+  int nc = 3;                  // components of U: velocity + transported scalars
   bool have_step = false;
   std::vector<DevArr> step_allocs;
   xg_chostep cfg{};
@@ -65,6 +66,7 @@ struct ChoMesh {
   std::vector<double> pval0, pzero;         // initial values / increments (0)
   double *pgrad = nullptr, *vgrad = nullptr, *sgrad = nullptr, *R = nullptr;
   double *div = nullptr, *F = nullptr, *red = nullptr;
+  double* usave = nullptr;     // a stage's starting state (frozen flow)
   std::vector<int> h_dirnode, h_noslip;    // host copies of the BC lists
   std::vector<uint8_t> h_dirmask;
   // theta-implicit momentum solve (xg_chocg_momentum_setup)
@@ -86,14 +88,18 @@ constexpr int kCap = xg::kBlockCap;
 
 enum { OP_GRAD = 0, OP_VGRAD, OP_DIV, OP_DIVS, OP_FLUX, OP_ADV2, OP_ADV4 };
 
-template <int OP> struct OpShape;
-template <> struct OpShape<OP_GRAD>  { static constexpr int NV = 1,  NF = 3; };
-template <> struct OpShape<OP_VGRAD> { static constexpr int NV = 3,  NF = 9; };
-template <> struct OpShape<OP_DIV>   { static constexpr int NV = 3,  NF = 1; };
-template <> struct OpShape<OP_DIVS>  { static constexpr int NV = 10, NF = 1; };
-template <> struct OpShape<OP_FLUX>  { static constexpr int NV = 12, NF = 3; };
-template <> struct OpShape<OP_ADV2>  { static constexpr int NV = 4,  NF = 3; };
-template <> struct OpShape<OP_ADV4>  { static constexpr int NV = 16, NF = 3; };
+// NV staged values per node, NF values per edge; NC: components of U
+// (velocity + transported scalars) for vgrad and the advection operators
+template <int OP, int NC> struct OpShape;
+template <int NC> struct OpShape<OP_GRAD, NC>  { static constexpr int NV = 1,  NF = 3; };
+template <int NC> struct OpShape<OP_VGRAD, NC> { static constexpr int NV = NC, NF = 3 * NC; };
+template <int NC> struct OpShape<OP_DIV, NC>   { static constexpr int NV = 3,  NF = 1; };
+template <int NC> struct OpShape<OP_DIVS, NC>  { static constexpr int NV = 10, NF = 1; };
+template <int NC> struct OpShape<OP_FLUX, NC>  { static constexpr int NV = 12, NF = 3; };
+template <int NC> struct OpShape<OP_ADV2, NC>  { static constexpr int NV = NC + 1, NF = NC; };
+template <int NC> struct OpShape<OP_ADV4, NC>  { static constexpr int NV = 4 * NC + 4, NF = NC; };
+// values per halo slot: the widest operator, vgrad of velocity + scalar
+constexpr int kSlotVals = OpShape<OP_VGRAD, 4>::NF;
 
 struct ChoArgs {
   int nblk;
@@ -115,27 +121,29 @@ struct ChoArgs {
   double* out;
   double* S;
   int accum;
-  double dt, mu, stab2coef;
+  double dt, mu, stab2coef, diff;
   int stab, stab2;
 };
 
-// staged value c of global node g; layout per operator:
-//   GRAD: u | VGRAD, DIV: u0 u1 u2 | DIVS: u0..2 P G0..2 x y z
-//   FLUX: u0..2 G0..8 | ADV2: u0..2 P | ADV4: u0..2 P G0..8 x y z
-template <int OP>
+// staged value c of global node g; layout per operator (NC components):
+//   GRAD: u | VGRAD: u0..NC-1 | DIV: u0 u1 u2 | DIVS: u0..2 P G0..2 x y z
+//   FLUX: u0..2 G0..8 | ADV2: u0..NC-1 P | ADV4: u0..NC-1 P G0..3NC-1 x y z
+template <int OP, int NC>
 __device__ inline double node_value(const ChoArgs& a, int g, int c) {
   if (OP == OP_GRAD || OP == OP_VGRAD || OP == OP_DIV) return a.U[(size_t)g * a.ldu + c];
-  if (c < 3) return a.U[(size_t)g * a.ldu + c];
-  if (OP == OP_FLUX) return a.G[(size_t)g * a.ldg + (c - 3)];
-  if (c == 3) return a.P[g];
-  if (OP == OP_ADV2) return 0.0;
+  if (OP == OP_FLUX) return c < 3 ? a.U[(size_t)g * a.ldu + c] : a.G[(size_t)g * a.ldg + (c - 3)];
   if (OP == OP_DIVS) {
+    if (c < 3) return a.U[(size_t)g * a.ldu + c];
+    if (c == 3) return a.P[g];
     if (c < 7) return a.G[(size_t)g * a.ldg + (c - 4)];
     return c == 7 ? a.x[g] : c == 8 ? a.y[g] : a.z[g];
   }
-  // ADV4
-  if (c < 13) return a.G[(size_t)g * a.ldg + (c - 4)];
-  return c == 13 ? a.x[g] : c == 14 ? a.y[g] : a.z[g];
+  // ADV2, ADV4
+  if (c < NC) return a.U[(size_t)g * a.ldu + c];
+  if (c == NC) return a.P[g];
+  if (c < 4 * NC + 1) return a.G[(size_t)g * a.ldg + (c - NC - 1)];
+  const int k = c - 4 * NC - 1;
+  return k == 0 ? a.x[g] : k == 1 ? a.y[g] : a.z[g];
 }
 
 // momentum flux of chorin::flux for edge nodes (P, Q) (Chorin.cpp:451-479),
@@ -159,30 +167,32 @@ __device__ inline double mflux_point(const double* sN, int P, int i, int j, doub
   return mu * vis - inv;
 }
 
-// advective flux of adv_damp2/adv_damp4 (Chorin.cpp:640-829) on edge (p, q)
-template <int OP>
+// advective flux of adv_damp2/adv_damp4 (Chorin.cpp:640-829) on edge (p, q):
+// velocity (0..2) and NC - 3 transported scalars
+template <int OP, int NC>
 __device__ inline void adv_edge(const ChoArgs& a, const double* sN, int p, int q,
-                                const double d[3], double dwv, double f[3]) {
-  double uL[3], uR[3];
+                                const double d[3], double dwv, double f[NC]) {
+  double uL[NC], uR[NC];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < NC; ++c) {
     uL[c] = sN[c * kCap + p];
     uR[c] = sN[c * kCap + q];
   }
-  double vdL[3];   // unreconstructed differences for the viscous term
+  double vdL[NC];  // unreconstructed differences for the viscous/diffusive terms
 #pragma unroll
-  for (int c = 0; c < 3; ++c) vdL[c] = uR[c] - uL[c];
+  for (int c = 0; c < NC; ++c) vdL[c] = uR[c] - uL[c];
   if (OP == OP_ADV4) {
-    const double vx = sN[13 * kCap + q] - sN[13 * kCap + p];
-    const double vy = sN[14 * kCap + q] - sN[14 * kCap + p];
-    const double vz = sN[15 * kCap + q] - sN[15 * kCap + p];
+    constexpr int X0 = 4 * NC + 1, G0 = NC + 1;
+    const double vx = sN[X0 * kCap + q] - sN[X0 * kCap + p];
+    const double vy = sN[(X0 + 1) * kCap + q] - sN[(X0 + 1) * kCap + p];
+    const double vz = sN[(X0 + 2) * kCap + q] - sN[(X0 + 2) * kCap + p];
     constexpr double eps = 1.0e-9, c1 = 1.0 - 1.0 / 3.0, c2 = 1.0 + 1.0 / 3.0;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const double g1 = sN[(4 + c * 3) * kCap + p] * vx + sN[(5 + c * 3) * kCap + p] * vy +
-                        sN[(6 + c * 3) * kCap + p] * vz;
-      const double g2 = sN[(4 + c * 3) * kCap + q] * vx + sN[(5 + c * 3) * kCap + q] * vy +
-                        sN[(6 + c * 3) * kCap + q] * vz;
+    for (int c = 0; c < NC; ++c) {
+      const double g1 = sN[(G0 + c * 3) * kCap + p] * vx + sN[(G0 + 1 + c * 3) * kCap + p] * vy +
+                        sN[(G0 + 2 + c * 3) * kCap + p] * vz;
+      const double g2 = sN[(G0 + c * 3) * kCap + q] * vx + sN[(G0 + 1 + c * 3) * kCap + q] * vy +
+                        sN[(G0 + 2 + c * 3) * kCap + q] * vz;
       const double delta2 = uR[c] - uL[c];
       const double delta1 = 2.0 * g1 - delta2;
       const double delta3 = 2.0 * g2 - delta2;
@@ -201,7 +211,7 @@ __device__ inline void adv_edge(const ChoArgs& a, const double* sN, int p, int q
   if (a.stab) aw = fabs(vnL + vnR) * 0.5;
   if (a.stab2) aw += a.stab2coef * fmax(fabs(vnL), fabs(vnR));
   const double v = dwv * a.mu;
-  const double pf = sN[3 * kCap + p] + sN[3 * kCap + q];
+  const double pf = sN[NC * kCap + p] + sN[NC * kCap + q];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     if (OP == OP_ADV2)
@@ -209,19 +219,29 @@ __device__ inline void adv_edge(const ChoArgs& a, const double* sN, int p, int q
     else
       f[c] = uL[c] * vnL + uR[c] * vnR + pf * d[c] + aw * (uR[c] - uL[c]) - v * vdL[c];
   }
+  if constexpr (NC > 3) {
+    const double dd = dwv * a.diff;
+#pragma unroll
+    for (int c = 3; c < NC; ++c) {
+      if (OP == OP_ADV2)
+        f[c] = uL[c] * vnL + uR[c] * vnR + (aw - dd) * (uR[c] - uL[c]);
+      else
+        f[c] = uL[c] * vnL + uR[c] * vnR + aw * (uR[c] - uL[c]) - dd * vdL[c];
+    }
+  }
 }
 
-template <int OP>
+template <int OP, int NC>
 __device__ inline void edge_values(const ChoArgs& a, const double* sN, int p, int q,
                                    const double d[3], double dwv,
-                                   double f[OpShape<OP>::NF]) {
+                                   double f[OpShape<OP, NC>::NF]) {
   if (OP == OP_GRAD) {
     const double s = sN[q] + sN[p];
 #pragma unroll
     for (int j = 0; j < 3; ++j) f[j] = d[j] * s;
   } else if (OP == OP_VGRAD) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < NC; ++i) {
       const double s = sN[i * kCap + q] + sN[i * kCap + p];
 #pragma unroll
       for (int j = 0; j < 3; ++j) f[i * 3 + j] = d[j] * s;
@@ -252,15 +272,15 @@ __device__ inline void edge_values(const ChoArgs& a, const double* sN, int p, in
       f[i] = s;
     }
   } else {
-    adv_edge<OP>(a, sN, p, q, d, dwv, f);
+    adv_edge<OP, NC>(a, sN, p, q, d, dwv, f);
   }
 }
 
 // corner `k` (weight 6 on it, 1 on the others, /8) of boundary face (A,B,C)
-template <int OP>
+template <int OP, int NC>
 __device__ inline void face_values(const ChoArgs& a, const double* sN, const int N[3],
                                    int k, const double n[3],
-                                   double f[OpShape<OP>::NF]) {
+                                   double f[OpShape<OP, NC>::NF]) {
   const double w0 = k == 0 ? 6.0 : 1.0, w1 = k == 1 ? 6.0 : 1.0, w2 = k == 2 ? 6.0 : 1.0;
   if (OP == OP_GRAD) {
     const double u = (w0 * sN[N[0]] + w1 * sN[N[1]] + w2 * sN[N[2]]) / 8.0;
@@ -268,7 +288,7 @@ __device__ inline void face_values(const ChoArgs& a, const double* sN, const int
     for (int j = 0; j < 3; ++j) f[j] = u * n[j];
   } else if (OP == OP_VGRAD) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < NC; ++i) {
       const double u = (w0 * sN[i * kCap + N[0]] + w1 * sN[i * kCap + N[1]] +
                         w2 * sN[i * kCap + N[2]]) / 8.0;
 #pragma unroll
@@ -292,24 +312,26 @@ __device__ inline void face_values(const ChoArgs& a, const double* sN, const int
       f[i] = s;
     }
   } else {
-    double fn[3][3];   // [node][component]
+    double fn[3][NC];   // [node][component]
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
       const int P = N[m];
-      const double u = sN[P], v = sN[kCap + P], w = sN[2 * kCap + P], pr = sN[3 * kCap + P];
+      const double u = sN[P], v = sN[kCap + P], w = sN[2 * kCap + P], pr = sN[NC * kCap + P];
       const double vn = n[0] * u + n[1] * v + n[2] * w;
       fn[m][0] = u * vn + pr * n[0];
       fn[m][1] = v * vn + pr * n[1];
       fn[m][2] = w * vn + pr * n[2];
+#pragma unroll
+      for (int c = 3; c < NC; ++c) fn[m][c] = sN[c * kCap + P] * vn;
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) f[c] = (w0 * fn[0][c] + w1 * fn[1][c] + w2 * fn[2][c]) / 8.0;
+    for (int c = 0; c < NC; ++c) f[c] = (w0 * fn[0][c] + w1 * fn[1][c] + w2 * fn[2][c]) / 8.0;
   }
 }
 
-template <int OP>
+template <int OP, int NC>
 __global__ __launch_bounds__(kT) void k_cho_blocks(ChoArgs a) {
-  constexpr int NV = OpShape<OP>::NV, NF = OpShape<OP>::NF;
+  constexpr int NV = OpShape<OP, NC>::NV, NF = OpShape<OP, NC>::NF;
   __shared__ double sN[NV * kCap];
   __shared__ double F[kT * NF];
   __shared__ uint16_t sE[2 * kT];
@@ -331,7 +353,7 @@ __global__ __launch_bounds__(kT) void k_cho_blocks(ChoArgs a) {
       if (k < nloc * NV) {
         const int i = k / NV, c = k - NV * i;
         const int g = i < nown ? own0 + i : a.halo[B0.w + i - nown];
-        v[r] = node_value<OP>(a, g, c);
+        v[r] = node_value<OP, NC>(a, g, c);
         dst[r] = c * kCap + i;
       }
     }
@@ -361,7 +383,7 @@ __global__ __launch_bounds__(kT) void k_cho_blocks(ChoArgs a) {
         const double d[3] = {a.dx[e], a.dy[e], a.dz[e]};
         const double dwv = (OP == OP_ADV2 || OP == OP_ADV4) ? a.dw[e] : 0.0;
         double f[NF];
-        edge_values<OP>(a, sN, (int)(pq & 0xffff), (int)(pq >> 16), d, dwv, f);
+        edge_values<OP, NC>(a, sN, (int)(pq & 0xffff), (int)(pq >> 16), d, dwv, f);
 #pragma unroll
         for (int k = 0; k < NF; ++k) F[t * NF + k] = f[k];
       } else {
@@ -372,7 +394,7 @@ __global__ __launch_bounds__(kT) void k_cho_blocks(ChoArgs a) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
           double f[NF];
-          face_values<OP>(a, sN, N, k, n, f);
+          face_values<OP, NC>(a, sN, N, k, n, f);
 #pragma unroll
           for (int m = 0; m < NF; ++m) F[(3 * t + k) * NF + m] = f[m];
         }
@@ -454,17 +476,19 @@ ChoArgs base_args(const ChoMesh* m) {
   return a;
 }
 
-// Runs operator OP into `out` (leading dimension ldo >= NF).  Outputs with
-// ldo == NF are written in place; wider ones go through a dense temporary.
-template <int OP>
+// Runs operator OP (NC components) into `out` (leading dimension ldo >= NF).
+// Outputs with ldo == NF are written in place; wider ones go through a dense
+// temporary.
+template <int OP, int NC = 3>
 int run_op(xg_ctx* c, ChoArgs a, double* out, int ldo, int accum, double* tmp) {
-  constexpr int NF = OpShape<OP>::NF;
+  constexpr int NF = OpShape<OP, NC>::NF;
+  static_assert(NF <= kSlotVals, "the halo slots hold kSlotVals values each");
   ChoMesh* m = c->cho;
   const bool dense = ldo == NF;
   a.out = dense ? out : tmp;
   a.accum = dense ? accum : 0;
   if (m->nblk)
-    hipLaunchKernelGGL(k_cho_blocks<OP>, dim3(m->nblk), dim3(kT), 0, c->stream, a);
+    hipLaunchKernelGGL((k_cho_blocks<OP, NC>), dim3(m->nblk), dim3(kT), 0, c->stream, a);
   if (m->nfx)
     hipLaunchKernelGGL(k_fixup_ld<NF>, dim3(nblocks(m->nfx)), dim3(kBlock), 0, c->stream,
                        m->nfx, m->fx_node, m->fx_off, m->fx_slot, m->slots, a.out, NF);
@@ -524,32 +548,56 @@ __global__ void k_divvol_n(int npoin, int nc, double* __restrict__ G,
 }
 
 // explicit stage of ChoCG::solve (ChoCG.cpp:1555-1560): U = Un - dt R / vol
-__global__ void k_cho_update(int npoin, double dt, double* __restrict__ U,
+__global__ void k_cho_update(int npoin, int nc, double dt, double* __restrict__ U,
                              const double* __restrict__ Un, const double* __restrict__ R,
                              const double* __restrict__ vol) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)npoin * 3) return;
-  U[i] = Un[i] - dt * R[i] / vol[i / 3];
+  if (i >= (int64_t)npoin * nc) return;
+  U[i] = Un[i] - dt * R[i] / vol[i / nc];
 }
 
-// physics::dirbc on the 3 velocity components (BC.cpp:30-69)
-__global__ void k_cho_dirbc(int nd, const int* __restrict__ node,
+// physics::dirbc on the nc components (BC.cpp:30-69)
+__global__ void k_cho_dirbc(int nd, int nc, const int* __restrict__ node,
                             const uint8_t* __restrict__ mask,
                             const double* __restrict__ val, double* __restrict__ U) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nd) return;
-  double* u = U + (size_t)node[i] * 3;
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-    if (mask[i * 3 + c]) u[c] = val[i * 3 + c];
+  double* u = U + (size_t)node[i] * nc;
+  for (int c = 0; c < nc; ++c)
+    if (mask[i * nc + c]) u[c] = val[i * nc + c];
 }
 
-// physics::noslipbc (BC.cpp:139-150)
-__global__ void k_noslip(int n, const int* __restrict__ node, double* __restrict__ U) {
+// physics::noslipbc (BC.cpp:139-150): the velocity of nodes with nc
+// components
+__global__ void k_noslip(int n, int nc, const int* __restrict__ node, double* __restrict__ U) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double* u = U + (size_t)node[i] * 3;
+  double* u = U + (size_t)node[i] * nc;
   u[0] = u[1] = u[2] = 0.0;
+}
+
+// the velocity (components 0..2 of nc) of src copied into dst: the frozen
+// flow of ChoCG::solve (ChoCG.cpp:1550-1568)
+__global__ void k_copy_vel(int npoin, int nc, const double* __restrict__ src,
+                           double* __restrict__ dst) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)npoin * 3) return;
+  const int64_t n = i / 3, c = i - 3 * n;
+  dst[n * nc + c] = src[n * nc + c];
+}
+
+// chorin::src (Chorin.cpp:985-1008) of slot_cyl, the one ChoCG problem with a
+// source (Problems.cpp:640-672): s = (-(x - 0.5), 0.5 - y, 0, 0), the
+// acceleration of the prescribed rotation; R -= s vol
+__global__ void k_cho_src_slot_cyl(int npoin, int nc, const double* __restrict__ x,
+                                   const double* __restrict__ y,
+                                   const double* __restrict__ vol, double* __restrict__ R) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npoin) return;
+  const double v = vol[i];
+  double* r = R + (size_t)i * nc;
+  r[0] -= -(x[i] - 0.5) * v;
+  r[1] -= (0.5 - y[i]) * v;
 }
 
 // ChoCG::pinit's div /= dt (ChoCG.cpp:1044)
@@ -558,11 +606,14 @@ __global__ void k_divide(int n, double* __restrict__ x, double dt) {
   if (i < n) x[i] /= dt;
 }
 
-// ChoCG::psolved's projection (ChoCG.cpp:1208-1213): U -= dt grad(dp)
-__global__ void k_project(int npoin, double dt, double* __restrict__ U,
+// ChoCG::psolved's projection (ChoCG.cpp:1208-1213): U -= dt grad(dp) on
+// the velocity of nodes with nc components
+__global__ void k_project(int npoin, int nc, double dt, double* __restrict__ U,
                           const double* __restrict__ sg) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < (int64_t)npoin * 3) U[i] -= dt * sg[i];
+  if (i >= (int64_t)npoin * 3) return;
+  const int64_t n = i / 3, c = i - 3 * n;
+  U[n * nc + c] -= dt * sg[i];
 }
 
 __global__ void k_add_to(int n, double* __restrict__ P, const double* __restrict__ dP) {
@@ -585,14 +636,14 @@ __global__ void k_poisson_rhs(int n, const double* __restrict__ pr,
 
 // per-node limits of ChoCG::dt (ChoCG.cpp:1384-1394), block minimum
 __global__ __launch_bounds__(kBlock) void k_chodt_partial(
-    int npoin, const double* __restrict__ U, const double* __restrict__ vol,
+    int npoin, int nc, const double* __restrict__ U, const double* __restrict__ vol,
     double dif, double* __restrict__ part) {
   __shared__ double sm[kBlock / 64];
   constexpr double eps = 2.220446049250313e-16;
   double m = 1.7976931348623157e308;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npoin;
        p += (int64_t)gridDim.x * blockDim.x) {
-    const double* s = U + p * 3;
+    const double* s = U + p * nc;
     const double vel = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
     const double L = cbrt(vol[p]);
     m = fmin(m, L / fmax(vel, 1.0e-8));
@@ -689,13 +740,13 @@ int cho_bc(xg_ctx* c, double* U) {
   ChoMesh* m = c->cho;
   if (m->ndir)
     hipLaunchKernelGGL(k_cho_dirbc, dim3(nblocks(m->ndir)), dim3(kBlock), 0, c->stream,
-                       m->ndir, m->dirnode, m->dirmask, m->dirval, U);
+                       m->ndir, m->nc, m->dirnode, m->dirmask, m->dirval, U);
   if (m->nsymnode)
     hipLaunchKernelGGL(k_symbc, dim3(nblocks(m->nsymnode)), dim3(kBlock), 0, c->stream,
-                       m->nsymnode, m->symnode, m->symoff, m->symnorm, U, 3, 0);
+                       m->nsymnode, m->symnode, m->symoff, m->symnorm, U, m->nc, 0);
   if (m->nnoslip)
     hipLaunchKernelGGL(k_noslip, dim3(nblocks(m->nnoslip)), dim3(kBlock), 0, c->stream,
-                       m->nnoslip, m->noslip, U);
+                       m->nnoslip, m->nc, m->noslip, U);
   XG_HIP(c, hipGetLastError());
   return XG_OK;
 }
@@ -713,11 +764,12 @@ int cho_grad(xg_ctx* c, const double* s, double* G) {
 
 int cho_vgrad(xg_ctx* c, const double* U) {
   ChoMesh* m = c->cho;
-  zero(c, m->vgrad, (int64_t)m->npoin * 9);
-  int rc = xg_chorin_vgrad(c, 1, 3, U, m->vgrad);
+  const int ng = 3 * m->nc;
+  zero(c, m->vgrad, (int64_t)m->npoin * ng);
+  int rc = xg_chorin_vgrad(c, 1, m->nc, U, m->vgrad);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_divvol_n, dim3(nblocks((int64_t)m->npoin * 9)), dim3(kBlock), 0,
-                     c->stream, m->npoin, 9, m->vgrad, m->vol);
+  hipLaunchKernelGGL(k_divvol_n, dim3(nblocks((int64_t)m->npoin * ng)), dim3(kBlock), 0,
+                     c->stream, m->npoin, ng, m->vgrad, m->vol);
   return XG_OK;
 }
 
@@ -838,7 +890,8 @@ int xg_chocg_upload(xg_ctx* c, int64_t npoin, const double* x, const double* y,
       (rc = upload(c, M, &m->entries, L.entries)) || (rc = upload(c, M, &m->fab, L.fab)) ||
       (rc = upload(c, M, &m->fcs, L.fcs)) || (rc = upload(c, M, &m->fnx, fnx)) ||
       (rc = upload(c, M, &m->fny, fny)) || (rc = upload(c, M, &m->fnz, fnz)) ||
-      (rc = dev_alloc(c, M, (void**)&m->slots, (size_t)L.nslot * 9 * sizeof(double))) ||
+      (rc = dev_alloc(c, M, (void**)&m->slots,
+                      (size_t)L.nslot * kSlotVals * sizeof(double))) ||
       (rc = upload(c, M, &m->fx_node, L.fx_node)) ||
       (rc = upload(c, M, &m->fx_off, L.fx_off)) ||
       (rc = upload(c, M, &m->fx_slot, L.fx_slot))) {
@@ -909,23 +962,25 @@ int xg_chorin_grad(xg_ctx* c, int dev, const double* U, double* G, int ldg) {
 int xg_chorin_vgrad(xg_ctx* c, int dev, int ncomp, const double* U, double* G) {
   int rc = check_cho(c);
   if (rc) return rc;
-  if (ncomp != 3)
-    return fail(c, XG_EUNSUPPORTED, "chorin::vgrad on the device: ncomp must be 3");
+  if (ncomp != 3 && ncomp != 4)
+    return fail(c, XG_EUNSUPPORTED, "chorin::vgrad on the device: ncomp must be 3 or 4");
   if (!U || !G) return fail(c, XG_EINVAL, "xg_chorin_vgrad: null field");
   XG_HIP(c, hipSetDevice(c->device));
   ChoMesh* m = c->cho;
-  const size_t n = (size_t)m->npoin;
+  const size_t n = (size_t)m->npoin, nf = 3 * (size_t)ncomp;
   ChoArgs a = base_args(m);
-  a.ldu = 3;
-  if ((rc = dev_in(c, dev, 0, U, n * 3, &a.U))) return rc;
+  a.ldu = ncomp;
+  if ((rc = dev_in(c, dev, 0, U, n * ncomp, &a.U))) return rc;
   double* out = G;
   if (!dev) {
-    if ((rc = stage_buf(c, 3, n * 9 * 8, &out))) return rc;
-    XG_HIP(c, hipMemcpyAsync(out, G, n * 9 * 8, hipMemcpyHostToDevice, c->stream));
+    if ((rc = stage_buf(c, 3, n * nf * 8, &out))) return rc;
+    XG_HIP(c, hipMemcpyAsync(out, G, n * nf * 8, hipMemcpyHostToDevice, c->stream));
   }
-  if ((rc = run_op<OP_VGRAD>(c, a, out, 9, 1, nullptr))) return rc;
+  rc = ncomp == 3 ? run_op<OP_VGRAD, 3>(c, a, out, 9, 1, nullptr)
+                  : run_op<OP_VGRAD, 4>(c, a, out, 12, 1, nullptr);
+  if (rc) return rc;
   if (!dev) {
-    XG_HIP(c, hipMemcpyAsync(G, out, n * 9 * 8, hipMemcpyDeviceToHost, c->stream));
+    XG_HIP(c, hipMemcpyAsync(G, out, n * nf * 8, hipMemcpyDeviceToHost, c->stream));
     XG_HIP(c, hipStreamSynchronize(c->stream));
   }
   return XG_OK;
@@ -963,26 +1018,30 @@ int xg_chorin_rhs(xg_ctx* c, int dev, const xg_chocfg* cfg, int ncomp,
   if (rc) return rc;
   if (!cfg) return fail(c, XG_EINVAL, "xg_chorin_rhs: null config");
   if (cfg->flux != 0 && cfg->flux != 1) return fail(c, XG_EINVAL, "Flux not correctly configured");
-  if (ncomp != 3)
-    return fail(c, XG_EUNSUPPORTED, "chorin::rhs on the device: ncomp must be 3");
+  if (ncomp != 3 && ncomp != 4)
+    return fail(c, XG_EUNSUPPORTED, "chorin::rhs on the device: ncomp must be 3 or 4");
   if (!P || !U || !R || (cfg->flux == 1 && !G))
     return fail(c, XG_EINVAL, "xg_chorin_rhs: null field");
   XG_HIP(c, hipSetDevice(c->device));
   ChoMesh* m = c->cho;
-  const size_t n = (size_t)m->npoin;
+  const size_t n = (size_t)m->npoin, nc = (size_t)ncomp;
   ChoArgs a = base_args(m);
-  a.ldu = 3; a.ldg = 9; a.mu = cfg->mu; a.stab = cfg->stab; a.stab2 = cfg->stab2;
-  a.stab2coef = cfg->stab2coef;
-  if ((rc = dev_in(c, dev, 0, U, n * 3, &a.U)) || (rc = dev_in(c, dev, 1, P, n, &a.P)) ||
-      (cfg->flux == 1 && (rc = dev_in(c, dev, 2, G, n * 9, &a.G))))
+  a.ldu = ncomp; a.ldg = 3 * ncomp; a.mu = cfg->mu; a.diff = cfg->diff;
+  a.stab = cfg->stab; a.stab2 = cfg->stab2; a.stab2coef = cfg->stab2coef;
+  if ((rc = dev_in(c, dev, 0, U, n * nc, &a.U)) || (rc = dev_in(c, dev, 1, P, n, &a.P)) ||
+      (cfg->flux == 1 && (rc = dev_in(c, dev, 2, G, n * 3 * nc, &a.G))))
     return rc;
   double* out = R;
-  if (!dev && (rc = stage_buf(c, 3, n * 3 * 8, &out))) return rc;
-  rc = cfg->flux == 0 ? run_op<OP_ADV2>(c, a, out, 3, 0, nullptr)
-                      : run_op<OP_ADV4>(c, a, out, 3, 0, nullptr);
+  if (!dev && (rc = stage_buf(c, 3, n * nc * 8, &out))) return rc;
+  if (ncomp == 3)
+    rc = cfg->flux == 0 ? run_op<OP_ADV2, 3>(c, a, out, 3, 0, nullptr)
+                        : run_op<OP_ADV4, 3>(c, a, out, 3, 0, nullptr);
+  else
+    rc = cfg->flux == 0 ? run_op<OP_ADV2, 4>(c, a, out, 4, 0, nullptr)
+                        : run_op<OP_ADV4, 4>(c, a, out, 4, 0, nullptr);
   if (rc) return rc;
   if (!dev) {
-    XG_HIP(c, hipMemcpyAsync(R, out, n * 3 * 8, hipMemcpyDeviceToHost, c->stream));
+    XG_HIP(c, hipMemcpyAsync(R, out, n * nc * 8, hipMemcpyDeviceToHost, c->stream));
     XG_HIP(c, hipStreamSynchronize(c->stream));
   }
   return XG_OK;
@@ -1039,11 +1098,12 @@ int xg_chocg_setup(xg_ctx* c, const double* vol, const xg_chostep* cfg,
     }
     if (nsym) off.push_back((int)(nrm.size() / 3));
   }
+  const int nc = m->nc;
   std::vector<int> ns(noslipnodes, noslipnodes + nnoslip), dn(ndir);
-  std::vector<uint8_t> dm((size_t)ndir * 3);
+  std::vector<uint8_t> dm((size_t)ndir * nc);
   for (int64_t i = 0; i < ndir; ++i) {
     dn[i] = (int)dirnodes[i];
-    for (int q = 0; q < 3; ++q) dm[i * 3 + q] = dirmask[i * 3 + q] ? 1 : 0;
+    for (int q = 0; q < nc; ++q) dm[i * nc + q] = dirmask[i * nc + q] ? 1 : 0;
   }
   XG_HIP(c, hipSetDevice(c->device));
   (void)hipStreamSynchronize(c->stream);
@@ -1055,11 +1115,12 @@ int xg_chocg_setup(xg_ctx* c, const double* vol, const xg_chostep* cfg,
       (rc = upload(c, L, &m->symoff, off)) || (rc = upload(c, L, &m->symnorm, nrm)) ||
       (rc = upload(c, L, &m->noslip, ns)) || (rc = upload(c, L, &m->dirnode, dn)) ||
       (rc = upload(c, L, &m->dirmask, dm)) ||
-      (rc = upload(c, L, &m->dirval, dirvals, (size_t)ndir * 3)) ||
+      (rc = upload(c, L, &m->dirval, dirvals, (size_t)ndir * nc)) ||
       (rc = dev_alloc(c, L, (void**)&m->pgrad, nn * 3 * 8)) ||
-      (rc = dev_alloc(c, L, (void**)&m->vgrad, nn * 9 * 8)) ||
+      (rc = dev_alloc(c, L, (void**)&m->vgrad, nn * 3 * nc * 8)) ||
       (rc = dev_alloc(c, L, (void**)&m->sgrad, nn * 3 * 8)) ||
-      (rc = dev_alloc(c, L, (void**)&m->R, nn * 3 * 8)) ||
+      (rc = dev_alloc(c, L, (void**)&m->R, nn * nc * 8)) ||
+      (rc = dev_alloc(c, L, (void**)&m->usave, nn * nc * 8)) ||
       (rc = dev_alloc(c, L, (void**)&m->F, nn * 3 * 8)) ||
       (rc = dev_alloc(c, L, (void**)&m->div, nn * 8)) ||
       (rc = dev_alloc(c, L, (void**)&m->red, sizeof(double) * (kRedBlocks + 1)))) {
@@ -1067,7 +1128,7 @@ int xg_chocg_setup(xg_ctx* c, const double* vol, const xg_chostep* cfg,
     return rc;
   }
   zero(c, m->pgrad, (int64_t)nn * 3);
-  zero(c, m->vgrad, (int64_t)nn * 9);
+  zero(c, m->vgrad, (int64_t)nn * 3 * nc);
   m->nsymnode = (int)node.size();
   m->nnoslip = (int)nnoslip;
   m->ndir = (int)ndir;
@@ -1097,16 +1158,17 @@ int xg_chocg_initial_dev(xg_ctx* c, double* U, double* P, int64_t* pit) {
   // m_np 0: project the initial velocity (dt = 1)
   if ((rc = cho_bc(c, U))) return rc;
   zero(c, m->div, n);
-  if ((rc = xg_chorin_div(c, 1, 0.0, P, m->pgrad, 3, U, 3, m->div, 0))) return rc;
+  if ((rc = xg_chorin_div(c, 1, 0.0, P, m->pgrad, 3, U, m->nc, m->div, 0))) return rc;
   if ((rc = cho_psolve(c, m->pval0, &it0))) return rc;
   if ((rc = cho_grad(c, x, m->sgrad))) return rc;
   hipLaunchKernelGGL(k_project, dim3(nblocks((int64_t)n * 3)), dim3(kBlock), 0, c->stream,
-                     n, 1.0, U, m->sgrad);
+                     n, m->nc, 1.0, U, m->sgrad);
   if ((rc = cho_bc(c, U))) return rc;
   // m_np 1: velocity gradients, momentum flux, its divergence -> pressure
   if ((rc = cho_vgrad(c, U))) return rc;
   zero(c, m->F, (int64_t)n * 3);
-  if ((rc = xg_chorin_flux(c, 1, m->cfg.flux.mu, U, 3, m->vgrad, 9, m->F))) return rc;
+  if ((rc = xg_chorin_flux(c, 1, m->cfg.flux.mu, U, m->nc, m->vgrad, 3 * m->nc, m->F)))
+    return rc;
   hipLaunchKernelGGL(k_divvol_n, dim3(nblocks((int64_t)n * 3)), dim3(kBlock), 0, c->stream,
                      n, 3, m->F, m->vol);
   if (m->nsymnode)
@@ -1136,7 +1198,7 @@ int xg_chocg_poisson_dev(xg_ctx* c, const double* pr, const double* neubc, doubl
   // m_np 0: div(u), replaced by the problem's rhs where it has one
   if ((rc = cho_bc(c, U))) return rc;
   zero(c, m->div, n);
-  if ((rc = xg_chorin_div(c, 1, 0.0, P, m->pgrad, 3, U, 3, m->div, 0))) return rc;
+  if ((rc = xg_chorin_div(c, 1, 0.0, P, m->pgrad, 3, U, m->nc, m->div, 0))) return rc;
   if (pr || neubc)
     hipLaunchKernelGGL(k_poisson_rhs, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, pr,
                        m->vol, neubc, m->div);
@@ -1144,7 +1206,7 @@ int xg_chocg_poisson_dev(xg_ctx* c, const double* pr, const double* neubc, doubl
   // psolved with m_np 0: project u with dt = 1, BCs; nstep = 1: p = solution
   if ((rc = cho_grad(c, x, m->sgrad))) return rc;
   hipLaunchKernelGGL(k_project, dim3(nblocks((int64_t)n * 3)), dim3(kBlock), 0, c->stream,
-                     n, 1.0, U, m->sgrad);
+                     n, m->nc, 1.0, U, m->sgrad);
   if ((rc = cho_bc(c, U))) return rc;
   XG_HIP(c, hipMemcpyAsync(P, x, (size_t)n * 8, hipMemcpyDeviceToDevice, c->stream));
   XG_HIP(c, hipGetLastError());
@@ -1160,8 +1222,8 @@ int xg_chocg_dt_dev(xg_ctx* c, const double* U, double* dt) {
   XG_HIP(c, hipSetDevice(c->device));
   ChoMesh* m = c->cho;
   int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, nblocks(m->npoin)));
-  hipLaunchKernelGGL(k_chodt_partial, dim3(nb), dim3(kBlock), 0, c->stream, m->npoin, U,
-                     m->vol, m->cfg.flux.mu, m->red);
+  hipLaunchKernelGGL(k_chodt_partial, dim3(nb), dim3(kBlock), 0, c->stream, m->npoin, m->nc,
+                     U, m->vol, std::max(m->cfg.flux.mu, m->cfg.flux.diff), m->red);
   hipLaunchKernelGGL(k_min_final, dim3(1), dim3(kRedBlocks), 0, c->stream, nb, m->red,
                      m->cfg.cfl, m->red + kRedBlocks);
   XG_HIP(c, hipGetLastError());
@@ -1173,8 +1235,8 @@ int xg_chocg_dt_dev(xg_ctx* c, const double* U, double* dt) {
   return XG_OK;
 }
 
-int xg_chocg_step_dev(xg_ctx* c, double dt, double* U, double* Un, double* P, double* dP,
-                      int64_t* pit) {
+int xg_chocg_step2_dev(xg_ctx* c, double dt, int freeze, double* U, double* Un, double* P,
+                       double* dP, int64_t* pit) {
   static const double rkcoef[4][4] = {{1.0}, {1.0 / 2.0, 1.0},
                                       {1.0 / 3.0, 1.0 / 2.0, 1.0},
                                       {1.0 / 4.0, 1.0 / 3.0, 1.0 / 2.0, 1.0}};
@@ -1182,30 +1244,57 @@ int xg_chocg_step_dev(xg_ctx* c, double dt, double* U, double* Un, double* P, do
   if (rc) return rc;
   if (!U || !Un || !P || !dP || U == Un) return fail(c, XG_EINVAL, "xg_chocg_step: bad fields");
   if (!(dt > 0.0)) return fail(c, XG_EINVAL, "xg_chocg_step: dt must be positive");
-  XG_HIP(c, hipSetDevice(c->device));
   ChoMesh* m = c->cho;
-  const int n = m->npoin;
-  const int64_t n3 = (int64_t)n * 3;
+  const int kind = c->problem.kind;
+  if (kind != XG_PROBLEM_NONE && kind != XG_PROBLEM_SLOT_CYL)
+    return fail(c, XG_EUNSUPPORTED, "xg_chocg_step: no ChoCG source for this problem");
+  if (kind == XG_PROBLEM_SLOT_CYL && m->nc != 4)
+    return fail(c, XG_EINVAL, "xg_chocg_step: slot_cyl has 4 components (xg_chocg_set_ncomp)");
+  XG_HIP(c, hipSetDevice(c->device));
+  const int n = m->npoin, nc = m->nc;
+  const int64_t n3 = (int64_t)n * 3, nn = (int64_t)n * nc;
   const int nst = m->cfg.rk;
   const bool implicit = m->theta > 2.220446049250313e-16;
+  bool divdone = false;
   for (int st = 0; st < nst; ++st) {
-    // ChoCG::rhs, solve (explicit, or semi-implicit at the last stage when
-    // theta > 0), pred (BC, velocity gradients for damp4)
-    if ((rc = xg_chorin_rhs(c, 1, &m->cfg.flux, 3, P, U, m->vgrad, m->R))) return rc;
+    const bool last = st + 1 == nst;
+    // ChoCG::rhs (chorin::rhs with the problem's source), solve (explicit,
+    // or semi-implicit at the last stage when theta > 0), pred (BC,
+    // velocity gradients for damp4)
+    if ((rc = xg_chorin_rhs(c, 1, &m->cfg.flux, nc, P, U, m->vgrad, m->R))) return rc;
+    if (kind == XG_PROBLEM_SLOT_CYL)
+      hipLaunchKernelGGL(k_cho_src_slot_cyl, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n,
+                         nc, m->x, m->y, m->vol, m->R);
     if (st == 0)
-      XG_HIP(c, hipMemcpyAsync(Un, U, (size_t)n3 * 8, hipMemcpyDeviceToDevice, c->stream));
-    if (implicit && st + 1 == nst) {
+      XG_HIP(c, hipMemcpyAsync(Un, U, (size_t)nn * 8, hipMemcpyDeviceToDevice, c->stream));
+    const bool restore = freeze && !(implicit && last);
+    if (restore)
+      hipLaunchKernelGGL(k_copy_vel, dim3(nblocks(n3)), dim3(kBlock), 0, c->stream, n, nc, U,
+                         m->usave);
+    if (implicit && last) {
       if ((rc = cho_momentum(c, dt, U, Un))) return rc;
     } else {
-      hipLaunchKernelGGL(k_cho_update, dim3(nblocks(n3)), dim3(kBlock), 0, c->stream, n,
+      hipLaunchKernelGGL(k_cho_update, dim3(nblocks(nn)), dim3(kBlock), 0, c->stream, n, nc,
                          rkcoef[nst - 1][st] * dt, U, Un, m->R, m->vol);
     }
     if ((rc = cho_bc(c, U))) return rc;
-    if (m->cfg.flux.flux == 1 && (rc = cho_vgrad(c, U))) return rc;
+    if (m->cfg.flux.flux == 1) {
+      if ((rc = cho_vgrad(c, U))) return rc;
+    } else if (restore && last) {
+      // damp2: corr and its div run inside pred, before the flow is put back
+      zero(c, m->div, n);
+      if ((rc = xg_chorin_div(c, 1, dt, P, m->pgrad, 3, U, nc, m->div, 1))) return rc;
+      divdone = true;
+    }
+    if (restore)
+      hipLaunchKernelGGL(k_copy_vel, dim3(nblocks(n3)), dim3(kBlock), 0, c->stream, n, nc,
+                         m->usave, U);
   }
   // ChoCG::div, pinit, psolve, sgrad, psolved, pgrad
-  zero(c, m->div, n);
-  if ((rc = xg_chorin_div(c, 1, dt, P, m->pgrad, 3, U, 3, m->div, 1))) return rc;
+  if (!divdone) {
+    zero(c, m->div, n);
+    if ((rc = xg_chorin_div(c, 1, dt, P, m->pgrad, 3, U, nc, m->div, 1))) return rc;
+  }
   hipLaunchKernelGGL(k_divide, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, m->div, dt);
   int64_t it = 0;
   if ((rc = cho_psolve(c, m->pzero, &it))) return rc;
@@ -1213,13 +1302,35 @@ int xg_chocg_step_dev(xg_ctx* c, double dt, double* U, double* Un, double* P, do
   if ((rc = xg_cg_device_x(c, &x))) return rc;
   XG_HIP(c, hipMemcpyAsync(dP, x, (size_t)n * 8, hipMemcpyDeviceToDevice, c->stream));
   if ((rc = cho_grad(c, dP, m->sgrad))) return rc;
-  hipLaunchKernelGGL(k_project, dim3(nblocks(n3)), dim3(kBlock), 0, c->stream, n, dt, U,
+  hipLaunchKernelGGL(k_project, dim3(nblocks(n3)), dim3(kBlock), 0, c->stream, n, nc, dt, U,
                      m->sgrad);
   if ((rc = cho_bc(c, U))) return rc;
   hipLaunchKernelGGL(k_add_to, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, P, dP);
   if ((rc = cho_grad(c, P, m->pgrad))) return rc;
   XG_HIP(c, hipGetLastError());
   if (pit) *pit = it;
+  return XG_OK;
+}
+
+int xg_chocg_step_dev(xg_ctx* c, double dt, double* U, double* Un, double* P, double* dP,
+                      int64_t* pit) {
+  return xg_chocg_step2_dev(c, dt, 0, U, Un, P, dP, pit);
+}
+
+int xg_chocg_set_ncomp(xg_ctx* c, int ncomp) {
+  int rc = check_cho(c);
+  if (rc) return rc;
+  if (ncomp != 3 && ncomp != 4)
+    return fail(c, XG_EUNSUPPORTED, "xg_chocg_set_ncomp: ncomp must be 3 or 4");
+  ChoMesh* m = c->cho;
+  if (m->nc != ncomp) {
+    XG_HIP(c, hipSetDevice(c->device));
+    (void)hipStreamSynchronize(c->stream);
+    dev_free_all(m->step_allocs);
+    m->have_step = false;     // xg_chocg_setup sizes its buffers by ncomp
+    m->theta = 0.0;           // and the momentum system is 3-component only
+    m->nc = ncomp;
+  }
   return XG_OK;
 }
 
@@ -1231,6 +1342,9 @@ int xg_chocg_momentum_setup(xg_ctx* c, int64_t ntet, const size_t* inpoel, doubl
   if (ntet <= 0 || !inpoel || !(theta >= 0.0) || (m_pc != 0 && m_pc != 1) || m_maxit < 1 ||
       !(m_tol >= 0.0))
     return fail(c, XG_EINVAL, "xg_chocg_momentum_setup: bad arguments");
+  if (m->nc != 3)
+    return fail(c, XG_EUNSUPPORTED, "xg_chocg_momentum_setup: the momentum system has 3 "
+                                    "components (no transported scalars)");
   const int64_t n = m->npoin;
   for (int64_t i = 0; i < 4 * ntet; ++i)
     if (inpoel[i] >= (size_t)n) return fail(c, XG_EINVAL, "xg_chocg_momentum_setup: node id out of range");

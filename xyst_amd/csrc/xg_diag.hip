@@ -95,30 +95,34 @@ __global__ __launch_bounds__(kBlock) void k_diag_rho(int64_t n, const double* __
 }
 
 // precompute's sums (NodeDiagnostics.cpp:224-236): p^2 V, u_c^2 V, dp^2 V,
-// (u_c - un_c)^2 V
+// (u_c - un_c)^2 V, NC components of U per node
+template <int NC>
 __global__ __launch_bounds__(kBlock) void k_diag_pre(int64_t n, const double* __restrict__ U,
                                                      const double* __restrict__ Un,
                                                      const double* __restrict__ P,
                                                      const double* __restrict__ dP,
                                                      const double* __restrict__ V,
                                                      double* __restrict__ part) {
+  constexpr int NS = 2 * NC + 2;
   __shared__ double sm[kBlock / 64];
-  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double acc[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) acc[q] = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kBlock) {
     const double v = V[i];
     acc[0] += P[i] * P[i] * v;
-    acc[4] += dP[i] * dP[i] * v;
+    acc[NC + 1] += dP[i] * dP[i] * v;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const double u = U[i * 3 + c];
-      const double d = u - Un[i * 3 + c];
+    for (int c = 0; c < NC; ++c) {
+      const double u = U[i * NC + c];
+      const double d = u - Un[i * NC + c];
       acc[1 + c] += u * u * v;
-      acc[5 + c] += d * d * v;
+      acc[NC + 2 + c] += d * d * v;
     }
   }
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
+  for (int q = 0; q < NS; ++q) {
     const double s = block_sum(acc[q], sm);
     if (threadIdx.x == 0) part[(size_t)q * gridDim.x + blockIdx.x] = s;
   }
@@ -126,8 +130,9 @@ __global__ __launch_bounds__(kBlock) void k_diag_pre(int64_t n, const double* __
 
 // rhocompute's analytic-error sums (NodeDiagnostics.cpp:112-128): the
 // numerical solution in primitive form (the scalars as they are) against
-// An, (nu_c - An_c)^2 V and |nu_c - An_c| V, NC components per node; NC = 1:
-// precompute's pressure error (NodeDiagnostics.cpp:237-241), p as it is
+// An, (nu_c - An_c)^2 V and |nu_c - An_c| V, NC components per node; NC = 1,
+// 3, 4: precompute's pressure, velocity and scalar errors
+// (NodeDiagnostics.cpp:237-250), the values as they are
 template <int NC>
 __global__ __launch_bounds__(kBlock) void k_diag_err(int64_t n, const double* __restrict__ U,
                                                      const double* __restrict__ V,
@@ -262,8 +267,8 @@ int xg_diag_err_dev(xg_ctx* c, int64_t npoin, int ncomp, const double* U, const 
   if (!c) return XG_EINVAL;
   if (npoin < 0 || !U || !V || !An || !out)
     return fail(c, XG_EINVAL, "xg_diag_err: bad arguments");
-  if (ncomp != 1 && (ncomp < 5 || ncomp > XG_MAX_NCOMP))
-    return fail(c, XG_EUNSUPPORTED, "xg_diag_err: ncomp must be 1 or 5..XG_MAX_NCOMP");
+  if (ncomp != 1 && ncomp != 3 && ncomp != 4 && (ncomp < 5 || ncomp > XG_MAX_NCOMP))
+    return fail(c, XG_EUNSUPPORTED, "xg_diag_err: ncomp must be 1, 3, 4 or 5..XG_MAX_NCOMP");
   XG_HIP(c, hipSetDevice(c->device));
   const int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, nblocks(npoin)));
   const int ns = 2 * ncomp;
@@ -272,6 +277,10 @@ int xg_diag_err_dev(xg_ctx* c, int64_t npoin, int ncomp, const double* U, const 
   if (rc) return rc;
   if (ncomp == 1)
     hipLaunchKernelGGL(k_diag_err<1>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, V, An, part);
+  else if (ncomp == 3)
+    hipLaunchKernelGGL(k_diag_err<3>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, V, An, part);
+  else if (ncomp == 4)
+    hipLaunchKernelGGL(k_diag_err<4>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, V, An, part);
   else if (ncomp == 5)
     hipLaunchKernelGGL(k_diag_err<5>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, V, An, part);
   else if (ncomp == 6)
@@ -281,19 +290,32 @@ int xg_diag_err_dev(xg_ctx* c, int64_t npoin, int ncomp, const double* U, const 
   return finish(c, ns, nb, part, dout, out);
 }
 
-int xg_diag_pre_dev(xg_ctx* c, int64_t npoin, const double* U, const double* Un,
-                    const double* P, const double* dP, const double* V, double* out) {
+int xg_diag_pre_nc_dev(xg_ctx* c, int64_t npoin, int ncomp, const double* U,
+                       const double* Un, const double* P, const double* dP, const double* V,
+                       double* out) {
   if (!c) return XG_EINVAL;
   if (npoin < 0 || !U || !Un || !P || !dP || !V || !out)
     return fail(c, XG_EINVAL, "xg_diag_pre: bad arguments");
+  if (ncomp != 3 && ncomp != 4)
+    return fail(c, XG_EUNSUPPORTED, "xg_diag_pre: ncomp must be 3 or 4");
   XG_HIP(c, hipSetDevice(c->device));
   const int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, nblocks(npoin)));
+  const int ns = 2 * ncomp + 2;
   double *part, *dout;
-  int rc = diag_buffers(c, 8, nb, &part, &dout);
+  int rc = diag_buffers(c, ns, nb, &part, &dout);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_diag_pre, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, Un, P, dP,
-                     V, part);
-  return finish(c, 8, nb, part, dout, out);
+  if (ncomp == 3)
+    hipLaunchKernelGGL(k_diag_pre<3>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, Un, P,
+                       dP, V, part);
+  else
+    hipLaunchKernelGGL(k_diag_pre<4>, dim3(nb), dim3(kBlock), 0, c->stream, npoin, U, Un, P,
+                       dP, V, part);
+  return finish(c, ns, nb, part, dout, out);
+}
+
+int xg_diag_pre_dev(xg_ctx* c, int64_t npoin, const double* U, const double* Un,
+                    const double* P, const double* dP, const double* V, double* out) {
+  return xg_diag_pre_nc_dev(c, npoin, 3, U, Un, P, dP, V, out);
 }
 
 }  // extern "C"

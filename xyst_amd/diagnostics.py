@@ -38,8 +38,10 @@ def errcompute(ctx, U, V, An):
 
 def precompute(ctx, U, Un, P, dP, V):
     """Sums of p^2 V, u_c^2 V, dp^2 V, (u_c - un_c)^2 V for the pressure-based
-    solver; U, Un (npoin, 3), P, dP, V (npoin,) device tensors."""
-    out = np.empty(8)
-    ctx._check(lib().xg_diag_pre_dev(ctx._h, int(U.shape[0]), ptr(U), ptr(Un), ptr(P),
-                                     ptr(dP), ptr(V), ptr(out)))
+    solver; U, Un (npoin, ncomp: velocity, and a transported scalar for
+    ncomp 4), P, dP, V (npoin,) device tensors."""
+    nc = int(U.shape[1])
+    out = np.empty(2 * nc + 2)
+    ctx._check(lib().xg_diag_pre_nc_dev(ctx._h, int(U.shape[0]), nc, ptr(U), ptr(Un), ptr(P),
+                                        ptr(dP), ptr(V), ptr(out)))
     return out

@@ -209,7 +209,8 @@ class Context:
         return G
 
     def set_problem(self, name="none", alpha=0.1, kappa=1.0, p0=10.0):
-        """The problem whose source term riemann::rhs adds (problems::SRC())."""
+        """The problem whose source term riemann::rhs, kozak::rhs and the ChoCG
+        step add (problems::SRC())."""
         p = _xg.Problem(_xg.PROBLEMS[name], alpha, kappa, p0)
         self._check(lib().xg_set_problem(self._h, C.byref(p)))
 
