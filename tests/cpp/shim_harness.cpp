@@ -1,6 +1,6 @@
 // Test harness around the reference-side shim (xyst_amd/shim): builds
-// tk::Fields from flat arrays, calls riemann::grad/rhs and kozak::rhs with
-// the reference's signatures, and reports a thrown tk::Exception as a
+// tk::Fields from flat arrays, calls riemann::grad/rhs, kozak::rhs and the
+// chorin:: operators with the reference's signatures, and reports a thrown tk::Exception as a
 // message.  Compiled against the stand-in headers of tests/cpp/shim_stub
 // into tests/cpp/libshim_harness.so, loaded by tests/test_shim.py.
 #include <algorithm>
@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "Chorin.hpp"
 #include "Kozak.hpp"
 #include "Problems.hpp"
 #include "Riemann.hpp"
@@ -18,6 +19,7 @@
 namespace {
 
 bool g_src = false;   // whether problems::SRC() returns a function
+int g_src_lin = 0;    // > 0: SRC() returns that many values, linear in x, y, z, t
 
 int report(const std::exception& e, char* err, int errlen) {
   if (err && errlen > 0) {
@@ -43,6 +45,12 @@ namespace problems {
 
 std::function<std::vector<tk::real>(tk::real, tk::real, tk::real, tk::real, std::size_t)>
 SRC() {
+  if (g_src_lin > 0)
+    return [](tk::real x, tk::real y, tk::real z, tk::real t, std::size_t) {
+      std::vector<tk::real> s((std::size_t)g_src_lin);
+      for (std::size_t c = 0; c < s.size(); ++c) s[c] = (c + 1.0) * x - 2.0 * y + z + t;
+      return s;
+    };
   if (!g_src) return {};
   return [](tk::real, tk::real, tk::real, tk::real, std::size_t) {
     return std::vector<tk::real>(5, 0.0);
@@ -54,6 +62,84 @@ SRC() {
 extern "C" {
 
 void shim_set_src(int on) { g_src = on != 0; }
+void shim_set_src_linear(int ncomp) { g_src_lin = ncomp; }
+
+void shim_set_chorin_cfg(int flux, int stab, int stab2, double stab2coef, double mu,
+                         double diff) {
+  xg_shim::chorin_cfg = xg_chocfg{flux, stab, stab2, stab2coef, mu, diff};
+}
+
+int shim_chorin_div(void* ctx, int64_t npoin, int ldu, const double* U, const double* P,
+                    int ldg, const double* G, double dt, int stab, double* D, char* err,
+                    int errlen) {
+  try {
+    xg_shim::t_ctx = static_cast<xg_ctx*>(ctx);
+    tk::Fields u = fields(npoin, ldu, U), g = fields(npoin, ldg, G);
+    std::vector<double> d = vec(npoin, D);
+    chorin::div({}, {}, {}, {}, dt, vec(npoin, P), g, u, d, stab != 0);
+    std::copy(d.begin(), d.end(), D);
+  } catch (const std::exception& e) {
+    return report(e, err, errlen);
+  }
+  return 0;
+}
+
+int shim_chorin_grad(void* ctx, int64_t npoin, const double* U, int ldg, double* G,
+                     char* err, int errlen) {
+  try {
+    xg_shim::t_ctx = static_cast<xg_ctx*>(ctx);
+    tk::Fields g = fields(npoin, ldg, G);
+    chorin::grad({}, {}, {}, {}, vec(npoin, U), g);
+    std::copy(g.vec().begin(), g.vec().end(), G);
+  } catch (const std::exception& e) {
+    return report(e, err, errlen);
+  }
+  return 0;
+}
+
+int shim_chorin_vgrad(void* ctx, int64_t npoin, int ncomp, const double* U, int ldg,
+                      double* G, char* err, int errlen) {
+  try {
+    xg_shim::t_ctx = static_cast<xg_ctx*>(ctx);
+    tk::Fields u = fields(npoin, ncomp, U), g = fields(npoin, ldg, G);
+    chorin::vgrad({}, {}, {}, {}, u, g);
+    std::copy(g.vec().begin(), g.vec().end(), G);
+  } catch (const std::exception& e) {
+    return report(e, err, errlen);
+  }
+  return 0;
+}
+
+int shim_chorin_flux(void* ctx, int64_t npoin, int ldu, const double* U, int ldg,
+                     const double* G, double* F, char* err, int errlen) {
+  try {
+    xg_shim::t_ctx = static_cast<xg_ctx*>(ctx);
+    tk::Fields u = fields(npoin, ldu, U), g = fields(npoin, ldg, G),
+               f = fields(npoin, 3, F);
+    chorin::flux({}, {}, {}, {}, u, g, f);
+    std::copy(f.vec().begin(), f.vec().end(), F);
+  } catch (const std::exception& e) {
+    return report(e, err, errlen);
+  }
+  return 0;
+}
+
+int shim_chorin_rhs(void* ctx, int64_t npoin, int ncomp, const double* x, const double* y,
+                    const double* z, const double* v, double t, const double* P,
+                    const double* U, int gprop, const double* G, double* R, char* err,
+                    int errlen) {
+  try {
+    xg_shim::t_ctx = static_cast<xg_ctx*>(ctx);
+    tk::Fields u = fields(npoin, ncomp, U), g = fields(npoin, gprop, G),
+               r = fields(npoin, ncomp, nullptr);
+    std::array<std::vector<tk::real>, 3> coord{vec(npoin, x), vec(npoin, y), vec(npoin, z)};
+    chorin::rhs({}, {}, coord, {}, vec(npoin, v), t, vec(npoin, P), u, g, r);
+    std::copy(r.vec().begin(), r.vec().end(), R);
+  } catch (const std::exception& e) {
+    return report(e, err, errlen);
+  }
+  return 0;
+}
 
 int shim_set_problem(void* ctx, const char* name, double alpha, double kappa, double p0,
                      char* err, int errlen) {

@@ -1,5 +1,6 @@
-"""The reference-side C++ shim (xyst_amd/shim: riemann::grad/rhs and
-kozak::rhs with the signatures of Riemann.hpp:19-39 and Kozak.hpp:19-27),
+"""The reference-side C++ shim (xyst_amd/shim: riemann::grad/rhs, kozak::rhs
+and chorin::div/grad/vgrad/flux/rhs with the signatures of
+Riemann.hpp:19-39, Kozak.hpp:19-27 and Chorin.hpp:19-70),
 compiled against stand-in headers (tests/cpp/shim_stub) into
 tests/cpp/libshim_harness.so by the build.
 
@@ -8,7 +9,9 @@ context, a host problem with a source term the device was not told about, a
 problem whose source the device lacks, mismatched field sizes.  GPU: through
 the shim, riemann::grad/rhs and kozak::rhs return exactly what the C-ABI
 returns, with and without the vortical-flow source and (kozak::rhs) with
-steady local time steps; more scalars than the device serves are refused.
+steady local time steps; more scalars than the device serves are refused;
+the chorin:: operators return what the C-ABI returns, chorin::rhs with the
+host problem's source subtracted (s vol per node).
 """
 import ctypes as C
 import os
@@ -36,6 +39,15 @@ def _harness():
                                    C.c_char_p, C.c_int]
     h.shim_kozak_rhs.argtypes = [VP, I64, C.c_int, D, D, VP, VP, VP, VP, C.c_char_p,
                                  C.c_int]
+    h.shim_set_src_linear.argtypes = [C.c_int]
+    h.shim_set_chorin_cfg.argtypes = [C.c_int, C.c_int, C.c_int, D, D, D]
+    h.shim_chorin_div.argtypes = [VP, I64, C.c_int, VP, VP, C.c_int, VP, D, C.c_int, VP,
+                                  C.c_char_p, C.c_int]
+    h.shim_chorin_grad.argtypes = [VP, I64, VP, C.c_int, VP, C.c_char_p, C.c_int]
+    h.shim_chorin_vgrad.argtypes = [VP, I64, C.c_int, VP, C.c_int, VP, C.c_char_p, C.c_int]
+    h.shim_chorin_flux.argtypes = [VP, I64, C.c_int, VP, C.c_int, VP, VP, C.c_char_p, C.c_int]
+    h.shim_chorin_rhs.argtypes = [VP, I64, C.c_int, VP, VP, VP, VP, D, VP, VP, C.c_int, VP,
+                                  VP, C.c_char_p, C.c_int]
     return h
 
 
@@ -68,6 +80,43 @@ class Shim:
         R = np.zeros_like(U)
         e = self._ret(self.h.shim_riemann_rhs(ctx, U.shape[0], U.shape[1], G.shape[1], _p(G), _p(U),
                                               _p(v), t, _p(tp), _p(R), self.err, 512))
+        return R, e
+
+    def cho_cfg(self, flux="damp2", stab=True, stab2=False, stab2coef=0.0, mu=0.0, diff=0.0):
+        self.h.shim_set_chorin_cfg({"damp2": 0, "damp4": 1}[flux], int(stab), int(stab2),
+                                   stab2coef, mu, diff)
+
+    def cho_div(self, ctx, dt, P, G, U, D, stab):
+        D = D.copy()
+        e = self._ret(self.h.shim_chorin_div(ctx, U.shape[0], U.shape[1], _p(U), _p(P),
+                                             G.shape[1], _p(G), dt, int(stab), _p(D),
+                                             self.err, 512))
+        return D, e
+
+    def cho_grad(self, ctx, U, G):
+        G = G.copy()
+        e = self._ret(self.h.shim_chorin_grad(ctx, U.size, _p(U), G.shape[1], _p(G),
+                                              self.err, 512))
+        return G, e
+
+    def cho_vgrad(self, ctx, U, G):
+        G = G.copy()
+        e = self._ret(self.h.shim_chorin_vgrad(ctx, U.shape[0], U.shape[1], _p(U), G.shape[1],
+                                               _p(G), self.err, 512))
+        return G, e
+
+    def cho_flux(self, ctx, U, G, F):
+        F = F.copy()
+        e = self._ret(self.h.shim_chorin_flux(ctx, U.shape[0], U.shape[1], _p(U), G.shape[1],
+                                              _p(G), _p(F), self.err, 512))
+        return F, e
+
+    def cho_rhs(self, ctx, coord, v, t, P, U, G):
+        R = np.zeros_like(U)
+        x, y, z = (np.ascontiguousarray(coord[i]) for i in range(3))
+        e = self._ret(self.h.shim_chorin_rhs(ctx, U.shape[0], U.shape[1], _p(x), _p(y), _p(z),
+                                             _p(v), t, _p(P), _p(U), G.shape[1], _p(G), _p(R),
+                                             self.err, 512))
         return R, e
 
     def kozak(self, ctx, t, dt, U, tp=None, dtp=None):
@@ -147,4 +196,78 @@ def test_shim_on_the_device_matches_the_c_abi():
     _, e = s.kozak(h, 0.0, 1e-3, np.ones((U.shape[0], 8)))
     assert e is not None and "scalars" in e
     s.set_src(False)
+    ctx.close()
+
+
+def test_chorin_shim_refusals():
+    s = Shim()
+    n = 5
+    U, P, D = np.ones((n, 3)), np.ones(n), np.zeros(n)
+    G9, G3 = np.zeros((n, 9)), np.zeros((n, 3))
+    s.cho_cfg()
+    for _, e in (s.cho_div(None, 0.1, P, G3, U, D, True), s.cho_grad(None, P, G3),
+                 s.cho_vgrad(None, U, G9), s.cho_flux(None, U, G9, np.zeros((n, 3))),
+                 s.cho_rhs(None, np.zeros((3, n)), np.ones(n), 0.0, P, U, G9)):
+        assert e is not None and e.startswith("xg: ")
+    _, e = s.cho_vgrad(None, U, np.zeros((n, 12)))
+    assert "3 gradients per component" in e
+    _, e = s.cho_flux(None, U, G3, np.zeros((n, 3)))
+    assert "sizes disagree" in e
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("flux", ["damp2", "damp4"])
+def test_chorin_shim_on_the_device_matches_the_c_abi(flux):
+    from xyst_amd import chorin as X
+    from xyst_amd import riemann as R
+    from xyst_amd.meshgen import box_mesh
+    s = Shim()
+    m = box_mesh(7, seed=6)
+    coord, inpoel, tri = m["coord"], m["inpoel"], m["triinpoel"]
+    e_, d_ = O.chocg_domint(coord, inpoel)
+    se = O.domsuped(inpoel, e_, d_)
+    ctx = R.Context(0)
+    X.upload(ctx, coord, se.sup, se.int, tri)
+    h = ctx._h
+    rng = np.random.default_rng(3)
+    n = coord.shape[1]
+    x, y, z = coord
+    U = np.stack([np.sin(x + 2 * y), np.cos(z - x), x * z, np.exp(-4 * (x * x + y * y))], 1)
+    U += 0.05 * rng.standard_normal(U.shape)
+    P = np.cos(2 * x) * y
+    G3 = rng.standard_normal((n, 3))
+    kw = dict(flux=flux, stab=True, stab2=True, stab2coef=0.2, mu=0.01, diff=0.02)
+    s.cho_cfg(**kw)
+    cfg = X.config(**kw)
+    for stab in (False, True):
+        D0 = np.linspace(0.0, 1.0, n)
+        D, e = s.cho_div(h, 0.01, P, G3, U, D0, stab)
+        assert e is None
+        assert np.array_equal(D, X.div(ctx, 0.01, P, G3, U, D0.copy(), stab=stab))
+    G, e = s.cho_grad(h, P, np.zeros((n, 3)))
+    assert e is None and np.array_equal(G, X.grad(ctx, P, np.zeros((n, 3))))
+    for nc in (3, 4):
+        Gv, e = s.cho_vgrad(h, U[:, :nc].copy(), np.zeros((n, 3 * nc)))
+        assert e is None
+        assert np.array_equal(Gv, X.vgrad(ctx, U[:, :nc].copy(), np.zeros((n, 3 * nc))))
+    Gv = Gv / O.vol(coord, inpoel)[:, None]
+    F, e = s.cho_flux(h, U, Gv, np.zeros((n, 3)))
+    assert e is None
+    assert np.array_equal(F, X.flux(ctx, 0.01, U, Gv, np.zeros((n, 3))))
+    v = O.vol(coord, inpoel)
+    for nc in (3, 4):
+        Uc, Gc = U[:, :nc].copy(), Gv[:, :3 * nc].copy()
+        Rc = X.rhs(ctx, cfg, P, Uc, Gc)
+        s.h.shim_set_src_linear(0)
+        Rs, e = s.cho_rhs(h, coord, v, 0.3, P, Uc, Gc)
+        assert e is None and np.array_equal(Rs, Rc)
+        # the host problem's source, s_c = (c+1) x - 2 y + z + t, times vol
+        s.h.shim_set_src_linear(nc)
+        Rs, e = s.cho_rhs(h, coord, v, 0.3, P, Uc, Gc)
+        src = np.stack([(c + 1.0) * x - 2.0 * y + z + 0.3 for c in range(nc)], 1)
+        assert e is None and np.array_equal(Rs, Rc - src * v[:, None])
+        s.h.shim_set_src_linear(nc + 1)
+        _, e = s.cho_rhs(h, coord, v, 0.3, P, Uc, Gc)
+        assert "more components" in e
+    s.h.shim_set_src_linear(0)
     ctx.close()

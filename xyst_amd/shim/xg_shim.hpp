@@ -1,6 +1,6 @@
 // Reference-side binding of libxg.so: what the replaced bodies of
-// riemann::grad/rhs (Riemann.hpp:19-39) and kozak::rhs (Kozak.hpp:19-27)
-// share.  These files go into the reference's src/Physics next to the
+// riemann::grad/rhs (Riemann.hpp:19-39), kozak::rhs (Kozak.hpp:19-27) and
+// chorin::div/grad/vgrad/flux/rhs (Chorin.hpp:19-70) share.  These files go into the reference's src/Physics next to the
 // headers they implement and are compiled by its own build, linked against
 // libxg.so; nothing here is part of libxg.so itself.
 #pragma once
@@ -17,6 +17,11 @@ namespace xg_shim {
 // (one entry method runs at a time on a PE's thread, so a thread-local is
 // enough in SMP mode too).
 extern thread_local xg_ctx* t_ctx;
+
+// g_cfg's settings of the chorin:: operators (flux "damp2" 0 / "damp4" 1,
+// stab, stab2, stab2coef, mat_dyn_viscosity, mat_dyn_diffusivity), which the
+// reference's bodies read from g_cfg: set once after the control file is read.
+extern xg_chocfg chorin_cfg;
 
 // A failed C-ABI call becomes a tk::Exception with the library's message,
 // as the reference's ErrChk/Throw report errors.
