@@ -17,8 +17,13 @@
 // remaining launches are no-ops and workgroup 0 records the final state,
 // which the host reads every few iterations.  Results are deterministic.
 //
-// SpMV: 8 lanes per CSR row (rows hold ~15 entries on tet meshes), lane
-// partial sums combined by a fixed butterfly.
+// SpMV of the recurrence: tiles of consecutive rows (at most 2048 nonzeros in
+// FP64, 4096 in FP32);
+// a workgroup forms a tile's products with coalesced loads of the values
+// and column indices into LDS, then one thread per row sums them in CSR
+// order.  The setup and flexible-CG products keep 8 lanes per CSR row (rows
+// hold ~15 entries on tet meshes), lane partials combined by a fixed
+// butterfly.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,6 +44,11 @@ namespace {
 constexpr int kCgThreads = 256;
 constexpr int kLanesPerRow = 8;
 constexpr int kMaxParts = 2048;
+// row tiles of the CG SpMV: consecutive rows holding at most tile_nnz<T>
+// nonzeros (a longer row is a tile of its own) and at most kCgThreads rows;
+// 16 KiB of LDS products per tile in FP64 and in FP32
+template <class T>
+constexpr int tile_nnz() { return sizeof(T) == 4 ? 4096 : 2048; }
 
 constexpr int kVecParts = 512;    // workgroups of the vector kernels (max)
 constexpr int kBufLen = 2 * kVecParts + 8;   // rz[.], rr[.], finished flag
@@ -65,10 +75,15 @@ struct CgSolver {
   std::vector<int64_t> bc_key;   // Dirichlet (node, mask...) set of the device values
   std::vector<DevArr> allocs;
   int *dia = nullptr, *dja = nullptr;
+  int ntile = 0, ntile32 = 0;
+  int* dtile = nullptr;          // first row of each FP64 SpMV tile (ntile + 1)
+  int* dtile32 = nullptr;        // the same for the FP32 products
   double* da = nullptr;          // values as used by the solve
   float* da32 = nullptr;         // FP32 copy for the mixed-precision inner solve
   double *x = nullptr, *b = nullptr, *r = nullptr, *z = nullptr, *d = nullptr;
   double *p0 = nullptr, *p1 = nullptr, *q = nullptr;
+  double* zp = nullptr;          // (z_i, p_i) pairs the SpMV gathers (FP64)
+  float* zp32 = nullptr;         // the same for the FP32 recurrence
   double *q2 = nullptr;          // FCG: q of the previous iteration
   double* fb = nullptr;          // FCG partial sums (2 x 4 x kMaxParts)
   double *xs = nullptr, *ebuf = nullptr;   // reference solution, error (training)
@@ -86,6 +101,21 @@ struct CgSolver {
 namespace {
 
 // ---------------------------------------------------------------- kernels --
+
+// (z_i, p_i) of one row, gathered by the SpMV with one load
+template <class T>
+struct alignas(2 * sizeof(T)) Pair {
+  T z, p;
+};
+
+// zp = (z_i, p_i) pairs: the recurrence's state for its first SpMV
+template <class T>
+__global__ __launch_bounds__(kCgThreads) void k_pack_zp(int64_t n, const T* __restrict__ z,
+                                                        const T* __restrict__ p,
+                                                        T* __restrict__ zp) {
+  const int64_t i = (int64_t)blockIdx.x * kCgThreads + threadIdx.x;
+  if (i < n) reinterpret_cast<Pair<T>*>(zp)[i] = Pair<T>{z[i], p[i]};
+}
 
 template <class T>
 __device__ inline T row_sum8(T v) {
@@ -259,15 +289,19 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_setup_final(
   }
 }
 
-// iteration j, first half: p_j = z + beta p_{j-1}, q = A p_j, partials of
+// iteration j, first half: p_j = z + beta p_{j-1} (both read from the (z, p)
+// pairs the previous update wrote), q = A p_j, partials of
 // (p_j, q) (ConjugateGradients::next/qAp, :585-635)
 template <class T>
 __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(
-    int j, int64_t nrow, const int* __restrict__ ia, const int* __restrict__ ja,
-    const T* __restrict__ a, const T* __restrict__ z, const T* __restrict__ pold,
-    T* __restrict__ pnew, T* __restrict__ q, const double* __restrict__ rzb,
-    int nvp, double* __restrict__ pq, CgState* st, long long maxit, double tol_nb) {
+    int j, int ntile, const int* __restrict__ tile, const int* __restrict__ ia,
+    const int* __restrict__ ja, const T* __restrict__ a, const T* __restrict__ zp,
+    T* __restrict__ pnew, T* __restrict__ q,
+    const double* __restrict__ rzb, int nvp, double* __restrict__ pq, CgState* st,
+    long long maxit, double tol_nb) {
+  constexpr int kTileNnz = tile_nnz<T>();
   __shared__ double sm[kCgThreads / 64];
+  __shared__ T prod[kTileNnz];
   // after the stop the partial sums of the two buffers are one update apart
   // from each other no more: every later launch of the batch is a no-op
   if (st->done) return;
@@ -279,27 +313,64 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(
   double beta = 0.0;   // the reference's m_alpha = rho/rho0 (:590)
   if (j >= 2) beta = s.rz / sum_parts(rzb + (j & 1) * kBufLen, nvp, sm);
   const T bt = (T)beta;
-  const int lane = threadIdx.x & (kLanesPerRow - 1);
+  const int t = threadIdx.x;
   double spq = 0.0;
-  // rows row and row + stride per trip (the partial sums keep row order)
-  const int64_t stride = (int64_t)gridDim.x * kCgThreads / kLanesPerRow;
-  for (int64_t row = ((int64_t)blockIdx.x * kCgThreads + threadIdx.x) / kLanesPerRow;
-       row < nrow; row += 2 * stride) {
-    const int64_t r1 = row + stride < nrow ? row + stride : -1;
-    T acc0, acc1;
-    rows_zp2(ia, ja, a, z, pold, bt, row, r1, lane, acc0, acc1);
-    acc0 = row_sum8(acc0);
-    acc1 = row_sum8(acc1);
-    if (lane == 0) {
-      const T pr = z[row] + bt * pold[row];
-      pnew[row] = pr;
-      q[row] = acc0;
-      spq += (double)pr * (double)acc0;
-      if (r1 >= 0) {
-        const T pr1 = z[r1] + bt * pold[r1];
-        pnew[r1] = pr1;
-        q[r1] = acc1;
-        spq += (double)pr1 * (double)acc1;
+  // tiles of consecutive rows: the tile's products a_k (z_c + beta p_c) are
+  // formed with coalesced loads of a and ja into LDS, then each row is summed
+  // by one thread in CSR order (the order of ConjugateGradients' CSR::mult)
+  for (int ti = blockIdx.x; ti < ntile; ti += gridDim.x) {
+    const int r0 = tile[ti], r1 = tile[ti + 1];
+    const int k0 = ia[r0], nz = ia[r1] - k0;
+    if (nz <= kTileNnz) {
+      for (int k = t; k < nz; k += 4 * kCgThreads) {
+        int cc[4];
+        T av[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int kk = k + u * kCgThreads;
+          cc[u] = kk < nz ? ja[k0 + kk] : 0;
+          av[u] = kk < nz ? a[k0 + kk] : T(0);
+        }
+        T zv[4], pv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const Pair<T> v = reinterpret_cast<const Pair<T>*>(zp)[cc[u]];
+          zv[u] = v.z;
+          pv[u] = v.p;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int kk = k + u * kCgThreads;
+          if (kk < nz) prod[kk] = av[u] * (zv[u] + bt * pv[u]);
+        }
+      }
+      __syncthreads();
+      const int r = r0 + t;
+      if (r < r1) {
+        const int e0 = ia[r] - k0, e1 = ia[r + 1] - k0;
+        T acc = T(0);
+        for (int k = e0; k < e1; ++k) acc += prod[k];
+        const Pair<T> v = reinterpret_cast<const Pair<T>*>(zp)[r];
+        const T pr = v.z + bt * v.p;
+        pnew[r] = pr;
+        q[r] = acc;
+        spq += (double)pr * (double)acc;
+      }
+      __syncthreads();        // prod is the next tile's
+    } else {
+      // one row longer than a tile: strided partial sums, fixed-order tree
+      T acc = T(0);
+      for (int k = t; k < nz; k += kCgThreads) {
+        const Pair<T> v = reinterpret_cast<const Pair<T>*>(zp)[ja[k0 + k]];
+        acc += a[k0 + k] * (v.z + bt * v.p);
+      }
+      const double sum = block_sum_all((double)acc, sm);
+      if (t == 0) {
+        const Pair<T> v = reinterpret_cast<const Pair<T>*>(zp)[r0];
+        const T pr = v.z + bt * v.p;
+        pnew[r0] = pr;
+        q[r0] = (T)sum;
+        spq += (double)pr * (double)(T)sum;
       }
     }
   }
@@ -313,8 +384,8 @@ template <class T>
 __global__ __launch_bounds__(kCgThreads) void k_cg_update(
     int j, int64_t n, const T* __restrict__ q, const T* __restrict__ d,
     const T* __restrict__ p, T* __restrict__ r, T* __restrict__ z, T* __restrict__ x,
-    double* __restrict__ rzb, int nvp, const double* __restrict__ pq, int nsp,
-    const CgState* st, long long maxit, double tol_nb, double brk) {
+    T* __restrict__ zp, double* __restrict__ rzb, int nvp, const double* __restrict__ pq,
+    int nsp, const CgState* st, long long maxit, double tol_nb, double brk) {
   __shared__ double sm[kCgThreads / 64];
   if (st->done) return;
   const StopTest s = stop_test(rzb + ((j - 1) & 1) * kBufLen, nvp, j, maxit, tol_nb, sm);
@@ -328,9 +399,11 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_update(
        i += (int64_t)gridDim.x * kCgThreads) {
     const T ri = r[i] - al * q[i];
     const T zi = ri / d[i];
+    const T pi = p[i];
     r[i] = ri;
     z[i] = zi;
-    x[i] += al * p[i];
+    x[i] += al * pi;
+    reinterpret_cast<Pair<T>*>(zp)[i] = Pair<T>{zi, pi};   // the next SpMV's gathers
     rz += (double)ri * (double)zi;
     rr += (double)ri * (double)ri;
   }
@@ -632,20 +705,27 @@ template <class T>
 int run_cg(xg_ctx* c, const T* a, T* z, T* p0, T* p1, T* q, const T* d, T* r, T* x,
            int64_t maxit, double tol_nb, double brk, int64_t jstart = 1) {
   CgSolver* s = c->cg;
-  const int gr = grid_rows(s->nrow), gv = grid_vec(s->nrow);
+  const bool f32 = sizeof(T) == 4;
+  const int ntile = f32 ? s->ntile32 : s->ntile;
+  const int* dtile = f32 ? s->dtile32 : s->dtile;
+  const int gr = std::max(1, std::min(ntile, kMaxParts)), gv = grid_vec(s->nrow);
   const int batch = 8;
+  // the (z, p) pairs of the first SpMV; each update writes the next ones
+  T* zp = f32 ? reinterpret_cast<T*>(s->zp32) : reinterpret_cast<T*>(s->zp);
+  hipLaunchKernelGGL(k_pack_zp<T>, dim3((unsigned)((s->nrow + kCgThreads - 1) / kCgThreads)),
+                     dim3(kCgThreads), 0, c->stream, s->nrow, z,
+                     (jstart - 1) & 1 ? p1 : p0, zp);
   for (int64_t j0 = jstart;; j0 += batch) {
     if (j0 > maxit + 1 + batch)
       return fail(c, XG_EHIP, "CG: stop test not reached on the device");
     for (int i = 0; i < batch; ++i) {
       const int j = (int)(j0 + i);
-      T* pold = (j - 1) & 1 ? p1 : p0;
       T* pnew = j & 1 ? p1 : p0;
       hipLaunchKernelGGL(k_cg_spmv<T>, dim3(gr), dim3(kCgThreads), 0, c->stream, j,
-                         s->nrow, s->dia, s->dja, a, z, pold, pnew, q, s->rzb, gv,
-                         s->pq, s->st, (long long)maxit, tol_nb);
+                         ntile, dtile, s->dia, s->dja, a, zp, pnew, q, s->rzb, gv, s->pq,
+                         s->st, (long long)maxit, tol_nb);
       hipLaunchKernelGGL(k_cg_update<T>, dim3(gv), dim3(kCgThreads), 0, c->stream, j,
-                         s->nrow, q, d, pnew, r, z, x, s->rzb, gv, s->pq, gr, s->st,
+                         s->nrow, q, d, pnew, r, z, x, zp, s->rzb, gv, s->pq, gr, s->st,
                          (long long)maxit, tol_nb, brk);
     }
     XG_HIP(c, hipGetLastError());
@@ -794,12 +874,32 @@ int xg_cg_matrix(xg_ctx* c, int ncomp, int64_t npoin, const size_t* ia,
   s->a0.assign(a, a + nnz);
   s->abc = s->a0;
   std::vector<int32_t> ia32(s->ia.begin(), s->ia.end());
+  // SpMV tiles: consecutive rows, at most kCgThreads rows and tnz nonzeros
+  // each (a longer row alone)
+  auto tiles = [&](int64_t tnz) {
+    std::vector<int32_t> tile(1, 0);
+    for (int64_t r = 0; r < nrow;) {
+      int64_t r1 = r, nz = 0;
+      while (r1 < nrow && r1 - r < kCgThreads && nz + (s->ia[r1 + 1] - s->ia[r1]) <= tnz) {
+        nz += s->ia[r1 + 1] - s->ia[r1];
+        ++r1;
+      }
+      if (r1 == r) r1 = r + 1;
+      tile.push_back((int32_t)r1);
+      r = r1;
+    }
+    return tile;
+  };
+  const std::vector<int32_t> tile = tiles(tile_nnz<double>()), tile32 = tiles(tile_nnz<float>());
+  s->ntile = (int)(tile.size() - 1);
+  s->ntile32 = (int)(tile32.size() - 1);
   std::vector<float> a32(nnz);
   for (int64_t j = 0; j < nnz; ++j) a32[j] = (float)a[j];
   int rc;
   auto& L = s->allocs;
   const size_t n = (size_t)nrow;
   if ((rc = upload(c, L, &s->dia, ia32)) || (rc = upload(c, L, &s->dja, s->ja)) ||
+      (rc = upload(c, L, &s->dtile, tile)) || (rc = upload(c, L, &s->dtile32, tile32)) ||
       (rc = upload(c, L, &s->da, s->a0)) || (rc = upload(c, L, &s->da32, a32)) ||
       (rc = dev_alloc(c, L, (void**)&s->x, n * 8)) ||
       (rc = dev_alloc(c, L, (void**)&s->b, n * 8)) ||
@@ -809,6 +909,8 @@ int xg_cg_matrix(xg_ctx* c, int ncomp, int64_t npoin, const size_t* ia,
       (rc = dev_alloc(c, L, (void**)&s->p0, n * 8)) ||
       (rc = dev_alloc(c, L, (void**)&s->p1, n * 8)) ||
       (rc = dev_alloc(c, L, (void**)&s->q, n * 8)) ||
+      (rc = dev_alloc(c, L, (void**)&s->zp, n * 16)) ||
+      (rc = dev_alloc(c, L, (void**)&s->zp32, n * 8)) ||
       (rc = dev_alloc(c, L, (void**)&s->q2, n * 8)) ||
       (rc = dev_alloc(c, L, (void**)&s->fb, sizeof(double) * 8 * kMaxParts)) ||
       (rc = dev_alloc(c, L, (void**)&s->xs, n * 8)) ||
