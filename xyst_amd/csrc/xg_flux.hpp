@@ -51,7 +51,8 @@ struct ScalarFlux {
 //    so both limiter values of an edge end share one reciprocal 1/(a+b).  (The
 //    reference evaluates 0/0 -> NaN when a or b is exactly zero; here that
 //    measure-zero case limits to 0.)
-//  * sound speed sqrt(g p/rho) with p = (g-1) rho e equals sqrt(g (g-1) e).
+//  * sound speed sqrt(g p/rho) with p = (g-1) rho e equals sqrt(g (g-1) e);
+//    Rusanov's c |n| is taken as one root, sqrt(g (g-1) e n.n).
 // ---------------------------------------------------------------------------
 
 __device__ inline double rcp(double x) {
@@ -136,16 +137,17 @@ __device__ inline void rusanov_fast(const Cfg& cfg, const double n[3],
   const double gg1 = cfg.gg1;
   double pL = (l[0] * l[4]) * cfg.gm1;
   double pR = (r[0] * r[4]) * cfg.gm1;
-  double cL = sqrt(gg1 * l[4]);
-  double cR = sqrt(gg1 * r[4]);
+  // the wave speeds times the dual-face area, c len = sqrt(g (g-1) e n.n)
+  const double nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+  double cL = sqrt(gg1 * l[4] * nn);
+  double cR = sqrt(gg1 * r[4] * nn);
   double vnL = l[1] * n[0] + l[2] * n[1] + l[3] * n[2];
   double vnR = r[1] * n[0] + r[2] * n[1] + r[3] * n[2];
   l[4] = (l[4] + 0.5 * (l[1] * l[1] + l[2] * l[2] + l[3] * l[3])) * l[0];
   l[1] *= l[0]; l[2] *= l[0]; l[3] *= l[0];
   r[4] = (r[4] + 0.5 * (r[1] * r[1] + r[2] * r[2] + r[3] * r[3])) * r[0];
   r[1] *= r[0]; r[2] *= r[0]; r[3] *= r[0];
-  double len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-  double fw = fmax(fabs(vnL) + cL * len, fabs(vnR) + cR * len);
+  double fw = fmax(fabs(vnL) + cL, fabs(vnR) + cR);
   f[0] = l[0] * vnL + r[0] * vnR + fw * (r[0] - l[0]);
   f[1] = l[1] * vnL + r[1] * vnR + (pL + pR) * n[0] + fw * (r[1] - l[1]);
   f[2] = l[2] * vnL + r[2] * vnR + (pL + pR) * n[1] + fw * (r[2] - l[2]);
