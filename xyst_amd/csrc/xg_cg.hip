@@ -17,13 +17,12 @@
 // remaining launches are no-ops and workgroup 0 records the final state,
 // which the host reads every few iterations.  Results are deterministic.
 //
-// SpMV of the recurrence: tiles of consecutive rows (at most 2048 nonzeros in
-// FP64, 4096 in FP32);
-// a workgroup forms a tile's products with coalesced loads of the values
-// and column indices into LDS, then one thread per row sums them in CSR
-// order.  The setup and flexible-CG products keep 8 lanes per CSR row (rows
-// hold ~15 entries on tet meshes), lane partials combined by a fixed
-// butterfly.
+// SpMV of the recurrences (CG and flexible CG): tiles of consecutive rows (at
+// most 2048 nonzeros in FP64, 4096 in FP32); a workgroup forms a tile's
+// products with coalesced loads of the values and column indices into LDS,
+// (z, p) gathered as one pair, then one thread per row sums them in CSR
+// order.  The setup's residual keeps 8 lanes per CSR row (rows hold ~15
+// entries on tet meshes), lane partials combined by a fixed butterfly.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -124,47 +123,6 @@ __device__ inline T row_sum8(T v) {
   v += __shfl_xor(v, 2, 64);
   v += __shfl_xor(v, 4, 64);
   return v;
-}
-
-// Row sums of a[k] * (z[c] + bt * p[c]) for rows r0 and r1 (r1 < 0: none),
-// 8 lanes per row.  The first two passes of both rows (16 entries each, most
-// rows of a P1 tetrahedral Laplacian) are loaded before any is used, so a
-// wave keeps four gathers per lane in flight; longer rows finish in a loop.
-// Each row's terms are added in the order of a plain loop over k = ia[r] +
-// lane, + 8, ...  Returns the lane partials (row_sum8 completes them).
-// (Also tried: 16 lanes per row, 1.4x slower; a contiguous row range per
-// XCD, no faster.)
-template <class T>
-__device__ inline void rows_zp2(const int* __restrict__ ia, const int* __restrict__ ja,
-                                const T* __restrict__ a, const T* __restrict__ z,
-                                const T* __restrict__ p, T bt, int64_t r0, int64_t r1,
-                                int lane, T& s0, T& s1) {
-  constexpr int L = kLanesPerRow;
-  const int b0 = ia[r0], e0 = ia[r0 + 1];
-  const int b1 = r1 >= 0 ? ia[r1] : 0, e1 = r1 >= 0 ? ia[r1 + 1] : 0;
-  const int k00 = b0 + lane, k01 = k00 + L, k10 = b1 + lane, k11 = k10 + L;
-  const bool u00 = k00 < e0, u01 = k01 < e0, u10 = k10 < e1, u11 = k11 < e1;
-  const int c00 = u00 ? ja[k00] : 0, c01 = u01 ? ja[k01] : 0;
-  const int c10 = u10 ? ja[k10] : 0, c11 = u11 ? ja[k11] : 0;
-  const T a00 = u00 ? a[k00] : T(0), a01 = u01 ? a[k01] : T(0);
-  const T a10 = u10 ? a[k10] : T(0), a11 = u11 ? a[k11] : T(0);
-  const T z00 = z[c00], z01 = z[c01], z10 = z[c10], z11 = z[c11];
-  const T p00 = p[c00], p01 = p[c01], p10 = p[c10], p11 = p[c11];
-  T v0 = 0, v1 = 0;
-  if (u00) v0 += a00 * (z00 + bt * p00);
-  if (u01) v0 += a01 * (z01 + bt * p01);
-  for (int k = k01 + L; k < e0; k += L) {
-    const int c = ja[k];
-    v0 += a[k] * (z[c] + bt * p[c]);
-  }
-  if (u10) v1 += a10 * (z10 + bt * p10);
-  if (u11) v1 += a11 * (z11 + bt * p11);
-  for (int k = k11 + L; k < e1; k += L) {
-    const int c = ja[k];
-    v1 += a[k] * (z[c] + bt * p[c]);
-  }
-  s0 = v0;
-  s1 = v1;
 }
 
 // Workgroup sum returned to every thread (fixed tree, so every workgroup that
@@ -289,6 +247,73 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_setup_final(
   }
 }
 
+// q = A p with p = z + bt p_old for the rows of this workgroup's tiles
+// (ti = blockIdx.x, + gridDim.x, ...): a tile's products a_k (z_c + bt p_c)
+// are formed with coalesced loads of a and ja into LDS, (z_c, p_c) read as
+// one pair, then each row is summed by one thread in CSR order (the order of
+// ConjugateGradients' CSR::mult); row(r, p_r, q_r) is called once per row,
+// by that thread (thread 0 for a row longer than a tile).  Every thread of
+// the workgroup calls this (it holds barriers).
+template <class T, class Row>
+__device__ inline void tiles_spmv(int ntile, const int* __restrict__ tile,
+                                  const int* __restrict__ ia, const int* __restrict__ ja,
+                                  const T* __restrict__ a, const T* __restrict__ zp, T bt,
+                                  T* prod, double* sm, Row row) {
+  constexpr int kTileNnz = tile_nnz<T>();
+  const Pair<T>* zpp = reinterpret_cast<const Pair<T>*>(zp);
+  const int t = threadIdx.x;
+  for (int ti = blockIdx.x; ti < ntile; ti += gridDim.x) {
+    const int r0 = tile[ti], r1 = tile[ti + 1];
+    const int k0 = ia[r0], nz = ia[r1] - k0;
+    if (nz <= kTileNnz) {
+      for (int k = t; k < nz; k += 4 * kCgThreads) {
+        int cc[4];
+        T av[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int kk = k + u * kCgThreads;
+          cc[u] = kk < nz ? ja[k0 + kk] : 0;
+          av[u] = kk < nz ? a[k0 + kk] : T(0);
+        }
+        T zv[4], pv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const Pair<T> v = zpp[cc[u]];
+          zv[u] = v.z;
+          pv[u] = v.p;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int kk = k + u * kCgThreads;
+          if (kk < nz) prod[kk] = av[u] * (zv[u] + bt * pv[u]);
+        }
+      }
+      __syncthreads();
+      const int r = r0 + t;
+      if (r < r1) {
+        const int e0 = ia[r] - k0, e1 = ia[r + 1] - k0;
+        T acc = T(0);
+        for (int k = e0; k < e1; ++k) acc += prod[k];
+        const Pair<T> v = zpp[r];
+        row(r, v.z + bt * v.p, acc);
+      }
+      __syncthreads();        // prod is the next tile's
+    } else {
+      // one row longer than a tile: strided partial sums, fixed-order tree
+      T acc = T(0);
+      for (int k = t; k < nz; k += kCgThreads) {
+        const Pair<T> v = zpp[ja[k0 + k]];
+        acc += a[k0 + k] * (v.z + bt * v.p);
+      }
+      const double sum = block_sum_all((double)acc, sm);
+      if (t == 0) {
+        const Pair<T> v = zpp[r0];
+        row(r0, v.z + bt * v.p, (T)sum);
+      }
+    }
+  }
+}
+
 // iteration j, first half: p_j = z + beta p_{j-1} (both read from the (z, p)
 // pairs the previous update wrote), q = A p_j, partials of
 // (p_j, q) (ConjugateGradients::next/qAp, :585-635)
@@ -313,67 +338,12 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(
   double beta = 0.0;   // the reference's m_alpha = rho/rho0 (:590)
   if (j >= 2) beta = s.rz / sum_parts(rzb + (j & 1) * kBufLen, nvp, sm);
   const T bt = (T)beta;
-  const int t = threadIdx.x;
   double spq = 0.0;
-  // tiles of consecutive rows: the tile's products a_k (z_c + beta p_c) are
-  // formed with coalesced loads of a and ja into LDS, then each row is summed
-  // by one thread in CSR order (the order of ConjugateGradients' CSR::mult)
-  for (int ti = blockIdx.x; ti < ntile; ti += gridDim.x) {
-    const int r0 = tile[ti], r1 = tile[ti + 1];
-    const int k0 = ia[r0], nz = ia[r1] - k0;
-    if (nz <= kTileNnz) {
-      for (int k = t; k < nz; k += 4 * kCgThreads) {
-        int cc[4];
-        T av[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int kk = k + u * kCgThreads;
-          cc[u] = kk < nz ? ja[k0 + kk] : 0;
-          av[u] = kk < nz ? a[k0 + kk] : T(0);
-        }
-        T zv[4], pv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const Pair<T> v = reinterpret_cast<const Pair<T>*>(zp)[cc[u]];
-          zv[u] = v.z;
-          pv[u] = v.p;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int kk = k + u * kCgThreads;
-          if (kk < nz) prod[kk] = av[u] * (zv[u] + bt * pv[u]);
-        }
-      }
-      __syncthreads();
-      const int r = r0 + t;
-      if (r < r1) {
-        const int e0 = ia[r] - k0, e1 = ia[r + 1] - k0;
-        T acc = T(0);
-        for (int k = e0; k < e1; ++k) acc += prod[k];
-        const Pair<T> v = reinterpret_cast<const Pair<T>*>(zp)[r];
-        const T pr = v.z + bt * v.p;
-        pnew[r] = pr;
-        q[r] = acc;
-        spq += (double)pr * (double)acc;
-      }
-      __syncthreads();        // prod is the next tile's
-    } else {
-      // one row longer than a tile: strided partial sums, fixed-order tree
-      T acc = T(0);
-      for (int k = t; k < nz; k += kCgThreads) {
-        const Pair<T> v = reinterpret_cast<const Pair<T>*>(zp)[ja[k0 + k]];
-        acc += a[k0 + k] * (v.z + bt * v.p);
-      }
-      const double sum = block_sum_all((double)acc, sm);
-      if (t == 0) {
-        const Pair<T> v = reinterpret_cast<const Pair<T>*>(zp)[r0];
-        const T pr = v.z + bt * v.p;
-        pnew[r0] = pr;
-        q[r0] = (T)sum;
-        spq += (double)pr * (double)(T)sum;
-      }
-    }
-  }
+  tiles_spmv<T>(ntile, tile, ia, ja, a, zp, bt, prod, sm, [&](int r, T pr, T qr) {
+    pnew[r] = pr;
+    q[r] = qr;
+    spq += (double)pr * (double)qr;
+  });
   put_part(pq, blockIdx.x, spq, sm);
 }
 
@@ -535,49 +505,48 @@ __global__ __launch_bounds__(kCgThreads) void k_fcg_dot(
   put_part(fb, blockIdx.x, v, sm);
 }
 
+// FCG: partials of (z, q_{k-1}) (dot != 0) and the (z, p_{k-1}) pairs the
+// SpMV gathers
+__global__ __launch_bounds__(kCgThreads) void k_fcg_pack_dot(
+    int64_t n, int dot, const double* __restrict__ z, const double* __restrict__ qp,
+    const double* __restrict__ pold, double* __restrict__ zp, double* __restrict__ fb) {
+  __shared__ double sm[kCgThreads / 64];
+  double v = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kCgThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kCgThreads) {
+    const double zi = z[i];
+    // the (z, p_{k-1}) pairs of the SpMV (p = 0 in the first iteration)
+    reinterpret_cast<Pair<double>*>(zp)[i] = Pair<double>{zi, dot ? pold[i] : 0.0};
+    if (dot) v += zi * qp[i];
+  }
+  put_part(fb, blockIdx.x, v, sm);
+}
+
 // p_k = z + beta p_{k-1}, q_k = A p_k; partials of (p,q) and (p,r) into
 // the buffer of iteration k.  first != 0: beta = 0.  `cur` holds the
 // (z_k, q_{k-1}) partials, `prev` the (p,q) partials of iteration k-1 (the
 // buffers alternate by iteration parity, so nothing read is overwritten).
 __global__ __launch_bounds__(kCgThreads) void k_fcg_spmv(
-    int first, int64_t nrow, const int* __restrict__ ia, const int* __restrict__ ja,
-    const double* __restrict__ a, const double* __restrict__ z,
-    const double* __restrict__ pold, double* __restrict__ pnew,
-    double* __restrict__ q, const double* __restrict__ r,
+    int first, int ntile, const int* __restrict__ tile, const int* __restrict__ ia,
+    const int* __restrict__ ja, const double* __restrict__ a, const double* __restrict__ zp,
+    double* __restrict__ pnew, double* __restrict__ q, const double* __restrict__ r,
     const double* __restrict__ prev, double* __restrict__ cur, int nvp, int nsp) {
   __shared__ double sm[kCgThreads / 64];
+  __shared__ double prod[tile_nnz<double>()];
   double beta = 0.0;
   if (!first) {
     const double zq = sum_parts(cur, nvp, sm);                 // (z, q_prev)
     const double pq = sum_parts(prev + kMaxParts, nsp, sm);    // (p, q)_prev
     beta = pq != 0.0 ? -zq / pq : 0.0;
   }
-  const int lane = threadIdx.x & (kLanesPerRow - 1);
   double spq = 0.0, spr = 0.0;
-  // rows row and row + stride per trip (the partial sums keep row order)
-  const int64_t stride = (int64_t)gridDim.x * kCgThreads / kLanesPerRow;
-  for (int64_t row = ((int64_t)blockIdx.x * kCgThreads + threadIdx.x) / kLanesPerRow;
-       row < nrow; row += 2 * stride) {
-    const int64_t r1 = row + stride < nrow ? row + stride : -1;
-    double acc0, acc1;
-    rows_zp2(ia, ja, a, z, pold, beta, row, r1, lane, acc0, acc1);
-    acc0 = row_sum8(acc0);
-    acc1 = row_sum8(acc1);
-    if (lane == 0) {
-      const double pr = z[row] + beta * pold[row];
-      pnew[row] = pr;
-      q[row] = acc0;
-      spq += pr * acc0;
-      spr += pr * r[row];
-      if (r1 >= 0) {
-        const double pr1 = z[r1] + beta * pold[r1];
-        pnew[r1] = pr1;
-        q[r1] = acc1;
-        spq += pr1 * acc1;
-        spr += pr1 * r[r1];
-      }
-    }
-  }
+  tiles_spmv<double>(ntile, tile, ia, ja, a, zp, beta, prod, sm,
+                     [&](int row, double pr, double qr) {
+                       pnew[row] = pr;
+                       q[row] = qr;
+                       spq += pr * qr;
+                       spr += pr * r[row];
+                     });
   put_part(cur + kMaxParts, blockIdx.x, spq, sm);
   put_part(cur + 2 * kMaxParts, blockIdx.x, spr, sm);
 }
@@ -1263,6 +1232,7 @@ int xg_cg_solve_flexible(xg_ctx* c, int pc_kind, int64_t inner, int64_t maxit,
   if (s->restore && (rc = restore_values(c))) return rc;
   const int64_t n = s->nrow;
   const int gr = grid_rows(n), gv = grid_vec(n);
+  const int gt = std::max(1, std::min(s->ntile, kMaxParts));   // FCG SpMV tiles
   hipLaunchKernelGGL(k_cg_residual, dim3(gr), dim3(kCgThreads), 0, c->stream, n,
                      s->dia, s->dja, s->da, s->x, s->b, s->r, s->d, s->pc);
   hipLaunchKernelGGL(k_f32_from_f64, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n,
@@ -1321,14 +1291,14 @@ int xg_cg_solve_flexible(xg_ctx* c, int pc_kind, int64_t inner, int64_t maxit,
       hipLaunchKernelGGL(k_f64_from_f32, dim3(gv), dim3(kCgThreads), 0, c->stream, n,
                          s->e32, s->z);
     }
-    if (k > 0)
-      hipLaunchKernelGGL(k_fcg_dot, dim3(gv), dim3(kCgThreads), 0, c->stream, n, s->z,
-                         Q[(k + 1) & 1], Bk);
-    hipLaunchKernelGGL(k_fcg_spmv, dim3(gr), dim3(kCgThreads), 0, c->stream, k == 0 ? 1 : 0,
-                       n, s->dia, s->dja, s->da, s->z, P[(k + 1) & 1], P[k & 1],
-                       Q[k & 1], s->r, Bp, Bk, gv, gr);
+    hipLaunchKernelGGL(k_fcg_pack_dot, dim3(gv), dim3(kCgThreads), 0, c->stream, n,
+                       k > 0 ? 1 : 0,
+                       s->z, Q[(k + 1) & 1], P[(k + 1) & 1], s->zp, Bk);
+    hipLaunchKernelGGL(k_fcg_spmv, dim3(gt), dim3(kCgThreads), 0, c->stream, k == 0 ? 1 : 0,
+                       s->ntile, s->dtile, s->dia, s->dja, s->da, s->zp, P[k & 1], Q[k & 1],
+                       s->r, Bp, Bk, gv, gt);
     hipLaunchKernelGGL(k_fcg_update, dim3(gv), dim3(kCgThreads), 0, c->stream, n,
-                       P[k & 1], Q[k & 1], s->r, s->x, Bk, gr);
+                       P[k & 1], Q[k & 1], s->r, s->x, Bk, gt);
     hipLaunchKernelGGL(k_sum_to_state, dim3(1), dim3(kCgThreads), 0, c->stream,
                        Bk + 3 * kMaxParts, gv, s->st);
     XG_HIP(c, hipGetLastError());
