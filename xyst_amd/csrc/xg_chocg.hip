@@ -199,8 +199,8 @@ __device__ inline void adv_edge(const ChoArgs& a, const double* sN, int p, int q
       // van Leer: phi(r) = (|r|+r)/(|r|+1) with r = s/t is 2s/(s+t) when s
       // and t have the same sign and 0 otherwise; phi(1/r) likewise
       const double A = delta2 + eps, BL = delta1 + eps, BR = delta3 + eps;
-      const double il = xg::same_sign(A, BL) ? xg::rcp(A + BL) : 0.0;
-      const double ir = xg::same_sign(A, BR) ? xg::rcp(A + BR) : 0.0;
+      const double il = xg::vl_rcp(A, BL);
+      const double ir = xg::vl_rcp(A, BR);
       uL[c] += 0.5 * il * (delta1 * c1 * A + delta2 * c2 * BL);
       uR[c] -= 0.5 * ir * (delta3 * c1 * A + delta2 * c2 * BR);
     }
