@@ -28,6 +28,14 @@ volume.  BCs (ChoCG::BC, 1341-1355): dirbc (mask 1: initial value, mask 2:
 configured value), then symmetry (pos 0), then noslip.  With theta > 0 the
 last RK stage solves the momentum system (ChoCG::lhs, solve, msolve,
 msolved) with the 3-component ConjugateGradients.
+
+Frozen flow (freezeflow > 1 after freezetime, ChoCG.cpp:1550-1568): the
+velocity is put back after each stage's update, BCs and velocity
+gradients, and with damp2 after the last stage's divergence.  The frozen
+velocity staying constant is checked; the order of those restores (which
+velocity the stage's gradients and divergence see) is parity unpinned:
+the reference's frozen-flow case (SlotCyl) holds the velocity by its BCs,
+so no fixture tells the orders apart.
 """
 import numpy as np
 
